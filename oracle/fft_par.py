@@ -1,0 +1,238 @@
+"""Restatement of the ``FFT_PAR_*`` pseudo-spectral products, ``nb_procs = 1``.
+
+Reference: SOURCES/POST_PROCESSING/fft_parallel.F90.  With one rank the two
+MPI_ALLTOALL transposes are identity copies, so each routine reduces to, per
+point (``fft_parallel.F90:225-324`` for the cross product):
+
+1. pack ``c_0 = a_0``, ``c_m = (a_m - i b_m)/2``, zero-padded to ``m_max_pad``
+   modes (``:227-241``);
+2. unnormalised complex-to-real DFT of length ``N = 2*m_max_pad-1``
+   (``:254-267``);
+3. pointwise operation on the real grid (``:270-274``);
+4. real-to-complex DFT, scaled by ``1/N`` (``:277-288``);
+5. keep modes ``0..m_max-1``: mode 0 as is, mode m>=1 as ``2*conj`` (``:295-299``),
+   unpacked as (real, imaginary) = (cos, sin) columns (``:311-324``).
+
+Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
+used for larger parity cases and the CPU baseline) and ``"direct"`` (a direct
+sum in 80-bit long double, the pin for the fast engine on small inputs).
+Every routine processes points in chunks so that large inputs stay bounded
+in memory; points are independent, so chunking does not change the result.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = [
+    "n_grid", "pack", "c2r", "r2c", "unpack", "fft_product",
+    "fft_par_real", "fft_par_cross_prod_dcl", "fft_par_dot_prod_dcl",
+    "fft_par_prod_dcl", "fft_scalar_vect_dcl", "fft_tensor_dcl",
+    "OPS",
+]
+
+_PI_LD = np.longdouble("3.141592653589793238462643383279502884197")
+_CHUNK = 16384
+
+
+def n_grid(m_max_pad: int) -> int:
+    """Azimuthal grid size ``N_r_pad = 2*m_max_pad-1`` (fft_parallel.F90:185)."""
+    return 2 * m_max_pad - 1
+
+
+def pack(V: np.ndarray, m_max_pad: int, dtype=np.complex128) -> np.ndarray:
+    """(np, nb_field, M) cos/sin columns -> (np, nb_field/2, m_max_pad) complex.
+
+    fft_parallel.F90:236-241: ``cu = 0.5*CMPLX(a, -b)``, then ``cu(1) = 2*REAL(cu(1))``.
+    """
+    npt, nb, M = V.shape
+    cu = np.zeros((npt, nb // 2, m_max_pad), dtype=dtype)
+    mm = min(M, m_max_pad)
+    cu[:, :, :mm] = 0.5 * (V[:, 0::2, :mm] - 1j * V[:, 1::2, :mm])
+    cu[:, :, 0] = 2 * cu[:, :, 0].real
+    return cu
+
+
+def _angles(N: int, rows: int, cols: int, sign: int) -> np.ndarray:
+    k = np.arange(rows, dtype=np.int64)[:, None]
+    j = np.arange(cols, dtype=np.int64)[None, :]
+    t = (2 * _PI_LD * np.asarray((k * j) % N, dtype=np.longdouble)) / N
+    return np.cos(t) + sign * 1j * np.sin(t)
+
+
+def c2r(cu: np.ndarray, N: int, engine: str = "fft") -> np.ndarray:
+    """Unnormalised Hermitian complex-to-real DFT along the last axis (FFTW c2r)."""
+    K = cu.shape[-1]
+    if engine == "fft":
+        return np.fft.irfft(cu, n=N, axis=-1) * N
+    if engine == "direct":
+        E = _angles(N, K, N, +1)                       # (K, N), e^{+2 pi i m j / N}
+        c = cu.astype(np.clongdouble)
+        out = c[..., :1].real * np.ones(N, dtype=np.longdouble)
+        if K > 1:
+            out = out + 2 * np.real(c[..., 1:] @ E[1:, :])
+        return out
+    raise ValueError(engine)
+
+
+def r2c(ru: np.ndarray, K: int, engine: str = "fft") -> np.ndarray:
+    """Unnormalised real-to-complex DFT, first ``K`` outputs (FFTW r2c)."""
+    N = ru.shape[-1]
+    if engine == "fft":
+        return np.fft.rfft(ru, axis=-1)[..., :K]
+    if engine == "direct":
+        E = _angles(N, N, K, -1)                       # (N, K), e^{-2 pi i j m / N}
+        return ru.astype(np.longdouble) @ E
+    raise ValueError(engine)
+
+
+def unpack(prod_cu: np.ndarray, N: int, m_max: int) -> np.ndarray:
+    """(np, nf, >=m_max) spectrum -> (np, 2*nf, m_max) cos/sin columns.
+
+    fft_parallel.F90:291-299 (scale 1/N, mode 0 as is, ``2*CONJG`` for m>=1)
+    and :311-324 (real -> odd column, imaginary -> even column).
+    """
+    pc = prod_cu[..., :m_max] * (1.0 / N)
+    pc[..., 1:] = 2 * np.conj(pc[..., 1:])
+    npt, nf, _ = pc.shape
+    out = np.empty((npt, 2 * nf, m_max), dtype=np.float64)
+    out[:, 0::2, :] = np.real(pc)
+    out[:, 1::2, :] = np.imag(pc)
+    return out
+
+
+def _check(nb_field: int, m_max_c: int, name: str):
+    # fft_parallel.F90:188-191 (WRITE + STOP on odd nb_field or empty mode range)
+    if nb_field % 2 != 0 or m_max_c == 0:
+        raise ValueError(f"BUG in {name}: MOD(nb_field,2)/=0 .OR. m_max_c==0")
+
+
+def _op_cross(ru):
+    a, b = ru[:, 0:3], ru[:, 3:6]
+    return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1],
+                     a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                     a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], axis=1)
+
+
+def _op_dot(ru):
+    return (ru[:, 0] * ru[:, 3] + ru[:, 1] * ru[:, 4] + ru[:, 2] * ru[:, 5])[:, None]
+
+
+def _op_prod(ru):
+    return (ru[:, 0] * ru[:, 1])[:, None]
+
+
+def _op_scalar_vect_1(ru):
+    return ru[:, 3:4] * ru[:, 0:3]
+
+
+def _op_scalar_vect_2(ru):
+    return ru[:, 0:3] / ru[:, 3:4]
+
+
+def _op_tensor(ru):
+    # prod_ru_a(:, b) = ru(:, a) * ru(:, 3+b)   (fft_parallel.F90:1674-1686)
+    return (ru[:, 0:3, None, :] * ru[:, None, 3:6, :]).reshape(ru.shape[0], 9, ru.shape[-1])
+
+
+OPS = {
+    "cross": _op_cross,
+    "dot": _op_dot,
+    "prod": _op_prod,
+    "scalar_vect_1": _op_scalar_vect_1,
+    "scalar_vect_2": _op_scalar_vect_2,
+    "tensor": _op_tensor,
+}
+
+
+def fft_product(inputs, op, m_max_pad: int, engine: str = "fft") -> np.ndarray:
+    """Generic de-aliased product: concatenate input columns, c2r, ``op``, r2c.
+
+    ``inputs`` is a list of (np, nb_k, M) arrays; ``op`` maps the real fields
+    (np, nfields, N) to outputs (np, nout, N).  Returns (np, 2*nout, M).
+    """
+    V = np.concatenate([np.asarray(x, dtype=np.float64) for x in inputs], axis=1)
+    npt, nb, M = V.shape
+    N = n_grid(m_max_pad)
+    f = OPS[op] if isinstance(op, str) else op
+    outs = []
+    for s in range(0, npt, _CHUNK):
+        cu = pack(V[s:s + _CHUNK], m_max_pad)
+        ru = c2r(cu, N, engine)
+        pr = f(ru)
+        pc = r2c(pr, m_max_pad, engine)
+        outs.append(unpack(np.asarray(pc, dtype=np.complex128), N, M))
+    return np.concatenate(outs, axis=0) if outs else np.zeros((0, 0, M))
+
+
+def fft_par_real(V1_in, m_max_pad=None, engine="fft"):
+    """FFT_PAR_REAL (fft_parallel.F90:28-145): real planes (N, nb_field/2, np).
+
+    ``m_max_pad`` defaults to ``m_max`` (no ``opt_nb_plane``).
+    """
+    V = np.asarray(V1_in, dtype=np.float64)
+    npt, nb, M = V.shape
+    _check(nb, M, "FFT_PAR_REAL")
+    if m_max_pad is None:
+        m_max_pad = M
+    N = n_grid(m_max_pad)
+    ru = c2r(pack(V, m_max_pad), N, engine)
+    return np.transpose(np.asarray(ru, dtype=np.float64), (2, 1, 0))
+
+
+def fft_par_cross_prod_dcl(V1_in, V2_in, m_max_pad, engine="fft"):
+    """FFT_PAR_CROSS_PROD_DCL (fft_parallel.F90:147-328): V1 x V2, (np,6,M)."""
+    nb = np.shape(V1_in)[1]
+    _check(nb, np.shape(V1_in)[2], "FFT_PAR_CROSS_PROD_DCL")
+    if nb != 6:
+        raise ValueError("cross product needs nb_field == 6 (fft_parallel.F90:270)")
+    return fft_product([V1_in, V2_in], "cross", m_max_pad, engine)
+
+
+def fft_par_dot_prod_dcl(V1_in, V2_in, m_max_pad, engine="fft"):
+    """FFT_PAR_DOT_PROD_DCL (fft_parallel.F90:330-500): V1 . V2, (np,2,M)."""
+    nb = np.shape(V1_in)[1]
+    _check(nb, np.shape(V1_in)[2], "FFT_PAR_DOT_PROD_DCL")
+    if nb != 6:
+        raise ValueError("dot product needs nb_field == 6 (fft_parallel.F90:448)")
+    return fft_product([V1_in, V2_in], "dot", m_max_pad, engine)
+
+
+def fft_par_prod_dcl(c1_in, c2_in, m_max_pad, engine="fft"):
+    """FFT_PAR_PROD_DCL (fft_parallel.F90:502-665): c1 * c2, (np,2,M)."""
+    if np.shape(c1_in)[2] == 0:
+        raise ValueError("BUG FFT_PAR_PROD_DCL: m_max_c==0")
+    return fft_product([np.asarray(c1_in)[:, 0:2], np.asarray(c2_in)[:, 0:2]],
+                       "prod", m_max_pad, engine)
+
+
+def fft_scalar_vect_dcl(V1_in, V2_in, pb, m_max_pad, engine="fft"):
+    """FFT_SCALAR_VECT_DCL (fft_parallel.F90:1021-1220): pb=1 s*v, pb=2 v/s."""
+    nb1, nb2 = np.shape(V1_in)[1], np.shape(V2_in)[1]
+    if nb1 % 2 or nb2 % 2 or np.shape(V1_in)[2] == 0:
+        raise ValueError("BUG in FFT_SCALAR_VECT_DCL")
+    if not (nb1 == 6 and nb2 == 2):
+        raise ValueError("error in dimension of INPUT variables while calling FFT_SCALAR_VECT_DCL")
+    if pb not in (1, 2):
+        raise ValueError("error in problem type while calling FFT_SCALAR_VECT_DCL")
+    return fft_product([V1_in, V2_in], f"scalar_vect_{pb}", m_max_pad, engine)
+
+
+def fft_tensor_dcl(V1_in, V2_in, m_max_pad, engine="fft"):
+    """FFT_TENSOR_DCL without opt_tension (fft_parallel.F90:1543-1796).
+
+    Returns V_out(3, np, 6, M) with ``V_out(a, n, 2b-1:2b, m)`` the (cos, sin)
+    coefficients of ``V1_a * V2_b``.
+    """
+    nb = np.shape(V1_in)[1]
+    _check(nb, np.shape(V1_in)[2], "FFT_TENSOR_DCL")
+    if nb != 6:
+        raise ValueError("tensor product needs nb_field == 6")
+    out = fft_product([V1_in, V2_in], "tensor", m_max_pad, engine)  # (np, 18, M)
+    npt, _, M = out.shape
+    # column 2*(3a+b) / 2*(3a+b)+1 -> V_out(a, n, 2b, m) / (a, n, 2b+1, m)
+    res = np.empty((3, npt, 6, M))
+    for a in range(3):
+        for b in range(3):
+            res[a, :, 2 * b, :] = out[:, 2 * (3 * a + b), :]
+            res[a, :, 2 * b + 1, :] = out[:, 2 * (3 * a + b) + 1, :]
+    return res
