@@ -1,0 +1,73 @@
+"""ctypes binding of the C-ABI in ``include/sfx.h`` (``libsfx.so``).
+
+The library is built in-tree (``sfemans_amd/libsfx.so``) by
+``sfemans_amd/build.py``.  Loading fails loudly when it is missing: there is
+no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+__all__ = ["lib", "LIB_PATH", "SfxError", "check", "SIGNATURES"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsfx.so")
+
+SFX_OK = 0
+
+
+class SfxError(RuntimeError):
+    """A non-zero status returned by libsfx (message from sfx_last_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sfx error {code}: {msg}")
+        self.code = code
+
+
+_d = C.c_double
+_pd = C.POINTER(C.c_double)
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_pi32 = C.POINTER(C.c_int32)
+
+# name -> argument types (restype is int unless listed in _RESTYPE)
+SIGNATURES = {
+    "sfx_last_error": [],
+    "sfx_version": [],
+    "sfx_init": [_i32],
+    "sfx_fft_par_cross_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_par_dot_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_par_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_scalar_vect_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_tensor_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _i32, _vp],
+    "sfx_fft_par_cross_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_par_dot_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_par_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_scalar_vect_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
+    "sfx_fft_tensor_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+}
+_RESTYPE = {"sfx_last_error": C.c_char_p, "sfx_version": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load libsfx.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not found: build it with `python -m sfemans_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != SFX_OK:
+        msg = lib().sfx_last_error()
+        raise SfxError(rc, msg.decode() if msg else "")

@@ -1,0 +1,207 @@
+// Body of the FFT_PAR_*_DCL kernels (compiled for gfx950 and for the CPU emulator).
+//
+// Replaces, for nb_procs = 1, the per-point pipeline of
+// SOURCES/POST_PROCESSING/fft_parallel.F90: FFT_PAR_CROSS_PROD_DCL (:147-328),
+// FFT_PAR_DOT_PROD_DCL (:330-500), FFT_PAR_PROD_DCL (:502-665),
+// FFT_SCALAR_VECT_DCL pb=1 (:1021-1220) and FFT_TENSOR_DCL without
+// opt_tension (:1543-1796).  Inputs and outputs keep the reference layout
+// V(np, nb_field, m_max) column-major (point index fastest), so a wave's
+// loads and stores for one (column, mode) touch PW consecutive points.
+#pragma once
+#include "sfx_engine.h"
+#include "sfx_kernels.h"
+
+namespace sfx {
+
+template <int OP> struct OpT;
+// NF: real physical fields after the inverse transform; NO: real outputs per
+// group; NG: output groups processed one after the other (bounds registers).
+template <> struct OpT<OP_CROSS>  { static constexpr int NF = 6, NO = 3, NG = 1; };
+template <> struct OpT<OP_DOT>    { static constexpr int NF = 6, NO = 1, NG = 1; };
+template <> struct OpT<OP_PROD>   { static constexpr int NF = 2, NO = 1, NG = 1; };
+template <> struct OpT<OP_SVECT1> { static constexpr int NF = 4, NO = 3, NG = 1; };
+template <> struct OpT<OP_TENSOR> { static constexpr int NF = 6, NO = 3, NG = 3; };
+
+// Field f -> (array, first column, columns of that array, element count).
+template <int OP>
+SFX_DEV const double* field_base(const FftProdArgs& a, int f, int& col, int& nb, int64_t& n) {
+  const bool first = (OP == OP_PROD) ? (f == 0) : (f < 3);
+  if (OP == OP_PROD) col = 0;
+  else if (OP == OP_SVECT1) col = first ? 2 * f : 0;
+  else col = first ? 2 * f : 2 * (f - 3);
+  nb = first ? a.nb1 : a.nb2;
+  n = first ? a.n1 : a.n2;
+  return first ? a.V1 : a.V2;
+}
+
+// Output group g at one physical point from the real fields u.
+template <int OP>
+SFX_DEV void apply_op(const double (&u)[6], int g, double (&o)[3]) {
+  if (OP == OP_CROSS) {
+    o[0] = u[1] * u[5] - u[2] * u[4];
+    o[1] = u[2] * u[3] - u[0] * u[5];
+    o[2] = u[0] * u[4] - u[1] * u[3];
+  } else if (OP == OP_DOT) {
+    o[0] = u[0] * u[3] + u[1] * u[4] + u[2] * u[5];
+  } else if (OP == OP_PROD) {
+    o[0] = u[0] * u[1];
+  } else if (OP == OP_SVECT1) {
+    o[0] = u[3] * u[0]; o[1] = u[3] * u[1]; o[2] = u[3] * u[2];
+  } else {  // tensor row g: V1_g * V2_b
+    o[0] = u[g] * u[3]; o[1] = u[g] * u[4]; o[2] = u[g] * u[5];
+  }
+}
+
+// Packed spectra of the complex signal made of fields (f1, f2) at point p:
+// Z at own modes m = l + L r, B (partner part) from mode MP - m.
+template <int MP, int OP>
+SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int l, int f1, int f2,
+                         double2 (&Z)[4], double2 (&B)[4]) {
+  constexpr int L = Shape<MP>::L;
+  int c1, c2, nb1, nb2;
+  int64_t n1, n2;
+  const double* b1 = field_base<OP>(a, f1, c1, nb1, n1);
+  const double* b2 = field_base<OP>(a, f2, c2, nb2, n2);
+  const int64_t s1 = a.np * nb1, s2 = a.np * nb2;   // mode strides
+  const int64_t o1 = p + a.np * c1, o2 = p + a.np * c2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = l + L * r;
+    double a1 = 0, bb1 = 0, a2 = 0, bb2 = 0;
+    if (valid && m < a.M) {
+      const int64_t i1 = o1 + s1 * m, i2 = o2 + s2 * m;
+      SFX_CHECK("V1 load", i1 + a.np, n1);
+      SFX_CHECK("V2 load", i2 + a.np, n2);
+      a1 = b1[i1]; bb1 = b1[i1 + a.np]; a2 = b2[i2]; bb2 = b2[i2 + a.np];
+    }
+    if (m == 0) Z[r] = make_double2(a1, a2);
+    else Z[r] = make_double2(0.5 * (a1 + bb2), 0.5 * (a2 - bb1));
+    const int k = MP - m;
+    double ka1 = 0, kb1 = 0, ka2 = 0, kb2 = 0;
+    if (valid && m > 0 && k < a.M) {
+      const int64_t i1 = o1 + s1 * k, i2 = o2 + s2 * k;
+      SFX_CHECK("V1 load", i1 + a.np, n1);
+      SFX_CHECK("V2 load", i2 + a.np, n2);
+      ka1 = b1[i1]; kb1 = b1[i1 + a.np]; ka2 = b2[i2]; kb2 = b2[i2 + a.np];
+    }
+    B[r] = make_double2(0.5 * (ka1 - kb2), 0.5 * (kb1 + ka2));
+  }
+}
+
+// Pair Q[m] (lo) with Q[N-m] (hi of the partner lane) and store the (cos,sin)
+// coefficients of the two real outputs packed in each complex output signal.
+// fft_parallel.F90:291-324: scale 1/N, mode 0 as is, 2*conj for m >= 1.
+template <int MP, int NOC>
+SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4], int pt, int l,
+                        double2* lds, double (&oa)[NOC][2][4], double (&ob)[NOC][2][4]) {
+  constexpr int L = Shape<MP>::L, N = Shape<MP>::N, WC = Shape<MP>::WAVE_CPLX;
+#pragma unroll
+  for (int t = 0; t < NOC; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = t * WC + lds_slot<MP>(pt, l + L * r);
+      SFX_CHECK("lds pair write", i, NOC * WC);
+      lds[i] = hi[t][r];
+    }
+  SFX_SYNC();
+  double2 qn[NOC][4];
+#pragma unroll
+  for (int t = 0; t < NOC; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = t * WC + lds_slot<MP>(pt, (MP - (l + L * r)) & (MP - 1));
+      SFX_CHECK("lds pair read", i, NOC * WC);
+      qn[t][r] = lds[i];
+    }
+  SFX_SYNC();
+  const double invN = 1.0 / (double)N;
+#pragma unroll
+  for (int t = 0; t < NOC; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool m0 = (l + L * r) == 0;
+      const double2 Qm = lo[t][r];
+      const double2 Qn = m0 ? Qm : qn[t][r];
+      oa[t][0][r] = m0 ? Qm.x * invN : (Qm.x + Qn.x) * invN;
+      ob[t][0][r] = m0 ? 0.0 : (Qn.y - Qm.y) * invN;
+      oa[t][1][r] = m0 ? Qm.y * invN : (Qm.y + Qn.y) * invN;
+      ob[t][1][r] = m0 ? 0.0 : (Qm.x - Qn.x) * invN;
+    }
+}
+
+template <int MP, int OP>
+SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, double2* smem) {
+  using S = Shape<MP>;
+  constexpr int L = S::L, PW = S::PW;
+  constexpr int NF = OpT<OP>::NF, NS = NF / 2, NO = OpT<OP>::NO, NG = OpT<OP>::NG;
+  constexpr int NOC = (NO + 1) / 2;
+  constexpr int GX = 3;                           // signals per LDS exchange
+  const int wave = tid >> 6, lane = tid & 63;
+  const int pt = lane / L, l = lane % L;
+  double2* lds = smem + wave * (GX * S::WAVE_CPLX);
+  const int64_t p = (bid * wpb + wave) * PW + pt;
+  const bool valid = p < a.np;
+
+  // inverse: NS complex signals x 3 branches
+  double2 y[3 * NS][4];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double2 Z[4], B[4];
+    load_signal<MP, OP>(a, p, valid, l, 2 * s, 2 * s + 1, Z, B);
+    prestep<MP>(Z, B, l, y[3 * s + 0], y[3 * s + 1], y[3 * s + 2], a.wN);
+  }
+  inverse_fft<MP, 3 * NS, GX>(y, pt, l, lds, a.wMP);
+
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    // pointwise op on the 3 branches x 4 positions held by this lane
+    double2 q[3 * NOC][4];
+#pragma unroll
+    for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double u[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < NS; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
+        double o[3] = {0, 0, 0};
+        apply_op<OP>(u, g, o);
+#pragma unroll
+        for (int t = 0; t < NOC; ++t)
+          q[3 * t + j1][r] = make_double2(o[2 * t], (2 * t + 1 < NO) ? o[2 * t + 1] : 0.0);
+      }
+    forward_fft<MP, 3 * NOC, GX>(q, pt, l, lds, a.wMP);
+
+    double2 lo[NOC][4], hi[NOC][4];
+#pragma unroll
+    for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], a.wN);
+    double oa[NOC][2][4], ob[NOC][2][4];
+    pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
+    if (!valid) continue;
+#pragma unroll
+    for (int t = 0; t < NOC; ++t)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int o = 2 * t + h;
+        if (o >= NO) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = l + L * r;
+          if (m >= a.M) continue;
+          if (OP == OP_TENSOR) {
+            // V_out(g, p, 2o:2o+1, m) in a (3, np, 6, M) array
+            const int64_t base = g + 3 * (p + a.np * (int64_t)(2 * o + 6 * m));
+            SFX_CHECK("Vout store", base + 3 * a.np, a.nout);
+            a.Vout[base] = oa[t][h][r];
+            a.Vout[base + 3 * a.np] = ob[t][h][r];
+          } else {
+            const int64_t base = p + a.np * (int64_t)(2 * o + 2 * NO * m);
+            SFX_CHECK("Vout store", base + a.np, a.nout);
+            a.Vout[base] = oa[t][h][r];
+            a.Vout[base + a.np] = ob[t][h][r];
+          }
+        }
+      }
+  }
+}
+
+}  // namespace sfx

@@ -1,0 +1,27 @@
+// Portability layer: the kernel bodies compile for gfx950 (hipcc) and, with
+// SFX_EMULATE defined, for the host CPU, where tests run one work-group as
+// std::threads with a barrier in place of __syncthreads and check every
+// global and LDS index against its buffer.
+#pragma once
+#include <stdint.h>
+
+#ifdef SFX_EMULATE
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+struct double2 { double x, y; };
+inline double2 make_double2(double x, double y) { return double2{x, y}; }
+using std::fma;
+#define SFX_DEV inline
+#define __restrict__
+void sfx_emu_sync();
+[[noreturn]] void sfx_emu_oob(const char* what, long long idx, long long n, const char* file, int line);
+#define SFX_SYNC() sfx_emu_sync()
+#define SFX_CHECK(what, idx, n) \
+  do { if ((long long)(idx) < 0 || (long long)(idx) >= (long long)(n)) sfx_emu_oob(what, (long long)(idx), (long long)(n), __FILE__, __LINE__); } while (0)
+#else
+#include <hip/hip_runtime.h>
+#define SFX_DEV __device__ __forceinline__
+#define SFX_SYNC() __syncthreads()
+#define SFX_CHECK(what, idx, n) do { } while (0)
+#endif

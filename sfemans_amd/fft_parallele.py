@@ -1,0 +1,156 @@
+"""Python mirror of the reference module ``fft_parallele`` (one GPU).
+
+Same routine names, argument order and meaning as
+SOURCES/POST_PROCESSING/fft_parallel.F90 (``communicator`` must be ``None``:
+one rank; ``V_out`` is INTENT(OUT) and is filled in place).  Arrays are
+either Fortran-ordered numpy arrays (host path: staged to the GPU and back)
+or CUDA tensors with Fortran strides (device path: no copies, launched on
+torch's current stream).  ``fortran_tensor`` builds such tensors.
+
+Errors raise ``SfxError`` where the reference would STOP.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import lib, check
+
+__all__ = [
+    "FFT_PAR_CROSS_PROD_DCL", "FFT_PAR_DOT_PROD_DCL", "FFT_PAR_PROD_DCL",
+    "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "fortran_tensor", "fortran_empty",
+    "is_device", "ptr", "fdims", "stream_ptr",
+]
+
+
+def is_device(x) -> bool:
+    return hasattr(x, "is_cuda") and x.is_cuda
+
+
+def fdims(x):
+    """Fortran dimensions of an array/tensor with column-major layout."""
+    if is_device(x):
+        shape, st = tuple(x.shape), tuple(x.stride())
+        exp = 1
+        for d, s in zip(shape, st):
+            if d > 1 and s != exp:
+                raise ValueError(f"tensor is not Fortran-contiguous: shape {shape} strides {st}")
+            exp *= d
+        if x.dtype.__str__() != "torch.float64":
+            raise TypeError("tensors must be float64")
+        return shape
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.f_contiguous):
+        raise TypeError("arrays must be Fortran-contiguous float64 numpy arrays")
+    return x.shape
+
+
+def ptr(x):
+    if x is None:
+        return None
+    if is_device(x):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+def stream_ptr():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def fortran_tensor(a, device="cuda"):
+    """Copy a numpy array to a CUDA tensor with the same column-major layout."""
+    import torch
+    a = np.asfortranarray(a, dtype=np.float64)
+    t = torch.as_tensor(np.ascontiguousarray(a.T), device=device)
+    return t.permute(*reversed(range(a.ndim)))
+
+
+def fortran_empty(shape, device="cuda"):
+    import torch
+    t = torch.empty(tuple(reversed(shape)), dtype=torch.float64, device=device)
+    return t.permute(*reversed(range(len(shape))))
+
+
+def _defaults(np_, M, nb_procs, bloc_size, m_max_pad):
+    if bloc_size is None:
+        bloc_size = np_ // nb_procs + 1
+    if m_max_pad is None:
+        m_max_pad = (3 * M * nb_procs) // 2
+    return bloc_size, m_max_pad
+
+
+def _single_rank(communicator):
+    if communicator is not None and getattr(communicator, "size", 1) != 1:
+        raise ValueError("one GPU: the Fourier communicator must have a single rank")
+
+
+def _temps_ptr(temps):
+    if temps is None:
+        return None
+    if not (isinstance(temps, np.ndarray) and temps.dtype == np.float64 and temps.size >= 3):
+        raise TypeError("temps must be a float64 array of 3 entries")
+    return temps.ctypes.data
+
+
+def _binary(name, communicator, V1_in, V2_in, V_out, nb_procs, bloc_size, m_max_pad, temps,
+            extra=()):
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    fdims(V2_in), fdims(V_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        dev = getattr(L, f"sfx_{name}_dev")
+        if name == "fft_scalar_vect_dcl":
+            check(dev(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, extra[1], m_max_pad, stream_ptr()))
+        else:
+            check(dev(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, m_max_pad, stream_ptr()))
+        return V_out
+    fn = getattr(L, f"sfx_{name}")
+    if name == "fft_par_prod_dcl":
+        check(fn(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
+    elif name == "fft_scalar_vect_dcl":
+        nb2 = fdims(V2_in)[1]
+        check(fn(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, nb, nb2, M, extra[1], nb_procs, bloc_size,
+                 m_max_pad, _temps_ptr(temps)))
+    elif name == "fft_tensor_dcl":
+        check(fn(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, nb, M, nb_procs, bloc_size, m_max_pad,
+                 int(bool(extra[0])), _temps_ptr(temps)))
+    else:
+        check(fn(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, nb, M, nb_procs, bloc_size, m_max_pad,
+                 _temps_ptr(temps)))
+    return V_out
+
+
+def FFT_PAR_CROSS_PROD_DCL(communicator, V1_in, V2_in, V_out, nb_procs=1, bloc_size=None,
+                           m_max_pad=None, temps=None):
+    """V_out(np,6,m) = V1 x V2 (fft_parallel.F90:147)."""
+    return _binary("fft_par_cross_prod_dcl", communicator, V1_in, V2_in, V_out, nb_procs,
+                   bloc_size, m_max_pad, temps)
+
+
+def FFT_PAR_DOT_PROD_DCL(communicator, V1_in, V2_in, c_out, nb_procs=1, bloc_size=None,
+                         m_max_pad=None, temps=None):
+    """c_out(np,2,m) = V1 . V2 (fft_parallel.F90:330)."""
+    return _binary("fft_par_dot_prod_dcl", communicator, V1_in, V2_in, c_out, nb_procs,
+                   bloc_size, m_max_pad, temps)
+
+
+def FFT_PAR_PROD_DCL(communicator, c1_in, c2_in, c_out, nb_procs=1, bloc_size=None,
+                     m_max_pad=None, temps=None):
+    """c_out(np,2,m) = c1 * c2 (fft_parallel.F90:502)."""
+    return _binary("fft_par_prod_dcl", communicator, c1_in, c2_in, c_out, nb_procs,
+                   bloc_size, m_max_pad, temps)
+
+
+def FFT_SCALAR_VECT_DCL(communicator, V1_in, V2_in, V_out, pb, nb_procs=1, bloc_size=None,
+                        m_max_pad=None, temps=None):
+    """V_out(np,6,m) = s * v for pb = 1 (fft_parallel.F90:1021)."""
+    return _binary("fft_scalar_vect_dcl", communicator, V1_in, V2_in, V_out, nb_procs,
+                   bloc_size, m_max_pad, temps, extra=(None, pb))
+
+
+def FFT_TENSOR_DCL(communicator, V1_in, V2_in, V_out, nb_procs=1, bloc_size=None,
+                   m_max_pad=None, temps=None, opt_tension=None):
+    """V_out(3,np,6,m) = V1 (x) V2 (fft_parallel.F90:1543)."""
+    return _binary("fft_tensor_dcl", communicator, V1_in, V2_in, V_out, nb_procs,
+                   bloc_size, m_max_pad, temps, extra=(opt_tension, None))

@@ -1,0 +1,148 @@
+"""FFT_PAR_* products on the GPU through the C-ABI vs the oracle.
+
+Tolerance: relative max-norm error <= 1e-12 (FP64; the GPU grid N = 3*MP
+and the reference grid N = 2*m_max_pad-1 agree up to rounding).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import fft_par as F  # noqa: E402
+from sfemans_amd import fft_parallele as FP  # noqa: E402
+from sfemans_amd._lib import SfxError  # noqa: E402
+from tests.known_answers import ka_test1_fields, ka_test1_rot_cross_v  # noqa: E402
+
+TOL = 1e-12
+
+ROUTINES = {
+    "cross": (6, 6, 6, FP.FFT_PAR_CROSS_PROD_DCL, lambda a, b, mp: F.fft_par_cross_prod_dcl(a, b, mp)),
+    "dot": (6, 6, 2, FP.FFT_PAR_DOT_PROD_DCL, lambda a, b, mp: F.fft_par_dot_prod_dcl(a, b, mp)),
+    "prod": (2, 2, 2, FP.FFT_PAR_PROD_DCL, lambda a, b, mp: F.fft_par_prod_dcl(a, b, mp)),
+}
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+
+
+def fr(shape, seed):
+    return np.asfortranarray(np.random.default_rng(seed).standard_normal(shape))
+
+
+@pytest.mark.parametrize("name", list(ROUTINES))
+@pytest.mark.parametrize("M,npt", [(1, 5), (2, 17), (3, 1), (4, 64), (5, 33), (8, 1000), (16, 257),
+                                   (32, 129), (64, 77), (100, 21), (128, 40), (256, 9)])
+def test_host_path_matches_oracle(name, M, npt):
+    nb1, nb2, nbo, fn, ref = ROUTINES[name]
+    print(f"case {name} M={M} np={npt}", flush=True)
+    V1, V2 = fr((npt, nb1, M), M), fr((npt, nb2, M), M + 1)
+    out = np.zeros((npt, nbo, M), order="F")
+    temps = np.full(3, -1.0)
+    fn(None, V1, V2, out, 1, npt + 1, 3 * M // 2, temps)
+    assert temps[2] == 0.0 and temps[1] >= 0.0
+    assert rel(out, ref(V1, V2, 3 * M // 2)) <= TOL
+
+
+@pytest.mark.parametrize("M", [3, 8, 64])
+def test_scalar_vect_and_tensor(M):
+    npt = 50
+    print(f"case svect/tensor M={M}", flush=True)
+    V1, S2, V2 = fr((npt, 6, M), 1), fr((npt, 2, M), 2), fr((npt, 6, M), 3)
+    out = np.zeros((npt, 6, M), order="F")
+    FP.FFT_SCALAR_VECT_DCL(None, V1, S2, out, 1)
+    assert rel(out, F.fft_scalar_vect_dcl(V1, S2, 1, 3 * M // 2)) <= TOL
+    T = np.zeros((3, npt, 6, M), order="F")
+    FP.FFT_TENSOR_DCL(None, V1, V2, T)
+    assert rel(T, F.fft_tensor_dcl(V1, V2, 3 * M // 2)) <= TOL
+
+
+def test_zero_points_and_device_path():
+    M = 8
+    out = np.zeros((0, 6, M), order="F")
+    FP.FFT_PAR_CROSS_PROD_DCL(None, np.zeros((0, 6, M), order="F"), np.zeros((0, 6, M), order="F"), out)
+    V1, V2 = fr((300, 6, M), 5), fr((300, 6, M), 6)
+    d1, d2 = FP.fortran_tensor(V1), FP.fortran_tensor(V2)
+    dout = FP.fortran_empty((300, 6, M))
+    FP.FFT_PAR_CROSS_PROD_DCL(None, d1, d2, dout)
+    torch.cuda.synchronize()
+    got = dout.cpu().numpy()
+    assert rel(got, F.fft_par_cross_prod_dcl(V1, V2, 12)) <= TOL
+
+
+def test_test1_known_answer_on_gpu():
+    pts = np.random.default_rng(9).uniform([0.05, -1.0], [1.0, 1.0], size=(257, 2))
+    V, Rot = ka_test1_fields(pts[:, 0], pts[:, 1])
+    V1 = np.zeros((257, 6, 3), order="F"); V2 = np.zeros((257, 6, 3), order="F")
+    V1[:, :, 1] = Rot
+    V2[:, :, 1] = V
+    out = np.zeros_like(V1)
+    FP.FFT_PAR_CROSS_PROD_DCL(None, V1, V2, out, 1, 258, 4)
+    e0, e2 = ka_test1_rot_cross_v(V, Rot)
+    s = np.max(np.abs(e2))
+    assert np.max(np.abs(out[:, :, 0] - e0)) <= 1e-13 * s
+    assert np.max(np.abs(out[:, :, 2] - e2)) <= 1e-13 * s
+    assert np.max(np.abs(out[:, :, 1])) <= 1e-13 * s
+
+
+def test_error_codes_mirror_reference():
+    z = np.zeros((4, 5, 2), order="F")
+    with pytest.raises(SfxError) as e:
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z, z, z.copy())
+    assert e.value.code == 1
+    z6 = np.zeros((4, 6, 2), order="F")
+    with pytest.raises(SfxError) as e:
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(), 2)
+    assert e.value.code == 2
+    with pytest.raises(SfxError) as e:        # aliased grid
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(), 1, 5, 2)
+    assert e.value.code == 2
+    with pytest.raises(SfxError) as e:        # bloc_size too small
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(), 1, 3, 3)
+    assert e.value.code == 1
+    s2 = np.zeros((4, 2, 2), order="F")
+    with pytest.raises(SfxError) as e:
+        FP.FFT_SCALAR_VECT_DCL(None, z6, s2, z6.copy(), 2)
+    assert e.value.code == 2
+
+
+def test_nan_propagates_without_crash():
+    V1, V2 = fr((40, 6, 8), 1), fr((40, 6, 8), 2)
+    V1[3, 2, 5] = np.nan
+    out = np.zeros_like(V1)
+    FP.FFT_PAR_CROSS_PROD_DCL(None, V1, V2, out)
+    assert np.isnan(out[3]).any() and np.isfinite(np.delete(out, 3, axis=0)).all()
+
+
+def test_full_size_properties():
+    """Gauss-point arrays of the 224x448 mesh (1,404,928 points, m_max = 64)."""
+    npt, M = 1404928, 64
+    print("case full-size cross", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    a = torch.randn((M, 6, npt), generator=g, device="cuda", dtype=torch.float64).permute(2, 1, 0)
+    b = torch.randn((M, 6, npt), generator=g, device="cuda", dtype=torch.float64).permute(2, 1, 0)
+    ab = FP.fortran_empty((npt, 6, M)); ba = FP.fortran_empty((npt, 6, M))
+    FP.FFT_PAR_CROSS_PROD_DCL(None, a, b, ab)
+    FP.FFT_PAR_CROSS_PROD_DCL(None, b, a, ba)
+    torch.cuda.synchronize()
+    scale = ab.abs().max().item()
+    assert (ab + ba).abs().max().item() <= 1e-13 * scale          # antisymmetry
+    # sampled points against the oracle
+    idx = torch.randint(0, npt, (2000,), generator=g, device="cuda")
+    A = a[idx].cpu().numpy(); B = b[idx].cpu().numpy()
+    ref = F.fft_par_cross_prod_dcl(np.asfortranarray(A), np.asfortranarray(B), 96)
+    assert rel(ab[idx].cpu().numpy(), ref) <= TOL
+    # linearity in the first argument
+    c = 2.0 * a
+    ac = FP.fortran_empty((npt, 6, M))
+    FP.FFT_PAR_CROSS_PROD_DCL(None, c, b, ac)
+    torch.cuda.synchronize()
+    assert (ac - 2.0 * ab).abs().max().item() <= 1e-13 * scale
