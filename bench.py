@@ -62,6 +62,46 @@ def bench_fftprod(args, torch):
     return step, units, bytes_launch, flops_launch, cfg
 
 
+def _fields_device(torch, npts, M, decay, seeds):
+    """Seeded N(0,1)*exp(-m/decay) nodal fields generated on the GPU (Fortran strides)."""
+    out = []
+    dec = torch.exp(-torch.arange(M, device="cuda", dtype=torch.float64) / decay)
+    for sd in seeds:
+        g = torch.Generator(device="cuda").manual_seed(sd)
+        t = torch.randn((M, 6, npts), generator=g, device="cuda", dtype=torch.float64) * dec[:, None, None]
+        t[0, 1::2, :] = 0.0
+        out.append(t.permute(2, 1, 0))
+    return out
+
+
+def bench_mhd(args, torch):
+    """MHD nonlinear terms of one explicit step on the 224x448 P2 mesh, m_max = 64."""
+    from sfemans_amd import fft_parallele as FP
+    from sfemans_amd import nonlinear as NL
+    from sfemans_amd.mesh import rectangle_p2
+    m = rectangle_p2(224, 448)
+    mesh = NL.Mesh.from_p2(m)
+    M = 64
+    modes = list(range(M))
+    u, H, B = _fields_device(torch, m.np, M, 4.0, (0x5F3E + 2, 0x5F3E + 3, 0x5F3E + 4))
+    rotv = FP.fortran_empty((m.n_gauss, 6, M))
+    uxb = FP.fortran_empty((m.np, 6, M))
+
+    def step():
+        NL.nl_rotv_mhd(mesh, modes, u, H, B, 1.0, rotv)
+        FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb)
+
+    units = m.n_gauss * M
+    # dominant kernel: the fused Gauss-point MHD term (reads u, H, B nodal once, writes rotv_v)
+    bytes_launch = (3 * m.np * 6 * M + m.n_gauss * 6 * M) * 8
+    flops_launch = 2 * flops_fft_unit(M) * units
+    cfg = {"workload": "MHD nonlinear terms, 224x448 P2 mesh (200,704 triangles), m_max=64: "
+                       "(curl u)xu - (curl H)xB at Gauss points + u x B at nodes",
+           "me": m.me, "np": m.np, "gauss_points": m.n_gauss, "m_max": M, "grid": 192,
+           "kernel": "gauss_kernel<64,MHD>"}
+    return step, units, bytes_launch, flops_launch, cfg
+
+
 def cpu_baseline_fftprod(sample_s: float):
     from oracle import fft_par as F
     M = 64
@@ -96,7 +136,7 @@ def main():
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import torch
     torch.cuda.set_device(0)
-    step, units, bytes_launch, flops_launch, cfg = {"fftprod": bench_fftprod}[args.workload](args, torch)
+    step, units, bytes_launch, flops_launch, cfg = {"fftprod": bench_fftprod, "mhd": bench_mhd}[args.workload](args, torch)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

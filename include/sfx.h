@@ -99,6 +99,96 @@ int sfx_fft_scalar_vect_dcl_dev(const double* V1_in, const double* V2_in, double
 int sfx_fft_tensor_dcl_dev(const double* V1_in, const double* V2_in, double* V_out,
                            int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Meshes.  A P2 meridian mesh is registered once (the reference builds it at
+ * initialisation: INIT, SOURCES/INITIALISATION/initialization.F90:1153-1222)
+ * and is immutable afterwards.  jj(6,me) is 1-based with the reference local
+ * numbering (mesh%jj, SOURCES/MESH/def_type_mesh.F90:74-112), rr(2,np) the
+ * node coordinates (r, z).  Elements must be straight-sided (each midpoint
+ * node at the middle of its edge), as built by create_iso_grid_distributed.
+ * ------------------------------------------------------------------------- */
+typedef struct sfx_mesh_s* sfx_mesh_t;
+
+int sfx_mesh_create(const int32_t* jj, const double* rr, int32_t n_w, int32_t l_G, int64_t me, int64_t np,
+                    sfx_mesh_t* out);
+int sfx_mesh_destroy(sfx_mesh_t mesh);
+int sfx_mesh_info(sfx_mesh_t mesh, int64_t* me, int64_t* np);
+
+/* ---------------------------------------------------------------------------
+ * Fused Gauss-point nonlinear terms (interpolation + curl/gradient +
+ * de-aliased product), output at the l_G*me Gauss points, index
+ * (m-1)*l_G + l.  list_mode(m_max_c) holds the mode numbers (the product
+ * treats slots as contiguous modes, fft_parallel.F90:19-25).
+ * ------------------------------------------------------------------------- */
+
+/* Replaces smb_cross_prod_gauss_sft_par
+ * (SOURCES/SIMULATION/subroutine_ns_with_velocity.F90:514): V_out(G,6,m) =
+ * (curl V_in + 2 Omega) x V_in with the precession vector of taux_precession,
+ * angle_s_pi when precession_in != 0. */
+int sfx_smb_cross_prod_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
+                                     const double* V_in, double* V_out, int32_t precession_in,
+                                     double taux_precession, double angle_s_pi);
+int sfx_smb_cross_prod_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                         const double* V_in, double* V_out, int32_t precession_in,
+                                         double taux_precession, double angle_s_pi, void* stream);
+
+/* Replaces smb_CurlH_cross_B_gauss_sft_par (subroutine_ns_with_velocity.F90:632):
+ * V_out(G,6,m) = (curl V_in) x W_in, V_in and W_in nodal (np,6,m). */
+int sfx_smb_curlh_cross_b_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
+                                        const double* V_in, const double* W_in, double* V_out);
+int sfx_smb_curlh_cross_b_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                            const double* V_in, const double* W_in, double* V_out, void* stream);
+
+/* Replaces smb_ugradc_gauss_fft_par
+ * (SOURCES/SIMULATION/subroutine_temperature_with_T.F90:247):
+ * c_out(G,2,m) = (hc grad c).u + (div u)(hc c), heat_capa_in(me) per element. */
+int sfx_smb_ugradc_gauss_fft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
+                                 const double* heat_capa_in, const double* V_in, const double* c_in,
+                                 double* c_out);
+int sfx_smb_ugradc_gauss_fft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                     const double* heat_capa_dev, const double* V_in, const double* c_in,
+                                     double* c_out, void* stream);
+
+/* The MHD nonlinear term of the Navier-Stokes step in one launch:
+ * rotv_v(G,6,m) = (curl u) x u - coeff_lorentz (curl H) x B, as formed by
+ * subroutine_ns_with_velocity.F90:227 and :246-248 from two smb_* calls. */
+int sfx_nl_rotv_mhd(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* u,
+                    const double* H, const double* B, double coeff_lorentz, double* rotv_v);
+int sfx_nl_rotv_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
+                        const double* H, const double* B, double coeff_lorentz, double* rotv_v, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Right-hand-side assembly into dense local vectors.  On one rank the PETSc
+ * index of component ki of node i is (ki-1)*np + i
+ * (SOURCES/INITIALISATION/st_matrix.F90:101), so vb_145/vb_236 are (np,3)
+ * per mode.  Sums are formed node by node in element order (the order of the
+ * reference's VecSetValues loop), without atomics.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces rhs_3x3 without opt_tensor/opt_tensor_scaln_bdy
+ * (SOURCES/SOLVER/assembling_rhs.F90:5): one mode, rhs_in(G,6) ->
+ * vb_145(np,3), vb_236(np,3).  `mode` is accepted for signature parity. */
+int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_145, double* vb_236);
+/* All modes at once on the device: rhs_gauss(G,6,m) -> vb_145(np,3,m),
+ * vb_236(np,3,m), each multiplied by `scale`. */
+int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, double* vb_145, double* vb_236,
+                    double scale, void* stream);
+
+/* Replaces qs_00_gauss (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) from
+ * heat_capa(me), nodal ff(np) and Gauss ff_gauss(G). */
+int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, const double* ff_gauss,
+                    double* vect);
+/* ncols independent right-hand sides: ff(np,ncols), ff_gauss(G,ncols) -> vect(np,ncols). */
+int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa, const double* ff,
+                        const double* ff_gauss, double* vect, void* stream);
+
+/* Nonlinear share of the Navier-Stokes right-hand sides of an MHD step,
+ * all modes: vb = rhs_3x3(-[(curl u) x u - coeff_lorentz (curl H) x B]).
+ * work_gauss is a caller-provided (G,6,m) device buffer. */
+int sfx_nl_rhs_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
+                       const double* H, const double* B, double coeff_lorentz, double* work_gauss,
+                       double* vb_145, double* vb_236, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,22 @@ SIGNATURES = {
     "sfx_fft_par_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_scalar_vect_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_tensor_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_mesh_create": [_vp, _vp, _i32, _i32, _i64, _i64, _vp],
+    "sfx_mesh_destroy": [_vp],
+    "sfx_mesh_info": [_vp, _vp, _vp],
+    "sfx_smb_cross_prod_gauss_sft_par": [_vp, _vp, _i32, _vp, _vp, _i32, _d, _d],
+    "sfx_smb_cross_prod_gauss_sft_par_dev": [_vp, _vp, _i32, _vp, _vp, _i32, _d, _d, _vp],
+    "sfx_smb_curlh_cross_b_gauss_sft_par": [_vp, _vp, _i32, _vp, _vp, _vp],
+    "sfx_smb_curlh_cross_b_gauss_sft_par_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp],
+    "sfx_smb_ugradc_gauss_fft_par": [_vp, _vp, _i32, _vp, _vp, _vp, _vp],
+    "sfx_smb_ugradc_gauss_fft_par_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp],
+    "sfx_nl_rotv_mhd": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp],
+    "sfx_nl_rotv_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp],
+    "sfx_rhs_3x3": [_vp, _i32, _vp, _vp, _vp],
+    "sfx_rhs_3x3_dev": [_vp, _i32, _vp, _vp, _vp, _d, _vp],
+    "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],
+    "sfx_qs_00_gauss_dev": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
+    "sfx_nl_rhs_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {"sfx_last_error": C.c_char_p, "sfx_version": C.c_char_p}
 

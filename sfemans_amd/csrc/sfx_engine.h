@@ -70,12 +70,41 @@ SFX_DEV int stage_pos(int s, int l, int r) {
   return 4 * l + r;
 }
 
-// LDS slot of (point, signal, position) inside a wave region.  An XOR of the
-// block bits into the low bits spreads the strided patterns of the late stages.
+// LDS slot of (point, position) inside a wave region for exchange `ex`
+// (ex = s for the exchange between stages s and s+1, ex = 4 for the mode
+// pairing).  q = pt*MP + pos indexes the 256 slots of one signal; the low
+// four bits are XORed with a GF(2)-linear function of bits 2..7 of q.  The
+// per-exchange masks were found by exhaustive search so that every 8- and
+// 16-lane group of every access pattern hits distinct 16-byte bank groups;
+// masks on bits 2 and 3 only touch bits 0 and 1, so the map is a bijection.
+constexpr unsigned char SWZ[7][5][6] = {
+  {{0,0,0,0,0,0}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {0,0,0,0,0,0}},  // MP=4
+  {{1,3,15,2,6,6}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {3,2,5,7,3,7}},  // MP=8
+  {{0,3,14,6,9,4}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {3,0,15,7,11,10}},  // MP=16
+  {{0,2,14,12,15,10}, {2,3,7,8,13,7}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {3,0,8,8,0,4}},  // MP=32
+  {{2,0,4,14,1,5}, {3,2,12,5,15,3}, {0,0,0,0,0,0}, {0,0,0,0,0,0}, {0,0,0,0,12,3}},  // MP=64
+  {{0,2,15,13,15,14}, {1,2,14,8,11,11}, {0,2,15,6,0,1}, {0,0,0,0,0,0}, {3,0,0,0,0,15}},  // MP=128
+  {{3,3,9,2,13,3}, {0,2,7,12,8,14}, {3,1,12,8,0,14}, {0,0,0,0,0,0}, {0,0,0,0,0,0}},  // MP=256
+};
+
 template <int MP>
-SFX_DEV int lds_slot(int pt, int pos) {
-  const int sw = pos ^ ((pos >> 4) & 3);
-  return pt * MP + sw;
+SFX_DEV int lds_slot(int ex, int pt, int pos) {
+  const int q = pt * MP + pos;
+  const unsigned char* A = SWZ[Shape<MP>::LOG2 - 2][ex];
+  int x = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) x ^= ((q >> (2 + i)) & 1) * A[i];
+  return q ^ x;
+}
+
+// Work-groups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Remap the hardware block id so that each XCD walks a contiguous range of
+// logical blocks: neighbouring points, which share cache lines, then meet in
+// the same L2.  The map is a bijection on [0, nblk).
+SFX_DEV int64_t xcd_block(int64_t b, int64_t nblk) {
+  const int64_t per = nblk / 8, rem = nblk % 8;
+  const int64_t x = b % 8, local = b / 8;
+  return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
 // Inverse (e^{+i}) radix stage s on the 4 values of lane l, in place.
@@ -137,20 +166,20 @@ SFX_DEV void exchange(double2 (*v)[4], int sa, int sb, int pt, int l, double2* l
   for (int g = 0; g < NS; ++g)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = g * WC + lds_slot<MP>(pt, stage_pos<MP>(sa, l, r));
+      const int i = g * WC + lds_slot<MP>(sa < sb ? sa : sb, pt, stage_pos<MP>(sa, l, r));
       SFX_CHECK("lds exchange write", i, NS * WC);
       lds[i] = v[g][r];
     }
-  SFX_SYNC();
+  SFX_WAVE_SYNC();
 #pragma unroll
   for (int g = 0; g < NS; ++g)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = g * WC + lds_slot<MP>(pt, stage_pos<MP>(sb, l, r));
+      const int i = g * WC + lds_slot<MP>(sa < sb ? sa : sb, pt, stage_pos<MP>(sb, l, r));
       SFX_CHECK("lds exchange read", i, NS * WC);
       v[g][r] = lds[i];
     }
-  SFX_SYNC();
+  SFX_WAVE_SYNC();
 }
 
 // Full inverse length-MP transform of NS signals (register arrays v[NS][4]),

@@ -3,12 +3,12 @@
 
 namespace sfx {
 
-constexpr int FFTPROD_WPB = 2;   // waves per work-group
+constexpr int FFTPROD_WPB = 4;   // waves per work-group
 
 template <int MP, int OP>
 __global__ void __launch_bounds__(64 * FFTPROD_WPB) fftprod_kernel(FftProdArgs a) {
   extern __shared__ double2 smem[];
-  fftprod_body<MP, OP>(a, threadIdx.x, blockIdx.x, FFTPROD_WPB, smem);
+  fftprod_body<MP, OP>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), FFTPROD_WPB, smem);
 }
 
 template <int MP, int OP>
@@ -17,7 +17,7 @@ static hipError_t launch_mp(const FftProdArgs& a, hipStream_t st) {
   const int64_t pts_per_block = (int64_t)PW * FFTPROD_WPB;
   const int64_t nblk = (a.np + pts_per_block - 1) / pts_per_block;
   if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)FFTPROD_WPB * 3 * Shape<MP>::WAVE_CPLX * sizeof(double2);
+  const size_t shmem = fftprod_smem_cplx<MP>(FFTPROD_WPB) * sizeof(double2);
   hipLaunchKernelGGL((fftprod_kernel<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
   return hipGetLastError();
 }

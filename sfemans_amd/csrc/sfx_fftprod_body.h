@@ -63,27 +63,37 @@ SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int l, int
   const double* b1 = field_base<OP>(a, f1, c1, nb1, n1);
   const double* b2 = field_base<OP>(a, f2, c2, nb2, n2);
   const int64_t s1 = a.np * nb1, s2 = a.np * nb2;   // mode strides
-  const int64_t o1 = p + a.np * c1, o2 = p + a.np * c2;
+  // Loads are unconditional at clamped indices (selected to zero afterwards)
+  // so that all of them issue back to back instead of one branch at a time.
+  const int64_t pc = valid ? p : 0;
+  const int64_t o1 = pc + a.np * c1, o2 = pc + a.np * c2;
+  double v[4][8];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int m = l + L * r;
-    double a1 = 0, bb1 = 0, a2 = 0, bb2 = 0;
-    if (valid && m < a.M) {
-      const int64_t i1 = o1 + s1 * m, i2 = o2 + s2 * m;
-      SFX_CHECK("V1 load", i1 + a.np, n1);
-      SFX_CHECK("V2 load", i2 + a.np, n2);
-      a1 = b1[i1]; bb1 = b1[i1 + a.np]; a2 = b2[i2]; bb2 = b2[i2 + a.np];
-    }
+    const int k = MP - m;
+    const int mc = (m < a.M) ? m : 0;
+    const int kc = (m > 0 && k < a.M) ? k : 0;
+    const int64_t i1 = o1 + s1 * mc, i2 = o2 + s2 * mc, j1 = o1 + s1 * kc, j2 = o2 + s2 * kc;
+    SFX_CHECK("V1 load", i1 + a.np, n1);
+    SFX_CHECK("V2 load", i2 + a.np, n2);
+    SFX_CHECK("V1 load", j1 + a.np, n1);
+    SFX_CHECK("V2 load", j2 + a.np, n2);
+    v[r][0] = b1[i1]; v[r][1] = b1[i1 + a.np]; v[r][2] = b2[i2]; v[r][3] = b2[i2 + a.np];
+    v[r][4] = b1[j1]; v[r][5] = b1[j1 + a.np]; v[r][6] = b2[j2]; v[r][7] = b2[j2 + a.np];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = l + L * r;
+    const int k = MP - m;
+    const bool own = valid && m < a.M;
+    const bool par = valid && m > 0 && k < a.M;
+    const double a1 = own ? v[r][0] : 0.0, bb1 = own ? v[r][1] : 0.0;
+    const double a2 = own ? v[r][2] : 0.0, bb2 = own ? v[r][3] : 0.0;
+    const double ka1 = par ? v[r][4] : 0.0, kb1 = par ? v[r][5] : 0.0;
+    const double ka2 = par ? v[r][6] : 0.0, kb2 = par ? v[r][7] : 0.0;
     if (m == 0) Z[r] = make_double2(a1, a2);
     else Z[r] = make_double2(0.5 * (a1 + bb2), 0.5 * (a2 - bb1));
-    const int k = MP - m;
-    double ka1 = 0, kb1 = 0, ka2 = 0, kb2 = 0;
-    if (valid && m > 0 && k < a.M) {
-      const int64_t i1 = o1 + s1 * k, i2 = o2 + s2 * k;
-      SFX_CHECK("V1 load", i1 + a.np, n1);
-      SFX_CHECK("V2 load", i2 + a.np, n2);
-      ka1 = b1[i1]; kb1 = b1[i1 + a.np]; ka2 = b2[i2]; kb2 = b2[i2 + a.np];
-    }
     B[r] = make_double2(0.5 * (ka1 - kb2), 0.5 * (kb1 + ka2));
   }
 }
@@ -99,21 +109,21 @@ SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4]
   for (int t = 0; t < NOC; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = t * WC + lds_slot<MP>(pt, l + L * r);
+      const int i = t * WC + lds_slot<MP>(4, pt, l + L * r);
       SFX_CHECK("lds pair write", i, NOC * WC);
       lds[i] = hi[t][r];
     }
-  SFX_SYNC();
+  SFX_WAVE_SYNC();
   double2 qn[NOC][4];
 #pragma unroll
   for (int t = 0; t < NOC; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = t * WC + lds_slot<MP>(pt, (MP - (l + L * r)) & (MP - 1));
+      const int i = t * WC + lds_slot<MP>(4, pt, (MP - (l + L * r)) & (MP - 1));
       SFX_CHECK("lds pair read", i, NOC * WC);
       qn[t][r] = lds[i];
     }
-  SFX_SYNC();
+  SFX_WAVE_SYNC();
   const double invN = 1.0 / (double)N;
 #pragma unroll
   for (int t = 0; t < NOC; ++t)
@@ -129,6 +139,10 @@ SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4]
     }
 }
 
+// LDS per work-group: twiddles (4 MP complex) + per wave 3 signals x 256 complex.
+template <int MP>
+constexpr size_t fftprod_smem_cplx(int wpb) { return 4 * MP + (size_t)wpb * 3 * Shape<MP>::WAVE_CPLX; }
+
 template <int MP, int OP>
 SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, double2* smem) {
   using S = Shape<MP>;
@@ -138,7 +152,13 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
   constexpr int GX = 3;                           // signals per LDS exchange
   const int wave = tid >> 6, lane = tid & 63;
   const int pt = lane / L, l = lane % L;
-  double2* lds = smem + wave * (GX * S::WAVE_CPLX);
+  // twiddle tables staged in LDS once per work-group: wN (3 MP) then wMP (MP)
+  double2* tw = smem;
+  for (int i = tid; i < 4 * MP; i += 64 * wpb) tw[i] = (i < 3 * MP) ? a.wN[i] : a.wMP[i - 3 * MP];
+  SFX_SYNC();
+  const double2* wN = tw;
+  const double2* wMP = tw + 3 * MP;
+  double2* lds = smem + 4 * MP + wave * (GX * S::WAVE_CPLX);
   const int64_t p = (bid * wpb + wave) * PW + pt;
   const bool valid = p < a.np;
 
@@ -148,9 +168,9 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
   for (int s = 0; s < NS; ++s) {
     double2 Z[4], B[4];
     load_signal<MP, OP>(a, p, valid, l, 2 * s, 2 * s + 1, Z, B);
-    prestep<MP>(Z, B, l, y[3 * s + 0], y[3 * s + 1], y[3 * s + 2], a.wN);
+    prestep<MP>(Z, B, l, y[3 * s + 0], y[3 * s + 1], y[3 * s + 2], wN);
   }
-  inverse_fft<MP, 3 * NS, GX>(y, pt, l, lds, a.wMP);
+  inverse_fft<MP, 3 * NS, GX>(y, pt, l, lds, wMP);
 
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
@@ -169,11 +189,11 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
         for (int t = 0; t < NOC; ++t)
           q[3 * t + j1][r] = make_double2(o[2 * t], (2 * t + 1 < NO) ? o[2 * t + 1] : 0.0);
       }
-    forward_fft<MP, 3 * NOC, GX>(q, pt, l, lds, a.wMP);
+    forward_fft<MP, 3 * NOC, GX>(q, pt, l, lds, wMP);
 
     double2 lo[NOC][4], hi[NOC][4];
 #pragma unroll
-    for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], a.wN);
+    for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
     double oa[NOC][2][4], ob[NOC][2][4];
     pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
     if (!valid) continue;

@@ -1,0 +1,46 @@
+// Launchers of the fused Gauss-point kernels (body in sfx_gauss_body.h).
+#include "sfx_gauss_body.h"
+
+namespace sfx {
+
+template <int MP, int V>
+__global__ void __launch_bounds__(64 * GAUSS_WAVES) gauss_kernel(GaussArgs a) {
+  extern __shared__ double2 smem[];
+  gauss_body<MP, V>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+}
+
+template <int MP, int V>
+static hipError_t launch_mp(const GaussArgs& a, hipStream_t st) {
+  constexpr int E = GaussLds<MP>::E;
+  const int64_t nblk = (a.me + E - 1) / E;
+  if (nblk == 0) return hipSuccess;
+  const size_t shmem = (size_t)GaussLds<MP>::TOTAL * sizeof(double2);
+  hipLaunchKernelGGL((gauss_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * GAUSS_WAVES), shmem, st, a);
+  return hipGetLastError();
+}
+
+template <int V>
+static hipError_t launch_v(const GaussArgs& a, hipStream_t st) {
+  switch (a.MP) {
+    case 4: return launch_mp<4, V>(a, st);
+    case 8: return launch_mp<8, V>(a, st);
+    case 16: return launch_mp<16, V>(a, st);
+    case 32: return launch_mp<32, V>(a, st);
+    case 64: return launch_mp<64, V>(a, st);
+    case 128: return launch_mp<128, V>(a, st);
+    case 256: return launch_mp<256, V>(a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st) {
+  switch (variant) {
+    case GV_CROSS_CURL: return launch_v<GV_CROSS_CURL>(a, st);
+    case GV_CURLH_B: return launch_v<GV_CURLH_B>(a, st);
+    case GV_UGRADC: return launch_v<GV_UGRADC>(a, st);
+    case GV_MHD: return launch_v<GV_MHD>(a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace sfx

@@ -1,0 +1,443 @@
+// Fused Gauss-point nonlinear terms: P2 interpolation (+curl, gradient,
+// divergence) of nodal Fourier fields, de-aliased azimuthal product, Gauss-
+// point output.  Body compiled for gfx950 and for the CPU emulator.
+//
+// Variants (reference routines replaced, SOURCES/SIMULATION/...):
+//   GV_CROSS_CURL  smb_cross_prod_gauss_sft_par (subroutine_ns_with_velocity.F90:514-630):
+//                  V_out = (curl u + 2 Omega) x u
+//   GV_CURLH_B     smb_CurlH_cross_B_gauss_sft_par (:632-727): V_out = (curl H) x B
+//   GV_UGRADC      smb_ugradc_gauss_fft_par (subroutine_temperature_with_T.F90:247-350):
+//                  c_out = (hc grad c).u + (div u)(hc c)
+//   GV_MHD         rotv_v = (curl u) x u - c_L (curl H) x B, the combination formed at
+//                  subroutine_ns_with_velocity.F90:227, 246-248, in one launch.
+//
+// Work decomposition: a work-group of 7 waves owns E = 256/MP consecutive
+// elements, i.e. 7E consecutive Gauss points (index (m-1)*7 + l), PW = 256/MP
+// points per wave with L = MP/4 lanes each (see sfx_engine.h).  The nodal
+// data of the E elements is staged in LDS one chunk of L modes at a time.
+// Lane l interpolates mode cL + l of its Gauss point; the partner part of the
+// packed spectrum (mode MP - m) is taken from the partner lane through LDS.
+// Geometry is recomputed per element from the node coordinates (straight-
+// sided P2: constant Jacobian), radius from the six nodes as the reference.
+#pragma once
+#include "sfx_engine.h"
+#include "sfx_fftprod_body.h"
+#include "sfx_kernels.h"
+
+namespace sfx {
+
+template <int V> struct GVT;
+// NPASS: input passes (each: nodal staging + inverse transforms); NSIG:
+// complex signals per pass; NOUT: real outputs; NC[pass][k]: columns of the
+// k-th nodal array of a pass (0 = unused).
+template <> struct GVT<GV_CROSS_CURL> { static constexpr int NPASS = 1, NSIG = 3, NOUT = 3; };
+template <> struct GVT<GV_CURLH_B>    { static constexpr int NPASS = 1, NSIG = 3, NOUT = 3; };
+template <> struct GVT<GV_UGRADC>     { static constexpr int NPASS = 1, NSIG = 4, NOUT = 1; };
+template <> struct GVT<GV_MHD>        { static constexpr int NPASS = 2, NSIG = 3, NOUT = 3; };
+
+constexpr int GAUSS_WAVES = 7;
+
+// LDS layout (double2 units) of the fused kernel.
+template <int MP>
+struct GaussLds {
+  static constexpr int E = 256 / MP;                    // elements per work-group
+  static constexpr int L = MP / 4;
+  static constexpr int TW = 0;                          // twiddles, 4 MP
+  static constexpr int TAB = 4 * MP;                    // ww(6,7), dd(2,6,7): 126 doubles
+  static constexpr int MODE = TAB + 64;                 // list_mode as doubles, MP
+  static constexpr int GEO = MODE + MP;                 // per element 6 nodes (r,z)
+  static constexpr int NID = GEO + E * 6;               // per element 6 node indices (int)
+  static constexpr int BIG = NID + (E * 6 + 3) / 4;
+  static constexpr int ROW6 = L * 6 + 2;                // doubles per node slot, 6 columns
+  static constexpr int ROW2 = L * 2 + 2;                // doubles per node slot, 2 columns
+  static constexpr int NOD6 = E * 6 * ROW6;             // doubles of one staged 6-column field
+  static constexpr int NOD2 = E * 6 * ROW2;
+  // per-wave region in doubles: 2 exchanged signals (2 x 256 complex) or the
+  // 64 lanes x 16 doubles of the partner swap, whichever is larger
+  static constexpr int RW = (4 * Shape<MP>::WAVE_CPLX > 64 * 16) ? 4 * Shape<MP>::WAVE_CPLX : 64 * 16;
+  static constexpr int NODAL_CPLX = (2 * NOD6 + 1) / 2;
+  static constexpr int XCH_CPLX = GAUSS_WAVES * RW / 2;
+  static constexpr int TOTAL = BIG + (NODAL_CPLX > XCH_CPLX ? NODAL_CPLX : XCH_CPLX);
+};
+
+struct GaussGeom {
+  const double* w;       // shape values at this Gauss point (LDS table)
+  const double* d;       // reference derivatives (dx, dy) pairs (LDS table)
+  double a11, a12, a21, a22;   // d/dr = a11 d/dx + a12 d/dy, d/dz = a21 d/dx + a22 d/dy
+  double iray;
+};
+
+// Interpolation at one Gauss point and one mode from a staged node row set:
+// values and reference derivatives, then the affine map to (r, z) derivatives
+// (dw = J^{-T} dd, gauss_points_2d.F90:108-121).
+// nod: staged block; s0: first node slot of the element; row: doubles per slot.
+template <int NC, int MP>
+SFX_DEV void interp_mode(const double* nod, int s0, int row, int mi, const GaussGeom& g,
+                         double (&W)[NC], double (&Dr)[NC], double (&Dz)[NC]) {
+  double Dx[NC], Dy[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) { W[k] = 0; Dx[k] = 0; Dy[k] = 0; }
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const double* v = nod + (s0 + n) * row + mi * NC;
+    const double wn = g.w[n], dx = g.d[2 * n], dy = g.d[2 * n + 1];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const double x = v[k];
+      W[k] = fma(wn, x, W[k]);
+      Dx[k] = fma(dx, x, Dx[k]);
+      Dy[k] = fma(dy, x, Dy[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    Dr[k] = fma(g.a11, Dx[k], g.a12 * Dy[k]);
+    Dz[k] = fma(g.a21, Dx[k], g.a22 * Dy[k]);
+  }
+}
+
+// Curl in (cos, sin) form (subroutine_ns_with_velocity.F90:594-608).
+SFX_DEV void curl6(const double (&W)[6], const double (&Dr)[6], const double (&Dz)[6], double mr,
+                   double ir, double (&R)[6]) {
+  R[0] = mr * W[5] - Dz[2];
+  R[3] = Dz[1] - Dr[5];
+  R[4] = ir * W[2] + Dr[2] - mr * W[1];
+  R[1] = -mr * W[4] - Dz[3];
+  R[2] = Dz[0] - Dr[4];
+  R[5] = ir * W[3] + Dr[3] + mr * W[0];
+}
+
+// Geometry of Gauss point l of the element with node coordinates geo[0..5]:
+// the affine Jacobian from the three vertices and the radius
+// SUM(rr(1,jj(:,m))*ww(:,l)) from all six nodes (subroutine_ns_with_velocity.F90:574).
+SFX_DEV void gauss_geom(const double2* geo, const double* tab, int l, GaussGeom& g) {
+  const double2 v1 = geo[0], v2 = geo[1], v3 = geo[2];
+  const double j11 = v2.x - v1.x, j12 = v3.x - v1.x;   // dr/dx, dr/dy
+  const double j21 = v2.y - v1.y, j22 = v3.y - v1.y;   // dz/dx, dz/dy
+  const double idet = 1.0 / (j11 * j22 - j12 * j21);
+  g.a11 = j22 * idet;  g.a12 = -j21 * idet;
+  g.a21 = -j12 * idet; g.a22 = j11 * idet;
+  g.w = tab + 6 * l;
+  g.d = tab + 42 + 12 * l;
+  double ray = 0;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) ray = fma(geo[n].x, g.w[n], ray);
+  g.iray = 1.0 / ray;
+}
+
+// Stage the nodal values of modes [c*L, c*L+L) of a (np, NC, M) array for
+// the E elements of this work-group: nod[(slot)*row + mi*NC + k].
+template <int NC, int MP>
+struct NodalFetch {
+  static constexpr int TOT = GaussLds<MP>::E * 6 * NC * (MP / 4);
+  static constexpr int NT = 64 * GAUSS_WAVES;
+  static constexpr int IT = (TOT + NT - 1) / NT;
+  double v[IT];
+};
+
+// Issue the loads of chunk c (modes [c*L, c*L+L)) of a (np, NC, M) nodal array
+// for the E elements of this work-group into registers; unconditional loads at
+// clamped indices, so they all issue back to back and overlap other work.
+template <int NC, int MP>
+SFX_DEV void nodal_fetch(const GaussArgs& a, const double* X, int64_t nX, int64_t e0, int c, int tid,
+                         const int* nid, NodalFetch<NC, MP>& f) {
+  using NF_ = NodalFetch<NC, MP>;
+  constexpr int L = MP / 4, S = GaussLds<MP>::E * 6;
+#pragma unroll
+  for (int it = 0; it < NF_::IT; ++it) {
+    const int idx = tid + it * NF_::NT;
+    const int id = idx < NF_::TOT ? idx : 0;
+    const int s = id % S;
+    const int rest = id / S;
+    const int k = rest % NC;
+    const int mi = rest / NC;
+    const int64_t e = e0 + s / 6;
+    const int m = c * L + mi;
+    const bool ok = idx < NF_::TOT && e < a.me && m < a.M;
+    const int64_t gi = (int64_t)nid[s] + a.np * (k + (int64_t)NC * (m < a.M ? m : 0));
+    SFX_CHECK("nodal load", gi, nX);
+    const double x = X[gi];
+    f.v[it] = ok ? x : 0.0;
+  }
+}
+
+// Store a fetched chunk to LDS: nod[slot*row + mi*NC + k].
+template <int NC, int MP>
+SFX_DEV void nodal_store(const NodalFetch<NC, MP>& f, int tid, double* nod, int row) {
+  using NF_ = NodalFetch<NC, MP>;
+  constexpr int S = GaussLds<MP>::E * 6;
+#pragma unroll
+  for (int it = 0; it < NF_::IT; ++it) {
+    const int idx = tid + it * NF_::NT;
+    if (idx < NF_::TOT) {
+      const int s = idx % S;
+      const int rest = idx / S;
+      const int li = s * row + (rest / NC) * NC + rest % NC;
+      SFX_CHECK("nodal stage", li, S * row);
+      nod[li] = f.v[it];
+    }
+  }
+}
+
+// Exchange NF doubles per lane with the partner lane (mode MP - m <-> m).
+template <int NF, int MP>
+SFX_DEV void partner_swap(const double (&own)[NF], double (&par)[NF], int pt, int l, double* xf) {
+  constexpr int L = MP / 4;
+  const int me_ = (pt * L + l) * NF;
+  const int pa = (pt * L + ((L - l) % L)) * NF;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { SFX_CHECK("field swap", me_ + f, 64 * NF); xf[me_ + f] = own[f]; }
+  SFX_WAVE_SYNC();
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { SFX_CHECK("field swap", pa + f, 64 * NF); par[f] = xf[pa + f]; }
+  SFX_WAVE_SYNC();
+}
+
+// Accumulate the packed spectra of complex signal (f1, f2) from the own and
+// partner (cos, sin) values of chunk c.  own/par: [a1, b1, a2, b2].
+template <int MP>
+SFX_DEV void pack_chunk(int c, int l, const double (&o)[4], const double (&p)[4], double2 (&Z)[4],
+                        double2 (&B)[4]) {
+  const bool m0 = (c == 0 && l == 0);
+  Z[c] = m0 ? make_double2(o[0], o[2]) : make_double2(0.5 * (o[0] + o[3]), 0.5 * (o[2] - o[1]));
+  const double2 bv = make_double2(0.5 * (p[0] - p[3]), 0.5 * (p[1] + p[2]));
+  // lanes l >= 1: partner of slot 3-c; lane 0: partner of slot 4-c (c >= 1)
+  B[3 - c] = (l != 0) ? bv : B[3 - c];
+  if (c >= 1) B[4 - c] = (l == 0) ? bv : B[4 - c];
+}
+
+// Real fields (cos, sin pairs) of one pass at own mode; returns NF = 4*NSIG doubles
+// ordered signal by signal as [a1, b1, a2, b2].
+template <int V, int PASS, int MP>
+SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* nodB, int s0, int l,
+                         int mi, double mval, const GaussGeom& g, double hc, double (&F)[4 * GVT<V>::NSIG]) {
+  using LY = GaussLds<MP>;
+  const double mr = mval * g.iray;
+  if constexpr (V == GV_CROSS_CURL || (V == GV_MHD && PASS == 1)) {
+    double W[6], Dr[6], Dz[6], R[6];
+    interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
+    curl6(W, Dr, Dz, mr, g.iray, R);
+    if constexpr (V == GV_CROSS_CURL) {
+      // precession: RotV += 2 Omega (mode 1: r_c and theta_s, mode 0: z_c)
+      if (mval == 1.0) { R[0] += a.om[0]; R[3] += a.om[1]; }
+      if (mval == 0.0) { R[4] += a.om[2]; }
+    }
+    // signals (R_r, R_theta), (R_z, W_r), (W_theta, W_z)
+    F[0] = R[0]; F[1] = R[1]; F[2] = R[2]; F[3] = R[3];
+    F[4] = R[4]; F[5] = R[5]; F[6] = W[0]; F[7] = W[1];
+    F[8] = W[2]; F[9] = W[3]; F[10] = W[4]; F[11] = W[5];
+  } else if constexpr (V == GV_CURLH_B || (V == GV_MHD && PASS == 0)) {
+    double W[6], Dr[6], Dz[6], R[6], Wb[6], Drb[6], Dzb[6];
+    interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
+    curl6(W, Dr, Dz, mr, g.iray, R);
+    interp_mode<6, MP>(nodB, s0, LY::ROW6, mi, g, Wb, Drb, Dzb);
+    F[0] = R[0]; F[1] = R[1]; F[2] = R[2]; F[3] = R[3];
+    F[4] = R[4]; F[5] = R[5]; F[6] = Wb[0]; F[7] = Wb[1];
+    F[8] = Wb[2]; F[9] = Wb[3]; F[10] = Wb[4]; F[11] = Wb[5];
+  } else {  // GV_UGRADC
+    double W[6], Dr[6], Dz[6], C[2], Cr[2], Cz[2];
+    interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
+    interp_mode<2, MP>(nodB, s0, LY::ROW2, mi, g, C, Cr, Cz);
+    const double d0 = Dr[0] + W[0] * g.iray + mr * W[3] + Dz[4];
+    const double d1 = Dr[1] + W[1] * g.iray - mr * W[2] + Dz[5];
+    // signals (G_r, G_theta), (G_z, W_r), (W_theta, W_z), (Div, C)
+    F[0] = hc * Cr[0]; F[1] = hc * Cr[1]; F[2] = hc * (mr * C[1]); F[3] = hc * (-mr * C[0]);
+    F[4] = hc * Cz[0]; F[5] = hc * Cz[1]; F[6] = W[0]; F[7] = W[1];
+    F[8] = W[2]; F[9] = W[3]; F[10] = W[4]; F[11] = W[5];
+    F[12] = d0; F[13] = d1; F[14] = hc * C[0]; F[15] = hc * C[1];
+  }
+}
+
+template <int MP, int V>
+SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem) {
+  using S = Shape<MP>;
+  using LY = GaussLds<MP>;
+  constexpr int L = S::L, PW = S::PW, E = LY::E;
+  constexpr int NSIG = GVT<V>::NSIG, NPASS = GVT<V>::NPASS, NO = GVT<V>::NOUT;
+  constexpr int NF = 4 * NSIG;
+  constexpr int NOC = (NO + 1) / 2;
+  constexpr int GX = 2;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int pt = lane / L, l = lane % L;
+  double* smd = reinterpret_cast<double*>(smem);
+
+  // ---- per work-group tables
+  double2* tw = smem + LY::TW;
+  double* tab = smd + 2 * LY::TAB;
+  double* lmode = smd + 2 * LY::MODE;
+  double2* geo = smem + LY::GEO;
+  int* nid = reinterpret_cast<int*>(smem + LY::NID);
+  const int64_t e0 = bid * E;
+  {
+    // all prologue loads issue together (clamped indices), then the LDS stores
+    constexpr int NT = 64 * GAUSS_WAVES;
+    constexpr int ITW = (4 * MP + NT - 1) / NT;
+    double2 twv[ITW];
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int i = tid + it * NT;
+      const int ic = i < 4 * MP ? i : 0;
+      twv[it] = (ic < 3 * MP) ? a.wN[ic] : a.wMP[ic - 3 * MP];
+    }
+    const double tv = a.tab[tid < 126 ? tid : 0];
+    const int lmv = a.list_mode[tid < a.M ? tid : 0];
+    const int ig = tid < E * 6 ? tid : 0;
+    const int64_t eg = e0 + ig / 6;
+    const int64_t egc = eg < a.me ? eg : 0;
+    const int node = a.jj[6 * egc + ig % 6] - 1;
+    SFX_CHECK("rr load", 2 * (int64_t)node + 1, 2 * a.np);
+    const double2 rz = make_double2(a.rr[2 * (int64_t)node], a.rr[2 * (int64_t)node + 1]);
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int i = tid + it * NT;
+      if (i < 4 * MP) tw[i] = twv[it];
+    }
+    if (tid < 126) tab[tid] = tv;
+    if (tid < MP) lmode[tid] = (tid < a.M) ? (double)lmv : 0.0;
+    if (tid < E * 6) {
+      geo[tid] = (eg < a.me) ? rz : make_double2(1.0, 0.0);
+      nid[tid] = (eg < a.me) ? node : 0;
+    }
+  }
+  static_assert(E * 6 <= 64 * GAUSS_WAVES && MP <= 64 * GAUSS_WAVES, "prologue assumes one item per thread");
+  SFX_SYNC();
+  const double2* wN = tw;
+  const double2* wMP = tw + 3 * MP;
+
+  // ---- this lane's Gauss point
+  const int lp = wave * PW + pt;                  // local point 0 .. 7E-1
+  const int el = lp / 7, lg = lp % 7;
+  const int64_t p = e0 * 7 + lp;
+  const bool valid = (e0 + el) < a.me;
+  GaussGeom g;
+  gauss_geom(geo + 6 * el, tab, valid ? lg : 0, g);
+  const double hc = (V == GV_UGRADC && valid) ? a.heat_capa[e0 + el] : 1.0;
+
+  double* big = smd + 2 * LY::BIG;
+  double* xf = big + (size_t)wave * LY::RW;
+  double2* lds = reinterpret_cast<double2*>(xf);
+
+  double acc[3][4][3];   // MHD: physical product accumulated over passes
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[j][r][0] = acc[j][r][1] = acc[j][r][2] = 0.0;
+
+  double2 q[3 * NOC][4];
+#pragma unroll
+  for (int pass = 0; pass < NPASS; ++pass) {
+    double2 Z[NSIG][4], B[NSIG][4];
+#pragma unroll
+    for (int s = 0; s < NSIG; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { Z[s][r] = make_double2(0, 0); B[s][r] = make_double2(0, 0); }
+    // nodal arrays of this pass: A (6 columns) and B (6 or 2 columns, or none)
+    constexpr bool TWO = (V == GV_CURLH_B || V == GV_UGRADC || V == GV_MHD);
+    constexpr int NCB = (V == GV_UGRADC) ? 2 : 6;
+    // MHD: pass 0 = (curl H) x B, pass 1 = (curl u) x u
+    const double* XA = (V == GV_MHD && pass == 0) ? a.U2 : a.U;
+    const int64_t nXA = (V == GV_MHD && pass == 0) ? a.nU2 : a.nU;
+    const double* XB = (V == GV_MHD) ? a.U3 : a.U2;
+    const int64_t nXB = (V == GV_MHD) ? a.nU3 : a.nU2;
+    const bool useB = TWO && !(V == GV_MHD && pass == 1);
+    NodalFetch<6, MP> fa;
+    NodalFetch<NCB, MP> fb;
+    nodal_fetch<6, MP>(a, XA, nXA, e0, 0, tid, nid, fa);
+    if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, 0, tid, nid, fb);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      // stage nodal chunk c (the field-exchange regions of the waves alias it:
+      // the barrier orders the previous chunk's swaps before these writes),
+      // then start fetching chunk c+1 while chunk c is interpolated
+      SFX_SYNC();
+      double* nodA = big;
+      double* nodB = big + LY::NOD6;
+      nodal_store<6, MP>(fa, tid, nodA, LY::ROW6);
+      if (useB) nodal_store<NCB, MP>(fb, tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
+      SFX_SYNC();
+      if (c < 3) {
+        nodal_fetch<6, MP>(a, XA, nXA, e0, c + 1, tid, nid, fa);
+        if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, c + 1, tid, nid, fb);
+      }
+      const int m = c * L + l;
+      double own[NF];
+      if (m < a.M && valid) {
+        if (pass == 0) pass_fields<V, 0, MP>(a, nodA, nodB, 6 * el, l, l, lmode[m], g, hc, own);
+        else pass_fields<V, 1, MP>(a, nodA, nodB, 6 * el, l, l, lmode[m], g, hc, own);
+      } else {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) own[f] = 0.0;
+      }
+      SFX_SYNC();   // all interpolation reads of the staged chunk are done
+      double par[NF];
+      partner_swap<NF, MP>(own, par, pt, l, xf);
+#pragma unroll
+      for (int s = 0; s < NSIG; ++s) {
+        const double o4[4] = {own[4 * s], own[4 * s + 1], own[4 * s + 2], own[4 * s + 3]};
+        const double p4[4] = {par[4 * s], par[4 * s + 1], par[4 * s + 2], par[4 * s + 3]};
+        pack_chunk<MP>(c, l, o4, p4, Z[s], B[s]);
+      }
+    }
+    // ---- inverse transforms of this pass
+    double2 y[3 * NSIG][4];
+#pragma unroll
+    for (int s = 0; s < NSIG; ++s) prestep<MP>(Z[s], B[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
+    SFX_SYNC();   // nodal staging region is reused as exchange region
+    inverse_fft<MP, 3 * NSIG, GX>(y, pt, l, lds, wMP);
+    // ---- pointwise op
+#pragma unroll
+    for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double u[8];
+#pragma unroll
+        for (int s = 0; s < NSIG; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
+        if constexpr (V == GV_UGRADC) {
+          const double o0 = u[0] * u[3] + u[1] * u[4] + u[2] * u[5] + u[6] * u[7];
+          q[j1][r] = make_double2(o0, 0.0);
+        } else {
+          const double cx = u[1] * u[5] - u[2] * u[4];
+          const double cy = u[2] * u[3] - u[0] * u[5];
+          const double cz = u[0] * u[4] - u[1] * u[3];
+          const double f = (V == GV_MHD && pass == 0) ? -a.c_lorentz : 1.0;
+          acc[j1][r][0] = fma(f, cx, acc[j1][r][0]);
+          acc[j1][r][1] = fma(f, cy, acc[j1][r][1]);
+          acc[j1][r][2] = fma(f, cz, acc[j1][r][2]);
+        }
+      }
+  }
+  if constexpr (V != GV_UGRADC) {
+#pragma unroll
+    for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        q[j1][r] = make_double2(acc[j1][r][0], acc[j1][r][1]);
+        q[3 + j1][r] = make_double2(acc[j1][r][2], 0.0);
+      }
+  }
+  forward_fft<MP, 3 * NOC, GX>(q, pt, l, lds, wMP);
+  double2 lo[NOC][4], hi[NOC][4];
+#pragma unroll
+  for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
+  double oa[NOC][2][4], ob[NOC][2][4];
+  pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
+  if (!valid) return;
+#pragma unroll
+  for (int t = 0; t < NOC; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int o = 2 * t + h;
+      if (o >= NO) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = l + L * r;
+        if (m >= a.M) continue;
+        const int64_t base = p + a.G * (int64_t)(2 * o + 2 * NO * m);
+        SFX_CHECK("gauss out", base + a.G, a.nout);
+        a.Vout[base] = oa[t][h][r];
+        a.Vout[base + a.G] = ob[t][h][r];
+      }
+    }
+}
+
+}  // namespace sfx

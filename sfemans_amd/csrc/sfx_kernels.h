@@ -5,6 +5,7 @@
 namespace sfx {
 
 enum { OP_CROSS = 0, OP_DOT = 1, OP_PROD = 2, OP_SVECT1 = 3, OP_TENSOR = 4 };
+enum { GV_CROSS_CURL = 0, GV_CURLH_B = 1, GV_UGRADC = 2, GV_MHD = 3 };
 
 // Twiddles for one power-of-two MP: wN[j] = e^{2 pi i j/(3 MP)}, wMP[j] = e^{2 pi i j/MP}.
 struct Twiddles {
@@ -23,7 +24,32 @@ struct FftProdArgs {
   int64_t n1, n2, nout;   // element counts of V1, V2, Vout (bounds checks on the CPU build)
 };
 
+// Fused Gauss-point nonlinear terms (see sfx_gauss_body.h).
+struct GaussArgs {
+  const int32_t* jj;        // (6, me), 1-based node numbers
+  const double* rr;         // (2, np) node coordinates (r, z)
+  int64_t me, np, G;        // elements, nodes, Gauss points (7 me)
+  const double* U;          // first nodal field  (np, 6, M): u or H
+  const double* U2;         // second nodal field (np, 6 or 2, M): B, c or H
+  const double* U3;         // third nodal field  (np, 6, M): B (MHD)
+  int64_t nU, nU2, nU3;     // element counts (bounds checks on the CPU build)
+  const double* heat_capa;  // (me) for u.grad c
+  const int32_t* list_mode; // (M) mode numbers
+  const double* tab;        // ww(6,7) then dd(2,6,7) of the reference element
+  double* Vout;             // (G, 2*NO, M)
+  int64_t nout;
+  int M, MP;
+  double c_lorentz;
+  double om[3];             // precession: 2*Omega added to RotV (mode 1: r_c, theta_s; mode 0: z_c)
+  const double2* wN;
+  const double2* wMP;
+};
+
+struct RhsArgs;
 #ifndef SFX_EMULATE
+hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st);
+hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
+hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);
 hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
 #endif
 

@@ -1,0 +1,363 @@
+// Mesh registration and the Gauss-point entry points of the C-ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/sfx.h"
+#include "sfx_api_internal.h"
+#include "sfx_kernels.h"
+#include "sfx_p2tab.h"
+#include "sfx_rhs_body.h"
+
+// Opaque mesh handle: device copies of the connectivity and coordinates plus
+// the reference-element tables.
+struct sfx_mesh_s {
+  int device = 0;
+  int64_t me = 0, np = 0;
+  int32_t* jj = nullptr;    // (6, me)
+  double* rr = nullptr;     // (2, np)
+  double* tab = nullptr;    // ww(6,7), dd(2,6,7)
+  // deterministic node-centred gather for the right-hand-side assembly
+  int64_t* adj_ptr = nullptr;   // (np+1) CSR over nodes
+  int32_t* adj_ent = nullptr;   // entries e*6 + local node, element order
+  int64_t n_adj = 0;
+  double* wgt = nullptr;        // (7, me) rj * ray
+};
+
+namespace sfx {
+
+}  // namespace sfx
+
+using namespace sfx;
+
+static int mesh_upload(const int32_t* jj, const double* rr, int64_t me, int64_t np, sfx_mesh_t* out) {
+  sfx_mesh_s* m = new sfx_mesh_s();
+  (void)hipGetDevice(&m->device);
+  m->me = me;
+  m->np = np;
+  double tab[126];
+  p2_reference_tables(tab);
+  // node -> (element, local node) adjacency in element order (deterministic sums)
+  std::vector<int64_t> ptr(np + 1, 0);
+  for (int64_t e = 0; e < me; ++e)
+    for (int n = 0; n < 6; ++n) ptr[jj[6 * e + n]] += 1;   // jj is 1-based: counts land at node+1
+  for (int64_t i = 0; i < np; ++i) ptr[i + 1] += ptr[i];
+  std::vector<int32_t> ent(ptr[np]);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < me; ++e)
+    for (int n = 0; n < 6; ++n) ent[fill[jj[6 * e + n] - 1]++] = (int32_t)(6 * e + n);
+  std::vector<double> wgt(7 * me);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  m->n_adj = (int64_t)ent.size();
+  hipError_t e;
+  if ((e = hipMalloc(&m->wgt, 7 * me * sizeof(double) + 8)) != hipSuccess ||
+      (e = hipMalloc(&m->jj, 6 * me * sizeof(int32_t) + 4)) != hipSuccess ||
+      (e = hipMalloc(&m->rr, 2 * np * sizeof(double) + 8)) != hipSuccess ||
+      (e = hipMalloc(&m->tab, 126 * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&m->adj_ptr, (np + 1) * sizeof(int64_t))) != hipSuccess ||
+      (e = hipMalloc(&m->adj_ent, ent.size() * sizeof(int32_t) + 4)) != hipSuccess) {
+    sfx_mesh_destroy(m);
+    return fail(SFX_ERR_NOMEM, "sfx_mesh_create: hipMalloc: %s", hipGetErrorString(e));
+  }
+  if ((e = hipMemcpy(m->jj, jj, 6 * me * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->rr, rr, 2 * np * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(m->tab, tab, sizeof(tab), hipMemcpyHostToDevice)) != hipSuccess ||
+      (me && (e = hipMemcpy(m->wgt, wgt.data(), 7 * me * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) ||
+      (e = hipMemcpy(m->adj_ptr, ptr.data(), (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+      (ent.size() && (e = hipMemcpy(m->adj_ent, ent.data(), ent.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)) {
+    sfx_mesh_destroy(m);
+    return hip_fail(e, "sfx_mesh_create: upload");
+  }
+  *out = m;
+  return SFX_OK;
+}
+
+extern "C" {
+
+int sfx_mesh_create(const int32_t* jj, const double* rr, int32_t n_w, int32_t l_G, int64_t me, int64_t np,
+                    sfx_mesh_t* out) {
+  clear_err();
+  if (!out) return fail(SFX_ERR_ARG, "sfx_mesh_create: null handle pointer");
+  *out = nullptr;
+  if (n_w != 6 || l_G != 7)
+    return fail(SFX_ERR_UNSUPPORTED, "sfx_mesh_create: only P2 triangles (n_w=6, l_G=7) are supported");
+  if (me < 0 || np < 0 || (me > 0 && (!jj || !rr))) return fail(SFX_ERR_ARG, "sfx_mesh_create: bad sizes");
+  for (int64_t e = 0; e < me; ++e) {
+    for (int n = 0; n < 6; ++n) {
+      const int32_t v = jj[6 * e + n];
+      if (v < 1 || v > np) return fail(SFX_ERR_ARG, "sfx_mesh_create: jj(%d,%lld)=%d out of 1..np", n + 1, (long long)e + 1, v);
+    }
+    // straight-sided P2: node 3+k is the midpoint of the edge opposite vertex k
+    for (int k = 0; k < 3; ++k) {
+      const int a = (k + 1) % 3, b = (k + 2) % 3;
+      const int64_t na = jj[6 * e + a] - 1, nb = jj[6 * e + b] - 1, nm = jj[6 * e + 3 + k] - 1;
+      for (int c = 0; c < 2; ++c) {
+        const double mid = 0.5 * (rr[2 * na + c] + rr[2 * nb + c]);
+        const double len = std::fabs(rr[2 * na + c] - rr[2 * nb + c]) + std::fabs(rr[2 * na + 1 - c] - rr[2 * nb + 1 - c]);
+        if (std::fabs(rr[2 * nm + c] - mid) > 1e-10 * (len + 1e-300) + 1e-14)
+          return fail(SFX_ERR_UNSUPPORTED, "sfx_mesh_create: element %lld is not straight-sided P2", (long long)e + 1);
+      }
+    }
+  }
+  return mesh_upload(jj, rr, me, np, out);
+}
+
+int sfx_mesh_destroy(sfx_mesh_t m) {
+  if (!m) return SFX_OK;
+  (void)hipFree(m->jj);
+  (void)hipFree(m->rr);
+  (void)hipFree(m->tab);
+  (void)hipFree(m->adj_ptr);
+  (void)hipFree(m->adj_ent);
+  (void)hipFree(m->wgt);
+  delete m;
+  return SFX_OK;
+}
+
+int sfx_mesh_info(sfx_mesh_t m, int64_t* me, int64_t* np) {
+  clear_err();
+  if (!m) return fail(SFX_ERR_ARG, "sfx_mesh_info: null mesh");
+  if (me) *me = m->me;
+  if (np) *np = m->np;
+  return SFX_OK;
+}
+
+}  // extern "C"
+
+// ---- Gauss-point entry points
+namespace sfx {
+
+int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M,
+                  const double* U, int ncU, const double* U2, int ncU2, const double* U3,
+                  const double* heat_capa_dev, double* out, int ncout, double c_lorentz, const double om[3],
+                  hipStream_t st) {
+  if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
+  if (M <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
+  if (M > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, M);
+  GaussArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.jj = mesh->jj; a.rr = mesh->rr; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me;
+  a.U = U; a.U2 = U2; a.U3 = U3;
+  a.nU = mesh->np * ncU * (int64_t)M; a.nU2 = mesh->np * ncU2 * (int64_t)M; a.nU3 = mesh->np * 6 * (int64_t)M;
+  a.heat_capa = heat_capa_dev; a.list_mode = list_mode_dev; a.tab = mesh->tab;
+  a.Vout = out; a.nout = a.G * ncout * (int64_t)M;
+  a.M = M; a.MP = mp_for(M);
+  a.c_lorentz = c_lorentz;
+  for (int i = 0; i < 3; ++i) a.om[i] = om ? om[i] : 0.0;
+  Twiddles tw;
+  int rc = get_twiddles(a.MP, &tw);
+  if (rc) return rc;
+  a.wN = tw.wN; a.wMP = tw.wMP;
+  if (mesh->me == 0) return SFX_OK;
+  hipError_t e = launch_gauss(variant, a, st);
+  if (e != hipSuccess) return hip_fail(e, nm);
+  return SFX_OK;
+}
+
+// Host path: stage inputs in slots 0..4, run, copy the Gauss output back.
+static int run_gauss_host(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* list_mode, int32_t M,
+                          const double* U, int ncU, const double* U2, int ncU2, const double* U3,
+                          const double* heat_capa, double* out, int ncout, double c_lorentz, const double om[3]) {
+  if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
+  if (M <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
+  const size_t nU = (size_t)mesh->np * ncU * M, nU2 = U2 ? (size_t)mesh->np * ncU2 * M : 0;
+  const size_t nU3 = U3 ? (size_t)mesh->np * 6 * M : 0, nO = (size_t)7 * mesh->me * ncout * M;
+  void *dU, *dU2 = nullptr, *dU3 = nullptr, *dO, *dL, *dH = nullptr;
+  int rc;
+  if ((rc = staging(0, nU * 8, &dU)) || (rc = staging(2, nO * 8, &dO)) || (rc = staging(3, M * 4 + 64, &dL)))
+    return rc;
+  if (U2 && (rc = staging(1, nU2 * 8, &dU2))) return rc;
+  if (U3 && (rc = staging(4, nU3 * 8, &dU3))) return rc;
+  if (heat_capa && (rc = staging(5, mesh->me * 8 + 8, &dH))) return rc;
+  hipError_t e;
+  if ((e = hipMemcpy(dU, U, nU * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  if (U2 && (e = hipMemcpy(dU2, U2, nU2 * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  if (U3 && (e = hipMemcpy(dU3, U3, nU3 * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  if (heat_capa && (e = hipMemcpy(dH, heat_capa, mesh->me * 8, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "H2D");
+  if ((e = hipMemcpy(dL, list_mode, M * 4, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  rc = run_gauss_dev(variant, nm, mesh, (const int32_t*)dL, M, (const double*)dU, ncU, (const double*)dU2, ncU2,
+                     (const double*)dU3, (const double*)dH, (double*)dO, ncout, c_lorentz, om, nullptr);
+  if (rc) return rc;
+  if ((e = hipMemcpy(out, dO, nO * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  return SFX_OK;
+}
+
+static void precession_om(int32_t precession, double taux, double angle_s_pi, double om[3]) {
+  om[0] = om[1] = om[2] = 0.0;
+  if (!precession) return;
+  const double PI = std::acos(-1.0);
+  // Omega at nodes (subroutine_ns_with_velocity.F90:540-548); RotV += 2 Omega (:621)
+  om[0] = 2.0 * (taux * std::sin(angle_s_pi * PI));
+  om[1] = 2.0 * (-taux * std::sin(angle_s_pi * PI));
+  om[2] = 2.0 * (taux * std::cos(angle_s_pi * PI));
+}
+
+}  // namespace sfx
+
+extern "C" {
+
+int sfx_smb_cross_prod_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
+                                     const double* V_in, double* V_out, int32_t precession_in,
+                                     double taux_precession, double angle_s_pi) {
+  clear_err();
+  double om[3];
+  precession_om(precession_in, taux_precession, angle_s_pi, om);
+  return run_gauss_host(GV_CROSS_CURL, "smb_cross_prod_gauss_sft_par", mesh, list_mode, m_max_c, V_in, 6,
+                        nullptr, 0, nullptr, nullptr, V_out, 6, 0.0, om);
+}
+
+int sfx_smb_cross_prod_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                         const double* V_in, double* V_out, int32_t precession_in,
+                                         double taux_precession, double angle_s_pi, void* stream) {
+  clear_err();
+  double om[3];
+  precession_om(precession_in, taux_precession, angle_s_pi, om);
+  return run_gauss_dev(GV_CROSS_CURL, "smb_cross_prod_gauss_sft_par", mesh, list_mode_dev, m_max_c, V_in, 6,
+                       nullptr, 0, nullptr, nullptr, V_out, 6, 0.0, om, (hipStream_t)stream);
+}
+
+int sfx_smb_curlh_cross_b_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
+                                        const double* V_in, const double* W_in, double* V_out) {
+  clear_err();
+  return run_gauss_host(GV_CURLH_B, "smb_CurlH_cross_B_gauss_sft_par", mesh, list_mode, m_max_c, V_in, 6, W_in,
+                        6, nullptr, nullptr, V_out, 6, 0.0, nullptr);
+}
+
+int sfx_smb_curlh_cross_b_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                            const double* V_in, const double* W_in, double* V_out, void* stream) {
+  clear_err();
+  return run_gauss_dev(GV_CURLH_B, "smb_CurlH_cross_B_gauss_sft_par", mesh, list_mode_dev, m_max_c, V_in, 6,
+                       W_in, 6, nullptr, nullptr, V_out, 6, 0.0, nullptr, (hipStream_t)stream);
+}
+
+int sfx_smb_ugradc_gauss_fft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
+                                 const double* heat_capa_in, const double* V_in, const double* c_in,
+                                 double* c_out) {
+  clear_err();
+  if (!heat_capa_in) return fail(SFX_ERR_ARG, "smb_ugradc_gauss_fft_par: heat_capa_in is required");
+  return run_gauss_host(GV_UGRADC, "smb_ugradc_gauss_fft_par", mesh, list_mode, m_max_c, V_in, 6, c_in, 2,
+                        nullptr, heat_capa_in, c_out, 2, 0.0, nullptr);
+}
+
+int sfx_smb_ugradc_gauss_fft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                     const double* heat_capa_dev, const double* V_in, const double* c_in,
+                                     double* c_out, void* stream) {
+  clear_err();
+  if (!heat_capa_dev) return fail(SFX_ERR_ARG, "smb_ugradc_gauss_fft_par: heat_capa_in is required");
+  return run_gauss_dev(GV_UGRADC, "smb_ugradc_gauss_fft_par", mesh, list_mode_dev, m_max_c, V_in, 6, c_in, 2,
+                       nullptr, heat_capa_dev, c_out, 2, 0.0, nullptr, (hipStream_t)stream);
+}
+
+int sfx_nl_rotv_mhd(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* u,
+                    const double* H, const double* B, double coeff_lorentz, double* rotv_v) {
+  clear_err();
+  return run_gauss_host(GV_MHD, "nl_rotv_mhd", mesh, list_mode, m_max_c, u, 6, H, 6, B, nullptr, rotv_v, 6,
+                        coeff_lorentz, nullptr);
+}
+
+int sfx_nl_rotv_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
+                        const double* H, const double* B, double coeff_lorentz, double* rotv_v, void* stream) {
+  clear_err();
+  return run_gauss_dev(GV_MHD, "nl_rotv_mhd", mesh, list_mode_dev, m_max_c, u, 6, H, 6, B, nullptr, rotv_v, 6,
+                       coeff_lorentz, nullptr, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ---- right-hand-side assembly entry points
+namespace sfx {
+
+static RhsArgs rhs_args(sfx_mesh_t mesh, int32_t M, double scale) {
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.adj_ptr = mesh->adj_ptr; a.adj_ent = mesh->adj_ent; a.wgt = mesh->wgt; a.tab = mesh->tab;
+  a.jj = mesh->jj; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me; a.M = M; a.scale = scale;
+  a.n_adj = mesh->n_adj;
+  return a;
+}
+
+int rhs3x3_dev(sfx_mesh_t mesh, int32_t M, const double* rhs, double* v145, double* v236, double scale,
+               hipStream_t st) {
+  RhsArgs a = rhs_args(mesh, M, scale);
+  a.rhs = rhs; a.v145 = v145; a.v236 = v236;
+  a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = a.np * 3 * (int64_t)M;
+  hipError_t e = launch_rhs3x3(a, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, "rhs_3x3");
+}
+
+}  // namespace sfx
+
+extern "C" {
+
+int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, double* vb_145, double* vb_236,
+                    double scale, void* stream) {
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3: null mesh");
+  if (m_max_c < 0) return fail(SFX_ERR_ARG, "rhs_3x3: m_max_c < 0");
+  return rhs3x3_dev(mesh, m_max_c, rhs_gauss, vb_145, vb_236, scale, (hipStream_t)stream);
+}
+
+int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_145, double* vb_236) {
+  clear_err();
+  (void)mode;   // only used by the opt_tensor terms of the reference
+  if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3: null mesh");
+  const size_t nG = (size_t)7 * mesh->me * 6, nN = (size_t)mesh->np * 3;
+  void *dr, *d1, *d2;
+  int rc;
+  if ((rc = staging(0, nG * 8 + 8, &dr)) || (rc = staging(1, nN * 8 + 8, &d1)) || (rc = staging(2, nN * 8 + 8, &d2)))
+    return rc;
+  hipError_t e;
+  if ((e = hipMemcpy(dr, rhs_in, nG * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  if ((rc = rhs3x3_dev(mesh, 1, (const double*)dr, (double*)d1, (double*)d2, 1.0, nullptr))) return rc;
+  if ((e = hipMemcpy(vb_145, d1, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  if ((e = hipMemcpy(vb_236, d2, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  return SFX_OK;
+}
+
+int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa, const double* ff,
+                        const double* ff_gauss, double* vect, void* stream) {
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "qs_00_gauss: null mesh");
+  RhsArgs a = rhs_args(mesh, ncols, 1.0);
+  a.heat_capa = heat_capa; a.ff = ff; a.ffg = ff_gauss; a.vect = vect;
+  a.n_ff = a.np * (int64_t)ncols; a.n_ffg = a.G * (int64_t)ncols; a.n_out = a.np * (int64_t)ncols;
+  hipError_t e = launch_qs00(a, (hipStream_t)stream);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, "qs_00_gauss");
+}
+
+int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, const double* ff_gauss,
+                    double* vect) {
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "qs_00_gauss: null mesh");
+  const size_t nG = (size_t)7 * mesh->me, nN = (size_t)mesh->np;
+  void *dh, *df, *dg, *dv;
+  int rc;
+  if ((rc = staging(0, mesh->me * 8 + 8, &dh)) || (rc = staging(1, nN * 8 + 8, &df)) ||
+      (rc = staging(2, nG * 8 + 8, &dg)) || (rc = staging(3, nN * 8 + 8, &dv)))
+    return rc;
+  hipError_t e;
+  if ((e = hipMemcpy(dh, heat_capa, mesh->me * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(df, ff, nN * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dg, ff_gauss, nG * 8, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "H2D");
+  if ((rc = sfx_qs_00_gauss_dev(mesh, 1, (const double*)dh, (const double*)df, (const double*)dg, (double*)dv,
+                                nullptr)))
+    return rc;
+  if ((e = hipMemcpy(vect, dv, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  return SFX_OK;
+}
+
+int sfx_nl_rhs_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
+                       const double* H, const double* B, double coeff_lorentz, double* work_gauss,
+                       double* vb_145, double* vb_236, void* stream) {
+  clear_err();
+  int rc = sfx_nl_rotv_mhd_dev(mesh, list_mode_dev, m_max_c, u, H, B, coeff_lorentz, work_gauss, stream);
+  if (rc) return rc;
+  // the nonlinear share of rhs_gauss is -rotv_v (computing_rhs_gauss.F90:100)
+  return rhs3x3_dev(mesh, m_max_c, work_gauss, vb_145, vb_236, -1.0, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -17,11 +17,20 @@ using std::fma;
 void sfx_emu_sync();
 [[noreturn]] void sfx_emu_oob(const char* what, long long idx, long long n, const char* file, int line);
 #define SFX_SYNC() sfx_emu_sync()
+#define SFX_WAVE_SYNC() sfx_emu_sync()
 #define SFX_CHECK(what, idx, n) \
   do { if ((long long)(idx) < 0 || (long long)(idx) >= (long long)(n)) sfx_emu_oob(what, (long long)(idx), (long long)(n), __FILE__, __LINE__); } while (0)
 #else
 #include <hip/hip_runtime.h>
 #define SFX_DEV __device__ __forceinline__
 #define SFX_SYNC() __syncthreads()
+// Exchange inside one wave through its private LDS region: a wave's LDS
+// instructions execute in order, so only compiler reordering must be stopped.
+#define SFX_WAVE_SYNC()                                   \
+  do {                                                    \
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+    __builtin_amdgcn_wave_barrier();                      \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+  } while (0)
 #define SFX_CHECK(what, idx, n) do { } while (0)
 #endif

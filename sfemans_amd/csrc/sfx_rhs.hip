@@ -1,0 +1,29 @@
+// Launchers of the right-hand-side assembly kernels (body in sfx_rhs_body.h).
+#include "sfx_kernels.h"
+#include "sfx_rhs_body.h"
+
+namespace sfx {
+
+__global__ void __launch_bounds__(256) rhs3x3_kernel(RhsArgs a) {
+  rhs3x3_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) qs00_kernel(RhsArgs a) {
+  qs00_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
+  const int64_t n = a.np * a.M;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(rhs3x3_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_qs00(const RhsArgs& a, hipStream_t st) {
+  const int64_t n = a.np * a.M;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(qs00_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace sfx

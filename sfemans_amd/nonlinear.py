@@ -1,0 +1,189 @@
+"""Python mirror of the reference's Gauss-point nonlinear-term routines.
+
+Routine names and argument order follow SOURCES/SIMULATION:
+``smb_cross_prod_gauss_sft_par`` and ``smb_CurlH_cross_B_gauss_sft_par``
+(subroutine_ns_with_velocity.F90:514, 632) and ``smb_ugradc_gauss_fft_par``
+(subroutine_temperature_with_T.F90:247).  ``communicator`` must be ``None``
+(one rank).  ``mesh`` is a :class:`Mesh` registered once from the reference
+arrays ``jj(6,me)`` and ``rr(2,np)``.  Outputs are INTENT(OUT) arrays filled
+in place: Fortran-ordered numpy arrays (host path) or Fortran-strided CUDA
+tensors (device path, launched on torch's current stream).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib, check
+from .fft_parallele import fdims, is_device, ptr, stream_ptr
+
+__all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "smb_CurlH_cross_B_gauss_sft_par",
+           "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes",
+           "qs_00_gauss", "nl_rhs_mhd"]
+
+
+class Mesh:
+    """A P2 meridian mesh registered with libsfx (device copies of jj, rr)."""
+
+    def __init__(self, jj, rr):
+        jj = np.asfortranarray(jj, dtype=np.int32)
+        rr = np.asfortranarray(rr, dtype=np.float64)
+        if jj.ndim != 2 or jj.shape[0] != 6 or rr.ndim != 2 or rr.shape[0] != 2:
+            raise ValueError("expected jj(6, me) and rr(2, np)")
+        self.me, self.np = int(jj.shape[1]), int(rr.shape[1])
+        h = C.c_void_p()
+        check(lib().sfx_mesh_create(jj.ctypes.data, rr.ctypes.data, 6, 7, self.me, self.np, C.byref(h)))
+        self._h = h
+        self._lm_cache = {}
+
+    @classmethod
+    def from_p2(cls, m):
+        return cls(m.jj, m.rr)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_gauss(self):
+        return 7 * self.me
+
+    def list_mode_device(self, list_mode):
+        """Device int32 copy of list_mode (cached)."""
+        import torch
+        key = tuple(int(x) for x in list_mode)
+        t = self._lm_cache.get(key)
+        if t is None:
+            t = torch.tensor(key, dtype=torch.int32, device="cuda")
+            self._lm_cache[key] = t
+        return t
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sfx_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _lm(list_mode):
+    return np.ascontiguousarray(np.asarray(list_mode, dtype=np.int32))
+
+
+def _check_comm(communicator):
+    if communicator is not None and getattr(communicator, "size", 1) != 1:
+        raise ValueError("one GPU: the Fourier communicator must have a single rank")
+
+
+def smb_cross_prod_gauss_sft_par(communicator, mesh: Mesh, list_mode, V_in, V_out, precession_in=False,
+                                 taux_precession=0.0, angle_s_pi=0.0):
+    """V_out(G,6,m) = (curl V_in + 2 Omega) x V_in at the Gauss points."""
+    _check_comm(communicator)
+    M = len(list_mode)
+    fdims(V_in), fdims(V_out)
+    L = lib()
+    if is_device(V_in):
+        lm = mesh.list_mode_device(list_mode)
+        check(L.sfx_smb_cross_prod_gauss_sft_par_dev(mesh.handle, lm.data_ptr(), M, ptr(V_in), ptr(V_out),
+                                                     int(bool(precession_in)), float(taux_precession),
+                                                     float(angle_s_pi), stream_ptr()))
+    else:
+        lm = _lm(list_mode)
+        check(L.sfx_smb_cross_prod_gauss_sft_par(mesh.handle, lm.ctypes.data, M, ptr(V_in), ptr(V_out),
+                                                 int(bool(precession_in)), float(taux_precession),
+                                                 float(angle_s_pi)))
+    return V_out
+
+
+def smb_CurlH_cross_B_gauss_sft_par(communicator, mesh: Mesh, list_mode, V_in, W_in, V_out):
+    """V_out(G,6,m) = (curl V_in) x W_in at the Gauss points."""
+    _check_comm(communicator)
+    M = len(list_mode)
+    L = lib()
+    if is_device(V_in):
+        lm = mesh.list_mode_device(list_mode)
+        check(L.sfx_smb_curlh_cross_b_gauss_sft_par_dev(mesh.handle, lm.data_ptr(), M, ptr(V_in), ptr(W_in),
+                                                        ptr(V_out), stream_ptr()))
+    else:
+        lm = _lm(list_mode)
+        check(L.sfx_smb_curlh_cross_b_gauss_sft_par(mesh.handle, lm.ctypes.data, M, ptr(V_in), ptr(W_in),
+                                                    ptr(V_out)))
+    return V_out
+
+
+def smb_ugradc_gauss_fft_par(communicator, mesh: Mesh, list_mode, heat_capa_in, V_in, c_in, c_out):
+    """c_out(G,2,m) = (hc grad c).u + (div u)(hc c) at the Gauss points."""
+    _check_comm(communicator)
+    M = len(list_mode)
+    L = lib()
+    if is_device(V_in):
+        lm = mesh.list_mode_device(list_mode)
+        check(L.sfx_smb_ugradc_gauss_fft_par_dev(mesh.handle, lm.data_ptr(), M, ptr(heat_capa_in), ptr(V_in),
+                                                 ptr(c_in), ptr(c_out), stream_ptr()))
+    else:
+        lm = _lm(list_mode)
+        hc = np.ascontiguousarray(heat_capa_in, dtype=np.float64)
+        check(L.sfx_smb_ugradc_gauss_fft_par(mesh.handle, lm.ctypes.data, M, hc.ctypes.data, ptr(V_in),
+                                             ptr(c_in), ptr(c_out)))
+    return c_out
+
+
+def nl_rotv_mhd(mesh: Mesh, list_mode, u, H, B, coeff_lorentz, rotv_v):
+    """rotv_v(G,6,m) = (curl u) x u - coeff_lorentz (curl H) x B in one launch."""
+    M = len(list_mode)
+    L = lib()
+    if is_device(u):
+        lm = mesh.list_mode_device(list_mode)
+        check(L.sfx_nl_rotv_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(u), ptr(H), ptr(B),
+                                    float(coeff_lorentz), ptr(rotv_v), stream_ptr()))
+    else:
+        lm = _lm(list_mode)
+        check(L.sfx_nl_rotv_mhd(mesh.handle, lm.ctypes.data, M, ptr(u), ptr(H), ptr(B),
+                                float(coeff_lorentz), ptr(rotv_v)))
+    return rotv_v
+
+
+def rhs_3x3(mesh: Mesh, mode, rhs_in, vb_145, vb_236):
+    """One mode (SOURCES/SOLVER/assembling_rhs.F90:5): rhs_in(G,6) -> vb_145(np,3), vb_236(np,3).
+
+    Host arrays; ``mode`` is only used by the reference's opt_tensor terms.
+    """
+    fdims(rhs_in), fdims(vb_145), fdims(vb_236)
+    check(lib().sfx_rhs_3x3(mesh.handle, int(mode), ptr(rhs_in), ptr(vb_145), ptr(vb_236)))
+    return vb_145, vb_236
+
+
+def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0):
+    """All modes on the device: rhs_gauss(G,6,m) -> vb_145(np,3,m), vb_236(np,3,m)."""
+    M = fdims(rhs_gauss)[2]
+    check(lib().sfx_rhs_3x3_dev(mesh.handle, M, ptr(rhs_gauss), ptr(vb_145), ptr(vb_236), float(scale),
+                                stream_ptr()))
+    return vb_145, vb_236
+
+
+def qs_00_gauss(mesh: Mesh, heat_capa, ff, ff_gauss, vect):
+    """Scalar RHS (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) (host) or vect(np,ncols) (device)."""
+    if is_device(ff):
+        nc = ff.shape[1] if ff.dim() > 1 else 1
+        check(lib().sfx_qs_00_gauss_dev(mesh.handle, nc, ptr(heat_capa), ptr(ff), ptr(ff_gauss), ptr(vect),
+                                        stream_ptr()))
+    else:
+        hc = np.ascontiguousarray(heat_capa, dtype=np.float64)
+        ffa = np.ascontiguousarray(ff, dtype=np.float64)
+        fga = np.ascontiguousarray(ff_gauss, dtype=np.float64)
+        check(lib().sfx_qs_00_gauss(mesh.handle, hc.ctypes.data, ffa.ctypes.data, fga.ctypes.data, ptr(vect)))
+    return vect
+
+
+def nl_rhs_mhd(mesh: Mesh, list_mode, u, H, B, coeff_lorentz, work_gauss, vb_145, vb_236):
+    """Nonlinear share of the NS right-hand sides of an MHD step, all modes (device)."""
+    M = len(list_mode)
+    lm = mesh.list_mode_device(list_mode)
+    check(lib().sfx_nl_rhs_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(u), ptr(H), ptr(B), float(coeff_lorentz),
+                                   ptr(work_gauss), ptr(vb_145), ptr(vb_236), stream_ptr()))
+    return vb_145, vb_236

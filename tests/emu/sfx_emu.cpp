@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <thread>
 #include <vector>
@@ -44,7 +45,7 @@ template <int MP, int OP>
 static void emu_fftprod_mp(const FftProdArgs& a, int wpb) {
   const int64_t pts = (int64_t)Shape<MP>::PW * wpb;
   const int64_t nblk = (a.np + pts - 1) / pts;
-  run_grid(nblk, 64 * wpb, (size_t)wpb * 3 * Shape<MP>::WAVE_CPLX,
+  run_grid(nblk, 64 * wpb, fftprod_smem_cplx<MP>(wpb),
            [&](int t, int64_t b, double2* s) { fftprod_body<MP, OP>(a, t, b, wpb, s); });
 }
 
@@ -79,4 +80,99 @@ extern "C" int emu_fftprod(int op, const double* V1, const double* V2, double* V
     case OP_TENSOR: return emu_op<OP_TENSOR>(a, wpb);
   }
   return 2;
+}
+
+// ---- fused Gauss-point kernels
+#include "../../sfemans_amd/csrc/sfx_gauss_body.h"
+#include "../../sfemans_amd/csrc/sfx_p2tab.h"
+
+template <int MP, int V>
+static void emu_gauss_mp(const GaussArgs& a) {
+  constexpr int E = GaussLds<MP>::E;
+  const int64_t nblk = (a.me + E - 1) / E;
+  run_grid(nblk, 64 * GAUSS_WAVES, GaussLds<MP>::TOTAL,
+           [&](int t, int64_t b, double2* s) { gauss_body<MP, V>(a, t, b, s); });
+}
+
+template <int V>
+static int emu_gauss_v(const GaussArgs& a) {
+  switch (a.MP) {
+    case 4: emu_gauss_mp<4, V>(a); return 0;
+    case 8: emu_gauss_mp<8, V>(a); return 0;
+    case 16: emu_gauss_mp<16, V>(a); return 0;
+    case 32: emu_gauss_mp<32, V>(a); return 0;
+    case 64: emu_gauss_mp<64, V>(a); return 0;
+    case 128: emu_gauss_mp<128, V>(a); return 0;
+    case 256: emu_gauss_mp<256, V>(a); return 0;
+  }
+  return 1;
+}
+
+extern "C" int emu_gauss(int variant, const int32_t* jj, const double* rr, int64_t me, int64_t np,
+                         const double* U, int ncU, const double* U2, int ncU2, const double* U3,
+                         const double* heat_capa, const int32_t* list_mode, int M, double* Vout, int ncout,
+                         double c_lorentz, double om0, double om1, double om2) {
+  GaussArgs a;
+  a.jj = jj; a.rr = rr; a.me = me; a.np = np; a.G = 7 * me;
+  a.U = U; a.U2 = U2; a.U3 = U3;
+  a.nU = np * ncU * (int64_t)M; a.nU2 = np * ncU2 * (int64_t)M; a.nU3 = np * 6 * (int64_t)M;
+  a.heat_capa = heat_capa; a.list_mode = list_mode;
+  double tab[126];
+  p2_reference_tables(tab);
+  a.tab = tab;
+  a.Vout = Vout; a.nout = a.G * ncout * (int64_t)M;
+  a.M = M; a.MP = mp_for(M);
+  a.c_lorentz = c_lorentz;
+  a.om[0] = om0; a.om[1] = om1; a.om[2] = om2;
+  std::vector<double2> wN, wMP;
+  host_twiddles(a.MP, wN, wMP);
+  a.wN = wN.data(); a.wMP = wMP.data();
+  switch (variant) {
+    case GV_CROSS_CURL: return emu_gauss_v<GV_CROSS_CURL>(a);
+    case GV_CURLH_B: return emu_gauss_v<GV_CURLH_B>(a);
+    case GV_UGRADC: return emu_gauss_v<GV_UGRADC>(a);
+    case GV_MHD: return emu_gauss_v<GV_MHD>(a);
+  }
+  return 2;
+}
+
+// ---- right-hand-side assembly
+#include "../../sfemans_amd/csrc/sfx_rhs_body.h"
+
+static void build_adjacency(const int32_t* jj, int64_t me, int64_t np, std::vector<int64_t>& ptr,
+                            std::vector<int32_t>& ent) {
+  ptr.assign(np + 1, 0);
+  for (int64_t e = 0; e < me; ++e)
+    for (int n = 0; n < 6; ++n) ptr[jj[6 * e + n]] += 1;
+  for (int64_t i = 0; i < np; ++i) ptr[i + 1] += ptr[i];
+  ent.assign(ptr[np], 0);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < me; ++e)
+    for (int n = 0; n < 6; ++n) ent[fill[jj[6 * e + n] - 1]++] = (int32_t)(6 * e + n);
+}
+
+extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t me, int64_t np, int M,
+                       const double* rhs, double* v145, double* v236, const double* hc, const double* ff,
+                       const double* ffg, double* vect, double scale) {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> ent;
+  build_adjacency(jj, me, np, ptr, ent);
+  double tab[126];
+  p2_reference_tables(tab);
+  std::vector<double> wgt(7 * me);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.adj_ptr = ptr.data(); a.adj_ent = ent.data(); a.wgt = wgt.data(); a.tab = tab;
+  a.jj = jj; a.me = me; a.np = np; a.G = 7 * me; a.M = M; a.scale = scale; a.n_adj = (int64_t)ent.size();
+  if (which == 0) {
+    a.rhs = rhs; a.v145 = v145; a.v236 = v236;
+    a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = np * 3 * (int64_t)M;
+    for (int64_t g = 0; g < np * M; ++g) rhs3x3_body(a, g);
+  } else {
+    a.heat_capa = hc; a.ff = ff; a.ffg = ffg; a.vect = vect;
+    a.n_ff = np * (int64_t)M; a.n_ffg = a.G * (int64_t)M; a.n_out = np * (int64_t)M;
+    for (int64_t g = 0; g < np * M; ++g) qs00_body(a, g);
+  }
+  return 0;
 }

@@ -38,6 +38,11 @@ def _load():
     vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
     L.emu_fftprod.argtypes = [i32, vp, vp, vp, i64, i32, i32, i32, i32, i32]
     L.emu_fftprod.restype = C.c_int
+    d = C.c_double
+    L.emu_gauss.argtypes = [i32, vp, vp, i64, i64, vp, i32, vp, i32, vp, vp, vp, i32, vp, i32, d, d, d, d]
+    L.emu_gauss.restype = C.c_int
+    L.emu_rhs.argtypes = [i32, vp, vp, i64, i64, i32, vp, vp, vp, vp, vp, vp, vp, d]
+    L.emu_rhs.restype = C.c_int
     return L
 
 
@@ -46,6 +51,7 @@ def _child(conn, fname, args):
         L = _load()
         arrays = [np.asfortranarray(a) if isinstance(a, np.ndarray) else a for a in args]
         ptrs = [a.ctypes.data if isinstance(a, np.ndarray) else a for a in arrays]
+        ptrs = [None if (a is None) else p for a, p in zip(arrays, ptrs)]
         rc = getattr(L, fname)(*ptrs)
         conn.send((rc, [a for a in arrays if isinstance(a, np.ndarray)]))
     except Exception as e:  # pragma: no cover

@@ -96,21 +96,21 @@ def test_test1_known_answer_on_gpu():
 def test_error_codes_mirror_reference():
     z = np.zeros((4, 5, 2), order="F")
     with pytest.raises(SfxError) as e:
-        FP.FFT_PAR_CROSS_PROD_DCL(None, z, z, z.copy())
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z, z, z.copy(order="F"))
     assert e.value.code == 1
     z6 = np.zeros((4, 6, 2), order="F")
     with pytest.raises(SfxError) as e:
-        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(), 2)
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(order="F"), 2)
     assert e.value.code == 2
     with pytest.raises(SfxError) as e:        # aliased grid
-        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(), 1, 5, 2)
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(order="F"), 1, 5, 2)
     assert e.value.code == 2
     with pytest.raises(SfxError) as e:        # bloc_size too small
-        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(), 1, 3, 3)
+        FP.FFT_PAR_CROSS_PROD_DCL(None, z6, z6, z6.copy(order="F"), 1, 3, 3)
     assert e.value.code == 1
     s2 = np.zeros((4, 2, 2), order="F")
     with pytest.raises(SfxError) as e:
-        FP.FFT_SCALAR_VECT_DCL(None, z6, s2, z6.copy(), 2)
+        FP.FFT_SCALAR_VECT_DCL(None, z6, s2, z6.copy(order="F"), 2)
     assert e.value.code == 2
 
 
