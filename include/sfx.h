@@ -158,6 +158,24 @@ int sfx_nl_rotv_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m
                         const double* H, const double* B, double coeff_lorentz, double* rotv_v, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Tangent of the nonlinear terms (no counterpart in the reference, whose
+ * LightKrylov operator binds rmatvec to the forward matvec,
+ * SOURCES/LIGHTKRYLOV_OBJECTS/exponential_propagator.F90:61-62).  Around a
+ * base flow U, with B(a,b) = P[(curl a) x b] the bilinear map of
+ * smb_cross_prod_gauss_sft_par: out(G,6,m) = B(U,dU) + B(dU,U).  The base
+ * flow's Gauss values are recomputed from its nodal coefficients.
+ * ------------------------------------------------------------------------- */
+int sfx_nl_tangent(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* U,
+                   const double* dU, double* out);
+int sfx_nl_tangent_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                       const double* dU, double* out, void* stream);
+/* Linearized convective and Lorentz terms around (U, H0, B0):
+ * out = B(U,dU) + B(dU,U) - coeff_lorentz [B(H0,dB) + B(dH,B0)]. */
+int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                           const double* H0, const double* B0, const double* dU, const double* dH,
+                           const double* dB, double coeff_lorentz, double* out, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Right-hand-side assembly into dense local vectors.  On one rank the PETSc
  * index of component ki of node i is (ki-1)*np + i
  * (SOURCES/INITIALISATION/st_matrix.F90:101), so vb_145/vb_236 are (np,3)

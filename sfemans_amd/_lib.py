@@ -57,6 +57,9 @@ SIGNATURES = {
     "sfx_smb_ugradc_gauss_fft_par_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "sfx_nl_rotv_mhd": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp],
     "sfx_nl_rotv_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp],
+    "sfx_nl_tangent": [_vp, _vp, _i32, _vp, _vp, _vp],
+    "sfx_nl_tangent_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp],
+    "sfx_nl_tangent_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp],
     "sfx_rhs_3x3": [_vp, _i32, _vp, _vp, _vp],
     "sfx_rhs_3x3_dev": [_vp, _i32, _vp, _vp, _vp, _d, _vp],
     "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],

@@ -74,8 +74,8 @@ int get_twiddles(int MP, Twiddles* tw) {
 
 // ---- grow-only device staging buffers for the host-pointer entry points
 struct Staging {
-  void* p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  size_t n[6] = {0, 0, 0, 0, 0, 0};
+  void* p[16] = {nullptr};
+  size_t n[16] = {0};
 };
 static std::map<int, Staging> g_staging;
 

@@ -36,9 +36,8 @@ static hipError_t launch_v(const GaussArgs& a, hipStream_t st) {
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st) {
   switch (variant) {
     case GV_CROSS_CURL: return launch_v<GV_CROSS_CURL>(a, st);
-    case GV_CURLH_B: return launch_v<GV_CURLH_B>(a, st);
+    case GV_PAIRS: return launch_v<GV_PAIRS>(a, st);
     case GV_UGRADC: return launch_v<GV_UGRADC>(a, st);
-    case GV_MHD: return launch_v<GV_MHD>(a, st);
   }
   return hipErrorInvalidValue;
 }

@@ -8,8 +8,11 @@
 //   GV_CURLH_B     smb_CurlH_cross_B_gauss_sft_par (:632-727): V_out = (curl H) x B
 //   GV_UGRADC      smb_ugradc_gauss_fft_par (subroutine_temperature_with_T.F90:247-350):
 //                  c_out = (hc grad c).u + (div u)(hc c)
-//   GV_MHD         rotv_v = (curl u) x u - c_L (curl H) x B, the combination formed at
-//                  subroutine_ns_with_velocity.F90:227, 246-248, in one launch.
+//   GV_PAIRS       sum over passes of coef * (curl pc) x pv: (curl H) x B
+//                  (smb_CurlH_cross_B_gauss_sft_par, :632-727); the MHD term
+//                  rotv_v = (curl u) x u - c_L (curl H) x B formed at
+//                  subroutine_ns_with_velocity.F90:227, 246-248, in one launch; and
+//                  the tangents (curl U) x d + (curl d) x U (+ Lorentz terms).
 //
 // Work decomposition: a work-group of 7 waves owns E = 256/MP consecutive
 // elements, i.e. 7E consecutive Gauss points (index (m-1)*7 + l), PW = 256/MP
@@ -27,13 +30,11 @@
 namespace sfx {
 
 template <int V> struct GVT;
-// NPASS: input passes (each: nodal staging + inverse transforms); NSIG:
-// complex signals per pass; NOUT: real outputs; NC[pass][k]: columns of the
-// k-th nodal array of a pass (0 = unused).
-template <> struct GVT<GV_CROSS_CURL> { static constexpr int NPASS = 1, NSIG = 3, NOUT = 3; };
-template <> struct GVT<GV_CURLH_B>    { static constexpr int NPASS = 1, NSIG = 3, NOUT = 3; };
-template <> struct GVT<GV_UGRADC>     { static constexpr int NPASS = 1, NSIG = 4, NOUT = 1; };
-template <> struct GVT<GV_MHD>        { static constexpr int NPASS = 2, NSIG = 3, NOUT = 3; };
+// NSIG: complex signals per pass; NOUT: real outputs; MAXP: passes (runtime
+// count a.npass for GV_PAIRS: each pass adds coef * (curl pc) x pv).
+template <> struct GVT<GV_CROSS_CURL> { static constexpr int NSIG = 3, NOUT = 3, MAXP = 1; };
+template <> struct GVT<GV_PAIRS>      { static constexpr int NSIG = 3, NOUT = 3, MAXP = 4; };
+template <> struct GVT<GV_UGRADC>     { static constexpr int NSIG = 4, NOUT = 1, MAXP = 1; };
 
 constexpr int GAUSS_WAVES = 7;
 
@@ -206,14 +207,15 @@ SFX_DEV void pack_chunk(int c, int l, const double (&o)[4], const double (&p)[4]
   if (c >= 1) B[4 - c] = (l == 0) ? bv : B[4 - c];
 }
 
-// Real fields (cos, sin pairs) of one pass at own mode; returns NF = 4*NSIG doubles
-// ordered signal by signal as [a1, b1, a2, b2].
-template <int V, int PASS, int MP>
-SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* nodB, int s0, int l,
-                         int mi, double mval, const GaussGeom& g, double hc, double (&F)[4 * GVT<V>::NSIG]) {
+// Real fields (cos, sin pairs) of one pass at own mode; NF = 4*NSIG doubles
+// ordered signal by signal as [a1, b1, a2, b2].  nodA holds the field whose
+// curl is taken (or u for u.grad c), nodB the second field (nullptr: same).
+template <int V, int MP>
+SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* nodB, int s0, int mi,
+                         double mval, const GaussGeom& g, double hc, double (&F)[4 * GVT<V>::NSIG]) {
   using LY = GaussLds<MP>;
   const double mr = mval * g.iray;
-  if constexpr (V == GV_CROSS_CURL || (V == GV_MHD && PASS == 1)) {
+  if constexpr (V == GV_CROSS_CURL || V == GV_PAIRS) {
     double W[6], Dr[6], Dz[6], R[6];
     interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
     curl6(W, Dr, Dz, mr, g.iray, R);
@@ -222,18 +224,14 @@ SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* n
       if (mval == 1.0) { R[0] += a.om[0]; R[3] += a.om[1]; }
       if (mval == 0.0) { R[4] += a.om[2]; }
     }
+    if (V == GV_PAIRS && nodB != nullptr) {
+      double Drb[6], Dzb[6];
+      interp_mode<6, MP>(nodB, s0, LY::ROW6, mi, g, W, Drb, Dzb);
+    }
     // signals (R_r, R_theta), (R_z, W_r), (W_theta, W_z)
     F[0] = R[0]; F[1] = R[1]; F[2] = R[2]; F[3] = R[3];
     F[4] = R[4]; F[5] = R[5]; F[6] = W[0]; F[7] = W[1];
     F[8] = W[2]; F[9] = W[3]; F[10] = W[4]; F[11] = W[5];
-  } else if constexpr (V == GV_CURLH_B || (V == GV_MHD && PASS == 0)) {
-    double W[6], Dr[6], Dz[6], R[6], Wb[6], Drb[6], Dzb[6];
-    interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
-    curl6(W, Dr, Dz, mr, g.iray, R);
-    interp_mode<6, MP>(nodB, s0, LY::ROW6, mi, g, Wb, Drb, Dzb);
-    F[0] = R[0]; F[1] = R[1]; F[2] = R[2]; F[3] = R[3];
-    F[4] = R[4]; F[5] = R[5]; F[6] = Wb[0]; F[7] = Wb[1];
-    F[8] = Wb[2]; F[9] = Wb[3]; F[10] = Wb[4]; F[11] = Wb[5];
   } else {  // GV_UGRADC
     double W[6], Dr[6], Dz[6], C[2], Cr[2], Cz[2];
     interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
@@ -253,7 +251,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   using S = Shape<MP>;
   using LY = GaussLds<MP>;
   constexpr int L = S::L, PW = S::PW, E = LY::E;
-  constexpr int NSIG = GVT<V>::NSIG, NPASS = GVT<V>::NPASS, NO = GVT<V>::NOUT;
+  constexpr int NSIG = GVT<V>::NSIG, NO = GVT<V>::NOUT;
   constexpr int NF = 4 * NSIG;
   constexpr int NOC = (NO + 1) / 2;
   constexpr int GX = 2;
@@ -317,29 +315,28 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   double* xf = big + (size_t)wave * LY::RW;
   double2* lds = reinterpret_cast<double2*>(xf);
 
-  double acc[3][4][3];   // MHD: physical product accumulated over passes
+  double acc[3][4][3];   // physical product accumulated over the passes
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[j][r][0] = acc[j][r][1] = acc[j][r][2] = 0.0;
 
   double2 q[3 * NOC][4];
-#pragma unroll
-  for (int pass = 0; pass < NPASS; ++pass) {
+  const int npass = (V == GV_PAIRS) ? a.npass : 1;
+  for (int pass = 0; pass < npass; ++pass) {
     double2 Z[NSIG][4], B[NSIG][4];
 #pragma unroll
     for (int s = 0; s < NSIG; ++s)
 #pragma unroll
       for (int r = 0; r < 4; ++r) { Z[s][r] = make_double2(0, 0); B[s][r] = make_double2(0, 0); }
     // nodal arrays of this pass: A (6 columns) and B (6 or 2 columns, or none)
-    constexpr bool TWO = (V == GV_CURLH_B || V == GV_UGRADC || V == GV_MHD);
     constexpr int NCB = (V == GV_UGRADC) ? 2 : 6;
-    // MHD: pass 0 = (curl H) x B, pass 1 = (curl u) x u
-    const double* XA = (V == GV_MHD && pass == 0) ? a.U2 : a.U;
-    const int64_t nXA = (V == GV_MHD && pass == 0) ? a.nU2 : a.nU;
-    const double* XB = (V == GV_MHD) ? a.U3 : a.U2;
-    const int64_t nXB = (V == GV_MHD) ? a.nU3 : a.nU2;
-    const bool useB = TWO && !(V == GV_MHD && pass == 1);
+    const double* XA = (V == GV_PAIRS) ? a.pc[pass] : a.U;
+    const int64_t nXA = (V == GV_PAIRS) ? a.npc[pass] : a.nU;
+    const double* XB = (V == GV_PAIRS) ? a.pv[pass] : a.U2;
+    const int64_t nXB = (V == GV_PAIRS) ? a.npv[pass] : a.nU2;
+    const bool useB = (V == GV_UGRADC) || (V == GV_PAIRS && XB != XA);
+    const double coef = (V == GV_PAIRS) ? a.coef[pass] : 1.0;
     NodalFetch<6, MP> fa;
     NodalFetch<NCB, MP> fb;
     nodal_fetch<6, MP>(a, XA, nXA, e0, 0, tid, nid, fa);
@@ -362,8 +359,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
       const int m = c * L + l;
       double own[NF];
       if (m < a.M && valid) {
-        if (pass == 0) pass_fields<V, 0, MP>(a, nodA, nodB, 6 * el, l, l, lmode[m], g, hc, own);
-        else pass_fields<V, 1, MP>(a, nodA, nodB, 6 * el, l, l, lmode[m], g, hc, own);
+        pass_fields<V, MP>(a, nodA, useB ? nodB : nullptr, 6 * el, l, lmode[m], g, hc, own);
       } else {
 #pragma unroll
         for (int f = 0; f < NF; ++f) own[f] = 0.0;
@@ -399,10 +395,9 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           const double cx = u[1] * u[5] - u[2] * u[4];
           const double cy = u[2] * u[3] - u[0] * u[5];
           const double cz = u[0] * u[4] - u[1] * u[3];
-          const double f = (V == GV_MHD && pass == 0) ? -a.c_lorentz : 1.0;
-          acc[j1][r][0] = fma(f, cx, acc[j1][r][0]);
-          acc[j1][r][1] = fma(f, cy, acc[j1][r][1]);
-          acc[j1][r][2] = fma(f, cz, acc[j1][r][2]);
+          acc[j1][r][0] = fma(coef, cx, acc[j1][r][0]);
+          acc[j1][r][1] = fma(coef, cy, acc[j1][r][1]);
+          acc[j1][r][2] = fma(coef, cz, acc[j1][r][2]);
         }
       }
   }

@@ -5,7 +5,7 @@
 namespace sfx {
 
 enum { OP_CROSS = 0, OP_DOT = 1, OP_PROD = 2, OP_SVECT1 = 3, OP_TENSOR = 4 };
-enum { GV_CROSS_CURL = 0, GV_CURLH_B = 1, GV_UGRADC = 2, GV_MHD = 3 };
+enum { GV_CROSS_CURL = 0, GV_PAIRS = 1, GV_UGRADC = 2 };
 
 // Twiddles for one power-of-two MP: wN[j] = e^{2 pi i j/(3 MP)}, wMP[j] = e^{2 pi i j/MP}.
 struct Twiddles {
@@ -40,6 +40,11 @@ struct GaussArgs {
   int64_t nout;
   int M, MP;
   double c_lorentz;
+  int npass;                // GV_PAIRS: passes, each adds coef * (curl pc) x pv
+  const double* pc[4];
+  const double* pv[4];
+  int64_t npc[4], npv[4];
+  double coef[4];
   double om[3];             // precession: 2*Omega added to RotV (mode 1: r_c, theta_s; mode 0: z_c)
   const double2* wN;
   const double2* wMP;

@@ -129,22 +129,36 @@ int sfx_mesh_info(sfx_mesh_t m, int64_t* me, int64_t* np) {
 // ---- Gauss-point entry points
 namespace sfx {
 
+// Curl-type passes of GV_PAIRS: out += coef * (curl pc) x pv.
+struct Passes {
+  int n = 0;
+  const double* pc[4] = {nullptr, nullptr, nullptr, nullptr};
+  const double* pv[4] = {nullptr, nullptr, nullptr, nullptr};
+  double coef[4] = {0, 0, 0, 0};
+  void add(const double* c, const double* v, double k) { pc[n] = c; pv[n] = v; coef[n] = k; ++n; }
+};
+
 int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M,
-                  const double* U, int ncU, const double* U2, int ncU2, const double* U3,
-                  const double* heat_capa_dev, double* out, int ncout, double c_lorentz, const double om[3],
-                  hipStream_t st) {
+                  const double* U, int ncU, const double* U2, int ncU2, const Passes* ps,
+                  const double* heat_capa_dev, double* out, int ncout, const double om[3], hipStream_t st) {
   if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
   if (M <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
   if (M > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, M);
   GaussArgs a;
   std::memset(&a, 0, sizeof(a));
   a.jj = mesh->jj; a.rr = mesh->rr; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me;
-  a.U = U; a.U2 = U2; a.U3 = U3;
-  a.nU = mesh->np * ncU * (int64_t)M; a.nU2 = mesh->np * ncU2 * (int64_t)M; a.nU3 = mesh->np * 6 * (int64_t)M;
+  a.U = U; a.U2 = U2;
+  a.nU = mesh->np * ncU * (int64_t)M; a.nU2 = mesh->np * ncU2 * (int64_t)M;
+  if (ps) {
+    a.npass = ps->n;
+    for (int i = 0; i < ps->n; ++i) {
+      a.pc[i] = ps->pc[i]; a.pv[i] = ps->pv[i]; a.coef[i] = ps->coef[i];
+      a.npc[i] = a.npv[i] = mesh->np * 6 * (int64_t)M;
+    }
+  }
   a.heat_capa = heat_capa_dev; a.list_mode = list_mode_dev; a.tab = mesh->tab;
   a.Vout = out; a.nout = a.G * ncout * (int64_t)M;
   a.M = M; a.MP = mp_for(M);
-  a.c_lorentz = c_lorentz;
   for (int i = 0; i < 3; ++i) a.om[i] = om ? om[i] : 0.0;
   Twiddles tw;
   int rc = get_twiddles(a.MP, &tw);
@@ -156,30 +170,38 @@ int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* l
   return SFX_OK;
 }
 
-// Host path: stage inputs in slots 0..4, run, copy the Gauss output back.
+// Host path: copy the distinct nodal arrays of the call to the device (slots
+// 4..), run, copy the Gauss output back.
+struct HostIn {
+  const double* h;
+  int nc;
+};
+
 static int run_gauss_host(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* list_mode, int32_t M,
-                          const double* U, int ncU, const double* U2, int ncU2, const double* U3,
-                          const double* heat_capa, double* out, int ncout, double c_lorentz, const double om[3]) {
+                          const HostIn* in, int nin, const int* pass_c, const int* pass_v, const double* coef,
+                          int npass, const double* heat_capa, double* out, int ncout, const double om[3]) {
   if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
   if (M <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
-  const size_t nU = (size_t)mesh->np * ncU * M, nU2 = U2 ? (size_t)mesh->np * ncU2 * M : 0;
-  const size_t nU3 = U3 ? (size_t)mesh->np * 6 * M : 0, nO = (size_t)7 * mesh->me * ncout * M;
-  void *dU, *dU2 = nullptr, *dU3 = nullptr, *dO, *dL, *dH = nullptr;
+  const size_t nO = (size_t)7 * mesh->me * ncout * M;
+  void *dO, *dL, *dH = nullptr;
+  void* din[8] = {nullptr};
   int rc;
-  if ((rc = staging(0, nU * 8, &dU)) || (rc = staging(2, nO * 8, &dO)) || (rc = staging(3, M * 4 + 64, &dL)))
-    return rc;
-  if (U2 && (rc = staging(1, nU2 * 8, &dU2))) return rc;
-  if (U3 && (rc = staging(4, nU3 * 8, &dU3))) return rc;
-  if (heat_capa && (rc = staging(5, mesh->me * 8 + 8, &dH))) return rc;
+  if ((rc = staging(0, nO * 8 + 8, &dO)) || (rc = staging(1, M * 4 + 64, &dL))) return rc;
+  if (heat_capa && (rc = staging(2, mesh->me * 8 + 8, &dH))) return rc;
   hipError_t e;
-  if ((e = hipMemcpy(dU, U, nU * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
-  if (U2 && (e = hipMemcpy(dU2, U2, nU2 * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
-  if (U3 && (e = hipMemcpy(dU3, U3, nU3 * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  for (int i = 0; i < nin; ++i) {
+    const size_t n = (size_t)mesh->np * in[i].nc * M;
+    if ((rc = staging(4 + i, n * 8 + 8, &din[i]))) return rc;
+    if ((e = hipMemcpy(din[i], in[i].h, n * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
+  }
   if (heat_capa && (e = hipMemcpy(dH, heat_capa, mesh->me * 8, hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "H2D");
   if ((e = hipMemcpy(dL, list_mode, M * 4, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
-  rc = run_gauss_dev(variant, nm, mesh, (const int32_t*)dL, M, (const double*)dU, ncU, (const double*)dU2, ncU2,
-                     (const double*)dU3, (const double*)dH, (double*)dO, ncout, c_lorentz, om, nullptr);
+  Passes ps;
+  for (int i = 0; i < npass; ++i) ps.add((const double*)din[pass_c[i]], (const double*)din[pass_v[i]], coef[i]);
+  rc = run_gauss_dev(variant, nm, mesh, (const int32_t*)dL, M, (const double*)din[0], in[0].nc,
+                     nin > 1 ? (const double*)din[1] : nullptr, nin > 1 ? in[1].nc : 0, npass ? &ps : nullptr,
+                     (const double*)dH, (double*)dO, ncout, om, nullptr);
   if (rc) return rc;
   if ((e = hipMemcpy(out, dO, nO * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
   return SFX_OK;
@@ -205,8 +227,9 @@ int sfx_smb_cross_prod_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, 
   clear_err();
   double om[3];
   precession_om(precession_in, taux_precession, angle_s_pi, om);
-  return run_gauss_host(GV_CROSS_CURL, "smb_cross_prod_gauss_sft_par", mesh, list_mode, m_max_c, V_in, 6,
-                        nullptr, 0, nullptr, nullptr, V_out, 6, 0.0, om);
+  const HostIn in[1] = {{V_in, 6}};
+  return run_gauss_host(GV_CROSS_CURL, "smb_cross_prod_gauss_sft_par", mesh, list_mode, m_max_c, in, 1, nullptr,
+                        nullptr, nullptr, 0, nullptr, V_out, 6, om);
 }
 
 int sfx_smb_cross_prod_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
@@ -216,21 +239,26 @@ int sfx_smb_cross_prod_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mo
   double om[3];
   precession_om(precession_in, taux_precession, angle_s_pi, om);
   return run_gauss_dev(GV_CROSS_CURL, "smb_cross_prod_gauss_sft_par", mesh, list_mode_dev, m_max_c, V_in, 6,
-                       nullptr, 0, nullptr, nullptr, V_out, 6, 0.0, om, (hipStream_t)stream);
+                       nullptr, 0, nullptr, nullptr, V_out, 6, om, (hipStream_t)stream);
 }
 
 int sfx_smb_curlh_cross_b_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
                                         const double* V_in, const double* W_in, double* V_out) {
   clear_err();
-  return run_gauss_host(GV_CURLH_B, "smb_CurlH_cross_B_gauss_sft_par", mesh, list_mode, m_max_c, V_in, 6, W_in,
-                        6, nullptr, nullptr, V_out, 6, 0.0, nullptr);
+  const HostIn in[2] = {{V_in, 6}, {W_in, 6}};
+  const int pc[1] = {0}, pv[1] = {1};
+  const double k[1] = {1.0};
+  return run_gauss_host(GV_PAIRS, "smb_CurlH_cross_B_gauss_sft_par", mesh, list_mode, m_max_c, in, 2, pc, pv, k, 1,
+                        nullptr, V_out, 6, nullptr);
 }
 
 int sfx_smb_curlh_cross_b_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
                                             const double* V_in, const double* W_in, double* V_out, void* stream) {
   clear_err();
-  return run_gauss_dev(GV_CURLH_B, "smb_CurlH_cross_B_gauss_sft_par", mesh, list_mode_dev, m_max_c, V_in, 6,
-                       W_in, 6, nullptr, nullptr, V_out, 6, 0.0, nullptr, (hipStream_t)stream);
+  Passes ps;
+  ps.add(V_in, W_in, 1.0);
+  return run_gauss_dev(GV_PAIRS, "smb_CurlH_cross_B_gauss_sft_par", mesh, list_mode_dev, m_max_c, V_in, 6, W_in,
+                       6, &ps, nullptr, V_out, 6, nullptr, (hipStream_t)stream);
 }
 
 int sfx_smb_ugradc_gauss_fft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
@@ -238,8 +266,9 @@ int sfx_smb_ugradc_gauss_fft_par(sfx_mesh_t mesh, const int32_t* list_mode, int3
                                  double* c_out) {
   clear_err();
   if (!heat_capa_in) return fail(SFX_ERR_ARG, "smb_ugradc_gauss_fft_par: heat_capa_in is required");
-  return run_gauss_host(GV_UGRADC, "smb_ugradc_gauss_fft_par", mesh, list_mode, m_max_c, V_in, 6, c_in, 2,
-                        nullptr, heat_capa_in, c_out, 2, 0.0, nullptr);
+  const HostIn in[2] = {{V_in, 6}, {c_in, 2}};
+  return run_gauss_host(GV_UGRADC, "smb_ugradc_gauss_fft_par", mesh, list_mode, m_max_c, in, 2, nullptr, nullptr,
+                        nullptr, 0, heat_capa_in, c_out, 2, nullptr);
 }
 
 int sfx_smb_ugradc_gauss_fft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
@@ -248,21 +277,60 @@ int sfx_smb_ugradc_gauss_fft_par_dev(sfx_mesh_t mesh, const int32_t* list_mode_d
   clear_err();
   if (!heat_capa_dev) return fail(SFX_ERR_ARG, "smb_ugradc_gauss_fft_par: heat_capa_in is required");
   return run_gauss_dev(GV_UGRADC, "smb_ugradc_gauss_fft_par", mesh, list_mode_dev, m_max_c, V_in, 6, c_in, 2,
-                       nullptr, heat_capa_dev, c_out, 2, 0.0, nullptr, (hipStream_t)stream);
+                       nullptr, heat_capa_dev, c_out, 2, nullptr, (hipStream_t)stream);
 }
 
 int sfx_nl_rotv_mhd(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* u,
                     const double* H, const double* B, double coeff_lorentz, double* rotv_v) {
   clear_err();
-  return run_gauss_host(GV_MHD, "nl_rotv_mhd", mesh, list_mode, m_max_c, u, 6, H, 6, B, nullptr, rotv_v, 6,
-                        coeff_lorentz, nullptr);
+  const HostIn in[3] = {{u, 6}, {H, 6}, {B, 6}};
+  const int pc[2] = {1, 0}, pv[2] = {2, 0};
+  const double k[2] = {-coeff_lorentz, 1.0};
+  return run_gauss_host(GV_PAIRS, "nl_rotv_mhd", mesh, list_mode, m_max_c, in, 3, pc, pv, k, 2, nullptr, rotv_v,
+                        6, nullptr);
 }
 
 int sfx_nl_rotv_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
                         const double* H, const double* B, double coeff_lorentz, double* rotv_v, void* stream) {
   clear_err();
-  return run_gauss_dev(GV_MHD, "nl_rotv_mhd", mesh, list_mode_dev, m_max_c, u, 6, H, 6, B, nullptr, rotv_v, 6,
-                       coeff_lorentz, nullptr, (hipStream_t)stream);
+  Passes ps;
+  ps.add(H, B, -coeff_lorentz);
+  ps.add(u, u, 1.0);
+  return run_gauss_dev(GV_PAIRS, "nl_rotv_mhd", mesh, list_mode_dev, m_max_c, u, 6, nullptr, 0, &ps, nullptr,
+                       rotv_v, 6, nullptr, (hipStream_t)stream);
+}
+
+int sfx_nl_tangent(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* U,
+                   const double* dU, double* out) {
+  clear_err();
+  const HostIn in[2] = {{U, 6}, {dU, 6}};
+  const int pc[2] = {0, 1}, pv[2] = {1, 0};
+  const double k[2] = {1.0, 1.0};
+  return run_gauss_host(GV_PAIRS, "nl_tangent", mesh, list_mode, m_max_c, in, 2, pc, pv, k, 2, nullptr, out, 6,
+                        nullptr);
+}
+
+int sfx_nl_tangent_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                       const double* dU, double* out, void* stream) {
+  clear_err();
+  Passes ps;
+  ps.add(U, dU, 1.0);
+  ps.add(dU, U, 1.0);
+  return run_gauss_dev(GV_PAIRS, "nl_tangent", mesh, list_mode_dev, m_max_c, U, 6, nullptr, 0, &ps, nullptr, out,
+                       6, nullptr, (hipStream_t)stream);
+}
+
+int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                           const double* H0, const double* B0, const double* dU, const double* dH,
+                           const double* dB, double coeff_lorentz, double* out, void* stream) {
+  clear_err();
+  Passes ps;
+  ps.add(U, dU, 1.0);
+  ps.add(dU, U, 1.0);
+  ps.add(H0, dB, -coeff_lorentz);
+  ps.add(dH, B0, -coeff_lorentz);
+  return run_gauss_dev(GV_PAIRS, "nl_tangent_mhd", mesh, list_mode_dev, m_max_c, U, 6, nullptr, 0, &ps, nullptr,
+                       out, 6, nullptr, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -20,7 +20,7 @@ from .fft_parallele import fdims, is_device, ptr, stream_ptr
 
 __all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "smb_CurlH_cross_B_gauss_sft_par",
            "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes",
-           "qs_00_gauss", "nl_rhs_mhd"]
+           "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd"]
 
 
 class Mesh:
@@ -187,3 +187,25 @@ def nl_rhs_mhd(mesh: Mesh, list_mode, u, H, B, coeff_lorentz, work_gauss, vb_145
     check(lib().sfx_nl_rhs_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(u), ptr(H), ptr(B), float(coeff_lorentz),
                                    ptr(work_gauss), ptr(vb_145), ptr(vb_236), stream_ptr()))
     return vb_145, vb_236
+
+
+def nl_tangent(mesh: Mesh, list_mode, U, dU, out):
+    """Tangent of smb_cross_prod_gauss_sft_par around U: B(U,dU) + B(dU,U), B(a,b) = P[(curl a) x b]."""
+    M = len(list_mode)
+    L = lib()
+    if is_device(U):
+        lm = mesh.list_mode_device(list_mode)
+        check(L.sfx_nl_tangent_dev(mesh.handle, lm.data_ptr(), M, ptr(U), ptr(dU), ptr(out), stream_ptr()))
+    else:
+        lm = _lm(list_mode)
+        check(L.sfx_nl_tangent(mesh.handle, lm.ctypes.data, M, ptr(U), ptr(dU), ptr(out)))
+    return out
+
+
+def nl_tangent_mhd(mesh: Mesh, list_mode, U, H0, B0, dU, dH, dB, coeff_lorentz, out):
+    """Linearized convective + Lorentz terms around (U, H0, B0) (device arrays)."""
+    M = len(list_mode)
+    lm = mesh.list_mode_device(list_mode)
+    check(lib().sfx_nl_tangent_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(U), ptr(H0), ptr(B0), ptr(dU), ptr(dH),
+                                       ptr(dB), float(coeff_lorentz), ptr(out), stream_ptr()))
+    return out

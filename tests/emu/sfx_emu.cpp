@@ -112,26 +112,36 @@ extern "C" int emu_gauss(int variant, const int32_t* jj, const double* rr, int64
                          const double* U, int ncU, const double* U2, int ncU2, const double* U3,
                          const double* heat_capa, const int32_t* list_mode, int M, double* Vout, int ncout,
                          double c_lorentz, double om0, double om1, double om2) {
+  // variants: 0 cross-curl, 1 (curl U) x U2, 2 u.grad c, 3 MHD (U=u, U2=H, U3=B), 4 tangent (U, U2=dU)
   GaussArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.jj = jj; a.rr = rr; a.me = me; a.np = np; a.G = 7 * me;
-  a.U = U; a.U2 = U2; a.U3 = U3;
-  a.nU = np * ncU * (int64_t)M; a.nU2 = np * ncU2 * (int64_t)M; a.nU3 = np * 6 * (int64_t)M;
+  a.U = U; a.U2 = U2;
+  a.nU = np * ncU * (int64_t)M; a.nU2 = np * ncU2 * (int64_t)M;
+  const int64_t n6 = np * 6 * (int64_t)M;
+  int kernel = GV_PAIRS;
+  auto add = [&](const double* c, const double* v, double k) {
+    a.pc[a.npass] = c; a.pv[a.npass] = v; a.coef[a.npass] = k; a.npc[a.npass] = a.npv[a.npass] = n6; ++a.npass;
+  };
+  if (variant == 0) kernel = GV_CROSS_CURL;
+  else if (variant == 2) kernel = GV_UGRADC;
+  else if (variant == 1) add(U, U2, 1.0);
+  else if (variant == 3) { add(U2, U3, -c_lorentz); add(U, U, 1.0); }
+  else if (variant == 4) { add(U, U2, 1.0); add(U2, U, 1.0); }
   a.heat_capa = heat_capa; a.list_mode = list_mode;
   double tab[126];
   p2_reference_tables(tab);
   a.tab = tab;
   a.Vout = Vout; a.nout = a.G * ncout * (int64_t)M;
   a.M = M; a.MP = mp_for(M);
-  a.c_lorentz = c_lorentz;
   a.om[0] = om0; a.om[1] = om1; a.om[2] = om2;
   std::vector<double2> wN, wMP;
   host_twiddles(a.MP, wN, wMP);
   a.wN = wN.data(); a.wMP = wMP.data();
-  switch (variant) {
+  switch (kernel) {
     case GV_CROSS_CURL: return emu_gauss_v<GV_CROSS_CURL>(a);
-    case GV_CURLH_B: return emu_gauss_v<GV_CURLH_B>(a);
+    case GV_PAIRS: return emu_gauss_v<GV_PAIRS>(a);
     case GV_UGRADC: return emu_gauss_v<GV_UGRADC>(a);
-    case GV_MHD: return emu_gauss_v<GV_MHD>(a);
   }
   return 2;
 }

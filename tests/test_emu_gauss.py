@@ -85,3 +85,15 @@ def test_mhd_rotv(nx, ny, M):
     exp = (smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u)
            - cl * smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, H, B))
     assert rel(got, exp) <= TOL
+
+
+@pytest.mark.parametrize("nx,ny,M", [(2, 3, 5), (3, 2, 32)])
+def test_tangent_is_symmetrised_bilinear_form(nx, ny, M):
+    m = rectangle_p2(nx, ny)
+    modes = np.arange(M)
+    U = fourier_field(m.np, 6, M, 61, 2.0)
+    D = fourier_field(m.np, 6, M, 62, 2.0)
+    got = run(4, m, U, D, None, None, modes, 6)
+    exp = (smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, U, D)
+           + smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, D, U))
+    assert rel(got, exp) <= TOL

@@ -157,3 +157,24 @@ def test_nl_rhs_mhd_all_modes(M):
             - 0.5 * smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, H, B))
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, -rotv)
     assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+
+
+@pytest.mark.parametrize("M", [5, 32])
+def test_tangent_matches_bilinear_form_and_central_difference(M):
+    m = rectangle_p2(6, 7)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    U = fourier_field(m.np, 6, M, 71, 2.0)
+    D = fourier_field(m.np, 6, M, 72, 2.0)
+    out = np.zeros((m.n_gauss, 6, M), order="F")
+    NL.nl_tangent(mesh, modes, U, D, out)
+    exp = (smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, U, D)
+           + smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, D, U))
+    assert rel(out, exp) <= TOL
+    # central difference of the forward kernel: exact for a bilinear map (eps = 1),
+    # rounding-limited at eps = 1e-6
+    fp = np.zeros_like(out); fm = np.zeros_like(out)
+    for eps, tol in ((1.0, 1e-12), (1e-6, 1e-8)):
+        NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, np.asfortranarray(U + eps * D), fp)
+        NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, np.asfortranarray(U - eps * D), fm)
+        assert rel((fp - fm) / (2 * eps), out) <= tol
