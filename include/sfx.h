@@ -169,6 +169,14 @@ int sfx_nl_tangent(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, c
                    const double* dU, double* out);
 int sfx_nl_tangent_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
                        const double* dU, double* out, void* stream);
+/* Exact transpose of sfx_nl_tangent in the Euclidean inner product of the
+ * coefficient arrays: dbar(np,6,m) = T_U^T ybar for ybar(G,6,m).  work_elem
+ * is a caller-provided (6*me*6*m) device buffer of per-element nodal
+ * contributions, summed node by node in element order (deterministic). */
+int sfx_nl_adjoint(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* U, const double* ybar,
+                   double* dbar);
+int sfx_nl_adjoint_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                       const double* ybar, double* work_elem, double* dbar, void* stream);
 /* Linearized convective and Lorentz terms around (U, H0, B0):
  * out = B(U,dU) + B(dU,U) - coeff_lorentz [B(H0,dB) + B(dH,B0)]. */
 int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,

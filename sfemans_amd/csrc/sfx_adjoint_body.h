@@ -1,0 +1,323 @@
+// Adjoint (exact transpose, Euclidean inner product on the coefficient
+// arrays) of the tangent of the convective term, body for gfx950 and the CPU
+// emulator.  No counterpart in the reference: its LightKrylov operator binds
+// rmatvec to the forward matvec (SOURCES/LIGHTKRYLOV_OBJECTS/exponential_propagator.F90:61-62).
+//
+// Tangent around U, per Gauss point p (W_p: interpolation, R_p: curl in
+// cos/sin form, Psi: synthesis on the grid, Phi: analysis and truncation):
+//     y_p = Phi( r_U x Psi(W_p d) + Psi(R_p d) x w_U )
+// so, with pbar = Phi^T ybar_p,
+//     dbar = sum_p  W_p^T Psi^T (pbar x r_U)  +  R_p^T Psi^T (w_U x pbar).
+// Phi^T is a synthesis of the coefficients scaled by (1/N, 2/N, 2/N) (mode 0,
+// cos, sin; the mode-0 sine is dropped); Psi^T is an analysis scaled by
+// (N, N/2, N/2) with a zero mode-0 sine.  R_p^T and W_p^T are applied per
+// mode and Gauss point; the 7 Gauss points of an element are summed through
+// LDS into per-element nodal contributions elo(6 me, 6, M), which a node-
+// centred gather (adjoint_gather_body) sums in element order.
+#pragma once
+#include "sfx_gauss_body.h"
+
+namespace sfx {
+
+struct AdjArgs {
+  GaussArgs g;              // mesh, base flow U (g.U), list_mode, twiddles, tables
+  const double* ybar;       // (G, 6, M)
+  int64_t nybar;
+  double* elo;              // (6 me, 6, M) per-element nodal contributions
+  int64_t nelo;
+};
+
+template <int MP>
+struct AdjLds {
+  using LY = GaussLds<MP>;
+  // per-wave region: 3 exchanged signals (3 x 256 complex) = 1536 doubles
+  static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
+  static constexpr int RED = 7 * LY::E * (MP / 4) * 18;    // doubles: points x modes-in-slot x 18
+  static constexpr int BIGD = (GAUSS_WAVES * RW > 2 * LY::NOD6) ? GAUSS_WAVES * RW : 2 * LY::NOD6;
+  static constexpr int BIGD2 = BIGD > RED ? BIGD : RED;
+  static constexpr int TOTAL = LY::BIG + (BIGD2 + 1) / 2;   // double2 units
+};
+
+// Load ybar at own and partner modes of Gauss point p, scaled for Phi^T, and
+// pack the three real output fields as signals (pr, pth) and (pz, 0).
+template <int MP>
+SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (&Z)[2][4], double2 (&B)[2][4]) {
+  constexpr int L = MP / 4, N = 3 * MP;
+  const GaussArgs& a = A.g;
+  const int64_t pc = valid ? p : 0;
+  const double two_n = 2.0 / N, one_n = 1.0 / N;
+  double v[4][2][6];   // [slot][own/partner][a_r, b_r, a_th, b_th, a_z, b_z]
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = l + L * r, k = MP - m;
+    const int mc = (m < a.M) ? m : 0;
+    const int kc = (m > 0 && k < a.M) ? k : 0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const int64_t i1 = pc + a.G * (c + 6 * (int64_t)mc);
+      const int64_t i2 = pc + a.G * (c + 6 * (int64_t)kc);
+      SFX_CHECK("ybar load", i1, A.nybar);
+      SFX_CHECK("ybar load", i2, A.nybar);
+      v[r][0][c] = A.ybar[i1];
+      v[r][1][c] = A.ybar[i2];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = l + L * r, k = MP - m;
+    const bool own = valid && m < a.M;
+    const bool par = valid && m > 0 && k < a.M;
+    const double so = own ? (m == 0 ? one_n : two_n) : 0.0;
+    const double sp = par ? two_n : 0.0;
+    double o[6], q[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { o[c] = so * v[r][0][c]; q[c] = sp * v[r][1][c]; }
+    if (m == 0) { o[1] = 0; o[3] = 0; o[5] = 0; }
+    // signal 0: fields (r, theta); signal 1: (z, 0)
+    Z[0][r] = (m == 0) ? make_double2(o[0], o[2]) : make_double2(0.5 * (o[0] + o[3]), 0.5 * (o[2] - o[1]));
+    B[0][r] = make_double2(0.5 * (q[0] - q[3]), 0.5 * (q[1] + q[2]));
+    Z[1][r] = (m == 0) ? make_double2(o[4], 0.0) : make_double2(0.5 * o[4], -0.5 * o[5]);
+    B[1][r] = make_double2(0.5 * q[4], 0.5 * q[5]);
+  }
+}
+
+template <int MP>
+SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem) {
+  using S = Shape<MP>;
+  using LY = GaussLds<MP>;
+  using AL = AdjLds<MP>;
+  const GaussArgs& a = A.g;
+  constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N;
+  constexpr int GX = 3;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int pt = lane / L, l = lane % L;
+  double* smd = reinterpret_cast<double*>(smem);
+  double2* tw = smem + LY::TW;
+  double* tab = smd + 2 * LY::TAB;
+  double* lmode = smd + 2 * LY::MODE;
+  double2* geo = smem + LY::GEO;
+  int* nid = reinterpret_cast<int*>(smem + LY::NID);
+  const int64_t e0 = bid * E;
+  {
+    constexpr int NT = 64 * GAUSS_WAVES;
+    constexpr int ITW = (4 * MP + NT - 1) / NT;
+    double2 twv[ITW];
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int i = tid + it * NT;
+      const int ic = i < 4 * MP ? i : 0;
+      twv[it] = (ic < 3 * MP) ? a.wN[ic] : a.wMP[ic - 3 * MP];
+    }
+    const double tv = a.tab[tid < 126 ? tid : 0];
+    const int lmv = a.list_mode[tid < a.M ? tid : 0];
+    const int ig = tid < E * 6 ? tid : 0;
+    const int64_t eg = e0 + ig / 6;
+    const int64_t egc = eg < a.me ? eg : 0;
+    const int node = a.jj[6 * egc + ig % 6] - 1;
+    SFX_CHECK("rr load", 2 * (int64_t)node + 1, 2 * a.np);
+    const double2 rz = make_double2(a.rr[2 * (int64_t)node], a.rr[2 * (int64_t)node + 1]);
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int i = tid + it * NT;
+      if (i < 4 * MP) tw[i] = twv[it];
+    }
+    if (tid < 126) tab[tid] = tv;
+    if (tid < MP) lmode[tid] = (tid < a.M) ? (double)lmv : 0.0;
+    if (tid < E * 6) {
+      geo[tid] = (eg < a.me) ? rz : make_double2(1.0, 0.0);
+      nid[tid] = (eg < a.me) ? node : 0;
+    }
+  }
+  SFX_SYNC();
+  const double2* wN = tw;
+  const double2* wMP = tw + 3 * MP;
+  const int lp = wave * PW + pt;
+  const int el = lp / 7, lg = lp % 7;
+  const int64_t p = e0 * 7 + lp;
+  const bool valid = (e0 + el) < a.me;
+  GaussGeom g;
+  gauss_geom(geo + 6 * el, tab, valid ? lg : 0, g);
+  double* big = smd + 2 * LY::BIG;
+  double* xf = big + (size_t)wave * AL::RW;
+  double2* lds = reinterpret_cast<double2*>(xf);
+
+  // ---- pbar = Phi^T ybar on the grid: 2 signals x 3 branches
+  double pb[3][4][3];
+  {
+    double2 Z[2][4], B[2][4];
+    load_ybar<MP>(A, p, valid, l, Z, B);
+    double2 y[6][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) prestep<MP>(Z[s], B[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
+    inverse_fft<MP, 6, GX>(y, pt, l, lds, wMP);
+#pragma unroll
+    for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        pb[j1][r][0] = y[j1][r].x;
+        pb[j1][r][1] = y[j1][r].y;
+        pb[j1][r][2] = y[3 + j1][r].x;
+      }
+  }
+
+  // ---- base flow on the grid: (curl U, U), then the two products
+  double2 q[9][4];
+  {
+    double2 Z[3][4], B[3][4];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { Z[s][r] = make_double2(0, 0); B[s][r] = make_double2(0, 0); }
+    NodalFetch<6, MP> fa;
+    nodal_fetch<6, MP>(a, a.U, a.nU, e0, 0, tid, nid, fa);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      SFX_SYNC();
+      nodal_store<6, MP>(fa, tid, big, LY::ROW6);
+      SFX_SYNC();
+      if (c < 3) nodal_fetch<6, MP>(a, a.U, a.nU, e0, c + 1, tid, nid, fa);
+      const int m = c * L + l;
+      double own[12];
+      if (m < a.M && valid) {
+        pass_fields<GV_CROSS_CURL, MP>(a, big, nullptr, 6 * el, l, lmode[m], g, 1.0, own);
+      } else {
+#pragma unroll
+        for (int f = 0; f < 12; ++f) own[f] = 0.0;
+      }
+      SFX_SYNC();
+      double par[12];
+      partner_swap<12, MP>(own, par, pt, l, xf);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const double o4[4] = {own[4 * s], own[4 * s + 1], own[4 * s + 2], own[4 * s + 3]};
+        const double p4[4] = {par[4 * s], par[4 * s + 1], par[4 * s + 2], par[4 * s + 3]};
+        pack_chunk<MP>(c, l, o4, p4, Z[s], B[s]);
+      }
+    }
+    double2 y[9][4];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) prestep<MP>(Z[s], B[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
+    SFX_SYNC();
+    inverse_fft<MP, 9, GX>(y, pt, l, lds, wMP);
+    // fields (rU_r, rU_th), (rU_z, wU_r), (wU_th, wU_z);
+    // outputs wbar = pbar x rU, rbar = wU x pbar packed as (wb_r, wb_th), (wb_z, rb_r), (rb_th, rb_z)
+#pragma unroll
+    for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double r0 = y[j1][r].x, r1 = y[j1][r].y, r2 = y[3 + j1][r].x;
+        const double w0 = y[3 + j1][r].y, w1 = y[6 + j1][r].x, w2 = y[6 + j1][r].y;
+        const double p0 = pb[j1][r][0], p1 = pb[j1][r][1], p2 = pb[j1][r][2];
+        const double wb0 = p1 * r2 - p2 * r1, wb1 = p2 * r0 - p0 * r2, wb2 = p0 * r1 - p1 * r0;
+        const double rb0 = w1 * p2 - w2 * p1, rb1 = w2 * p0 - w0 * p2, rb2 = w0 * p1 - w1 * p0;
+        q[j1][r] = make_double2(wb0, wb1);
+        q[3 + j1][r] = make_double2(wb2, rb0);
+        q[6 + j1][r] = make_double2(rb1, rb2);
+      }
+  }
+  forward_fft<MP, 9, GX>(q, pt, l, lds, wMP);
+  double2 lo[3][4], hi[3][4];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
+  double oa[3][2][4], ob[3][2][4];
+  pair_modes<MP, 3>(lo, hi, pt, l, lds, oa, ob);
+
+  // ---- Psi^T scaling, curl transpose, reference-coordinate adjoints,
+  //      then the sum over the element's Gauss points through LDS
+  double* red = big;
+  SFX_SYNC();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = l + L * r;
+    const bool ok = valid && m < a.M;
+    const double sa = (m == 0) ? (double)N : 0.5 * N;
+    const double sb = (m == 0) ? 0.0 : 0.5 * N;
+    double Wb[6], Rb[6];
+    // real fields: t0 = (wb_r, wb_th), t1 = (wb_z, rb_r), t2 = (rb_th, rb_z)
+    Wb[0] = sa * oa[0][0][r]; Wb[1] = sb * ob[0][0][r];
+    Wb[2] = sa * oa[0][1][r]; Wb[3] = sb * ob[0][1][r];
+    Wb[4] = sa * oa[1][0][r]; Wb[5] = sb * ob[1][0][r];
+    Rb[0] = sa * oa[1][1][r]; Rb[1] = sb * ob[1][1][r];
+    Rb[2] = sa * oa[2][0][r]; Rb[3] = sb * ob[2][0][r];
+    Rb[4] = sa * oa[2][1][r]; Rb[5] = sb * ob[2][1][r];
+    const double mr = lmode[m < MP ? m : 0] * g.iray, ir = g.iray;
+    double Dr[6] = {0, 0, 0, 0, 0, 0}, Dz[6] = {0, 0, 0, 0, 0, 0};
+    // transpose of curl6 (subroutine_ns_with_velocity.F90:594-608)
+    Wb[5] += mr * Rb[0]; Dz[2] -= Rb[0];
+    Wb[4] -= mr * Rb[1]; Dz[3] -= Rb[1];
+    Dz[0] += Rb[2];      Dr[4] -= Rb[2];
+    Dz[1] += Rb[3];      Dr[5] -= Rb[3];
+    Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
+    Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
+    double* dst = red + ((size_t)lp * L + l) * 18;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double xb = fma(g.a11, Dr[k], g.a21 * Dz[k]);
+      const double yb = fma(g.a12, Dr[k], g.a22 * Dz[k]);
+      SFX_CHECK("red write", ((size_t)lp * L + l) * 18 + 3 * k + 2, AL::RED);
+      dst[3 * k] = ok ? Wb[k] : 0.0;
+      dst[3 * k + 1] = ok ? xb : 0.0;
+      dst[3 * k + 2] = ok ? yb : 0.0;
+    }
+    SFX_SYNC();
+    // work item (element, local node n, mode offset mi): six components
+    for (int it = tid; it < E * 6 * L; it += 64 * GAUSS_WAVES) {
+      const int mi = it % L;
+      const int n = (it / L) % 6;
+      const int e = it / (6 * L);
+      const int mm = mi + L * r;
+      const int64_t eg = e0 + e;
+      if (eg >= a.me || mm >= a.M) continue;
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int lg2 = 0; lg2 < 7; ++lg2) {
+        const double* src = red + ((size_t)(7 * e + lg2) * L + mi) * 18;
+        const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] = fma(w, src[3 * k], fma(dx, src[3 * k + 1], fma(dy, src[3 * k + 2], acc[k])));
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const int64_t o = (6 * eg + n) + 6 * a.me * (k + 6 * (int64_t)mm);
+        SFX_CHECK("elo store", o, A.nelo);
+        A.elo[o] = acc[k];
+      }
+    }
+    SFX_SYNC();
+  }
+}
+
+// dbar(n, k, m) = sum over the (element, local node) pairs of node n, in
+// element order, of elo(6e+i, k, m).  One thread per (node, mode).
+struct GatherArgs {
+  const int64_t* adj_ptr;
+  const int32_t* adj_ent;
+  const double* elo;        // (6 me, 6, M)
+  double* out;              // (np, 6, M)
+  int64_t me, np, n_adj, nelo, nout;
+  int M;
+};
+
+SFX_DEV void adjoint_gather_body(const GatherArgs& a, int64_t gid) {
+  const int64_t n = gid % a.np;
+  const int64_t m = gid / a.np;
+  if (m >= a.M) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t q = a.adj_ptr[n]; q < a.adj_ptr[n + 1]; ++q) {
+    SFX_CHECK("adj", q, a.n_adj);
+    const int64_t ent = a.adj_ent[q];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int64_t i = ent + 6 * a.me * (k + 6 * m);
+      SFX_CHECK("elo load", i, a.nelo);
+      acc[k] += a.elo[i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int64_t o = n + a.np * (k + 6 * m);
+    SFX_CHECK("adjoint out", o, a.nout);
+    a.out[o] = acc[k];
+  }
+}
+
+}  // namespace sfx

@@ -51,7 +51,10 @@ struct GaussArgs {
 };
 
 struct RhsArgs;
+struct AdjArgs;
+struct GatherArgs;
 #ifndef SFX_EMULATE
+hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, hipStream_t st);
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);

@@ -10,6 +10,7 @@
 #include "sfx_kernels.h"
 #include "sfx_p2tab.h"
 #include "sfx_rhs_body.h"
+#include "sfx_adjoint_body.h"
 
 // Opaque mesh handle: device copies of the connectivity and coordinates plus
 // the reference-element tables.
@@ -318,6 +319,59 @@ int sfx_nl_tangent_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
   ps.add(dU, U, 1.0);
   return run_gauss_dev(GV_PAIRS, "nl_tangent", mesh, list_mode_dev, m_max_c, U, 6, nullptr, 0, &ps, nullptr, out,
                        6, nullptr, (hipStream_t)stream);
+}
+
+int sfx_nl_adjoint_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                       const double* ybar, double* work_elem, double* dbar, void* stream) {
+  clear_err();
+  const char* nm = "nl_adjoint";
+  if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
+  if (m_max_c <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
+  if (m_max_c > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, m_max_c);
+  AdjArgs A;
+  std::memset(&A, 0, sizeof(A));
+  GaussArgs& a = A.g;
+  a.jj = mesh->jj; a.rr = mesh->rr; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me;
+  a.U = U; a.nU = mesh->np * 6 * (int64_t)m_max_c;
+  a.list_mode = list_mode_dev; a.tab = mesh->tab;
+  a.M = m_max_c; a.MP = mp_for(m_max_c);
+  Twiddles tw;
+  int rc = get_twiddles(a.MP, &tw);
+  if (rc) return rc;
+  a.wN = tw.wN; a.wMP = tw.wMP;
+  A.ybar = ybar; A.nybar = a.G * 6 * (int64_t)m_max_c;
+  A.elo = work_elem; A.nelo = 6 * mesh->me * 6 * (int64_t)m_max_c;
+  GatherArgs ga;
+  ga.adj_ptr = mesh->adj_ptr; ga.adj_ent = mesh->adj_ent; ga.elo = work_elem; ga.out = dbar;
+  ga.me = mesh->me; ga.np = mesh->np; ga.n_adj = mesh->n_adj; ga.nelo = A.nelo;
+  ga.nout = mesh->np * 6 * (int64_t)m_max_c; ga.M = m_max_c;
+  if (mesh->me == 0) return SFX_OK;
+  hipError_t e = launch_adjoint(A, ga, (hipStream_t)stream);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
+int sfx_nl_adjoint(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* U, const double* ybar,
+                   double* dbar) {
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "nl_adjoint: null mesh");
+  if (m_max_c <= 0) return fail(SFX_ERR_ARG, "nl_adjoint: SIZE(list_mode) == 0");
+  const size_t nU = (size_t)mesh->np * 6 * m_max_c, nY = (size_t)7 * mesh->me * 6 * m_max_c;
+  const size_t nE = (size_t)6 * mesh->me * 6 * m_max_c;
+  void *dU, *dY, *dE, *dO, *dL;
+  int rc;
+  if ((rc = staging(0, nU * 8 + 8, &dU)) || (rc = staging(1, nY * 8 + 8, &dY)) || (rc = staging(2, nE * 8 + 8, &dE)) ||
+      (rc = staging(3, nU * 8 + 8, &dO)) || (rc = staging(4, m_max_c * 4 + 64, &dL)))
+    return rc;
+  hipError_t e;
+  if ((e = hipMemcpy(dU, U, nU * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dY, ybar, nY * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dL, list_mode, m_max_c * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "H2D");
+  if ((rc = sfx_nl_adjoint_dev(mesh, (const int32_t*)dL, m_max_c, (const double*)dU, (const double*)dY, (double*)dE,
+                               (double*)dO, nullptr)))
+    return rc;
+  if ((e = hipMemcpy(dbar, dO, nU * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  return SFX_OK;
 }
 
 int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,

@@ -20,7 +20,7 @@ from .fft_parallele import fdims, is_device, ptr, stream_ptr
 
 __all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "smb_CurlH_cross_B_gauss_sft_par",
            "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes",
-           "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd"]
+           "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd", "nl_adjoint"]
 
 
 class Mesh:
@@ -209,3 +209,24 @@ def nl_tangent_mhd(mesh: Mesh, list_mode, U, H0, B0, dU, dH, dB, coeff_lorentz, 
     check(lib().sfx_nl_tangent_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(U), ptr(H0), ptr(B0), ptr(dU), ptr(dH),
                                        ptr(dB), float(coeff_lorentz), ptr(out), stream_ptr()))
     return out
+
+
+def nl_adjoint(mesh: Mesh, list_mode, U, ybar, dbar, work_elem=None):
+    """Exact transpose of :func:`nl_tangent`: dbar(np,6,m) = T_U^T ybar(G,6,m).
+
+    Device path: ``work_elem`` is a (6*me*6*m) float64 CUDA buffer (allocated
+    when omitted).
+    """
+    M = len(list_mode)
+    L = lib()
+    if is_device(U):
+        import torch
+        if work_elem is None:
+            work_elem = torch.empty(6 * mesh.me * 6 * M, dtype=torch.float64, device="cuda")
+        lm = mesh.list_mode_device(list_mode)
+        check(L.sfx_nl_adjoint_dev(mesh.handle, lm.data_ptr(), M, ptr(U), ptr(ybar), work_elem.data_ptr(), ptr(dbar),
+                                   stream_ptr()))
+    else:
+        lm = _lm(list_mode)
+        check(L.sfx_nl_adjoint(mesh.handle, lm.ctypes.data, M, ptr(U), ptr(ybar), ptr(dbar)))
+    return dbar

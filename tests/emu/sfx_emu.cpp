@@ -186,3 +186,52 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
   }
   return 0;
 }
+
+// ---- adjoint of the tangent
+#include "../../sfemans_amd/csrc/sfx_adjoint_body.h"
+
+template <int MP>
+static void emu_adjoint_mp(const AdjArgs& A) {
+  constexpr int E = GaussLds<MP>::E;
+  const int64_t nblk = (A.g.me + E - 1) / E;
+  run_grid(nblk, 64 * GAUSS_WAVES, AdjLds<MP>::TOTAL,
+           [&](int t, int64_t b, double2* s) { adjoint_body<MP>(A, t, b, s); });
+}
+
+extern "C" int emu_adjoint(const int32_t* jj, const double* rr, int64_t me, int64_t np, const double* U,
+                           const double* ybar, const int32_t* list_mode, int M, double* dbar) {
+  AdjArgs A;
+  std::memset(&A, 0, sizeof(A));
+  GaussArgs& a = A.g;
+  a.jj = jj; a.rr = rr; a.me = me; a.np = np; a.G = 7 * me;
+  a.U = U; a.nU = np * 6 * (int64_t)M;
+  a.list_mode = list_mode;
+  double tab[126];
+  p2_reference_tables(tab);
+  a.tab = tab;
+  a.M = M; a.MP = mp_for(M);
+  std::vector<double2> wN, wMP;
+  host_twiddles(a.MP, wN, wMP);
+  a.wN = wN.data(); a.wMP = wMP.data();
+  std::vector<double> elo((size_t)6 * me * 6 * M, -1e300);
+  A.ybar = ybar; A.nybar = a.G * 6 * (int64_t)M;
+  A.elo = elo.data(); A.nelo = (int64_t)elo.size();
+  switch (a.MP) {
+    case 4: emu_adjoint_mp<4>(A); break;
+    case 8: emu_adjoint_mp<8>(A); break;
+    case 16: emu_adjoint_mp<16>(A); break;
+    case 32: emu_adjoint_mp<32>(A); break;
+    case 64: emu_adjoint_mp<64>(A); break;
+    case 128: emu_adjoint_mp<128>(A); break;
+    case 256: emu_adjoint_mp<256>(A); break;
+  }
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> ent;
+  build_adjacency(jj, me, np, ptr, ent);
+  GatherArgs ga;
+  ga.adj_ptr = ptr.data(); ga.adj_ent = ent.data(); ga.elo = elo.data(); ga.out = dbar;
+  ga.me = me; ga.np = np; ga.n_adj = (int64_t)ent.size(); ga.nelo = (int64_t)elo.size();
+  ga.nout = np * 6 * (int64_t)M; ga.M = M;
+  for (int64_t g = 0; g < np * M; ++g) adjoint_gather_body(ga, g);
+  return 0;
+}
