@@ -1,0 +1,57 @@
+"""The Fortran ISO_C_BINDING shim (module fft_parallele over libsfx.so).
+
+Runs the compiled driver ``sfemans_amd/fortran/sfx_fortran_driver`` in a
+child process on raw float64 files and checks FFT_PAR_CROSS_PROD_DCL,
+FFT_PAR_DOT_PROD_DCL and the smb_cross_prod_gauss_sft_par body against the
+oracle (relative max-norm error <= 1e-12).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import fft_par as F  # noqa: E402
+from oracle import smb  # noqa: E402
+from sfemans_amd.mesh import rectangle_p2  # noqa: E402
+from sfemans_amd.synth import fourier_field  # noqa: E402
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "sfemans_amd", "fortran", "sfx_fortran_driver")
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_fortran_shim(tmp_path):
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (amdflang absent at build time)")
+    npt, M = 333, 8
+    v1 = fourier_field(npt, 6, M, 21, 2.0)
+    v2 = fourier_field(npt, 6, M, 22, 2.0)
+    m = rectangle_p2(3, 4)
+    vn = fourier_field(m.np, 6, M, 23, 2.0)
+    (tmp_path / "dims.txt").write_text(f"{npt} 6 {M} {m.me} {m.np}\n")
+    for name, a in (("v1", v1), ("v2", v2), ("vn", vn)):
+        np.asfortranarray(a, dtype="<f8").T.tofile(tmp_path / f"{name}.bin")
+    np.asfortranarray(m.jj, dtype="<i4").T.tofile(tmp_path / "jj.bin")
+    np.asfortranarray(m.rr, dtype="<f8").T.tofile(tmp_path / "rr.bin")
+    print(f"case fortran driver np={npt} M={M} mesh 3x4", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    def load(name, shape):
+        return np.fromfile(tmp_path / f"{name}.bin", dtype="<f8").reshape(shape[::-1]).T
+
+    mp = 3 * M // 2
+    assert rel(load("cross", (npt, 6, M)), F.fft_par_cross_prod_dcl(v1, v2, mp)) <= TOL
+    assert rel(load("dot", (npt, 2, M)), F.fft_par_dot_prod_dcl(v1, v2, mp)) <= TOL
+    exp = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, np.arange(M), vn)
+    assert rel(load("smb", (m.n_gauss, 6, M)), exp) <= TOL
