@@ -215,6 +215,39 @@ int sfx_nl_rhs_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
                        const double* H, const double* B, double coeff_lorentz, double* work_gauss,
                        double* vb_145, double* vb_236, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched persistent launch.  No single counterpart in the reference: a batch
+ * stands for many independent calls of smb_cross_prod_gauss_sft_par
+ * (SOURCES/SIMULATION/subroutine_ns_with_velocity.F90:514), one per problem,
+ * on meshes of different sizes and different m_max_c, run as ONE kernel
+ * launch whose work-groups walk host-built tile lists.  All arrays are device
+ * pointers and must stay valid for the life of the batch. */
+typedef struct {
+  sfx_mesh_t mesh;
+  const int32_t* list_mode_dev;   /* (m_max_c) */
+  int32_t m_max_c;                /* 1..256 */
+  const double* V_in;             /* (np, 6, m_max_c) */
+  double* V_out;                  /* (7 me, 6, m_max_c) */
+} sfx_batch_problem_t;
+typedef struct sfx_batch_s* sfx_batch_t;
+
+#define SFX_BATCH_LPT 0          /* longest tile first to the least-loaded list */
+#define SFX_BATCH_CONTIGUOUS 1   /* equal-cost contiguous ranges, grouped by grid size */
+#define SFX_BATCH_ROUND_ROBIN 2  /* tile i to list i mod n_wg */
+
+int sfx_batch_create(const sfx_batch_problem_t* probs, int32_t n_prob, sfx_batch_t* out);
+/* Set the schedule: policy and number of persistent work-groups. */
+int sfx_batch_schedule(sfx_batch_t batch, int32_t policy, int32_t n_wg);
+/* Time every (policy, n_wg) candidate `reps` times on `stream` and keep the
+ * fastest; cached per batch shape (me, m_max_c of every problem). */
+int sfx_batch_autotune(sfx_batch_t batch, int32_t reps, void* stream);
+/* Capture the launch into a hipGraph; sfx_batch_run then replays it. */
+int sfx_batch_capture(sfx_batch_t batch);
+int sfx_batch_run(sfx_batch_t batch, void* stream);
+int sfx_batch_info(sfx_batch_t batch, int32_t* policy, int32_t* n_wg, int64_t* n_tiles, double* tuned_ms,
+                   int32_t* captured);
+int sfx_batch_destroy(sfx_batch_t batch);
+
 #ifdef __cplusplus
 }
 #endif

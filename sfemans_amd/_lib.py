@@ -67,6 +67,13 @@ SIGNATURES = {
     "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],
     "sfx_qs_00_gauss_dev": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
     "sfx_nl_rhs_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp],
+    "sfx_batch_create": [_vp, _i32, _vp],
+    "sfx_batch_schedule": [_vp, _i32, _i32],
+    "sfx_batch_autotune": [_vp, _i32, _vp],
+    "sfx_batch_capture": [_vp],
+    "sfx_batch_run": [_vp, _vp],
+    "sfx_batch_info": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "sfx_batch_destroy": [_vp],
 }
 _RESTYPE = {"sfx_last_error": C.c_char_p, "sfx_version": C.c_char_p}
 

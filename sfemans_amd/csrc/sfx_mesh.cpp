@@ -12,24 +12,6 @@
 #include "sfx_rhs_body.h"
 #include "sfx_adjoint_body.h"
 
-// Opaque mesh handle: device copies of the connectivity and coordinates plus
-// the reference-element tables.
-struct sfx_mesh_s {
-  int device = 0;
-  int64_t me = 0, np = 0;
-  int32_t* jj = nullptr;    // (6, me)
-  double* rr = nullptr;     // (2, np)
-  double* tab = nullptr;    // ww(6,7), dd(2,6,7)
-  // deterministic node-centred gather for the right-hand-side assembly
-  int64_t* adj_ptr = nullptr;   // (np+1) CSR over nodes
-  int32_t* adj_ent = nullptr;   // entries e*6 + local node, element order
-  int64_t n_adj = 0;
-  double* wgt = nullptr;        // (7, me) rj * ray
-};
-
-namespace sfx {
-
-}  // namespace sfx
 
 using namespace sfx;
 

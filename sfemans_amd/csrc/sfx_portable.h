@@ -13,6 +13,7 @@ struct double2 { double x, y; };
 inline double2 make_double2(double x, double y) { return double2{x, y}; }
 using std::fma;
 #define SFX_DEV inline
+#define SFX_DEV_NOINLINE inline
 #define __restrict__
 void sfx_emu_sync();
 [[noreturn]] void sfx_emu_oob(const char* what, long long idx, long long n, const char* file, int line);
@@ -23,6 +24,8 @@ void sfx_emu_sync();
 #else
 #include <hip/hip_runtime.h>
 #define SFX_DEV __device__ __forceinline__
+// a device function compiled on its own (own register allocation and stack frame)
+#define SFX_DEV_NOINLINE __device__ __noinline__
 #define SFX_SYNC() __syncthreads()
 // Exchange inside one wave through its private LDS region: a wave's LDS
 // instructions execute in order, so only compiler reordering must be stopped.

@@ -29,7 +29,10 @@ using namespace sfx;
 
 static void run_grid(int64_t nblk, int threads, size_t smem_cplx,
                      const std::function<void(int, int64_t, double2*)>& body) {
-  std::vector<double2> smem(smem_cplx);
+  // LDS followed by a guard zone: a write past the end of the declared LDS
+  // size changes the guard and aborts the run
+  constexpr size_t GUARD = 256;
+  std::vector<double2> smem(smem_cplx + GUARD);
   for (int64_t b = 0; b < nblk; ++b) {
     std::barrier<> bar(threads);
     for (auto& x : smem) x = make_double2(1e300, -1e300);   // poison: stale reads show up
@@ -38,6 +41,8 @@ static void run_grid(int64_t nblk, int threads, size_t smem_cplx,
     for (int t = 0; t < threads; ++t)
       th.emplace_back([&, t] { g_bar = &bar; body(t, b, smem.data()); });
     for (auto& x : th) x.join();
+    for (size_t i = smem_cplx; i < smem.size(); ++i)
+      if (smem[i].x != 1e300 || smem[i].y != -1e300) sfx_emu_oob("LDS guard", (long long)i, (long long)smem_cplx, __FILE__, __LINE__);
   }
 }
 
@@ -233,5 +238,48 @@ extern "C" int emu_adjoint(const int32_t* jj, const double* rr, int64_t me, int6
   ga.me = me; ga.np = np; ga.n_adj = (int64_t)ent.size(); ga.nelo = (int64_t)elo.size();
   ga.nout = np * 6 * (int64_t)M; ga.M = M;
   for (int64_t g = 0; g < np * M; ++g) adjoint_gather_body(ga, g);
+  return 0;
+}
+
+// ---- batched persistent launch: problems packed in flat buffers with offsets
+#include "../../sfemans_amd/csrc/sfx_batch_body.h"
+
+extern "C" int emu_batch(int n, const int32_t* jj, const int64_t* jj_off, const double* rr, const int64_t* rr_off,
+                         const int64_t* me, const int64_t* np, const double* U, const int64_t* U_off,
+                         const int32_t* lm, const int64_t* lm_off, const int32_t* M, double* out,
+                         const int64_t* out_off, int policy, int n_wg, int32_t* n_wg_used) {
+  std::vector<GaussArgs> probs(n);
+  std::vector<std::vector<double2>> wN(n), wMP(n);
+  double tab[126];
+  p2_reference_tables(tab);
+  std::vector<int64_t> mev(n);
+  std::vector<int> mp(n);
+  size_t smem = 0;
+  for (int i = 0; i < n; ++i) {
+    GaussArgs& a = probs[i];
+    std::memset(&a, 0, sizeof(a));
+    a.jj = jj + jj_off[i]; a.rr = rr + rr_off[i]; a.me = me[i]; a.np = np[i]; a.G = 7 * me[i];
+    a.U = U + U_off[i]; a.nU = np[i] * 6 * (int64_t)M[i];
+    a.list_mode = lm + lm_off[i]; a.tab = tab;
+    a.Vout = out + out_off[i]; a.nout = a.G * 6 * (int64_t)M[i];
+    a.M = M[i]; a.MP = mp_for(M[i]);
+    host_twiddles(a.MP, wN[i], wMP[i]);
+    a.wN = wN[i].data(); a.wMP = wMP[i].data();
+    mev[i] = me[i];
+    mp[i] = a.MP;
+    if (batch_smem_cplx(a.MP) > smem) smem = batch_smem_cplx(a.MP);
+  }
+  std::vector<BatchTile> nat, tiles;
+  std::vector<double> cost;
+  std::vector<int32_t> ptr;
+  batch_tiles(mev, mp, nat, cost);
+  const int32_t used = batch_schedule(nat, cost, mp, policy, n_wg, tiles, ptr);
+  if (used < 0) return 1;
+  *n_wg_used = used;
+  BatchArgs b;
+  b.probs = probs.data(); b.tiles = tiles.data(); b.wg_ptr = ptr.data();
+  b.n_prob = n; b.n_wg = used; b.n_tiles = (int64_t)tiles.size();
+  run_grid(used, 64 * GAUSS_WAVES, smem,
+           [&](int t, int64_t w, double2* s) { batch_body(b, t, (int)w, s); });
   return 0;
 }
