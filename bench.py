@@ -127,13 +127,16 @@ def wl_mhd(torch, args):
     v145 = FP.fortran_empty((npn, 3, M))
     v236 = FP.fortran_empty((npn, 3, M))
     uxb = FP.fortran_empty((npn, 6, M))
+    welem = torch.empty(36 * m.me * M, dtype=torch.float64, device="cuda")
     mesh_bytes = m.me * 24 + npn * 16
     rotv_bytes = 3 * npn * 6 * M * 8 + G * 6 * M * 8 + mesh_bytes
     phases = [
         Phase("rotv_mhd", lambda: NL.nl_rotv_mhd(mesh, modes, u, H, B, 1.0, work), rotv_bytes,
               G * M * 300 + G * fft_flops_per_point(M, 18, 3, 18), "gauss_kernel<64,PAIRS> (2 passes)"),
-        Phase("rhs_3x3", lambda: NL.rhs_3x3_all_modes(mesh, work, v145, v236, -1.0),
-              G * 6 * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8 + 8 * 6 * m.me, G * M * 6 * 2 * 6, "rhs3x3_kernel"),
+        # Gauss array read once, element contributions written and read back, vb written
+        Phase("rhs_3x3", lambda: NL.rhs_3x3_all_modes(mesh, work, v145, v236, -1.0, welem),
+              G * 6 * M * 8 + 2 * 36 * m.me * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8 + 12 * 6 * m.me,
+              G * M * 6 * 2 * 6, "rhs_elem_kernel + rhs_gather_kernel"),
         Phase("u_x_B_nodes", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb), 3 * npn * 6 * M * 8,
               npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<64,CROSS>"),
     ]

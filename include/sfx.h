@@ -196,9 +196,10 @@ int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_
  * vb_145(np,3), vb_236(np,3).  `mode` is accepted for signature parity. */
 int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_145, double* vb_236);
 /* All modes at once on the device: rhs_gauss(G,6,m) -> vb_145(np,3,m),
- * vb_236(np,3,m), each multiplied by `scale`. */
+ * vb_236(np,3,m), each multiplied by `scale`.  work_elem is a caller-provided
+ * device buffer of 36*me*m doubles (the per-element contributions). */
 int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, double* vb_145, double* vb_236,
-                    double scale, void* stream);
+                    double scale, double* work_elem, void* stream);
 
 /* Replaces qs_00_gauss (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) from
  * heat_capa(me), nodal ff(np) and Gauss ff_gauss(G). */
@@ -210,10 +211,11 @@ int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa,
 
 /* Nonlinear share of the Navier-Stokes right-hand sides of an MHD step,
  * all modes: vb = rhs_3x3(-[(curl u) x u - coeff_lorentz (curl H) x B]).
- * work_gauss is a caller-provided (G,6,m) device buffer. */
+ * work_gauss (G*6*m doubles) and work_elem (36*me*m doubles) are
+ * caller-provided device buffers. */
 int sfx_nl_rhs_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
                        const double* H, const double* B, double coeff_lorentz, double* work_gauss,
-                       double* vb_145, double* vb_236, void* stream);
+                       double* work_elem, double* vb_145, double* vb_236, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Batched persistent launch.  No single counterpart in the reference: a batch

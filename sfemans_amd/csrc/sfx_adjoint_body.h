@@ -260,11 +260,12 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       dst[3 * k + 2] = ok ? yb : 0.0;
     }
     SFX_SYNC();
-    // work item (element, local node n, mode offset mi): six components
+    // work item (element, local node n, mode offset mi): six components;
+    // (element, node) fastest so that consecutive lanes store consecutive entries
     for (int it = tid; it < E * 6 * L; it += 64 * GAUSS_WAVES) {
-      const int mi = it % L;
-      const int n = (it / L) % 6;
-      const int e = it / (6 * L);
+      const int n = it % 6;
+      const int e = (it / 6) % E;
+      const int mi = it / (6 * E);
       const int mm = mi + L * r;
       const int64_t eg = e0 + e;
       if (eg >= a.me || mm >= a.M) continue;

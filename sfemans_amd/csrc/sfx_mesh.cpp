@@ -384,10 +384,11 @@ static RhsArgs rhs_args(sfx_mesh_t mesh, int32_t M, double scale) {
 }
 
 int rhs3x3_dev(sfx_mesh_t mesh, int32_t M, const double* rhs, double* v145, double* v236, double scale,
-               hipStream_t st) {
+               double* work_elem, hipStream_t st) {
+  if (!work_elem && mesh->me > 0 && M > 0) return fail(SFX_ERR_ARG, "rhs_3x3: null work_elem");
   RhsArgs a = rhs_args(mesh, M, scale);
-  a.rhs = rhs; a.v145 = v145; a.v236 = v236;
-  a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = a.np * 3 * (int64_t)M;
+  a.rhs = rhs; a.v145 = v145; a.v236 = v236; a.elo = work_elem;
+  a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = a.np * 3 * (int64_t)M; a.n_elo = 36 * a.me * (int64_t)M;
   hipError_t e = launch_rhs3x3(a, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, "rhs_3x3");
 }
@@ -397,11 +398,11 @@ int rhs3x3_dev(sfx_mesh_t mesh, int32_t M, const double* rhs, double* v145, doub
 extern "C" {
 
 int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, double* vb_145, double* vb_236,
-                    double scale, void* stream) {
+                    double scale, double* work_elem, void* stream) {
   clear_err();
   if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3: null mesh");
   if (m_max_c < 0) return fail(SFX_ERR_ARG, "rhs_3x3: m_max_c < 0");
-  return rhs3x3_dev(mesh, m_max_c, rhs_gauss, vb_145, vb_236, scale, (hipStream_t)stream);
+  return rhs3x3_dev(mesh, m_max_c, rhs_gauss, vb_145, vb_236, scale, work_elem, (hipStream_t)stream);
 }
 
 int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_145, double* vb_236) {
@@ -409,13 +410,14 @@ int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_
   (void)mode;   // only used by the opt_tensor terms of the reference
   if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3: null mesh");
   const size_t nG = (size_t)7 * mesh->me * 6, nN = (size_t)mesh->np * 3;
-  void *dr, *d1, *d2;
+  void *dr, *d1, *d2, *dw;
   int rc;
-  if ((rc = staging(0, nG * 8 + 8, &dr)) || (rc = staging(1, nN * 8 + 8, &d1)) || (rc = staging(2, nN * 8 + 8, &d2)))
+  if ((rc = staging(0, nG * 8 + 8, &dr)) || (rc = staging(1, nN * 8 + 8, &d1)) || (rc = staging(2, nN * 8 + 8, &d2)) ||
+      (rc = staging(3, (size_t)36 * mesh->me * 8 + 8, &dw)))
     return rc;
   hipError_t e;
   if ((e = hipMemcpy(dr, rhs_in, nG * 8, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "H2D");
-  if ((rc = rhs3x3_dev(mesh, 1, (const double*)dr, (double*)d1, (double*)d2, 1.0, nullptr))) return rc;
+  if ((rc = rhs3x3_dev(mesh, 1, (const double*)dr, (double*)d1, (double*)d2, 1.0, (double*)dw, nullptr))) return rc;
   if ((e = hipMemcpy(vb_145, d1, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
   if ((e = hipMemcpy(vb_236, d2, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
   return SFX_OK;
@@ -456,12 +458,12 @@ int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, 
 
 int sfx_nl_rhs_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* u,
                        const double* H, const double* B, double coeff_lorentz, double* work_gauss,
-                       double* vb_145, double* vb_236, void* stream) {
+                       double* work_elem, double* vb_145, double* vb_236, void* stream) {
   clear_err();
   int rc = sfx_nl_rotv_mhd_dev(mesh, list_mode_dev, m_max_c, u, H, B, coeff_lorentz, work_gauss, stream);
   if (rc) return rc;
   // the nonlinear share of rhs_gauss is -rotv_v (computing_rhs_gauss.F90:100)
-  return rhs3x3_dev(mesh, m_max_c, work_gauss, vb_145, vb_236, -1.0, (hipStream_t)stream);
+  return rhs3x3_dev(mesh, m_max_c, work_gauss, vb_145, vb_236, -1.0, work_elem, (hipStream_t)stream);
 }
 
 }  // extern "C"

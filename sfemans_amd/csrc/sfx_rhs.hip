@@ -4,18 +4,25 @@
 
 namespace sfx {
 
-__global__ void __launch_bounds__(256) rhs3x3_kernel(RhsArgs a) {
-  rhs3x3_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+__global__ void __launch_bounds__(256) rhs_elem_kernel(RhsArgs a) {
+  rhs_elem_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) rhs_gather_kernel(RhsArgs a) {
+  rhs_gather_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(256) qs00_kernel(RhsArgs a) {
   qs00_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
+// rhs_3x3: element contributions, then the node-centred gather.
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
-  const int64_t n = a.np * a.M;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(rhs3x3_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  const int64_t ne = a.me * a.M, nn = a.np * a.M;
+  if (ne > 0) hipLaunchKernelGGL(rhs_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nn == 0) return e;
+  hipLaunchKernelGGL(rhs_gather_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -7,6 +7,8 @@
 // element after the other.  Here each output entry is gathered by one thread
 // from the node's (element, local node) list in element order, so the sums
 // are formed in the reference's order, deterministically and without atomics.
+// For rhs_3x3 the element contributions are formed first, one thread per
+// (element, mode) reading the Gauss array once, into a work array.
 //
 //   wgt(e,l)     = rj(l,e) * ray(l,e)                 (assembling_rhs.F90:73)
 //   vb(n, ki, m) = sum_(e,i) sum_l ww(i,l) wgt(e,l) rhs(7e+l, comp(ki), m)
@@ -34,11 +36,51 @@ struct RhsArgs {
   const double* ff;         // (np, M) nodal source
   const double* ffg;        // (G, M) Gauss source
   double* vect;             // (np, M)
-  int64_t n_rhs, n_ff, n_ffg, n_out, n_adj;
+  double* elo;              // (6 me, 6, M) per-element nodal contributions
+  int64_t n_rhs, n_ff, n_ffg, n_out, n_adj, n_elo;
 };
 
-// One thread per (node, mode): all six components.
-SFX_DEV void rhs3x3_body(const RhsArgs& a, int64_t gid) {
+// Element share of rhs_3x3, one thread per (element, mode):
+//   elo(6e+i, k, m) = sum_l ww(i,l) wgt(e,l) rhs(7e+l, comp k, m)
+// (the values the reference adds with VecSetValues, assembling_rhs.F90:73-95).
+// rhs is read once, in element order; rhs_gather_body then sums the entries
+// of each node in element order.
+SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
+  const int64_t e = gid % a.me;
+  const int64_t m = gid / a.me;
+  if (m >= a.M) return;
+  double w[7], r[7][6];
+#pragma unroll
+  for (int l = 0; l < 7; ++l) {
+    w[l] = a.wgt[7 * e + l];
+    const int64_t gi = 7 * e + l + a.G * 6 * m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      SFX_CHECK("rhs load", gi + a.G * k, a.n_rhs);
+      r[l][k] = a.rhs[gi + a.G * k];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double loc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+      const double c = a.tab[i + 6 * l] * w[l];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) loc[k] = fma(c, r[l][k], loc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int64_t o = (6 * e + i) + 6 * a.me * (k + 6 * m);
+      SFX_CHECK("elo store", o, a.n_elo);
+      a.elo[o] = loc[k];
+    }
+  }
+}
+
+// vb_145 / vb_236 of node n, mode m: the node's element contributions summed
+// in element order, components (1,4,5) and (2,3,6), times scale.
+SFX_DEV void rhs_gather_body(const RhsArgs& a, int64_t gid) {
   const int64_t n = gid % a.np;
   const int64_t m = gid / a.np;
   if (m >= a.M) return;
@@ -46,21 +88,13 @@ SFX_DEV void rhs3x3_body(const RhsArgs& a, int64_t gid) {
   const int64_t b = a.adj_ptr[n], e_ = a.adj_ptr[n + 1];
   for (int64_t q = b; q < e_; ++q) {
     SFX_CHECK("adj", q, a.n_adj);
-    const int32_t ent = a.adj_ent[q];
-    const int64_t e = ent / 6;
-    const int i = ent % 6;
-    double loc[6] = {0, 0, 0, 0, 0, 0};
-    for (int l = 0; l < 7; ++l) {
-      const double c = a.tab[i + 6 * l] * a.wgt[7 * e + l];
-      const int64_t gi = 7 * e + l + a.G * 6 * m;
+    const int64_t ent = a.adj_ent[q];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        SFX_CHECK("rhs load", gi + a.G * k, a.n_rhs);
-        loc[k] = fma(c, a.rhs[gi + a.G * k], loc[k]);
-      }
+    for (int k = 0; k < 6; ++k) {
+      const int64_t i = ent + 6 * a.me * (k + 6 * m);
+      SFX_CHECK("elo load", i, a.n_elo);
+      acc[k] += a.elo[i];
     }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) acc[k] += loc[k];
   }
   const int64_t o = n + a.np * 3 * m;
   SFX_CHECK("v145 store", o + 2 * a.np, a.n_out);

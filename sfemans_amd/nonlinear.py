@@ -158,11 +158,22 @@ def rhs_3x3(mesh: Mesh, mode, rhs_in, vb_145, vb_236):
     return vb_145, vb_236
 
 
-def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0):
-    """All modes on the device: rhs_gauss(G,6,m) -> vb_145(np,3,m), vb_236(np,3,m)."""
+def _work_elem(mesh: Mesh, M, work_elem):
+    if work_elem is None:
+        import torch
+        work_elem = torch.empty(36 * mesh.me * M, dtype=torch.float64, device="cuda")
+    return work_elem
+
+
+def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0, work_elem=None):
+    """All modes on the device: rhs_gauss(G,6,m) -> vb_145(np,3,m), vb_236(np,3,m).
+
+    ``work_elem``: (36*me*m) float64 CUDA buffer (allocated when omitted).
+    """
     M = fdims(rhs_gauss)[2]
+    work_elem = _work_elem(mesh, M, work_elem)
     check(lib().sfx_rhs_3x3_dev(mesh.handle, M, ptr(rhs_gauss), ptr(vb_145), ptr(vb_236), float(scale),
-                                stream_ptr()))
+                                work_elem.data_ptr(), stream_ptr()))
     return vb_145, vb_236
 
 
@@ -180,12 +191,13 @@ def qs_00_gauss(mesh: Mesh, heat_capa, ff, ff_gauss, vect):
     return vect
 
 
-def nl_rhs_mhd(mesh: Mesh, list_mode, u, H, B, coeff_lorentz, work_gauss, vb_145, vb_236):
+def nl_rhs_mhd(mesh: Mesh, list_mode, u, H, B, coeff_lorentz, work_gauss, vb_145, vb_236, work_elem=None):
     """Nonlinear share of the NS right-hand sides of an MHD step, all modes (device)."""
     M = len(list_mode)
     lm = mesh.list_mode_device(list_mode)
+    work_elem = _work_elem(mesh, M, work_elem)
     check(lib().sfx_nl_rhs_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(u), ptr(H), ptr(B), float(coeff_lorentz),
-                                   ptr(work_gauss), ptr(vb_145), ptr(vb_236), stream_ptr()))
+                                   ptr(work_gauss), work_elem.data_ptr(), ptr(vb_145), ptr(vb_236), stream_ptr()))
     return vb_145, vb_236
 
 

@@ -181,9 +181,11 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
   a.adj_ptr = ptr.data(); a.adj_ent = ent.data(); a.wgt = wgt.data(); a.tab = tab;
   a.jj = jj; a.me = me; a.np = np; a.G = 7 * me; a.M = M; a.scale = scale; a.n_adj = (int64_t)ent.size();
   if (which == 0) {
-    a.rhs = rhs; a.v145 = v145; a.v236 = v236;
-    a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = np * 3 * (int64_t)M;
-    for (int64_t g = 0; g < np * M; ++g) rhs3x3_body(a, g);
+    std::vector<double> elo((size_t)36 * me * M + 1);
+    a.rhs = rhs; a.v145 = v145; a.v236 = v236; a.elo = elo.data();
+    a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = np * 3 * (int64_t)M; a.n_elo = 36 * me * (int64_t)M;
+    for (int64_t g = 0; g < me * M; ++g) rhs_elem_body(a, g);
+    for (int64_t g = 0; g < np * M; ++g) rhs_gather_body(a, g);
   } else {
     a.heat_capa = hc; a.ff = ff; a.ffg = ffg; a.vect = vect;
     a.n_ff = np * (int64_t)M; a.n_ffg = a.G * (int64_t)M; a.n_out = np * (int64_t)M;
