@@ -28,6 +28,9 @@ coefficients, mode-0 sine set to zero):
   0.1) on a 50k-triangle mesh (158x316), m_max = 32.
 * ``batch``: 32 independent problems (2k-50k triangles, m_max in {4..128})
   in one persistent launch replayed from a hipGraph, schedule autotuned.
+* ``full``: the explicit nonlinear step at m_max = 128 on the 224x448 mesh:
+  (curl u) x u -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and
+  (rho u) (x) u with a level-set density, and the convective adjoint.
 """
 from __future__ import annotations
 
@@ -221,7 +224,83 @@ def wl_batch(torch, args):
     return phases, units, cfg, ("batch", None, None)
 
 
-WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch}
+def _level_set_density(torch, rr, M, x0=0.3, radius=0.25, eps=0.05):
+    """rho = 1 + 0.5 tanh(phi/eps), phi = |x - x0| - radius for a sphere centred
+    at (x0, 0, 0), sampled on 2M azimuthal points per node and transformed once
+    into (cos, sin) coefficients of modes 0..M-1 (np, 2, M)."""
+    r = torch.tensor(rr[0], device="cuda", dtype=torch.float64)[:, None]
+    z = torch.tensor(rr[1], device="cuda", dtype=torch.float64)[:, None]
+    n = 2 * M
+    th = torch.arange(n, device="cuda", dtype=torch.float64)[None, :] * (2 * np.pi / n)
+    out = torch.empty((M, 2, r.shape[0]), device="cuda", dtype=torch.float64)
+    for i0 in range(0, r.shape[0], 65536):
+        rs, zs = r[i0:i0 + 65536], z[i0:i0 + 65536]
+        phi = torch.sqrt((rs * torch.cos(th) - x0) ** 2 + (rs * torch.sin(th)) ** 2 + zs ** 2) - radius
+        rho = 1.0 + 0.5 * torch.tanh(phi / eps)
+        c = torch.fft.rfft(rho, dim=1)[:, :M] / n      # c_m, rho = sum c_m e^{i m theta}
+        a = 2.0 * c.real
+        b = -2.0 * c.imag
+        a[:, 0] = c[:, 0].real
+        b[:, 0] = 0.0
+        out[:, 0, i0:i0 + 65536] = a.T
+        out[:, 1, i0:i0 + 65536] = b.T
+    return out.permute(2, 1, 0)
+
+
+def wl_full(torch, args):
+    """Full explicit nonlinear step, forward and adjoint, 200,704 triangles, m_max = 128."""
+    from sfemans_amd import fft_parallele as FP
+    from sfemans_amd import nonlinear as NL
+    from sfemans_amd.mesh import rectangle_p2
+    m = rectangle_p2(224, 448)
+    mesh = NL.Mesh.from_p2(m)
+    M = 128
+    modes = list(range(M))
+    G, npn, me = m.n_gauss, m.np, m.me
+    u, B = _fields(torch, npn, M, 4.0, (SEED + 5, SEED + 6))
+    T, = _fields(torch, npn, M, 4.0, (SEED + 7,), ncol=2)
+    rho = _level_set_density(torch, m.rr, M)
+    ybar, = _fields(torch, G, M, 4.0, (SEED + 8,))
+    hc = torch.ones(me, dtype=torch.float64, device="cuda")
+    rotv = FP.fortran_empty((G, 6, M))
+    v145, v236 = FP.fortran_empty((npn, 3, M)), FP.fortran_empty((npn, 3, M))
+    welem = torch.empty(36 * me * M, dtype=torch.float64, device="cuda")
+    uxb = FP.fortran_empty((npn, 6, M))
+    cg = FP.fortran_empty((G, 2, M))
+    ff0 = torch.zeros((2 * M, npn), dtype=torch.float64, device="cuda").T
+    vT = FP.fortran_empty((npn, 2 * M))
+    rhou = FP.fortran_empty((npn, 6, M))
+    mten = torch.empty((M, 6, npn, 3), dtype=torch.float64, device="cuda").permute(3, 2, 1, 0)
+    dbar = FP.fortran_empty((npn, 6, M))
+    nod6, gau6 = npn * 6 * M * 8, G * 6 * M * 8
+    phases = [
+        Phase("cross_curl", lambda: NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, rotv),
+              nod6 + gau6, G * fft_flops_per_point(M, 9, 3, 9), "gauss_kernel<128,CROSS_CURL>"),
+        Phase("rhs_3x3", lambda: NL.rhs_3x3_all_modes(mesh, rotv, v145, v236, -1.0, welem),
+              gau6 + 2 * 36 * me * M * 8 + nod6, G * M * 72, "rhs_elem_kernel + rhs_gather_kernel"),
+        Phase("u_x_B_nodes", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb), 3 * nod6,
+              npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<128,CROSS>"),
+        Phase("u_grad_T", lambda: NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, u, T, cg),
+              nod6 + nod6 // 3 + G * 2 * M * 8, G * fft_flops_per_point(M, 12, 3, 8), "gauss_kernel<128,UGRADC>"),
+        Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, cg.reshape(G, 2 * M), vT),
+              2 * npn * 2 * M * 8 + G * 2 * M * 8, G * 2 * M * 30, "qs00_kernel"),
+        Phase("rho_u", lambda: FP.FFT_SCALAR_VECT_DCL(None, u, rho, rhou, 1), 2 * nod6 + nod6 // 3,
+              npn * fft_flops_per_point(M, 6, 3, 3), "fftprod_kernel<128,SVECT1>"),
+        Phase("m_tensor_u", lambda: FP.FFT_TENSOR_DCL(None, rhou, u, mten), 5 * nod6,
+              npn * fft_flops_per_point(M, 9, 9, 9), "fftprod_kernel<128,TENSOR>"),
+        Phase("adjoint", lambda: NL.nl_adjoint(mesh, modes, u, ybar, dbar, welem),
+              gau6 + 2 * nod6 + 2 * 36 * me * M * 8, 2 * G * fft_flops_per_point(M, 9, 3, 9),
+              "adjoint_kernel<128> + adjoint_gather_kernel"),
+    ]
+    cfg = {"workload": "full explicit nonlinear step, forward + adjoint, 224x448 P2 mesh (200,704 triangles), "
+                       "m_max=128: (curl u)xu -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and (rho u) (x) u "
+                       "with rho = 1 + 0.5 tanh(phi/0.05) of a sphere, adjoint of the convective term",
+           "me": me, "np": npn, "gauss_points": G, "m_max": M, "grid": 3 * mp_of(M)}
+    return phases, G * M, cfg, ("full", m, M)
+
+
+WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch,
+             "full": wl_full}
 
 
 # ------------------------------------------------------------- CPU baseline
@@ -231,7 +310,7 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
     from oracle import smb
     from sfemans_amd.synth import fourier_field
     t_end = time.perf_counter() + sample_s
-    if kind in ("nl", "mhd", "tangent", "batch") and mesh is not None:
+    if kind in ("nl", "mhd", "tangent", "batch", "full") and mesh is not None:
         # forward nonlinear term on a slab of the mesh's first elements
         ne = 512
         jj = np.asfortranarray(mesh.jj[:, :ne])

@@ -182,6 +182,18 @@ int sfx_nl_adjoint_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
 int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
                            const double* H0, const double* B0, const double* dU, const double* dH,
                            const double* dB, double coeff_lorentz, double* out, void* stream);
+/* Exact transposes of the Lorentz part and of the whole of sfx_nl_tangent_mhd_dev
+ * (no counterpart in the reference, whose LightKrylov operator uses the
+ * forward map for rmatvec: SOURCES/LIGHTKRYLOV_OBJECTS/exponential_propagator.F90:61-62).
+ * For ybar(G,6,m): dHbar, dBbar (np,6,m) of -coeff_lorentz [B(H0,dB) + B(dH,B0)];
+ * the MHD form adds dUbar of the convective part.  work_elem holds
+ * 2*36*me*m doubles. */
+int sfx_nl_adjoint_lorentz_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* H0,
+                               const double* B0, double coeff_lorentz, const double* ybar, double* work_elem,
+                               double* dHbar, double* dBbar, void* stream);
+int sfx_nl_adjoint_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                           const double* H0, const double* B0, double coeff_lorentz, const double* ybar,
+                           double* work_elem, double* dUbar, double* dHbar, double* dBbar, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Right-hand-side assembly into dense local vectors.  On one rank the PETSc

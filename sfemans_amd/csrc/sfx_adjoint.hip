@@ -3,42 +3,53 @@
 
 namespace sfx {
 
-template <int MP>
+template <int MP, bool SPLIT>
 __global__ void __launch_bounds__(64 * GAUSS_WAVES) adjoint_kernel(AdjArgs a) {
   extern __shared__ double2 smem[];
-  adjoint_body<MP>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+  adjoint_body<MP, SPLIT>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }
 
 __global__ void __launch_bounds__(256) adjoint_gather_kernel(GatherArgs a) {
   adjoint_gather_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
-template <int MP>
+template <int MP, bool SPLIT>
 static hipError_t launch_mp(const AdjArgs& a, hipStream_t st) {
   constexpr int E = GaussLds<MP>::E;
   const int64_t nblk = (a.g.me + E - 1) / E;
   if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)AdjLds<MP>::TOTAL * sizeof(double2);
-  hipLaunchKernelGGL((adjoint_kernel<MP>), dim3((unsigned)nblk), dim3(64 * GAUSS_WAVES), shmem, st, a);
+  const size_t shmem = (size_t)AdjLds<MP, SPLIT>::TOTAL * sizeof(double2);
+  hipLaunchKernelGGL((adjoint_kernel<MP, SPLIT>), dim3((unsigned)nblk), dim3(64 * GAUSS_WAVES), shmem, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, hipStream_t st) {
-  hipError_t e = hipErrorInvalidValue;
+template <bool SPLIT>
+static hipError_t launch_split(const AdjArgs& a, hipStream_t st) {
   switch (a.g.MP) {
-    case 4: e = launch_mp<4>(a, st); break;
-    case 8: e = launch_mp<8>(a, st); break;
-    case 16: e = launch_mp<16>(a, st); break;
-    case 32: e = launch_mp<32>(a, st); break;
-    case 64: e = launch_mp<64>(a, st); break;
-    case 128: e = launch_mp<128>(a, st); break;
-    case 256: e = launch_mp<256>(a, st); break;
+    case 4: return launch_mp<4, SPLIT>(a, st);
+    case 8: return launch_mp<8, SPLIT>(a, st);
+    case 16: return launch_mp<16, SPLIT>(a, st);
+    case 32: return launch_mp<32, SPLIT>(a, st);
+    case 64: return launch_mp<64, SPLIT>(a, st);
+    case 128: return launch_mp<128, SPLIT>(a, st);
+    case 256: return launch_mp<256, SPLIT>(a, st);
   }
-  if (e != hipSuccess) return e;
+  return hipErrorInvalidValue;
+}
+
+static hipError_t launch_gather(const GatherArgs& ga, hipStream_t st) {
   const int64_t n = ga.np * ga.M;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(adjoint_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ga);
   return hipGetLastError();
+}
+
+// Element kernel, then one gather per output (two in the split form).
+hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, const GatherArgs* ga2, hipStream_t st) {
+  hipError_t e = ga2 ? launch_split<true>(a, st) : launch_split<false>(a, st);
+  if (e != hipSuccess) return e;
+  if ((e = launch_gather(ga, st)) != hipSuccess) return e;
+  return ga2 ? launch_gather(*ga2, st) : hipSuccess;
 }
 
 }  // namespace sfx

@@ -20,19 +20,28 @@
 namespace sfx {
 
 struct AdjArgs {
-  GaussArgs g;              // mesh, base flow U (g.U), list_mode, twiddles, tables
+  GaussArgs g;              // mesh, base field A (g.U: its curl enters), list_mode, twiddles, tables
+  const double* UB;         // split form: base field B (its values enter), (np, 6, M)
+  int64_t nUB;
+  double coef;              // multiplies both products
   const double* ybar;       // (G, 6, M)
   int64_t nybar;
-  double* elo;              // (6 me, 6, M) per-element nodal contributions
+  double* elo;              // (6 me, 6, M) per-element nodal contributions (W^T part in the split form)
+  double* elo2;             // split form: R^T part
   int64_t nelo;
 };
 
-template <int MP>
+// Doubles per (Gauss point, mode) in the element reduction: value, d/dx and
+// d/dy weights of the six components; the split form keeps the value
+// weights of the two outputs apart.
+template <bool SPLIT> struct AdjRed { static constexpr int K = SPLIT ? 24 : 18; };
+
+template <int MP, bool SPLIT = false>
 struct AdjLds {
   using LY = GaussLds<MP>;
   // per-wave region: 3 exchanged signals (3 x 256 complex) = 1536 doubles
   static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
-  static constexpr int RED = 7 * LY::E * (MP / 4) * 18;    // doubles: points x modes-in-slot x 18
+  static constexpr int RED = 7 * LY::E * (MP / 4) * AdjRed<SPLIT>::K;   // points x modes-in-slot x K
   static constexpr int BIGD = (GAUSS_WAVES * RW > 2 * LY::NOD6) ? GAUSS_WAVES * RW : 2 * LY::NOD6;
   static constexpr int BIGD2 = BIGD > RED ? BIGD : RED;
   static constexpr int TOTAL = LY::BIG + (BIGD2 + 1) / 2;   // double2 units
@@ -81,11 +90,17 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (
   }
 }
 
-template <int MP>
+// SPLIT = false: transpose of T_U d = P[(curl U) x W d + (curl d) x U]
+// (A = B = U, both parts summed into one output).
+// SPLIT = true: transpose, times coef, of the two Lorentz passes
+// (curl A) x W dB and (curl dH) x B: the W^T part (pbar x curl A) is the
+// adjoint for dB (elo), the R^T part (B x pbar) the adjoint for dH (elo2).
+template <int MP, bool SPLIT>
 SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
   using LY = GaussLds<MP>;
-  using AL = AdjLds<MP>;
+  using AL = AdjLds<MP, SPLIT>;
+  constexpr int KR = AdjRed<SPLIT>::K;
   const GaussArgs& a = A.g;
   constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N;
   constexpr int GX = 3;
@@ -160,7 +175,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       }
   }
 
-  // ---- base flow on the grid: (curl U, U), then the two products
+  // ---- base fields on the grid: (curl A, B), then the two products
   double2 q[9][4];
   {
     double2 Z[3][4], B[3][4];
@@ -168,18 +183,24 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     for (int s = 0; s < 3; ++s)
 #pragma unroll
       for (int r = 0; r < 4; ++r) { Z[s][r] = make_double2(0, 0); B[s][r] = make_double2(0, 0); }
-    NodalFetch<6, MP> fa;
+    NodalFetch<6, MP> fa, fb;
     nodal_fetch<6, MP>(a, a.U, a.nU, e0, 0, tid, nid, fa);
+    if (SPLIT) nodal_fetch<6, MP>(a, A.UB, A.nUB, e0, 0, tid, nid, fb);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       SFX_SYNC();
       nodal_store<6, MP>(fa, tid, big, LY::ROW6);
+      if (SPLIT) nodal_store<6, MP>(fb, tid, big + LY::NOD6, LY::ROW6);
       SFX_SYNC();
-      if (c < 3) nodal_fetch<6, MP>(a, a.U, a.nU, e0, c + 1, tid, nid, fa);
+      if (c < 3) {
+        nodal_fetch<6, MP>(a, a.U, a.nU, e0, c + 1, tid, nid, fa);
+        if (SPLIT) nodal_fetch<6, MP>(a, A.UB, A.nUB, e0, c + 1, tid, nid, fb);
+      }
       const int m = c * L + l;
       double own[12];
       if (m < a.M && valid) {
-        pass_fields<GV_CROSS_CURL, MP>(a, big, nullptr, 6 * el, l, lmode[m], g, 1.0, own);
+        if (SPLIT) pass_fields<GV_PAIRS, MP>(a, big, big + LY::NOD6, 6 * el, l, lmode[m], g, 1.0, own);
+        else pass_fields<GV_CROSS_CURL, MP>(a, big, nullptr, 6 * el, l, lmode[m], g, 1.0, own);
       } else {
 #pragma unroll
         for (int f = 0; f < 12; ++f) own[f] = 0.0;
@@ -207,7 +228,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       for (int r = 0; r < 4; ++r) {
         const double r0 = y[j1][r].x, r1 = y[j1][r].y, r2 = y[3 + j1][r].x;
         const double w0 = y[3 + j1][r].y, w1 = y[6 + j1][r].x, w2 = y[6 + j1][r].y;
-        const double p0 = pb[j1][r][0], p1 = pb[j1][r][1], p2 = pb[j1][r][2];
+        const double p0 = A.coef * pb[j1][r][0], p1 = A.coef * pb[j1][r][1], p2 = A.coef * pb[j1][r][2];
         const double wb0 = p1 * r2 - p2 * r1, wb1 = p2 * r0 - p0 * r2, wb2 = p0 * r1 - p1 * r0;
         const double rb0 = w1 * p2 - w2 * p1, rb1 = w2 * p0 - w0 * p2, rb2 = w0 * p1 - w1 * p0;
         q[j1][r] = make_double2(wb0, wb1);
@@ -242,6 +263,13 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     Rb[4] = sa * oa[2][1][r]; Rb[5] = sb * ob[2][1][r];
     const double mr = lmode[m < MP ? m : 0] * g.iray, ir = g.iray;
     double Dr[6] = {0, 0, 0, 0, 0, 0}, Dz[6] = {0, 0, 0, 0, 0, 0};
+    double Ww[6];   // value weights of the W^T part alone (split form)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Ww[k] = Wb[k];
+    if (SPLIT) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) Wb[k] = 0.0;
+    }
     // transpose of curl6 (subroutine_ns_with_velocity.F90:594-608)
     Wb[5] += mr * Rb[0]; Dz[2] -= Rb[0];
     Wb[4] -= mr * Rb[1]; Dz[3] -= Rb[1];
@@ -249,15 +277,22 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     Dz[1] += Rb[3];      Dr[5] -= Rb[3];
     Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
     Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
-    double* dst = red + ((size_t)lp * L + l) * 18;
+    double* dst = red + ((size_t)lp * L + l) * KR;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const double xb = fma(g.a11, Dr[k], g.a21 * Dz[k]);
       const double yb = fma(g.a12, Dr[k], g.a22 * Dz[k]);
-      SFX_CHECK("red write", ((size_t)lp * L + l) * 18 + 3 * k + 2, AL::RED);
-      dst[3 * k] = ok ? Wb[k] : 0.0;
-      dst[3 * k + 1] = ok ? xb : 0.0;
-      dst[3 * k + 2] = ok ? yb : 0.0;
+      SFX_CHECK("red write", ((size_t)lp * L + l) * KR + KR - 1, AL::RED);
+      if (SPLIT) {
+        dst[4 * k] = ok ? Ww[k] : 0.0;
+        dst[4 * k + 1] = ok ? Wb[k] : 0.0;
+        dst[4 * k + 2] = ok ? xb : 0.0;
+        dst[4 * k + 3] = ok ? yb : 0.0;
+      } else {
+        dst[3 * k] = ok ? Wb[k] : 0.0;
+        dst[3 * k + 1] = ok ? xb : 0.0;
+        dst[3 * k + 2] = ok ? yb : 0.0;
+      }
     }
     SFX_SYNC();
     // work item (element, local node n, mode offset mi): six components;
@@ -269,18 +304,28 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       const int mm = mi + L * r;
       const int64_t eg = e0 + e;
       if (eg >= a.me || mm >= a.M) continue;
-      double acc[6] = {0, 0, 0, 0, 0, 0};
+      double acc[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
       for (int lg2 = 0; lg2 < 7; ++lg2) {
-        const double* src = red + ((size_t)(7 * e + lg2) * L + mi) * 18;
+        const double* src = red + ((size_t)(7 * e + lg2) * L + mi) * KR;
         const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
+        if (SPLIT) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) acc[k] = fma(w, src[3 * k], fma(dx, src[3 * k + 1], fma(dy, src[3 * k + 2], acc[k])));
+          for (int k = 0; k < 6; ++k) {
+            acc[k] = fma(w, src[4 * k], acc[k]);
+            acc2[k] = fma(w, src[4 * k + 1], fma(dx, src[4 * k + 2], fma(dy, src[4 * k + 3], acc2[k])));
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            acc[k] = fma(w, src[3 * k], fma(dx, src[3 * k + 1], fma(dy, src[3 * k + 2], acc[k])));
+        }
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const int64_t o = (6 * eg + n) + 6 * a.me * (k + 6 * (int64_t)mm);
         SFX_CHECK("elo store", o, A.nelo);
         A.elo[o] = acc[k];
+        if (SPLIT) A.elo2[o] = acc2[k];
       }
     }
     SFX_SYNC();

@@ -3,7 +3,7 @@
 
 namespace sfx {
 
-constexpr int FFTPROD_WPB = 4;   // waves per work-group
+constexpr int FFTPROD_WPB = 2;   // waves per work-group (measured: 2 beats 1 and 4 at MP = 64)
 
 template <int MP, int OP>
 __global__ void __launch_bounds__(64 * FFTPROD_WPB) fftprod_kernel(FftProdArgs a) {

@@ -303,33 +303,79 @@ int sfx_nl_tangent_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
                        6, nullptr, (hipStream_t)stream);
 }
 
-int sfx_nl_adjoint_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
-                       const double* ybar, double* work_elem, double* dbar, void* stream) {
-  clear_err();
-  const char* nm = "nl_adjoint";
+}  // extern "C"
+
+namespace sfx {
+
+// Transpose of the linearized pass pair (curl A) x W d + (curl d) x B, times
+// coef, applied to ybar.  B == nullptr: A = B and both parts go to out_w
+// (the convective adjoint).  Otherwise the W^T part goes to out_w and the
+// R^T part to out_r (the Lorentz adjoint for dB and dH).  work_elem holds
+// 36*me*M doubles per output.
+static int adjoint_dev(const char* nm, sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                       const double* Afield, const double* Bfield, double coef, const double* ybar,
+                       double* work_elem, double* out_w, double* out_r, hipStream_t st) {
   if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
   if (m_max_c <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
   if (m_max_c > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, m_max_c);
+  if (!work_elem) return fail(SFX_ERR_ARG, "%s: null work_elem", nm);
+  const bool split = Bfield != nullptr;
   AdjArgs A;
   std::memset(&A, 0, sizeof(A));
   GaussArgs& a = A.g;
   a.jj = mesh->jj; a.rr = mesh->rr; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me;
-  a.U = U; a.nU = mesh->np * 6 * (int64_t)m_max_c;
+  a.U = Afield; a.nU = mesh->np * 6 * (int64_t)m_max_c;
   a.list_mode = list_mode_dev; a.tab = mesh->tab;
   a.M = m_max_c; a.MP = mp_for(m_max_c);
   Twiddles tw;
   int rc = get_twiddles(a.MP, &tw);
   if (rc) return rc;
   a.wN = tw.wN; a.wMP = tw.wMP;
+  A.UB = Bfield; A.nUB = split ? a.nU : 0;
+  A.coef = coef;
   A.ybar = ybar; A.nybar = a.G * 6 * (int64_t)m_max_c;
-  A.elo = work_elem; A.nelo = 6 * mesh->me * 6 * (int64_t)m_max_c;
+  A.nelo = 6 * mesh->me * 6 * (int64_t)m_max_c;
+  A.elo = work_elem;
+  A.elo2 = split ? work_elem + A.nelo : nullptr;
   GatherArgs ga;
-  ga.adj_ptr = mesh->adj_ptr; ga.adj_ent = mesh->adj_ent; ga.elo = work_elem; ga.out = dbar;
+  ga.adj_ptr = mesh->adj_ptr; ga.adj_ent = mesh->adj_ent; ga.elo = A.elo; ga.out = out_w;
   ga.me = mesh->me; ga.np = mesh->np; ga.n_adj = mesh->n_adj; ga.nelo = A.nelo;
   ga.nout = mesh->np * 6 * (int64_t)m_max_c; ga.M = m_max_c;
+  GatherArgs ga2 = ga;
+  ga2.elo = A.elo2; ga2.out = out_r;
   if (mesh->me == 0) return SFX_OK;
-  hipError_t e = launch_adjoint(A, ga, (hipStream_t)stream);
+  hipError_t e = launch_adjoint(A, ga, split ? &ga2 : nullptr, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
+}  // namespace sfx
+
+extern "C" {
+
+int sfx_nl_adjoint_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                       const double* ybar, double* work_elem, double* dbar, void* stream) {
+  clear_err();
+  return adjoint_dev("nl_adjoint", mesh, list_mode_dev, m_max_c, U, nullptr, 1.0, ybar, work_elem, dbar, nullptr,
+                     (hipStream_t)stream);
+}
+
+int sfx_nl_adjoint_lorentz_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* H0,
+                               const double* B0, double coeff_lorentz, const double* ybar, double* work_elem,
+                               double* dHbar, double* dBbar, void* stream) {
+  clear_err();
+  if (!H0 || !B0) return fail(SFX_ERR_ARG, "nl_adjoint_lorentz: null base field");
+  return adjoint_dev("nl_adjoint_lorentz", mesh, list_mode_dev, m_max_c, H0, B0, -coeff_lorentz, ybar, work_elem,
+                     dBbar, dHbar, (hipStream_t)stream);
+}
+
+int sfx_nl_adjoint_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                           const double* H0, const double* B0, double coeff_lorentz, const double* ybar,
+                           double* work_elem, double* dUbar, double* dHbar, double* dBbar, void* stream) {
+  clear_err();
+  int rc = sfx_nl_adjoint_dev(mesh, list_mode_dev, m_max_c, U, ybar, work_elem, dUbar, stream);
+  if (rc) return rc;
+  return sfx_nl_adjoint_lorentz_dev(mesh, list_mode_dev, m_max_c, H0, B0, coeff_lorentz, ybar, work_elem, dHbar,
+                                    dBbar, stream);
 }
 
 int sfx_nl_adjoint(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, const double* U, const double* ybar,

@@ -20,7 +20,8 @@ from .fft_parallele import fdims, is_device, ptr, stream_ptr
 
 __all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "smb_CurlH_cross_B_gauss_sft_par",
            "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes",
-           "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd", "nl_adjoint"]
+           "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd", "nl_adjoint", "nl_adjoint_lorentz",
+           "nl_adjoint_mhd"]
 
 
 class Mesh:
@@ -242,3 +243,31 @@ def nl_adjoint(mesh: Mesh, list_mode, U, ybar, dbar, work_elem=None):
         lm = _lm(list_mode)
         check(L.sfx_nl_adjoint(mesh.handle, lm.ctypes.data, M, ptr(U), ptr(ybar), ptr(dbar)))
     return dbar
+
+
+def nl_adjoint_lorentz(mesh: Mesh, list_mode, H0, B0, coeff_lorentz, ybar, dHbar, dBbar, work_elem=None):
+    """Transpose of -coeff_lorentz [(curl H0) x dB + (curl dH) x B0] (device arrays).
+
+    ``work_elem``: (2*36*me*m) float64 CUDA buffer (allocated when omitted).
+    """
+    import torch
+    M = len(list_mode)
+    if work_elem is None:
+        work_elem = torch.empty(2 * 36 * mesh.me * M, dtype=torch.float64, device="cuda")
+    lm = mesh.list_mode_device(list_mode)
+    check(lib().sfx_nl_adjoint_lorentz_dev(mesh.handle, lm.data_ptr(), M, ptr(H0), ptr(B0), float(coeff_lorentz),
+                                           ptr(ybar), work_elem.data_ptr(), ptr(dHbar), ptr(dBbar), stream_ptr()))
+    return dHbar, dBbar
+
+
+def nl_adjoint_mhd(mesh: Mesh, list_mode, U, H0, B0, coeff_lorentz, ybar, dUbar, dHbar, dBbar, work_elem=None):
+    """Transpose of :func:`nl_tangent_mhd` around (U, H0, B0) (device arrays)."""
+    import torch
+    M = len(list_mode)
+    if work_elem is None:
+        work_elem = torch.empty(2 * 36 * mesh.me * M, dtype=torch.float64, device="cuda")
+    lm = mesh.list_mode_device(list_mode)
+    check(lib().sfx_nl_adjoint_mhd_dev(mesh.handle, lm.data_ptr(), M, ptr(U), ptr(H0), ptr(B0),
+                                       float(coeff_lorentz), ptr(ybar), work_elem.data_ptr(), ptr(dUbar),
+                                       ptr(dHbar), ptr(dBbar), stream_ptr()))
+    return dUbar, dHbar, dBbar

@@ -197,16 +197,34 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
 // ---- adjoint of the tangent
 #include "../../sfemans_amd/csrc/sfx_adjoint_body.h"
 
-template <int MP>
+template <int MP, bool SPLIT>
 static void emu_adjoint_mp(const AdjArgs& A) {
   constexpr int E = GaussLds<MP>::E;
   const int64_t nblk = (A.g.me + E - 1) / E;
-  run_grid(nblk, 64 * GAUSS_WAVES, AdjLds<MP>::TOTAL,
-           [&](int t, int64_t b, double2* s) { adjoint_body<MP>(A, t, b, s); });
+  run_grid(nblk, 64 * GAUSS_WAVES, AdjLds<MP, SPLIT>::TOTAL,
+           [&](int t, int64_t b, double2* s) { adjoint_body<MP, SPLIT>(A, t, b, s); });
 }
 
+template <bool SPLIT>
+static void emu_adjoint_split(const AdjArgs& A) {
+  switch (A.g.MP) {
+    case 4: emu_adjoint_mp<4, SPLIT>(A); break;
+    case 8: emu_adjoint_mp<8, SPLIT>(A); break;
+    case 16: emu_adjoint_mp<16, SPLIT>(A); break;
+    case 32: emu_adjoint_mp<32, SPLIT>(A); break;
+    case 64: emu_adjoint_mp<64, SPLIT>(A); break;
+    case 128: emu_adjoint_mp<128, SPLIT>(A); break;
+    case 256: emu_adjoint_mp<256, SPLIT>(A); break;
+  }
+}
+
+// UB == nullptr: convective adjoint around U into dbar.  Otherwise the split
+// (Lorentz) form around (A = U, B = UB) times coef: W^T part into dbar, R^T
+// part into dbar2.
 extern "C" int emu_adjoint(const int32_t* jj, const double* rr, int64_t me, int64_t np, const double* U,
-                           const double* ybar, const int32_t* list_mode, int M, double* dbar) {
+                           const double* ybar, const int32_t* list_mode, int M, double* dbar, const double* UB,
+                           double coef, double* dbar2) {
+  const bool split = UB != nullptr;
   AdjArgs A;
   std::memset(&A, 0, sizeof(A));
   GaussArgs& a = A.g;
@@ -220,18 +238,12 @@ extern "C" int emu_adjoint(const int32_t* jj, const double* rr, int64_t me, int6
   std::vector<double2> wN, wMP;
   host_twiddles(a.MP, wN, wMP);
   a.wN = wN.data(); a.wMP = wMP.data();
-  std::vector<double> elo((size_t)6 * me * 6 * M, -1e300);
+  std::vector<double> elo((size_t)6 * me * 6 * M, -1e300), elo2(split ? elo.size() : 1, -1e300);
+  A.UB = UB; A.nUB = split ? a.nU : 0; A.coef = split ? coef : 1.0;
   A.ybar = ybar; A.nybar = a.G * 6 * (int64_t)M;
-  A.elo = elo.data(); A.nelo = (int64_t)elo.size();
-  switch (a.MP) {
-    case 4: emu_adjoint_mp<4>(A); break;
-    case 8: emu_adjoint_mp<8>(A); break;
-    case 16: emu_adjoint_mp<16>(A); break;
-    case 32: emu_adjoint_mp<32>(A); break;
-    case 64: emu_adjoint_mp<64>(A); break;
-    case 128: emu_adjoint_mp<128>(A); break;
-    case 256: emu_adjoint_mp<256>(A); break;
-  }
+  A.elo = elo.data(); A.elo2 = split ? elo2.data() : nullptr; A.nelo = (int64_t)elo.size();
+  if (split) emu_adjoint_split<true>(A);
+  else emu_adjoint_split<false>(A);
   std::vector<int64_t> ptr;
   std::vector<int32_t> ent;
   build_adjacency(jj, me, np, ptr, ent);
@@ -240,6 +252,10 @@ extern "C" int emu_adjoint(const int32_t* jj, const double* rr, int64_t me, int6
   ga.me = me; ga.np = np; ga.n_adj = (int64_t)ent.size(); ga.nelo = (int64_t)elo.size();
   ga.nout = np * 6 * (int64_t)M; ga.M = M;
   for (int64_t g = 0; g < np * M; ++g) adjoint_gather_body(ga, g);
+  if (split) {
+    ga.elo = elo2.data(); ga.out = dbar2;
+    for (int64_t g = 0; g < np * M; ++g) adjoint_gather_body(ga, g);
+  }
   return 0;
 }
 

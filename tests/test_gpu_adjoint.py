@@ -75,3 +75,55 @@ def test_dot_product_config4_size():
     lhs = float((TD * Y).sum())
     rhs = float((D * TtY).sum())
     assert abs(lhs - rhs) <= 1e-12 * float(TD.norm()) * float(Y.norm())
+
+
+@pytest.mark.parametrize("nx,ny,M", [(5, 6, 8), (8, 12, 32)])
+def test_mhd_adjoint_dot_product(nx, ny, M):
+    """<T(dU, dH, dB), Y> = <dU, dUbar> + <dH, dHbar> + <dB, dBbar> for the
+    linearized convective + Lorentz terms (tangent and adjoint both on the GPU)."""
+    print(f"case mhd adjoint dot-product {nx}x{ny} M={M}", flush=True)
+    m = rectangle_p2(nx, ny)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    U, H0, B0 = (base_flow(m.np, M, s) for s in (96, 97, 98))
+    dU, dH, dB = (fortran_tensor(fourier_field(m.np, 6, M, s, 2.0)) for s in (99, 100, 101))
+    Ud, Hd, Bd = (fortran_tensor(x) for x in (U, H0, B0))
+    Y = fortran_tensor(np.random.default_rng(102).standard_normal((m.n_gauss, 6, M)))
+    cl = 0.8
+    T = fortran_empty((m.n_gauss, 6, M))
+    NL.nl_tangent_mhd(mesh, modes, Ud, Hd, Bd, dU, dH, dB, cl, T)
+    dUb, dHb, dBb = (fortran_empty((m.np, 6, M)) for _ in range(3))
+    NL.nl_adjoint_mhd(mesh, modes, Ud, Hd, Bd, cl, Y, dUb, dHb, dBb)
+    torch.cuda.synchronize()
+    lhs = float(torch.sum(T * Y))
+    rhs = float(torch.sum(dU * dUb) + torch.sum(dH * dHb) + torch.sum(dB * dBb))
+    assert abs(lhs - rhs) <= 1e-12 * float(torch.linalg.norm(T)) * float(torch.linalg.norm(Y))
+
+
+def test_lorentz_adjoint_vs_oracle_bruteforce():
+    """The Lorentz adjoint equals the transpose of the oracle's bilinear form,
+    column by column, on a tiny mesh."""
+    from oracle import smb
+    m = rectangle_p2(1, 1)
+    M = 2
+    modes = np.arange(M)
+    H0, B0 = fourier_field(m.np, 6, M, 103, 2.0), fourier_field(m.np, 6, M, 104, 2.0)
+    cl = 1.3
+    Y = np.random.default_rng(105).standard_normal((m.n_gauss, 6, M))
+    mesh = NL.Mesh.from_p2(m)
+    dHb, dBb = fortran_empty((m.np, 6, M)), fortran_empty((m.np, 6, M))
+    NL.nl_adjoint_lorentz(mesh, modes, fortran_tensor(H0), fortran_tensor(B0), cl, fortran_tensor(Y), dHb, dBb)
+    torch.cuda.synchronize()
+    n = m.np * 6 * M
+    expH, expB = np.zeros(n), np.zeros(n)
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        E = e.reshape((m.np, 6, M), order="F")
+        expB[j] = np.sum(-cl * smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, H0, E) * Y)
+        expH[j] = np.sum(-cl * smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, E, B0) * Y)
+    expH = expH.reshape((m.np, 6, M), order="F")
+    expB = expB.reshape((m.np, 6, M), order="F")
+    # mode-0 sine columns do not enter the forward map: the adjoint has zeros there
+    assert np.max(np.abs(dHb.cpu().numpy() - expH)) <= 1e-12 * np.max(np.abs(expH))
+    assert np.max(np.abs(dBb.cpu().numpy() - expB)) <= 1e-12 * np.max(np.abs(expB))
