@@ -269,6 +269,7 @@ def wl_full(torch, args):
     cg = FP.fortran_empty((G, 2, M))
     ff0 = torch.zeros((2 * M, npn), dtype=torch.float64, device="cuda").T
     vT = FP.fortran_empty((npn, 2 * M))
+    wq = torch.empty(6 * me * 2 * M, dtype=torch.float64, device="cuda")
     rhou = FP.fortran_empty((npn, 6, M))
     mten = torch.empty((M, 6, npn, 3), dtype=torch.float64, device="cuda").permute(3, 2, 1, 0)
     dbar = FP.fortran_empty((npn, 6, M))
@@ -282,8 +283,9 @@ def wl_full(torch, args):
               npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<128,CROSS>"),
         Phase("u_grad_T", lambda: NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, u, T, cg),
               nod6 + nod6 // 3 + G * 2 * M * 8, G * fft_flops_per_point(M, 12, 3, 8), "gauss_kernel<128,UGRADC>"),
-        Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, cg.reshape(G, 2 * M), vT),
-              2 * npn * 2 * M * 8 + G * 2 * M * 8, G * 2 * M * 30, "qs00_kernel"),
+        Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, cg.reshape(G, 2 * M), vT, wq),
+              2 * npn * 2 * M * 8 + G * 2 * M * 8 + 2 * 6 * me * 2 * M * 8, G * 2 * M * 30,
+              "qs00_elem_kernel + qs00_gather_kernel"),
         Phase("rho_u", lambda: FP.FFT_SCALAR_VECT_DCL(None, u, rho, rhou, 1), 2 * nod6 + nod6 // 3,
               npn * fft_flops_per_point(M, 6, 3, 3), "fftprod_kernel<128,SVECT1>"),
         Phase("m_tensor_u", lambda: FP.FFT_TENSOR_DCL(None, rhou, u, mten), 5 * nod6,

@@ -217,9 +217,10 @@ int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, d
  * heat_capa(me), nodal ff(np) and Gauss ff_gauss(G). */
 int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, const double* ff_gauss,
                     double* vect);
-/* ncols independent right-hand sides: ff(np,ncols), ff_gauss(G,ncols) -> vect(np,ncols). */
+/* ncols independent right-hand sides: ff(np,ncols), ff_gauss(G,ncols) -> vect(np,ncols).
+ * work_elem is a caller-provided device buffer of 6*me*ncols doubles. */
 int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa, const double* ff,
-                        const double* ff_gauss, double* vect, void* stream);
+                        const double* ff_gauss, double* work_elem, double* vect, void* stream);
 
 /* Nonlinear share of the Navier-Stokes right-hand sides of an MHD step,
  * all modes: vb = rhs_3x3(-[(curl u) x u - coeff_lorentz (curl H) x B]).

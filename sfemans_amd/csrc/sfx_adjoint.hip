@@ -10,7 +10,7 @@ __global__ void __launch_bounds__(64 * GAUSS_WAVES) adjoint_kernel(AdjArgs a) {
 }
 
 __global__ void __launch_bounds__(256) adjoint_gather_kernel(GatherArgs a) {
-  adjoint_gather_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+  adjoint_gather_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
 template <int MP, bool SPLIT>

@@ -26,7 +26,8 @@ struct AdjArgs {
   double coef;              // multiplies both products
   const double* ybar;       // (G, 6, M)
   int64_t nybar;
-  double* elo;              // (6 me, 6, M) per-element nodal contributions (W^T part in the split form)
+  double* elo;              // per-element nodal contributions, index ((6e+i)*6 + k)*M + m
+                            // (W^T part in the split form)
   double* elo2;             // split form: R^T part
   int64_t nelo;
 };
@@ -296,11 +297,12 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     }
     SFX_SYNC();
     // work item (element, local node n, mode offset mi): six components;
-    // (element, node) fastest so that consecutive lanes store consecutive entries
+    // mode fastest, so that consecutive lanes store consecutive modes of one
+    // entry (elo layout: entry, component, mode)
     for (int it = tid; it < E * 6 * L; it += 64 * GAUSS_WAVES) {
-      const int n = it % 6;
-      const int e = (it / 6) % E;
-      const int mi = it / (6 * E);
+      const int mi = it % L;
+      const int n = (it / L) % 6;
+      const int e = it / (6 * L);
       const int mm = mi + L * r;
       const int64_t eg = e0 + e;
       if (eg >= a.me || mm >= a.M) continue;
@@ -322,7 +324,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        const int64_t o = (6 * eg + n) + 6 * a.me * (k + 6 * (int64_t)mm);
+        const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
         SFX_CHECK("elo store", o, A.nelo);
         A.elo[o] = acc[k];
         if (SPLIT) A.elo2[o] = acc2[k];
@@ -337,23 +339,25 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
 struct GatherArgs {
   const int64_t* adj_ptr;
   const int32_t* adj_ent;
-  const double* elo;        // (6 me, 6, M)
+  const double* elo;        // index ((6e+i)*6 + k)*M + m
   double* out;              // (np, 6, M)
   int64_t me, np, n_adj, nelo, nout;
   int M;
 };
 
+// One thread per (node, mode), mode fastest: the loads of a wave read
+// consecutive modes of one entry.
 SFX_DEV void adjoint_gather_body(const GatherArgs& a, int64_t gid) {
-  const int64_t n = gid % a.np;
-  const int64_t m = gid / a.np;
-  if (m >= a.M) return;
+  const int64_t m = gid % a.M;
+  const int64_t n = gid / a.M;
+  if (n >= a.np) return;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t q = a.adj_ptr[n]; q < a.adj_ptr[n + 1]; ++q) {
     SFX_CHECK("adj", q, a.n_adj);
     const int64_t ent = a.adj_ent[q];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const int64_t i = ent + 6 * a.me * (k + 6 * m);
+      const int64_t i = (ent * 6 + k) * a.M + m;
       SFX_CHECK("elo load", i, a.nelo);
       acc[k] += a.elo[i];
     }

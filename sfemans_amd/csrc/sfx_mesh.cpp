@@ -470,12 +470,15 @@ int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_
 }
 
 int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa, const double* ff,
-                        const double* ff_gauss, double* vect, void* stream) {
+                        const double* ff_gauss, double* work_elem, double* vect, void* stream) {
   clear_err();
   if (!mesh) return fail(SFX_ERR_ARG, "qs_00_gauss: null mesh");
+  if (ncols < 0) return fail(SFX_ERR_ARG, "qs_00_gauss: ncols < 0");
+  if (!work_elem && mesh->me > 0 && ncols > 0) return fail(SFX_ERR_ARG, "qs_00_gauss: null work_elem");
   RhsArgs a = rhs_args(mesh, ncols, 1.0);
-  a.heat_capa = heat_capa; a.ff = ff; a.ffg = ff_gauss; a.vect = vect;
+  a.heat_capa = heat_capa; a.ff = ff; a.ffg = ff_gauss; a.vect = vect; a.elo = work_elem;
   a.n_ff = a.np * (int64_t)ncols; a.n_ffg = a.G * (int64_t)ncols; a.n_out = a.np * (int64_t)ncols;
+  a.n_elo = 6 * a.me * (int64_t)ncols;
   hipError_t e = launch_qs00(a, (hipStream_t)stream);
   return e == hipSuccess ? SFX_OK : hip_fail(e, "qs_00_gauss");
 }
@@ -485,18 +488,19 @@ int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, 
   clear_err();
   if (!mesh) return fail(SFX_ERR_ARG, "qs_00_gauss: null mesh");
   const size_t nG = (size_t)7 * mesh->me, nN = (size_t)mesh->np;
-  void *dh, *df, *dg, *dv;
+  void *dh, *df, *dg, *dv, *dw;
   int rc;
   if ((rc = staging(0, mesh->me * 8 + 8, &dh)) || (rc = staging(1, nN * 8 + 8, &df)) ||
-      (rc = staging(2, nG * 8 + 8, &dg)) || (rc = staging(3, nN * 8 + 8, &dv)))
+      (rc = staging(2, nG * 8 + 8, &dg)) || (rc = staging(3, nN * 8 + 8, &dv)) ||
+      (rc = staging(4, (size_t)6 * mesh->me * 8 + 8, &dw)))
     return rc;
   hipError_t e;
   if ((e = hipMemcpy(dh, heat_capa, mesh->me * 8, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(df, ff, nN * 8, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(dg, ff_gauss, nG * 8, hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "H2D");
-  if ((rc = sfx_qs_00_gauss_dev(mesh, 1, (const double*)dh, (const double*)df, (const double*)dg, (double*)dv,
-                                nullptr)))
+  if ((rc = sfx_qs_00_gauss_dev(mesh, 1, (const double*)dh, (const double*)df, (const double*)dg, (double*)dw,
+                                (double*)dv, nullptr)))
     return rc;
   if ((e = hipMemcpy(vect, dv, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
   return SFX_OK;

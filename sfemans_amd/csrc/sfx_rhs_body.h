@@ -36,19 +36,20 @@ struct RhsArgs {
   const double* ff;         // (np, M) nodal source
   const double* ffg;        // (G, M) Gauss source
   double* vect;             // (np, M)
-  double* elo;              // (6 me, 6, M) per-element nodal contributions
+  double* elo;              // per-element nodal contributions, index ((6e+i)*6 + k)*M + m
   int64_t n_rhs, n_ff, n_ffg, n_out, n_adj, n_elo;
 };
 
-// Element share of rhs_3x3, one thread per (element, mode):
+// Element share of rhs_3x3, one thread per (element, mode), mode fastest:
 //   elo(6e+i, k, m) = sum_l ww(i,l) wgt(e,l) rhs(7e+l, comp k, m)
 // (the values the reference adds with VecSetValues, assembling_rhs.F90:73-95).
-// rhs is read once, in element order; rhs_gather_body then sums the entries
-// of each node in element order.
+// A thread reads the 7 consecutive Gauss values of its element per
+// component; the stores of a wave cover consecutive modes.  rhs_gather_body
+// then sums the entries of each node in element order.
 SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
-  const int64_t e = gid % a.me;
-  const int64_t m = gid / a.me;
-  if (m >= a.M) return;
+  const int64_t m = gid % a.M;
+  const int64_t e = gid / a.M;
+  if (e >= a.me) return;
   double w[7], r[7][6];
 #pragma unroll
   for (int l = 0; l < 7; ++l) {
@@ -71,7 +72,7 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const int64_t o = (6 * e + i) + 6 * a.me * (k + 6 * m);
+      const int64_t o = ((6 * e + i) * 6 + k) * a.M + m;
       SFX_CHECK("elo store", o, a.n_elo);
       a.elo[o] = loc[k];
     }
@@ -79,11 +80,12 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
 }
 
 // vb_145 / vb_236 of node n, mode m: the node's element contributions summed
-// in element order, components (1,4,5) and (2,3,6), times scale.
+// in element order, components (1,4,5) and (2,3,6), times scale.  Mode
+// fastest: the loads of a wave read consecutive modes of one entry.
 SFX_DEV void rhs_gather_body(const RhsArgs& a, int64_t gid) {
-  const int64_t n = gid % a.np;
-  const int64_t m = gid / a.np;
-  if (m >= a.M) return;
+  const int64_t m = gid % a.M;
+  const int64_t n = gid / a.M;
+  if (n >= a.np) return;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const int64_t b = a.adj_ptr[n], e_ = a.adj_ptr[n + 1];
   for (int64_t q = b; q < e_; ++q) {
@@ -91,7 +93,7 @@ SFX_DEV void rhs_gather_body(const RhsArgs& a, int64_t gid) {
     const int64_t ent = a.adj_ent[q];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const int64_t i = ent + 6 * a.me * (k + 6 * m);
+      const int64_t i = (ent * 6 + k) * a.M + m;
       SFX_CHECK("elo load", i, a.n_elo);
       acc[k] += a.elo[i];
     }
@@ -106,37 +108,54 @@ SFX_DEV void rhs_gather_body(const RhsArgs& a, int64_t gid) {
   a.v236[o + 2 * a.np] = a.scale * acc[5];
 }
 
-// One thread per (node, column): fl = (hc(e) SUM(ff_loc*ww(:,l)) + ffg(7e+l)) * rj * ray.
-SFX_DEV void qs00_body(const RhsArgs& a, int64_t gid) {
-  const int64_t n = gid % a.np;
-  const int64_t c = gid / a.np;
-  if (c >= a.M) return;
+// Element share of qs_00_gauss, one thread per (element, column), column fastest:
+//   fl(l)       = (hc(e) SUM(ff_loc * ww(:,l)) + ff_gauss(7e+l)) * rj * ray
+//   elo(6e+i,c) = sum_l ww(i,l) fl(l)                    (fem_rhs_axi.F90:958-975)
+SFX_DEV void qs00_elem_body(const RhsArgs& a, int64_t gid) {
+  const int64_t c = gid % a.M;
+  const int64_t e = gid / a.M;
+  if (e >= a.me) return;
+  double ffl[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int64_t fi = (int64_t)a.jj[6 * e + j] - 1 + a.np * c;
+    SFX_CHECK("ff load", fi, a.n_ff);
+    ffl[j] = a.ff[fi];
+  }
+  const double hc = a.heat_capa[e];
+  double fl[7];
+#pragma unroll
+  for (int l = 0; l < 7; ++l) {
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) s = fma(ffl[j], a.tab[j + 6 * l], s);
+    const int64_t gi = 7 * e + l + a.G * c;
+    SFX_CHECK("ffg load", gi, a.n_ffg);
+    fl[l] = (hc * s + a.ffg[gi]) * a.wgt[7 * e + l];
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double loc = 0;
+#pragma unroll
+    for (int l = 0; l < 7; ++l) loc = fma(a.tab[i + 6 * l], fl[l], loc);
+    const int64_t o = (6 * e + i) * a.M + c;
+    SFX_CHECK("elo store", o, a.n_elo);
+    a.elo[o] = loc;
+  }
+}
+
+// vect(n, c): the node's element contributions summed in element order.
+SFX_DEV void qs00_gather_body(const RhsArgs& a, int64_t gid) {
+  const int64_t c = gid % a.M;
+  const int64_t n = gid / a.M;
+  if (n >= a.np) return;
   double acc = 0;
   const int64_t b = a.adj_ptr[n], e_ = a.adj_ptr[n + 1];
   for (int64_t q = b; q < e_; ++q) {
     SFX_CHECK("adj", q, a.n_adj);
-    const int32_t ent = a.adj_ent[q];
-    const int64_t e = ent / 6;
-    const int i = ent % 6;
-    double ffl[6];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const int64_t fi = (int64_t)a.jj[6 * e + j] - 1 + a.np * c;
-      SFX_CHECK("ff load", fi, a.n_ff);
-      ffl[j] = a.ff[fi];
-    }
-    const double hc = a.heat_capa[e];
-    double loc = 0;
-    for (int l = 0; l < 7; ++l) {
-      double s = 0;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) s = fma(ffl[j], a.tab[j + 6 * l], s);
-      const int64_t gi = 7 * e + l + a.G * c;
-      SFX_CHECK("ffg load", gi, a.n_ffg);
-      const double fl = (hc * s + a.ffg[gi]) * a.wgt[7 * e + l];
-      loc = fma(a.tab[i + 6 * l], fl, loc);
-    }
-    acc += loc;
+    const int64_t i = (int64_t)a.adj_ent[q] * a.M + c;
+    SFX_CHECK("elo load", i, a.n_elo);
+    acc += a.elo[i];
   }
   SFX_CHECK("vect store", n + a.np * c, a.n_out);
   a.vect[n + a.np * c] = a.scale * acc;

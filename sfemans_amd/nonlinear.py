@@ -178,12 +178,18 @@ def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0, work_ele
     return vb_145, vb_236
 
 
-def qs_00_gauss(mesh: Mesh, heat_capa, ff, ff_gauss, vect):
-    """Scalar RHS (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) (host) or vect(np,ncols) (device)."""
+def qs_00_gauss(mesh: Mesh, heat_capa, ff, ff_gauss, vect, work_elem=None):
+    """Scalar RHS (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) (host) or vect(np,ncols) (device).
+
+    Device path: ``work_elem`` is a (6*me*ncols) float64 CUDA buffer (allocated when omitted).
+    """
     if is_device(ff):
+        import torch
         nc = ff.shape[1] if ff.dim() > 1 else 1
-        check(lib().sfx_qs_00_gauss_dev(mesh.handle, nc, ptr(heat_capa), ptr(ff), ptr(ff_gauss), ptr(vect),
-                                        stream_ptr()))
+        if work_elem is None:
+            work_elem = torch.empty(6 * mesh.me * nc, dtype=torch.float64, device="cuda")
+        check(lib().sfx_qs_00_gauss_dev(mesh.handle, nc, ptr(heat_capa), ptr(ff), ptr(ff_gauss),
+                                        work_elem.data_ptr(), ptr(vect), stream_ptr()))
     else:
         hc = np.ascontiguousarray(heat_capa, dtype=np.float64)
         ffa = np.ascontiguousarray(ff, dtype=np.float64)

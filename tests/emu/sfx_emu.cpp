@@ -187,9 +187,11 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
     for (int64_t g = 0; g < me * M; ++g) rhs_elem_body(a, g);
     for (int64_t g = 0; g < np * M; ++g) rhs_gather_body(a, g);
   } else {
-    a.heat_capa = hc; a.ff = ff; a.ffg = ffg; a.vect = vect;
-    a.n_ff = np * (int64_t)M; a.n_ffg = a.G * (int64_t)M; a.n_out = np * (int64_t)M;
-    for (int64_t g = 0; g < np * M; ++g) qs00_body(a, g);
+    std::vector<double> elo((size_t)6 * me * M + 1);
+    a.heat_capa = hc; a.ff = ff; a.ffg = ffg; a.vect = vect; a.elo = elo.data();
+    a.n_ff = np * (int64_t)M; a.n_ffg = a.G * (int64_t)M; a.n_out = np * (int64_t)M; a.n_elo = 6 * me * (int64_t)M;
+    for (int64_t g = 0; g < me * M; ++g) qs00_elem_body(a, g);
+    for (int64_t g = 0; g < np * M; ++g) qs00_gather_body(a, g);
   }
   return 0;
 }

@@ -142,6 +142,39 @@ def test_rhs_3x3_and_qs00():
     assert rel(v, assembly.qs_00_gauss(m.jj, m.rr, hc, ff, fg)) <= 1e-13
 
 
+def test_qs00_device_many_columns():
+    """Device path with ncols independent right-hand sides, each vs the oracle."""
+    from oracle import assembly
+    m = rectangle_p2(9, 6)
+    mesh = NL.Mesh.from_p2(m)
+    rng = np.random.default_rng(6)
+    nc = 7
+    hc = rng.uniform(0.5, 2, m.me)
+    ff = np.asfortranarray(rng.standard_normal((m.np, nc)))
+    fg = np.asfortranarray(rng.standard_normal((m.n_gauss, nc)))
+    v = fortran_empty((m.np, nc))
+    NL.qs_00_gauss(mesh, torch.as_tensor(hc, device="cuda"), fortran_tensor(ff), fortran_tensor(fg), v)
+    torch.cuda.synchronize()
+    got = v.cpu().numpy()
+    for c in range(nc):
+        exp = assembly.qs_00_gauss(m.jj, m.rr, hc, ff[:, c], fg[:, c])
+        assert rel(got[:, c], exp) <= 1e-13, c
+
+
+def test_rhs_3x3_device_all_modes():
+    """Device path over many modes (element share + gather) vs the oracle."""
+    from oracle import assembly
+    m = rectangle_p2(8, 5)
+    mesh = NL.Mesh.from_p2(m)
+    M = 37
+    f = np.asfortranarray(np.random.default_rng(7).standard_normal((m.n_gauss, 6, M)))
+    v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_3x3_all_modes(mesh, fortran_tensor(f), v1, v2, -2.0)
+    torch.cuda.synchronize()
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, f)
+    assert rel(v1.cpu().numpy(), -2.0 * e1) <= 1e-13 and rel(v2.cpu().numpy(), -2.0 * e2) <= 1e-13
+
+
 @pytest.mark.parametrize("M", [3, 16])
 def test_nl_rhs_mhd_all_modes(M):
     from oracle import assembly
