@@ -3,10 +3,24 @@
 
 namespace sfx {
 
-constexpr int FFTPROD_WPB = 2;   // waves per work-group (measured: 2 beats 1 and 4 at MP = 64)
+// Waves per work-group: 4 waves hold 16 consecutive points at MP = 64, so the
+// staged output stores write whole 128-byte lines.  The kernels are held to
+// 256 registers (2 waves per SIMD); left alone the compiler goes a few
+// registers over and the kernel runs at 1 wave per SIMD (measured 5.9 ms vs
+// 7.7 ms for the cross product at 1.4M points, m_max = 64).
+constexpr int FFTPROD_WPB = 4;
 
 template <int MP, int OP>
-__global__ void __launch_bounds__(64 * FFTPROD_WPB) fftprod_kernel(FftProdArgs a) {
+__global__ void __launch_bounds__(64 * FFTPROD_WPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+fftprod_kernel(FftProdArgs a) {
+  extern __shared__ double2 smem[];
+  fftprod_body<MP, OP>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), FFTPROD_WPB, smem);
+}
+
+// The tensor product keeps three output groups live and would spill under
+// the 256-register limit: it runs at 1 wave per SIMD.
+template <int MP, int OP>
+__global__ void __launch_bounds__(64 * FFTPROD_WPB) fftprod_kernel_wide(FftProdArgs a) {
   extern __shared__ double2 smem[];
   fftprod_body<MP, OP>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), FFTPROD_WPB, smem);
 }
@@ -18,7 +32,10 @@ static hipError_t launch_mp(const FftProdArgs& a, hipStream_t st) {
   const int64_t nblk = (a.np + pts_per_block - 1) / pts_per_block;
   if (nblk == 0) return hipSuccess;
   const size_t shmem = fftprod_smem_cplx<MP>(FFTPROD_WPB) * sizeof(double2);
-  hipLaunchKernelGGL((fftprod_kernel<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
+  if (OP == OP_TENSOR)
+    hipLaunchKernelGGL((fftprod_kernel_wide<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
+  else
+    hipLaunchKernelGGL((fftprod_kernel<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
   return hipGetLastError();
 }
 

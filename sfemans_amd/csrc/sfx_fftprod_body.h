@@ -139,6 +139,49 @@ SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4]
     }
 }
 
+// Output of the work-group's PW*wpb consecutive points through LDS, one
+// output field (cos and sin planes of all modes) at a time: each store
+// instruction then writes runs of consecutive points of one plane instead of
+// the 4-point pieces the transform layout holds.  `stage` is the exchange
+// region of the work-group (free once the modes are paired); the rows of
+// PTS points are padded by one double.
+template <int MP, int NOC, int NO>
+SFX_DEV void stage_store(const FftProdArgs& a, const double (&oa)[NOC][2][4], const double (&ob)[NOC][2][4],
+                         int tid, int wave, int pt, int l, int wpb, int64_t bid, double2* stage2) {
+  constexpr int L = Shape<MP>::L, PW = Shape<MP>::PW;
+  double* stage = reinterpret_cast<double*>(stage2);
+  const int PTS = PW * wpb, ROW = PTS + 1;
+  const int q = wave * PW + pt;
+  const int64_t p0 = bid * PTS;
+  const int nst = 2 * a.M * PTS;
+#pragma unroll
+  for (int t = 0; t < NOC; ++t)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int o = 2 * t + h;
+      if (o >= NO) continue;
+      SFX_SYNC();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = l + L * r;
+        if (m >= a.M) continue;
+        SFX_CHECK("stage write", (2 * m + 1) * ROW + q, 2 * a.M * ROW);
+        stage[(2 * m) * ROW + q] = oa[t][h][r];
+        stage[(2 * m + 1) * ROW + q] = ob[t][h][r];
+      }
+      SFX_SYNC();
+      for (int i = tid; i < nst; i += 64 * wpb) {
+        const int plane = i / PTS, qq = i % PTS;
+        const int64_t pp = p0 + qq;
+        if (pp >= a.np) continue;
+        const int c = plane & 1, m = plane >> 1;
+        const int64_t idx = pp + a.np * (int64_t)(2 * o + c + 2 * NO * m);
+        SFX_CHECK("Vout store", idx, a.nout);
+        a.Vout[idx] = stage[plane * ROW + qq];
+      }
+    }
+}
+
 // LDS per work-group: twiddles (4 MP complex) + per wave 3 signals x 256 complex.
 template <int MP>
 constexpr size_t fftprod_smem_cplx(int wpb) { return 4 * MP + (size_t)wpb * 3 * Shape<MP>::WAVE_CPLX; }
@@ -196,6 +239,10 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
     for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
     double oa[NOC][2][4], ob[NOC][2][4];
     pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
+    if (OP != OP_TENSOR) {
+      stage_store<MP, NOC, NO>(a, oa, ob, tid, wave, pt, l, wpb, bid, smem + 4 * MP);
+      continue;
+    }
     if (!valid) continue;
 #pragma unroll
     for (int t = 0; t < NOC; ++t)
@@ -207,18 +254,11 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
         for (int r = 0; r < 4; ++r) {
           const int m = l + L * r;
           if (m >= a.M) continue;
-          if (OP == OP_TENSOR) {
-            // V_out(g, p, 2o:2o+1, m) in a (3, np, 6, M) array
-            const int64_t base = g + 3 * (p + a.np * (int64_t)(2 * o + 6 * m));
-            SFX_CHECK("Vout store", base + 3 * a.np, a.nout);
-            a.Vout[base] = oa[t][h][r];
-            a.Vout[base + 3 * a.np] = ob[t][h][r];
-          } else {
-            const int64_t base = p + a.np * (int64_t)(2 * o + 2 * NO * m);
-            SFX_CHECK("Vout store", base + a.np, a.nout);
-            a.Vout[base] = oa[t][h][r];
-            a.Vout[base + a.np] = ob[t][h][r];
-          }
+          // V_out(g, p, 2o:2o+1, m) in a (3, np, 6, M) array
+          const int64_t base = g + 3 * (p + a.np * (int64_t)(2 * o + 6 * m));
+          SFX_CHECK("Vout store", base + 3 * a.np, a.nout);
+          a.Vout[base] = oa[t][h][r];
+          a.Vout[base + 3 * a.np] = ob[t][h][r];
         }
       }
   }
