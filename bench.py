@@ -350,6 +350,21 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
                       f"{n} calls x {npt} points x m_max={M}, 1 thread, {dt:.1f} s"}
 
 
+def measured_traffic(workload: str, phase: str):
+    """HBM bytes per launch of the dominant phase from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, each divided by the factor the
+    same counter reads for a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    ph = t.get("phases", {}).get(phase)
+    if not ph:
+        return None
+    return ph["fetch_bytes"] / t["copy_fetch_factor"] + ph["write_bytes"] / t["copy_write_factor"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,6 +372,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="nl", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample-s", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
+                    help="run one plain copy kernel (y = x * 1.0) of this size first, for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -365,6 +382,14 @@ def main():
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import torch
     torch.cuda.set_device(0)
+    if args.calibrate_copy_gib > 0:
+        n = int(args.calibrate_copy_gib * (1 << 30)) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
     phases, units, cfg, cpu_kind = WORKLOADS[args.workload](torch, args)
     cfg.pop("_keep", None) if False else None
     for _ in range(args.warmup):
@@ -405,7 +430,8 @@ def main():
         "data": "synthetic (seeded N(0,1)*exp(-m/decay) nodal Fourier coefficients on structured P2 meshes)",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dp.kernel,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dp.name),
+                     "kernel": dp.kernel,
                      "phase": dp.name, "kernel_ms": ph_ms[dom],
                      "fp64_tflops": dp.flops / (ph_ms[dom] / 1000.0) / 1e12},
         "cpu_baseline": cpu,

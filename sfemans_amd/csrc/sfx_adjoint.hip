@@ -4,7 +4,7 @@
 namespace sfx {
 
 template <int MP, bool SPLIT>
-__global__ void __launch_bounds__(64 * GAUSS_WAVES) adjoint_kernel(AdjArgs a) {
+__global__ void __launch_bounds__(64 * GAUSS_WAVES) __attribute__((amdgpu_waves_per_eu(1, 1))) adjoint_kernel(AdjArgs a) {
   extern __shared__ double2 smem[];
   adjoint_body<MP, SPLIT>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }

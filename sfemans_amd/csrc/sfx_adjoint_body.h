@@ -42,10 +42,12 @@ struct AdjLds {
   using LY = GaussLds<MP>;
   // per-wave region: 3 exchanged signals (3 x 256 complex) = 1536 doubles
   static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
+  static constexpr int NODAL = 2 * LY::NOD6;                              // doubles from BIG
+  static constexpr int XCH = NODAL;                                       // first wave region
   static constexpr int RED = 7 * LY::E * (MP / 4) * AdjRed<SPLIT>::K;   // points x modes-in-slot x K
-  static constexpr int BIGD = (GAUSS_WAVES * RW > 2 * LY::NOD6) ? GAUSS_WAVES * RW : 2 * LY::NOD6;
-  static constexpr int BIGD2 = BIGD > RED ? BIGD : RED;
-  static constexpr int TOTAL = LY::BIG + (BIGD2 + 1) / 2;   // double2 units
+  // the reduction area aliases the staging and wave regions (used after them)
+  static constexpr int BIGD = (NODAL + GAUSS_WAVES * RW > RED) ? NODAL + GAUSS_WAVES * RW : RED;
+  static constexpr int TOTAL = LY::BIG + (BIGD + 1) / 2;   // double2 units
 };
 
 // Load ybar at own and partner modes of Gauss point p, scaled for Phi^T, and
@@ -84,9 +86,9 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (
     for (int c = 0; c < 6; ++c) { o[c] = so * v[r][0][c]; q[c] = sp * v[r][1][c]; }
     if (m == 0) { o[1] = 0; o[3] = 0; o[5] = 0; }
     // signal 0: fields (r, theta); signal 1: (z, 0)
-    Z[0][r] = (m == 0) ? make_double2(o[0], o[2]) : make_double2(0.5 * (o[0] + o[3]), 0.5 * (o[2] - o[1]));
+    Z[0][r] = make_double2(m == 0 ? o[0] : 0.5 * (o[0] + o[3]), m == 0 ? o[2] : 0.5 * (o[2] - o[1]));
     B[0][r] = make_double2(0.5 * (q[0] - q[3]), 0.5 * (q[1] + q[2]));
-    Z[1][r] = (m == 0) ? make_double2(o[4], 0.0) : make_double2(0.5 * o[4], -0.5 * o[5]);
+    Z[1][r] = make_double2(m == 0 ? o[4] : 0.5 * o[4], m == 0 ? 0.0 : -0.5 * o[5]);
     B[1][r] = make_double2(0.5 * q[4], 0.5 * q[5]);
   }
 }
@@ -103,8 +105,9 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
   using AL = AdjLds<MP, SPLIT>;
   constexpr int KR = AdjRed<SPLIT>::K;
   const GaussArgs& a = A.g;
-  constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N;
+  constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N, NG = GAUSS_GPW;
   constexpr int GX = 3;
+  constexpr int NT = 64 * GAUSS_WAVES;
   const int wave = tid >> 6, lane = tid & 63;
   const int pt = lane / L, l = lane % L;
   double* smd = reinterpret_cast<double*>(smem);
@@ -114,76 +117,39 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
   double2* geo = smem + LY::GEO;
   int* nid = reinterpret_cast<int*>(smem + LY::NID);
   const int64_t e0 = bid * E;
-  {
-    constexpr int NT = 64 * GAUSS_WAVES;
-    constexpr int ITW = (4 * MP + NT - 1) / NT;
-    double2 twv[ITW];
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) {
-      const int i = tid + it * NT;
-      const int ic = i < 4 * MP ? i : 0;
-      twv[it] = (ic < 3 * MP) ? a.wN[ic] : a.wMP[ic - 3 * MP];
-    }
-    const double tv = a.tab[tid < 126 ? tid : 0];
-    const int lmv = a.list_mode[tid < a.M ? tid : 0];
-    const int ig = tid < E * 6 ? tid : 0;
-    const int64_t eg = e0 + ig / 6;
-    const int64_t egc = eg < a.me ? eg : 0;
-    const int node = a.jj[6 * egc + ig % 6] - 1;
-    SFX_CHECK("rr load", 2 * (int64_t)node + 1, 2 * a.np);
-    const double2 rz = make_double2(a.rr[2 * (int64_t)node], a.rr[2 * (int64_t)node + 1]);
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) {
-      const int i = tid + it * NT;
-      if (i < 4 * MP) tw[i] = twv[it];
-    }
-    if (tid < 126) tab[tid] = tv;
-    if (tid < MP) lmode[tid] = (tid < a.M) ? (double)lmv : 0.0;
-    if (tid < E * 6) {
-      geo[tid] = (eg < a.me) ? rz : make_double2(1.0, 0.0);
-      nid[tid] = (eg < a.me) ? node : 0;
-    }
-  }
+  gauss_prologue<MP>(a, tid, e0, smem);
   SFX_SYNC();
   const double2* wN = tw;
   const double2* wMP = tw + 3 * MP;
-  const int lp = wave * PW + pt;
-  const int el = lp / 7, lg = lp % 7;
-  const int64_t p = e0 * 7 + lp;
-  const bool valid = (e0 + el) < a.me;
-  GaussGeom g;
-  gauss_geom(geo + 6 * el, tab, valid ? lg : 0, g);
+
+  // ---- this lane's Gauss point in each of the wave's groups (group wave + 4 gi)
+  GaussGeom g[NG];
+  bool has[NG], valid[NG];
+  int el[NG];
+  int64_t p[NG];
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    const int grp = wave + GAUSS_WAVES * gi;
+    has[gi] = grp < GAUSS_GROUPS;
+    const int lp = (has[gi] ? grp : 0) * PW + pt;
+    el[gi] = lp / 7;
+    p[gi] = e0 * 7 + lp;
+    valid[gi] = has[gi] && (e0 + el[gi]) < a.me;
+    gauss_geom(geo + 6 * el[gi], tab, valid[gi] ? lp % 7 : 0, g[gi]);
+  }
   double* big = smd + 2 * LY::BIG;
-  double* xf = big + (size_t)wave * AL::RW;
+  double* xf = big + AL::XCH + (size_t)wave * AL::RW;
   double2* lds = reinterpret_cast<double2*>(xf);
 
-  // ---- pbar = Phi^T ybar on the grid: 2 signals x 3 branches
-  double pb[3][4][3];
-  {
-    double2 Z[2][4], B[2][4];
-    load_ybar<MP>(A, p, valid, l, Z, B);
-    double2 y[6][4];
+  // ---- base fields at the Gauss points: (curl A, B), spectra of both groups
+  double2 Z[NG][3][4], B[NG][3][4];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) prestep<MP>(Z[s], B[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
-    inverse_fft<MP, 6, GX>(y, pt, l, lds, wMP);
-#pragma unroll
-    for (int j1 = 0; j1 < 3; ++j1)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        pb[j1][r][0] = y[j1][r].x;
-        pb[j1][r][1] = y[j1][r].y;
-        pb[j1][r][2] = y[3 + j1][r].x;
-      }
-  }
-
-  // ---- base fields on the grid: (curl A, B), then the two products
-  double2 q[9][4];
-  {
-    double2 Z[3][4], B[3][4];
+  for (int gi = 0; gi < NG; ++gi)
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) { Z[s][r] = make_double2(0, 0); B[s][r] = make_double2(0, 0); }
+      for (int r = 0; r < 4; ++r) { Z[gi][s][r] = make_double2(0, 0); B[gi][s][r] = make_double2(0, 0); }
+  {
     NodalFetch<6, MP> fa, fb;
     nodal_fetch<6, MP>(a, a.U, a.nU, e0, 0, tid, nid, fa);
     if (SPLIT) nodal_fetch<6, MP>(a, A.UB, A.nUB, e0, 0, tid, nid, fb);
@@ -198,108 +164,143 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
         if (SPLIT) nodal_fetch<6, MP>(a, A.UB, A.nUB, e0, c + 1, tid, nid, fb);
       }
       const int m = c * L + l;
-      double own[12];
-      if (m < a.M && valid) {
-        if (SPLIT) pass_fields<GV_PAIRS, MP>(a, big, big + LY::NOD6, 6 * el, l, lmode[m], g, 1.0, own);
-        else pass_fields<GV_CROSS_CURL, MP>(a, big, nullptr, 6 * el, l, lmode[m], g, 1.0, own);
-      } else {
 #pragma unroll
-        for (int f = 0; f < 12; ++f) own[f] = 0.0;
-      }
-      SFX_SYNC();
-      double par[12];
-      partner_swap<12, MP>(own, par, pt, l, xf);
+      for (int gi = 0; gi < NG; ++gi) {
+        if (!has[gi]) continue;
+        double own[12];
+        if (m < a.M && valid[gi]) {
+          if (SPLIT) pass_fields<GV_PAIRS, MP>(a, big, big + LY::NOD6, 6 * el[gi], l, lmode[m], g[gi], 1.0, own);
+          else pass_fields<GV_CROSS_CURL, MP>(a, big, nullptr, 6 * el[gi], l, lmode[m], g[gi], 1.0, own);
+        } else {
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const double o4[4] = {own[4 * s], own[4 * s + 1], own[4 * s + 2], own[4 * s + 3]};
-        const double p4[4] = {par[4 * s], par[4 * s + 1], par[4 * s + 2], par[4 * s + 3]};
-        pack_chunk<MP>(c, l, o4, p4, Z[s], B[s]);
+          for (int f = 0; f < 12; ++f) own[f] = 0.0;
+        }
+        double par[12];
+        partner_swap<12, MP>(own, par, pt, l, xf);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const double o4[4] = {own[4 * s], own[4 * s + 1], own[4 * s + 2], own[4 * s + 3]};
+          const double p4[4] = {par[4 * s], par[4 * s + 1], par[4 * s + 2], par[4 * s + 3]};
+          pack_chunk<MP>(c, l, o4, p4, Z[gi][s], B[gi][s]);
+        }
       }
     }
-    double2 y[9][4];
-#pragma unroll
-    for (int s = 0; s < 3; ++s) prestep<MP>(Z[s], B[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
-    SFX_SYNC();
-    inverse_fft<MP, 9, GX>(y, pt, l, lds, wMP);
-    // fields (rU_r, rU_th), (rU_z, wU_r), (wU_th, wU_z);
-    // outputs wbar = pbar x rU, rbar = wU x pbar packed as (wb_r, wb_th), (wb_z, rb_r), (rb_th, rb_z)
-#pragma unroll
-    for (int j1 = 0; j1 < 3; ++j1)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double r0 = y[j1][r].x, r1 = y[j1][r].y, r2 = y[3 + j1][r].x;
-        const double w0 = y[3 + j1][r].y, w1 = y[6 + j1][r].x, w2 = y[6 + j1][r].y;
-        const double p0 = A.coef * pb[j1][r][0], p1 = A.coef * pb[j1][r][1], p2 = A.coef * pb[j1][r][2];
-        const double wb0 = p1 * r2 - p2 * r1, wb1 = p2 * r0 - p0 * r2, wb2 = p0 * r1 - p1 * r0;
-        const double rb0 = w1 * p2 - w2 * p1, rb1 = w2 * p0 - w0 * p2, rb2 = w0 * p1 - w1 * p0;
-        q[j1][r] = make_double2(wb0, wb1);
-        q[3 + j1][r] = make_double2(wb2, rb0);
-        q[6 + j1][r] = make_double2(rb1, rb2);
-      }
   }
-  forward_fft<MP, 9, GX>(q, pt, l, lds, wMP);
-  double2 lo[3][4], hi[3][4];
+
+  // ---- per group: pbar and the base fields on the grid, the two products, Psi^T
+  double oa[NG][3][2][4], ob[NG][3][2][4];
 #pragma unroll
-  for (int t = 0; t < 3; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
-  double oa[3][2][4], ob[3][2][4];
-  pair_modes<MP, 3>(lo, hi, pt, l, lds, oa, ob);
+  for (int gi = 0; gi < NG; ++gi) {
+    if (!has[gi]) continue;
+    // pbar = Phi^T ybar on the grid: 2 signals x 3 branches
+    double pb[3][4][3];
+    {
+      double2 Zy[2][4], By[2][4];
+      load_ybar<MP>(A, p[gi], valid[gi], l, Zy, By);
+      double2 y[6][4];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) prestep<MP>(Zy[s], By[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
+      inverse_fft<MP, 6, GX>(y, pt, l, lds, wMP);
+#pragma unroll
+      for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          pb[j1][r][0] = y[j1][r].x;
+          pb[j1][r][1] = y[j1][r].y;
+          pb[j1][r][2] = y[3 + j1][r].x;
+        }
+    }
+    double2 q[9][4];
+    {
+      double2 y[9][4];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) prestep<MP>(Z[gi][s], B[gi][s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
+      inverse_fft<MP, 9, GX>(y, pt, l, lds, wMP);
+      // fields (rA_r, rA_th), (rA_z, wB_r), (wB_th, wB_z);
+      // outputs wbar = pbar x rA, rbar = wB x pbar packed as (wb_r, wb_th), (wb_z, rb_r), (rb_th, rb_z)
+#pragma unroll
+      for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double r0 = y[j1][r].x, r1 = y[j1][r].y, r2 = y[3 + j1][r].x;
+          const double w0 = y[3 + j1][r].y, w1 = y[6 + j1][r].x, w2 = y[6 + j1][r].y;
+          const double p0 = A.coef * pb[j1][r][0], p1 = A.coef * pb[j1][r][1], p2 = A.coef * pb[j1][r][2];
+          const double wb0 = p1 * r2 - p2 * r1, wb1 = p2 * r0 - p0 * r2, wb2 = p0 * r1 - p1 * r0;
+          const double rb0 = w1 * p2 - w2 * p1, rb1 = w2 * p0 - w0 * p2, rb2 = w0 * p1 - w1 * p0;
+          q[j1][r] = make_double2(wb0, wb1);
+          q[3 + j1][r] = make_double2(wb2, rb0);
+          q[6 + j1][r] = make_double2(rb1, rb2);
+        }
+    }
+    forward_fft<MP, 9, GX>(q, pt, l, lds, wMP);
+    double2 lo[3][4], hi[3][4];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
+    pair_modes<MP, 3>(lo, hi, pt, l, lds, oa[gi], ob[gi]);
+  }
 
   // ---- Psi^T scaling, curl transpose, reference-coordinate adjoints,
   //      then the sum over the element's Gauss points through LDS
   double* red = big;
-  SFX_SYNC();
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    SFX_SYNC();   // the reduction area aliases the staging and wave regions
     const int m = l + L * r;
-    const bool ok = valid && m < a.M;
     const double sa = (m == 0) ? (double)N : 0.5 * N;
     const double sb = (m == 0) ? 0.0 : 0.5 * N;
-    double Wb[6], Rb[6];
-    // real fields: t0 = (wb_r, wb_th), t1 = (wb_z, rb_r), t2 = (rb_th, rb_z)
-    Wb[0] = sa * oa[0][0][r]; Wb[1] = sb * ob[0][0][r];
-    Wb[2] = sa * oa[0][1][r]; Wb[3] = sb * ob[0][1][r];
-    Wb[4] = sa * oa[1][0][r]; Wb[5] = sb * ob[1][0][r];
-    Rb[0] = sa * oa[1][1][r]; Rb[1] = sb * ob[1][1][r];
-    Rb[2] = sa * oa[2][0][r]; Rb[3] = sb * ob[2][0][r];
-    Rb[4] = sa * oa[2][1][r]; Rb[5] = sb * ob[2][1][r];
-    const double mr = lmode[m < MP ? m : 0] * g.iray, ir = g.iray;
-    double Dr[6] = {0, 0, 0, 0, 0, 0}, Dz[6] = {0, 0, 0, 0, 0, 0};
-    double Ww[6];   // value weights of the W^T part alone (split form)
 #pragma unroll
-    for (int k = 0; k < 6; ++k) Ww[k] = Wb[k];
-    if (SPLIT) {
+    for (int gi = 0; gi < NG; ++gi) {
+      if (!has[gi]) continue;
+      const bool ok = valid[gi] && m < a.M;
+      double Wb[6], Rb[6];
+      // real fields: t0 = (wb_r, wb_th), t1 = (wb_z, rb_r), t2 = (rb_th, rb_z)
+      Wb[0] = sa * oa[gi][0][0][r]; Wb[1] = sb * ob[gi][0][0][r];
+      Wb[2] = sa * oa[gi][0][1][r]; Wb[3] = sb * ob[gi][0][1][r];
+      Wb[4] = sa * oa[gi][1][0][r]; Wb[5] = sb * ob[gi][1][0][r];
+      Rb[0] = sa * oa[gi][1][1][r]; Rb[1] = sb * ob[gi][1][1][r];
+      Rb[2] = sa * oa[gi][2][0][r]; Rb[3] = sb * ob[gi][2][0][r];
+      Rb[4] = sa * oa[gi][2][1][r]; Rb[5] = sb * ob[gi][2][1][r];
+      const GaussGeom& gg = g[gi];
+      const double mr = lmode[m < MP ? m : 0] * gg.iray, ir = gg.iray;
+      double Dr[6] = {0, 0, 0, 0, 0, 0}, Dz[6] = {0, 0, 0, 0, 0, 0};
+      double Ww[6];   // value weights of the W^T part alone (split form)
 #pragma unroll
-      for (int k = 0; k < 6; ++k) Wb[k] = 0.0;
-    }
-    // transpose of curl6 (subroutine_ns_with_velocity.F90:594-608)
-    Wb[5] += mr * Rb[0]; Dz[2] -= Rb[0];
-    Wb[4] -= mr * Rb[1]; Dz[3] -= Rb[1];
-    Dz[0] += Rb[2];      Dr[4] -= Rb[2];
-    Dz[1] += Rb[3];      Dr[5] -= Rb[3];
-    Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
-    Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
-    double* dst = red + ((size_t)lp * L + l) * KR;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const double xb = fma(g.a11, Dr[k], g.a21 * Dz[k]);
-      const double yb = fma(g.a12, Dr[k], g.a22 * Dz[k]);
-      SFX_CHECK("red write", ((size_t)lp * L + l) * KR + KR - 1, AL::RED);
+      for (int k = 0; k < 6; ++k) Ww[k] = Wb[k];
       if (SPLIT) {
-        dst[4 * k] = ok ? Ww[k] : 0.0;
-        dst[4 * k + 1] = ok ? Wb[k] : 0.0;
-        dst[4 * k + 2] = ok ? xb : 0.0;
-        dst[4 * k + 3] = ok ? yb : 0.0;
-      } else {
-        dst[3 * k] = ok ? Wb[k] : 0.0;
-        dst[3 * k + 1] = ok ? xb : 0.0;
-        dst[3 * k + 2] = ok ? yb : 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Wb[k] = 0.0;
+      }
+      // transpose of curl6 (subroutine_ns_with_velocity.F90:594-608)
+      Wb[5] += mr * Rb[0]; Dz[2] -= Rb[0];
+      Wb[4] -= mr * Rb[1]; Dz[3] -= Rb[1];
+      Dz[0] += Rb[2];      Dr[4] -= Rb[2];
+      Dz[1] += Rb[3];      Dr[5] -= Rb[3];
+      Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
+      Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
+      const int lp = (wave + GAUSS_WAVES * gi) * PW + pt;
+      double* dst = red + ((size_t)lp * L + l) * KR;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const double xb = fma(gg.a11, Dr[k], gg.a21 * Dz[k]);
+        const double yb = fma(gg.a12, Dr[k], gg.a22 * Dz[k]);
+        SFX_CHECK("red write", ((size_t)lp * L + l) * KR + KR - 1, AL::RED);
+        if (SPLIT) {
+          dst[4 * k] = ok ? Ww[k] : 0.0;
+          dst[4 * k + 1] = ok ? Wb[k] : 0.0;
+          dst[4 * k + 2] = ok ? xb : 0.0;
+          dst[4 * k + 3] = ok ? yb : 0.0;
+        } else {
+          dst[3 * k] = ok ? Wb[k] : 0.0;
+          dst[3 * k + 1] = ok ? xb : 0.0;
+          dst[3 * k + 2] = ok ? yb : 0.0;
+        }
       }
     }
     SFX_SYNC();
     // work item (element, local node n, mode offset mi): six components;
     // mode fastest, so that consecutive lanes store consecutive modes of one
     // entry (elo layout: entry, component, mode)
-    for (int it = tid; it < E * 6 * L; it += 64 * GAUSS_WAVES) {
+    for (int it = tid; it < E * 6 * L; it += NT) {
       const int mi = it % L;
       const int n = (it / L) % 6;
       const int e = it / (6 * L);
@@ -330,7 +331,6 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
         if (SPLIT) A.elo2[o] = acc2[k];
       }
     }
-    SFX_SYNC();
   }
 }
 

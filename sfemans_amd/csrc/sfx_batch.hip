@@ -3,7 +3,7 @@
 
 namespace sfx {
 
-__global__ void __launch_bounds__(64 * GAUSS_WAVES) batch_kernel(BatchArgs b) {
+__global__ void __launch_bounds__(64 * GAUSS_WAVES) __attribute__((amdgpu_waves_per_eu(1, 1))) batch_kernel(BatchArgs b) {
   extern __shared__ double2 smem[];
   batch_body(b, threadIdx.x, blockIdx.x, smem);
 }

@@ -41,6 +41,11 @@ SFX_DEV double2 cmulc(double2 a, double2 w) {
   return make_double2(fma(a.x, w.x, a.y * w.y), fma(a.y, w.x, -a.x * w.y));
 }
 SFX_DEV double2 mul_i(double2 a) { return make_double2(-a.y, a.x); }
+// c ? a : b, component by component.  A conditional expression whose operands
+// are double2 lvalues (array elements) is lowered as a select of their
+// addresses, which keeps the whole array in scratch memory; scalar selects
+// keep it in registers.
+SFX_DEV double2 sel2(bool c, double2 a, double2 b) { return make_double2(c ? a.x : b.x, c ? a.y : b.y); }
 SFX_DEV double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }
 
 template <int MP>

@@ -131,7 +131,7 @@ SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4]
     for (int r = 0; r < 4; ++r) {
       const bool m0 = (l + L * r) == 0;
       const double2 Qm = lo[t][r];
-      const double2 Qn = m0 ? Qm : qn[t][r];
+      const double2 Qn = sel2(m0, Qm, qn[t][r]);
       oa[t][0][r] = m0 ? Qm.x * invN : (Qm.x + Qn.x) * invN;
       ob[t][0][r] = m0 ? 0.0 : (Qn.y - Qm.y) * invN;
       oa[t][1][r] = m0 ? Qm.y * invN : (Qm.y + Qn.y) * invN;

@@ -4,7 +4,7 @@
 namespace sfx {
 
 template <int MP, int V>
-__global__ void __launch_bounds__(64 * GAUSS_WAVES) gauss_kernel(GaussArgs a) {
+__global__ void __launch_bounds__(64 * GAUSS_WAVES) __attribute__((amdgpu_waves_per_eu(1, 1))) gauss_kernel(GaussArgs a) {
   extern __shared__ double2 smem[];
   gauss_body<MP, V>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }

@@ -14,10 +14,13 @@
 //                  subroutine_ns_with_velocity.F90:227, 246-248, in one launch; and
 //                  the tangents (curl U) x d + (curl d) x U (+ Lorentz terms).
 //
-// Work decomposition: a work-group of 7 waves owns E = 256/MP consecutive
-// elements, i.e. 7E consecutive Gauss points (index (m-1)*7 + l), PW = 256/MP
-// points per wave with L = MP/4 lanes each (see sfx_engine.h).  The nodal
-// data of the E elements is staged in LDS one chunk of L modes at a time.
+// Work decomposition: a work-group owns E = 256/MP consecutive elements, i.e.
+// 7E consecutive Gauss points (index (m-1)*7 + l), cut into 7 groups of
+// PW = 256/MP points (L = MP/4 lanes each, see sfx_engine.h).  The work-group
+// has GAUSS_WAVES = 4 waves; wave w handles groups w and w + 4.  With 4 waves
+// a wave may use 512 registers (one wave per SIMD), which holds the state of
+// two groups without spilling to scratch memory.  The nodal data of the E
+// elements is staged in LDS one chunk of L modes at a time.
 // Lane l interpolates mode cL + l of its Gauss point; the partner part of the
 // packed spectrum (mode MP - m) is taken from the partner lane through LDS.
 // Geometry is recomputed per element from the node coordinates (straight-
@@ -36,7 +39,9 @@ template <> struct GVT<GV_CROSS_CURL> { static constexpr int NSIG = 3, NOUT = 3,
 template <> struct GVT<GV_PAIRS>      { static constexpr int NSIG = 3, NOUT = 3, MAXP = 4; };
 template <> struct GVT<GV_UGRADC>     { static constexpr int NSIG = 4, NOUT = 1, MAXP = 1; };
 
-constexpr int GAUSS_WAVES = 7;
+constexpr int GAUSS_WAVES = 4;
+constexpr int GAUSS_GROUPS = 7;                                          // point groups per work-group
+constexpr int GAUSS_GPW = (GAUSS_GROUPS + GAUSS_WAVES - 1) / GAUSS_WAVES;  // groups per wave
 
 // LDS layout (double2 units) of the fused kernel.
 template <int MP>
@@ -56,9 +61,10 @@ struct GaussLds {
   // per-wave region in doubles: 2 exchanged signals (2 x 256 complex) or the
   // 64 lanes x 16 doubles of the partner swap, whichever is larger
   static constexpr int RW = (4 * Shape<MP>::WAVE_CPLX > 64 * 16) ? 4 * Shape<MP>::WAVE_CPLX : 64 * 16;
-  static constexpr int NODAL_CPLX = (2 * NOD6 + 1) / 2;
-  static constexpr int XCH_CPLX = GAUSS_WAVES * RW / 2;
-  static constexpr int TOTAL = BIG + (NODAL_CPLX > XCH_CPLX ? NODAL_CPLX : XCH_CPLX);
+  // nodal staging (two fields) and the wave regions do not overlap
+  static constexpr int NODAL = 2 * NOD6;                // doubles
+  static constexpr int XCH = NODAL;                     // doubles from BIG: first wave region
+  static constexpr int TOTAL = BIG + (NODAL + GAUSS_WAVES * RW + 1) / 2;
 };
 
 struct GaussGeom {
@@ -124,6 +130,33 @@ SFX_DEV void gauss_geom(const double2* geo, const double* tab, int l, GaussGeom&
 #pragma unroll
   for (int n = 0; n < 6; ++n) ray = fma(geo[n].x, g.w[n], ray);
   g.iray = 1.0 / ray;
+}
+
+// g[i] for a run-time i, field by field through selects (indexing the array
+// with a run-time value would place it in scratch memory).
+template <int N>
+SFX_DEV GaussGeom pick_geom(const GaussGeom (&g)[N], int i) {
+  GaussGeom r = g[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    const bool on = (i == k);
+    r.w = on ? g[k].w : r.w;
+    r.d = on ? g[k].d : r.d;
+    r.a11 = on ? g[k].a11 : r.a11;
+    r.a12 = on ? g[k].a12 : r.a12;
+    r.a21 = on ? g[k].a21 : r.a21;
+    r.a22 = on ? g[k].a22 : r.a22;
+    r.iray = on ? g[k].iray : r.iray;
+  }
+  return r;
+}
+
+template <typename T, int N>
+SFX_DEV T pick(const T (&v)[N], int i) {
+  T r = v[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) r = (i == k) ? v[k] : r;
+  return r;
 }
 
 // Stage the nodal values of modes [c*L, c*L+L) of a (np, NC, M) array for
@@ -200,11 +233,17 @@ template <int MP>
 SFX_DEV void pack_chunk(int c, int l, const double (&o)[4], const double (&p)[4], double2 (&Z)[4],
                         double2 (&B)[4]) {
   const bool m0 = (c == 0 && l == 0);
-  Z[c] = m0 ? make_double2(o[0], o[2]) : make_double2(0.5 * (o[0] + o[3]), 0.5 * (o[2] - o[1]));
+  const double2 zv = make_double2(m0 ? o[0] : 0.5 * (o[0] + o[3]), m0 ? o[2] : 0.5 * (o[2] - o[1]));
   const double2 bv = make_double2(0.5 * (p[0] - p[3]), 0.5 * (p[1] + p[2]));
-  // lanes l >= 1: partner of slot 3-c; lane 0: partner of slot 4-c (c >= 1)
-  B[3 - c] = (l != 0) ? bv : B[3 - c];
-  if (c >= 1) B[4 - c] = (l == 0) ? bv : B[4 - c];
+  // lanes l >= 1: partner of slot 3-c; lane 0: partner of slot 4-c (c >= 1).
+  // Selects over all four slots keep the arrays in registers even where c is
+  // not a compile-time constant.
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Z[r] = sel2(r == c, zv, Z[r]);
+    const bool tb = (l != 0) ? (r == 3 - c) : (c >= 1 && r == 4 - c);
+    B[r] = sel2(tb, bv, B[r]);
+  }
 }
 
 // Real fields (cos, sin pairs) of one pass at own mode; NF = 4*NSIG doubles
@@ -246,15 +285,71 @@ SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* n
   }
 }
 
+// Per work-group tables in LDS: twiddles, reference-element tables, mode
+// numbers, node coordinates and node numbers of the E elements.  All loads
+// issue together (clamped indices), then the LDS stores.
+template <int MP>
+SFX_DEV void gauss_prologue(const GaussArgs& a, int tid, int64_t e0, double2* smem) {
+  using LY = GaussLds<MP>;
+  constexpr int E = LY::E, NT = 64 * GAUSS_WAVES;
+  double* smd = reinterpret_cast<double*>(smem);
+  double2* tw = smem + LY::TW;
+  double* tab = smd + 2 * LY::TAB;
+  double* lmode = smd + 2 * LY::MODE;
+  double2* geo = smem + LY::GEO;
+  int* nid = reinterpret_cast<int*>(smem + LY::NID);
+  {
+    constexpr int ITW = (4 * MP + NT - 1) / NT;
+    constexpr int ITG = (E * 6 + NT - 1) / NT;
+    double2 twv[ITW], rz[ITG];
+    int node[ITG];
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int i = tid + it * NT;
+      const int ic = i < 4 * MP ? i : 0;
+      twv[it] = (ic < 3 * MP) ? a.wN[ic] : a.wMP[ic - 3 * MP];
+    }
+    const double tv = a.tab[tid < 126 ? tid : 0];
+    const int lmv = a.list_mode[tid < a.M ? tid : 0];
+#pragma unroll
+    for (int it = 0; it < ITG; ++it) {
+      const int i = tid + it * NT;
+      const int ig = i < E * 6 ? i : 0;
+      const int64_t eg = e0 + ig / 6;
+      const int64_t egc = eg < a.me ? eg : 0;
+      node[it] = a.jj[6 * egc + ig % 6] - 1;
+      SFX_CHECK("rr load", 2 * (int64_t)node[it] + 1, 2 * a.np);
+      rz[it] = make_double2(a.rr[2 * (int64_t)node[it]], a.rr[2 * (int64_t)node[it] + 1]);
+    }
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int i = tid + it * NT;
+      if (i < 4 * MP) tw[i] = twv[it];
+    }
+    if (tid < 126) tab[tid] = tv;
+    if (tid < MP) lmode[tid] = (tid < a.M) ? (double)lmv : 0.0;
+#pragma unroll
+    for (int it = 0; it < ITG; ++it) {
+      const int i = tid + it * NT;
+      if (i < E * 6) {
+        const bool ok = e0 + i / 6 < a.me;
+        geo[i] = sel2(ok, rz[it], make_double2(1.0, 0.0));
+        nid[i] = ok ? node[it] : 0;
+      }
+    }
+  }
+}
+
 template <int MP, int V>
 SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
   using LY = GaussLds<MP>;
-  constexpr int L = S::L, PW = S::PW, E = LY::E;
+  constexpr int L = S::L, PW = S::PW, E = LY::E, NG = GAUSS_GPW;
   constexpr int NSIG = GVT<V>::NSIG, NO = GVT<V>::NOUT;
   constexpr int NF = 4 * NSIG;
   constexpr int NOC = (NO + 1) / 2;
   constexpr int GX = 2;
+  constexpr int NT = 64 * GAUSS_WAVES;
   const int wave = tid >> 6, lane = tid & 63;
   const int pt = lane / L, l = lane % L;
   double* smd = reinterpret_cast<double*>(smem);
@@ -266,173 +361,189 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   double2* geo = smem + LY::GEO;
   int* nid = reinterpret_cast<int*>(smem + LY::NID);
   const int64_t e0 = bid * E;
-  {
-    // all prologue loads issue together (clamped indices), then the LDS stores
-    constexpr int NT = 64 * GAUSS_WAVES;
-    constexpr int ITW = (4 * MP + NT - 1) / NT;
-    double2 twv[ITW];
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) {
-      const int i = tid + it * NT;
-      const int ic = i < 4 * MP ? i : 0;
-      twv[it] = (ic < 3 * MP) ? a.wN[ic] : a.wMP[ic - 3 * MP];
-    }
-    const double tv = a.tab[tid < 126 ? tid : 0];
-    const int lmv = a.list_mode[tid < a.M ? tid : 0];
-    const int ig = tid < E * 6 ? tid : 0;
-    const int64_t eg = e0 + ig / 6;
-    const int64_t egc = eg < a.me ? eg : 0;
-    const int node = a.jj[6 * egc + ig % 6] - 1;
-    SFX_CHECK("rr load", 2 * (int64_t)node + 1, 2 * a.np);
-    const double2 rz = make_double2(a.rr[2 * (int64_t)node], a.rr[2 * (int64_t)node + 1]);
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) {
-      const int i = tid + it * NT;
-      if (i < 4 * MP) tw[i] = twv[it];
-    }
-    if (tid < 126) tab[tid] = tv;
-    if (tid < MP) lmode[tid] = (tid < a.M) ? (double)lmv : 0.0;
-    if (tid < E * 6) {
-      geo[tid] = (eg < a.me) ? rz : make_double2(1.0, 0.0);
-      nid[tid] = (eg < a.me) ? node : 0;
-    }
-  }
-  static_assert(E * 6 <= 64 * GAUSS_WAVES && MP <= 64 * GAUSS_WAVES, "prologue assumes one item per thread");
+  gauss_prologue<MP>(a, tid, e0, smem);
+  static_assert(126 <= NT && MP <= NT, "prologue assumes one table entry per thread");
   SFX_SYNC();
   const double2* wN = tw;
   const double2* wMP = tw + 3 * MP;
 
-  // ---- this lane's Gauss point
-  const int lp = wave * PW + pt;                  // local point 0 .. 7E-1
-  const int el = lp / 7, lg = lp % 7;
-  const int64_t p = e0 * 7 + lp;
-  const bool valid = (e0 + el) < a.me;
-  GaussGeom g;
-  gauss_geom(geo + 6 * el, tab, valid ? lg : 0, g);
-  const double hc = (V == GV_UGRADC && valid) ? a.heat_capa[e0 + el] : 1.0;
+  // ---- this lane's Gauss point in each of the wave's groups (group wave + NW*gi)
+  GaussGeom g[NG];
+  bool has[NG], valid[NG];
+  int el[NG];
+  int64_t p[NG];
+  double hc[NG];
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    const int grp = wave + GAUSS_WAVES * gi;
+    has[gi] = grp < GAUSS_GROUPS;
+    const int lp = (has[gi] ? grp : 0) * PW + pt;    // local point 0 .. 7E-1
+    el[gi] = lp / 7;
+    const int lg = lp % 7;
+    p[gi] = e0 * 7 + lp;
+    valid[gi] = has[gi] && (e0 + el[gi]) < a.me;
+    gauss_geom(geo + 6 * el[gi], tab, valid[gi] ? lg : 0, g[gi]);
+    hc[gi] = (V == GV_UGRADC && valid[gi]) ? a.heat_capa[e0 + el[gi]] : 1.0;
+  }
 
   double* big = smd + 2 * LY::BIG;
-  double* xf = big + (size_t)wave * LY::RW;
+  double* xf = big + LY::XCH + (size_t)wave * LY::RW;
   double2* lds = reinterpret_cast<double2*>(xf);
 
-  double acc[3][4][3];   // physical product accumulated over the passes
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[j][r][0] = acc[j][r][1] = acc[j][r][2] = 0.0;
-
-  double2 q[3 * NOC][4];
+  // Groups are handled together (NGS = NG) or, for the pass list, one per
+  // sweep over the passes (NGS = 1), which halves the live state at the cost
+  // of staging the nodal chunks once per group.
+  constexpr bool GROUP_OUTER = (V == GV_PAIRS);
+  constexpr int NGS = GROUP_OUTER ? 1 : NG;
   const int npass = (V == GV_PAIRS) ? a.npass : 1;
-  for (int pass = 0; pass < npass; ++pass) {
-    double2 Z[NSIG][4], B[NSIG][4];
+  for (int go = 0; go < (GROUP_OUTER ? NG : 1); ++go) {
+    double acc[NGS][3][4][3];   // physical product accumulated over the passes
 #pragma unroll
-    for (int s = 0; s < NSIG; ++s)
+    for (int gs = 0; gs < NGS; ++gs)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) { Z[s][r] = make_double2(0, 0); B[s][r] = make_double2(0, 0); }
-    // nodal arrays of this pass: A (6 columns) and B (6 or 2 columns, or none)
-    constexpr int NCB = (V == GV_UGRADC) ? 2 : 6;
-    const double* XA = (V == GV_PAIRS) ? a.pc[pass] : a.U;
-    const int64_t nXA = (V == GV_PAIRS) ? a.npc[pass] : a.nU;
-    const double* XB = (V == GV_PAIRS) ? a.pv[pass] : a.U2;
-    const int64_t nXB = (V == GV_PAIRS) ? a.npv[pass] : a.nU2;
-    const bool useB = (V == GV_UGRADC) || (V == GV_PAIRS && XB != XA);
-    const double coef = (V == GV_PAIRS) ? a.coef[pass] : 1.0;
-    NodalFetch<6, MP> fa;
-    NodalFetch<NCB, MP> fb;
-    nodal_fetch<6, MP>(a, XA, nXA, e0, 0, tid, nid, fa);
-    if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, 0, tid, nid, fb);
+      for (int j = 0; j < 3; ++j)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      // stage nodal chunk c (the field-exchange regions of the waves alias it:
-      // the barrier orders the previous chunk's swaps before these writes),
-      // then start fetching chunk c+1 while chunk c is interpolated
-      SFX_SYNC();
-      double* nodA = big;
-      double* nodB = big + LY::NOD6;
-      nodal_store<6, MP>(fa, tid, nodA, LY::ROW6);
-      if (useB) nodal_store<NCB, MP>(fb, tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
-      SFX_SYNC();
-      if (c < 3) {
-        nodal_fetch<6, MP>(a, XA, nXA, e0, c + 1, tid, nid, fa);
-        if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, c + 1, tid, nid, fb);
-      }
-      const int m = c * L + l;
-      double own[NF];
-      if (m < a.M && valid) {
-        pass_fields<V, MP>(a, nodA, useB ? nodB : nullptr, 6 * el, l, lmode[m], g, hc, own);
-      } else {
+        for (int r = 0; r < 4; ++r) acc[gs][j][r][0] = acc[gs][j][r][1] = acc[gs][j][r][2] = 0.0;
+    double2 q[NGS][3 * NOC][4];
+    for (int pass = 0; pass < npass; ++pass) {
+      double2 Z[NGS][NSIG][4], B[NGS][NSIG][4];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) own[f] = 0.0;
-      }
-      SFX_SYNC();   // all interpolation reads of the staged chunk are done
-      double par[NF];
-      partner_swap<NF, MP>(own, par, pt, l, xf);
+      for (int gs = 0; gs < NGS; ++gs)
 #pragma unroll
-      for (int s = 0; s < NSIG; ++s) {
-        const double o4[4] = {own[4 * s], own[4 * s + 1], own[4 * s + 2], own[4 * s + 3]};
-        const double p4[4] = {par[4 * s], par[4 * s + 1], par[4 * s + 2], par[4 * s + 3]};
-        pack_chunk<MP>(c, l, o4, p4, Z[s], B[s]);
-      }
-    }
-    // ---- inverse transforms of this pass
-    double2 y[3 * NSIG][4];
+        for (int s = 0; s < NSIG; ++s)
 #pragma unroll
-    for (int s = 0; s < NSIG; ++s) prestep<MP>(Z[s], B[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
-    SFX_SYNC();   // nodal staging region is reused as exchange region
-    inverse_fft<MP, 3 * NSIG, GX>(y, pt, l, lds, wMP);
-    // ---- pointwise op
+          for (int r = 0; r < 4; ++r) { Z[gs][s][r] = make_double2(0, 0); B[gs][s][r] = make_double2(0, 0); }
+      // nodal arrays of this pass: A (6 columns) and B (6 or 2 columns, or none)
+      constexpr int NCB = (V == GV_UGRADC) ? 2 : 6;
+      // (the pass entries are picked with selects: indexing the argument
+      // arrays with the loop counter would copy them to scratch memory)
+      const double* XA = a.U;
+      const double* XB = a.U2;
+      int64_t nXA = a.nU, nXB = a.nU2;
+      double coef = 1.0;
+      if (V == GV_PAIRS) {
 #pragma unroll
-    for (int j1 = 0; j1 < 3; ++j1)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        double u[8];
-#pragma unroll
-        for (int s = 0; s < NSIG; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
-        if constexpr (V == GV_UGRADC) {
-          const double o0 = u[0] * u[3] + u[1] * u[4] + u[2] * u[5] + u[6] * u[7];
-          q[j1][r] = make_double2(o0, 0.0);
-        } else {
-          const double cx = u[1] * u[5] - u[2] * u[4];
-          const double cy = u[2] * u[3] - u[0] * u[5];
-          const double cz = u[0] * u[4] - u[1] * u[3];
-          acc[j1][r][0] = fma(coef, cx, acc[j1][r][0]);
-          acc[j1][r][1] = fma(coef, cy, acc[j1][r][1]);
-          acc[j1][r][2] = fma(coef, cz, acc[j1][r][2]);
+        for (int k = 0; k < 4; ++k) {
+          const bool on = (pass == k);
+          XA = on ? a.pc[k] : XA;
+          XB = on ? a.pv[k] : XB;
+          nXA = on ? a.npc[k] : nXA;
+          nXB = on ? a.npv[k] : nXB;
+          coef = on ? a.coef[k] : coef;
         }
       }
-  }
-  if constexpr (V != GV_UGRADC) {
+      const bool useB = (V == GV_UGRADC) || (V == GV_PAIRS && XB != XA);
+      NodalFetch<6, MP> fa;
+      NodalFetch<NCB, MP> fb;
+      nodal_fetch<6, MP>(a, XA, nXA, e0, 0, tid, nid, fa);
+      if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, 0, tid, nid, fb);
+      double* nodA = big;
+      double* nodB = big + LY::NOD6;
 #pragma unroll
-    for (int j1 = 0; j1 < 3; ++j1)
+      for (int c = 0; c < 4; ++c) {
+        // stage nodal chunk c once every wave is done with chunk c-1, then
+        // start fetching chunk c+1 while chunk c is interpolated
+        SFX_SYNC();
+        nodal_store<6, MP>(fa, tid, nodA, LY::ROW6);
+        if (useB) nodal_store<NCB, MP>(fb, tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
+        SFX_SYNC();
+        if (c < 3) {
+          nodal_fetch<6, MP>(a, XA, nXA, e0, c + 1, tid, nid, fa);
+          if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, c + 1, tid, nid, fb);
+        }
+        const int m = c * L + l;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        q[j1][r] = make_double2(acc[j1][r][0], acc[j1][r][1]);
-        q[3 + j1][r] = make_double2(acc[j1][r][2], 0.0);
+        for (int gs = 0; gs < NGS; ++gs) {
+          const int gi = GROUP_OUTER ? go : gs;
+          if (!pick(has, gi)) continue;   // uniform over the wave
+          double own[NF];
+          if (m < a.M && pick(valid, gi)) {
+            pass_fields<V, MP>(a, nodA, useB ? nodB : nullptr, 6 * pick(el, gi), l, lmode[m], pick_geom(g, gi),
+                               pick(hc, gi), own);
+          } else {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) own[f] = 0.0;
+          }
+          double par[NF];
+          partner_swap<NF, MP>(own, par, pt, l, xf);
+#pragma unroll
+          for (int s = 0; s < NSIG; ++s) {
+            const double o4[4] = {own[4 * s], own[4 * s + 1], own[4 * s + 2], own[4 * s + 3]};
+            const double p4[4] = {par[4 * s], par[4 * s + 1], par[4 * s + 2], par[4 * s + 3]};
+            pack_chunk<MP>(c, l, o4, p4, Z[gs][s], B[gs][s]);
+          }
+        }
       }
-  }
-  forward_fft<MP, 3 * NOC, GX>(q, pt, l, lds, wMP);
-  double2 lo[NOC][4], hi[NOC][4];
+      // ---- inverse transforms and pointwise op of this pass, group by group
 #pragma unroll
-  for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
-  double oa[NOC][2][4], ob[NOC][2][4];
-  pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
-  if (!valid) return;
+      for (int gs = 0; gs < NGS; ++gs) {
+        const int gi = GROUP_OUTER ? go : gs;
+        if (!pick(has, gi)) continue;
+        double2 y[3 * NSIG][4];
 #pragma unroll
-  for (int t = 0; t < NOC; ++t)
+        for (int s = 0; s < NSIG; ++s) prestep<MP>(Z[gs][s], B[gs][s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
+        inverse_fft<MP, 3 * NSIG, GX>(y, pt, l, lds, wMP);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int o = 2 * t + h;
-      if (o >= NO) continue;
+        for (int j1 = 0; j1 < 3; ++j1)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = l + L * r;
-        if (m >= a.M) continue;
-        const int64_t base = p + a.G * (int64_t)(2 * o + 2 * NO * m);
-        SFX_CHECK("gauss out", base + a.G, a.nout);
-        a.Vout[base] = oa[t][h][r];
-        a.Vout[base + a.G] = ob[t][h][r];
+          for (int r = 0; r < 4; ++r) {
+            double u[8];
+#pragma unroll
+            for (int s = 0; s < NSIG; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
+            if constexpr (V == GV_UGRADC) {
+              const double o0 = u[0] * u[3] + u[1] * u[4] + u[2] * u[5] + u[6] * u[7];
+              q[gs][j1][r] = make_double2(o0, 0.0);
+            } else {
+              const double cx = u[1] * u[5] - u[2] * u[4];
+              const double cy = u[2] * u[3] - u[0] * u[5];
+              const double cz = u[0] * u[4] - u[1] * u[3];
+              acc[gs][j1][r][0] = fma(coef, cx, acc[gs][j1][r][0]);
+              acc[gs][j1][r][1] = fma(coef, cy, acc[gs][j1][r][1]);
+              acc[gs][j1][r][2] = fma(coef, cz, acc[gs][j1][r][2]);
+            }
+          }
       }
     }
+    // ---- forward transforms, pairing and stores, group by group
+#pragma unroll
+    for (int gs = 0; gs < NGS; ++gs) {
+      const int gi = GROUP_OUTER ? go : gs;
+      if (!pick(has, gi)) continue;
+      if constexpr (V != GV_UGRADC) {
+#pragma unroll
+        for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            q[gs][j1][r] = make_double2(acc[gs][j1][r][0], acc[gs][j1][r][1]);
+            q[gs][3 + j1][r] = make_double2(acc[gs][j1][r][2], 0.0);
+          }
+      }
+      forward_fft<MP, 3 * NOC, GX>(q[gs], pt, l, lds, wMP);
+      double2 lo[NOC][4], hi[NOC][4];
+#pragma unroll
+      for (int t = 0; t < NOC; ++t) poststep<MP>(q[gs][3 * t], q[gs][3 * t + 1], q[gs][3 * t + 2], l, lo[t], hi[t], wN);
+      double oa[NOC][2][4], ob[NOC][2][4];
+      pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
+      if (!pick(valid, gi)) continue;
+      const int64_t pg = pick(p, gi);
+#pragma unroll
+      for (int t = 0; t < NOC; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int o = 2 * t + h;
+          if (o >= NO) continue;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int m = l + L * r;
+            if (m >= a.M) continue;
+            const int64_t base = pg + a.G * (int64_t)(2 * o + 2 * NO * m);
+            SFX_CHECK("gauss out", base + a.G, a.nout);
+            a.Vout[base] = oa[t][h][r];
+            a.Vout[base + a.G] = ob[t][h][r];
+          }
+        }
+    }
+  }
 }
 
 }  // namespace sfx

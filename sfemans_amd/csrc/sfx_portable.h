@@ -16,9 +16,10 @@ using std::fma;
 #define SFX_DEV_NOINLINE inline
 #define __restrict__
 void sfx_emu_sync();
+void sfx_emu_wave_sync();
 [[noreturn]] void sfx_emu_oob(const char* what, long long idx, long long n, const char* file, int line);
 #define SFX_SYNC() sfx_emu_sync()
-#define SFX_WAVE_SYNC() sfx_emu_sync()
+#define SFX_WAVE_SYNC() sfx_emu_wave_sync()
 #define SFX_CHECK(what, idx, n) \
   do { if ((long long)(idx) < 0 || (long long)(idx) >= (long long)(n)) sfx_emu_oob(what, (long long)(idx), (long long)(n), __FILE__, __LINE__); } while (0)
 #else

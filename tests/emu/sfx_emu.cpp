@@ -10,15 +10,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <functional>
+#include <memory>
 #include <thread>
 #include <vector>
 
 #include "../../sfemans_amd/csrc/sfx_fftprod_body.h"
 #include "../../sfemans_amd/csrc/sfx_twiddle.h"
 
-static thread_local std::barrier<>* g_bar = nullptr;
+static thread_local std::barrier<>* g_bar = nullptr;        // work-group barrier
+static thread_local std::barrier<>* g_wave_bar = nullptr;   // barrier of the thread's wave (64 threads)
 void sfx_emu_sync() { g_bar->arrive_and_wait(); }
+void sfx_emu_wave_sync() { g_wave_bar->arrive_and_wait(); }
 void sfx_emu_oob(const char* what, long long idx, long long n, const char* file, int line) {
   std::fprintf(stderr, "SFX_EMU out of bounds: %s index %lld not in [0,%lld) at %s:%d\n", what, idx, n, file, line);
   std::fflush(stderr);
@@ -35,11 +39,14 @@ static void run_grid(int64_t nblk, int threads, size_t smem_cplx,
   std::vector<double2> smem(smem_cplx + GUARD);
   for (int64_t b = 0; b < nblk; ++b) {
     std::barrier<> bar(threads);
+    std::vector<std::unique_ptr<std::barrier<>>> wbar;
+    for (int w = 0; w < (threads + 63) / 64; ++w)
+      wbar.emplace_back(new std::barrier<>(std::min(64, threads - 64 * w)));
     for (auto& x : smem) x = make_double2(1e300, -1e300);   // poison: stale reads show up
     std::vector<std::thread> th;
     th.reserve(threads);
     for (int t = 0; t < threads; ++t)
-      th.emplace_back([&, t] { g_bar = &bar; body(t, b, smem.data()); });
+      th.emplace_back([&, t] { g_bar = &bar; g_wave_bar = wbar[t / 64].get(); body(t, b, smem.data()); });
     for (auto& x : th) x.join();
     for (size_t i = smem_cplx; i < smem.size(); ++i)
       if (smem[i].x != 1e300 || smem[i].y != -1e300) sfx_emu_oob("LDS guard", (long long)i, (long long)smem_cplx, __FILE__, __LINE__);
