@@ -504,10 +504,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           }
       }
     }
-    // ---- forward transforms and pairing, group by group; the pass list
-    // stores each group at once, the other variants keep the paired modes of
-    // both groups and store the work-group's points together (below)
-    double oa[NGS][NOC][2][4], ob[NGS][NOC][2][4];
+    // ---- forward transforms, pairing and stores, group by group
 #pragma unroll
     for (int gs = 0; gs < NGS; ++gs) {
       const int gi = GROUP_OUTER ? go : gs;
@@ -525,8 +522,8 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
       double2 lo[NOC][4], hi[NOC][4];
 #pragma unroll
       for (int t = 0; t < NOC; ++t) poststep<MP>(q[gs][3 * t], q[gs][3 * t + 1], q[gs][3 * t + 2], l, lo[t], hi[t], wN);
-      pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa[gs], ob[gs]);
-      if (!GROUP_OUTER) continue;
+      double oa[NOC][2][4], ob[NOC][2][4];
+      pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
       if (!pick(valid, gi)) continue;
       const int64_t pg = pick(p, gi);
 #pragma unroll
@@ -541,50 +538,8 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
             if (m >= a.M) continue;
             const int64_t base = pg + a.G * (int64_t)(2 * o + 2 * NO * m);
             SFX_CHECK("gauss out", base + a.G, a.nout);
-            a.Vout[base] = oa[gs][t][h][r];
-            a.Vout[base + a.G] = ob[gs][t][h][r];
-          }
-        }
-    }
-    if constexpr (!GROUP_OUTER) {
-      // Output of the work-group's 7E points, one output field (cos and sin
-      // planes of every mode) at a time through LDS (the staging and wave
-      // regions are free now): each plane's run of 7E consecutive points is
-      // written at once, so the cache lines of a plane are completed together
-      // instead of by the two groups of a wave at different times.
-      constexpr int PTS = 7 * E, ROW = PTS + 1;
-      static_assert(2 * MP * ROW <= 2 * LY::NOD6 + GAUSS_WAVES * LY::RW, "output staging fits");
-      double* stage = big;
-      const int64_t p0 = e0 * 7;
-#pragma unroll
-      for (int t = 0; t < NOC; ++t)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int o = 2 * t + h;
-          if (o >= NO) continue;
-          SFX_SYNC();
-#pragma unroll
-          for (int gs = 0; gs < NGS; ++gs) {
-            if (!has[gs]) continue;
-            const int q0 = (wave + GAUSS_WAVES * gs) * PW + pt;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int m = l + L * r;
-              if (m >= a.M) continue;
-              SFX_CHECK("out stage", (2 * m + 1) * ROW + q0, 2 * LY::NOD6 + GAUSS_WAVES * LY::RW);
-              stage[(2 * m) * ROW + q0] = oa[gs][t][h][r];
-              stage[(2 * m + 1) * ROW + q0] = ob[gs][t][h][r];
-            }
-          }
-          SFX_SYNC();
-          const int nst = 2 * a.M * PTS;
-          for (int i = tid; i < nst; i += NT) {
-            const int plane = i / PTS, qq = i % PTS;
-            if (e0 + qq / 7 >= a.me) continue;
-            const int c = plane & 1, m = plane >> 1;
-            const int64_t idx = p0 + qq + a.G * (int64_t)(2 * o + c + 2 * NO * m);
-            SFX_CHECK("gauss out", idx, a.nout);
-            a.Vout[idx] = stage[plane * ROW + qq];
+            a.Vout[base] = oa[t][h][r];
+            a.Vout[base + a.G] = ob[t][h][r];
           }
         }
     }
