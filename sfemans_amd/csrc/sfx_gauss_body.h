@@ -58,10 +58,9 @@ struct GaussLds {
   static constexpr int ROW2 = L * 2 + 2;                // doubles per node slot, 2 columns
   static constexpr int NOD6 = E * 6 * ROW6;             // doubles of one staged 6-column field
   static constexpr int NOD2 = E * 6 * ROW2;
-  // per-wave region in doubles: 4 exchanged signals (4 x 256 complex; the
-  // transforms of a wave's two groups run together) or the 64 lanes x 16
-  // doubles of the partner swap, whichever is larger
-  static constexpr int RW = (8 * Shape<MP>::WAVE_CPLX > 64 * 16) ? 8 * Shape<MP>::WAVE_CPLX : 64 * 16;
+  // per-wave region in doubles: 2 exchanged signals (2 x 256 complex) or the
+  // 64 lanes x 16 doubles of the partner swap, whichever is larger
+  static constexpr int RW = (4 * Shape<MP>::WAVE_CPLX > 64 * 16) ? 4 * Shape<MP>::WAVE_CPLX : 64 * 16;
   // nodal staging (two fields) and the wave regions do not overlap
   static constexpr int NODAL = 2 * NOD6;                // doubles
   static constexpr int XCH = NODAL;                     // doubles from BIG: first wave region
@@ -396,8 +395,6 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   // of staging the nodal chunks once per group.
   constexpr bool GROUP_OUTER = (V == GV_PAIRS);
   constexpr int NGS = GROUP_OUTER ? 1 : NG;
-  // the two groups' transforms as one batch (u.grad c, with 4 signals, would spill)
-  constexpr bool MERGE = NGS > 1 && V == GV_CROSS_CURL;
   const int npass = (V == GV_PAIRS) ? a.npass : 1;
   for (int go = 0; go < (GROUP_OUTER ? NG : 1); ++go) {
     double acc[NGS][3][4][3];   // physical product accumulated over the passes
@@ -477,44 +474,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           }
         }
       }
-      // ---- inverse transforms and pointwise op of this pass.  With two
-      // groups per sweep their transforms run as one batch of signals: the
-      // wave-level exchanges then carry both groups, and the exchange latency
-      // of a radix stage is spread over twice the arithmetic.
-      if constexpr (MERGE) {
-        constexpr int NY = 3 * NSIG;
-        double2 y[NGS * NY][4];
-#pragma unroll
-        for (int gs = 0; gs < NGS; ++gs)
-#pragma unroll
-          for (int s = 0; s < NSIG; ++s)
-            prestep<MP>(Z[gs][s], B[gs][s], l, y[gs * NY + 3 * s], y[gs * NY + 3 * s + 1], y[gs * NY + 3 * s + 2], wN);
-        inverse_fft<MP, NGS * NY, 4>(y, pt, l, lds, wMP);
-#pragma unroll
-        for (int gs = 0; gs < NGS; ++gs)
-#pragma unroll
-          for (int j1 = 0; j1 < 3; ++j1)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              double u[8];
-#pragma unroll
-              for (int s = 0; s < NSIG; ++s) {
-                u[2 * s] = y[gs * NY + 3 * s + j1][r].x;
-                u[2 * s + 1] = y[gs * NY + 3 * s + j1][r].y;
-              }
-              if constexpr (V == GV_UGRADC) {
-                const double o0 = u[0] * u[3] + u[1] * u[4] + u[2] * u[5] + u[6] * u[7];
-                q[gs][j1][r] = make_double2(o0, 0.0);
-              } else {
-                const double cx = u[1] * u[5] - u[2] * u[4];
-                const double cy = u[2] * u[3] - u[0] * u[5];
-                const double cz = u[0] * u[4] - u[1] * u[3];
-                acc[gs][j1][r][0] = fma(coef, cx, acc[gs][j1][r][0]);
-                acc[gs][j1][r][1] = fma(coef, cy, acc[gs][j1][r][1]);
-                acc[gs][j1][r][2] = fma(coef, cz, acc[gs][j1][r][2]);
-              }
-            }
-      } else {
+      // ---- inverse transforms and pointwise op of this pass, group by group
 #pragma unroll
       for (int gs = 0; gs < NGS; ++gs) {
         const int gi = GROUP_OUTER ? go : gs;
@@ -544,55 +504,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           }
       }
     }
-      }
-    // ---- forward transforms, pairing and stores (both groups in one batch
-    // when the sweep holds two)
-    if constexpr (MERGE) {
-      constexpr int NQ = 3 * NOC;
-      double2 qq[NGS * NQ][4];
-#pragma unroll
-      for (int gs = 0; gs < NGS; ++gs)
-#pragma unroll
-        for (int i = 0; i < NQ; ++i)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            if constexpr (V != GV_UGRADC) {
-              const int j1 = i % 3;
-              qq[gs * NQ + i][r] = (i < 3) ? make_double2(acc[gs][j1][r][0], acc[gs][j1][r][1])
-                                           : make_double2(acc[gs][j1][r][2], 0.0);
-            } else {
-              qq[gs * NQ + i][r] = q[gs][i][r];
-            }
-          }
-      forward_fft<MP, NGS * NQ, 4>(qq, pt, l, lds, wMP);
-      double2 lo[NGS * NOC][4], hi[NGS * NOC][4];
-#pragma unroll
-      for (int t = 0; t < NGS * NOC; ++t) poststep<MP>(qq[3 * t], qq[3 * t + 1], qq[3 * t + 2], l, lo[t], hi[t], wN);
-      double oa[NGS * NOC][2][4], ob[NGS * NOC][2][4];
-      pair_modes<MP, NGS * NOC>(lo, hi, pt, l, lds, oa, ob);
-#pragma unroll
-      for (int gs = 0; gs < NGS; ++gs) {
-        if (!valid[gs]) continue;
-#pragma unroll
-        for (int t = 0; t < NOC; ++t)
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int o = 2 * t + h;
-            if (o >= NO) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int m = l + L * r;
-              if (m >= a.M) continue;
-              const int64_t base = p[gs] + a.G * (int64_t)(2 * o + 2 * NO * m);
-              SFX_CHECK("gauss out", base + a.G, a.nout);
-              a.Vout[base] = oa[gs * NOC + t][h][r];
-              a.Vout[base + a.G] = ob[gs * NOC + t][h][r];
-            }
-          }
-      }
-      continue;
-    }
-    // one group per sweep
+    // ---- forward transforms, pairing and stores, group by group
 #pragma unroll
     for (int gs = 0; gs < NGS; ++gs) {
       const int gi = GROUP_OUTER ? go : gs;
