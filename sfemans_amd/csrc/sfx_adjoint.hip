@@ -9,8 +9,9 @@ __global__ void __launch_bounds__(64 * GAUSS_WAVES) __attribute__((amdgpu_waves_
   adjoint_body<MP, SPLIT>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }
 
-__global__ void __launch_bounds__(256) adjoint_gather_kernel(GatherArgs a) {
-  adjoint_gather_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+__global__ void __launch_bounds__(GATHER_THREADS) adjoint_gather_kernel(GatherArgs a) {
+  __shared__ double tile[gather_tile_doubles<6>()];
+  adjoint_gather_block(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, tile);
 }
 
 template <int MP, bool SPLIT>
@@ -38,9 +39,9 @@ static hipError_t launch_split(const AdjArgs& a, hipStream_t st) {
 }
 
 static hipError_t launch_gather(const GatherArgs& ga, hipStream_t st) {
-  const int64_t n = ga.np * ga.M;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(adjoint_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ga);
+  if (ga.np == 0 || ga.M == 0) return hipSuccess;
+  const int64_t nb = (ga.np + GATHER_NODES - 1) / GATHER_NODES;
+  hipLaunchKernelGGL(adjoint_gather_kernel, dim3((unsigned)nb), dim3(GATHER_THREADS), 0, st, ga);
   return hipGetLastError();
 }
 

@@ -13,8 +13,9 @@
 // (N, N/2, N/2) with a zero mode-0 sine.  R_p^T and W_p^T are applied per
 // mode and Gauss point; the 7 Gauss points of an element are summed through
 // LDS into per-element nodal contributions elo(6 me, 6, M), which a node-
-// centred gather (adjoint_gather_body) sums in element order.
+// centred gather (adjoint_gather_block) sums in element order.
 #pragma once
+#include "sfx_gather.h"
 #include "sfx_gauss_body.h"
 
 namespace sfx {
@@ -345,29 +346,14 @@ struct GatherArgs {
   int M;
 };
 
-// One thread per (node, mode), mode fastest: the loads of a wave read
-// consecutive modes of one entry.
-SFX_DEV void adjoint_gather_body(const GatherArgs& a, int64_t gid) {
-  const int64_t m = gid % a.M;
-  const int64_t n = gid / a.M;
-  if (n >= a.np) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t q = a.adj_ptr[n]; q < a.adj_ptr[n + 1]; ++q) {
-    SFX_CHECK("adj", q, a.n_adj);
-    const int64_t ent = a.adj_ent[q];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const int64_t i = (ent * 6 + k) * a.M + m;
-      SFX_CHECK("elo load", i, a.nelo);
-      acc[k] += a.elo[i];
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int64_t o = n + a.np * (k + 6 * m);
-    SFX_CHECK("adjoint out", o, a.nout);
-    a.out[o] = acc[k];
-  }
+// Work-group bid of the gather (sfx_gather.h): dbar(n, k, m).
+SFX_DEV void adjoint_gather_block(const GatherArgs& a, int64_t bid, int tid, double* tile) {
+  gather_body<6>(a.adj_ptr, a.adj_ent, a.n_adj, a.elo, a.nelo, a.np, a.M, bid, tid, tile,
+                 [&](int k, int m, int64_t n, double v) {
+                   const int64_t o = n + a.np * (k + 6 * (int64_t)m);
+                   SFX_CHECK("adjoint out", o, a.nout);
+                   a.out[o] = v;
+                 });
 }
 
 }  // namespace sfx

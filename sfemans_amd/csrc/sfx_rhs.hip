@@ -11,16 +11,18 @@ __global__ void __launch_bounds__(256) rhs_elem_kernel(RhsArgs a) {
   rhs_elem_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
-__global__ void __launch_bounds__(256) rhs_gather_kernel(RhsArgs a) {
-  rhs_gather_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+__global__ void __launch_bounds__(GATHER_THREADS) rhs_gather_kernel(RhsArgs a) {
+  __shared__ double tile[gather_tile_doubles<6>()];
+  rhs_gather_block(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, tile);
 }
 
 __global__ void __launch_bounds__(256) qs00_elem_kernel(RhsArgs a) {
   qs00_elem_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
-__global__ void __launch_bounds__(256) qs00_gather_kernel(RhsArgs a) {
-  qs00_gather_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+__global__ void __launch_bounds__(GATHER_THREADS) qs00_gather_kernel(RhsArgs a) {
+  __shared__ double tile[gather_tile_doubles<1>()];
+  qs00_gather_block(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, tile);
 }
 
 // rhs_3x3: element contributions, then the node-centred gather.
@@ -29,7 +31,8 @@ hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
   if (ne > 0) hipLaunchKernelGGL(rhs_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nn == 0) return e;
-  hipLaunchKernelGGL(rhs_gather_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, a);
+  const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;
+  hipLaunchKernelGGL(rhs_gather_kernel, dim3((unsigned)nb), dim3(GATHER_THREADS), 0, st, a);
   return hipGetLastError();
 }
 
@@ -38,7 +41,8 @@ hipError_t launch_qs00(const RhsArgs& a, hipStream_t st) {
   if (ne > 0) hipLaunchKernelGGL(qs00_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nn == 0) return e;
-  hipLaunchKernelGGL(qs00_gather_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, a);
+  const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;
+  hipLaunchKernelGGL(qs00_gather_kernel, dim3((unsigned)nb), dim3(GATHER_THREADS), 0, st, a);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 //   vb(n, ki, m) = sum_(e,i) sum_l ww(i,l) wgt(e,l) rhs(7e+l, comp(ki), m)
 // with comp = (1,4,5) for vb_145 and (2,3,6) for vb_236 (:75-95).
 #pragma once
+#include "sfx_gather.h"
 #include "sfx_portable.h"
 
 namespace sfx {
@@ -44,7 +45,7 @@ struct RhsArgs {
 //   elo(6e+i, k, m) = sum_l ww(i,l) wgt(e,l) rhs(7e+l, comp k, m)
 // (the values the reference adds with VecSetValues, assembling_rhs.F90:73-95).
 // A thread reads the 7 consecutive Gauss values of its element per
-// component; the stores of a wave cover consecutive modes.  rhs_gather_body
+// component; the stores of a wave cover consecutive modes.  rhs_gather_block
 // then sums the entries of each node in element order.
 SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
   const int64_t m = gid % a.M;
@@ -79,33 +80,18 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
   }
 }
 
-// vb_145 / vb_236 of node n, mode m: the node's element contributions summed
-// in element order, components (1,4,5) and (2,3,6), times scale.  Mode
-// fastest: the loads of a wave read consecutive modes of one entry.
-SFX_DEV void rhs_gather_body(const RhsArgs& a, int64_t gid) {
-  const int64_t m = gid % a.M;
-  const int64_t n = gid / a.M;
-  if (n >= a.np) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  const int64_t b = a.adj_ptr[n], e_ = a.adj_ptr[n + 1];
-  for (int64_t q = b; q < e_; ++q) {
-    SFX_CHECK("adj", q, a.n_adj);
-    const int64_t ent = a.adj_ent[q];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const int64_t i = (ent * 6 + k) * a.M + m;
-      SFX_CHECK("elo load", i, a.n_elo);
-      acc[k] += a.elo[i];
-    }
-  }
-  const int64_t o = n + a.np * 3 * m;
-  SFX_CHECK("v145 store", o + 2 * a.np, a.n_out);
-  a.v145[o] = a.scale * acc[0];
-  a.v145[o + a.np] = a.scale * acc[3];
-  a.v145[o + 2 * a.np] = a.scale * acc[4];
-  a.v236[o] = a.scale * acc[1];
-  a.v236[o + a.np] = a.scale * acc[2];
-  a.v236[o + 2 * a.np] = a.scale * acc[5];
+// vb_145 / vb_236 of work-group bid: the nodes' element contributions summed
+// in element order (sfx_gather.h), components (1,4,5) and (2,3,6), times scale.
+SFX_DEV void rhs_gather_block(const RhsArgs& a, int64_t bid, int tid, double* tile) {
+  gather_body<6>(a.adj_ptr, a.adj_ent, a.n_adj, a.elo, a.n_elo, a.np, a.M, bid, tid, tile,
+                 [&](int k, int m, int64_t n, double v) {
+                   // component k -> (array, column): 1,4,5 -> vb_145; 2,3,6 -> vb_236
+                   const bool v145 = (k == 0 || k == 3 || k == 4);
+                   const int col = (k == 0 || k == 1) ? 0 : (k == 2 || k == 3) ? 1 : 2;
+                   const int64_t o = n + a.np * (col + 3 * (int64_t)m);
+                   SFX_CHECK("vb store", o, a.n_out);
+                   (v145 ? a.v145 : a.v236)[o] = a.scale * v;
+                 });
 }
 
 // Element share of qs_00_gauss, one thread per (element, column), column fastest:
@@ -144,21 +130,14 @@ SFX_DEV void qs00_elem_body(const RhsArgs& a, int64_t gid) {
   }
 }
 
-// vect(n, c): the node's element contributions summed in element order.
-SFX_DEV void qs00_gather_body(const RhsArgs& a, int64_t gid) {
-  const int64_t c = gid % a.M;
-  const int64_t n = gid / a.M;
-  if (n >= a.np) return;
-  double acc = 0;
-  const int64_t b = a.adj_ptr[n], e_ = a.adj_ptr[n + 1];
-  for (int64_t q = b; q < e_; ++q) {
-    SFX_CHECK("adj", q, a.n_adj);
-    const int64_t i = (int64_t)a.adj_ent[q] * a.M + c;
-    SFX_CHECK("elo load", i, a.n_elo);
-    acc += a.elo[i];
-  }
-  SFX_CHECK("vect store", n + a.np * c, a.n_out);
-  a.vect[n + a.np * c] = a.scale * acc;
+// vect(n, c) of work-group bid: the nodes' element contributions summed in
+// element order (sfx_gather.h).
+SFX_DEV void qs00_gather_block(const RhsArgs& a, int64_t bid, int tid, double* tile) {
+  gather_body<1>(a.adj_ptr, a.adj_ent, a.n_adj, a.elo, a.n_elo, a.np, a.M, bid, tid, tile,
+                 [&](int, int c, int64_t n, double v) {
+                   SFX_CHECK("vect store", n + a.np * c, a.n_out);
+                   a.vect[n + a.np * c] = a.scale * v;
+                 });
 }
 
 }  // namespace sfx

@@ -192,13 +192,15 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
     a.rhs = rhs; a.v145 = v145; a.v236 = v236; a.elo = elo.data();
     a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = np * 3 * (int64_t)M; a.n_elo = 36 * me * (int64_t)M;
     for (int64_t g = 0; g < me * M; ++g) rhs_elem_body(a, g);
-    for (int64_t g = 0; g < np * M; ++g) rhs_gather_body(a, g);
+    run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<6>() + 1) / 2,
+             [&](int t, int64_t b, double2* sm) { rhs_gather_block(a, b, t, reinterpret_cast<double*>(sm)); });
   } else {
     std::vector<double> elo((size_t)6 * me * M + 1);
     a.heat_capa = hc; a.ff = ff; a.ffg = ffg; a.vect = vect; a.elo = elo.data();
     a.n_ff = np * (int64_t)M; a.n_ffg = a.G * (int64_t)M; a.n_out = np * (int64_t)M; a.n_elo = 6 * me * (int64_t)M;
     for (int64_t g = 0; g < me * M; ++g) qs00_elem_body(a, g);
-    for (int64_t g = 0; g < np * M; ++g) qs00_gather_body(a, g);
+    run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<1>() + 1) / 2,
+             [&](int t, int64_t b, double2* sm) { qs00_gather_block(a, b, t, reinterpret_cast<double*>(sm)); });
   }
   return 0;
 }
@@ -260,10 +262,14 @@ extern "C" int emu_adjoint(const int32_t* jj, const double* rr, int64_t me, int6
   ga.adj_ptr = ptr.data(); ga.adj_ent = ent.data(); ga.elo = elo.data(); ga.out = dbar;
   ga.me = me; ga.np = np; ga.n_adj = (int64_t)ent.size(); ga.nelo = (int64_t)elo.size();
   ga.nout = np * 6 * (int64_t)M; ga.M = M;
-  for (int64_t g = 0; g < np * M; ++g) adjoint_gather_body(ga, g);
+  auto gather = [&](const GatherArgs& g) {
+    run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<6>() + 1) / 2,
+             [&](int t, int64_t b, double2* sm) { adjoint_gather_block(g, b, t, reinterpret_cast<double*>(sm)); });
+  };
+  gather(ga);
   if (split) {
     ga.elo = elo2.data(); ga.out = dbar2;
-    for (int64_t g = 0; g < np * M; ++g) adjoint_gather_body(ga, g);
+    gather(ga);
   }
   return 0;
 }
