@@ -45,7 +45,13 @@ struct AdjLds {
   static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
   static constexpr int NODAL = 2 * LY::NOD6;                              // doubles from BIG
   static constexpr int XCH = NODAL;                                       // first wave region
-  static constexpr int RED = 7 * LY::E * (MP / 4) * AdjRed<SPLIT>::K;   // points x modes-in-slot x K
+  // red[i * SI + k * SK] for entry i = local point * L + mode offset and
+  // weight k: entry-major for the convective form, weight-major for the
+  // split form (its 24 weights per entry would conflict on LDS banks)
+  static constexpr int RPL = 7 * LY::E * (MP / 4);
+  static constexpr int RED = RPL * AdjRed<SPLIT>::K;
+  static constexpr int SI = SPLIT ? 1 : AdjRed<SPLIT>::K;
+  static constexpr int SK = SPLIT ? RPL : 1;
   // the reduction area aliases the staging and wave regions (used after them)
   static constexpr int BIGD = (NODAL + GAUSS_WAVES * RW > RED) ? NODAL + GAUSS_WAVES * RW : RED;
   static constexpr int TOTAL = LY::BIG + (BIGD + 1) / 2;   // double2 units
@@ -60,9 +66,13 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (
   const int64_t pc = valid ? p : 0;
   const double two_n = 2.0 / N, one_n = 1.0 / N;
   double v[4][2][6];   // [slot][own/partner][a_r, b_r, a_th, b_th, a_z, b_z]
+  // the plane offsets are the same for every group: kept live between calls
+  // they would be spilled, so they are recomputed each time
+  int lo = l;
+  SFX_OPAQUE(lo);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int m = l + L * r, k = MP - m;
+    const int m = lo + L * r, k = MP - m;
     const int mc = (m < a.M) ? m : 0;
     const int kc = (m > 0 && k < a.M) ? k : 0;
 #pragma unroll
@@ -279,21 +289,22 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
       Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
       const int lp = (wave + GAUSS_WAVES * gi) * PW + pt;
-      double* dst = red + ((size_t)lp * L + l) * KR;
+      double* dst = red + (lp * L + l) * AL::SI;
+      constexpr int RP = AL::SK;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const double xb = fma(gg.a11, Dr[k], gg.a21 * Dz[k]);
         const double yb = fma(gg.a12, Dr[k], gg.a22 * Dz[k]);
-        SFX_CHECK("red write", ((size_t)lp * L + l) * KR + KR - 1, AL::RED);
+        SFX_CHECK("red write", (KR - 1) * RP + (lp * L + l) * AL::SI, AL::RED);
         if (SPLIT) {
-          dst[4 * k] = ok ? Ww[k] : 0.0;
-          dst[4 * k + 1] = ok ? Wb[k] : 0.0;
-          dst[4 * k + 2] = ok ? xb : 0.0;
-          dst[4 * k + 3] = ok ? yb : 0.0;
+          dst[(4 * k) * RP] = ok ? Ww[k] : 0.0;
+          dst[(4 * k + 1) * RP] = ok ? Wb[k] : 0.0;
+          dst[(4 * k + 2) * RP] = ok ? xb : 0.0;
+          dst[(4 * k + 3) * RP] = ok ? yb : 0.0;
         } else {
-          dst[3 * k] = ok ? Wb[k] : 0.0;
-          dst[3 * k + 1] = ok ? xb : 0.0;
-          dst[3 * k + 2] = ok ? yb : 0.0;
+          dst[(3 * k) * RP] = ok ? Wb[k] : 0.0;
+          dst[(3 * k + 1) * RP] = ok ? xb : 0.0;
+          dst[(3 * k + 2) * RP] = ok ? yb : 0.0;
         }
       }
     }
@@ -309,19 +320,20 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       const int64_t eg = e0 + e;
       if (eg >= a.me || mm >= a.M) continue;
       double acc[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
+      constexpr int RP = AL::SK;
       for (int lg2 = 0; lg2 < 7; ++lg2) {
-        const double* src = red + ((size_t)(7 * e + lg2) * L + mi) * KR;
+        const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
         const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
         if (SPLIT) {
 #pragma unroll
           for (int k = 0; k < 6; ++k) {
-            acc[k] = fma(w, src[4 * k], acc[k]);
-            acc2[k] = fma(w, src[4 * k + 1], fma(dx, src[4 * k + 2], fma(dy, src[4 * k + 3], acc2[k])));
+            acc[k] = fma(w, src[(4 * k) * RP], acc[k]);
+            acc2[k] = fma(w, src[(4 * k + 1) * RP], fma(dx, src[(4 * k + 2) * RP], fma(dy, src[(4 * k + 3) * RP], acc2[k])));
           }
         } else {
 #pragma unroll
           for (int k = 0; k < 6; ++k)
-            acc[k] = fma(w, src[3 * k], fma(dx, src[3 * k + 1], fma(dy, src[3 * k + 2], acc[k])));
+            acc[k] = fma(w, src[(3 * k) * RP], fma(dx, src[(3 * k + 1) * RP], fma(dy, src[(3 * k + 2) * RP], acc[k])));
         }
       }
 #pragma unroll

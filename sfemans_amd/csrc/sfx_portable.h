@@ -20,6 +20,7 @@ void sfx_emu_wave_sync();
 [[noreturn]] void sfx_emu_oob(const char* what, long long idx, long long n, const char* file, int line);
 #define SFX_SYNC() sfx_emu_sync()
 #define SFX_WAVE_SYNC() sfx_emu_wave_sync()
+#define SFX_OPAQUE(x) do { } while (0)
 #define SFX_CHECK(what, idx, n) \
   do { if ((long long)(idx) < 0 || (long long)(idx) >= (long long)(n)) sfx_emu_oob(what, (long long)(idx), (long long)(n), __FILE__, __LINE__); } while (0)
 #else
@@ -37,4 +38,8 @@ void sfx_emu_wave_sync();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
   } while (0)
 #define SFX_CHECK(what, idx, n) do { } while (0)
+// Hide a value's origin from the optimizer, so that what is computed from it
+// is recomputed where it is used instead of being kept live (or spilled)
+// across a long stretch of code.  Emits no instruction.
+#define SFX_OPAQUE(x) __asm__ volatile("" : "+v"(x))
 #endif
