@@ -3,10 +3,11 @@
 
 namespace sfx {
 
-template <int MP, bool SPLIT>
-__global__ void __launch_bounds__(64 * GAUSS_WAVES) __attribute__((amdgpu_waves_per_eu(1, 1))) adjoint_kernel(AdjArgs a) {
+template <int MP, bool SPLIT, int NW = adjoint_waves<MP, SPLIT>()>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(waves_per_simd(NW), waves_per_simd(NW))))
+adjoint_kernel(AdjArgs a) {
   extern __shared__ double2 smem[];
-  adjoint_body<MP, SPLIT>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+  adjoint_body<MP, SPLIT, NW>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }
 
 __global__ void __launch_bounds__(GATHER_THREADS) adjoint_gather_kernel(GatherArgs a) {
@@ -16,11 +17,12 @@ __global__ void __launch_bounds__(GATHER_THREADS) adjoint_gather_kernel(GatherAr
 
 template <int MP, bool SPLIT>
 static hipError_t launch_mp(const AdjArgs& a, hipStream_t st) {
-  constexpr int E = GaussLds<MP>::E;
+  constexpr int NW = adjoint_waves<MP, SPLIT>();
+  constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (a.g.me + E - 1) / E;
   if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)AdjLds<MP, SPLIT>::TOTAL * sizeof(double2);
-  hipLaunchKernelGGL((adjoint_kernel<MP, SPLIT>), dim3((unsigned)nblk), dim3(64 * GAUSS_WAVES), shmem, st, a);
+  const size_t shmem = (size_t)AdjLds<MP, SPLIT, NW>::TOTAL * sizeof(double2);
+  hipLaunchKernelGGL((adjoint_kernel<MP, SPLIT>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
   return hipGetLastError();
 }
 

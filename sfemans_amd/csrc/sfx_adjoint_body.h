@@ -38,9 +38,15 @@ struct AdjArgs {
 // weights of the two outputs apart.
 template <bool SPLIT> struct AdjRed { static constexpr int K = SPLIT ? 24 : 18; };
 
-template <int MP, bool SPLIT = false>
+// Waves per work-group of the adjoint kernel (see gauss_waves): 7 for the
+// convective form at MP = 64 (measured faster there, slower at MP = 32 and
+// 128), 4 elsewhere.
+template <int MP, bool SPLIT>
+constexpr int adjoint_waves() { return (!SPLIT && MP == 64) ? 7 : 4; }
+
+template <int MP, bool SPLIT, int NW>
 struct AdjLds {
-  using LY = GaussLds<MP>;
+  using LY = GaussLds<MP, NW>;
   // per-wave region: 3 exchanged signals (3 x 256 complex) = 1536 doubles
   static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
   static constexpr int NODAL = 2 * LY::NOD6;                              // doubles from BIG
@@ -53,7 +59,7 @@ struct AdjLds {
   static constexpr int SI = SPLIT ? 1 : AdjRed<SPLIT>::K;
   static constexpr int SK = SPLIT ? RPL : 1;
   // the reduction area aliases the staging and wave regions (used after them)
-  static constexpr int BIGD = (NODAL + GAUSS_WAVES * RW > RED) ? NODAL + GAUSS_WAVES * RW : RED;
+  static constexpr int BIGD = (NODAL + NW * RW > RED) ? NODAL + NW * RW : RED;
   static constexpr int TOTAL = LY::BIG + (BIGD + 1) / 2;   // double2 units
 };
 
@@ -109,16 +115,16 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (
 // SPLIT = true: transpose, times coef, of the two Lorentz passes
 // (curl A) x W dB and (curl dH) x B: the W^T part (pbar x curl A) is the
 // adjoint for dB (elo), the R^T part (B x pbar) the adjoint for dH (elo2).
-template <int MP, bool SPLIT>
+template <int MP, bool SPLIT, int NW>
 SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
-  using LY = GaussLds<MP>;
-  using AL = AdjLds<MP, SPLIT>;
+  using LY = GaussLds<MP, NW>;
+  using AL = AdjLds<MP, SPLIT, NW>;
   constexpr int KR = AdjRed<SPLIT>::K;
   const GaussArgs& a = A.g;
-  constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N, NG = GAUSS_GPW;
+  constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N, NG = groups_per_wave(NW);
   constexpr int GX = 3;
-  constexpr int NT = 64 * GAUSS_WAVES;
+  constexpr int NT = 64 * NW;
   const int wave = tid >> 6, lane = tid & 63;
   const int pt = lane / L, l = lane % L;
   double* smd = reinterpret_cast<double*>(smem);
@@ -128,7 +134,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
   double2* geo = smem + LY::GEO;
   int* nid = reinterpret_cast<int*>(smem + LY::NID);
   const int64_t e0 = bid * E;
-  gauss_prologue<MP>(a, tid, e0, smem);
+  gauss_prologue<MP, NW>(a, tid, e0, smem);
   SFX_SYNC();
   const double2* wN = tw;
   const double2* wMP = tw + 3 * MP;
@@ -140,7 +146,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
   int64_t p[NG];
 #pragma unroll
   for (int gi = 0; gi < NG; ++gi) {
-    const int grp = wave + GAUSS_WAVES * gi;
+    const int grp = wave + NW * gi;
     has[gi] = grp < GAUSS_GROUPS;
     const int lp = (has[gi] ? grp : 0) * PW + pt;
     el[gi] = lp / 7;
@@ -161,18 +167,18 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
 #pragma unroll
       for (int r = 0; r < 4; ++r) { Z[gi][s][r] = make_double2(0, 0); B[gi][s][r] = make_double2(0, 0); }
   {
-    NodalFetch<6, MP> fa, fb;
-    nodal_fetch<6, MP>(a, a.U, a.nU, e0, 0, tid, nid, fa);
-    if (SPLIT) nodal_fetch<6, MP>(a, A.UB, A.nUB, e0, 0, tid, nid, fb);
+    NodalFetch<6, MP, NW> fa, fb;
+    nodal_fetch<6, MP, NW>(a, a.U, a.nU, e0, 0, tid, nid, fa);
+    if (SPLIT) nodal_fetch<6, MP, NW>(a, A.UB, A.nUB, e0, 0, tid, nid, fb);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       SFX_SYNC();
-      nodal_store<6, MP>(fa, tid, big, LY::ROW6);
-      if (SPLIT) nodal_store<6, MP>(fb, tid, big + LY::NOD6, LY::ROW6);
+      nodal_store<6, MP, NW>(fa, tid, big, LY::ROW6);
+      if (SPLIT) nodal_store<6, MP, NW>(fb, tid, big + LY::NOD6, LY::ROW6);
       SFX_SYNC();
       if (c < 3) {
-        nodal_fetch<6, MP>(a, a.U, a.nU, e0, c + 1, tid, nid, fa);
-        if (SPLIT) nodal_fetch<6, MP>(a, A.UB, A.nUB, e0, c + 1, tid, nid, fb);
+        nodal_fetch<6, MP, NW>(a, a.U, a.nU, e0, c + 1, tid, nid, fa);
+        if (SPLIT) nodal_fetch<6, MP, NW>(a, A.UB, A.nUB, e0, c + 1, tid, nid, fb);
       }
       const int m = c * L + l;
 #pragma unroll
@@ -288,7 +294,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       Dz[1] += Rb[3];      Dr[5] -= Rb[3];
       Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
       Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
-      const int lp = (wave + GAUSS_WAVES * gi) * PW + pt;
+      const int lp = (wave + NW * gi) * PW + pt;
       double* dst = red + (lp * L + l) * AL::SI;
       constexpr int RP = AL::SK;
 #pragma unroll

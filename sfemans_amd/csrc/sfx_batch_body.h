@@ -27,7 +27,7 @@ struct BatchArgs {
 // one register allocation or one stack frame.
 template <int MP>
 SFX_DEV_NOINLINE void batch_tile(const GaussArgs& a, int tid, int64_t block, double2* smem) {
-  gauss_body<MP, GV_CROSS_CURL>(a, tid, block, smem);
+  gauss_body<MP, GV_CROSS_CURL, GAUSS_WAVES>(a, tid, block, smem);
 }
 
 // Work-group `wg`: every tile of its list, one after the other.  The barrier

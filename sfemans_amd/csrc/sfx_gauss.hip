@@ -3,19 +3,23 @@
 
 namespace sfx {
 
-template <int MP, int V>
-__global__ void __launch_bounds__(64 * GAUSS_WAVES) __attribute__((amdgpu_waves_per_eu(1, 1))) gauss_kernel(GaussArgs a) {
+// NW waves per work-group, held to the register budget of waves_per_simd(NW)
+// waves per SIMD.
+template <int MP, int V, int NW = gauss_waves<MP, V>()>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(waves_per_simd(NW), waves_per_simd(NW))))
+gauss_kernel(GaussArgs a) {
   extern __shared__ double2 smem[];
-  gauss_body<MP, V>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+  gauss_body<MP, V, NW>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }
 
 template <int MP, int V>
 static hipError_t launch_mp(const GaussArgs& a, hipStream_t st) {
-  constexpr int E = GaussLds<MP>::E;
+  constexpr int NW = gauss_waves<MP, V>();
+  constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (a.me + E - 1) / E;
   if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)GaussLds<MP>::TOTAL * sizeof(double2);
-  hipLaunchKernelGGL((gauss_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * GAUSS_WAVES), shmem, st, a);
+  const size_t shmem = (size_t)GaussLds<MP, NW>::TOTAL * sizeof(double2);
+  hipLaunchKernelGGL((gauss_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
   return hipGetLastError();
 }
 

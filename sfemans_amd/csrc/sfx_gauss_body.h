@@ -17,9 +17,9 @@
 // Work decomposition: a work-group owns E = 256/MP consecutive elements, i.e.
 // 7E consecutive Gauss points (index (m-1)*7 + l), cut into 7 groups of
 // PW = 256/MP points (L = MP/4 lanes each, see sfx_engine.h).  The work-group
-// has GAUSS_WAVES = 4 waves; wave w handles groups w and w + 4.  With 4 waves
-// a wave may use 512 registers (one wave per SIMD), which holds the state of
-// two groups without spilling to scratch memory.  The nodal data of the E
+// has NW waves (gauss_waves): with NW = 4 wave w handles groups w and w + 4
+// in up to 512 registers (one wave per SIMD); with NW = 7 each wave handles
+// one group in 256 registers (two waves per SIMD).  The nodal data of the E
 // elements is staged in LDS one chunk of L modes at a time.
 // Lane l interpolates mode cL + l of its Gauss point; the partner part of the
 // packed spectrum (mode MP - m) is taken from the partner lane through LDS.
@@ -39,12 +39,20 @@ template <> struct GVT<GV_CROSS_CURL> { static constexpr int NSIG = 3, NOUT = 3,
 template <> struct GVT<GV_PAIRS>      { static constexpr int NSIG = 3, NOUT = 3, MAXP = 4; };
 template <> struct GVT<GV_UGRADC>     { static constexpr int NSIG = 4, NOUT = 1, MAXP = 1; };
 
-constexpr int GAUSS_WAVES = 4;
-constexpr int GAUSS_GROUPS = 7;                                          // point groups per work-group
-constexpr int GAUSS_GPW = (GAUSS_GROUPS + GAUSS_WAVES - 1) / GAUSS_WAVES;  // groups per wave
+constexpr int GAUSS_GROUPS = 7;   // point groups per work-group
+constexpr int GAUSS_WAVES = 4;    // default wave count (batched launches)
+// Waves per work-group of the fused kernel for variant V at size MP.  With 7
+// waves a wave holds one group and a CU runs two waves per SIMD (256
+// registers each), which hides more latency; with 4 waves a wave holds two
+// groups in 512 registers at one wave per SIMD.  7 where one group's state
+// fits in 256 registers (measured faster there), 4 elsewhere.
+template <int MP, int V>
+constexpr int gauss_waves() { return (V == GV_UGRADC || (V == GV_CROSS_CURL && MP <= 64)) ? 7 : 4; }
+constexpr int groups_per_wave(int nw) { return (GAUSS_GROUPS + nw - 1) / nw; }
+constexpr int waves_per_simd(int nw) { return nw > 4 ? 2 : 1; }
 
-// LDS layout (double2 units) of the fused kernel.
-template <int MP>
+// LDS layout (double2 units) of the fused kernel with NW waves.
+template <int MP, int NW = GAUSS_WAVES>
 struct GaussLds {
   static constexpr int E = 256 / MP;                    // elements per work-group
   static constexpr int L = MP / 4;
@@ -64,7 +72,7 @@ struct GaussLds {
   // nodal staging (two fields) and the wave regions do not overlap
   static constexpr int NODAL = 2 * NOD6;                // doubles
   static constexpr int XCH = NODAL;                     // doubles from BIG: first wave region
-  static constexpr int TOTAL = BIG + (NODAL + GAUSS_WAVES * RW + 1) / 2;
+  static constexpr int TOTAL = BIG + (NODAL + NW * RW + 1) / 2;
 };
 
 struct GaussGeom {
@@ -161,10 +169,10 @@ SFX_DEV T pick(const T (&v)[N], int i) {
 
 // Stage the nodal values of modes [c*L, c*L+L) of a (np, NC, M) array for
 // the E elements of this work-group: nod[(slot)*row + mi*NC + k].
-template <int NC, int MP>
+template <int NC, int MP, int NW>
 struct NodalFetch {
   static constexpr int TOT = GaussLds<MP>::E * 6 * NC * (MP / 4);
-  static constexpr int NT = 64 * GAUSS_WAVES;
+  static constexpr int NT = 64 * NW;
   static constexpr int IT = (TOT + NT - 1) / NT;
   double v[IT];
 };
@@ -172,10 +180,10 @@ struct NodalFetch {
 // Issue the loads of chunk c (modes [c*L, c*L+L)) of a (np, NC, M) nodal array
 // for the E elements of this work-group into registers; unconditional loads at
 // clamped indices, so they all issue back to back and overlap other work.
-template <int NC, int MP>
+template <int NC, int MP, int NW>
 SFX_DEV void nodal_fetch(const GaussArgs& a, const double* X, int64_t nX, int64_t e0, int c, int tid,
-                         const int* nid, NodalFetch<NC, MP>& f) {
-  using NF_ = NodalFetch<NC, MP>;
+                         const int* nid, NodalFetch<NC, MP, NW>& f) {
+  using NF_ = NodalFetch<NC, MP, NW>;
   constexpr int L = MP / 4, S = GaussLds<MP>::E * 6;
 #pragma unroll
   for (int it = 0; it < NF_::IT; ++it) {
@@ -196,9 +204,9 @@ SFX_DEV void nodal_fetch(const GaussArgs& a, const double* X, int64_t nX, int64_
 }
 
 // Store a fetched chunk to LDS: nod[slot*row + mi*NC + k].
-template <int NC, int MP>
-SFX_DEV void nodal_store(const NodalFetch<NC, MP>& f, int tid, double* nod, int row) {
-  using NF_ = NodalFetch<NC, MP>;
+template <int NC, int MP, int NW>
+SFX_DEV void nodal_store(const NodalFetch<NC, MP, NW>& f, int tid, double* nod, int row) {
+  using NF_ = NodalFetch<NC, MP, NW>;
   constexpr int S = GaussLds<MP>::E * 6;
 #pragma unroll
   for (int it = 0; it < NF_::IT; ++it) {
@@ -288,10 +296,10 @@ SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* n
 // Per work-group tables in LDS: twiddles, reference-element tables, mode
 // numbers, node coordinates and node numbers of the E elements.  All loads
 // issue together (clamped indices), then the LDS stores.
-template <int MP>
+template <int MP, int NW>
 SFX_DEV void gauss_prologue(const GaussArgs& a, int tid, int64_t e0, double2* smem) {
-  using LY = GaussLds<MP>;
-  constexpr int E = LY::E, NT = 64 * GAUSS_WAVES;
+  using LY = GaussLds<MP, NW>;
+  constexpr int E = LY::E, NT = 64 * NW;
   double* smd = reinterpret_cast<double*>(smem);
   double2* tw = smem + LY::TW;
   double* tab = smd + 2 * LY::TAB;
@@ -340,16 +348,16 @@ SFX_DEV void gauss_prologue(const GaussArgs& a, int tid, int64_t e0, double2* sm
   }
 }
 
-template <int MP, int V>
+template <int MP, int V, int NW>
 SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
-  using LY = GaussLds<MP>;
-  constexpr int L = S::L, PW = S::PW, E = LY::E, NG = GAUSS_GPW;
+  using LY = GaussLds<MP, NW>;
+  constexpr int L = S::L, PW = S::PW, E = LY::E, NG = groups_per_wave(NW);
   constexpr int NSIG = GVT<V>::NSIG, NO = GVT<V>::NOUT;
   constexpr int NF = 4 * NSIG;
   constexpr int NOC = (NO + 1) / 2;
   constexpr int GX = 2;
-  constexpr int NT = 64 * GAUSS_WAVES;
+  constexpr int NT = 64 * NW;
   const int wave = tid >> 6, lane = tid & 63;
   const int pt = lane / L, l = lane % L;
   double* smd = reinterpret_cast<double*>(smem);
@@ -361,7 +369,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   double2* geo = smem + LY::GEO;
   int* nid = reinterpret_cast<int*>(smem + LY::NID);
   const int64_t e0 = bid * E;
-  gauss_prologue<MP>(a, tid, e0, smem);
+  gauss_prologue<MP, NW>(a, tid, e0, smem);
   static_assert(126 <= NT && MP <= NT, "prologue assumes one table entry per thread");
   SFX_SYNC();
   const double2* wN = tw;
@@ -375,7 +383,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   double hc[NG];
 #pragma unroll
   for (int gi = 0; gi < NG; ++gi) {
-    const int grp = wave + GAUSS_WAVES * gi;
+    const int grp = wave + NW * gi;
     has[gi] = grp < GAUSS_GROUPS;
     const int lp = (has[gi] ? grp : 0) * PW + pt;    // local point 0 .. 7E-1
     el[gi] = lp / 7;
@@ -433,10 +441,10 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         }
       }
       const bool useB = (V == GV_UGRADC) || (V == GV_PAIRS && XB != XA);
-      NodalFetch<6, MP> fa;
-      NodalFetch<NCB, MP> fb;
-      nodal_fetch<6, MP>(a, XA, nXA, e0, 0, tid, nid, fa);
-      if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, 0, tid, nid, fb);
+      NodalFetch<6, MP, NW> fa;
+      NodalFetch<NCB, MP, NW> fb;
+      nodal_fetch<6, MP, NW>(a, XA, nXA, e0, 0, tid, nid, fa);
+      if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, 0, tid, nid, fb);
       double* nodA = big;
       double* nodB = big + LY::NOD6;
 #pragma unroll
@@ -444,12 +452,12 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         // stage nodal chunk c once every wave is done with chunk c-1, then
         // start fetching chunk c+1 while chunk c is interpolated
         SFX_SYNC();
-        nodal_store<6, MP>(fa, tid, nodA, LY::ROW6);
-        if (useB) nodal_store<NCB, MP>(fb, tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
+        nodal_store<6, MP, NW>(fa, tid, nodA, LY::ROW6);
+        if (useB) nodal_store<NCB, MP, NW>(fb, tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
         SFX_SYNC();
         if (c < 3) {
-          nodal_fetch<6, MP>(a, XA, nXA, e0, c + 1, tid, nid, fa);
-          if (useB) nodal_fetch<NCB, MP>(a, XB, nXB, e0, c + 1, tid, nid, fb);
+          nodal_fetch<6, MP, NW>(a, XA, nXA, e0, c + 1, tid, nid, fa);
+          if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, c + 1, tid, nid, fb);
         }
         const int m = c * L + l;
 #pragma unroll

@@ -100,10 +100,11 @@ extern "C" int emu_fftprod(int op, const double* V1, const double* V2, double* V
 
 template <int MP, int V>
 static void emu_gauss_mp(const GaussArgs& a) {
-  constexpr int E = GaussLds<MP>::E;
+  constexpr int NW = gauss_waves<MP, V>();
+  constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (a.me + E - 1) / E;
-  run_grid(nblk, 64 * GAUSS_WAVES, GaussLds<MP>::TOTAL,
-           [&](int t, int64_t b, double2* s) { gauss_body<MP, V>(a, t, b, s); });
+  run_grid(nblk, 64 * NW, GaussLds<MP, NW>::TOTAL,
+           [&](int t, int64_t b, double2* s) { gauss_body<MP, V, NW>(a, t, b, s); });
 }
 
 template <int V>
@@ -210,10 +211,11 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
 
 template <int MP, bool SPLIT>
 static void emu_adjoint_mp(const AdjArgs& A) {
-  constexpr int E = GaussLds<MP>::E;
+  constexpr int NW = adjoint_waves<MP, SPLIT>();
+  constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (A.g.me + E - 1) / E;
-  run_grid(nblk, 64 * GAUSS_WAVES, AdjLds<MP, SPLIT>::TOTAL,
-           [&](int t, int64_t b, double2* s) { adjoint_body<MP, SPLIT>(A, t, b, s); });
+  run_grid(nblk, 64 * NW, AdjLds<MP, SPLIT, NW>::TOTAL,
+           [&](int t, int64_t b, double2* s) { adjoint_body<MP, SPLIT, NW>(A, t, b, s); });
 }
 
 template <bool SPLIT>
