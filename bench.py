@@ -9,8 +9,8 @@ launched on (torch's current stream, passed to every entry point), so each
 phase's average duration is measured live inside the timed region.
 ``roofline`` reports the dominant phase: its algorithmic HBM bytes per launch
 divided by its average duration, against MI355X's 8 TB/s.  ``cpu_baseline``
-times the numpy restatement of the reference algorithm (the oracle) on one
-host core on a bounded sample.
+times the CPU restatement of the reference algorithm (the oracle's C
+restatement, on one core and on all cores) on a bounded sample.
 
 Workloads (all FP64, synthetic seeded inputs, N(0,1)*exp(-m/decay) nodal
 coefficients, mode-0 sine set to zero):
@@ -306,14 +306,53 @@ WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_ta
 
 
 # ------------------------------------------------------------- CPU baseline
+def _cpu_threads() -> int:
+    """Host threads for the all-core CPU timing: the cores this process may
+    use, at most 16 (a one-GPU share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(kind, mesh, M, sample_s: float):
-    """The numpy oracle on one core, on a bounded sample of the same workload."""
+    """The CPU restatement of the reference algorithm on a bounded sample of
+    the same workload: the compiled C restatement (OpenMP over element
+    blocks) timed on one core and on all cores, or the numpy one on one core
+    when the C library is not built."""
+    from oracle import cpu as OC
     from oracle import fft_par as F
-    from oracle import smb
     from sfemans_amd.synth import fourier_field
-    t_end = time.perf_counter() + sample_s
+    if kind in ("nl", "mhd", "tangent", "batch", "full") and mesh is not None and OC.available():
+        from sfemans_amd.mesh import rectangle_p2
+        # forward nonlinear term on a 64x64 P2 mesh (8,192 triangles), same m_max
+        sm = rectangle_p2(64, 64)
+        u = fourier_field(sm.np, 6, M, 3, 4.0)
+        modes = np.arange(M, dtype=np.int32)
+        units = sm.n_gauss * M
+        out = np.empty((sm.n_gauss, 6, M), order="F")
+        nth = _cpu_threads()
+        rates = {}
+        for th in ((1, nth) if nth > 1 else (1,)):
+            OC.smb_cross_prod_gauss(sm.jj, sm.rr, modes, u, th, out)          # warm-up
+            times = []
+            t_end = time.perf_counter() + 0.5 * sample_s
+            while time.perf_counter() < t_end or len(times) < 3:
+                t0 = time.perf_counter()
+                OC.smb_cross_prod_gauss(sm.jj, sm.rr, modes, u, th, out)
+                times.append(time.perf_counter() - t0)
+            rates[th] = (units / float(np.median(times)), len(times))
+        nmax = max(rates)
+        return {"value": rates[nmax][0], "unit": "Gauss-pt*modes/s", "cores": nmax, "kind": "port",
+                "value_1core": rates[1][0],
+                "sample": f"smb_cross_prod_gauss_sft_par restated in C (OpenMP over element blocks, direct "
+                          f"transforms on the reference grid N={2 * (3 * M // 2) - 1}), 64x64 P2 mesh "
+                          f"({sm.me} triangles) x m_max={M}, median of {rates[nmax][1]} calls on {nmax} threads "
+                          f"and of {rates[1][1]} calls on 1 thread; forward term only"}
     if kind in ("nl", "mhd", "tangent", "batch", "full") and mesh is not None:
-        # forward nonlinear term on a slab of the mesh's first elements
+        from oracle import smb
+        t_end = time.perf_counter() + sample_s
         ne = 512
         jj = np.asfortranarray(mesh.jj[:, :ne])
         used = np.unique(jj)

@@ -26,6 +26,9 @@ SFEMaNS source tree):
   (SOURCES/SOLVER/fem_rhs_axi.F90:940-987) into dense local vectors.
 * ``linearized`` - tangent and adjoint of the bilinear maps (no reference
   counterpart; pinned by brute-force transposition on tiny inputs).
+* ``cpu``        - ctypes wrapper of ``c/sfx_cpu.c``, a compiled C restatement
+  (OpenMP over element blocks) of ``smb_cross_prod_gauss_sft_par`` on the
+  reference grid, checked against ``smb`` and timed as the CPU baseline.
 
 Parity pins (see ``tests/test_oracle_*.py``): the long-double direct DFT,
 single-mode closed forms, the TEST1 known answer
