@@ -324,7 +324,7 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
     from oracle import cpu as OC
     from oracle import fft_par as F
     from sfemans_amd.synth import fourier_field
-    if kind in ("nl", "mhd", "tangent", "batch", "full") and mesh is not None and OC.available():
+    if kind in ("nl", "mhd", "tangent", "batch", "full") and OC.available():
         from sfemans_amd.mesh import rectangle_p2
         # forward nonlinear term on a 64x64 P2 mesh (8,192 triangles), same m_max
         sm = rectangle_p2(64, 64)
@@ -372,6 +372,7 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
                 "sample": f"smb_cross_prod_gauss_sft_par restated in numpy (pocketfft, reference grid "
                           f"N={2 * (3 * M // 2) - 1}), {n} calls on {ne} triangles x m_max={M}, 1 thread, "
                           f"{dt:.1f} s; forward term only"}
+    t_end = time.perf_counter() + sample_s
     M = M or 64
     npt = 4096
     rng = np.random.default_rng(1)

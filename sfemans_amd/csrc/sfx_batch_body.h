@@ -27,7 +27,7 @@ struct BatchArgs {
 // one register allocation or one stack frame.
 template <int MP>
 SFX_DEV_NOINLINE void batch_tile(const GaussArgs& a, int tid, int64_t block, double2* smem) {
-  gauss_body<MP, GV_CROSS_CURL, GAUSS_WAVES>(a, tid, block, smem);
+  gauss_body<MP, GV_CROSS_CURL, GAUSS_WAVES, true>(a, tid, block, smem);
 }
 
 // Work-group `wg`: every tile of its list, one after the other.  The barrier
@@ -58,13 +58,13 @@ SFX_DEV void batch_body(const BatchArgs& b, int tid, int wg, double2* smem) {
 // needs the largest value over the grid sizes present (not monotonic in mp).
 inline size_t batch_smem_cplx(int mp) {
   switch (mp) {
-    case 4: return GaussLds<4>::TOTAL;
-    case 8: return GaussLds<8>::TOTAL;
-    case 16: return GaussLds<16>::TOTAL;
-    case 32: return GaussLds<32>::TOTAL;
-    case 64: return GaussLds<64>::TOTAL;
-    case 128: return GaussLds<128>::TOTAL;
-    case 256: return GaussLds<256>::TOTAL;
+    case 4: return GaussCfg<4, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 8: return GaussCfg<8, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 16: return GaussCfg<16, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 32: return GaussCfg<32, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 64: return GaussCfg<64, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 128: return GaussCfg<128, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 256: return GaussCfg<256, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
   }
   return 0;
 }

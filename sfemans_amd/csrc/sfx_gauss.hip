@@ -18,7 +18,7 @@ static hipError_t launch_mp(const GaussArgs& a, hipStream_t st) {
   constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (a.me + E - 1) / E;
   if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)GaussLds<MP, NW>::TOTAL * sizeof(double2);
+  const size_t shmem = (size_t)GaussCfg<MP, V, NW>::LY::TOTAL * sizeof(double2);
   hipLaunchKernelGGL((gauss_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
   return hipGetLastError();
 }

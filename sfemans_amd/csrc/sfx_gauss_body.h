@@ -51,8 +51,10 @@ constexpr int gauss_waves() { return (V == GV_UGRADC || (V == GV_CROSS_CURL && M
 constexpr int groups_per_wave(int nw) { return (GAUSS_GROUPS + nw - 1) / nw; }
 constexpr int waves_per_simd(int nw) { return nw > 4 ? 2 : 1; }
 
-// LDS layout (double2 units) of the fused kernel with NW waves.
-template <int MP, int NW = GAUSS_WAVES>
+// LDS layout (double2 units) of the fused kernel with NW waves.  The nodal
+// staging holds NCH chunks (1: one chunk at a time; 4: all modes at once)
+// of a 6-column field and a second field of NCB columns (6, 2 or 0).
+template <int MP, int NW = GAUSS_WAVES, int NCH = 1, int NCB = 6>
 struct GaussLds {
   static constexpr int E = 256 / MP;                    // elements per work-group
   static constexpr int L = MP / 4;
@@ -69,10 +71,28 @@ struct GaussLds {
   // per-wave region in doubles: 2 exchanged signals (2 x 256 complex) or the
   // 64 lanes x 16 doubles of the partner swap, whichever is larger
   static constexpr int RW = (4 * Shape<MP>::WAVE_CPLX > 64 * 16) ? 4 * Shape<MP>::WAVE_CPLX : 64 * 16;
-  // nodal staging (two fields) and the wave regions do not overlap
-  static constexpr int NODAL = 2 * NOD6;                // doubles
+  // nodal staging and the wave regions do not overlap
+  static constexpr int NODC = NOD6 + (NCB == 6 ? NOD6 : NCB == 2 ? NOD2 : 0);   // doubles per chunk
+  static constexpr int NODAL = NCH * NODC;              // doubles
   static constexpr int XCH = NODAL;                     // doubles from BIG: first wave region
   static constexpr int TOTAL = BIG + (NODAL + NW * RW + 1) / 2;
+};
+
+// LDS per CU (bytes) the work-group of the fused kernel may use.
+constexpr int GAUSS_LDS_BYTES = 160 * 1024;
+
+// Layout of the fused kernel for variant V.  By default the nodal data is
+// staged one chunk at a time with the next chunk prefetched into registers,
+// so its loads overlap the interpolation of the current one.  With
+// STAGE_ALL (used by the batched launch, measured faster there) all four
+// chunks are staged at once where that fits in LDS and the variant makes a
+// single pass over its fields: one barrier, every load in flight together.
+template <int MP, int V, int NW, bool STAGE_ALL = false>
+struct GaussCfg {
+  static constexpr int NCB = (V == GV_UGRADC) ? 2 : (V == GV_PAIRS ? 6 : 0);
+  static constexpr bool ALL =
+      STAGE_ALL && V != GV_PAIRS && GaussLds<MP, NW, 4, NCB>::TOTAL * 16 <= GAUSS_LDS_BYTES;
+  using LY = GaussLds<MP, NW, ALL ? 4 : 1, ALL ? NCB : 6>;
 };
 
 struct GaussGeom {
@@ -348,10 +368,11 @@ SFX_DEV void gauss_prologue(const GaussArgs& a, int tid, int64_t e0, double2* sm
   }
 }
 
-template <int MP, int V, int NW>
+template <int MP, int V, int NW, bool STAGE_ALL = false>
 SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
-  using LY = GaussLds<MP, NW>;
+  using LY = typename GaussCfg<MP, V, NW, STAGE_ALL>::LY;
+  constexpr bool ALL = GaussCfg<MP, V, NW, STAGE_ALL>::ALL;
   constexpr int L = S::L, PW = S::PW, E = LY::E, NG = groups_per_wave(NW);
   constexpr int NSIG = GVT<V>::NSIG, NO = GVT<V>::NOUT;
   constexpr int NF = 4 * NSIG;
@@ -441,23 +462,40 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         }
       }
       const bool useB = (V == GV_UGRADC) || (V == GV_PAIRS && XB != XA);
-      NodalFetch<6, MP, NW> fa;
-      NodalFetch<NCB, MP, NW> fb;
-      nodal_fetch<6, MP, NW>(a, XA, nXA, e0, 0, tid, nid, fa);
-      if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, 0, tid, nid, fb);
-      double* nodA = big;
-      double* nodB = big + LY::NOD6;
+      NodalFetch<6, MP, NW> fa[ALL ? 4 : 1];
+      NodalFetch<NCB, MP, NW> fb[ALL ? 4 : 1];
+      if constexpr (ALL) {
+        // every chunk at once: all loads issue back to back, one barrier
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          nodal_fetch<6, MP, NW>(a, XA, nXA, e0, c, tid, nid, fa[c]);
+          if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, c, tid, nid, fb[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          nodal_store<6, MP, NW>(fa[c], tid, big + c * LY::NODC, LY::ROW6);
+          if (useB) nodal_store<NCB, MP, NW>(fb[c], tid, big + c * LY::NODC + LY::NOD6, NCB == 6 ? LY::ROW6 : LY::ROW2);
+        }
+        SFX_SYNC();
+      } else {
+        nodal_fetch<6, MP, NW>(a, XA, nXA, e0, 0, tid, nid, fa[0]);
+        if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, 0, tid, nid, fb[0]);
+      }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        // stage nodal chunk c once every wave is done with chunk c-1, then
-        // start fetching chunk c+1 while chunk c is interpolated
-        SFX_SYNC();
-        nodal_store<6, MP, NW>(fa, tid, nodA, LY::ROW6);
-        if (useB) nodal_store<NCB, MP, NW>(fb, tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
-        SFX_SYNC();
-        if (c < 3) {
-          nodal_fetch<6, MP, NW>(a, XA, nXA, e0, c + 1, tid, nid, fa);
-          if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, c + 1, tid, nid, fb);
+        double* nodA = big + (ALL ? c * LY::NODC : 0);
+        double* nodB = nodA + LY::NOD6;
+        if constexpr (!ALL) {
+          // stage nodal chunk c once every wave is done with chunk c-1, then
+          // start fetching chunk c+1 while chunk c is interpolated
+          SFX_SYNC();
+          nodal_store<6, MP, NW>(fa[0], tid, nodA, LY::ROW6);
+          if (useB) nodal_store<NCB, MP, NW>(fb[0], tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
+          SFX_SYNC();
+          if (c < 3) {
+            nodal_fetch<6, MP, NW>(a, XA, nXA, e0, c + 1, tid, nid, fa[0]);
+            if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, c + 1, tid, nid, fb[0]);
+          }
         }
         const int m = c * L + l;
 #pragma unroll

@@ -103,7 +103,7 @@ static void emu_gauss_mp(const GaussArgs& a) {
   constexpr int NW = gauss_waves<MP, V>();
   constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (a.me + E - 1) / E;
-  run_grid(nblk, 64 * NW, GaussLds<MP, NW>::TOTAL,
+  run_grid(nblk, 64 * NW, GaussCfg<MP, V, NW>::LY::TOTAL,
            [&](int t, int64_t b, double2* s) { gauss_body<MP, V, NW>(a, t, b, s); });
 }
 
