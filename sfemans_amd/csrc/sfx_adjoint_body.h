@@ -315,39 +315,60 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       }
     }
     SFX_SYNC();
-    // work item (element, local node n, mode offset mi): six components;
-    // mode fastest, so that consecutive lanes store consecutive modes of one
-    // entry (elo layout: entry, component, mode)
+    // Split form: work item (element, component k, mode offset mi) for the
+    // six local nodes of the element, so the item reads its 28 weights from
+    // LDS once for all six nodes.  Convective form: work item (element,
+    // local node n, mode offset mi) for the six components (its 18 weights
+    // per point are fewer LDS reads than the shape tables of six nodes).
+    // Either way each sum runs over the 7 Gauss points in order, and mode is
+    // fastest: consecutive lanes store consecutive modes of one entry (elo
+    // layout: entry, component, mode).
     for (int it = tid; it < E * 6 * L; it += NT) {
       const int mi = it % L;
-      const int n = (it / L) % 6;
+      const int nk = (it / L) % 6;
       const int e = it / (6 * L);
       const int mm = mi + L * r;
       const int64_t eg = e0 + e;
       if (eg >= a.me || mm >= a.M) continue;
       double acc[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
       constexpr int RP = AL::SK;
-      for (int lg2 = 0; lg2 < 7; ++lg2) {
-        const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
-        const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
-        if (SPLIT) {
+      if constexpr (SPLIT) {
+        const int k = nk;
 #pragma unroll
-          for (int k = 0; k < 6; ++k) {
-            acc[k] = fma(w, src[(4 * k) * RP], acc[k]);
-            acc2[k] = fma(w, src[(4 * k + 1) * RP], fma(dx, src[(4 * k + 2) * RP], fma(dy, src[(4 * k + 3) * RP], acc2[k])));
+        for (int lg2 = 0; lg2 < 7; ++lg2) {
+          const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
+          double v[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[j] = src[(4 * k + j) * RP];
+#pragma unroll
+          for (int n = 0; n < 6; ++n) {
+            const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
+            acc[n] = fma(w, v[0], acc[n]);
+            acc2[n] = fma(w, v[1], fma(dx, v[2], fma(dy, v[3], acc2[n])));
           }
-        } else {
+        }
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+          const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
+          SFX_CHECK("elo store", o, A.nelo);
+          A.elo[o] = acc[n];
+          A.elo2[o] = acc2[n];
+        }
+      } else {
+        const int n = nk;
+        for (int lg2 = 0; lg2 < 7; ++lg2) {
+          const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
+          const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
 #pragma unroll
           for (int k = 0; k < 6; ++k)
             acc[k] = fma(w, src[(3 * k) * RP], fma(dx, src[(3 * k + 1) * RP], fma(dy, src[(3 * k + 2) * RP], acc[k])));
         }
-      }
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
-        SFX_CHECK("elo store", o, A.nelo);
-        A.elo[o] = acc[k];
-        if (SPLIT) A.elo2[o] = acc2[k];
+        for (int k = 0; k < 6; ++k) {
+          const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
+          SFX_CHECK("elo store", o, A.nelo);
+          A.elo[o] = acc[k];
+        }
       }
     }
   }
