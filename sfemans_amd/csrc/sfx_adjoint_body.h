@@ -63,15 +63,20 @@ struct AdjLds {
   static constexpr int TOTAL = LY::BIG + (BIGD + 1) / 2;   // double2 units
 };
 
-// Load ybar at own and partner modes of Gauss point p, scaled for Phi^T, and
-// pack the three real output fields as signals (pr, pth) and (pz, 0).
-template <int MP>
-SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (&Z)[2][4], double2 (&B)[2][4]) {
+// Load ybar at the own and partner modes (MP - m) of Gauss point p, scale
+// for Phi^T and pack the three real output fields as signals (pr, pth) and
+// (pz, 0).  SWAP: each lane loads its own modes only and takes the partner
+// modes from the lanes that loaded them, through the wave's LDS region xf
+// (half the global loads; measured faster with 512 registers per wave,
+// slower with 256, where the extra live values spill).
+template <int MP, bool SWAP>
+SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, double* xf,
+                       double2 (&Z)[2][4], double2 (&B)[2][4]) {
   constexpr int L = MP / 4, N = 3 * MP;
   const GaussArgs& a = A.g;
   const int64_t pc = valid ? p : 0;
   const double two_n = 2.0 / N, one_n = 1.0 / N;
-  double v[4][2][6];   // [slot][own/partner][a_r, b_r, a_th, b_th, a_z, b_z]
+  double v[4][6], w[4][6];   // own and partner [slot][a_r, b_r, a_th, b_th, a_z, b_z]
   // the plane offsets are the same for every group: kept live between calls
   // they would be spilled, so they are recomputed each time
   int lo = l;
@@ -84,12 +89,37 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const int64_t i1 = pc + a.G * (c + 6 * (int64_t)mc);
-      const int64_t i2 = pc + a.G * (c + 6 * (int64_t)kc);
       SFX_CHECK("ybar load", i1, A.nybar);
-      SFX_CHECK("ybar load", i2, A.nybar);
-      v[r][0][c] = A.ybar[i1];
-      v[r][1][c] = A.ybar[i2];
+      v[r][c] = A.ybar[i1];
+      if (!SWAP) {
+        const int64_t i2 = pc + a.G * (c + 6 * (int64_t)kc);
+        SFX_CHECK("ybar load", i2, A.nybar);
+        w[r][c] = A.ybar[i2];
+      }
     }
+  }
+  if (SWAP) {
+    // mode MP - m of slot r is held by lane (L - l) % L in slot 3 - r (l >= 1)
+    // or 4 - r (l = 0, r >= 1); layout xf[(6 slot + c) * 64 + lane]
+    const int lane = pt * L + l, plane = pt * L + (L - l) % L;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        SFX_CHECK("ybar swap", (6 * r + c) * 64 + lane, 64 * 24);
+        xf[(6 * r + c) * 64 + lane] = v[r][c];
+      }
+    SFX_WAVE_SYNC();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rp = (l != 0) ? 3 - r : ((4 - r) & 3);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        SFX_CHECK("ybar swap", (6 * rp + c) * 64 + plane, 64 * 24);
+        w[r][c] = xf[(6 * rp + c) * 64 + plane];
+      }
+    }
+    SFX_WAVE_SYNC();
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -100,7 +130,7 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int l, double2 (
     const double sp = par ? two_n : 0.0;
     double o[6], q[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) { o[c] = so * v[r][0][c]; q[c] = sp * v[r][1][c]; }
+    for (int c = 0; c < 6; ++c) { o[c] = so * v[r][c]; q[c] = sp * w[r][c]; }
     if (m == 0) { o[1] = 0; o[3] = 0; o[5] = 0; }
     // signal 0: fields (r, theta); signal 1: (z, 0)
     Z[0][r] = make_double2(m == 0 ? o[0] : 0.5 * (o[0] + o[3]), m == 0 ? o[2] : 0.5 * (o[2] - o[1]));
@@ -213,7 +243,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     double pb[3][4][3];
     {
       double2 Zy[2][4], By[2][4];
-      load_ybar<MP>(A, p[gi], valid[gi], l, Zy, By);
+      load_ybar<MP, (NW <= 4)>(A, p[gi], valid[gi], pt, l, xf, Zy, By);
       double2 y[6][4];
 #pragma unroll
       for (int s = 0; s < 2; ++s) prestep<MP>(Zy[s], By[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
