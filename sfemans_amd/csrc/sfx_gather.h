@@ -19,7 +19,7 @@ namespace sfx {
 
 constexpr int GATHER_NODES = 16;
 constexpr int GATHER_THREADS = 256;
-constexpr int GATHER_MCHUNK = 64;
+constexpr int GATHER_MCHUNK = 32;
 
 // LDS doubles of the tile for NK components.
 template <int NK>

@@ -42,6 +42,7 @@ hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st) {
     case GV_CROSS_CURL: return launch_v<GV_CROSS_CURL>(a, st);
     case GV_PAIRS: return launch_v<GV_PAIRS>(a, st);
     case GV_UGRADC: return launch_v<GV_UGRADC>(a, st);
+    case GV_CURL_CROSS: return launch_v<GV_CURL_CROSS>(a, st);
   }
   return hipErrorInvalidValue;
 }

@@ -8,6 +8,7 @@
 //   GV_CURLH_B     smb_CurlH_cross_B_gauss_sft_par (:632-727): V_out = (curl H) x B
 //   GV_UGRADC      smb_ugradc_gauss_fft_par (subroutine_temperature_with_T.F90:247-350):
 //                  c_out = (hc grad c).u + (div u)(hc c)
+//   GV_CURL_CROSS  one pass coef * (curl pc) x pv, stored or added to V_out
 //   GV_PAIRS       sum over passes of coef * (curl pc) x pv: (curl H) x B
 //                  (smb_CurlH_cross_B_gauss_sft_par, :632-727); the MHD term
 //                  rotv_v = (curl u) x u - c_L (curl H) x B formed at
@@ -37,6 +38,7 @@ template <int V> struct GVT;
 // count a.npass for GV_PAIRS: each pass adds coef * (curl pc) x pv).
 template <> struct GVT<GV_CROSS_CURL> { static constexpr int NSIG = 3, NOUT = 3, MAXP = 1; };
 template <> struct GVT<GV_PAIRS>      { static constexpr int NSIG = 3, NOUT = 3, MAXP = 4; };
+template <> struct GVT<GV_CURL_CROSS> { static constexpr int NSIG = 3, NOUT = 3, MAXP = 1; };
 template <> struct GVT<GV_UGRADC>     { static constexpr int NSIG = 4, NOUT = 1, MAXP = 1; };
 
 constexpr int GAUSS_GROUPS = 7;   // point groups per work-group
@@ -47,7 +49,9 @@ constexpr int GAUSS_WAVES = 4;    // default wave count (batched launches)
 // groups in 512 registers at one wave per SIMD.  7 where one group's state
 // fits in 256 registers (measured faster there), 4 elsewhere.
 template <int MP, int V>
-constexpr int gauss_waves() { return (V == GV_UGRADC || (V == GV_CROSS_CURL && MP <= 64)) ? 7 : 4; }
+constexpr int gauss_waves() {
+  return (V == GV_UGRADC || ((V == GV_CROSS_CURL || V == GV_CURL_CROSS) && MP <= 64)) ? 7 : 4;
+}
 constexpr int groups_per_wave(int nw) { return (GAUSS_GROUPS + nw - 1) / nw; }
 constexpr int waves_per_simd(int nw) { return nw > 4 ? 2 : 1; }
 
@@ -89,7 +93,7 @@ constexpr int GAUSS_LDS_BYTES = 160 * 1024;
 // single pass over its fields: one barrier, every load in flight together.
 template <int MP, int V, int NW, bool STAGE_ALL = false>
 struct GaussCfg {
-  static constexpr int NCB = (V == GV_UGRADC) ? 2 : (V == GV_PAIRS ? 6 : 0);
+  static constexpr int NCB = (V == GV_UGRADC) ? 2 : ((V == GV_PAIRS || V == GV_CURL_CROSS) ? 6 : 0);
   static constexpr bool ALL =
       STAGE_ALL && V != GV_PAIRS && GaussLds<MP, NW, 4, NCB>::TOTAL * 16 <= GAUSS_LDS_BYTES;
   using LY = GaussLds<MP, NW, ALL ? 4 : 1, ALL ? NCB : 6>;
@@ -282,7 +286,7 @@ SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* n
                          double mval, const GaussGeom& g, double hc, double (&F)[4 * GVT<V>::NSIG]) {
   using LY = GaussLds<MP>;
   const double mr = mval * g.iray;
-  if constexpr (V == GV_CROSS_CURL || V == GV_PAIRS) {
+  if constexpr (V == GV_CROSS_CURL || V == GV_PAIRS || V == GV_CURL_CROSS) {
     double W[6], Dr[6], Dz[6], R[6];
     interp_mode<6, MP>(nodA, s0, LY::ROW6, mi, g, W, Dr, Dz);
     curl6(W, Dr, Dz, mr, g.iray, R);
@@ -291,7 +295,7 @@ SFX_DEV void pass_fields(const GaussArgs& a, const double* nodA, const double* n
       if (mval == 1.0) { R[0] += a.om[0]; R[3] += a.om[1]; }
       if (mval == 0.0) { R[4] += a.om[2]; }
     }
-    if (V == GV_PAIRS && nodB != nullptr) {
+    if ((V == GV_PAIRS || V == GV_CURL_CROSS) && nodB != nullptr) {
       double Drb[6], Dzb[6];
       interp_mode<6, MP>(nodB, s0, LY::ROW6, mi, g, W, Drb, Dzb);
     }
@@ -450,6 +454,9 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
       const double* XB = a.U2;
       int64_t nXA = a.nU, nXB = a.nU2;
       double coef = 1.0;
+      if (V == GV_CURL_CROSS) {
+        XA = a.pc[0]; XB = a.pv[0]; nXA = a.npc[0]; nXB = a.npv[0]; coef = a.coef[0];
+      }
       if (V == GV_PAIRS) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -461,7 +468,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           coef = on ? a.coef[k] : coef;
         }
       }
-      const bool useB = (V == GV_UGRADC) || (V == GV_PAIRS && XB != XA);
+      const bool useB = (V == GV_UGRADC) || ((V == GV_PAIRS || V == GV_CURL_CROSS) && XB != XA);
       NodalFetch<6, MP, NW> fa[ALL ? 4 : 1];
       NodalFetch<NCB, MP, NW> fb[ALL ? 4 : 1];
       if constexpr (ALL) {
@@ -584,8 +591,13 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
             if (m >= a.M) continue;
             const int64_t base = pg + a.G * (int64_t)(2 * o + 2 * NO * m);
             SFX_CHECK("gauss out", base + a.G, a.nout);
-            a.Vout[base] = oa[t][h][r];
-            a.Vout[base + a.G] = ob[t][h][r];
+            if (V == GV_CURL_CROSS && a.accum) {
+              a.Vout[base] += oa[t][h][r];
+              a.Vout[base + a.G] += ob[t][h][r];
+            } else {
+              a.Vout[base] = oa[t][h][r];
+              a.Vout[base + a.G] = ob[t][h][r];
+            }
           }
         }
     }

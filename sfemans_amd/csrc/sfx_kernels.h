@@ -5,7 +5,7 @@
 namespace sfx {
 
 enum { OP_CROSS = 0, OP_DOT = 1, OP_PROD = 2, OP_SVECT1 = 3, OP_TENSOR = 4 };
-enum { GV_CROSS_CURL = 0, GV_PAIRS = 1, GV_UGRADC = 2 };
+enum { GV_CROSS_CURL = 0, GV_PAIRS = 1, GV_UGRADC = 2, GV_CURL_CROSS = 3 };
 
 // Twiddles for one power-of-two MP: wN[j] = e^{2 pi i j/(3 MP)}, wMP[j] = e^{2 pi i j/MP}.
 struct Twiddles {
@@ -41,6 +41,7 @@ struct GaussArgs {
   int M, MP;
   double c_lorentz;
   int npass;                // GV_PAIRS: passes, each adds coef * (curl pc) x pv
+  int accum;                // GV_CURL_CROSS: 1 adds the result to Vout, 0 stores it
   const double* pc[4];
   const double* pv[4];
   int64_t npc[4], npv[4];
@@ -60,6 +61,12 @@ hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);
 hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
 #endif
+
+// Pass lists (GV_PAIRS) run as one single-pass GV_CURL_CROSS launch per
+// pass, each adding to the output, where that kernel uses seven waves of 256
+// registers (MP <= 64, measured faster than one launch of the pass list,
+// whose accumulator needs 512 registers per wave); as one launch otherwise.
+inline bool pairs_as_passes(int MP) { return MP <= 64; }
 
 // Smallest supported power of two >= max(M, 4); 0 if M > 256.
 inline int mp_for(int M) {

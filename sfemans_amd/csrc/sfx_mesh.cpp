@@ -148,6 +148,19 @@ int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* l
   if (rc) return rc;
   a.wN = tw.wN; a.wMP = tw.wMP;
   if (mesh->me == 0) return SFX_OK;
+  if (variant == GV_PAIRS && pairs_as_passes(a.MP)) {
+    // one launch per pass, the first storing, the others adding to the output
+    const GaussArgs all = a;
+    for (int i = 0; i < all.npass; ++i) {
+      a.npass = 1;
+      a.pc[0] = all.pc[i]; a.pv[0] = all.pv[i]; a.coef[0] = all.coef[i];
+      a.npc[0] = all.npc[i]; a.npv[0] = all.npv[i];
+      a.accum = i > 0;
+      hipError_t e = launch_gauss(GV_CURL_CROSS, a, st);
+      if (e != hipSuccess) return hip_fail(e, nm);
+    }
+    return SFX_OK;
+  }
   hipError_t e = launch_gauss(variant, a, st);
   if (e != hipSuccess) return hip_fail(e, nm);
   return SFX_OK;

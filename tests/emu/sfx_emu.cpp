@@ -151,6 +151,18 @@ extern "C" int emu_gauss(int variant, const int32_t* jj, const double* rr, int64
   std::vector<double2> wN, wMP;
   host_twiddles(a.MP, wN, wMP);
   a.wN = wN.data(); a.wMP = wMP.data();
+  if (kernel == GV_PAIRS && pairs_as_passes(a.MP)) {
+    // as the library: one single-pass launch per pass, adding to the output
+    const GaussArgs all = a;
+    for (int i = 0; i < all.npass; ++i) {
+      a.npass = 1;
+      a.pc[0] = all.pc[i]; a.pv[0] = all.pv[i]; a.coef[0] = all.coef[i];
+      a.npc[0] = all.npc[i]; a.npv[0] = all.npv[i];
+      a.accum = i > 0;
+      if (emu_gauss_v<GV_CURL_CROSS>(a)) return 1;
+    }
+    return 0;
+  }
   switch (kernel) {
     case GV_CROSS_CURL: return emu_gauss_v<GV_CROSS_CURL>(a);
     case GV_PAIRS: return emu_gauss_v<GV_PAIRS>(a);
