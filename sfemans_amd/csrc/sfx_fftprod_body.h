@@ -53,10 +53,12 @@ SFX_DEV void apply_op(const double (&u)[6], int g, double (&o)[3]) {
 }
 
 // Packed spectra of the complex signal made of fields (f1, f2) at point p:
-// Z at own modes m = l + L r, B (partner part) from mode MP - m.
+// Z at own modes m = l + L r, B (partner part) from mode MP - m.  Each lane
+// loads its own modes only; the partner modes come from the lane that
+// loaded them, through the wave's LDS region xf (64 x 16 doubles).
 template <int MP, int OP>
-SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int l, int f1, int f2,
-                         double2 (&Z)[4], double2 (&B)[4]) {
+SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int pt, int l, int f1, int f2,
+                         double* xf, double2 (&Z)[4], double2 (&B)[4]) {
   constexpr int L = Shape<MP>::L;
   int c1, c2, nb1, nb2;
   int64_t n1, n2;
@@ -67,21 +69,37 @@ SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int l, int
   // so that all of them issue back to back instead of one branch at a time.
   const int64_t pc = valid ? p : 0;
   const int64_t o1 = pc + a.np * c1, o2 = pc + a.np * c2;
-  double v[4][8];
+  double v[4][4], w[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int m = l + L * r;
-    const int k = MP - m;
     const int mc = (m < a.M) ? m : 0;
-    const int kc = (m > 0 && k < a.M) ? k : 0;
-    const int64_t i1 = o1 + s1 * mc, i2 = o2 + s2 * mc, j1 = o1 + s1 * kc, j2 = o2 + s2 * kc;
+    const int64_t i1 = o1 + s1 * mc, i2 = o2 + s2 * mc;
     SFX_CHECK("V1 load", i1 + a.np, n1);
     SFX_CHECK("V2 load", i2 + a.np, n2);
-    SFX_CHECK("V1 load", j1 + a.np, n1);
-    SFX_CHECK("V2 load", j2 + a.np, n2);
     v[r][0] = b1[i1]; v[r][1] = b1[i1 + a.np]; v[r][2] = b2[i2]; v[r][3] = b2[i2 + a.np];
-    v[r][4] = b1[j1]; v[r][5] = b1[j1 + a.np]; v[r][6] = b2[j2]; v[r][7] = b2[j2 + a.np];
   }
+  // mode MP - m of slot r is held by lane (L - l) % L in slot 3 - r (l >= 1)
+  // or 4 - r (l = 0, r >= 1); layout xf[(4 slot + j) * 64 + lane]
+  const int lane = pt * L + l, plane = pt * L + (L - l) % L;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      SFX_CHECK("signal swap", (4 * r + j) * 64 + lane, 64 * 16);
+      xf[(4 * r + j) * 64 + lane] = v[r][j];
+    }
+  SFX_WAVE_SYNC();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rp = (l != 0) ? 3 - r : ((4 - r) & 3);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      SFX_CHECK("signal swap", (4 * rp + j) * 64 + plane, 64 * 16);
+      w[r][j] = xf[(4 * rp + j) * 64 + plane];
+    }
+  }
+  SFX_WAVE_SYNC();
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int m = l + L * r;
@@ -90,8 +108,8 @@ SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int l, int
     const bool par = valid && m > 0 && k < a.M;
     const double a1 = own ? v[r][0] : 0.0, bb1 = own ? v[r][1] : 0.0;
     const double a2 = own ? v[r][2] : 0.0, bb2 = own ? v[r][3] : 0.0;
-    const double ka1 = par ? v[r][4] : 0.0, kb1 = par ? v[r][5] : 0.0;
-    const double ka2 = par ? v[r][6] : 0.0, kb2 = par ? v[r][7] : 0.0;
+    const double ka1 = par ? w[r][0] : 0.0, kb1 = par ? w[r][1] : 0.0;
+    const double ka2 = par ? w[r][2] : 0.0, kb2 = par ? w[r][3] : 0.0;
     if (m == 0) Z[r] = make_double2(a1, a2);
     else Z[r] = make_double2(0.5 * (a1 + bb2), 0.5 * (a2 - bb1));
     B[r] = make_double2(0.5 * (ka1 - kb2), 0.5 * (kb1 + ka2));
@@ -210,7 +228,7 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     double2 Z[4], B[4];
-    load_signal<MP, OP>(a, p, valid, l, 2 * s, 2 * s + 1, Z, B);
+    load_signal<MP, OP>(a, p, valid, pt, l, 2 * s, 2 * s + 1, reinterpret_cast<double*>(lds), Z, B);
     prestep<MP>(Z, B, l, y[3 * s + 0], y[3 * s + 1], y[3 * s + 2], wN);
   }
   inverse_fft<MP, 3 * NS, GX>(y, pt, l, lds, wMP);
