@@ -76,11 +76,28 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
   const GaussArgs& a = A.g;
   const int64_t pc = valid ? p : 0;
   const double two_n = 2.0 / N, one_n = 1.0 / N;
-  double v[4][6], w[4][6];   // own and partner [slot][a_r, b_r, a_th, b_th, a_z, b_z]
   // the plane offsets are the same for every group: kept live between calls
   // they would be spilled, so they are recomputed each time
   int lo = l;
   SFX_OPAQUE(lo);
+  // own (v) and partner (w) values [slot][a_r, b_r, a_th, b_th, a_z, b_z]
+  double v[4][6], w[4][6];
+  auto pack = [&](int r) {
+    const int m = l + L * r, k = MP - m;
+    const bool own = valid && m < a.M;
+    const bool par = valid && m > 0 && k < a.M;
+    const double so = own ? (m == 0 ? one_n : two_n) : 0.0;
+    const double sp = par ? two_n : 0.0;
+    double o[6], q[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { o[c] = so * v[r][c]; q[c] = sp * w[r][c]; }
+    if (m == 0) { o[1] = 0; o[3] = 0; o[5] = 0; }
+    // signal 0: fields (r, theta); signal 1: (z, 0)
+    Z[0][r] = make_double2(m == 0 ? o[0] : 0.5 * (o[0] + o[3]), m == 0 ? o[2] : 0.5 * (o[2] - o[1]));
+    B[0][r] = make_double2(0.5 * (q[0] - q[3]), 0.5 * (q[1] + q[2]));
+    Z[1][r] = make_double2(m == 0 ? o[4] : 0.5 * o[4], m == 0 ? 0.0 : -0.5 * o[5]);
+    B[1][r] = make_double2(0.5 * q[4], 0.5 * q[5]);
+  };
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int m = lo + L * r, k = MP - m;
@@ -96,6 +113,14 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
         SFX_CHECK("ybar load", i2, A.nybar);
         w[r][c] = A.ybar[i2];
       }
+    }
+    if (!SWAP && (r & 1)) {
+      // without SWAP: two slots loaded and packed at a time, so that at most
+      // 24 loaded values are live (all 48 at once would push the caller's
+      // live spectra out to scratch)
+      pack(r - 1);
+      pack(r);
+      SFX_SCHED_BARRIER();
     }
   }
   if (SWAP) {
@@ -120,23 +145,8 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
       }
     }
     SFX_WAVE_SYNC();
-  }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int m = l + L * r, k = MP - m;
-    const bool own = valid && m < a.M;
-    const bool par = valid && m > 0 && k < a.M;
-    const double so = own ? (m == 0 ? one_n : two_n) : 0.0;
-    const double sp = par ? two_n : 0.0;
-    double o[6], q[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) { o[c] = so * v[r][c]; q[c] = sp * w[r][c]; }
-    if (m == 0) { o[1] = 0; o[3] = 0; o[5] = 0; }
-    // signal 0: fields (r, theta); signal 1: (z, 0)
-    Z[0][r] = make_double2(m == 0 ? o[0] : 0.5 * (o[0] + o[3]), m == 0 ? o[2] : 0.5 * (o[2] - o[1]));
-    B[0][r] = make_double2(0.5 * (q[0] - q[3]), 0.5 * (q[1] + q[2]));
-    Z[1][r] = make_double2(m == 0 ? o[4] : 0.5 * o[4], m == 0 ? 0.0 : -0.5 * o[5]);
-    B[1][r] = make_double2(0.5 * q[4], 0.5 * q[5]);
+    for (int r = 0; r < 4; ++r) pack(r);
   }
 }
 

@@ -21,6 +21,7 @@ void sfx_emu_wave_sync();
 #define SFX_SYNC() sfx_emu_sync()
 #define SFX_WAVE_SYNC() sfx_emu_wave_sync()
 #define SFX_OPAQUE(x) do { } while (0)
+#define SFX_SCHED_BARRIER() do { } while (0)
 #define SFX_CHECK(what, idx, n) \
   do { if ((long long)(idx) < 0 || (long long)(idx) >= (long long)(n)) sfx_emu_oob(what, (long long)(idx), (long long)(n), __FILE__, __LINE__); } while (0)
 #else
@@ -42,4 +43,7 @@ void sfx_emu_wave_sync();
 // is recomputed where it is used instead of being kept live (or spilled)
 // across a long stretch of code.  Emits no instruction.
 #define SFX_OPAQUE(x) __asm__ volatile("" : "+v"(x))
+// Keep the instruction scheduler from moving code across this point (for
+// example loads hoisted above it, whose destinations would be live early).
+#define SFX_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
 #endif
