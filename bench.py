@@ -283,7 +283,8 @@ def wl_full(torch, args):
               npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<128,CROSS>"),
         Phase("u_grad_T", lambda: NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, u, T, cg),
               nod6 + nod6 // 3 + G * 2 * M * 8, G * fft_flops_per_point(M, 12, 3, 8), "gauss_kernel<128,UGRADC>"),
-        Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, cg.reshape(G, 2 * M), vT, wq),
+        # (G, 2, M) column-major viewed as (G, 2M) column-major: column c + 2m
+        Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, torch.as_strided(cg, (G, 2 * M), (1, G)), vT, wq),
               2 * npn * 2 * M * 8 + G * 2 * M * 8 + 2 * 6 * me * 2 * M * 8, G * 2 * M * 30,
               "qs00_elem_kernel + qs00_gather_kernel"),
         Phase("rho_u", lambda: FP.FFT_SCALAR_VECT_DCL(None, u, rho, rhou, 1), 2 * nod6 + nod6 // 3,

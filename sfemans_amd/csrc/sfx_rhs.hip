@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(GATHER_THREADS) qs00_gather_kernel(RhsArgs a) 
 
 // rhs_3x3: element contributions, then the node-centred gather.
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
-  const int64_t ne = a.me * a.M, nn = a.np * a.M;
+  const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0, nn = a.np * a.M;
   if (ne > 0) hipLaunchKernelGGL(rhs_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nn == 0) return e;
@@ -37,7 +37,7 @@ hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st) {
-  const int64_t ne = a.me * a.M, nn = a.np * a.M;
+  const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0, nn = a.np * a.M;
   if (ne > 0) hipLaunchKernelGGL(qs00_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nn == 0) return e;

@@ -41,16 +41,35 @@ struct RhsArgs {
   int64_t n_rhs, n_ff, n_ffg, n_out, n_adj, n_elo;
 };
 
-// Element share of rhs_3x3, one thread per (element, mode), mode fastest:
+// Work item gid of the element kernels -> (element e, mode or column m).  A
+// wave takes ELEM_EB consecutive elements x ELEM_MB consecutive modes, modes
+// fastest: a load of one Gauss plane then touches the 7 * ELEM_EB
+// consecutive points of those elements (two cache lines) for ELEM_MB planes
+// instead of 64 planes, and the stores still write ELEM_MB consecutive modes
+// (one whole line) of each entry.
+constexpr int ELEM_EB = 4, ELEM_MB = 16;
+SFX_DEV void elem_item(int64_t gid, int M, int64_t& e, int& m) {
+  const int64_t w = gid / 64;
+  const int lane = (int)(gid % 64);
+  const int nmb = (M + ELEM_MB - 1) / ELEM_MB;
+  e = (w / nmb) * ELEM_EB + lane / ELEM_MB;
+  m = (int)(w % nmb) * ELEM_MB + lane % ELEM_MB;
+}
+inline int64_t elem_items(int64_t me, int M) {
+  return ((me + ELEM_EB - 1) / ELEM_EB) * (int64_t)((M + ELEM_MB - 1) / ELEM_MB) * 64;
+}
+
+// Element share of rhs_3x3, one thread per (element, mode) (elem_item):
 //   elo(6e+i, k, m) = sum_l ww(i,l) wgt(e,l) rhs(7e+l, comp k, m)
 // (the values the reference adds with VecSetValues, assembling_rhs.F90:73-95).
 // A thread reads the 7 consecutive Gauss values of its element per
 // component; the stores of a wave cover consecutive modes.  rhs_gather_block
 // then sums the entries of each node in element order.
 SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
-  const int64_t m = gid % a.M;
-  const int64_t e = gid / a.M;
-  if (e >= a.me) return;
+  int64_t e;
+  int m;
+  elem_item(gid, a.M, e, m);
+  if (e >= a.me || m >= a.M) return;
   double w[7], r[7][6];
 #pragma unroll
   for (int l = 0; l < 7; ++l) {
@@ -94,17 +113,18 @@ SFX_DEV void rhs_gather_block(const RhsArgs& a, int64_t bid, int tid, double* ti
                  });
 }
 
-// Element share of qs_00_gauss, one thread per (element, column), column fastest:
+// Element share of qs_00_gauss, one thread per (element, column) (elem_item):
 //   fl(l)       = (hc(e) SUM(ff_loc * ww(:,l)) + ff_gauss(7e+l)) * rj * ray
 //   elo(6e+i,c) = sum_l ww(i,l) fl(l)                    (fem_rhs_axi.F90:958-975)
 SFX_DEV void qs00_elem_body(const RhsArgs& a, int64_t gid) {
-  const int64_t c = gid % a.M;
-  const int64_t e = gid / a.M;
-  if (e >= a.me) return;
+  int64_t e;
+  int c;
+  elem_item(gid, a.M, e, c);
+  if (e >= a.me || c >= a.M) return;
   double ffl[6];
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const int64_t fi = (int64_t)a.jj[6 * e + j] - 1 + a.np * c;
+    const int64_t fi = (int64_t)a.jj[6 * e + j] - 1 + a.np * (int64_t)c;
     SFX_CHECK("ff load", fi, a.n_ff);
     ffl[j] = a.ff[fi];
   }
@@ -115,7 +135,7 @@ SFX_DEV void qs00_elem_body(const RhsArgs& a, int64_t gid) {
     double s = 0;
 #pragma unroll
     for (int j = 0; j < 6; ++j) s = fma(ffl[j], a.tab[j + 6 * l], s);
-    const int64_t gi = 7 * e + l + a.G * c;
+    const int64_t gi = 7 * e + l + a.G * (int64_t)c;
     SFX_CHECK("ffg load", gi, a.n_ffg);
     fl[l] = (hc * s + a.ffg[gi]) * a.wgt[7 * e + l];
   }

@@ -204,14 +204,14 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
     std::vector<double> elo((size_t)36 * me * M + 1);
     a.rhs = rhs; a.v145 = v145; a.v236 = v236; a.elo = elo.data();
     a.n_rhs = a.G * 6 * (int64_t)M; a.n_out = np * 3 * (int64_t)M; a.n_elo = 36 * me * (int64_t)M;
-    for (int64_t g = 0; g < me * M; ++g) rhs_elem_body(a, g);
+    for (int64_t g = 0; g < elem_items(me, M); ++g) rhs_elem_body(a, g);
     run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<6>() + 1) / 2,
              [&](int t, int64_t b, double2* sm) { rhs_gather_block(a, b, t, reinterpret_cast<double*>(sm)); });
   } else {
     std::vector<double> elo((size_t)6 * me * M + 1);
     a.heat_capa = hc; a.ff = ff; a.ffg = ffg; a.vect = vect; a.elo = elo.data();
     a.n_ff = np * (int64_t)M; a.n_ffg = a.G * (int64_t)M; a.n_out = np * (int64_t)M; a.n_elo = 6 * me * (int64_t)M;
-    for (int64_t g = 0; g < me * M; ++g) qs00_elem_body(a, g);
+    for (int64_t g = 0; g < elem_items(me, M); ++g) qs00_elem_body(a, g);
     run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<1>() + 1) / 2,
              [&](int t, int64_t b, double2* sm) { qs00_gather_block(a, b, t, reinterpret_cast<double*>(sm)); });
   }
