@@ -234,9 +234,10 @@ int sfx_nl_rhs_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
  * Batched persistent launch.  No single counterpart in the reference: a batch
  * stands for many independent calls of smb_cross_prod_gauss_sft_par
  * (SOURCES/SIMULATION/subroutine_ns_with_velocity.F90:514), one per problem,
- * on meshes of different sizes and different m_max_c, run as ONE kernel
- * launch whose work-groups walk host-built tile lists.  All arrays are device
- * pointers and must stay valid for the life of the batch. */
+ * on meshes of different sizes and different m_max_c, run as one kernel
+ * launch whose work-groups walk host-built tile lists (or, with the BY_SIZE
+ * schedule, one grid per grid size), replayable as one hipGraph.  All arrays
+ * are device pointers and must stay valid for the life of the batch. */
 typedef struct {
   sfx_mesh_t mesh;
   const int32_t* list_mode_dev;   /* (m_max_c) */
@@ -249,6 +250,7 @@ typedef struct sfx_batch_s* sfx_batch_t;
 #define SFX_BATCH_LPT 0          /* longest tile first to the least-loaded list */
 #define SFX_BATCH_CONTIGUOUS 1   /* equal-cost contiguous ranges, grouped by grid size */
 #define SFX_BATCH_ROUND_ROBIN 2  /* tile i to list i mod n_wg */
+#define SFX_BATCH_BY_SIZE 3      /* one ordinary grid per grid size, launched in turn (n_wg ignored) */
 
 int sfx_batch_create(const sfx_batch_problem_t* probs, int32_t n_prob, sfx_batch_t* out);
 /* Set the schedule: policy and number of persistent work-groups. */

@@ -17,9 +17,9 @@ import ctypes as C
 from ._lib import lib, check
 from .fft_parallele import fdims, ptr, stream_ptr
 
-__all__ = ["Batch", "LPT", "CONTIGUOUS", "ROUND_ROBIN"]
+__all__ = ["Batch", "LPT", "CONTIGUOUS", "ROUND_ROBIN", "BY_SIZE"]
 
-LPT, CONTIGUOUS, ROUND_ROBIN = 0, 1, 2
+LPT, CONTIGUOUS, ROUND_ROBIN, BY_SIZE = 0, 1, 2, 3
 
 
 class _Problem(C.Structure):
@@ -61,7 +61,7 @@ class Batch:
         pol, nwg, nt, cap = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int32()
         ms = C.c_double()
         check(lib().sfx_batch_info(self._h, C.byref(pol), C.byref(nwg), C.byref(nt), C.byref(ms), C.byref(cap)))
-        return {"policy": ["lpt", "contiguous", "round_robin"][pol.value], "n_wg": nwg.value,
+        return {"policy": ["lpt", "contiguous", "round_robin", "by_size"][pol.value], "n_wg": nwg.value,
                 "n_tiles": nt.value, "tuned_ms": ms.value, "captured": bool(cap.value)}
 
     def close(self):

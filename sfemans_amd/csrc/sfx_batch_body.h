@@ -23,48 +23,72 @@ struct BatchArgs {
   int64_t n_tiles;
 };
 
-// One tile as a separate function per grid size: the variants do not share
-// one register allocation or one stack frame.
-template <int MP>
-SFX_DEV_NOINLINE void batch_tile(const GaussArgs& a, int tid, int64_t block, double2* smem) {
-  gauss_body<MP, GV_CROSS_CURL, GAUSS_WAVES, true>(a, tid, block, smem);
-}
+// Waves per work-group of the batched kernel (one register budget for all
+// grid sizes).
+constexpr int BATCH_WAVES = GAUSS_WAVES;
 
-// Work-group `wg`: every tile of its list, one after the other.  The barrier
-// before each tile orders the previous tile's last LDS reads before the next
-// tile's table staging.
-SFX_DEV void batch_body(const BatchArgs& b, int tid, int wg, double2* smem) {
-  const int32_t t0 = b.wg_ptr[wg], t1 = b.wg_ptr[wg + 1];
-  for (int32_t q = t0; q < t1; ++q) {
+// A run of consecutive tiles of one grid size, as a separate function per
+// grid size (the variants do not share one register allocation or one stack
+// frame).  One call per run rather than per tile: a call saves and restores
+// the callee-saved registers the body uses.  The barrier before each tile
+// orders the previous tile's last LDS reads before the next tile's table
+// staging.
+template <int MP>
+SFX_DEV_NOINLINE void batch_run(const BatchArgs& b, int tid, int32_t q0, int32_t q1, double2* smem) {
+  for (int32_t q = q0; q < q1; ++q) {
     SFX_CHECK("tile", q, b.n_tiles);
     const BatchTile t = b.tiles[q];
     SFX_CHECK("problem", t.prob, b.n_prob);
-    const GaussArgs& a = b.probs[t.prob];
     SFX_SYNC();
-    switch (a.MP) {
-      case 4: batch_tile<4>(a, tid, t.block, smem); break;
-      case 8: batch_tile<8>(a, tid, t.block, smem); break;
-      case 16: batch_tile<16>(a, tid, t.block, smem); break;
-      case 32: batch_tile<32>(a, tid, t.block, smem); break;
-      case 64: batch_tile<64>(a, tid, t.block, smem); break;
-      case 128: batch_tile<128>(a, tid, t.block, smem); break;
-      case 256: batch_tile<256>(a, tid, t.block, smem); break;
+    gauss_body<MP, GV_CROSS_CURL, BATCH_WAVES, true>(b.probs[t.prob], tid, t.block, smem);
+  }
+}
+
+// Work-group `wg`: every tile of its list, one run of equal grid sizes at a
+// time.
+SFX_DEV void batch_body(const BatchArgs& b, int tid, int wg, double2* smem) {
+  const int32_t t0 = b.wg_ptr[wg], t1 = b.wg_ptr[wg + 1];
+  int32_t q = t0;
+  while (q < t1) {
+    SFX_CHECK("tile", q, b.n_tiles);
+    const int mp = b.probs[b.tiles[q].prob].MP;
+    int32_t r = q + 1;
+    while (r < t1 && b.probs[b.tiles[r].prob].MP == mp) ++r;
+    switch (mp) {
+      case 4: batch_run<4>(b, tid, q, r, smem); break;
+      case 8: batch_run<8>(b, tid, q, r, smem); break;
+      case 16: batch_run<16>(b, tid, q, r, smem); break;
+      case 32: batch_run<32>(b, tid, q, r, smem); break;
+      case 64: batch_run<64>(b, tid, q, r, smem); break;
+      case 128: batch_run<128>(b, tid, q, r, smem); break;
+      case 256: batch_run<256>(b, tid, q, r, smem); break;
       default: break;
     }
+    q = r;
   }
+}
+
+// BY_SIZE schedule: work-group bid of the grid of one grid size runs tile
+// t0 + bid with the fused kernel's own layout for that size.
+template <int MP>
+SFX_DEV void batch_class_body(const BatchArgs& b, int32_t t0, int tid, int64_t bid, double2* smem) {
+  SFX_CHECK("tile", t0 + bid, b.n_tiles);
+  const BatchTile t = b.tiles[t0 + bid];
+  SFX_CHECK("problem", t.prob, b.n_prob);
+  gauss_body<MP, GV_CROSS_CURL, gauss_waves<MP, GV_CROSS_CURL>()>(b.probs[t.prob], tid, t.block, smem);
 }
 
 // LDS (double2 units) of the fused kernel for grid size mp; the batch kernel
 // needs the largest value over the grid sizes present (not monotonic in mp).
 inline size_t batch_smem_cplx(int mp) {
   switch (mp) {
-    case 4: return GaussCfg<4, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
-    case 8: return GaussCfg<8, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
-    case 16: return GaussCfg<16, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
-    case 32: return GaussCfg<32, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
-    case 64: return GaussCfg<64, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
-    case 128: return GaussCfg<128, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
-    case 256: return GaussCfg<256, GV_CROSS_CURL, GAUSS_WAVES, true>::LY::TOTAL;
+    case 4: return GaussCfg<4, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
+    case 8: return GaussCfg<8, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
+    case 16: return GaussCfg<16, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
+    case 32: return GaussCfg<32, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
+    case 64: return GaussCfg<64, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
+    case 128: return GaussCfg<128, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
+    case 256: return GaussCfg<256, GV_CROSS_CURL, BATCH_WAVES, true>::LY::TOTAL;
   }
   return 0;
 }

@@ -10,7 +10,10 @@
 //   SFX_BATCH_CONTIGUOUS  tiles sorted by grid size then problem, cut into
 //                         equal-cost contiguous ranges (neighbouring elements
 //                         of one problem stay on one work-group);
-//   SFX_BATCH_ROUND_ROBIN tile i to list i mod n_wg, in problem order.
+//   SFX_BATCH_ROUND_ROBIN tile i to list i mod n_wg, in problem order;
+//   SFX_BATCH_BY_SIZE     one ordinary grid per grid size, launched one after
+//                         the other (each with the fused kernel's own layout
+//                         for that size; no persistent loop).
 //
 // The autotuner times every (policy, work-group count) candidate on the
 // caller's stream with HIP events and keeps the fastest; results are cached
@@ -38,6 +41,8 @@ struct sfx_batch_s {
   sfx::BatchTile* d_tiles = nullptr;
   int32_t* d_wg_ptr = nullptr;
   int32_t policy = -1, n_wg = 0;
+  std::vector<int32_t> wg_ptr_host;   // the lists' bounds (BY_SIZE: per grid size)
+  std::vector<int> class_mp;          // BY_SIZE: grid size of each list
   double best_ms = -1.0;
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -46,6 +51,8 @@ struct sfx_batch_s {
 
 namespace sfx {
 hipError_t launch_batch(const BatchArgs& b, size_t smem_cplx, hipStream_t st);
+hipError_t launch_batch_classes(const BatchArgs& b, const int32_t* wg_ptr_host, const int* class_mp,
+                                hipStream_t st);
 }
 
 using namespace sfx;
@@ -88,6 +95,10 @@ int schedule(sfx_batch_s* b, int32_t policy, int32_t n_wg) {
   clear_graph(b);
   b->policy = policy;
   b->n_wg = used;
+  b->wg_ptr_host = ptr;
+  b->class_mp.clear();
+  for (int32_t c = 0; c < used; ++c)
+    b->class_mp.push_back(ptr[c + 1] > ptr[c] ? b->probs[tiles[ptr[c]].prob].MP : 0);
   return SFX_OK;
 }
 
@@ -102,8 +113,14 @@ BatchArgs batch_args(const sfx_batch_s* b) {
   return a;
 }
 
+hipError_t launch_any(sfx_batch_s* b, hipStream_t st) {
+  if (b->policy == SFX_BATCH_BY_SIZE)
+    return launch_batch_classes(batch_args(b), b->wg_ptr_host.data(), b->class_mp.data(), st);
+  return launch_batch(batch_args(b), b->smem_cplx, st);
+}
+
 int launch(sfx_batch_s* b, hipStream_t st) {
-  hipError_t e = launch_batch(batch_args(b), b->smem_cplx, st);
+  hipError_t e = launch_any(b, st);
   if (e != hipSuccess) return hip_fail(e, "sfx_batch_run");
   return SFX_OK;
 }
@@ -209,8 +226,9 @@ int sfx_batch_autotune(sfx_batch_t b, int32_t reps, void* stream) {
   int32_t best_p = SFX_BATCH_LPT, best_w = wgs.front();
   double best = 1e300;
   int rc = SFX_OK;
-  for (int32_t pol : {SFX_BATCH_LPT, SFX_BATCH_CONTIGUOUS, SFX_BATCH_ROUND_ROBIN}) {
+  for (int32_t pol : {SFX_BATCH_LPT, SFX_BATCH_CONTIGUOUS, SFX_BATCH_ROUND_ROBIN, SFX_BATCH_BY_SIZE}) {
     for (int32_t w : wgs) {
+      if (pol == SFX_BATCH_BY_SIZE && w != wgs.front()) break;   // n_wg does not apply
       if ((rc = schedule(b, pol, w))) break;
       if ((rc = launch(b, st))) break;   // warm-up
       if ((e = hipEventRecord(e0, st)) != hipSuccess) { rc = hip_fail(e, "sfx_batch_autotune"); break; }
@@ -247,7 +265,7 @@ int sfx_batch_capture(sfx_batch_t b) {
     return hip_fail(e, "sfx_batch_capture");
   if ((e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
     return hip_fail(e, "sfx_batch_capture");
-  const hipError_t el = launch_batch(batch_args(b), b->smem_cplx, b->cap_stream);
+  const hipError_t el = launch_any(b, b->cap_stream);
   e = hipStreamEndCapture(b->cap_stream, &b->graph);
   if (el != hipSuccess) { clear_graph(b); return hip_fail(el, "sfx_batch_capture (launch)"); }
   if (e != hipSuccess) { clear_graph(b); return hip_fail(e, "sfx_batch_capture"); }

@@ -7,7 +7,11 @@
 //   1 CONTIGUOUS   tiles sorted by grid size (largest first) then problem,
 //                  cut into equal-cost contiguous ranges, so neighbouring
 //                  elements of one problem stay on one work-group;
-//   2 ROUND_ROBIN  tile i to list i mod n_wg, in problem order.
+//   2 ROUND_ROBIN  tile i to list i mod n_wg, in problem order;
+//   3 BY_SIZE      one list per grid size (smallest first), tiles in problem
+//                  order: each list runs as an ordinary grid of its own
+//                  (one work-group per tile), with the wave count best for
+//                  that grid size; n_wg is ignored.
 #pragma once
 #include <stdint.h>
 
@@ -86,6 +90,17 @@ inline int32_t batch_schedule(const std::vector<BatchTile>& tiles, const std::ve
     }
   } else if (policy == 2) {
     for (int64_t t = 0; t < nt; ++t) lists[t % n_wg].push_back(t);
+  } else if (policy == 3) {
+    std::vector<int> sizes;
+    for (int64_t t = 0; t < nt; ++t) sizes.push_back(mp[tiles[t].prob]);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    n_wg = (int32_t)std::max<size_t>(sizes.size(), 1);
+    lists.assign(n_wg, {});
+    for (int64_t t = 0; t < nt; ++t) {
+      const int c = (int)(std::lower_bound(sizes.begin(), sizes.end(), mp[tiles[t].prob]) - sizes.begin());
+      lists[c].push_back(t);
+    }
   } else {
     return -1;
   }

@@ -14,6 +14,7 @@
 #include <functional>
 #include <memory>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../sfemans_amd/csrc/sfx_fftprod_body.h"
@@ -326,7 +327,31 @@ extern "C" int emu_batch(int n, const int32_t* jj, const int64_t* jj_off, const 
   BatchArgs b;
   b.probs = probs.data(); b.tiles = tiles.data(); b.wg_ptr = ptr.data();
   b.n_prob = n; b.n_wg = used; b.n_tiles = (int64_t)tiles.size();
-  run_grid(used, 64 * GAUSS_WAVES, smem,
+  if (policy == 3) {
+    // as the library: one grid per grid size
+    for (int32_t c = 0; c < used; ++c) {
+      const int32_t t0 = ptr[c], nt = ptr[c + 1] - t0;
+      if (nt <= 0) continue;
+      auto grid = [&](auto mpc) {
+        constexpr int MPC = decltype(mpc)::value;
+        constexpr int NW = gauss_waves<MPC, GV_CROSS_CURL>();
+        run_grid(nt, 64 * NW, GaussCfg<MPC, GV_CROSS_CURL, NW>::LY::TOTAL,
+                 [&](int t, int64_t w, double2* s) { batch_class_body<MPC>(b, t0, t, w, s); });
+      };
+      switch (probs[tiles[t0].prob].MP) {
+        case 4: grid(std::integral_constant<int, 4>()); break;
+        case 8: grid(std::integral_constant<int, 8>()); break;
+        case 16: grid(std::integral_constant<int, 16>()); break;
+        case 32: grid(std::integral_constant<int, 32>()); break;
+        case 64: grid(std::integral_constant<int, 64>()); break;
+        case 128: grid(std::integral_constant<int, 128>()); break;
+        case 256: grid(std::integral_constant<int, 256>()); break;
+        default: return 1;
+      }
+    }
+    return 0;
+  }
+  run_grid(used, 64 * BATCH_WAVES, smem,
            [&](int t, int64_t w, double2* s) { batch_body(b, t, (int)w, s); });
   return 0;
 }

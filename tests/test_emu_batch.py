@@ -46,7 +46,7 @@ def _pack(probs):
     return meshes, fields, (jj, jj_off, rr, rr_off, me, npn, U, U_off, lm, lm_off, Ms, out, out_off)
 
 
-@pytest.mark.parametrize("policy,n_wg", [(0, 3), (1, 4), (2, 5), (0, 1000)])
+@pytest.mark.parametrize("policy,n_wg", [(0, 3), (1, 4), (2, 5), (0, 1000), (3, 1)])
 def test_batch_matches_oracle(policy, n_wg):
     meshes, fields, args = _pack(PROBS)
     used = np.zeros(1, dtype=np.int32)

@@ -2,7 +2,8 @@
 
 * every problem of a small batch vs smb_cross_prod_gauss_sft_par of the
   oracle (relative max-norm error <= 1e-12);
-* every schedule gives bit-identical results (a tile's arithmetic does not
+* every schedule (persistent lists or one grid per grid size) gives
+  bit-identical results (a tile's arithmetic does not
   depend on which work-group runs it);
 * hipGraph replays, with eager work in between, equal the eager launch bit
   for bit;
@@ -60,7 +61,7 @@ def test_batch_vs_oracle_and_schedules():
     for (m, _, M, u, _, _), got in zip(items, first):
         exp = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, np.arange(M), u)
         assert rel(got, exp) <= TOL, M
-    for policy in (BT.LPT, BT.CONTIGUOUS, BT.ROUND_ROBIN):
+    for policy in (BT.LPT, BT.CONTIGUOUS, BT.ROUND_ROBIN, BT.BY_SIZE):
         for n_wg in (1, 3, 256, 100000):
             print(f"case batch schedule policy={policy} n_wg={n_wg}", flush=True)
             for (*_, dout) in items:
