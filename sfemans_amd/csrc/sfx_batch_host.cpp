@@ -43,6 +43,10 @@ struct sfx_batch_s {
   int32_t policy = -1, n_wg = 0;
   std::vector<int32_t> wg_ptr_host;   // the lists' bounds (BY_SIZE: per grid size)
   std::vector<int> class_mp;          // BY_SIZE: grid size of each list
+  // BY_SIZE: the grids of the grid sizes run as parallel branches, forked from
+  // the caller's stream onto side streams (default priority) and joined back
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;
   double best_ms = -1.0;
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -113,9 +117,38 @@ BatchArgs batch_args(const sfx_batch_s* b) {
   return a;
 }
 
+// BY_SIZE: grid size c's grid on branch c (branch 0: the caller's stream).
+hipError_t launch_by_size(sfx_batch_s* b, hipStream_t st) {
+  const BatchArgs a = batch_args(b);
+  const int nc = b->n_wg;
+  hipError_t e;
+  while ((int)b->side.size() < nc) {
+    hipStream_t s;
+    if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return e;
+    b->side.push_back(s);
+  }
+  while ((int)b->ev.size() < nc + 1) {
+    hipEvent_t v;
+    if ((e = hipEventCreateWithFlags(&v, hipEventDisableTiming)) != hipSuccess) return e;
+    b->ev.push_back(v);
+  }
+  if ((e = hipEventRecord(b->ev[nc], st)) != hipSuccess) return e;   // fork point
+  for (int c = 0; c < nc; ++c) {
+    hipStream_t sc = (c == 0) ? st : b->side[c];
+    if (c > 0 && (e = hipStreamWaitEvent(sc, b->ev[nc], 0)) != hipSuccess) return e;
+    BatchArgs one = a;
+    one.n_wg = 1;
+    const int32_t ptr1[2] = {b->wg_ptr_host[c], b->wg_ptr_host[c + 1]};
+    if ((e = launch_batch_classes(one, ptr1, &b->class_mp[c], sc)) != hipSuccess) return e;
+    if (c > 0 && (e = hipEventRecord(b->ev[c], sc)) != hipSuccess) return e;
+  }
+  for (int c = 1; c < nc; ++c)
+    if ((e = hipStreamWaitEvent(st, b->ev[c], 0)) != hipSuccess) return e;   // join
+  return hipSuccess;
+}
+
 hipError_t launch_any(sfx_batch_s* b, hipStream_t st) {
-  if (b->policy == SFX_BATCH_BY_SIZE)
-    return launch_batch_classes(batch_args(b), b->wg_ptr_host.data(), b->class_mp.data(), st);
+  if (b->policy == SFX_BATCH_BY_SIZE) return launch_by_size(b, st);
   return launch_batch(batch_args(b), b->smem_cplx, st);
 }
 
@@ -128,6 +161,8 @@ int launch(sfx_batch_s* b, hipStream_t st) {
 void destroy(sfx_batch_s* b) {
   clear_graph(b);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+  for (hipEvent_t v : b->ev) (void)hipEventDestroy(v);
   (void)hipFree(b->d_probs);
   (void)hipFree(b->d_tiles);
   (void)hipFree(b->d_wg_ptr);
