@@ -56,13 +56,14 @@ def test_dot_product(nx, ny, M):
     assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(TD) * np.linalg.norm(Y)
 
 
-def test_dot_product_config4_size():
-    """50,000 triangles (125 x 200 squares), m_max = 32, config-4 base flow."""
+def test_dot_product_50k_triangles():
+    """50,000 triangles (125 x 200 squares), m_max = 32, base flow of mode 0
+    plus modes 1-4 at amplitude 0.1."""
     m = rectangle_p2(125, 200)
     mesh = NL.Mesh.from_p2(m)
     M = 32
     modes = np.arange(M)
-    print("case config-4 dot product", flush=True)
+    print("case dot product 50k triangles m_max=32", flush=True)
     U = fortran_tensor(base_flow(m.np, M, 0x5F3E + 4))
     D = fortran_tensor(fourier_field(m.np, 6, M, 0x5F3E + 40, 2.0))
     g = torch.Generator(device="cuda").manual_seed(7)
