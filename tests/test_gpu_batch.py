@@ -117,7 +117,7 @@ def test_batch_argument_errors():
     b.close()
 
 
-def config3_problems(seed=0x5F3E + 3, n=32):
+def uneven_problems(seed=0x5F3E + 3, n=32):
     """32 problems: triangles log-uniform in [2k, 50k] (nx = ny/2), m_max from {4..128}."""
     rng = np.random.default_rng(seed)
     out = []
@@ -130,8 +130,8 @@ def config3_problems(seed=0x5F3E + 3, n=32):
     return out
 
 
-def test_batch_config3_equals_single_problem_kernel():
-    probs = config3_problems()
+def test_batch_32_uneven_problems_equal_single_problem_kernel():
+    probs = uneven_problems()
     items = []
     for k, (nx, ny, M) in enumerate(probs):
         m = rectangle_p2(nx, ny)
@@ -139,7 +139,7 @@ def test_batch_config3_equals_single_problem_kernel():
         g = torch.Generator(device="cuda").manual_seed(100 + k)
         du = torch.randn((M, 6, m.np), generator=g, device="cuda", dtype=torch.float64).permute(2, 1, 0)
         items.append((m, mesh, M, du, fortran_empty((m.n_gauss, 6, M)), fortran_empty((m.n_gauss, 6, M))))
-    print(f"case batch config3: {len(items)} problems, {sum(m.me for m, *_ in items)} triangles", flush=True)
+    print(f"case batch 32 uneven problems: {len(items)} problems, {sum(m.me for m, *_ in items)} triangles", flush=True)
     b = BT.Batch([(mesh, list(range(M)), du, out) for (m, mesh, M, du, out, _) in items])
     b.autotune(reps=1)
     b.run()
