@@ -21,8 +21,10 @@ coefficients, mode-0 sine set to zero):
   the exact transpose of its linearization around u applied to a Gauss
   field.  units = 2 * G * m_max per step.
 * ``mhd``: MHD nonlinear terms of one explicit step on the same mesh:
-  rotv_v = (curl u) x u - (curl H) x B assembled into the per-mode NS
-  right-hand sides (rhs_3x3), and u x B at the nodes.  units = G * m_max.
+  rotv_v = (curl u) x u - (curl H) x B, the full per-mode NS right-hand
+  sides built from it (rhs_ns_gauss_3x3 with the BDF term, a source and
+  the pressure gradients, assembled by rhs_3x3), and u x B at the nodes.
+  units = G * m_max.
 * ``fftprod``: FFT_PAR_CROSS_PROD_DCL on Gauss-point arrays of that size.
 * ``tangent``: tangent and adjoint around a base flow (modes 0-4, amplitude
   0.1) on a 50k-triangle mesh (158x316), m_max = 32.
@@ -126,6 +128,9 @@ def wl_mhd(torch, args):
     modes = list(range(M))
     G, npn = m.n_gauss, m.np
     u, H, B = _fields(torch, npn, M, 4.0, (SEED + 2, SEED + 3, SEED + 4))
+    # BDF term, momentum source and pressures on the velocity mesh of the NS RHS
+    v1m, fsrc = _fields(torch, npn, M, 4.0, (SEED + 6, SEED + 7))
+    pr, pinc = _fields(torch, npn, M, 4.0, (SEED + 8, SEED + 9), ncol=2)
     work = FP.fortran_empty((G, 6, M))
     v145 = FP.fortran_empty((npn, 3, M))
     v236 = FP.fortran_empty((npn, 3, M))
@@ -136,15 +141,18 @@ def wl_mhd(torch, args):
     phases = [
         Phase("rotv_mhd", lambda: NL.nl_rotv_mhd(mesh, modes, u, H, B, 1.0, work), rotv_bytes,
               G * M * 300 + G * fft_flops_per_point(M, 18, 3, 18), "gauss_kernel<64,PAIRS> (2 passes)"),
-        # Gauss array read once, element contributions written and read back, vb written
-        Phase("rhs_3x3", lambda: NL.rhs_3x3_all_modes(mesh, work, v145, v236, -1.0, welem),
-              G * 6 * M * 8 + 2 * 36 * m.me * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8 + 12 * 6 * m.me,
-              G * M * 6 * 2 * 6, "rhs_elem_kernel + rhs_gather_kernel"),
+        # full NS RHS (rhs_ns_gauss_3x3 + rhs_3x3): rotv_v read once, V1m, ff (6 columns) and
+        # P, P_inc (2 columns) read, element contributions written and read back, vb written
+        Phase("rhs_ns_3x3", lambda: NL.rhs_ns_3x3(mesh, modes, work, v1m, fsrc, pr, pinc, v145, v236, welem),
+              G * 6 * M * 8 + npn * 16 * M * 8 + 2 * 36 * m.me * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8
+              + 12 * 6 * m.me + npn * 16,
+              G * M * (6 * 2 * 6 + 16 * 6 * 2), "rhs_ns_elem_kernel + rhs_gather_kernel"),
         Phase("u_x_B_nodes", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb), 3 * npn * 6 * M * 8,
               npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<64,CROSS>"),
     ]
     cfg = {"workload": "MHD nonlinear terms of one step, 224x448 P2 mesh (200,704 triangles), m_max=64: "
-                       "rotv_v=(curl u)xu-(curl H)xB at Gauss points -> per-mode NS RHS (rhs_3x3), u x B at nodes",
+                       "rotv_v=(curl u)xu-(curl H)xB at Gauss points -> full per-mode NS RHS (rhs_ns_gauss_3x3 + rhs_3x3), "
+                       "u x B at nodes",
            "me": m.me, "np": npn, "gauss_points": G, "m_max": M, "grid": 3 * mp_of(M)}
     return phases, G * M, cfg, ("mhd", m, M)
 

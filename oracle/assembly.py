@@ -11,8 +11,13 @@ so the dense vectors here are (np, 3), column ki.
 ``qs_00_gauss`` follows SOURCES/SOLVER/fem_rhs_axi.F90:940-987:
 ``fl = (heat_capa(m)*SUM(ff_loc*ww(:,l)) + ff_gauss(index))*rj(l,m)*ray``.
 
-``rhs_ns_gauss_nl`` is the nonlinear share ``-rotv_v`` of ``rhs_ns_gauss_3x3``
-(SOURCES/SOLVER/computing_rhs_gauss.F90:100) with
+``rhs_ns_gauss_3x3`` follows SOURCES/SOLVER/computing_rhs_gauss.F90:7-146
+without the penalty options (``if_ns_penalty`` false): per mode i and Gauss
+point, ``rhs_gauss = (ft + fp_inc + fs - rotv_v) + fp`` (:100, :145) with
+``ft = SUM(V1m*ww)``, ``fs = SUM(ff*ww)`` (``ff`` the nodal momentum
+source), and the pressure terms of the pressure injected onto the velocity
+mesh (``P``, ``P_inc``, both (np, 2, M)) (:69-97):
+``fp = (-d_r P_c, -d_r P_s, -m P_s/r, m P_c/r, -d_z P_c, -d_z P_s)``, where
 ``rotv_v = (curl u) x u - coeff_lorentz*(curl H) x B``
 (SOURCES/SIMULATION/subroutine_ns_with_velocity.F90:227, 246-248).
 """
@@ -22,7 +27,7 @@ import numpy as np
 
 from .gauss_p2 import gauss_tables
 
-__all__ = ["rhs_3x3", "rhs_3x3_all_modes", "qs_00_gauss"]
+__all__ = ["rhs_3x3", "rhs_3x3_all_modes", "qs_00_gauss", "rhs_ns_gauss_3x3"]
 
 _C145 = (0, 3, 4)
 _C236 = (1, 2, 5)
@@ -74,3 +79,38 @@ def qs_00_gauss(jj, rr, heat_capa, ff, ff_gauss):
     fl = (hc * (ffl @ ww) + fg) * rj.T * ray.T             # (me, 7)
     loc = fl @ ww.T                                        # (me, 6)
     return _nodal_sum(jj, np.shape(rr)[1], loc)
+
+
+def rhs_ns_gauss_3x3(jj, rr, list_mode, V1m, ff, P, P_inc, rotv_v):
+    """``rhs_gauss`` (G, 6, M) of computing_rhs_gauss.F90:7-146 (no penalty).
+
+    ``V1m``, ``ff``: nodal (np, 6, M); ``P``, ``P_inc``: pressure on the
+    velocity mesh, (np, 2, M); ``rotv_v``: (G, 6, M).
+    """
+    ww, dw, rj, ray = gauss_tables(jj, rr)
+    j0 = (np.asarray(jj, dtype=np.int64) - 1).T            # (me, 6)
+    me = j0.shape[0]
+    M = np.shape(V1m)[2]
+    mode = np.asarray(list_mode, dtype=np.float64)[None, None, :]
+    r = ray.T[:, :, None]                                  # (me, 7, 1)
+
+    def val(X):   # (np, c, M) -> (me, 7, c, M)
+        return np.einsum("enkm,nl->elkm", np.asarray(X, dtype=np.float64)[j0], ww)
+
+    def grad(X, d):   # physical derivative d (0: r, 1: z)
+        return np.einsum("enkm,nle->elkm", np.asarray(X, dtype=np.float64)[j0], dw[d])
+
+    def fpress(Q):
+        Qv, Qr, Qz = val(Q), grad(Q, 0), grad(Q, 1)
+        f = np.empty((me, 7, 6, M))
+        f[:, :, 0] = -Qr[:, :, 0]
+        f[:, :, 1] = -Qr[:, :, 1]
+        f[:, :, 2] = -Qv[:, :, 1] / r * mode
+        f[:, :, 3] = Qv[:, :, 0] / r * mode
+        f[:, :, 4] = -Qz[:, :, 0]
+        f[:, :, 5] = -Qz[:, :, 1]
+        return f
+    rv = np.asarray(rotv_v, dtype=np.float64).reshape(me, 7, 6, M)
+    out = (val(V1m) + fpress(P_inc) + val(ff) - rv) + fpress(P)
+    return np.ascontiguousarray(out.reshape(me * 7, 6, M))
+

@@ -57,6 +57,7 @@ struct GatherArgs;
 #ifndef SFX_EMULATE
 hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, const GatherArgs* ga2, hipStream_t st);
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st);
+hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);
 hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);

@@ -11,6 +11,10 @@ __global__ void __launch_bounds__(256) rhs_elem_kernel(RhsArgs a) {
   rhs_elem_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
+__global__ void __launch_bounds__(256) rhs_ns_elem_kernel(RhsArgs a) {
+  rhs_ns_elem_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+}
+
 __global__ void __launch_bounds__(GATHER_THREADS) rhs_gather_kernel(RhsArgs a) {
   __shared__ double tile[gather_tile_doubles<6>()];
   rhs_gather_block(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, tile);
@@ -29,6 +33,17 @@ __global__ void __launch_bounds__(GATHER_THREADS) qs00_gather_kernel(RhsArgs a) 
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
   const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0, nn = a.np * a.M;
   if (ne > 0) hipLaunchKernelGGL(rhs_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nn == 0) return e;
+  const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;
+  hipLaunchKernelGGL(rhs_gather_kernel, dim3((unsigned)nb), dim3(GATHER_THREADS), 0, st, a);
+  return hipGetLastError();
+}
+
+// Full NS right-hand side fused with its element share, then the gather.
+hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st) {
+  const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0, nn = a.np * a.M;
+  if (ne > 0) hipLaunchKernelGGL(rhs_ns_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nn == 0) return e;
   const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;

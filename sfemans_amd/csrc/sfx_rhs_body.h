@@ -39,6 +39,14 @@ struct RhsArgs {
   double* vect;             // (np, M)
   double* elo;              // per-element nodal contributions, index ((6e+i)*6 + k)*M + m
   int64_t n_rhs, n_ff, n_ffg, n_out, n_adj, n_elo;
+  // full NS right-hand side (rhs_ns_elem_body); a null field counts as zero
+  const double* rr;         // (2, np) node coordinates
+  const int32_t* lmode;     // (M) mode numbers
+  const double* v1m;        // (np, 6, M) time-derivative term
+  const double* fsrc;       // (np, 6, M) nodal momentum source
+  const double* pres;       // (np, 2, M) pressure on the velocity mesh
+  const double* pinc;       // (np, 2, M) pressure increment on the velocity mesh
+  int64_t n_v1m, n_fsrc, n_pres, n_pinc;
 };
 
 // Work item gid of the element kernels -> (element e, mode or column m).  A
@@ -97,6 +105,118 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
       a.elo[o] = loc[k];
     }
   }
+}
+
+// Full explicit NS right-hand side of rhs_ns_gauss_3x3 without the penalty
+// options (SOURCES/SOLVER/computing_rhs_gauss.F90:7-146) fused with the
+// element share of rhs_3x3, one thread per (element, mode) (elem_item):
+//   rhs(l, :) = (ft + fp_inc + fs - rotv_v(7e+l, :)) + fp         (:100, :145)
+//   ft = SUM(V1m*ww), fs = SUM(ff*ww),
+//   fp = (-d_r P_c, -d_r P_s, -P_s/ray*m, P_c/ray*m, -d_z P_c, -d_z P_s) (:69-97)
+// with P and P_inc the pressure injected onto the velocity mesh; then
+//   elo(6e+i, k, m) = sum_l ww(i,l) wgt(e,l) rhs(l, k)   as rhs_elem_body.
+// The Gauss array rhs_gauss is never stored.
+SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
+  int64_t e;
+  int m;
+  elem_item(gid, a.M, e, m);
+  if (e >= a.me || m >= a.M) return;
+  int64_t j[6];
+  double rn[6], zn[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    j[n] = (int64_t)a.jj[6 * e + n] - 1;
+    SFX_CHECK("rr load", 2 * j[n] + 1, 2 * a.np);
+    rn[n] = a.rr[2 * j[n]];
+    zn[n] = a.rr[2 * j[n] + 1];
+  }
+  // affine map of the straight-sided element from its vertices:
+  // d/dr = a11 d/dx + a12 d/dy, d/dz = a21 d/dx + a22 d/dy
+  const double j11 = rn[1] - rn[0], j12 = rn[2] - rn[0];
+  const double j21 = zn[1] - zn[0], j22 = zn[2] - zn[0];
+  const double idet = 1.0 / (j11 * j22 - j12 * j21);
+  const double a11 = j22 * idet, a12 = -j21 * idet, a21 = -j12 * idet, a22 = j11 * idet;
+  const double mv = (double)a.lmode[m];
+  double V[6][6], F[6][6], P[6][2], Q[6][2];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int64_t i6 = j[n] + a.np * (k + 6 * (int64_t)m);
+      SFX_CHECK("V1m load", i6, a.v1m ? a.n_v1m : 1 << 30);
+      SFX_CHECK("ff load", i6, a.fsrc ? a.n_fsrc : 1 << 30);
+      V[n][k] = a.v1m ? a.v1m[i6] : 0.0;
+      F[n][k] = a.fsrc ? a.fsrc[i6] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int64_t i2 = j[n] + a.np * (k + 2 * (int64_t)m);
+      SFX_CHECK("P load", i2, a.pres ? a.n_pres : 1 << 30);
+      SFX_CHECK("P_inc load", i2, a.pinc ? a.n_pinc : 1 << 30);
+      P[n][k] = a.pres ? a.pres[i2] : 0.0;
+      Q[n][k] = a.pinc ? a.pinc[i2] : 0.0;
+    }
+  }
+  double loc[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) loc[i][k] = 0.0;
+#pragma unroll
+  for (int l = 0; l < 7; ++l) {
+    const double* w = a.tab + 6 * l;
+    const double* d = a.tab + 42 + 12 * l;
+    double ray = 0, ft[6] = {0, 0, 0, 0, 0, 0}, fs[6] = {0, 0, 0, 0, 0, 0};
+    double pv[2] = {0, 0}, px[2] = {0, 0}, py[2] = {0, 0}, qv[2] = {0, 0}, qx[2] = {0, 0}, qy[2] = {0, 0};
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      ray = fma(rn[n], w[n], ray);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        ft[k] = fma(V[n][k], w[n], ft[k]);
+        fs[k] = fma(F[n][k], w[n], fs[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        pv[k] = fma(P[n][k], w[n], pv[k]);
+        px[k] = fma(P[n][k], d[2 * n], px[k]);
+        py[k] = fma(P[n][k], d[2 * n + 1], py[k]);
+        qv[k] = fma(Q[n][k], w[n], qv[k]);
+        qx[k] = fma(Q[n][k], d[2 * n], qx[k]);
+        qy[k] = fma(Q[n][k], d[2 * n + 1], qy[k]);
+      }
+    }
+    double fp[6], fq[6];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      fp[k] = -fma(a11, px[k], a12 * py[k]);
+      fp[4 + k] = -fma(a21, px[k], a22 * py[k]);
+      fq[k] = -fma(a11, qx[k], a12 * qy[k]);
+      fq[4 + k] = -fma(a21, qx[k], a22 * qy[k]);
+    }
+    fp[2] = -pv[1] / ray * mv;
+    fp[3] = pv[0] / ray * mv;
+    fq[2] = -qv[1] / ray * mv;
+    fq[3] = qv[0] / ray * mv;
+    const double wl = a.wgt[7 * e + l];
+    const int64_t gi = 7 * e + l + a.G * 6 * (int64_t)m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      SFX_CHECK("rotv load", gi + a.G * k, a.rhs ? a.n_rhs : 1 << 30);
+      const double rv = a.rhs ? a.rhs[gi + a.G * k] : 0.0;
+      const double r = ((ft[k] + fq[k] + fs[k] - rv) + fp[k]) * wl;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) loc[i][k] = fma(w[i], r, loc[i][k]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int64_t o = ((6 * e + i) * 6 + k) * a.M + m;
+      SFX_CHECK("elo store", o, a.n_elo);
+      a.elo[o] = loc[i][k];
+    }
 }
 
 // vb_145 / vb_236 of work-group bid: the nodes' element contributions summed

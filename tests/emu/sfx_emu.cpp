@@ -219,6 +219,32 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
   return 0;
 }
 
+extern "C" int emu_rhs_ns(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm, int M,
+                          const double* rotv, const double* V1m, const double* ff, const double* P,
+                          const double* Pinc, double* v145, double* v236) {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> ent;
+  build_adjacency(jj, me, np, ptr, ent);
+  double tab[126];
+  p2_reference_tables(tab);
+  std::vector<double> wgt(7 * me);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.adj_ptr = ptr.data(); a.adj_ent = ent.data(); a.wgt = wgt.data(); a.tab = tab;
+  a.jj = jj; a.me = me; a.np = np; a.G = 7 * me; a.M = M; a.scale = 1.0; a.n_adj = (int64_t)ent.size();
+  std::vector<double> elo((size_t)36 * me * M + 1);
+  a.rr = rr; a.lmode = lm;
+  a.rhs = rotv; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = Pinc;
+  a.v145 = v145; a.v236 = v236; a.elo = elo.data();
+  a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = np * 6 * (int64_t)M; a.n_pres = a.n_pinc = np * 2 * (int64_t)M;
+  a.n_out = np * 3 * (int64_t)M; a.n_elo = 36 * me * (int64_t)M;
+  for (int64_t g = 0; g < elem_items(me, M); ++g) rhs_ns_elem_body(a, g);
+  run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<6>() + 1) / 2,
+           [&](int t, int64_t b, double2* sm) { rhs_gather_block(a, b, t, reinterpret_cast<double*>(sm)); });
+  return 0;
+}
+
 // ---- adjoint of the tangent
 #include "../../sfemans_amd/csrc/sfx_adjoint_body.h"
 

@@ -161,6 +161,36 @@ def test_qs00_device_many_columns():
         assert rel(got[:, c], exp) <= 1e-13, c
 
 
+def test_rhs_ns_3x3_device():
+    """Full NS right-hand side (rhs_ns_gauss_3x3 + rhs_3x3, fused) vs the
+    oracle's two steps, relative error <= 1e-12, with every field and with
+    absent fields (counted as zero)."""
+    from oracle import assembly
+    m = rectangle_p2(9, 7)
+    mesh = NL.Mesh.from_p2(m)
+    modes = [0, 1, 2, 4, 5, 9, 11]
+    M = len(modes)
+    print(f"case rhs_ns_3x3 {m.me} triangles modes={modes}", flush=True)
+    rng = np.random.default_rng(9)
+    F = lambda *s: np.asfortranarray(rng.standard_normal(s))
+    rotv, V1m, ff = F(m.n_gauss, 6, M), F(m.np, 6, M), F(m.np, 6, M)
+    P, Pinc = F(m.np, 2, M), F(m.np, 2, M)
+    v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), fortran_tensor(V1m), fortran_tensor(ff),
+                  fortran_tensor(P), fortran_tensor(Pinc), v1, v2)
+    torch.cuda.synchronize()
+    g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, np.asarray(modes), V1m, ff, P, Pinc, rotv)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+    # no time-derivative term and no pressure increment
+    NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), None, fortran_tensor(ff), fortran_tensor(P), None, v1, v2)
+    torch.cuda.synchronize()
+    z6, z2 = np.zeros_like(V1m), np.zeros_like(P)
+    g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, np.asarray(modes), z6, ff, P, z2, rotv)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+
+
 def test_rhs_3x3_device_all_modes():
     """Device path over many modes (element share + gather) vs the oracle."""
     from oracle import assembly
