@@ -11,15 +11,20 @@ midpoint of the edge opposite local vertex ``k``
 
 ``rectangle_p2`` produces the same structure for an ``nx x ny`` grid of squares
 on ``[r0,r1] x [z0,z1]``, each square split into two triangles, elements
-numbered row by row and edges numbered in order of first encounter.  The
-arrays use the Fortran layout the C-ABI receives: ``jj(6, me)`` (1-based,
+numbered row by row and edges numbered in order of first encounter.
+``load_fem_p2`` reads the P1 section of a formatted SFEMaNS ``.FEM`` mesh file
+(SOURCES/MESH/load_mesh.F90:59-93: ``np nw me nws mes``, then per cell
+``jj(nw) neigh(3) i_d``, per boundary side ``jjs(nws) neighs sides``, per node
+``rr(2)``), keeps the cells of the listed domains, numbers their nodes in order
+of first appearance (:246-259) and builds the P2 grid as above.  The arrays
+use the Fortran layout the C-ABI receives: ``jj(6, me)`` (1-based,
 column-major) and ``rr(2, np)``.
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["P2Mesh", "rectangle_p2"]
+__all__ = ["P2Mesh", "rectangle_p2", "p2_from_p1", "read_fem_p1", "load_fem_p2"]
 
 
 class P2Mesh:
@@ -84,13 +89,25 @@ def rectangle_p2(nx: int, ny: int, r0: float = 0.0, r1: float = 1.0,
     tri[0::2] = t1
     tri[1::2] = t2
     tri.sort(axis=1)
-    me = tri.shape[0]
+    return p2_from_p1(tri, vr, vz, nx, ny)
 
+
+def p2_from_p1(tri: np.ndarray, vr: np.ndarray, vz: np.ndarray, nx: int = 0, ny: int = 0) -> P2Mesh:
+    """P2 mesh from P1 cells ``tri`` (me, 3), 0-based, each triplet in the
+    reference's local vertex order, and vertex coordinates ``vr``, ``vz``:
+    one node per edge after the vertices, edges numbered in order of first
+    encounter (cell by cell, edge opposite local vertex 0, 1, 2), as the
+    single-process mesh extraction numbers them
+    (SOURCES/MESH/mesh_metis_distribution.F90:728-745)."""
+    tri = np.asarray(tri, dtype=np.int64)
+    nv = int(np.asarray(vr).size)
+    me = tri.shape[0]
     # edge opposite local vertex k joins the two other vertices (k = 0,1,2)
     opp = np.stack([np.stack([tri[:, 1], tri[:, 2]], 1),
                     np.stack([tri[:, 0], tri[:, 2]], 1),
                     np.stack([tri[:, 0], tri[:, 1]], 1)], axis=1)  # (me, 3, 2)
-    keys = opp[:, :, 0] * nv + opp[:, :, 1]                        # (me, 3)
+    lo, hi = np.minimum(opp[:, :, 0], opp[:, :, 1]), np.maximum(opp[:, :, 0], opp[:, :, 1])
+    keys = lo * nv + hi                                            # (me, 3)
     flat = keys.ravel()                                            # order m, then k
     uniq, first_idx, inv = np.unique(flat, return_index=True, return_inverse=True)
     # renumber edges in order of first encounter
@@ -104,9 +121,10 @@ def rectangle_p2(nx: int, ny: int, r0: float = 0.0, r1: float = 1.0,
     rr = np.empty((2, npts), dtype=np.float64)
     rr[0, :nv] = vr
     rr[1, :nv] = vz
-    ekeys = uniq[order]
-    es = ekeys // nv
-    ee = ekeys % nv
+    # midpoint from the end with the lower local index in the cell that
+    # meets the edge first (refine_mesh.F90:323-329, 341-343)
+    first = opp.reshape(-1, 2)[first_idx[order]]
+    es, ee = first[:, 0], first[:, 1]
     rr[0, nv:] = vr[es] + (vr[ee] - vr[es]) / 2.0
     rr[1, nv:] = vz[es] + (vz[ee] - vz[es]) / 2.0
 
@@ -114,3 +132,49 @@ def rectangle_p2(nx: int, ny: int, r0: float = 0.0, r1: float = 1.0,
     jj[0:3, :] = tri.T + 1
     jj[3:6, :] = (nv + edge_id).T + 1
     return P2Mesh(jj, rr, nx, ny)
+
+
+def read_fem_p1(path: str):
+    """First (P1) section of a formatted ``.FEM`` file:
+    ``(jj (3, me) 1-based as in the file, i_d (me,), rr (2, np))``."""
+    with open(path) as f:
+        head = f.readline().split()
+        npt, nw, me, nws, mes = (int(x) for x in head[:5])
+        if nw != 3 or nws != 2:
+            raise ValueError(f"{path}: first section is not a P1 triangulation (nw={nw}, nws={nws})")
+        cells = np.array([f.readline().split()[:7] for _ in range(me)], dtype=np.int64)
+        for _ in range(mes):
+            f.readline()
+        rr = np.array([f.readline().replace("D", "E").split()[:2] for _ in range(npt)], dtype=np.float64)
+    return np.asfortranarray(cells[:, :3].T), cells[:, 6].copy(), np.asfortranarray(rr.T)
+
+
+def load_fem_p2(path: str, list_dom=None) -> P2Mesh:
+    """P2 mesh of the cells of ``list_dom`` (all domains, ascending, if None)
+    from the P1 section of a formatted ``.FEM`` file.  The vertex triplet of
+    every cell is sorted by file index (load_mesh.F90:101-112), the kept
+    cells are taken domain by domain in file order and their nodes numbered
+    in order of first appearance (:186-196, :246-259); a single process keeps
+    that order (mesh_metis_distribution.F90:62-65, 689-727).  Edges are then
+    added as in ``p2_from_p1``.  Nodes on the listed domains' common sides
+    are shared: the duplicated nodes of interfaces declared as real
+    interfaces (stat 3, :233-236) are not produced."""
+    jj1, i_d, rr1 = read_fem_p1(path)
+    tri = np.sort(jj1.T - 1, axis=1)
+    doms = np.unique(i_d) if list_dom is None else np.asarray(list_dom)
+    new = -np.ones(rr1.shape[1], dtype=np.int64)
+    cells, n = [], 0
+    for d in doms:
+        for m in np.nonzero(i_d == d)[0]:
+            for v in tri[m]:
+                if new[v] < 0:
+                    new[v] = n
+                    n += 1
+            cells.append(m)
+    if not cells:
+        raise ValueError(f"{path}: no cell in domains {list(doms)}")
+    tri_new = new[tri[cells]]
+    old = np.empty(n, dtype=np.int64)
+    old[new[new >= 0]] = np.nonzero(new >= 0)[0]
+    return p2_from_p1(tri_new, rr1[0, old], rr1[1, old])
+

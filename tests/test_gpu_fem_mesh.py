@@ -1,0 +1,106 @@
+"""GPU kernels on the reference's regression-suite meshes vs the oracle.
+
+Mesh_10_form.FEM, domain 1 with 3 modes, is the Navier-Stokes setting of the
+reference's TEST2; Mesh_20_form.FEM is TEST25's mesh, here with both domains.
+Cells of these unstructured meshes come in both orientations and their vertex
+triplets are not in ascending node order.  Tolerance: relative max-norm error
+<= 1e-12 (FP64); the adjoint through the dot-product test.
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import assembly, smb  # noqa: E402
+from sfemans_amd import nonlinear as NL  # noqa: E402
+from sfemans_amd.fft_parallele import fortran_empty, fortran_tensor  # noqa: E402
+from sfemans_amd.mesh import load_fem_p2  # noqa: E402
+from sfemans_amd.synth import fourier_field  # noqa: E402
+
+TOL = 1e-12
+MESHES = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+CASES = [("Mesh_10_form.FEM", [1], 3), ("Mesh_20_form.FEM", [1, 2], 4),
+         ("Mesh_20_form.FEM", [1, 2], 64), ("Mesh_10_form.FEM", [1, 2], 128)]
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+
+
+def load(name, doms):
+    return load_fem_p2(os.path.join(MESHES, name), doms)
+
+
+@pytest.mark.parametrize("name,doms,M", CASES)
+def test_cross_curl(name, doms, M):
+    print(f"case cross_curl {name} domains={doms} M={M}", flush=True)
+    m = load(name, doms)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    u = fourier_field(m.np, 6, M, 211, 2.0)
+    out = np.zeros((m.n_gauss, 6, M), order="F")
+    NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, out)
+    assert rel(out, smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u)) <= TOL
+
+
+@pytest.mark.parametrize("name,doms,M", CASES[:3])
+def test_curlh_cross_b_and_ugradc(name, doms, M):
+    print(f"case curlh_cross_b/ugradc {name} domains={doms} M={M}", flush=True)
+    m = load(name, doms)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    H, B = fourier_field(m.np, 6, M, 212, 2.0), fourier_field(m.np, 6, M, 213, 2.0)
+    out = np.zeros((m.n_gauss, 6, M), order="F")
+    NL.smb_CurlH_cross_B_gauss_sft_par(None, mesh, modes, H, B, out)
+    assert rel(out, smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, H, B)) <= TOL
+    c = fourier_field(m.np, 2, M, 214, 2.0)
+    hc = np.random.default_rng(215).uniform(0.5, 2.0, m.me)
+    out2 = np.zeros((m.n_gauss, 2, M), order="F")
+    NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, H, c, out2)
+    assert rel(out2, smb.smb_ugradc_gauss_fft_par(m.jj, m.rr, modes, hc, H, c)) <= TOL
+
+
+@pytest.mark.parametrize("name,doms,M", CASES[:3])
+def test_rhs_ns_3x3(name, doms, M):
+    print(f"case rhs_ns_3x3 {name} domains={doms} M={M}", flush=True)
+    m = load(name, doms)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    rng = np.random.default_rng(216)
+    F = lambda *s: np.asfortranarray(rng.standard_normal(s))
+    rotv, V1m, ff, P, Pinc = F(m.n_gauss, 6, M), F(m.np, 6, M), F(m.np, 6, M), F(m.np, 2, M), F(m.np, 2, M)
+    v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), fortran_tensor(V1m), fortran_tensor(ff),
+                  fortran_tensor(P), fortran_tensor(Pinc), v1, v2)
+    torch.cuda.synchronize()
+    g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, modes, V1m, ff, P, Pinc, rotv)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+
+
+@pytest.mark.parametrize("name,doms,M", CASES)
+def test_tangent_adjoint_dot_product(name, doms, M):
+    print(f"case dot-product {name} domains={doms} M={M}", flush=True)
+    m = load(name, doms)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    U, D = fourier_field(m.np, 6, M, 217, 2.0), fourier_field(m.np, 6, M, 218, 2.0)
+    Y = np.asfortranarray(np.random.default_rng(219).standard_normal((m.n_gauss, 6, M)))
+    TD = np.zeros((m.n_gauss, 6, M), order="F")
+    TtY = np.zeros((m.np, 6, M), order="F")
+    NL.nl_tangent(mesh, modes, U, D, TD)
+    NL.nl_adjoint(mesh, modes, U, Y, TtY)
+    exp = (smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, U, D)
+           + smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, D, U))
+    assert rel(TD, exp) <= TOL
+    assert abs(np.sum(TD * Y) - np.sum(D * TtY)) <= 1e-12 * np.linalg.norm(TD) * np.linalg.norm(Y)
