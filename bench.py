@@ -30,6 +30,8 @@ coefficients, mode-0 sine set to zero):
   0.1) on a 50k-triangle mesh (158x316), m_max = 32.
 * ``batch``: 32 independent problems (2k-50k triangles, m_max in {4..128})
   in one persistent launch replayed from a hipGraph, schedule autotuned.
+* ``nsrhs``: the full per-mode NS right-hand side with rhs_3x3's tensor
+  (LES or variable viscosity) on the 224x448 mesh, m_max = 64.
 * ``full``: the explicit nonlinear step at m_max = 128 on the 224x448 mesh:
   (curl u) x u -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and
   (rho u) (x) u with a level-set density, and the convective adjoint.
@@ -155,6 +157,41 @@ def wl_mhd(torch, args):
                        "u x B at nodes",
            "me": m.me, "np": npn, "gauss_points": G, "m_max": M, "grid": 3 * mp_of(M)}
     return phases, G * M, cfg, ("mhd", m, M)
+
+
+def wl_nsrhs(torch, args):
+    from sfemans_amd import fft_parallele as FP
+    from sfemans_amd import nonlinear as NL
+    from sfemans_amd.mesh import rectangle_p2
+    m = rectangle_p2(224, 448)
+    mesh = NL.Mesh.from_p2(m)
+    M = 64
+    modes = list(range(M))
+    G, npn = m.n_gauss, m.np
+    v1m, fsrc = _fields(torch, npn, M, 4.0, (SEED + 6, SEED + 7))
+    pr, pinc = _fields(torch, npn, M, 4.0, (SEED + 8, SEED + 9), ncol=2)
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 10)
+    rotv = torch.randn((M, 6, G), dtype=torch.float64, device="cuda", generator=gen).permute(2, 1, 0)
+    tens = torch.randn((M, 6, G, 3), dtype=torch.float64, device="cuda", generator=gen).permute(3, 2, 1, 0)
+    v145 = FP.fortran_empty((npn, 3, M))
+    v236 = FP.fortran_empty((npn, 3, M))
+    welem = torch.empty(36 * m.me * M, dtype=torch.float64, device="cuda")
+    # rotv_v and the tensor read once, V1m, ff (6 columns) and P, P_inc (2 columns) read,
+    # element contributions written and read back, vb written, mesh read
+    rest = (G * 6 * M * 8 + npn * 16 * M * 8 + 2 * 36 * m.me * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8
+            + 12 * 6 * m.me + npn * 16)
+    phases = [
+        Phase("rhs_ns_3x3_tensor",
+              lambda: NL.rhs_ns_3x3(mesh, modes, rotv, v1m, fsrc, pr, pinc, v145, v236, welem, opt_tensor=tens),
+              rest + G * 18 * M * 8, G * M * (6 * 2 * 6 + 16 * 6 * 2 + 6 * 6 * 6 + 20),
+              "rhs_ns_elem_kernel<true> + rhs_gather_kernel"),
+    ]
+    cfg = {"workload": "full per-mode NS right-hand side with rhs_3x3's tensor (LES / variable viscosity), "
+                       "224x448 P2 mesh (200,704 triangles, 1,404,928 Gauss points), m_max=64: "
+                       "rhs_ns_gauss_3x3 (BDF term, source, pressure gradients, -rotv_v) + rhs_3x3 with "
+                       "opt_tensor(3,G,6,m)",
+           "me": m.me, "np": npn, "gauss_points": G, "m_max": M}
+    return phases, G * M, cfg, ("nsrhs", m, M)
 
 
 def wl_fftprod(torch, args):
@@ -311,7 +348,7 @@ def wl_full(torch, args):
 
 
 WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch,
-             "full": wl_full}
+             "full": wl_full, "nsrhs": wl_nsrhs}
 
 
 # ------------------------------------------------------------- CPU baseline
@@ -333,6 +370,30 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
     from oracle import cpu as OC
     from oracle import fft_par as F
     from sfemans_amd.synth import fourier_field
+    if kind == "nsrhs":
+        from oracle import assembly
+        from sfemans_amd.mesh import rectangle_p2
+        # the oracle's rhs_ns_gauss_3x3 + rhs_3x3 with opt_tensor on a 16x16 P2 mesh, same m_max
+        sm = rectangle_p2(16, 16)
+        rng = np.random.default_rng(3)
+        F = lambda *sh: rng.standard_normal(sh)
+        lm = np.arange(M)
+        ins = (F(sm.np, 6, M), F(sm.np, 6, M), F(sm.np, 2, M), F(sm.np, 2, M), F(sm.n_gauss, 6, M))
+        T = F(3, sm.n_gauss, 6, M)
+
+        def run():
+            g = assembly.rhs_ns_gauss_3x3(sm.jj, sm.rr, lm, *ins)
+            assembly.rhs_3x3_all_modes(sm.jj, sm.rr, g, lm, T)
+        run()
+        n, t0 = 0, time.perf_counter()
+        t_end = t0 + sample_s
+        while time.perf_counter() < t_end or n == 0:
+            run()
+            n += 1
+        dt = time.perf_counter() - t0
+        return {"value": n * sm.n_gauss * M / dt, "unit": "Gauss-pt*modes/s", "cores": 1, "kind": "port",
+                "sample": f"rhs_ns_gauss_3x3 + rhs_3x3 with opt_tensor restated in numpy, {n} calls on a "
+                          f"16x16 P2 mesh ({sm.me} triangles) x m_max={M}, 1 thread, {dt:.1f} s"}
     if kind in ("nl", "mhd", "tangent", "batch", "full") and OC.available():
         from sfemans_amd.mesh import rectangle_p2
         # forward nonlinear term on a 64x64 P2 mesh (8,192 triangles), same m_max

@@ -214,19 +214,24 @@ int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, d
                     double scale, double* work_elem, void* stream);
 /* Replaces rhs_ns_gauss_3x3 without the penalty options
  * (SOURCES/SOLVER/computing_rhs_gauss.F90:7-146) followed by rhs_3x3 of its
- * output for every mode (SOURCES/SIMULATION/subroutine_ns_with_velocity.F90:313-328,
- * SOURCES/SOLVER/assembling_rhs.F90:5), fused: the Gauss array rhs_gauss is
- * never stored.  Per Gauss point rhs = (ft + fp_inc + fs - rotv_v) + fp,
+ * output for every mode, with or without opt_tensor
+ * (SOURCES/SIMULATION/subroutine_ns_with_velocity.F90:313-328,
+ * SOURCES/SOLVER/assembling_rhs.F90:5-126), fused: the Gauss array rhs_gauss
+ * is never stored.  Per Gauss point rhs = (ft + fp_inc + fs - rotv_v) + fp,
  * with ft, fs the Gauss values of V1m and of the nodal momentum source ff,
  * and fp, fp_inc the pressure-gradient terms of P and P_inc, the pressure
  * and its increment injected onto the velocity mesh (the reference's
- * inject_generic, done by the caller).  Device arrays: list_mode_dev(m),
- * rotv_v(G,6,m), V1m(np,6,m), ff(np,6,m), P(np,2,m), P_inc(np,2,m); a null
+ * inject_generic, done by the caller).  opt_tensor(3,G,6,m) is rhs_3x3's
+ * optional tensor (the LES or variable-viscosity stress the caller passes
+ * negated, subroutine_ns_with_velocity.F90:321-326), assembled in its weak
+ * divergence form.  Device arrays: list_mode_dev(m), rotv_v(G,6,m),
+ * V1m(np,6,m), ff(np,6,m), P(np,2,m), P_inc(np,2,m), opt_tensor; a null
  * field counts as zero.  Outputs vb_145(np,3,m), vb_236(np,3,m); work_elem
  * is a caller-provided device buffer of 36*me*m doubles. */
 int sfx_rhs_ns_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* rotv_v,
                        const double* V1m, const double* ff, const double* P, const double* P_inc,
-                       double* work_elem, double* vb_145, double* vb_236, void* stream);
+                       const double* opt_tensor, double* work_elem, double* vb_145, double* vb_236,
+                       void* stream);
 
 /* Replaces qs_00_gauss (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) from
  * heat_capa(me), nodal ff(np) and Gauss ff_gauss(G). */

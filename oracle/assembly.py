@@ -1,12 +1,18 @@
 """Element-to-node right-hand-side assembly (restatement).
 
 ``rhs_3x3`` follows SOURCES/SOLVER/assembling_rhs.F90:5-126, 185-188 without
-``opt_tensor``/``opt_tensor_scaln_bdy``: for each element and Gauss point
-``smb = rhs_in(index,:)*ray*rj(l,m)`` (:71-73); components (1,4,5) go to
-``vb_145`` and (2,3,6) to ``vb_236`` for velocity components ki = 1,2,3
-(:75-95), added at row ``loc_to_glob(ki, jj(ni,m))``.  On one rank
+``opt_tensor_scaln_bdy``: for each element and Gauss point
+``smb = (rhs_in(index,:) + f_tensor)*ray*rj(l,m)`` (:71-73); components
+(1,4,5) go to ``vb_145`` and (2,3,6) to ``vb_236`` for velocity components
+ki = 1,2,3 (:75-95), added at row ``loc_to_glob(ki, jj(ni,m))``.  On one rank
 ``loc_to_glob(ki, i) = (ki-1)*np + i`` (SOURCES/INITIALISATION/st_matrix.F90:101),
-so the dense vectors here are (np, 3), column ki.
+so the dense vectors here are (np, 3), column ki.  With ``opt_tensor``
+T(3, G, 6) per mode (row k = velocity component, column = the 6 (cos, sin)
+columns of the derivative direction r, theta, z), ``f_tensor`` holds its
+1/r and mode terms (:56-68) and every node adds
+``(dw(1,ni)*T(ki, c) + dw(2,ni)*T(ki, c+4))*ray*rj`` with c = 1 for the
+cos slot and 2 for the sin slot of row ki (:97-121): the weak form
+``int T : grad(phi) r dr dz`` of the tensor divergence.
 
 ``qs_00_gauss`` follows SOURCES/SOLVER/fem_rhs_axi.F90:940-987:
 ``fl = (heat_capa(m)*SUM(ff_loc*ww(:,l)) + ff_gauss(index))*rj(l,m)*ray``.
@@ -54,14 +60,36 @@ def rhs_3x3(jj, rr, rhs_in):
     return v145, v236
 
 
-def rhs_3x3_all_modes(jj, rr, rhs_gauss):
-    """All modes: (G, 6, M) -> ``(vb_145, vb_236)`` each (np, 3, M)."""
+def rhs_3x3_all_modes(jj, rr, rhs_gauss, list_mode=None, opt_tensor=None):
+    """All modes: (G, 6, M) -> ``(vb_145, vb_236)`` each (np, 3, M).
+
+    ``opt_tensor``: (3, G, 6, M) or None; ``list_mode`` (M,) is needed with it.
+    """
     ww, dw, rj, ray = gauss_tables(jj, rr)
     me = np.shape(jj)[1]
     M = rhs_gauss.shape[2]
     f = np.asarray(rhs_gauss, dtype=np.float64).reshape(me, 7, 6, M)
-    smb = f * (ray.T * rj.T)[:, :, None, None]
-    loc = np.einsum("nl,elkm->enkm", ww, smb)              # (me, 6, 6, M)
+    wgt = (ray.T * rj.T)[:, :, None, None]                 # (me, 7, 1, 1)
+    if opt_tensor is not None:
+        T = np.asarray(opt_tensor, dtype=np.float64).reshape(3, me, 7, 6, M)
+        mode = np.asarray(list_mode, dtype=np.float64)[None, None, :]
+        r = ray.T[:, :, None]
+        ft = np.empty((me, 7, 6, M))
+        ft[:, :, 0] = (-mode * T[0, :, :, 3] + T[1, :, :, 2]) / r
+        ft[:, :, 1] = (mode * T[0, :, :, 2] + T[1, :, :, 3]) / r
+        ft[:, :, 2] = (-T[0, :, :, 2] - mode * T[1, :, :, 3]) / r
+        ft[:, :, 3] = (-T[0, :, :, 3] + mode * T[1, :, :, 2]) / r
+        ft[:, :, 4] = -T[2, :, :, 3] * mode / r
+        ft[:, :, 5] = T[2, :, :, 2] * mode / r
+        f = f + ft
+    loc = np.einsum("nl,elkm->enkm", ww, f * wgt)          # (me, 6, 6, M)
+    if opt_tensor is not None:
+        # component c: row c // 2 of T, its cos (c even) or sin (c odd) columns
+        for c in range(6):
+            k, p = c // 2, c % 2
+            g = (np.einsum("nle,elm->enlm", dw[0], T[k, :, :, p])
+                 + np.einsum("nle,elm->enlm", dw[1], T[k, :, :, 4 + p]))   # (me, 6, 7, M)
+            loc[:, :, c, :] += np.einsum("enlm,el->enm", g, ray.T * rj.T)
     npts = np.shape(rr)[1]
     v145 = _nodal_sum(jj, npts, loc[:, :, list(_C145), :])
     v236 = _nodal_sum(jj, npts, loc[:, :, list(_C236), :])

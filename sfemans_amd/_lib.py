@@ -66,7 +66,7 @@ SIGNATURES = {
     "sfx_nl_adjoint_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_rhs_3x3": [_vp, _i32, _vp, _vp, _vp],
     "sfx_rhs_3x3_dev": [_vp, _i32, _vp, _vp, _vp, _d, _vp, _vp],
-    "sfx_rhs_ns_3x3_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "sfx_rhs_ns_3x3_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],
     "sfx_qs_00_gauss_dev": [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_nl_rhs_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp],

@@ -453,15 +453,15 @@ int rhs3x3_dev(sfx_mesh_t mesh, int32_t M, const double* rhs, double* v145, doub
 }
 
 int rhs_ns_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M, const double* rotv_v, const double* V1m,
-               const double* ff, const double* P, const double* P_inc, double* work_elem, double* v145,
-               double* v236, hipStream_t st) {
+               const double* ff, const double* P, const double* P_inc, const double* tensor, double* work_elem,
+               double* v145, double* v236, hipStream_t st) {
   if (!work_elem && mesh->me > 0 && M > 0) return fail(SFX_ERR_ARG, "rhs_ns_3x3: null work_elem");
   if (!list_mode_dev && M > 0) return fail(SFX_ERR_ARG, "rhs_ns_3x3: null list_mode");
   RhsArgs a = rhs_args(mesh, M, 1.0);
   a.rr = mesh->rr; a.lmode = list_mode_dev;
-  a.rhs = rotv_v; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = P_inc;
+  a.rhs = rotv_v; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = P_inc; a.tens = tensor;
   a.v145 = v145; a.v236 = v236; a.elo = work_elem;
-  a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = a.np * 6 * (int64_t)M;
+  a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = a.np * 6 * (int64_t)M; a.n_tens = 3 * a.n_rhs;
   a.n_pres = a.n_pinc = a.np * 2 * (int64_t)M;
   a.n_out = a.np * 3 * (int64_t)M; a.n_elo = 36 * a.me * (int64_t)M;
   hipError_t e = launch_rhs_ns(a, st);
@@ -474,12 +474,12 @@ extern "C" {
 
 int sfx_rhs_ns_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* rotv_v,
                        const double* V1m, const double* ff, const double* P, const double* P_inc,
-                       double* work_elem, double* vb_145, double* vb_236, void* stream) {
+                       const double* opt_tensor, double* work_elem, double* vb_145, double* vb_236, void* stream) {
   clear_err();
   if (!mesh) return fail(SFX_ERR_ARG, "rhs_ns_3x3: null mesh");
   if (m_max_c < 0) return fail(SFX_ERR_ARG, "rhs_ns_3x3: m_max_c < 0");
-  return rhs_ns_dev(mesh, list_mode_dev, m_max_c, rotv_v, V1m, ff, P, P_inc, work_elem, vb_145, vb_236,
-                    (hipStream_t)stream);
+  return rhs_ns_dev(mesh, list_mode_dev, m_max_c, rotv_v, V1m, ff, P, P_inc, opt_tensor, work_elem, vb_145,
+                    vb_236, (hipStream_t)stream);
 }
 
 int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, double* vb_145, double* vb_236,

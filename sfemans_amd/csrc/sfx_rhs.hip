@@ -11,8 +11,9 @@ __global__ void __launch_bounds__(256) rhs_elem_kernel(RhsArgs a) {
   rhs_elem_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
+template <bool TENSOR>
 __global__ void __launch_bounds__(256) rhs_ns_elem_kernel(RhsArgs a) {
-  rhs_ns_elem_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+  rhs_ns_elem_body<TENSOR>(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(GATHER_THREADS) rhs_gather_kernel(RhsArgs a) {
@@ -43,7 +44,12 @@ hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st) {
 // Full NS right-hand side fused with its element share, then the gather.
 hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st) {
   const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0, nn = a.np * a.M;
-  if (ne > 0) hipLaunchKernelGGL(rhs_ns_elem_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
+  if (ne > 0) {
+    if (a.tens)
+      hipLaunchKernelGGL(rhs_ns_elem_kernel<true>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL(rhs_ns_elem_kernel<false>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nn == 0) return e;
   const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;

@@ -46,7 +46,8 @@ struct RhsArgs {
   const double* fsrc;       // (np, 6, M) nodal momentum source
   const double* pres;       // (np, 2, M) pressure on the velocity mesh
   const double* pinc;       // (np, 2, M) pressure increment on the velocity mesh
-  int64_t n_v1m, n_fsrc, n_pres, n_pinc;
+  const double* tens;       // (3, G, 6, M) opt_tensor of rhs_3x3
+  int64_t n_v1m, n_fsrc, n_pres, n_pinc, n_tens;
 };
 
 // Work item gid of the element kernels -> (element e, mode or column m).  A
@@ -115,7 +116,13 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
 //   fp = (-d_r P_c, -d_r P_s, -P_s/ray*m, P_c/ray*m, -d_z P_c, -d_z P_s) (:69-97)
 // with P and P_inc the pressure injected onto the velocity mesh; then
 //   elo(6e+i, k, m) = sum_l ww(i,l) wgt(e,l) rhs(l, k)   as rhs_elem_body.
+// With TENSOR, rhs_3x3's opt_tensor T(3, G, 6, M) (assembling_rhs.F90:56-121):
+// node i of component k adds, per Gauss point,
+//   (ww(i,l) f_tensor(k) + d_r phi_i T(k/2, k%2) + d_z phi_i T(k/2, 4 + k%2)) wgt(e,l)
+// with f_tensor the 1/r and mode terms (row k/2 of T, its cos columns for
+// even k, sin for odd).
 // The Gauss array rhs_gauss is never stored.
+template <bool TENSOR>
 SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
   int64_t e;
   int m;
@@ -207,6 +214,45 @@ SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
       const double r = ((ft[k] + fq[k] + fs[k] - rv) + fp[k]) * wl;
 #pragma unroll
       for (int i = 0; i < 6; ++i) loc[i][k] = fma(w[i], r, loc[i][k]);
+    }
+  }
+  // The tensor terms in a second pass over the Gauss points, once the nodal
+  // fields are dead: the two passes never hold both in registers.
+  if constexpr (TENSOR) {
+#pragma unroll 1
+    for (int l = 0; l < 7; ++l) {
+      const double* w = a.tab + 6 * l;
+      const double* d = a.tab + 42 + 12 * l;
+      double ray = 0;
+#pragma unroll
+      for (int n = 0; n < 6; ++n) ray = fma(rn[n], w[n], ray);
+      const double wl = a.wgt[7 * e + l];
+      const int64_t gi = 7 * e + l + a.G * 6 * (int64_t)m;
+      double T[3][6];
+#pragma unroll
+      for (int t = 0; t < 6; ++t)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int64_t it = k + 3 * (gi + a.G * t);
+          SFX_CHECK("tensor load", it, a.n_tens);
+          T[k][t] = a.tens[it];
+        }
+      double fT[6];
+      fT[0] = (-mv * T[0][3] + T[1][2]) / ray;
+      fT[1] = (mv * T[0][2] + T[1][3]) / ray;
+      fT[2] = (-T[0][2] - mv * T[1][3]) / ray;
+      fT[3] = (-T[0][3] + mv * T[1][2]) / ray;
+      fT[4] = -T[2][3] * mv / ray;
+      fT[5] = T[2][2] * mv / ray;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const double dr = fma(a11, d[2 * i], a12 * d[2 * i + 1]) * wl;
+        const double dz = fma(a21, d[2 * i], a22 * d[2 * i + 1]) * wl;
+        const double wi = w[i] * wl;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          loc[i][k] = fma(wi, fT[k], fma(dr, T[k / 2][k % 2], fma(dz, T[k / 2][4 + k % 2], loc[i][k])));
+      }
     }
   }
 #pragma unroll

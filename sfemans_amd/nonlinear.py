@@ -178,21 +178,25 @@ def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0, work_ele
     return vb_145, vb_236
 
 
-def rhs_ns_3x3(mesh: Mesh, list_mode, rotv_v, V1m, ff, P, P_inc, vb_145, vb_236, work_elem=None):
+def rhs_ns_3x3(mesh: Mesh, list_mode, rotv_v, V1m, ff, P, P_inc, vb_145, vb_236, work_elem=None,
+               opt_tensor=None):
     """Full explicit NS right-hand side assembled for every mode (device arrays).
 
     rhs_ns_gauss_3x3 without the penalty options
     (SOURCES/SOLVER/computing_rhs_gauss.F90:7-146) followed by rhs_3x3
-    (SOURCES/SOLVER/assembling_rhs.F90:5), fused.  ``P``, ``P_inc``: the
-    pressure and its increment on the velocity mesh, (np, 2, m); any of
-    ``rotv_v``, ``V1m``, ``ff``, ``P``, ``P_inc`` may be None (zero).
+    (SOURCES/SOLVER/assembling_rhs.F90:5-126), fused.  ``P``, ``P_inc``: the
+    pressure and its increment on the velocity mesh, (np, 2, m);
+    ``opt_tensor``: rhs_3x3's tensor at the Gauss points, (3, G, 6, m).  Any
+    of ``rotv_v``, ``V1m``, ``ff``, ``P``, ``P_inc``, ``opt_tensor`` may be
+    None (zero).
     """
     M = len(list_mode)
     lm = mesh.list_mode_device(list_mode)
     work_elem = _work_elem(mesh, M, work_elem)
     opt = lambda x: 0 if x is None else ptr(x)
     check(lib().sfx_rhs_ns_3x3_dev(mesh.handle, lm.data_ptr(), M, opt(rotv_v), opt(V1m), opt(ff), opt(P),
-                                   opt(P_inc), work_elem.data_ptr(), ptr(vb_145), ptr(vb_236), stream_ptr()))
+                                   opt(P_inc), opt(opt_tensor), work_elem.data_ptr(), ptr(vb_145), ptr(vb_236),
+                                   stream_ptr()))
     return vb_145, vb_236
 
 

@@ -221,7 +221,7 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
 
 extern "C" int emu_rhs_ns(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm, int M,
                           const double* rotv, const double* V1m, const double* ff, const double* P,
-                          const double* Pinc, double* v145, double* v236) {
+                          const double* Pinc, const double* tens, double* v145, double* v236) {
   std::vector<int64_t> ptr;
   std::vector<int32_t> ent;
   build_adjacency(jj, me, np, ptr, ent);
@@ -235,11 +235,17 @@ extern "C" int emu_rhs_ns(const int32_t* jj, const double* rr, int64_t me, int64
   a.jj = jj; a.me = me; a.np = np; a.G = 7 * me; a.M = M; a.scale = 1.0; a.n_adj = (int64_t)ent.size();
   std::vector<double> elo((size_t)36 * me * M + 1);
   a.rr = rr; a.lmode = lm;
-  a.rhs = rotv; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = Pinc;
+  a.rhs = rotv; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = Pinc; a.tens = tens;
   a.v145 = v145; a.v236 = v236; a.elo = elo.data();
   a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = np * 6 * (int64_t)M; a.n_pres = a.n_pinc = np * 2 * (int64_t)M;
+  a.n_tens = 3 * a.n_rhs;
   a.n_out = np * 3 * (int64_t)M; a.n_elo = 36 * me * (int64_t)M;
-  for (int64_t g = 0; g < elem_items(me, M); ++g) rhs_ns_elem_body(a, g);
+  for (int64_t g = 0; g < elem_items(me, M); ++g) {
+    if (tens)
+      rhs_ns_elem_body<true>(a, g);
+    else
+      rhs_ns_elem_body<false>(a, g);
+  }
   run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<6>() + 1) / 2,
            [&](int t, int64_t b, double2* sm) { rhs_gather_block(a, b, t, reinterpret_cast<double*>(sm)); });
   return 0;

@@ -56,10 +56,30 @@ def test_rhs_ns_3x3(nx, ny, modes, with_all):
     lm = np.asarray(modes, dtype=np.int32)
     rotv, V1m, ff, P, Pinc = _ns_inputs(m, M, 3, with_all)
     v1 = np.zeros((m.np, 3, M), order="F"); v2 = np.zeros_like(v1)
-    rc, arrs = emu_runner.call("emu_rhs_ns", m.jj, m.rr, m.me, m.np, lm, M, rotv, V1m, ff, P, Pinc, v1, v2)
+    rc, arrs = emu_runner.call("emu_rhs_ns", m.jj, m.rr, m.me, m.np, lm, M, rotv, V1m, ff, P, Pinc, None, v1, v2)
     assert rc == 0
     z6, z2 = np.zeros((m.np, 6, M)), np.zeros((m.np, 2, M))
     g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, z6 if V1m is None else V1m, ff, P,
                                   z2 if Pinc is None else Pinc, rotv)
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g)
+    assert rel(arrs[-2], e1) <= 1e-12 and rel(arrs[-1], e2) <= 1e-12
+
+
+@pytest.mark.parametrize("nx,ny,modes,with_all", [(1, 1, [0], False), (3, 2, [0, 1, 2, 3], True),
+                                                  (2, 5, [0, 2, 3, 7, 8, 9, 11, 12, 13, 14, 15, 16, 17, 18,
+                                                          19, 20, 23], True)])
+def test_rhs_ns_3x3_with_tensor(nx, ny, modes, with_all):
+    """opt_tensor of rhs_3x3 (assembling_rhs.F90:56-121) in the fused kernel."""
+    m = rectangle_p2(nx, ny)
+    M = len(modes)
+    lm = np.asarray(modes, dtype=np.int32)
+    rotv, V1m, ff, P, Pinc = _ns_inputs(m, M, 4, with_all)
+    T = np.asfortranarray(np.random.default_rng(5).standard_normal((3, m.n_gauss, 6, M)))
+    v1 = np.zeros((m.np, 3, M), order="F"); v2 = np.zeros_like(v1)
+    rc, arrs = emu_runner.call("emu_rhs_ns", m.jj, m.rr, m.me, m.np, lm, M, rotv, V1m, ff, P, Pinc, T, v1, v2)
+    assert rc == 0
+    z6, z2 = np.zeros((m.np, 6, M)), np.zeros((m.np, 2, M))
+    g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, z6 if V1m is None else V1m, ff, P,
+                                  z2 if Pinc is None else Pinc, rotv)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g, lm, T)
     assert rel(arrs[-2], e1) <= 1e-12 and rel(arrs[-1], e2) <= 1e-12

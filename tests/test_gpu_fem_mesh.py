@@ -79,12 +79,19 @@ def test_rhs_ns_3x3(name, doms, M):
     rng = np.random.default_rng(216)
     F = lambda *s: np.asfortranarray(rng.standard_normal(s))
     rotv, V1m, ff, P, Pinc = F(m.n_gauss, 6, M), F(m.np, 6, M), F(m.np, 6, M), F(m.np, 2, M), F(m.np, 2, M)
+    T = F(3, m.n_gauss, 6, M)
     v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
     NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), fortran_tensor(V1m), fortran_tensor(ff),
                   fortran_tensor(P), fortran_tensor(Pinc), v1, v2)
     torch.cuda.synchronize()
     g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, modes, V1m, ff, P, Pinc, rotv)
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+    # with rhs_3x3's tensor
+    NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), fortran_tensor(V1m), fortran_tensor(ff),
+                  fortran_tensor(P), fortran_tensor(Pinc), v1, v2, opt_tensor=fortran_tensor(T))
+    torch.cuda.synchronize()
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g, modes, T)
     assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
 
 

@@ -205,6 +205,34 @@ def test_rhs_3x3_device_all_modes():
     assert rel(v1.cpu().numpy(), -2.0 * e1) <= 1e-13 and rel(v2.cpu().numpy(), -2.0 * e2) <= 1e-13
 
 
+@pytest.mark.parametrize("nx,ny,modes", [(9, 7, [0, 1, 2, 4, 5, 9, 11]), (6, 5, list(range(40)))])
+def test_rhs_ns_3x3_with_tensor_device(nx, ny, modes):
+    """rhs_3x3's opt_tensor (assembling_rhs.F90:56-121) in the fused NS
+    right-hand side vs the oracle, relative error <= 1e-12."""
+    from oracle import assembly
+    m = rectangle_p2(nx, ny)
+    mesh = NL.Mesh.from_p2(m)
+    M = len(modes)
+    print(f"case rhs_ns_3x3+tensor {m.me} triangles M={M}", flush=True)
+    rng = np.random.default_rng(10)
+    F = lambda *s: np.asfortranarray(rng.standard_normal(s))
+    rotv, V1m, ff = F(m.n_gauss, 6, M), F(m.np, 6, M), F(m.np, 6, M)
+    P, Pinc, T = F(m.np, 2, M), F(m.np, 2, M), F(3, m.n_gauss, 6, M)
+    v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), fortran_tensor(V1m), fortran_tensor(ff),
+                  fortran_tensor(P), fortran_tensor(Pinc), v1, v2, opt_tensor=fortran_tensor(T))
+    torch.cuda.synchronize()
+    lm = np.asarray(modes)
+    g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, V1m, ff, P, Pinc, rotv)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g, lm, T)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+    # the tensor alone
+    NL.rhs_ns_3x3(mesh, modes, None, None, None, None, None, v1, v2, opt_tensor=fortran_tensor(T))
+    torch.cuda.synchronize()
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, np.zeros((m.n_gauss, 6, M)), lm, T)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+
+
 @pytest.mark.parametrize("M", [3, 16])
 def test_nl_rhs_mhd_all_modes(M):
     from oracle import assembly

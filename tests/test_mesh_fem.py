@@ -164,7 +164,7 @@ def test_rhs_ns_3x3_on_fem_mesh(name, doms, M):
     F = lambda *s: np.asfortranarray(rng.standard_normal(s))
     rotv, V1m, ff, P, Pinc = F(m.n_gauss, 6, M), F(m.np, 6, M), F(m.np, 6, M), F(m.np, 2, M), F(m.np, 2, M)
     v1 = np.zeros((m.np, 3, M), order="F"); v2 = np.zeros_like(v1)
-    rc, arrs = emu_runner.call("emu_rhs_ns", m.jj, m.rr, m.me, m.np, lm, M, rotv, V1m, ff, P, Pinc, v1, v2)
+    rc, arrs = emu_runner.call("emu_rhs_ns", m.jj, m.rr, m.me, m.np, lm, M, rotv, V1m, ff, P, Pinc, None, v1, v2)
     assert rc == 0
     g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, V1m, ff, P, Pinc, rotv)
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g)

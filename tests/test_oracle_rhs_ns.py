@@ -72,3 +72,70 @@ def test_linear_in_all_inputs():
     fb = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, *b)
     fab = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, *[x + 2.0 * y for x, y in zip(a, b)])
     assert np.allclose(fab, fa + 2.0 * fb, atol=1e-12 * np.max(np.abs(fab)))
+
+
+def _quad_tensor(m, M, seed):
+    """T(3, G, 6, M) of per-(row, column, mode) quadratics at the Gauss points,
+    its value and r/z derivatives there, each (3, G, 6, M)."""
+    rg, zg = gauss_coords(m)
+    rng = np.random.default_rng(seed)
+    vals = [eval_quad(rng.standard_normal((6, M, 6)), rg, zg) for _ in range(3)]
+    T, Tr, Tz = (np.stack([v[i] for v in vals]) for i in range(3))
+    return T, Tr, Tz, rg
+
+
+def _minus_div(T, Tr, Tz, r, modes):
+    """-div T in (cos, sin) form, divergence over the column index, in
+    cylindrical coordinates:
+      (div T)_r = d_r T_rr + (1/r) d_th T_rth + d_z T_rz + (T_rr - T_thth)/r
+      (div T)_th = d_r T_thr + (1/r) d_th T_thth + d_z T_thz + (T_thr + T_rth)/r
+      (div T)_z = d_r T_zr + (1/r) d_th T_zth + d_z T_zz + T_zr/r
+    with (1/r) d_th (A cos + B sin) = (m B/r) cos - (m A/r) sin."""
+    mr = modes[None, :] / r[:, None]
+    ir = 1.0 / r[:, None]
+    d = np.empty(T.shape[1:])
+    for k in range(3):
+        for s in range(2):               # s = 0: cos, 1: sin
+            th = mr * T[k, :, 3] if s == 0 else -mr * T[k, :, 2]
+            d[:, 2 * k + s] = Tr[k, :, s] + th + Tz[k, :, 4 + s]
+    d[:, 0] += (T[0, :, 0] - T[1, :, 2]) * ir
+    d[:, 1] += (T[0, :, 1] - T[1, :, 3]) * ir
+    d[:, 2] += (T[1, :, 0] + T[0, :, 2]) * ir
+    d[:, 3] += (T[1, :, 1] + T[0, :, 3]) * ir
+    d[:, 4] += T[2, :, 0] * ir
+    d[:, 5] += T[2, :, 1] * ir
+    return -d
+
+
+def test_tensor_term_is_the_weak_form_of_minus_the_divergence():
+    """Integration by parts pins opt_tensor: with T quadratic, every integrand
+    is a polynomial of degree <= 4, so the 7-point rule is exact and, at
+    every node whose test function vanishes on the boundary r = 1, z = +-1
+    (the r = 0 term carries the factor r), rhs_3x3 with opt_tensor = T
+    equals rhs_3x3 of rhs_in = -div T."""
+    m = rectangle_p2(4, 5)
+    modes = np.array([0, 1, 2, 5])
+    M = len(modes)
+    T, Tr, Tz, rg = _quad_tensor(m, M, 8)
+    zero = np.zeros((m.n_gauss, 6, M))
+    a145, a236 = assembly.rhs_3x3_all_modes(m.jj, m.rr, zero, modes, T)
+    b145, b236 = assembly.rhs_3x3_all_modes(m.jj, m.rr, _minus_div(T, Tr, Tz, rg, modes))
+    inner = (m.rr[0] < 1 - 1e-12) & (np.abs(m.rr[1]) < 1 - 1e-12)
+    scale = np.max(np.abs(b145[inner])) + np.max(np.abs(b236[inner]))
+    assert np.max(np.abs(a145[inner] - b145[inner])) <= 1e-12 * scale
+    assert np.max(np.abs(a236[inner] - b236[inner])) <= 1e-12 * scale
+    # on the boundary the two differ by the flux term
+    assert np.max(np.abs(a145[~inner] - b145[~inner])) > 1e-3 * scale
+
+
+def test_tensor_term_adds_to_rhs_in():
+    m = rectangle_p2(3, 2)
+    modes = np.array([0, 3])
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal((m.n_gauss, 6, 2))
+    T = rng.standard_normal((3, m.n_gauss, 6, 2))
+    a = assembly.rhs_3x3_all_modes(m.jj, m.rr, f, modes, T)
+    b = assembly.rhs_3x3_all_modes(m.jj, m.rr, f)
+    c = assembly.rhs_3x3_all_modes(m.jj, m.rr, np.zeros_like(f), modes, T)
+    for x, y, z in zip(a, b, c):
+        assert np.allclose(x, y + z, atol=1e-13 * np.max(np.abs(x)))
