@@ -13,6 +13,14 @@ point (``fft_parallel.F90:225-324`` for the cross product):
 5. keep modes ``0..m_max-1``: mode 0 as is, mode m>=1 as ``2*conj`` (``:295-299``),
    unpacked as (real, imaginary) = (cos, sin) columns (``:311-324``).
 
+Non-polynomial products (division, square root, the smoothed Heaviside of
+level sets, the surface-tension tensor, the velocity maximum) depend on the
+grid size, so they are evaluated on exactly ``N = 2*m_max_pad-1`` points as
+the reference does: ``fft_par_div_dcl`` (:667-830), ``fft_scalar_vect_dcl``
+with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
+:2234-2246), ``fft_squareroot_scalar`` (:4601-4771), ``fft_tensor_dcl`` with
+``opt_tension`` (:1688-1706) and ``fft_max_vel_dcl`` (:1426-1541).
+
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
 used for larger parity cases and the CPU baseline) and ``"direct"`` (a direct
 sum in 80-bit long double, the pin for the fast engine on small inputs).
@@ -27,7 +35,8 @@ __all__ = [
     "n_grid", "pack", "c2r", "r2c", "unpack", "fft_product",
     "fft_par_real", "fft_par_cross_prod_dcl", "fft_par_dot_prod_dcl",
     "fft_par_prod_dcl", "fft_scalar_vect_dcl", "fft_tensor_dcl",
-    "OPS",
+    "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
+    "regul", "OPS",
 ]
 
 _PI_LD = np.longdouble("3.141592653589793238462643383279502884197")
@@ -134,7 +143,38 @@ def _op_tensor(ru):
     return (ru[:, 0:3, None, :] * ru[:, None, 3:6, :]).reshape(ru.shape[0], 9, ru.shape[-1])
 
 
+def _op_div(ru):
+    # prod_ru = ru(:,1,:)/ru(:,2,:)   (fft_parallel.F90:790)
+    return ru[:, 0:1] / ru[:, 1:2]
+
+
+def _op_sqrt(ru):
+    # SQRT(MAX(0, MIN(1, ru(:,1,n))))   (fft_parallel.F90:4724)
+    return np.sqrt(np.maximum(0.0, np.minimum(1.0, ru[:, 0:1])))
+
+
+def _op_tension(ru):
+    # prod_ru_a(:, b) = ru(:,a)*ru(:,3+b)/(|g| + 1e-14) - delta_ab |g|   (fft_parallel.F90:1688-1706)
+    g = ru[:, 3:6]
+    nrm = np.sqrt(g[:, 0] ** 2 + g[:, 1] ** 2 + g[:, 2] ** 2)
+    t = ru[:, 0:3, None, :] * g[:, None, :, :] / (nrm + 1e-14)[:, None, None, :]
+    for a in range(3):
+        t[:, a, a, :] -= nrm
+    return t.reshape(ru.shape[0], 9, ru.shape[-1])
+
+
+def regul(phi, eps):
+    """Smoothed Heaviside of a level set (fft_parallel.F90:2234-2246):
+    0 below 0.5-eps, 1 above 0.5+eps, (1 + x(x^2-3eps^2)/(-2eps^3))/2 between."""
+    x = np.asarray(phi, dtype=np.float64) - 0.5
+    mid = (1 + x * (x ** 2 - 3 * eps ** 2) / (-2 * eps ** 3)) / 2
+    return np.where(x <= -eps, 0.0, np.where(x <= eps, mid, 1.0))
+
+
 OPS = {
+    "div": _op_div,
+    "sqrt": _op_sqrt,
+    "tension": _op_tension,
     "cross": _op_cross,
     "dot": _op_dot,
     "prod": _op_prod,
@@ -217,17 +257,18 @@ def fft_scalar_vect_dcl(V1_in, V2_in, pb, m_max_pad, engine="fft"):
     return fft_product([V1_in, V2_in], f"scalar_vect_{pb}", m_max_pad, engine)
 
 
-def fft_tensor_dcl(V1_in, V2_in, m_max_pad, engine="fft"):
-    """FFT_TENSOR_DCL without opt_tension (fft_parallel.F90:1543-1796).
+def fft_tensor_dcl(V1_in, V2_in, m_max_pad, engine="fft", opt_tension=False):
+    """FFT_TENSOR_DCL (fft_parallel.F90:1543-1796).
 
     Returns V_out(3, np, 6, M) with ``V_out(a, n, 2b-1:2b, m)`` the (cos, sin)
-    coefficients of ``V1_a * V2_b``.
+    coefficients of ``V1_a * V2_b``, or with ``opt_tension`` of
+    ``V1_a V2_b/(|V2| + 1e-14) - delta_ab |V2|`` (:1688-1706).
     """
     nb = np.shape(V1_in)[1]
     _check(nb, np.shape(V1_in)[2], "FFT_TENSOR_DCL")
     if nb != 6:
         raise ValueError("tensor product needs nb_field == 6")
-    out = fft_product([V1_in, V2_in], "tensor", m_max_pad, engine)  # (np, 18, M)
+    out = fft_product([V1_in, V2_in], "tension" if opt_tension else "tensor", m_max_pad, engine)
     npt, _, M = out.shape
     # column 2*(3a+b) / 2*(3a+b)+1 -> V_out(a, n, 2b, m) / (a, n, 2b+1, m)
     res = np.empty((3, npt, 6, M))
@@ -236,3 +277,57 @@ def fft_tensor_dcl(V1_in, V2_in, m_max_pad, engine="fft"):
             res[a, :, 2 * b, :] = out[:, 2 * (3 * a + b), :]
             res[a, :, 2 * b + 1, :] = out[:, 2 * (3 * a + b) + 1, :]
     return res
+
+
+def fft_par_div_dcl(c1_in, c2_in, m_max_pad, engine="fft"):
+    """FFT_PAR_DIV_DCL (fft_parallel.F90:667-830): c1 / c2 on the grid, (np,2,M)."""
+    if np.shape(c1_in)[2] == 0:
+        raise ValueError("BUG in FFT_PAR_DIV_DCL: m_max_c==0")
+    return fft_product([np.asarray(c1_in)[:, 0:2], np.asarray(c2_in)[:, 0:2]], "div", m_max_pad, engine)
+
+
+def fft_squareroot_scalar(c1_in, m_max_pad, engine="fft"):
+    """FFT_SQUAREROOT_SCALAR (fft_parallel.F90:4601-4771): SQRT(MAX(0, MIN(1, c)))
+    of the first scalar; the reference overwrites c1_inout(:, 1:2, :)."""
+    c = np.asarray(c1_in, dtype=np.float64)
+    _check(c.shape[1], c.shape[2], "FFT_SQUAREROOT_SCALAR")
+    return fft_product([c[:, 0:2]], "sqrt", m_max_pad, engine)
+
+
+def fft_heaviside_dcl(V1_in, values, m_max_pad, coeff_tanh, engine="fft"):
+    """FFT_HEAVISIDE_DCL (fft_parallel.F90:832-1019): V1_in(n_int, np, 2, M)
+    level sets, ``values`` (n_int+1); on the grid
+    ``rho = values(1)``, then for each interface i
+    ``rho = rho*(1 - regul(phi_i)) + values(i+1)*regul(phi_i)``; (np, 2, M)."""
+    V = np.asarray(V1_in, dtype=np.float64)
+    vals = np.asarray(values, dtype=np.float64)
+    n_int = vals.size - 1
+    _check(V.shape[2], V.shape[3], "FFT_HEAVISIDE_DCL")
+
+    def op(ru):
+        rho = np.full((ru.shape[0], 1, ru.shape[-1]), vals[0])
+        for i in range(n_int):
+            h = regul(ru[:, i:i + 1], coeff_tanh)
+            rho = rho * (1.0 - h) + vals[i + 1] * h
+        return rho
+    # the fields of all interfaces side by side: (np, 2*n_int, M)
+    fields = np.concatenate([V[i, :, 0:2, :] for i in range(n_int)], axis=1)
+    return fft_product([fields], op, m_max_pad, engine)
+
+
+def fft_max_vel_dcl(V1_in, m_max_pad, engine="fft"):
+    """FFT_MAX_VEL_DCL (fft_parallel.F90:1426-1541): max over points and grid
+    angles of |v| (nb_field 6) or |c| (nb_field 2)."""
+    V = np.asarray(V1_in, dtype=np.float64)
+    npt, nb, M = V.shape
+    _check(nb, M, "FFT_MAX_VEL_DCL")
+    N = n_grid(m_max_pad)
+    best = 0.0
+    for s in range(0, npt, _CHUNK):
+        ru = np.asarray(c2r(pack(V[s:s + _CHUNK], m_max_pad), N, engine), dtype=np.float64)
+        if nb == 6:
+            nrm = np.sqrt(ru[:, 0] ** 2 + ru[:, 1] ** 2 + ru[:, 2] ** 2)
+        else:
+            nrm = np.sqrt(ru[:, 0] ** 2)
+        best = max(best, float(np.max(nrm)))
+    return best

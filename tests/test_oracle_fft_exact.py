@@ -1,0 +1,201 @@
+"""Pins for the oracle's non-polynomial FFT_PAR_* products (CPU).
+
+These products are evaluated on exactly N = 2*m_max_pad-1 angles, so every
+pin below fixes the whole chain (packing, c2r, pointwise operation, r2c,
+scaling, truncation) through a closed form that does not use the oracle:
+
+* division and pb = 2 scalar-vector: the quotient of q1*q2 by q2 is q1 when
+  q1*q2 is band-limited (its coefficients by the product-to-sum rule);
+* square root: the root of q**2 is q for 0 <= q <= 1; clamping at 0 and 1;
+* Heaviside: regul at its knots, and a level set whose smoothed step is
+  0 or 1 at every grid angle, whose coefficients are Dirichlet sums;
+* tension: a constant second field gives a tensor linear in the first;
+* velocity maximum: constant fields and single modes.
+
+The pocketfft engine is also checked against the long-double direct sum.
+"""
+import numpy as np
+import pytest
+
+from oracle import fft_par as F
+
+
+def times_alpha_beta_cos(C, alpha, beta):
+    """Coefficients (np, 2, M) of (alpha + beta cos t) * q, q with coefficients C
+    (cos column 0, sin column 1), by cos kt cos t = (cos(k+1)t + cos(k-1)t)/2 and
+    sin kt cos t = (sin(k+1)t + sin(k-1)t)/2; C must vanish in mode M-1."""
+    npt, _, M = C.shape
+    assert np.all(C[:, :, M - 1] == 0)
+    out = alpha[:, None, None] * C
+    b = beta[:, None]
+    for k in range(M - 1):
+        a_k, s_k = C[:, 0, k], C[:, 1, k] if k > 0 else 0.0 * C[:, 0, k]
+        if k == 0:
+            out[:, 0, 1] += b[:, 0] * a_k
+            continue
+        out[:, 0, k + 1] += 0.5 * b[:, 0] * a_k
+        out[:, 1, k + 1] += 0.5 * b[:, 0] * s_k
+        if k == 1:
+            out[:, 0, 0] += 0.5 * b[:, 0] * a_k       # cos 0 = 1; sin 0 = 0
+        else:
+            out[:, 0, k - 1] += 0.5 * b[:, 0] * a_k
+            out[:, 1, k - 1] += 0.5 * b[:, 0] * s_k
+    return out
+
+
+def band_limited(rng, npt, M, ncol=2):
+    C = rng.standard_normal((npt, ncol, M))
+    C[:, :, M - 1] = 0.0
+    C[:, 1::2, 0] = 0.0
+    return C
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def test_product_rule_helper_against_sampled_values():
+    """The helper's closed form, checked by sampling both sides at angles."""
+    rng = np.random.default_rng(0)
+    M = 5
+    C = band_limited(rng, 3, M)
+    al, be = rng.uniform(2, 3, 3), rng.uniform(-1, 1, 3)
+    P = times_alpha_beta_cos(C, al, be)
+    t = np.linspace(0, 2 * np.pi, 17)
+    m = np.arange(M)[:, None]
+    ev = lambda X: np.einsum("pm,mt->pt", X[:, 0], np.cos(m * t)) + np.einsum("pm,mt->pt", X[:, 1], np.sin(m * t))
+    assert np.allclose(ev(P), (al[:, None] + be[:, None] * np.cos(t)) * ev(C), atol=1e-13)
+
+
+@pytest.mark.parametrize("M,mp", [(4, 6), (8, 12), (16, 24), (7, 11)])
+def test_div_of_a_product_is_the_factor(M, mp):
+    rng = np.random.default_rng(M)
+    npt = 40
+    q1 = band_limited(rng, npt, M)
+    al, be = rng.uniform(2.0, 3.0, npt), rng.uniform(-1.5, 1.5, npt)
+    q2 = np.zeros((npt, 2, M))
+    q2[:, 0, 0], q2[:, 0, 1] = al, be
+    c1 = times_alpha_beta_cos(q1, al, be)
+    out = F.fft_par_div_dcl(c1, q2, mp)
+    assert rel(out, q1) <= 1e-13
+
+
+def test_scalar_vect_pb2_divides_every_component():
+    rng = np.random.default_rng(3)
+    M, mp, npt = 8, 12, 30
+    W = band_limited(rng, npt, M, 6)
+    al, be = rng.uniform(2.0, 3.0, npt), rng.uniform(-1.5, 1.5, npt)
+    s = np.zeros((npt, 2, M))
+    s[:, 0, 0], s[:, 0, 1] = al, be
+    V = np.concatenate([times_alpha_beta_cos(W[:, 2 * c:2 * c + 2], al, be) for c in range(3)], axis=1)
+    out = F.fft_scalar_vect_dcl(V, s, 2, mp)
+    assert rel(out, W) <= 1e-13
+
+
+def test_squareroot_of_a_square_and_clamping():
+    M, mp, npt = 6, 9, 4
+    c = np.zeros((npt, 2, M))
+    # q = 0.5 + 0.3 cos t in [0.2, 0.8]; q^2 = 0.295 + 0.3 cos t + 0.045 cos 2t
+    c[:, 0, 0], c[:, 0, 1], c[:, 0, 2] = 0.25 + 0.045, 0.3, 0.045
+    out = F.fft_squareroot_scalar(c, mp)
+    ex = np.zeros_like(c)
+    ex[:, 0, 0], ex[:, 0, 1] = 0.5, 0.3
+    assert np.max(np.abs(out - ex)) <= 1e-14
+    hi = np.zeros((npt, 2, M)); hi[:, 0, 0], hi[:, 0, 1] = 2.0, 0.5
+    lo = np.zeros((npt, 2, M)); lo[:, 0, 0], lo[:, 1, 3] = -2.0, 0.5
+    one = np.zeros_like(hi); one[:, 0, 0] = 1.0
+    assert np.max(np.abs(F.fft_squareroot_scalar(hi, mp) - one)) <= 1e-15
+    assert np.max(np.abs(F.fft_squareroot_scalar(lo, mp))) <= 1e-15
+
+
+def test_regul_knots():
+    eps = 0.04
+    x = np.array([0.5 - 2 * eps, 0.5 - eps, 0.5, 0.5 + eps / 2, 0.5 + eps, 0.5 + 3 * eps])
+    r = F.regul(x, eps)
+    # x = eps/2: (1 + (eps/2)(eps^2/4 - 3 eps^2)/(-2 eps^3))/2 = (1 + 11/16)/2 = 27/32
+    assert np.allclose(r, [0.0, 0.0, 0.5, 27 / 32, 1.0, 1.0], atol=1e-15)
+
+
+def test_heaviside_of_constant_level_sets():
+    M, mp, npt, eps = 4, 6, 3, 0.05
+    V = np.zeros((2, npt, 2, M))
+    V[0, :, 0, 0] = 0.5               # H1 = 1/2
+    V[1, :, 0, 0] = 0.5 + eps / 2     # H2 = 27/32
+    vals = np.array([1.0, 3.0, 10.0])
+    out = F.fft_heaviside_dcl(V, vals, mp, eps)
+    h1, h2 = 0.5, 27 / 32
+    rho = (vals[0] * (1 - h1) + vals[1] * h1) * (1 - h2) + vals[2] * h2
+    ex = np.zeros((npt, 2, M)); ex[:, 0, 0] = rho
+    assert np.max(np.abs(out - ex)) <= 1e-14
+
+
+@pytest.mark.parametrize("mp", [12, 16])
+def test_heaviside_step_is_a_dirichlet_sum(mp):
+    """phi = 0.5 + cos t: with eps = 0.01 no grid angle has |cos t_j| < eps,
+    so H(t_j) = 1 exactly for |j| <= J and 0 elsewhere, and
+    a_0 = v1 + (v2-v1)(2J+1)/N, a_m = (2/N)(v2-v1) sin((2J+1) m pi/N)/sin(m pi/N), b_m = 0."""
+    N = 2 * mp - 1
+    M = 8
+    t = 2 * np.pi * np.arange(N) / N
+    assert np.min(np.abs(np.cos(t))) > 0.01
+    J = int(np.sum(np.cos(t) > 0) // 2)
+    V = np.zeros((1, 2, 2, M))
+    V[0, :, 0, 0], V[0, :, 0, 1] = 0.5, 1.0
+    v1, v2 = 2.0, 7.0
+    out = F.fft_heaviside_dcl(V, [v1, v2], mp, 0.01)
+    m = np.arange(1, M)
+    ex = np.zeros((2, 2, M))
+    ex[:, 0, 0] = v1 + (v2 - v1) * (2 * J + 1) / N
+    ex[:, 0, 1:] = 2.0 / N * (v2 - v1) * np.sin((2 * J + 1) * m * np.pi / N) / np.sin(m * np.pi / N)
+    assert np.max(np.abs(out - ex)) <= 1e-13
+
+
+def test_tension_with_a_constant_second_field():
+    rng = np.random.default_rng(5)
+    M, mp, npt = 8, 12, 20
+    U = band_limited(rng, npt, M, 6)
+    g = rng.standard_normal((npt, 3))
+    G = np.zeros((npt, 6, M))
+    G[:, 0::2, 0] = g
+    out = F.fft_tensor_dcl(U, G, mp, opt_tension=True)          # (3, np, 6, M)
+    nrm = np.sqrt(np.sum(g ** 2, axis=1))
+    ex = np.empty_like(out)
+    for a in range(3):
+        for b in range(3):
+            ex[a, :, 2 * b:2 * b + 2] = U[:, 2 * a:2 * a + 2] * (g[:, b] / (nrm + 1e-14))[:, None, None]
+            if a == b:
+                ex[a, :, 2 * b, 0] -= nrm
+    assert rel(out, ex) <= 1e-13
+
+
+def test_max_vel():
+    M, mp = 5, 8
+    V = np.zeros((4, 6, M))
+    V[:, 0::2, 0] = [[1, 2, 2], [0, 3, 4], [1, 1, 1], [0, 0, 0]]     # |v| = 3, 5, sqrt 3, 0
+    assert abs(F.fft_max_vel_dcl(V, mp) - 5.0) <= 1e-14
+    W = np.zeros((3, 2, M))
+    W[1, 0, 1] = -2.5                                                 # max |c| at t = 0
+    assert abs(F.fft_max_vel_dcl(W, mp) - 2.5) <= 1e-14
+
+
+@pytest.mark.parametrize("op", ["div", "sqrt", "heaviside", "tension", "sv2"])
+def test_fft_engine_matches_direct_long_double_sum(op):
+    rng = np.random.default_rng(7)
+    M, mp, npt = 6, 9, 12
+    c = rng.standard_normal((npt, 2, M)) * 0.1
+    pos = c.copy(); pos[:, 0, 0] = 3.0
+    V = rng.standard_normal((npt, 6, M))
+    if op == "div":
+        f = lambda e: F.fft_par_div_dcl(c, pos, mp, e)
+    elif op == "sqrt":
+        cc = c.copy(); cc[:, 0, 0] = 0.5
+        f = lambda e: F.fft_squareroot_scalar(cc, mp, e)
+    elif op == "heaviside":
+        L = rng.standard_normal((2, npt, 2, M)) * 0.05 + 0.5
+        f = lambda e: F.fft_heaviside_dcl(L, [1.0, 2.0, 5.0], mp, 0.1, e)
+    elif op == "tension":
+        f = lambda e: F.fft_tensor_dcl(V, V[:, ::-1].copy(), mp, e, opt_tension=True)
+    else:
+        f = lambda e: F.fft_scalar_vect_dcl(V, pos, 2, mp, e)
+    a, b = f("fft"), f("direct")
+    assert rel(a, b) <= 1e-13
