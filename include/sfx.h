@@ -70,23 +70,26 @@ int sfx_fft_par_prod_dcl(const double* c1_in, const double* c2_in, double* c_out
                          int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
                          double* temps);
 
-/* Replaces FFT_SCALAR_VECT_DCL (fft_parallel.F90:1021) for pb = 1:
- * V_out(np,6,m) = s * v with V1 the vector (np,6,m) and V2 the scalar (np,2,m).
- * pb = 2 (v / s, not a polynomial product) returns SFX_ERR_UNSUPPORTED. */
+/* Replaces FFT_SCALAR_VECT_DCL (fft_parallel.F90:1021): V1 the vector (np,6,m),
+ * V2 the scalar (np,2,m); pb = 1: V_out(np,6,m) = s * v; pb = 2: v / s,
+ * evaluated on the reference's own grid N = 2*m_max_pad-1 (the quotient is
+ * not band-limited, so its coefficients depend on N). */
 int sfx_fft_scalar_vect_dcl(const double* V1_in, const double* V2_in, double* V_out,
                             int64_t np, int32_t nb_field1, int32_t nb_field2, int32_t m_max_c,
                             int32_t pb, int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
                             double* temps);
 
-/* Replaces FFT_TENSOR_DCL without opt_tension (fft_parallel.F90:1543):
- * V_out(3,np,6,m), V_out(a,:,2b-1:2b,:) = coefficients of V1_a * V2_b.
- * opt_tension != 0 returns SFX_ERR_UNSUPPORTED. */
+/* Replaces FFT_TENSOR_DCL (fft_parallel.F90:1543): V_out(3,np,6,m),
+ * V_out(a,:,2b-1:2b,:) = coefficients of V1_a * V2_b, or with opt_tension != 0
+ * of V1_a V2_b/(|V2| + 1e-14) - delta_ab |V2| (:1688-1706), evaluated on the
+ * reference's own grid N = 2*m_max_pad-1. */
 int sfx_fft_tensor_dcl(const double* V1_in, const double* V2_in, double* V_out,
                        int64_t np, int32_t nb_field, int32_t m_max_c,
                        int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
                        int32_t opt_tension, double* temps);
 
-/* Device-pointer forms of the five products above (no copies, no sync). */
+/* Device-pointer forms of the five products above (no copies, no sync);
+ * scalar-vector pb = 2 and the tension tensor use the exact-grid scratch below. */
 int sfx_fft_par_cross_prod_dcl_dev(const double* V1_in, const double* V2_in, double* V_out,
                                    int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
 int sfx_fft_par_dot_prod_dcl_dev(const double* V1_in, const double* V2_in, double* c_out,
@@ -97,7 +100,50 @@ int sfx_fft_scalar_vect_dcl_dev(const double* V1_in, const double* V2_in, double
                                 int64_t np, int32_t m_max_c, int32_t pb, int32_t m_max_pad,
                                 void* stream);
 int sfx_fft_tensor_dcl_dev(const double* V1_in, const double* V2_in, double* V_out,
-                           int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
+                           int64_t np, int32_t m_max_c, int32_t m_max_pad, int32_t opt_tension,
+                           void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Non-polynomial products on the reference's own azimuthal grid
+ * N = 2*m_max_pad-1 (odd), as real DFTs: their coefficients depend on N.
+ * m_max_pad >= m_max_c is required.  The library keeps a grow-only scratch
+ * buffer (2 * outputs * m_max_pad * np doubles) per device; calls that
+ * overlap on one device race on it.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces FFT_PAR_DIV_DCL (fft_parallel.F90:667): c_out(np,2,m) = c1 / c2. */
+int sfx_fft_par_div_dcl(const double* c1_in, const double* c2_in, double* c_out, int64_t np,
+                        int32_t m_max_c, int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
+                        double* temps);
+int sfx_fft_par_div_dcl_dev(const double* c1_in, const double* c2_in, double* c_out, int64_t np,
+                            int32_t m_max_c, int32_t m_max_pad, void* stream);
+
+/* Replaces FFT_HEAVISIDE_DCL (fft_parallel.F90:832): V1_in(n_values-1, np, 2, m)
+ * level sets, values(n_values) (host array), coeff_tanh the half-width of regul
+ * (:2234-2246); on the grid rho = values(1), then for each interface i
+ * rho = rho*(1 - regul(phi_i)) + values(i+1)*regul(phi_i); V_out(np,2,m).
+ * At most 4 interfaces. */
+int sfx_fft_heaviside_dcl(const double* V1_in, const double* values, double* V_out, int32_t n_values,
+                          int64_t np, int32_t nb_field, int32_t m_max_c, int32_t nb_procs,
+                          int64_t bloc_size, int32_t m_max_pad, double coeff_tanh, double* temps);
+int sfx_fft_heaviside_dcl_dev(const double* V1_in, const double* values, double* V_out, int32_t n_values,
+                              int64_t np, int32_t m_max_c, int32_t m_max_pad, double coeff_tanh,
+                              void* stream);
+
+/* Replaces FFT_SQUAREROOT_SCALAR (fft_parallel.F90:4601): c1_inout(np,nb_field,m),
+ * its first scalar (columns 1:2) overwritten by SQRT(MAX(0, MIN(1, c))). */
+int sfx_fft_squareroot_scalar(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                              int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps);
+int sfx_fft_squareroot_scalar_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                  int32_t m_max_pad, void* stream);
+
+/* Replaces FFT_MAX_VEL_DCL (fft_parallel.F90:1426): *V_out = max over points and
+ * grid angles of |v| (nb_field 6) or |c| (nb_field 2).  The _dev form writes
+ * the maximum to device memory. */
+int sfx_fft_max_vel_dcl(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                        int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad);
+int sfx_fft_max_vel_dcl_dev(const double* V1_in, double* V_out, int64_t np, int32_t nb_field,
+                            int32_t m_max_c, int32_t m_max_pad, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Meshes.  A P2 meridian mesh is registered once (the reference builds it at

@@ -45,7 +45,15 @@ SIGNATURES = {
     "sfx_fft_par_dot_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_par_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_scalar_vect_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
-    "sfx_fft_tensor_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_tensor_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
+    "sfx_fft_par_div_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_par_div_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_heaviside_dcl": [_vp, _vp, _vp, _i32, _i64, _i32, _i32, _i32, _i64, _i32, _d, _vp],
+    "sfx_fft_heaviside_dcl_dev": [_vp, _vp, _vp, _i32, _i64, _i32, _i32, _d, _vp],
+    "sfx_fft_squareroot_scalar": [_vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_squareroot_scalar_dev": [_vp, _i64, _i32, _i32, _i32, _vp],
+    "sfx_fft_max_vel_dcl": [_vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32],
+    "sfx_fft_max_vel_dcl_dev": [_vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_mesh_create": [_vp, _vp, _i32, _i32, _i64, _i64, _vp],
     "sfx_mesh_destroy": [_vp],
     "sfx_mesh_info": [_vp, _vp, _vp],

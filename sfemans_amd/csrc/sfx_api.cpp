@@ -12,6 +12,7 @@
 
 #include "../../include/sfx.h"
 #include "sfx_api_internal.h"
+#include "sfx_exact_body.h"
 #include "sfx_kernels.h"
 #include "sfx_twiddle.h"
 
@@ -219,9 +220,11 @@ int sfx_fft_scalar_vect_dcl(const double* V1_in, const double* V2_in, double* V_
   if (nb_field1 != 6 || nb_field2 != 2)
     return fail(SFX_ERR_ARG, "error in dimension of INPUT variables while calling FFT_SCALAR_VECT_DCL ");
   if (pb != 1 && pb != 2) return fail(SFX_ERR_ARG, "error in problem type while calling FFT_SCALAR_VECT_DCL ");
-  if (pb == 2) return fail(SFX_ERR_UNSUPPORTED, "%s: pb=2 (v/s) is not a polynomial product", nm);
   int rc;
-  if ((rc = check_modes(nm, np, m_max_c, m_max_pad)) || (rc = check_procs(nm, np, nb_procs, bloc_size))) return rc;
+  if ((rc = check_procs(nm, np, nb_procs, bloc_size))) return rc;
+  // v / s is not a polynomial product: it runs on the reference's own grid
+  if (pb == 2) return exact_binary_host(EX_SV2, V1_in, V2_in, V_out, np, m_max_c, m_max_pad, temps);
+  if ((rc = check_modes(nm, np, m_max_c, m_max_pad))) return rc;
   return run_fftprod_host(OP_SVECT1, nm, V1_in, V2_in, V_out, np, 6, 2, 6, m_max_c, temps);
 }
 
@@ -232,9 +235,11 @@ int sfx_fft_tensor_dcl(const double* V1_in, const double* V2_in, double* V_out, 
   const char* nm = "FFT_TENSOR_DCL";
   if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_TENSOR_DCL ");
   if (nb_field != 6) return fail(SFX_ERR_ARG, "%s: the tensor product needs nb_field == 6", nm);
-  if (opt_tension) return fail(SFX_ERR_UNSUPPORTED, "%s: opt_tension is not a polynomial product", nm);
   int rc;
-  if ((rc = check_modes(nm, np, m_max_c, m_max_pad)) || (rc = check_procs(nm, np, nb_procs, bloc_size))) return rc;
+  if ((rc = check_procs(nm, np, nb_procs, bloc_size))) return rc;
+  // the tension tensor divides by |V2|: it runs on the reference's own grid
+  if (opt_tension) return exact_binary_host(EX_TENSION, V1_in, V2_in, V_out, np, m_max_c, m_max_pad, temps);
+  if ((rc = check_modes(nm, np, m_max_c, m_max_pad))) return rc;
   return run_fftprod_host(OP_TENSOR, nm, V1_in, V2_in, V_out, np, 6, 6, 18, m_max_c, temps);
 }
 
@@ -265,13 +270,17 @@ int sfx_fft_par_prod_dcl_dev(const double* c1_in, const double* c2_in, double* c
 }
 int sfx_fft_scalar_vect_dcl_dev(const double* V1_in, const double* V2_in, double* V_out, int64_t np,
                                 int32_t m_max_c, int32_t pb, int32_t m_max_pad, void* stream) {
+  clear_err();
+  if (pb == 2) return exact_sv2_dev(V1_in, V2_in, V_out, np, m_max_c, m_max_pad, (hipStream_t)stream);
+  if (pb != 1) return fail(SFX_ERR_ARG, "error in problem type while calling FFT_SCALAR_VECT_DCL ");
   DEV_CHECK("FFT_SCALAR_VECT_DCL");
-  if (pb != 1) return fail(pb == 2 ? SFX_ERR_UNSUPPORTED : SFX_ERR_ARG, "FFT_SCALAR_VECT_DCL: pb=%d", pb);
   return run_fftprod_dev(OP_SVECT1, "FFT_SCALAR_VECT_DCL", V1_in, V2_in, V_out, np, 6, 2, 6, m_max_c,
                          (hipStream_t)stream);
 }
 int sfx_fft_tensor_dcl_dev(const double* V1_in, const double* V2_in, double* V_out, int64_t np,
-                           int32_t m_max_c, int32_t m_max_pad, void* stream) {
+                           int32_t m_max_c, int32_t m_max_pad, int32_t opt_tension, void* stream) {
+  clear_err();
+  if (opt_tension) return exact_tension_dev(V1_in, V2_in, V_out, np, m_max_c, m_max_pad, (hipStream_t)stream);
   DEV_CHECK("FFT_TENSOR_DCL");
   return run_fftprod_dev(OP_TENSOR, "FFT_TENSOR_DCL", V1_in, V2_in, V_out, np, 6, 6, 18, m_max_c,
                          (hipStream_t)stream);

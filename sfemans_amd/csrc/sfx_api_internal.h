@@ -30,5 +30,10 @@ int staging(int slot, size_t bytes, void** out);
 double now_s();
 int check_modes(const char* name, int64_t np, int32_t m_max_c, int32_t m_max_pad);
 int check_procs(const char* name, int64_t np, int32_t nb_procs, int64_t bloc_size);
+// Exact-grid products behind FFT_SCALAR_VECT_DCL (pb = 2) and FFT_TENSOR_DCL (opt_tension).
+int exact_sv2_dev(const double* V1, const double* V2, double* out, int64_t np, int M, int mp, hipStream_t st);
+int exact_tension_dev(const double* V1, const double* V2, double* out, int64_t np, int M, int mp, hipStream_t st);
+int exact_binary_host(int op, const double* V1, const double* V2, double* out, int64_t np, int M, int mp,
+                      double* temps);
 
 }  // namespace sfx

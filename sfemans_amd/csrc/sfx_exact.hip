@@ -1,0 +1,105 @@
+// Kernels of the exact-grid products (bodies in sfx_exact_body.h).
+#include "sfx_engine.h"
+#include "sfx_exact_body.h"
+#include "sfx_kernels.h"
+
+namespace sfx {
+
+constexpr int EXACT_THREADS = 256;
+
+// Logical block L = xcd_block(...) -> (point block L / nch, angle block L % nch):
+// the angle blocks of one point block run on one XCD, so the inputs they all
+// read come from that XCD's L2 after the first.
+template <int OP, int NI>
+__global__ void __launch_bounds__(EXACT_THREADS) exact_grid_kernel(ExactArgs a, int nch) {
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t p = (L / nch) * EXACT_THREADS + threadIdx.x;
+  double vmax = 0.0;
+  exact_grid_body<OP, NI>(a, p, (int)(L % nch), vmax);
+  if constexpr (OP == EX_MAXVEL) {
+    __shared__ double red[EXACT_THREADS];
+    red[threadIdx.x] = vmax;
+    __syncthreads();
+    for (int s = EXACT_THREADS / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.pmax[blockIdx.x] = red[0];
+  }
+}
+
+// Logical block -> (point block, output field and mode block).
+__global__ void __launch_bounds__(EXACT_THREADS) exact_r2c_kernel(ExactArgs a, int nmc) {
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t per = (int64_t)a.nout * nmc;
+  const int r = (int)(L % per);
+  exact_r2c_body(a, (L / per) * EXACT_THREADS + threadIdx.x, r / nmc, r % nmc);
+}
+
+// Maximum of n values into out[0] (one work-group).
+__global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* v, int64_t n, double* out) {
+  __shared__ double red[EXACT_THREADS];
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += EXACT_THREADS) m = fmax(m, v[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = EXACT_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Blocks of the angle pass (also the size of the EX_MAXVEL partial array).
+int64_t exact_grid_blocks(int op, int nin, int64_t np, int K) {
+  const int JC = exact_jc(op, nin);
+  return ((np + EXACT_THREADS - 1) / EXACT_THREADS) * ((K + 1 + JC - 1) / JC);
+}
+
+template <int OP, int NI>
+static hipError_t grid_launch(const ExactArgs& a, hipStream_t st, int64_t* nblk_out) {
+  constexpr int JC = exact_jc(OP, NI);
+  const int nch = (a.K + 1 + JC - 1) / JC;
+  const int64_t nblk = exact_grid_blocks(OP, NI, a.np, a.K);
+  *nblk_out = nblk;
+  if (nblk == 0) return hipSuccess;
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((exact_grid_kernel<OP, NI>), dim3((unsigned)nblk), dim3(EXACT_THREADS), 0, st, a, nch);
+  return hipGetLastError();
+}
+
+hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t st) {
+  hipError_t e = hipErrorInvalidValue;
+  int64_t nblk = 0;
+  switch (op) {
+    case EX_DIV: e = grid_launch<EX_DIV, 2>(a, st, &nblk); break;
+    case EX_SV2: e = grid_launch<EX_SV2, 4>(a, st, &nblk); break;
+    case EX_SQRT: e = grid_launch<EX_SQRT, 1>(a, st, &nblk); break;
+    case EX_TENSION: e = grid_launch<EX_TENSION, 6>(a, st, &nblk); break;
+    case EX_HEAVI:
+      switch (a.nin) {
+        case 1: e = grid_launch<EX_HEAVI, 1>(a, st, &nblk); break;
+        case 2: e = grid_launch<EX_HEAVI, 2>(a, st, &nblk); break;
+        case 3: e = grid_launch<EX_HEAVI, 3>(a, st, &nblk); break;
+        case 4: e = grid_launch<EX_HEAVI, 4>(a, st, &nblk); break;
+      }
+      break;
+    case EX_MAXVEL:
+      if (a.nin == 3) e = grid_launch<EX_MAXVEL, 3>(a, st, &nblk);
+      else if (a.nin == 1) e = grid_launch<EX_MAXVEL, 1>(a, st, &nblk);
+      break;
+  }
+  if (e != hipSuccess) return e;
+  if (op == EX_MAXVEL) {
+    hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nblk, max_out);
+    return hipGetLastError();
+  }
+  const int nmc = (a.M + EXACT_MC - 1) / EXACT_MC;
+  const int64_t nb2 = ((a.np + EXACT_THREADS - 1) / EXACT_THREADS) * (int64_t)a.nout * nmc;
+  if (nb2 == 0) return hipSuccess;
+  if (nb2 > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(exact_r2c_kernel, dim3((unsigned)nb2), dim3(EXACT_THREADS), 0, st, a, nmc);
+  return hipGetLastError();
+}
+
+}  // namespace sfx

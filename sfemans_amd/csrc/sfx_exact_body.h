@@ -1,0 +1,209 @@
+// Non-polynomial FFT_PAR_* products on the reference's own azimuthal grid.
+//
+// The reference evaluates every product on N = 2*m_max_pad - 1 angles
+// (SOURCES/POST_PROCESSING/fft_parallel.F90:185, :254-288).  For bilinear
+// products any alias-free grid gives the same coefficients, but a division,
+// a square root, a smoothed Heaviside or a norm depends on the grid, so these
+// run here on exactly that odd grid, as real DFTs written as sums:
+//
+//   c2r:  u(t_j) = E_j + O_j,  u(t_{N-j}) = E_j - O_j,  j = 0..K, K = m_max_pad-1,
+//         E_j = sum_m a_m cos(m t_j),  O_j = sum_{m>=1} b_m sin(m t_j);
+//   r2c:  with S_j = w(t_j) + w(t_{N-j}), D_j = w(t_j) - w(t_{N-j}) (S_0 = w(t_0), D_0 = 0),
+//         a_0 = (1/N) sum_j S_j,  a_m = (2/N) sum_j S_j cos(m t_j),
+//         b_m = (2/N) sum_j D_j sin(m t_j)      (the reference's 1/N scaling and
+//         2*CONJG of modes m >= 1, :291-299; mode-0 sine input dropped, :241).
+//
+// exact_grid_body: one thread per (point, block of JC angles) accumulates E
+// and O of every input field over the modes (the cos/sin tables are uniform
+// across the wave), applies the pointwise operation at t_j and t_{N-j} and
+// writes S and D of every output field.  exact_r2c_body: one thread per
+// (point, output field, block of MC modes) sums S and D against the r2c
+// tables.  Points are the fastest index everywhere (coalesced loads).
+#pragma once
+#include "sfx_portable.h"
+
+namespace sfx {
+
+constexpr int EXACT_MAX_IN = 6, EXACT_MAX_OUT = 9, EXACT_MAX_INT = 4;
+constexpr int EXACT_NJ_ALIGN = 32;   // c2r table rows padded to a multiple of every JC
+constexpr int EXACT_MC = 16;         // modes per r2c thread
+
+enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5 };
+
+// Angles per thread of the angle pass: 2*NI*JC accumulators stay at or below 64.
+constexpr int exact_jc(int op, int nin) {
+  return op == EX_TENSION ? 4 : op == EX_SQRT ? 32 : nin <= 2 ? 16 : 8;
+}
+// Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
+constexpr int exact_nout(int op) {
+  return op == EX_SV2 ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0 : 1;
+}
+
+// Fourier field: cos coefficient of mode m at point p at base[p*sp + m*sm],
+// sin coefficient at base[p*sp + off + m*sm].
+struct ExactField {
+  const double* base;
+  int64_t sp, sm, off;
+};
+struct ExactOut {
+  double* base;
+  int64_t sp, sm, off;
+};
+
+struct ExactArgs {
+  int64_t np;
+  int M;          // modes kept (m_max)
+  int K;          // m_max_pad - 1; N = 2K + 1
+  int nj;         // c2r table row length: K + 1 rounded up to EXACT_NJ_ALIGN
+  int MR;         // r2c table row length: M rounded up to EXACT_MC
+  const double* ctab;   // cos(2 pi m j/N) at [m*nj + j], sin at [(M + m)*nj + j], zero for j > K
+  const double* rtab;   // (c_m/N) cos(2 pi j m/N) at [j*MR + m], (2/N) sin at [(K+1 + j)*MR + m],
+                        // c_0 = 1, c_m = 2; zero for m >= M and for the sine of m = 0
+  int nin, nout;
+  ExactField in[EXACT_MAX_IN];
+  ExactOut out[EXACT_MAX_OUT];
+  double* W;            // S of output g, angle j, point p at W[(2g*(K+1) + j)*np + p]; D at +(K+1)*np
+  int64_t n_W;
+  double* pmax;         // EX_MAXVEL: one maximum per work-group
+  int n_int;            // EX_HEAVI: interfaces
+  double vals[EXACT_MAX_INT + 1];
+  double eps;           // EX_HEAVI: regul half-width (coeff_tanh)
+  int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
+};
+
+// regul of fft_parallel.F90:2234-2246.
+SFX_DEV double exact_regul(double phi, double eps) {
+  const double x = phi - 0.5;
+  if (x <= -eps) return 0.0;
+  if (x <= eps) return (1.0 + x * (x * x - 3.0 * eps * eps) / (-2.0 * eps * eps * eps)) / 2.0;
+  return 1.0;
+}
+
+// The pointwise operation at one angle: u (NI fields) -> w (NO fields).
+template <int OP, int NI, int NO>
+SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO > 0 ? NO : 1]) {
+  if constexpr (OP == EX_DIV) {
+    w[0] = u[0] / u[1];                                           // :782
+  } else if constexpr (OP == EX_SV2) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = u[k] / u[3];               // :1155-1157
+  } else if constexpr (OP == EX_SQRT) {
+    w[0] = sqrt(fmax(0.0, fmin(1.0, u[0])));                      // :4723
+  } else if constexpr (OP == EX_HEAVI) {
+    double rho = a.vals[0];                                       // :959-968
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const double h = exact_regul(u[i], a.eps);
+      rho = rho * (1.0 - h) + a.vals[i + 1] * h;
+    }
+    w[0] = rho;
+  } else if constexpr (OP == EX_TENSION) {
+    const double nrm = sqrt(u[3] * u[3] + u[4] * u[4] + u[5] * u[5]);   // :1690-1706
+    const double inv = nrm + 1.e-14;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) w[3 * r + c] = u[r] * u[3 + c] / inv - (r == c ? nrm : 0.0);
+  } else {
+    if constexpr (NI == 3)                                        // :1532-1536
+      w[0] = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    else
+      w[0] = sqrt(u[0] * u[0]);
+  }
+}
+
+// c2r of the NI input fields on angles j0..j0+JC-1 (and their mirrors), the
+// operation, S and D of the NO outputs (or the running maximum for EX_MAXVEL).
+template <int OP, int NI, int NO = exact_nout(OP), int JC = exact_jc(OP, NI)>
+SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax) {
+  if (p >= a.np) return;
+  const int j0 = jc * JC;
+  double E[NI][JC], O[NI][JC];
+#pragma unroll
+  for (int f = 0; f < NI; ++f)
+#pragma unroll
+    for (int q = 0; q < JC; ++q) E[f][q] = O[f][q] = 0.0;
+  for (int m = 0; m < a.M; ++m) {
+    const double* tc = a.ctab + (int64_t)m * a.nj + j0;
+    const double* ts = a.ctab + (int64_t)(a.M + m) * a.nj + j0;
+    double c[NI], s[NI];
+#pragma unroll
+    for (int f = 0; f < NI; ++f) {
+      const ExactField& F = a.in[f];
+      const int64_t ic = p * F.sp + m * F.sm;
+      SFX_CHECK("exact in", ic + (m ? F.off : 0), a.n_in[f]);
+      c[f] = F.base[ic];
+      s[f] = m ? F.base[ic + F.off] : 0.0;     // mode-0 sine dropped (:241)
+    }
+#pragma unroll
+    for (int q = 0; q < JC; ++q) {
+      const double cq = tc[q], sq = ts[q];
+#pragma unroll
+      for (int f = 0; f < NI; ++f) {
+        E[f][q] = fma(c[f], cq, E[f][q]);
+        O[f][q] = fma(s[f], sq, O[f][q]);
+      }
+    }
+  }
+  const int64_t plane = (int64_t)(a.K + 1) * a.np;
+#pragma unroll
+  for (int q = 0; q < JC; ++q) {
+    const int j = j0 + q;
+    if (j > a.K) break;
+    double up[NI], um[NI];
+#pragma unroll
+    for (int f = 0; f < NI; ++f) {
+      up[f] = E[f][q] + O[f][q];
+      um[f] = E[f][q] - O[f][q];
+    }
+    double wp[NO > 0 ? NO : 1], wm[NO > 0 ? NO : 1];
+    exact_op<OP, NI, NO>(a, up, wp);
+    if (j > 0) exact_op<OP, NI, NO>(a, um, wm);
+    if constexpr (OP == EX_MAXVEL) {
+      vmax = fmax(vmax, wp[0]);
+      if (j > 0) vmax = fmax(vmax, wm[0]);
+    } else {
+#pragma unroll
+      for (int g = 0; g < NO; ++g) {
+        const int64_t o = (2 * g * (int64_t)(a.K + 1) + j) * a.np + p;
+        SFX_CHECK("exact W", o + plane, a.n_W);
+        a.W[o] = j > 0 ? wp[g] + wm[g] : wp[g];
+        a.W[o + plane] = j > 0 ? wp[g] - wm[g] : 0.0;
+      }
+    }
+  }
+}
+
+// r2c of output g, modes mc*EXACT_MC .. +EXACT_MC-1, at point p.
+SFX_DEV void exact_r2c_body(const ExactArgs& a, int64_t p, int g, int mc) {
+  if (p >= a.np) return;
+  const int m0 = mc * EXACT_MC;
+  double A[EXACT_MC], B[EXACT_MC];
+#pragma unroll
+  for (int q = 0; q < EXACT_MC; ++q) A[q] = B[q] = 0.0;
+  const int64_t plane = (int64_t)(a.K + 1) * a.np;
+  const double* S = a.W + 2 * g * plane + p;
+  for (int j = 0; j <= a.K; ++j) {
+    SFX_CHECK("exact W", 2 * g * plane + p + plane + (int64_t)j * a.np, a.n_W);
+    const double sj = S[(int64_t)j * a.np], dj = S[plane + (int64_t)j * a.np];
+    const double* tc = a.rtab + (int64_t)j * a.MR + m0;
+    const double* ts = a.rtab + (int64_t)(a.K + 1 + j) * a.MR + m0;
+#pragma unroll
+    for (int q = 0; q < EXACT_MC; ++q) {
+      A[q] = fma(sj, tc[q], A[q]);
+      B[q] = fma(dj, ts[q], B[q]);
+    }
+  }
+  const ExactOut& F = a.out[g];
+#pragma unroll
+  for (int q = 0; q < EXACT_MC; ++q) {
+    const int m = m0 + q;
+    if (m >= a.M) break;
+    const int64_t o = p * F.sp + m * F.sm;
+    SFX_CHECK("exact out", o + F.off, a.n_out[g]);
+    F.base[o] = A[q];
+    F.base[o + F.off] = B[q];
+  }
+}
+
+}  // namespace sfx

@@ -1,0 +1,330 @@
+// C-ABI of the exact-grid products: DFT tables, scratch, argument checks.
+#include <math.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sfx.h"
+#include "sfx_api_internal.h"
+#include "sfx_exact_body.h"
+#include "sfx_exact_setup.h"
+
+namespace sfx {
+
+// ---- c2r and r2c tables per (device, M, K), built on the host in long double
+static std::mutex g_exact_mu;
+static std::map<std::tuple<int, int, int>, double*> g_exact_tabs;
+
+static int exact_tables(int M, int K, const double** ctab, const double** rtab) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  const size_t nc = exact_ctab_size(M, K), nr = exact_rtab_size(M, K);
+  std::lock_guard<std::mutex> lk(g_exact_mu);
+  auto key = std::make_tuple(dev, M, K);
+  auto it = g_exact_tabs.find(key);
+  if (it == g_exact_tabs.end()) {
+    std::vector<double> h;
+    exact_host_tables(M, K, h);
+    double* d = nullptr;
+    e = hipMalloc(&d, (nc + nr) * sizeof(double));
+    if (e != hipSuccess) return fail(SFX_ERR_NOMEM, "hipMalloc(exact tables): %s", hipGetErrorString(e));
+    e = hipMemcpy(d, h.data(), (nc + nr) * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(exact tables)");
+    it = g_exact_tabs.emplace(key, d).first;
+  }
+  *ctab = it->second;
+  *rtab = it->second + nc;
+  return SFX_OK;
+}
+
+// Staging slots of the exact-grid products (the FFT products use 0-2).
+enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX_MAX = 7 };
+
+// Common set-up: sizes, tables, scratch.  `a.in`, `a.out`, nin, nout filled by the caller.
+static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad) {
+  if (M <= 0) return fail(SFX_ERR_ARG, "BUG in %s: m_max_c==0", nm);
+  if (np < 0) return fail(SFX_ERR_ARG, "%s: np < 0", nm);
+  if (M > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, M);
+  if (m_max_pad < M) return fail(SFX_ERR_ARG, "%s: m_max_pad=%d < m_max=%d", nm, m_max_pad, M);
+  if (m_max_pad > 4096) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_pad=%d > 4096", nm, m_max_pad);
+  exact_setup_sizes(a, np, M, m_max_pad);
+  int rc = exact_tables(M, a.K, &a.ctab, &a.rtab);
+  if (rc) return rc;
+  void* w = nullptr;
+  if (op == EX_MAXVEL) {
+    const int64_t nb = exact_grid_blocks(op, a.nin, np, a.K);
+    if ((rc = staging(SLOT_EX_W, (size_t)(nb > 0 ? nb : 1) * 8, &w))) return rc;
+    a.pmax = (double*)w;
+  } else {
+    a.n_W = 2 * (int64_t)a.nout * (a.K + 1) * np;
+    if ((rc = staging(SLOT_EX_W, (size_t)(a.n_W > 0 ? a.n_W : 1) * 8, &w))) return rc;
+    a.W = (double*)w;
+  }
+  return SFX_OK;
+}
+
+static int exact_run(const char* nm, int op, const ExactArgs& a, double* max_out, hipStream_t st) {
+  if (a.np == 0 && op != EX_MAXVEL) return SFX_OK;
+  hipError_t e = launch_exact(op, a, max_out, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
+// ---- argument sets of each product on device arrays
+
+// c1, c2, c_out: (np, 2, M)
+static int div_dev(const double* c1, const double* c2, double* c, int64_t np, int M, int mp, hipStream_t st) {
+  ExactArgs a;
+  exact_setup_div(a, c1, c2, c, np, M);
+  int rc = exact_prepare("FFT_PAR_DIV_DCL", a, EX_DIV, np, M, mp);
+  return rc ? rc : exact_run("FFT_PAR_DIV_DCL", EX_DIV, a, nullptr, st);
+}
+
+// V (np, 6, M) / s (np, 2, M) -> (np, 6, M)
+static int sv2_dev(const double* V, const double* s, double* out, int64_t np, int M, int mp, hipStream_t st) {
+  ExactArgs a;
+  exact_setup_sv2(a, V, s, out, np, M);
+  int rc = exact_prepare("FFT_SCALAR_VECT_DCL", a, EX_SV2, np, M, mp);
+  return rc ? rc : exact_run("FFT_SCALAR_VECT_DCL", EX_SV2, a, nullptr, st);
+}
+
+// c (np, nb, M) -> first scalar of out (np, 2, M)
+static int sqrt_dev(const double* c, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
+  ExactArgs a;
+  exact_setup_sqrt(a, c, out, np, nb, M);
+  int rc = exact_prepare("FFT_SQUAREROOT_SCALAR", a, EX_SQRT, np, M, mp);
+  return rc ? rc : exact_run("FFT_SQUAREROOT_SCALAR", EX_SQRT, a, nullptr, st);
+}
+
+// V1 (n_int, np, 2, M), values (n_int + 1) -> (np, 2, M)
+static int heavi_dev(const double* V1, const double* values, int n_values, double* out, int64_t np, int M,
+                     int mp, double eps, hipStream_t st) {
+  const int n_int = n_values - 1;
+  if (n_int < 1) return fail(SFX_ERR_ARG, "FFT_HEAVISIDE_DCL: SIZE(values) < 2");
+  if (n_int > EXACT_MAX_INT)
+    return fail(SFX_ERR_UNSUPPORTED, "FFT_HEAVISIDE_DCL: %d interfaces > %d", n_int, EXACT_MAX_INT);
+  ExactArgs a;
+  exact_setup_heavi(a, V1, values, n_values, out, np, M, eps);
+  int rc = exact_prepare("FFT_HEAVISIDE_DCL", a, EX_HEAVI, np, M, mp);
+  return rc ? rc : exact_run("FFT_HEAVISIDE_DCL", EX_HEAVI, a, nullptr, st);
+}
+
+// V1, V2 (np, 6, M) -> V_out (3, np, 6, M)
+static int tension_dev(const double* V1, const double* V2, double* out, int64_t np, int M, int mp, hipStream_t st) {
+  ExactArgs a;
+  exact_setup_tension(a, V1, V2, out, np, M);
+  int rc = exact_prepare("FFT_TENSOR_DCL", a, EX_TENSION, np, M, mp);
+  return rc ? rc : exact_run("FFT_TENSOR_DCL", EX_TENSION, a, nullptr, st);
+}
+
+// V1 (np, nb, M), nb = 6 (|v|) or 2 (|c|) -> max_out[0] (device)
+static int maxvel_dev(const double* V1, double* max_out, int64_t np, int nb, int M, int mp, hipStream_t st) {
+  ExactArgs a;
+  exact_setup_maxvel(a, V1, np, nb, M);
+  int rc = exact_prepare("FFT_MAX_VEL_DCL", a, EX_MAXVEL, np, M, mp);
+  return rc ? rc : exact_run("FFT_MAX_VEL_DCL", EX_MAXVEL, a, max_out, st);
+}
+
+// ---- host-pointer paths: stage, run, copy back, synchronise
+struct HostRun {
+  double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+};
+
+static int to_dev(int slot, const double* h, size_t n, void** d) {
+  int rc = staging(slot, n * 8, d);
+  if (rc) return rc;
+  hipError_t e = n ? hipMemcpy(*d, h, n * 8, hipMemcpyHostToDevice) : hipSuccess;
+  return e == hipSuccess ? SFX_OK : hip_fail(e, "H2D");
+}
+
+static int from_dev(double* h, const void* d, size_t n) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "exact-grid product");
+  e = n ? hipMemcpy(h, d, n * 8, hipMemcpyDeviceToHost) : hipSuccess;
+  return e == hipSuccess ? SFX_OK : hip_fail(e, "D2H");
+}
+
+static void set_temps(double* temps, const HostRun& t) {
+  if (temps) {
+    temps[0] = (t.t1 - t.t0) + (t.t3 - t.t2);
+    temps[1] = t.t2 - t.t1;
+    temps[2] = 0.0;
+  }
+}
+
+}  // namespace sfx
+
+using namespace sfx;
+
+extern "C" {
+
+int sfx_fft_par_div_dcl_dev(const double* c1_in, const double* c2_in, double* c_out, int64_t np, int32_t m_max_c,
+                            int32_t m_max_pad, void* stream) {
+  clear_err();
+  return div_dev(c1_in, c2_in, c_out, np, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_par_div_dcl(const double* c1_in, const double* c2_in, double* c_out, int64_t np, int32_t m_max_c,
+                        int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  int rc = check_procs("FFT_PAR_DIV_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_PAR_DIV_DCL ");
+  const size_t n = (size_t)np * 2 * (m_max_c > 0 ? m_max_c : 0);
+  void *d1, *d2, *d3;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, c1_in, n, &d1)) || (rc = to_dev(SLOT_EX_IN2, c2_in, n, &d2))) return rc;
+  if ((rc = staging(SLOT_EX_OUT, n * 8, &d3))) return rc;
+  t.t1 = now_s();
+  if ((rc = div_dev((const double*)d1, (const double*)d2, (double*)d3, np, m_max_c, m_max_pad, nullptr))) return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(c_out, d3, n))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_heaviside_dcl_dev(const double* V1_in, const double* values, double* V_out, int32_t n_values,
+                              int64_t np, int32_t m_max_c, int32_t m_max_pad, double coeff_tanh, void* stream) {
+  clear_err();
+  if (!values) return fail(SFX_ERR_ARG, "FFT_HEAVISIDE_DCL: null values");
+  return heavi_dev(V1_in, values, n_values, V_out, np, m_max_c, m_max_pad, coeff_tanh, (hipStream_t)stream);
+}
+
+int sfx_fft_heaviside_dcl(const double* V1_in, const double* values, double* V_out, int32_t n_values, int64_t np,
+                          int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                          int32_t m_max_pad, double coeff_tanh, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_HEAVISIDE_DCL ");
+  if (nb_field != 2) return fail(SFX_ERR_ARG, "FFT_HEAVISIDE_DCL: the level sets are scalars (nb_field == 2)");
+  if (!values || n_values < 2) return fail(SFX_ERR_ARG, "FFT_HEAVISIDE_DCL: SIZE(values) < 2");
+  int rc = check_procs("FFT_HEAVISIDE_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  const size_t nin = (size_t)(n_values - 1) * np * 2 * (m_max_c > 0 ? m_max_c : 0);
+  const size_t nout = (size_t)np * 2 * (m_max_c > 0 ? m_max_c : 0);
+  void *d1, *d3;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, nin, &d1)) || (rc = staging(SLOT_EX_OUT, nout * 8, &d3))) return rc;
+  t.t1 = now_s();
+  if ((rc = heavi_dev((const double*)d1, values, n_values, (double*)d3, np, m_max_c, m_max_pad, coeff_tanh,
+                      nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(V_out, d3, nout))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_squareroot_scalar_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                  int32_t m_max_pad, void* stream) {
+  clear_err();
+  if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0)
+    return fail(SFX_ERR_ARG, " BUG in FFT_SQUAREROOT_SCALAR ");
+  // the result overwrites its own input: run on a copy of the first scalar
+  void* d = nullptr;
+  int rc = staging(SLOT_EX_IN1, (size_t)np * nb_field * (m_max_c > 0 ? m_max_c : 0) * 8, &d);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(d, c1_inout, (size_t)np * nb_field * (m_max_c > 0 ? m_max_c : 0) * 8,
+                                hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "FFT_SQUAREROOT_SCALAR copy");
+  return sqrt_dev((const double*)d, c1_inout, np, nb_field, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_squareroot_scalar(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c, int32_t nb_procs,
+                              int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0)
+    return fail(SFX_ERR_ARG, " BUG in FFT_SQUAREROOT_SCALAR ");
+  int rc = check_procs("FFT_SQUAREROOT_SCALAR", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  const size_t n = (size_t)np * nb_field * m_max_c;
+  void *d1, *d3;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, c1_inout, n, &d1)) || (rc = to_dev(SLOT_EX_OUT, c1_inout, n, &d3))) return rc;
+  t.t1 = now_s();
+  if ((rc = sqrt_dev((const double*)d1, (double*)d3, np, nb_field, m_max_c, m_max_pad, nullptr))) return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(c1_inout, d3, n))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_max_vel_dcl_dev(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                            int32_t m_max_pad, void* stream) {
+  clear_err();
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_MAX_VEL_DCL ");
+  if (nb_field != 6 && nb_field != 2)
+    return fail(SFX_ERR_UNSUPPORTED, "FFT_MAX_VEL_DCL: nb_field=%d (6 or 2)", nb_field);
+  return maxvel_dev(V1_in, V_out, np, nb_field, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_max_vel_dcl(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                        int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad) {
+  clear_err();
+  if (!V_out) return fail(SFX_ERR_ARG, "FFT_MAX_VEL_DCL: null V_out");
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_MAX_VEL_DCL ");
+  if (nb_field != 6 && nb_field != 2)
+    return fail(SFX_ERR_UNSUPPORTED, "FFT_MAX_VEL_DCL: nb_field=%d (6 or 2)", nb_field);
+  int rc = check_procs("FFT_MAX_VEL_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  void *d1, *dm;
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, (size_t)np * nb_field * m_max_c, &d1)) ||
+      (rc = staging(SLOT_EX_MAX, 8, &dm)))
+    return rc;
+  if ((rc = maxvel_dev((const double*)d1, (double*)dm, np, nb_field, m_max_c, m_max_pad, nullptr))) return rc;
+  return from_dev(V_out, dm, 1);
+}
+
+}  // extern "C"
+
+// The pb = 2 scalar-vector product and the tension tensor, reached from the
+// FFT_SCALAR_VECT_DCL / FFT_TENSOR_DCL entry points in sfx_api.cpp.
+namespace sfx {
+
+int exact_sv2_dev(const double* V1, const double* V2, double* out, int64_t np, int M, int mp, hipStream_t st) {
+  return sv2_dev(V1, V2, out, np, M, mp, st);
+}
+
+int exact_tension_dev(const double* V1, const double* V2, double* out, int64_t np, int M, int mp, hipStream_t st) {
+  return tension_dev(V1, V2, out, np, M, mp, st);
+}
+
+int exact_binary_host(int op, const double* V1, const double* V2, double* out, int64_t np, int M, int mp,
+                      double* temps) {
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  const size_t n1 = (size_t)np * 6 * M, n2 = (size_t)np * (op == EX_SV2 ? 2 : 6) * M;
+  const size_t no = (size_t)np * (op == EX_SV2 ? 6 : 18) * M;
+  void *d1, *d2, *d3;
+  int rc;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, V2, n2, &d2)) ||
+      (rc = staging(SLOT_EX_OUT, no * 8, &d3)))
+    return rc;
+  t.t1 = now_s();
+  rc = op == EX_SV2 ? sv2_dev((const double*)d1, (const double*)d2, (double*)d3, np, M, mp, nullptr)
+                    : tension_dev((const double*)d1, (const double*)d2, (double*)d3, np, M, mp, nullptr);
+  if (rc) return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(out, d3, no))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+}  // namespace sfx

@@ -1,0 +1,134 @@
+// Host-side set-up of the exact-grid products, shared by the library and the
+// CPU emulator: DFT tables and the field descriptors of each product.
+#pragma once
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "sfx_exact_body.h"
+
+namespace sfx {
+
+inline int exact_nj(int K) { return (K + 1 + EXACT_NJ_ALIGN - 1) / EXACT_NJ_ALIGN * EXACT_NJ_ALIGN; }
+inline int exact_mr(int M) { return (M + EXACT_MC - 1) / EXACT_MC * EXACT_MC; }
+inline size_t exact_ctab_size(int M, int K) { return (size_t)2 * M * exact_nj(K); }
+inline size_t exact_rtab_size(int M, int K) { return (size_t)2 * (K + 1) * exact_mr(M); }
+
+// ctab then rtab (layouts in ExactArgs), angles 2 pi (m j mod N)/N in long double.
+inline void exact_host_tables(int M, int K, std::vector<double>& h) {
+  const int N = 2 * K + 1, nj = exact_nj(K), MR = exact_mr(M);
+  const size_t nc = exact_ctab_size(M, K);
+  h.assign(nc + exact_rtab_size(M, K), 0.0);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < M; ++m)
+    for (int j = 0; j <= K; ++j) {
+      const long double t = two_pi * (long double)(((int64_t)m * j) % N) / (long double)N;
+      const long double c = cosl(t), s = sinl(t);
+      h[(size_t)m * nj + j] = (double)c;
+      h[(size_t)(M + m) * nj + j] = (double)s;
+      h[nc + (size_t)j * MR + m] = (double)((m == 0 ? 1.0L : 2.0L) / (long double)N * c);
+      h[nc + (size_t)(K + 1 + j) * MR + m] = m == 0 ? 0.0 : (double)(2.0L / (long double)N * s);
+    }
+}
+
+inline void exact_setup_sizes(ExactArgs& a, int64_t np, int M, int m_max_pad) {
+  a.np = np; a.M = M; a.K = m_max_pad - 1;
+  a.nj = exact_nj(a.K);
+  a.MR = exact_mr(M);
+}
+
+inline ExactField exact_field(const double* base, int64_t sp, int64_t sm, int64_t off) {
+  ExactField f;
+  f.base = base; f.sp = sp; f.sm = sm; f.off = off;
+  return f;
+}
+inline ExactOut exact_outf(double* base, int64_t sp, int64_t sm, int64_t off) {
+  ExactOut f;
+  f.base = base; f.sp = sp; f.sm = sm; f.off = off;
+  return f;
+}
+
+// Field k of an (np, nb, M) array: cos column 2k, sin column 2k+1.
+inline ExactField exact_col(const double* V, int64_t np, int nb, int k) {
+  return exact_field(V + 2 * k * np, 1, (int64_t)nb * np, np);
+}
+inline ExactOut exact_colo(double* V, int64_t np, int nb, int k) {
+  return exact_outf(V + 2 * k * np, 1, (int64_t)nb * np, np);
+}
+
+// FFT_PAR_DIV_DCL: c1, c2, c (np, 2, M).
+inline void exact_setup_div(ExactArgs& a, const double* c1, const double* c2, double* c, int64_t np, int M) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 2; a.nout = 1;
+  a.in[0] = exact_col(c1, np, 2, 0);
+  a.in[1] = exact_col(c2, np, 2, 0);
+  a.out[0] = exact_colo(c, np, 2, 0);
+  a.n_in[0] = a.n_in[1] = a.n_out[0] = np * 2 * (int64_t)M;
+}
+
+// FFT_SCALAR_VECT_DCL pb = 2: V (np, 6, M) / s (np, 2, M) -> out (np, 6, M).
+inline void exact_setup_sv2(ExactArgs& a, const double* V, const double* s, double* out, int64_t np, int M) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 4; a.nout = 3;
+  for (int k = 0; k < 3; ++k) {
+    a.in[k] = exact_col(V, np, 6, k);
+    a.out[k] = exact_colo(out, np, 6, k);
+    a.n_in[k] = a.n_out[k] = np * 6 * (int64_t)M - 2 * k * np;
+  }
+  a.in[3] = exact_col(s, np, 2, 0);
+  a.n_in[3] = np * 2 * (int64_t)M;
+}
+
+// FFT_SQUAREROOT_SCALAR: first scalar of c (np, nb, M) -> first scalar of out (np, nb, M).
+inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 1; a.nout = 1;
+  a.in[0] = exact_col(c, np, nb, 0);
+  a.out[0] = exact_colo(out, np, nb, 0);
+  a.n_in[0] = a.n_out[0] = np * nb * (int64_t)M;
+}
+
+// FFT_HEAVISIDE_DCL: V1 (n_int, np, 2, M), values (n_int + 1) -> out (np, 2, M).
+inline void exact_setup_heavi(ExactArgs& a, const double* V1, const double* values, int n_values, double* out,
+                              int64_t np, int M, double eps) {
+  memset(&a, 0, sizeof(a));
+  const int n_int = n_values - 1;
+  a.nin = n_int; a.nout = 1; a.n_int = n_int; a.eps = eps;
+  for (int i = 0; i < n_values; ++i) a.vals[i] = values[i];
+  for (int i = 0; i < n_int; ++i) {
+    a.in[i] = exact_field(V1 + i, n_int, (int64_t)n_int * np * 2, (int64_t)n_int * np);
+    a.n_in[i] = (int64_t)n_int * np * 2 * M - i;
+  }
+  a.out[0] = exact_colo(out, np, 2, 0);
+  a.n_out[0] = np * 2 * (int64_t)M;
+}
+
+// FFT_TENSOR_DCL with opt_tension: V1, V2 (np, 6, M) -> out (3, np, 6, M), out(r, :, 2c:2c+1, :).
+inline void exact_setup_tension(ExactArgs& a, const double* V1, const double* V2, double* out, int64_t np,
+                                int M) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 6; a.nout = 9;
+  for (int k = 0; k < 3; ++k) {
+    a.in[k] = exact_col(V1, np, 6, k);
+    a.in[3 + k] = exact_col(V2, np, 6, k);
+    a.n_in[k] = a.n_in[3 + k] = np * 6 * (int64_t)M - 2 * k * np;
+  }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      a.out[3 * r + c] = exact_outf(out + r + 3 * np * 2 * c, 3, 3 * np * 6, 3 * np);
+      a.n_out[3 * r + c] = 3 * np * 6 * (int64_t)M - r - 3 * np * 2 * c;
+    }
+}
+
+// FFT_MAX_VEL_DCL: V1 (np, nb, M), nb = 6 (|v|) or 2 (|c|).
+inline void exact_setup_maxvel(ExactArgs& a, const double* V1, int64_t np, int nb, int M) {
+  memset(&a, 0, sizeof(a));
+  a.nin = nb == 6 ? 3 : 1; a.nout = 0;
+  for (int k = 0; k < a.nin; ++k) {
+    a.in[k] = exact_col(V1, np, nb, k);
+    a.n_in[k] = np * nb * (int64_t)M - 2 * k * np;
+  }
+}
+
+}  // namespace sfx

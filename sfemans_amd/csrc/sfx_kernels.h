@@ -61,6 +61,10 @@ hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);
 hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
+// Exact-grid products (sfx_exact_body.h): angle pass, then r2c, or the maximum into max_out.
+struct ExactArgs;
+hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t st);
+int64_t exact_grid_blocks(int op, int nin, int64_t np, int K);
 #endif
 
 // Pass lists (GV_PAIRS) run as one single-pass GV_CURL_CROSS launch per

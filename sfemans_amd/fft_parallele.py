@@ -17,7 +17,8 @@ from ._lib import lib, check
 
 __all__ = [
     "FFT_PAR_CROSS_PROD_DCL", "FFT_PAR_DOT_PROD_DCL", "FFT_PAR_PROD_DCL",
-    "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "fortran_tensor", "fortran_empty",
+    "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "FFT_PAR_DIV_DCL", "FFT_HEAVISIDE_DCL",
+    "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
 
@@ -102,11 +103,16 @@ def _binary(name, communicator, V1_in, V2_in, V_out, nb_procs, bloc_size, m_max_
         dev = getattr(L, f"sfx_{name}_dev")
         if name == "fft_scalar_vect_dcl":
             check(dev(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, extra[1], m_max_pad, stream_ptr()))
+        elif name == "fft_tensor_dcl":
+            check(dev(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, m_max_pad, int(bool(extra[0])),
+                      stream_ptr()))
+        elif name == "fft_par_div_dcl":
+            check(dev(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, m_max_pad, stream_ptr()))
         else:
             check(dev(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, m_max_pad, stream_ptr()))
         return V_out
     fn = getattr(L, f"sfx_{name}")
-    if name == "fft_par_prod_dcl":
+    if name in ("fft_par_prod_dcl", "fft_par_div_dcl"):
         check(fn(ptr(V1_in), ptr(V2_in), ptr(V_out), np_, M, nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
     elif name == "fft_scalar_vect_dcl":
         nb2 = fdims(V2_in)[1]
@@ -144,13 +150,81 @@ def FFT_PAR_PROD_DCL(communicator, c1_in, c2_in, c_out, nb_procs=1, bloc_size=No
 
 def FFT_SCALAR_VECT_DCL(communicator, V1_in, V2_in, V_out, pb, nb_procs=1, bloc_size=None,
                         m_max_pad=None, temps=None):
-    """V_out(np,6,m) = s * v for pb = 1 (fft_parallel.F90:1021)."""
+    """V_out(np,6,m) = s * v (pb = 1) or v / s (pb = 2, on the grid of
+    2*m_max_pad-1 angles) (fft_parallel.F90:1021)."""
     return _binary("fft_scalar_vect_dcl", communicator, V1_in, V2_in, V_out, nb_procs,
                    bloc_size, m_max_pad, temps, extra=(None, pb))
 
 
 def FFT_TENSOR_DCL(communicator, V1_in, V2_in, V_out, nb_procs=1, bloc_size=None,
                    m_max_pad=None, temps=None, opt_tension=None):
-    """V_out(3,np,6,m) = V1 (x) V2 (fft_parallel.F90:1543)."""
+    """V_out(3,np,6,m) = V1 (x) V2, or with ``opt_tension`` the surface-tension
+    tensor V1_a V2_b/(|V2| + 1e-14) - delta_ab |V2| on the grid of
+    2*m_max_pad-1 angles (fft_parallel.F90:1543, :1688-1706)."""
     return _binary("fft_tensor_dcl", communicator, V1_in, V2_in, V_out, nb_procs,
                    bloc_size, m_max_pad, temps, extra=(opt_tension, None))
+
+
+def FFT_PAR_DIV_DCL(communicator, c1_in, c2_in, c_out, nb_procs=1, bloc_size=None,
+                    m_max_pad=None, temps=None):
+    """c_out(np,2,m) = c1 / c2 on the grid of 2*m_max_pad-1 angles (fft_parallel.F90:667)."""
+    return _binary("fft_par_div_dcl", communicator, c1_in, c2_in, c_out, nb_procs,
+                   bloc_size, m_max_pad, temps)
+
+
+def FFT_HEAVISIDE_DCL(communicator, V1_in, values, V_out, nb_procs=1, bloc_size=None,
+                      m_max_pad=None, coeff_tanh=None, temps=None):
+    """Density of level sets V1_in(n_int, np, 2, m) with ``values`` (n_int+1)
+    and regul of half-width ``coeff_tanh`` (fft_parallel.F90:832): V_out(np,2,m)."""
+    _single_rank(communicator)
+    n_int, np_, nb, M = fdims(V1_in)
+    fdims(V_out)
+    vals = np.ascontiguousarray(np.asarray(values, dtype=np.float64).ravel())
+    if vals.size != n_int + 1:
+        raise ValueError("SIZE(values) must be SIZE(V1_in,1) + 1")
+    if coeff_tanh is None:
+        raise TypeError("coeff_tanh is required")
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_heaviside_dcl_dev(ptr(V1_in), vals.ctypes.data, ptr(V_out), vals.size, np_, M,
+                                          m_max_pad, float(coeff_tanh), stream_ptr()))
+    else:
+        check(L.sfx_fft_heaviside_dcl(ptr(V1_in), vals.ctypes.data, ptr(V_out), vals.size, np_, nb, M,
+                                      nb_procs, bloc_size, m_max_pad, float(coeff_tanh), _temps_ptr(temps)))
+    return V_out
+
+
+def FFT_SQUAREROOT_SCALAR(communicator, c1_inout, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """c1_inout(np,nb,m): its first scalar becomes SQRT(MAX(0, MIN(1, c))) on the
+    grid of 2*m_max_pad-1 angles (fft_parallel.F90:4601), in place."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(c1_inout)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(c1_inout):
+        check(L.sfx_fft_squareroot_scalar_dev(ptr(c1_inout), np_, nb, M, m_max_pad, stream_ptr()))
+    else:
+        check(L.sfx_fft_squareroot_scalar(ptr(c1_inout), np_, nb, M, nb_procs, bloc_size, m_max_pad,
+                                          _temps_ptr(temps)))
+    return c1_inout
+
+
+def FFT_MAX_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=None, V_out=None):
+    """Maximum over points and the 2*m_max_pad-1 angles of |v| (nb_field 6) or |c|
+    (nb_field 2) (fft_parallel.F90:1426).  Host arrays: returns a float.  CUDA
+    tensors: writes the maximum into ``V_out`` (a 1-element float64 tensor,
+    allocated when omitted) on the current stream and returns it."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        import torch
+        if V_out is None:
+            V_out = torch.empty(1, dtype=torch.float64, device=V1_in.device)
+        check(L.sfx_fft_max_vel_dcl_dev(ptr(V1_in), ptr(V_out), np_, nb, M, m_max_pad, stream_ptr()))
+        return V_out
+    out = np.zeros(1)
+    check(L.sfx_fft_max_vel_dcl(ptr(V1_in), out.ctypes.data, np_, nb, M, nb_procs, bloc_size, m_max_pad))
+    return float(out[0])

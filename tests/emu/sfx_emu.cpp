@@ -387,3 +387,63 @@ extern "C" int emu_batch(int n, const int32_t* jj, const int64_t* jj_off, const 
            [&](int t, int64_t w, double2* s) { batch_body(b, t, (int)w, s); });
   return 0;
 }
+
+// ---- exact-grid products (non-polynomial FFT_PAR_*)
+#include "../../sfemans_amd/csrc/sfx_exact_setup.h"
+
+template <int OP, int NI>
+static void emu_exact_grid(const ExactArgs& a, double& vmax) {
+  constexpr int JC = exact_jc(OP, NI);
+  const int nch = (a.K + 1 + JC - 1) / JC;
+  for (int64_t p = 0; p < a.np; ++p)
+    for (int jc = 0; jc < nch; ++jc) exact_grid_body<OP, NI>(a, p, jc, vmax);
+}
+
+// op: 0 DIV (in1 c1, in2 c2), 1 SV2 (in1 V, in2 s), 2 SQRT (in1 c (np,nb,M), out a copy of c),
+// 3 HEAVISIDE (in1 V1 (n_int,np,2,M), values), 4 TENSION (in1, in2 (np,6,M), out (3,np,6,M)),
+// 5 MAX_VEL (in1 (np,nb,M), out[0] = maximum).
+extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, const double* in2,
+                         const double* values, int n_values, double eps, int nb, double* out) {
+  ExactArgs a;
+  switch (op) {
+    case EX_DIV: exact_setup_div(a, in1, in2, out, np, M); break;
+    case EX_SV2: exact_setup_sv2(a, in1, in2, out, np, M); break;
+    case EX_SQRT: exact_setup_sqrt(a, in1, out, np, nb, M); break;
+    case EX_HEAVI: exact_setup_heavi(a, in1, values, n_values, out, np, M, eps); break;
+    case EX_TENSION: exact_setup_tension(a, in1, in2, out, np, M); break;
+    case EX_MAXVEL: exact_setup_maxvel(a, in1, np, nb, M); break;
+    default: return 1;
+  }
+  exact_setup_sizes(a, np, M, mp);
+  std::vector<double> tab;
+  exact_host_tables(M, a.K, tab);
+  a.ctab = tab.data();
+  a.rtab = tab.data() + exact_ctab_size(M, a.K);
+  a.n_W = 2 * (int64_t)a.nout * (a.K + 1) * np;
+  std::vector<double> W((size_t)a.n_W + 1);
+  a.W = W.data();
+  double vmax = 0.0;
+  switch (op) {
+    case EX_DIV: emu_exact_grid<EX_DIV, 2>(a, vmax); break;
+    case EX_SV2: emu_exact_grid<EX_SV2, 4>(a, vmax); break;
+    case EX_SQRT: emu_exact_grid<EX_SQRT, 1>(a, vmax); break;
+    case EX_TENSION: emu_exact_grid<EX_TENSION, 6>(a, vmax); break;
+    case EX_HEAVI:
+      if (a.nin == 1) emu_exact_grid<EX_HEAVI, 1>(a, vmax);
+      else if (a.nin == 2) emu_exact_grid<EX_HEAVI, 2>(a, vmax);
+      else if (a.nin == 3) emu_exact_grid<EX_HEAVI, 3>(a, vmax);
+      else if (a.nin == 4) emu_exact_grid<EX_HEAVI, 4>(a, vmax);
+      else return 2;
+      break;
+    case EX_MAXVEL:
+      if (a.nin == 3) emu_exact_grid<EX_MAXVEL, 3>(a, vmax);
+      else emu_exact_grid<EX_MAXVEL, 1>(a, vmax);
+      out[0] = vmax;
+      return 0;
+  }
+  const int nmc = (M + EXACT_MC - 1) / EXACT_MC;
+  for (int64_t p = 0; p < np; ++p)
+    for (int g = 0; g < a.nout; ++g)
+      for (int mc = 0; mc < nmc; ++mc) exact_r2c_body(a, p, g, mc);
+  return 0;
+}

@@ -49,6 +49,8 @@ def _load():
     L.emu_adjoint.restype = C.c_int
     L.emu_batch.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]
     L.emu_batch.restype = C.c_int
+    L.emu_exact.argtypes = [i32, i64, i32, i32, vp, vp, vp, i32, C.c_double, i32, vp]
+    L.emu_exact.restype = C.c_int
     return L
 
 

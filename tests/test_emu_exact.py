@@ -1,0 +1,93 @@
+"""Exact-grid kernel bodies on the CPU emulator vs the oracle (CPU only).
+
+The non-polynomial FFT_PAR_* products run on N = 2*m_max_pad-1 angles as
+real DFT sums (sfx_exact_body.h); the oracle evaluates the same products
+with pocketfft on the same grid.  Relative max-norm error <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+from oracle import fft_par as F
+from tests import emu_runner
+
+EX_DIV, EX_SV2, EX_SQRT, EX_HEAVI, EX_TENSION, EX_MAXVEL = range(6)
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def emu(op, np_, M, mp, in1, in2=None, values=None, eps=0.0, nb=2, out=None):
+    nv = 0 if values is None else len(values)
+    vals = None if values is None else np.asarray(values, dtype=np.float64)
+    rc, arrs = emu_runner.call("emu_exact", op, np_, M, mp, in1, in2, vals, nv, eps, nb, out)
+    assert rc == 0
+    return arrs[-1]
+
+
+def fields(rng, npt, nb, M, scale=1.0):
+    V = rng.standard_normal((npt, nb, M)) * scale * np.exp(-np.arange(M) / 4.0)
+    return np.asfortranarray(V)
+
+
+CASES = [(5, 4, 6), (17, 8, 12), (9, 13, 20), (33, 16, 24), (3, 1, 1), (4, 32, 48), (2, 64, 96)]
+
+
+@pytest.mark.parametrize("npt,M,mp", CASES)
+def test_div(npt, M, mp):
+    rng = np.random.default_rng(M)
+    c1 = fields(rng, npt, 2, M)
+    c2 = fields(rng, npt, 2, M, 0.1)
+    c2[:, 0, 0] = 2.0 + np.abs(c2[:, 0, 0])
+    out = emu(EX_DIV, npt, M, mp, c1, c2, out=np.zeros((npt, 2, M), order="F"))
+    assert rel(out, F.fft_par_div_dcl(c1, c2, mp)) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp", CASES[:5])
+def test_scalar_vect_pb2(npt, M, mp):
+    rng = np.random.default_rng(10 + M)
+    V = fields(rng, npt, 6, M)
+    s = fields(rng, npt, 2, M, 0.1)
+    s[:, 0, 0] = 1.5
+    out = emu(EX_SV2, npt, M, mp, V, s, out=np.zeros((npt, 6, M), order="F"))
+    assert rel(out, F.fft_scalar_vect_dcl(V, s, 2, mp)) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp,nb", [(7, 5, 8, 2), (11, 16, 24, 4), (3, 64, 96, 2)])
+def test_squareroot(npt, M, mp, nb):
+    rng = np.random.default_rng(20 + M)
+    c = fields(rng, npt, nb, M, 0.4)
+    c[:, 0, 0] = 0.5           # values in and out of [0, 1]: the clamp is active
+    out = emu(EX_SQRT, npt, M, mp, c, nb=nb, out=np.asfortranarray(c.copy()))
+    exp = c.copy()
+    exp[:, 0:2] = F.fft_squareroot_scalar(c, mp)
+    assert rel(out, exp) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp,n_int", [(6, 4, 6, 1), (9, 8, 12, 2), (5, 16, 24, 3), (4, 32, 48, 4)])
+def test_heaviside(npt, M, mp, n_int):
+    rng = np.random.default_rng(30 + M)
+    V = rng.standard_normal((n_int, npt, 2, M)) * 0.3 * np.exp(-np.arange(M) / 3.0)
+    V[:, :, 0, 0] += 0.5
+    V = np.asfortranarray(V)
+    vals = list(np.linspace(1.0, 5.0, n_int + 1))
+    eps = 0.2
+    out = emu(EX_HEAVI, npt, M, mp, V, values=vals, eps=eps, out=np.zeros((npt, 2, M), order="F"))
+    assert rel(out, F.fft_heaviside_dcl(V, vals, mp, eps)) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp", CASES[:4])
+def test_tension(npt, M, mp):
+    rng = np.random.default_rng(40 + M)
+    U = fields(rng, npt, 6, M)
+    G = fields(rng, npt, 6, M)
+    out = emu(EX_TENSION, npt, M, mp, U, G, out=np.zeros((3, npt, 6, M), order="F"))
+    assert rel(out, F.fft_tensor_dcl(U, G, mp, opt_tension=True)) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp,nb", [(13, 8, 12, 6), (7, 16, 24, 2), (300, 5, 8, 6)])
+def test_max_vel(npt, M, mp, nb):
+    rng = np.random.default_rng(50 + M)
+    V = fields(rng, npt, nb, M)
+    out = emu(EX_MAXVEL, npt, M, mp, V, nb=nb, out=np.zeros(1))
+    assert abs(out[0] - F.fft_max_vel_dcl(V, mp)) <= 1e-13 * F.fft_max_vel_dcl(V, mp)

@@ -1,0 +1,176 @@
+"""Non-polynomial FFT_PAR_* products on the GPU through the C-ABI vs the oracle.
+
+These run on the reference's own grid of N = 2*m_max_pad-1 angles.
+Tolerance: relative max-norm error <= 1e-12 (FP64); the maximum to 1e-13.
+At 1,404,928 points (m_max = 64) the division is checked on sampled points
+against the oracle and on every point against a closed form (the quotient of
+a band-limited product by its factor).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import fft_par as F  # noqa: E402
+from sfemans_amd import fft_parallele as FP  # noqa: E402
+from sfemans_amd._lib import SfxError  # noqa: E402
+from tests.test_oracle_fft_exact import band_limited, times_alpha_beta_cos  # noqa: E402
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+
+
+def fields(rng, npt, nb, M, scale=1.0):
+    return np.asfortranarray(rng.standard_normal((npt, nb, M)) * scale * np.exp(-np.arange(M) / 4.0))
+
+
+SIZES = [(1000, 4), (3000, 8), (700, 13), (2000, 32), (1500, 64), (300, 128)]
+
+
+@pytest.mark.parametrize("npt,M", SIZES)
+def test_div_host_and_device(npt, M):
+    print(f"case div np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(M)
+    c1 = fields(rng, npt, 2, M)
+    c2 = fields(rng, npt, 2, M, 0.1)
+    c2[:, 0, 0] = 2.0
+    mp = 3 * M // 2
+    exp = F.fft_par_div_dcl(c1, c2, mp)
+    out = np.zeros((npt, 2, M), order="F")
+    FP.FFT_PAR_DIV_DCL(None, c1, c2, out)
+    assert rel(out, exp) <= TOL
+    d = FP.fortran_empty((npt, 2, M))
+    FP.FFT_PAR_DIV_DCL(None, FP.fortran_tensor(c1), FP.fortran_tensor(c2), d)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M", SIZES[:4])
+def test_scalar_vect_pb2(npt, M):
+    print(f"case scalar_vect pb=2 np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(10 + M)
+    V = fields(rng, npt, 6, M)
+    s = fields(rng, npt, 2, M, 0.1)
+    s[:, 0, 0] = 1.5
+    exp = F.fft_scalar_vect_dcl(V, s, 2, 3 * M // 2)
+    out = np.zeros((npt, 6, M), order="F")
+    FP.FFT_SCALAR_VECT_DCL(None, V, s, out, 2)
+    assert rel(out, exp) <= TOL
+    d = FP.fortran_empty((npt, 6, M))
+    FP.FFT_SCALAR_VECT_DCL(None, FP.fortran_tensor(V), FP.fortran_tensor(s), d, 2)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M,nb", [(900, 8, 2), (1200, 16, 4), (500, 64, 2)])
+def test_squareroot_in_place(npt, M, nb):
+    print(f"case squareroot np={npt} M={M} nb={nb}", flush=True)
+    rng = np.random.default_rng(20 + M)
+    c = fields(rng, npt, nb, M, 0.4)
+    c[:, 0, 0] = 0.5
+    exp = c.copy()
+    exp[:, 0:2] = F.fft_squareroot_scalar(c, 3 * M // 2)
+    h = np.asfortranarray(c.copy())
+    FP.FFT_SQUAREROOT_SCALAR(None, h)
+    assert rel(h, exp) <= TOL
+    d = FP.fortran_tensor(c)
+    FP.FFT_SQUAREROOT_SCALAR(None, d)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M,n_int", [(800, 4, 1), (1000, 8, 2), (600, 16, 3), (400, 32, 4), (300, 64, 2)])
+def test_heaviside(npt, M, n_int):
+    print(f"case heaviside np={npt} M={M} interfaces={n_int}", flush=True)
+    rng = np.random.default_rng(30 + M)
+    V = rng.standard_normal((n_int, npt, 2, M)) * 0.3 * np.exp(-np.arange(M) / 3.0)
+    V[:, :, 0, 0] += 0.5
+    V = np.asfortranarray(V)
+    vals = np.linspace(1.0, 5.0, n_int + 1)
+    exp = F.fft_heaviside_dcl(V, vals, 3 * M // 2, 0.2)
+    out = np.zeros((npt, 2, M), order="F")
+    FP.FFT_HEAVISIDE_DCL(None, V, vals, out, coeff_tanh=0.2)
+    assert rel(out, exp) <= TOL
+    d = FP.fortran_empty((npt, 2, M))
+    FP.FFT_HEAVISIDE_DCL(None, FP.fortran_tensor(V), vals, d, coeff_tanh=0.2)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M", SIZES[:5])
+def test_tension(npt, M):
+    print(f"case tension np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(40 + M)
+    U, G = fields(rng, npt, 6, M), fields(rng, npt, 6, M)
+    exp = F.fft_tensor_dcl(U, G, 3 * M // 2, opt_tension=True)
+    out = np.zeros((3, npt, 6, M), order="F")
+    FP.FFT_TENSOR_DCL(None, U, G, out, opt_tension=True)
+    assert rel(out, exp) <= TOL
+    d = FP.fortran_empty((3, npt, 6, M))
+    FP.FFT_TENSOR_DCL(None, FP.fortran_tensor(U), FP.fortran_tensor(G), d, opt_tension=True)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M,nb", [(5000, 8, 6), (3000, 16, 2), (100000, 4, 6), (1, 3, 6)])
+def test_max_vel(npt, M, nb):
+    print(f"case max_vel np={npt} M={M} nb={nb}", flush=True)
+    rng = np.random.default_rng(50 + M)
+    V = fields(rng, npt, nb, M)
+    exp = F.fft_max_vel_dcl(V, 3 * M // 2)
+    assert abs(FP.FFT_MAX_VEL_DCL(None, V) - exp) <= 1e-13 * exp
+    d = FP.FFT_MAX_VEL_DCL(None, FP.fortran_tensor(V))
+    torch.cuda.synchronize()
+    assert abs(float(d.cpu()[0]) - exp) <= 1e-13 * exp
+
+
+def test_m_max_pad_other_than_three_halves():
+    """The grid follows m_max_pad: odd and larger pads give other coefficients."""
+    rng = np.random.default_rng(60)
+    M = 8
+    c1, c2 = fields(rng, 500, 2, M), fields(rng, 500, 2, M, 0.1)
+    c2[:, 0, 0] = 2.0
+    for mp in (8, 13, 20):
+        out = np.zeros((500, 2, M), order="F")
+        FP.FFT_PAR_DIV_DCL(None, c1, c2, out, m_max_pad=mp)
+        assert rel(out, F.fft_par_div_dcl(c1, c2, mp)) <= TOL
+
+
+def test_errors():
+    c = np.zeros((4, 2, 4), order="F")
+    with pytest.raises(SfxError):
+        FP.FFT_PAR_DIV_DCL(None, c, c, np.zeros_like(c), m_max_pad=3)      # m_max_pad < m_max
+    with pytest.raises(SfxError):
+        FP.FFT_HEAVISIDE_DCL(None, np.zeros((5, 4, 2, 4), order="F"), np.arange(6.0), np.zeros_like(c),
+                             coeff_tanh=0.1)                                 # 5 interfaces > 4
+
+
+def test_div_full_size_sampled_and_closed_form():
+    """1,404,928 points, m_max = 64 (the Gauss points of the 224x448 mesh)."""
+    npt, M = 1404928, 64
+    print(f"case div full size np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(61)
+    q1 = band_limited(rng, npt, M)
+    al, be = rng.uniform(2.0, 3.0, npt), rng.uniform(-1.5, 1.5, npt)
+    q2 = np.zeros((npt, 2, M))
+    q2[:, 0, 0], q2[:, 0, 1] = al, be
+    c1 = np.asfortranarray(times_alpha_beta_cos(q1, al, be))
+    q2 = np.asfortranarray(q2)
+    d = FP.fortran_empty((npt, 2, M))
+    FP.FFT_PAR_DIV_DCL(None, FP.fortran_tensor(c1), FP.fortran_tensor(q2), d)
+    torch.cuda.synchronize()
+    out = d.cpu().numpy()
+    assert rel(out, q1) <= 1e-12
+    idx = rng.choice(npt, 2000, replace=False)
+    assert rel(out[idx], F.fft_par_div_dcl(c1[idx], q2[idx], 96)) <= TOL
