@@ -32,6 +32,9 @@ coefficients, mode-0 sine set to zero):
   in one persistent launch replayed from a hipGraph, schedule autotuned.
 * ``nsrhs``: the full per-mode NS right-hand side with rhs_3x3's tensor
   (LES or variable viscosity) on the 224x448 mesh, m_max = 64.
+* ``exact``: the six non-polynomial FFT_PAR products (Heaviside of a level
+  set, pb = 2 scalar-vector, division, square root, tension tensor, velocity
+  maximum) on the reference grid N = 191 at 1,404,928 points, m_max = 64.
 * ``full``: the explicit nonlinear step at m_max = 128 on the 224x448 mesh:
   (curl u) x u -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and
   (rho u) (x) u with a level-set density, and the convective adjoint.
@@ -194,6 +197,58 @@ def wl_nsrhs(torch, args):
     return phases, G * M, cfg, ("nsrhs", m, M)
 
 
+def _exact_ops(FP, torch, npt, M, seed):
+    """Inputs and calls of the six non-polynomial products on npt points."""
+    dec = torch.exp(-torch.arange(M, device="cuda", dtype=torch.float64) / 4.0)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+
+    def fld(nb, scale=1.0, mean=None):
+        t = torch.randn((M, nb, npt), generator=gen, device="cuda", dtype=torch.float64) * dec[:, None, None] * scale
+        t[0, 1::2, :] = 0.0
+        if mean is not None:
+            t[0, 0, :] = mean
+        return t.permute(2, 1, 0)
+    phi = torch.randn((M, 2, npt, 1), generator=gen, device="cuda", dtype=torch.float64) * 0.3 * dec[:, None, None, None]
+    phi[0, 0] += 0.5
+    phi[0, 1] = 0.0
+    phi = phi.permute(3, 2, 1, 0)                                   # (1, np, 2, M)
+    mom, rho, c, d = fld(6), fld(2, 0.1, 1.5), fld(2), fld(2, 0.1, 2.0)
+    u, g = fld(6), fld(6)
+    rho_out, u_out = FP.fortran_empty((npt, 2, M)), FP.fortran_empty((npt, 6, M))
+    q_out, tens = FP.fortran_empty((npt, 2, M)), FP.fortran_empty((3, npt, 6, M))
+    s_io = FP.fortran_empty((npt, 2, M))
+    vmax = torch.empty(1, dtype=torch.float64, device="cuda")
+    vals = [1.0, 3.0]
+    K = 3 * M // 2
+    f_t = 4.0 * M * K                                               # flops of one real DFT per point
+    b = 8 * M * npt
+    return [
+        ("heaviside", lambda: FP.FFT_HEAVISIDE_DCL(None, phi, vals, rho_out, coeff_tanh=0.05), 4 * b, 2 * f_t * npt,
+         "exact_grid_kernel<HEAVI,1> + exact_r2c_kernel"),
+        ("scalar_vect_pb2", lambda: FP.FFT_SCALAR_VECT_DCL(None, mom, rho, u_out, 2), 14 * b, 7 * f_t * npt,
+         "exact_grid_kernel<SV2,4> + exact_r2c_kernel"),
+        ("div", lambda: FP.FFT_PAR_DIV_DCL(None, c, d, q_out), 6 * b, 3 * f_t * npt,
+         "exact_grid_kernel<DIV,2> + exact_r2c_kernel"),
+        ("squareroot", lambda: (s_io.copy_(rho_out), FP.FFT_SQUAREROOT_SCALAR(None, s_io)), 6 * b, 2 * f_t * npt,
+         "copy + exact_grid_kernel<SQRT,1> + exact_r2c_kernel"),
+        ("tension", lambda: FP.FFT_TENSOR_DCL(None, u, g, tens, opt_tension=True), 30 * b, 15 * f_t * npt,
+         "exact_grid_kernel<TENSION,6> + exact_r2c_kernel"),
+        ("max_vel", lambda: FP.FFT_MAX_VEL_DCL(None, u, V_out=vmax), 6 * b, 3 * f_t * npt,
+         "exact_grid_kernel<MAXVEL,3> + exact_max_kernel"),
+    ]
+
+
+def wl_exact(torch, args):
+    from sfemans_amd import fft_parallele as FP
+    npt, M = 1404928, 64
+    phases = [Phase(n, f, b, fl, k) for n, f, b, fl, k in _exact_ops(FP, torch, npt, M, SEED + 11)]
+    cfg = {"workload": "non-polynomial FFT_PAR products on the reference grid N=2*m_max_pad-1=191 at 1,404,928 "
+                       "points, m_max=64: HEAVISIDE (one level set), SCALAR_VECT pb=2, DIV, SQUAREROOT, "
+                       "TENSOR with tension, MAX_VEL",
+           "points": npt, "m_max": M, "grid": 2 * (3 * M // 2) - 1}
+    return phases, npt * M, cfg, ("exact", None, M)
+
+
 def wl_fftprod(torch, args):
     from sfemans_amd import fft_parallele as FP
     npt, M = 1404928, 64
@@ -348,7 +403,7 @@ def wl_full(torch, args):
 
 
 WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch,
-             "full": wl_full, "nsrhs": wl_nsrhs}
+             "full": wl_full, "nsrhs": wl_nsrhs, "exact": wl_exact}
 
 
 # ------------------------------------------------------------- CPU baseline
@@ -370,6 +425,34 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
     from oracle import cpu as OC
     from oracle import fft_par as F
     from sfemans_amd.synth import fourier_field
+    if kind == "exact":
+        # the oracle's six products (pocketfft on the same grid) on 2,048 points, same m_max
+        rng = np.random.default_rng(3)
+        npt = 2048
+        dec = np.exp(-np.arange(M) / 4.0)
+        fld = lambda nb, sc=1.0: np.asfortranarray(rng.standard_normal((npt, nb, M)) * dec * sc)
+        phi = np.asfortranarray(rng.standard_normal((1, npt, 2, M)) * 0.3 * dec)
+        phi[0, :, 0, 0] += 0.5
+        mom, rho, c, d, u, g = fld(6), fld(2, 0.1), fld(2), fld(2, 0.1), fld(6), fld(6)
+        rho[:, 0, 0], d[:, 0, 0] = 1.5, 2.0
+        mp = 3 * M // 2
+
+        def run():
+            F.fft_heaviside_dcl(phi, [1.0, 3.0], mp, 0.05)
+            F.fft_scalar_vect_dcl(mom, rho, 2, mp)
+            F.fft_par_div_dcl(c, d, mp)
+            F.fft_squareroot_scalar(rho, mp)
+            F.fft_tensor_dcl(u, g, mp, opt_tension=True)
+            F.fft_max_vel_dcl(u, mp)
+        run()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() < t0 + sample_s or n == 0:
+            run()
+            n += 1
+        dt = time.perf_counter() - t0
+        return {"value": n * npt * M / dt, "unit": "Gauss-pt*modes/s", "cores": 1, "kind": "port",
+                "sample": f"the six non-polynomial FFT_PAR products restated in numpy (pocketfft on the grid "
+                          f"N={2 * mp - 1}), {n} passes over {npt} points x m_max={M}, 1 thread, {dt:.1f} s"}
     if kind == "nsrhs":
         from oracle import assembly
         from sfemans_amd.mesh import rectangle_p2
