@@ -30,9 +30,10 @@ constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5 };
 
-// Angles per thread of the angle pass: 2*NI*JC accumulators stay at or below 64.
+// Angles per thread of the angle pass (2*NI*JC accumulators; the tension
+// tensor, 96 of them, re-reads its six inputs fewest times at 8).
 constexpr int exact_jc(int op, int nin) {
-  return op == EX_TENSION ? 4 : op == EX_SQRT ? 32 : nin <= 2 ? 16 : 8;
+  return op == EX_SQRT ? 32 : nin <= 2 ? 16 : 8;
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
@@ -123,6 +124,7 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
   for (int f = 0; f < NI; ++f)
 #pragma unroll
     for (int q = 0; q < JC; ++q) E[f][q] = O[f][q] = 0.0;
+#pragma unroll 2
   for (int m = 0; m < a.M; ++m) {
     const double* tc = a.ctab + (int64_t)m * a.nj + j0;
     const double* ts = a.ctab + (int64_t)(a.M + m) * a.nj + j0;
@@ -183,6 +185,7 @@ SFX_DEV void exact_r2c_body(const ExactArgs& a, int64_t p, int g, int mc) {
   for (int q = 0; q < EXACT_MC; ++q) A[q] = B[q] = 0.0;
   const int64_t plane = (int64_t)(a.K + 1) * a.np;
   const double* S = a.W + 2 * g * plane + p;
+#pragma unroll 4
   for (int j = 0; j <= a.K; ++j) {
     SFX_CHECK("exact W", 2 * g * plane + p + plane + (int64_t)j * a.np, a.n_W);
     const double sj = S[(int64_t)j * a.np], dj = S[plane + (int64_t)j * a.np];

@@ -5,7 +5,8 @@
 !
 ! The routines keep the reference's names, argument order and meaning
 ! (FFT_PAR_CROSS_PROD_DCL :147, FFT_PAR_DOT_PROD_DCL :330, FFT_PAR_PROD_DCL :502,
-! FFT_SCALAR_VECT_DCL :1021, FFT_TENSOR_DCL :1543).  The communicator is the
+! FFT_PAR_DIV_DCL :667, FFT_HEAVISIDE_DCL :832, FFT_SCALAR_VECT_DCL :1021,
+! FFT_MAX_VEL_DCL :1426, FFT_TENSOR_DCL :1543, FFT_SQUAREROOT_SCALAR :4601).  The communicator is the
 ! legacy integer MPI handle; nb_procs must be 1.  A non-zero status from
 ! libsfx stops the program with the library's message, as the reference does
 ! with WRITE + STOP or error_petsc.  Non-contiguous actual arguments (array
@@ -16,7 +17,8 @@ MODULE fft_parallele
   IMPLICIT NONE
   PRIVATE
   PUBLIC :: FFT_PAR_CROSS_PROD_DCL, FFT_PAR_DOT_PROD_DCL, FFT_PAR_PROD_DCL, &
-       FFT_SCALAR_VECT_DCL, FFT_TENSOR_DCL, sfx_mesh_register, sfx_mesh_release, &
+       FFT_SCALAR_VECT_DCL, FFT_TENSOR_DCL, FFT_PAR_DIV_DCL, FFT_HEAVISIDE_DCL, &
+       FFT_SQUAREROOT_SCALAR, FFT_MAX_VEL_DCL, sfx_mesh_register, sfx_mesh_release, &
        sfx_smb_cross_prod_gauss, sfx_smb_curlh_cross_b_gauss
 
   INTERFACE
@@ -60,6 +62,39 @@ MODULE fft_parallele
        INTEGER(C_INT32_T), VALUE :: nbf, m, nbp, mpad, tens
        INTEGER(C_INT) :: rc
      END FUNCTION c_tensor
+     FUNCTION c_div(V1, V2, Vo, np, m, nbp, bloc, mpad, temps) &
+          BIND(C, NAME='sfx_fft_par_div_dcl') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: V1, V2, Vo, temps
+       INTEGER(C_INT64_T), VALUE :: np, bloc
+       INTEGER(C_INT32_T), VALUE :: m, nbp, mpad
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_div
+     FUNCTION c_heavi(V1, vals, Vo, nv, np, nbf, m, nbp, bloc, mpad, eps, temps) &
+          BIND(C, NAME='sfx_fft_heaviside_dcl') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT, C_DOUBLE
+       TYPE(C_PTR), VALUE :: V1, vals, Vo, temps
+       INTEGER(C_INT64_T), VALUE :: np, bloc
+       INTEGER(C_INT32_T), VALUE :: nv, nbf, m, nbp, mpad
+       REAL(C_DOUBLE), VALUE :: eps
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_heavi
+     FUNCTION c_sqrt(c, np, nbf, m, nbp, bloc, mpad, temps) &
+          BIND(C, NAME='sfx_fft_squareroot_scalar') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: c, temps
+       INTEGER(C_INT64_T), VALUE :: np, bloc
+       INTEGER(C_INT32_T), VALUE :: nbf, m, nbp, mpad
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_sqrt
+     FUNCTION c_maxvel(V1, vo, np, nbf, m, nbp, bloc, mpad) &
+          BIND(C, NAME='sfx_fft_max_vel_dcl') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: V1, vo
+       INTEGER(C_INT64_T), VALUE :: np, bloc
+       INTEGER(C_INT32_T), VALUE :: nbf, m, nbp, mpad
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_maxvel
      FUNCTION c_mesh_create(jj, rr, n_w, l_G, me, np, handle) BIND(C, NAME='sfx_mesh_create') RESULT(rc)
        IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
        TYPE(C_PTR), VALUE :: jj, rr
@@ -155,6 +190,10 @@ CONTAINS
             INT(SIZE(V2_in,2),C_INT32_T), INT(SIZE(V1_in,3),C_INT32_T), INT(pb,C_INT32_T), &
             INT(nb_procs,C_INT32_T), INT(bloc_size,C_INT64_T), INT(m_max_pad,C_INT32_T), &
             temps_ptr(temps, tloc))
+    CASE (5)
+       rc = c_div(C_LOC(a1), C_LOC(a2), C_LOC(ao), np, INT(SIZE(V1_in,3),C_INT32_T), &
+            INT(nb_procs,C_INT32_T), INT(bloc_size,C_INT64_T), INT(m_max_pad,C_INT32_T), &
+            temps_ptr(temps, tloc))
     END SELECT
     CALL sfx_stop(rc, name)
     V_out = ao
@@ -225,6 +264,76 @@ CONTAINS
     IF (PRESENT(temps)) temps(1:3) = tloc
     DEALLOCATE(a1, a2, ao)
   END SUBROUTINE FFT_TENSOR_DCL
+
+  SUBROUTINE FFT_PAR_DIV_DCL(communicator, c1_in, c2_in, c_out, nb_procs, bloc_size, m_max_pad, temps)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(IN) :: c1_in, c2_in
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(OUT) :: c_out
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), DIMENSION(:), OPTIONAL, INTENT(INOUT) :: temps
+    CALL product3(5, 'FFT_PAR_DIV_DCL', c1_in, c2_in, c_out, nb_procs, bloc_size, m_max_pad, temps)
+  END SUBROUTINE FFT_PAR_DIV_DCL
+
+  SUBROUTINE FFT_HEAVISIDE_DCL(communicator, V1_in, values, V_out, nb_procs, bloc_size, m_max_pad, &
+       coeff_tanh, temps)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:,:), INTENT(IN) :: V1_in
+    REAL(KIND=8), DIMENSION(:), INTENT(IN) :: values
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(OUT) :: V_out
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), INTENT(IN) :: coeff_tanh
+    REAL(KIND=8), DIMENSION(:), OPTIONAL, INTENT(INOUT) :: temps
+    REAL(KIND=8), DIMENSION(:,:,:,:), ALLOCATABLE, TARGET :: a1
+    REAL(KIND=8), DIMENSION(:), ALLOCATABLE, TARGET :: av
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE, TARGET :: ao
+    REAL(KIND=8), DIMENSION(3), TARGET :: tloc
+    INTEGER(C_INT) :: rc
+    ALLOCATE(a1, SOURCE=V1_in)
+    ALLOCATE(av, SOURCE=values)
+    ALLOCATE(ao(SIZE(V_out,1), SIZE(V_out,2), SIZE(V_out,3)))
+    rc = c_heavi(C_LOC(a1), C_LOC(av), C_LOC(ao), INT(SIZE(values),C_INT32_T), INT(SIZE(V1_in,2),C_INT64_T), &
+         INT(SIZE(V1_in,3),C_INT32_T), INT(SIZE(V1_in,4),C_INT32_T), INT(nb_procs,C_INT32_T), &
+         INT(bloc_size,C_INT64_T), INT(m_max_pad,C_INT32_T), REAL(coeff_tanh,C_DOUBLE), temps_ptr(temps, tloc))
+    CALL sfx_stop(rc, 'FFT_HEAVISIDE_DCL')
+    V_out = ao
+    IF (PRESENT(temps)) temps(1:3) = tloc
+    DEALLOCATE(a1, av, ao)
+  END SUBROUTINE FFT_HEAVISIDE_DCL
+
+  SUBROUTINE FFT_SQUAREROOT_SCALAR(communicator, c1_inout, nb_procs, bloc_size, m_max_pad, temps)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(INOUT) :: c1_inout
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), DIMENSION(:), OPTIONAL, INTENT(INOUT) :: temps
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE, TARGET :: a1
+    REAL(KIND=8), DIMENSION(3), TARGET :: tloc
+    INTEGER(C_INT) :: rc
+    ALLOCATE(a1, SOURCE=c1_inout)
+    rc = c_sqrt(C_LOC(a1), INT(SIZE(c1_inout,1),C_INT64_T), INT(SIZE(c1_inout,2),C_INT32_T), &
+         INT(SIZE(c1_inout,3),C_INT32_T), INT(nb_procs,C_INT32_T), INT(bloc_size,C_INT64_T), &
+         INT(m_max_pad,C_INT32_T), temps_ptr(temps, tloc))
+    CALL sfx_stop(rc, 'FFT_SQUAREROOT_SCALAR')
+    c1_inout = a1
+    IF (PRESENT(temps)) temps(1:3) = tloc
+    DEALLOCATE(a1)
+  END SUBROUTINE FFT_SQUAREROOT_SCALAR
+
+  SUBROUTINE FFT_MAX_VEL_DCL(communicator, V1_in, V_out, nb_procs, bloc_size, m_max_pad)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(IN) :: V1_in
+    REAL(KIND=8), INTENT(OUT) :: V_out
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE, TARGET :: a1
+    REAL(KIND=8), TARGET :: vo
+    INTEGER(C_INT) :: rc
+    ALLOCATE(a1, SOURCE=V1_in)
+    rc = c_maxvel(C_LOC(a1), C_LOC(vo), INT(SIZE(V1_in,1),C_INT64_T), INT(SIZE(V1_in,2),C_INT32_T), &
+         INT(SIZE(V1_in,3),C_INT32_T), INT(nb_procs,C_INT32_T), INT(bloc_size,C_INT64_T), &
+         INT(m_max_pad,C_INT32_T))
+    CALL sfx_stop(rc, 'FFT_MAX_VEL_DCL')
+    V_out = vo
+    DEALLOCATE(a1)
+  END SUBROUTINE FFT_MAX_VEL_DCL
 
   ! Register the P2 mesh once (mesh%jj, mesh%rr of def_type_mesh).
   FUNCTION sfx_mesh_register(jj, rr) RESULT(handle)

@@ -1,15 +1,17 @@
 !===
 ! Exercises the fft_parallele shim the way the reference's callers do: a
-! cross product, a dot product and the smb_cross_prod_gauss_sft_par body on a
-! small P2 mesh read from a file.  Inputs are read from and outputs written to
+! cross product, a dot product, a division, a square root and a velocity
+! maximum (on the reference's own grid), and the smb_cross_prod_gauss_sft_par
+! body on a small P2 mesh read from a file.  Inputs are read from and outputs written to
 ! raw little-endian float64 files named on the command line, so that the
 ! calling test can check them.
 !
 ! usage: sfx_fortran_driver <dir>
 !   <dir>/dims.txt : np nb_field m_max me npm
 !   <dir>/v1.bin, v2.bin (np,6,m_max); jj.bin (6,me) int32; rr.bin (2,npm);
-!   <dir>/vn.bin (npm,6,m_max) nodal velocity
-! writes <dir>/cross.bin (np,6,m), dot.bin (np,2,m), smb.bin (7*me,6,m)
+!   <dir>/vn.bin (npm,6,m_max) nodal velocity; den.bin (np,2,m_max) a positive scalar
+! writes <dir>/cross.bin (np,6,m), dot.bin (np,2,m), div.bin (np,2,m) = v1(:,1:2)/den,
+!   sqrt.bin (np,2,m) = SQRT(MAX(0,MIN(1,den))), maxvel.bin (1) = max |v1|, smb.bin (7*me,6,m)
 !===
 PROGRAM sfx_fortran_driver
   USE, INTRINSIC :: ISO_C_BINDING
@@ -17,7 +19,8 @@ PROGRAM sfx_fortran_driver
   IMPLICIT NONE
   CHARACTER(LEN=1024) :: dir
   INTEGER :: np, nbf, m_max, me, npm, u, i
-  REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE :: v1, v2, vc, vd, vn, vs
+  REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE :: v1, v2, vc, vd, vn, vs, den, vq, vr
+  REAL(KIND=8), DIMENSION(1,1,1) :: vmax
   INTEGER, DIMENSION(:,:), ALLOCATABLE :: jj
   INTEGER(C_INT32_T), DIMENSION(:,:), ALLOCATABLE :: jj32
   REAL(KIND=8), DIMENSION(:,:), ALLOCATABLE :: rr
@@ -39,6 +42,16 @@ PROGRAM sfx_fortran_driver
   CALL FFT_PAR_DOT_PROD_DCL(comm, v1, v2, vd, 1, np, 3*m_max/2)
   CALL write_real(TRIM(dir)//'/cross.bin', vc)
   CALL write_real(TRIM(dir)//'/dot.bin', vd)
+
+  ALLOCATE(den(np,2,m_max), vq(np,2,m_max), vr(np,2,m_max))
+  CALL read_real(TRIM(dir)//'/den.bin', den)
+  CALL FFT_PAR_DIV_DCL(comm, v1(:,1:2,:), den, vq, 1, np, 3*m_max/2)
+  vr = den
+  CALL FFT_SQUAREROOT_SCALAR(comm, vr, 1, np, 3*m_max/2)
+  CALL FFT_MAX_VEL_DCL(comm, v1, vmax(1,1,1), 1, np, 3*m_max/2)
+  CALL write_real(TRIM(dir)//'/div.bin', vq)
+  CALL write_real(TRIM(dir)//'/sqrt.bin', vr)
+  CALL write_real(TRIM(dir)//'/maxvel.bin', vmax)
 
   ALLOCATE(jj32(6,me), jj(6,me), rr(2,npm), vn(npm,6,m_max), vs(7*me,6,m_max), list_mode(m_max))
   OPEN(NEWUNIT=u, FILE=TRIM(dir)//'/jj.bin', ACCESS='stream', FORM='unformatted', STATUS='old')

@@ -2,8 +2,9 @@
 
 Runs the compiled driver ``sfemans_amd/fortran/sfx_fortran_driver`` in a
 child process on raw float64 files and checks FFT_PAR_CROSS_PROD_DCL,
-FFT_PAR_DOT_PROD_DCL and the smb_cross_prod_gauss_sft_par body against the
-oracle (relative max-norm error <= 1e-12).
+FFT_PAR_DOT_PROD_DCL, FFT_PAR_DIV_DCL, FFT_SQUAREROOT_SCALAR, FFT_MAX_VEL_DCL
+and the smb_cross_prod_gauss_sft_par body against the oracle (relative
+max-norm error <= 1e-12).
 """
 import os
 import subprocess
@@ -38,8 +39,10 @@ def test_fortran_shim(tmp_path):
     v2 = fourier_field(npt, 6, M, 22, 2.0)
     m = rectangle_p2(3, 4)
     vn = fourier_field(m.np, 6, M, 23, 2.0)
+    den = fourier_field(npt, 2, M, 24, 2.0) * 0.1
+    den[:, 0, 0] = 0.8
     (tmp_path / "dims.txt").write_text(f"{npt} 6 {M} {m.me} {m.np}\n")
-    for name, a in (("v1", v1), ("v2", v2), ("vn", vn)):
+    for name, a in (("v1", v1), ("v2", v2), ("vn", vn), ("den", den)):
         np.asfortranarray(a, dtype="<f8").T.tofile(tmp_path / f"{name}.bin")
     np.asfortranarray(m.jj, dtype="<i4").T.tofile(tmp_path / "jj.bin")
     np.asfortranarray(m.rr, dtype="<f8").T.tofile(tmp_path / "rr.bin")
@@ -53,5 +56,9 @@ def test_fortran_shim(tmp_path):
     mp = 3 * M // 2
     assert rel(load("cross", (npt, 6, M)), F.fft_par_cross_prod_dcl(v1, v2, mp)) <= TOL
     assert rel(load("dot", (npt, 2, M)), F.fft_par_dot_prod_dcl(v1, v2, mp)) <= TOL
+    assert rel(load("div", (npt, 2, M)), F.fft_par_div_dcl(v1[:, 0:2], den, mp)) <= TOL
+    assert rel(load("sqrt", (npt, 2, M)), F.fft_squareroot_scalar(den, mp)) <= TOL
+    vmax = F.fft_max_vel_dcl(v1, mp)
+    assert abs(load("maxvel", (1, 1, 1))[0, 0, 0] - vmax) <= 1e-13 * vmax
     exp = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, np.arange(M), vn)
     assert rel(load("smb", (m.n_gauss, 6, M)), exp) <= TOL
