@@ -137,6 +137,25 @@ int sfx_fft_squareroot_scalar(double* c1_inout, int64_t np, int32_t nb_field, in
 int sfx_fft_squareroot_scalar_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
                                   int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_SCALAR_VECT_NO_OVERSHOOT (fft_parallel.F90:1222): as
+ * sfx_fft_scalar_vect_dcl (pb = 1 product, pb = 2 quotient) with the scalar
+ * clamped on the grid to [MINVAL(scalar_bounds), MAXVAL(scalar_bounds)]
+ * (:1352-1361); scalar_bounds is a host array of n_bounds values. */
+int sfx_fft_scalar_vect_no_overshoot(const double* scalar_bounds, int32_t n_bounds, const double* V1_in,
+                                     const double* V2_in, double* V_out, int64_t np, int32_t nb_field1,
+                                     int32_t nb_field2, int32_t m_max_c, int32_t pb, int32_t nb_procs,
+                                     int64_t bloc_size, int32_t m_max_pad, double* temps);
+int sfx_fft_scalar_vect_no_overshoot_dev(const double* scalar_bounds, int32_t n_bounds, const double* V1_in,
+                                         const double* V2_in, double* V_out, int64_t np, int32_t m_max_c,
+                                         int32_t pb, int32_t m_max_pad, void* stream);
+
+/* Replaces FFT_NO_OVERSHOOT_LEVEL_SET (fft_parallel.F90:3649): c1_inout(np,nb_field,m),
+ * its first scalar overwritten by MAX(0, MIN(1, c)) on the grid (:3771). */
+int sfx_fft_no_overshoot_level_set(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                   int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps);
+int sfx_fft_no_overshoot_level_set_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                       int32_t m_max_pad, void* stream);
+
 /* Replaces FFT_MAX_VEL_DCL (fft_parallel.F90:1426): *V_out = max over points and
  * grid angles of |v| (nb_field 6) or |c| (nb_field 2).  The _dev form writes
  * the maximum to device memory. */

@@ -19,7 +19,9 @@ grid size, so they are evaluated on exactly ``N = 2*m_max_pad-1`` points as
 the reference does: ``fft_par_div_dcl`` (:667-830), ``fft_scalar_vect_dcl``
 with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 :2234-2246), ``fft_squareroot_scalar`` (:4601-4771), ``fft_tensor_dcl`` with
-``opt_tension`` (:1688-1706) and ``fft_max_vel_dcl`` (:1426-1541).
+``opt_tension`` (:1688-1706), ``fft_max_vel_dcl`` (:1426-1541),
+``fft_scalar_vect_no_overshoot`` (:1222-1424) and
+``fft_no_overshoot_level_set`` (:3649-3819).
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
 used for larger parity cases and the CPU baseline) and ``"direct"`` (a direct
@@ -36,6 +38,7 @@ __all__ = [
     "fft_par_real", "fft_par_cross_prod_dcl", "fft_par_dot_prod_dcl",
     "fft_par_prod_dcl", "fft_scalar_vect_dcl", "fft_tensor_dcl",
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
+    "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set",
     "regul", "OPS",
 ]
 
@@ -331,3 +334,29 @@ def fft_max_vel_dcl(V1_in, m_max_pad, engine="fft"):
             nrm = np.sqrt(ru[:, 0] ** 2)
         best = max(best, float(np.max(nrm)))
     return best
+
+
+def fft_scalar_vect_no_overshoot(scalar_bounds, V1_in, V2_in, pb, m_max_pad, engine="fft"):
+    """FFT_SCALAR_VECT_NO_OVERSHOOT (fft_parallel.F90:1222-1424): as
+    fft_scalar_vect_dcl with the scalar clamped on the grid to
+    [MINVAL(scalar_bounds), MAXVAL(scalar_bounds)] (:1352-1361)."""
+    b = np.atleast_1d(np.asarray(scalar_bounds, dtype=np.float64))
+    lo, hi = float(np.min(b)), float(np.max(b))
+    nb1, nb2 = np.shape(V1_in)[1], np.shape(V2_in)[1]
+    if nb1 % 2 or nb2 % 2 or np.shape(V1_in)[2] == 0:
+        raise ValueError("BUG in FFT_SCALAR_VECT_NO_OVERSHOOT")
+    if not (nb1 == 6 and nb2 == 2) or pb not in (1, 2):
+        raise ValueError("error in dimension or problem type while calling FFT_SCALAR_VECT_NO_OVERSHOOT")
+
+    def op(ru):
+        s = np.maximum(lo, np.minimum(ru[:, 3:4], hi))
+        return s * ru[:, 0:3] if pb == 1 else ru[:, 0:3] / s
+    return fft_product([V1_in, V2_in], op, m_max_pad, engine)
+
+
+def fft_no_overshoot_level_set(c1_in, m_max_pad, engine="fft"):
+    """FFT_NO_OVERSHOOT_LEVEL_SET (fft_parallel.F90:3649-3819): MAX(0, MIN(1, c))
+    of the first scalar on the grid (:3771); the reference overwrites c1_inout(:, 1:2, :)."""
+    c = np.asarray(c1_in, dtype=np.float64)
+    _check(c.shape[1], c.shape[2], "FFT_NO_OVERSHOOT_LEVEL_SET")
+    return fft_product([c[:, 0:2]], lambda ru: np.maximum(0.0, np.minimum(1.0, ru[:, 0:1])), m_max_pad, engine)

@@ -53,6 +53,11 @@ SIGNATURES = {
     "sfx_fft_squareroot_scalar": [_vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
     "sfx_fft_squareroot_scalar_dev": [_vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_max_vel_dcl": [_vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32],
+    "sfx_fft_scalar_vect_no_overshoot": [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i64,
+                                         _i32, _vp],
+    "sfx_fft_scalar_vect_no_overshoot_dev": [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
+    "sfx_fft_no_overshoot_level_set": [_vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_no_overshoot_level_set_dev": [_vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_max_vel_dcl_dev": [_vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_mesh_create": [_vp, _vp, _i32, _i32, _i64, _i64, _vp],
     "sfx_mesh_destroy": [_vp],

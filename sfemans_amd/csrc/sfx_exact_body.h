@@ -28,7 +28,8 @@ constexpr int EXACT_MAX_IN = 6, EXACT_MAX_OUT = 9, EXACT_MAX_INT = 4;
 constexpr int EXACT_NJ_ALIGN = 32;   // c2r table rows padded to a multiple of every JC
 constexpr int EXACT_MC = 16;         // modes per r2c thread
 
-enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5 };
+enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
+               EX_SVCLAMP = 6, EX_CLAMP01 = 7 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -37,7 +38,7 @@ constexpr int exact_jc(int op, int nin) {
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
-  return op == EX_SV2 ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0 : 1;
+  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0 : 1;
 }
 
 // Fourier field: cos coefficient of mode m at point p at base[p*sp + m*sm],
@@ -69,6 +70,8 @@ struct ExactArgs {
   int n_int;            // EX_HEAVI: interfaces
   double vals[EXACT_MAX_INT + 1];
   double eps;           // EX_HEAVI: regul half-width (coeff_tanh)
+  int pb;               // EX_SVCLAMP: 1 product, 2 quotient
+  double smin, smax;    // EX_SVCLAMP: bounds of the scalar
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -88,6 +91,12 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
   } else if constexpr (OP == EX_SV2) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) w[k] = u[k] / u[3];               // :1155-1157
+  } else if constexpr (OP == EX_SVCLAMP) {
+    const double sc = fmax(a.smin, fmin(u[3], a.smax));         // :1352-1361
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = a.pb == 1 ? sc * u[k] : u[k] / sc;
+  } else if constexpr (OP == EX_CLAMP01) {
+    w[0] = fmax(0.0, fmin(1.0, u[0]));                            // :3771
   } else if constexpr (OP == EX_SQRT) {
     w[0] = sqrt(fmax(0.0, fmin(1.0, u[0])));                      // :4723
   } else if constexpr (OP == EX_HEAVI) {

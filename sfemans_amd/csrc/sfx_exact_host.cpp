@@ -99,6 +99,29 @@ static int sqrt_dev(const double* c, double* out, int64_t np, int nb, int M, int
   return rc ? rc : exact_run("FFT_SQUAREROOT_SCALAR", EX_SQRT, a, nullptr, st);
 }
 
+// V (np, 6, M), s (np, 2, M) clamped to [min, max] of bounds -> (np, 6, M)
+static int svclamp_dev(const double* bounds, int nbd, const double* V, const double* s, double* out, int64_t np,
+                       int M, int pb, int mp, hipStream_t st) {
+  if (!bounds || nbd < 1) return fail(SFX_ERR_ARG, "FFT_SCALAR_VECT_NO_OVERSHOOT: empty scalar_bounds");
+  double lo = bounds[0], hi = bounds[0];
+  for (int i = 1; i < nbd; ++i) {
+    lo = fmin(lo, bounds[i]);
+    hi = fmax(hi, bounds[i]);
+  }
+  ExactArgs a;
+  exact_setup_svclamp(a, V, s, out, np, M, pb, lo, hi);
+  int rc = exact_prepare("FFT_SCALAR_VECT_NO_OVERSHOOT", a, EX_SVCLAMP, np, M, mp);
+  return rc ? rc : exact_run("FFT_SCALAR_VECT_NO_OVERSHOOT", EX_SVCLAMP, a, nullptr, st);
+}
+
+// first scalar of c (np, nb, M) clamped to [0, 1] on the grid -> first scalar of out
+static int clamp01_dev(const double* c, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
+  ExactArgs a;
+  exact_setup_sqrt(a, c, out, np, nb, M);
+  int rc = exact_prepare("FFT_NO_OVERSHOOT_LEVEL_SET", a, EX_CLAMP01, np, M, mp);
+  return rc ? rc : exact_run("FFT_NO_OVERSHOOT_LEVEL_SET", EX_CLAMP01, a, nullptr, st);
+}
+
 // V1 (n_int, np, 2, M), values (n_int + 1) -> (np, 2, M)
 static int heavi_dev(const double* V1, const double* values, int n_values, double* out, int64_t np, int M,
                      int mp, double eps, hipStream_t st) {
@@ -153,6 +176,44 @@ static void set_temps(double* temps, const HostRun& t) {
     temps[1] = t.t2 - t.t1;
     temps[2] = 0.0;
   }
+}
+
+// In-place scalar products (square root, clamp to [0, 1]): the result overwrites
+// the first scalar of its own input, so the kernels read a copy.
+static int inplace_dev(int op, const char* nm, double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                       int32_t m_max_pad, hipStream_t st) {
+  if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in %s ", nm);
+  const size_t n = (size_t)np * nb_field * (m_max_c > 0 ? m_max_c : 0);
+  void* d = nullptr;
+  int rc = staging(SLOT_EX_IN1, n * 8, &d);
+  if (rc) return rc;
+  hipError_t e = n ? hipMemcpyAsync(d, c1_inout, n * 8, hipMemcpyDeviceToDevice, st) : hipSuccess;
+  if (e != hipSuccess) return hip_fail(e, nm);
+  return op == EX_SQRT ? sqrt_dev((const double*)d, c1_inout, np, nb_field, m_max_c, m_max_pad, st)
+                       : clamp01_dev((const double*)d, c1_inout, np, nb_field, m_max_c, m_max_pad, st);
+}
+
+static int inplace_host(int op, const char* nm, double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                        int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in %s ", nm);
+  int rc = check_procs(nm, np, nb_procs, bloc_size);
+  if (rc) return rc;
+  const size_t n = (size_t)np * nb_field * m_max_c;
+  void *d1, *d3;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, c1_inout, n, &d1)) || (rc = to_dev(SLOT_EX_OUT, c1_inout, n, &d3))) return rc;
+  t.t1 = now_s();
+  rc = op == EX_SQRT ? sqrt_dev((const double*)d1, (double*)d3, np, nb_field, m_max_c, m_max_pad, nullptr)
+                     : clamp01_dev((const double*)d1, (double*)d3, np, nb_field, m_max_c, m_max_pad, nullptr);
+  if (rc) return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(c1_inout, d3, n))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
 }
 
 }  // namespace sfx
@@ -228,36 +289,66 @@ int sfx_fft_heaviside_dcl(const double* V1_in, const double* values, double* V_o
 int sfx_fft_squareroot_scalar_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
                                   int32_t m_max_pad, void* stream) {
   clear_err();
-  if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0)
-    return fail(SFX_ERR_ARG, " BUG in FFT_SQUAREROOT_SCALAR ");
-  // the result overwrites its own input: run on a copy of the first scalar
-  void* d = nullptr;
-  int rc = staging(SLOT_EX_IN1, (size_t)np * nb_field * (m_max_c > 0 ? m_max_c : 0) * 8, &d);
-  if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(d, c1_inout, (size_t)np * nb_field * (m_max_c > 0 ? m_max_c : 0) * 8,
-                                hipMemcpyDeviceToDevice, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(e, "FFT_SQUAREROOT_SCALAR copy");
-  return sqrt_dev((const double*)d, c1_inout, np, nb_field, m_max_c, m_max_pad, (hipStream_t)stream);
+  return inplace_dev(EX_SQRT, "FFT_SQUAREROOT_SCALAR", c1_inout, np, nb_field, m_max_c, m_max_pad,
+                     (hipStream_t)stream);
 }
 
 int sfx_fft_squareroot_scalar(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c, int32_t nb_procs,
                               int64_t bloc_size, int32_t m_max_pad, double* temps) {
   clear_err();
+  return inplace_host(EX_SQRT, "FFT_SQUAREROOT_SCALAR", c1_inout, np, nb_field, m_max_c, nb_procs, bloc_size,
+                      m_max_pad, temps);
+}
+
+int sfx_fft_no_overshoot_level_set_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                       int32_t m_max_pad, void* stream) {
+  clear_err();
+  return inplace_dev(EX_CLAMP01, "FFT_NO_OVERSHOOT_LEVEL_SET", c1_inout, np, nb_field, m_max_c, m_max_pad,
+                     (hipStream_t)stream);
+}
+
+int sfx_fft_no_overshoot_level_set(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                   int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  clear_err();
+  return inplace_host(EX_CLAMP01, "FFT_NO_OVERSHOOT_LEVEL_SET", c1_inout, np, nb_field, m_max_c, nb_procs,
+                      bloc_size, m_max_pad, temps);
+}
+
+int sfx_fft_scalar_vect_no_overshoot_dev(const double* scalar_bounds, int32_t n_bounds, const double* V1_in,
+                                         const double* V2_in, double* V_out, int64_t np, int32_t m_max_c,
+                                         int32_t pb, int32_t m_max_pad, void* stream) {
+  clear_err();
+  if (pb != 1 && pb != 2) return fail(SFX_ERR_ARG, "error in problem type while calling FFT_SCALAR_VECT_NO_OVERSHOOT ");
+  return svclamp_dev(scalar_bounds, n_bounds, V1_in, V2_in, V_out, np, m_max_c, pb, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_scalar_vect_no_overshoot(const double* scalar_bounds, int32_t n_bounds, const double* V1_in,
+                                     const double* V2_in, double* V_out, int64_t np, int32_t nb_field1,
+                                     int32_t nb_field2, int32_t m_max_c, int32_t pb, int32_t nb_procs,
+                                     int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  clear_err();
   if (temps) temps[0] = temps[1] = temps[2] = 0.0;
-  if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0)
-    return fail(SFX_ERR_ARG, " BUG in FFT_SQUAREROOT_SCALAR ");
-  int rc = check_procs("FFT_SQUAREROOT_SCALAR", np, nb_procs, bloc_size);
+  if (nb_field1 % 2 != 0 || nb_field2 % 2 != 0 || m_max_c == 0)
+    return fail(SFX_ERR_ARG, " BUG in FFT_SCALAR_VECT_NO_OVERSHOOT ");
+  if (nb_field1 != 6 || nb_field2 != 2)
+    return fail(SFX_ERR_ARG, "error in dimension of INPUT variables while calling FFT_SCALAR_VECT_NO_OVERSHOOT ");
+  if (pb != 1 && pb != 2) return fail(SFX_ERR_ARG, "error in problem type while calling FFT_SCALAR_VECT_NO_OVERSHOOT ");
+  int rc = check_procs("FFT_SCALAR_VECT_NO_OVERSHOOT", np, nb_procs, bloc_size);
   if (rc) return rc;
-  const size_t n = (size_t)np * nb_field * m_max_c;
-  void *d1, *d3;
+  const size_t n1 = (size_t)np * 6 * m_max_c, n2 = (size_t)np * 2 * m_max_c;
+  void *d1, *d2, *d3;
   HostRun t;
   t.t0 = now_s();
-  if ((rc = to_dev(SLOT_EX_IN1, c1_inout, n, &d1)) || (rc = to_dev(SLOT_EX_OUT, c1_inout, n, &d3))) return rc;
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, V2_in, n2, &d2)) ||
+      (rc = staging(SLOT_EX_OUT, n1 * 8, &d3)))
+    return rc;
   t.t1 = now_s();
-  if ((rc = sqrt_dev((const double*)d1, (double*)d3, np, nb_field, m_max_c, m_max_pad, nullptr))) return rc;
+  if ((rc = svclamp_dev(scalar_bounds, n_bounds, (const double*)d1, (const double*)d2, (double*)d3, np, m_max_c,
+                        pb, m_max_pad, nullptr)))
+    return rc;
   if ((rc = from_dev(nullptr, d3, 0))) return rc;
   t.t2 = now_s();
-  if ((rc = from_dev(c1_inout, d3, n))) return rc;
+  if ((rc = from_dev(V_out, d3, n1))) return rc;
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;

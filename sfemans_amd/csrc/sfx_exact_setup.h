@@ -80,7 +80,15 @@ inline void exact_setup_sv2(ExactArgs& a, const double* V, const double* s, doub
   a.n_in[3] = np * 2 * (int64_t)M;
 }
 
-// FFT_SQUAREROOT_SCALAR: first scalar of c (np, nb, M) -> first scalar of out (np, nb, M).
+// FFT_SCALAR_VECT_NO_OVERSHOOT: as pb = 1 / 2 with the scalar clamped to [smin, smax].
+inline void exact_setup_svclamp(ExactArgs& a, const double* V, const double* s, double* out, int64_t np, int M,
+                                int pb, double smin, double smax) {
+  exact_setup_sv2(a, V, s, out, np, M);
+  a.pb = pb; a.smin = smin; a.smax = smax;
+}
+
+// FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
+// -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {
   memset(&a, 0, sizeof(a));
   a.nin = 1; a.nout = 1;

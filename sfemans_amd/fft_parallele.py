@@ -18,7 +18,8 @@ from ._lib import lib, check
 __all__ = [
     "FFT_PAR_CROSS_PROD_DCL", "FFT_PAR_DOT_PROD_DCL", "FFT_PAR_PROD_DCL",
     "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "FFT_PAR_DIV_DCL", "FFT_HEAVISIDE_DCL",
-    "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "fortran_tensor", "fortran_empty",
+    "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "FFT_SCALAR_VECT_NO_OVERSHOOT",
+    "FFT_NO_OVERSHOOT_LEVEL_SET", "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
 
@@ -195,19 +196,49 @@ def FFT_HEAVISIDE_DCL(communicator, V1_in, values, V_out, nb_procs=1, bloc_size=
     return V_out
 
 
-def FFT_SQUAREROOT_SCALAR(communicator, c1_inout, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
-    """c1_inout(np,nb,m): its first scalar becomes SQRT(MAX(0, MIN(1, c))) on the
-    grid of 2*m_max_pad-1 angles (fft_parallel.F90:4601), in place."""
+def _inplace(name, communicator, c1_inout, nb_procs, bloc_size, m_max_pad, temps):
     _single_rank(communicator)
     np_, nb, M = fdims(c1_inout)
     bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
     L = lib()
     if is_device(c1_inout):
-        check(L.sfx_fft_squareroot_scalar_dev(ptr(c1_inout), np_, nb, M, m_max_pad, stream_ptr()))
+        check(getattr(L, f"sfx_{name}_dev")(ptr(c1_inout), np_, nb, M, m_max_pad, stream_ptr()))
     else:
-        check(L.sfx_fft_squareroot_scalar(ptr(c1_inout), np_, nb, M, nb_procs, bloc_size, m_max_pad,
-                                          _temps_ptr(temps)))
+        check(getattr(L, f"sfx_{name}")(ptr(c1_inout), np_, nb, M, nb_procs, bloc_size, m_max_pad,
+                                        _temps_ptr(temps)))
     return c1_inout
+
+
+def FFT_SQUAREROOT_SCALAR(communicator, c1_inout, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """c1_inout(np,nb,m): its first scalar becomes SQRT(MAX(0, MIN(1, c))) on the
+    grid of 2*m_max_pad-1 angles (fft_parallel.F90:4601), in place."""
+    return _inplace("fft_squareroot_scalar", communicator, c1_inout, nb_procs, bloc_size, m_max_pad, temps)
+
+
+def FFT_NO_OVERSHOOT_LEVEL_SET(communicator, c1_inout, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """c1_inout(np,nb,m): its first scalar becomes MAX(0, MIN(1, c)) on the grid
+    of 2*m_max_pad-1 angles (fft_parallel.F90:3649), in place."""
+    return _inplace("fft_no_overshoot_level_set", communicator, c1_inout, nb_procs, bloc_size, m_max_pad, temps)
+
+
+def FFT_SCALAR_VECT_NO_OVERSHOOT(communicator, scalar_bounds, V1_in, V2_in, V_out, pb, nb_procs=1,
+                                 bloc_size=None, m_max_pad=None, temps=None):
+    """FFT_SCALAR_VECT_DCL with the scalar clamped to [MINVAL(scalar_bounds),
+    MAXVAL(scalar_bounds)] on the grid of 2*m_max_pad-1 angles (fft_parallel.F90:1222)."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    nb2 = fdims(V2_in)[1]
+    fdims(V_out)
+    bnd = np.ascontiguousarray(np.atleast_1d(np.asarray(scalar_bounds, dtype=np.float64)).ravel())
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_scalar_vect_no_overshoot_dev(bnd.ctypes.data, bnd.size, ptr(V1_in), ptr(V2_in), ptr(V_out),
+                                                     np_, M, pb, m_max_pad, stream_ptr()))
+    else:
+        check(L.sfx_fft_scalar_vect_no_overshoot(bnd.ctypes.data, bnd.size, ptr(V1_in), ptr(V2_in), ptr(V_out), np_,
+                                                 nb, nb2, M, pb, nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
+    return V_out
 
 
 def FFT_MAX_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=None, V_out=None):

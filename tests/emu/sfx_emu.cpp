@@ -401,7 +401,8 @@ static void emu_exact_grid(const ExactArgs& a, double& vmax) {
 
 // op: 0 DIV (in1 c1, in2 c2), 1 SV2 (in1 V, in2 s), 2 SQRT (in1 c (np,nb,M), out a copy of c),
 // 3 HEAVISIDE (in1 V1 (n_int,np,2,M), values), 4 TENSION (in1, in2 (np,6,M), out (3,np,6,M)),
-// 5 MAX_VEL (in1 (np,nb,M), out[0] = maximum).
+// 5 MAX_VEL (in1 (np,nb,M), out[0] = maximum), 6 SCALAR_VECT_NO_OVERSHOOT (in1 V, in2 s,
+// nb = pb, values = {min, max}), 7 NO_OVERSHOOT_LEVEL_SET (as 2).
 extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, const double* in2,
                          const double* values, int n_values, double eps, int nb, double* out) {
   ExactArgs a;
@@ -412,6 +413,8 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_HEAVI: exact_setup_heavi(a, in1, values, n_values, out, np, M, eps); break;
     case EX_TENSION: exact_setup_tension(a, in1, in2, out, np, M); break;
     case EX_MAXVEL: exact_setup_maxvel(a, in1, np, nb, M); break;
+    case EX_SVCLAMP: exact_setup_svclamp(a, in1, in2, out, np, M, nb, values[0], values[1]); break;
+    case EX_CLAMP01: exact_setup_sqrt(a, in1, out, np, nb, M); break;
     default: return 1;
   }
   exact_setup_sizes(a, np, M, mp);
@@ -428,6 +431,8 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_SV2: emu_exact_grid<EX_SV2, 4>(a, vmax); break;
     case EX_SQRT: emu_exact_grid<EX_SQRT, 1>(a, vmax); break;
     case EX_TENSION: emu_exact_grid<EX_TENSION, 6>(a, vmax); break;
+    case EX_SVCLAMP: emu_exact_grid<EX_SVCLAMP, 4>(a, vmax); break;
+    case EX_CLAMP01: emu_exact_grid<EX_CLAMP01, 1>(a, vmax); break;
     case EX_HEAVI:
       if (a.nin == 1) emu_exact_grid<EX_HEAVI, 1>(a, vmax);
       else if (a.nin == 2) emu_exact_grid<EX_HEAVI, 2>(a, vmax);

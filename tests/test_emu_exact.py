@@ -91,3 +91,28 @@ def test_max_vel(npt, M, mp, nb):
     V = fields(rng, npt, nb, M)
     out = emu(EX_MAXVEL, npt, M, mp, V, nb=nb, out=np.zeros(1))
     assert abs(out[0] - F.fft_max_vel_dcl(V, mp)) <= 1e-13 * F.fft_max_vel_dcl(V, mp)
+
+
+EX_SVCLAMP, EX_CLAMP01 = 6, 7
+
+
+@pytest.mark.parametrize("npt,M,mp,pb", [(9, 8, 12, 1), (7, 16, 24, 2), (3, 64, 96, 1)])
+def test_scalar_vect_no_overshoot(npt, M, mp, pb):
+    rng = np.random.default_rng(70 + M)
+    V = fields(rng, npt, 6, M)
+    s = fields(rng, npt, 2, M, 0.4)
+    s[:, 0, 0] = 1.5
+    bounds = [1.2, 1.8]
+    out = emu(EX_SVCLAMP, npt, M, mp, V, s, values=bounds, nb=pb, out=np.zeros((npt, 6, M), order="F"))
+    assert rel(out, F.fft_scalar_vect_no_overshoot(bounds, V, s, pb, mp)) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp,nb", [(7, 5, 8, 2), (11, 32, 48, 4)])
+def test_no_overshoot_level_set(npt, M, mp, nb):
+    rng = np.random.default_rng(80 + M)
+    c = fields(rng, npt, nb, M, 0.4)
+    c[:, 0, 0] = 0.5
+    out = emu(EX_CLAMP01, npt, M, mp, c, nb=nb, out=np.asfortranarray(c.copy()))
+    exp = c.copy()
+    exp[:, 0:2] = F.fft_no_overshoot_level_set(c, mp)
+    assert rel(out, exp) <= 1e-12

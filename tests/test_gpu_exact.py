@@ -135,6 +135,41 @@ def test_max_vel(npt, M, nb):
     assert abs(float(d.cpu()[0]) - exp) <= 1e-13 * exp
 
 
+@pytest.mark.parametrize("npt,M,pb", [(1000, 8, 1), (700, 16, 2), (400, 64, 1)])
+def test_scalar_vect_no_overshoot(npt, M, pb):
+    print(f"case scalar_vect_no_overshoot np={npt} M={M} pb={pb}", flush=True)
+    rng = np.random.default_rng(70 + M)
+    V = fields(rng, npt, 6, M)
+    s = fields(rng, npt, 2, M, 0.4)
+    s[:, 0, 0] = 1.5
+    bounds = np.array([1.2, 1.8])
+    exp = F.fft_scalar_vect_no_overshoot(bounds, V, s, pb, 3 * M // 2)
+    out = np.zeros((npt, 6, M), order="F")
+    FP.FFT_SCALAR_VECT_NO_OVERSHOOT(None, bounds, V, s, out, pb)
+    assert rel(out, exp) <= TOL
+    d = FP.fortran_empty((npt, 6, M))
+    FP.FFT_SCALAR_VECT_NO_OVERSHOOT(None, bounds, FP.fortran_tensor(V), FP.fortran_tensor(s), d, pb)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M,nb", [(900, 8, 2), (500, 32, 4)])
+def test_no_overshoot_level_set_in_place(npt, M, nb):
+    print(f"case no_overshoot_level_set np={npt} M={M} nb={nb}", flush=True)
+    rng = np.random.default_rng(80 + M)
+    c = fields(rng, npt, nb, M, 0.4)
+    c[:, 0, 0] = 0.5
+    exp = c.copy()
+    exp[:, 0:2] = F.fft_no_overshoot_level_set(c, 3 * M // 2)
+    h = np.asfortranarray(c.copy())
+    FP.FFT_NO_OVERSHOOT_LEVEL_SET(None, h)
+    assert rel(h, exp) <= TOL
+    d = FP.fortran_tensor(c)
+    FP.FFT_NO_OVERSHOOT_LEVEL_SET(None, d)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
 def test_m_max_pad_other_than_three_halves():
     """The grid follows m_max_pad: odd and larger pads give other coefficients."""
     rng = np.random.default_rng(60)

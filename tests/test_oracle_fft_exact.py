@@ -10,7 +10,10 @@ scaling, truncation) through a closed form that does not use the oracle:
 * Heaviside: regul at its knots, and a level set whose smoothed step is
   0 or 1 at every grid angle, whose coefficients are Dirichlet sums;
 * tension: a constant second field gives a tensor linear in the first;
-* velocity maximum: constant fields and single modes.
+* velocity maximum: constant fields and single modes;
+* no-overshoot products: inactive bounds give the band-limited product or
+  quotient, active bounds a constant factor; the [0, 1] clamp is the
+  identity inside and a constant outside.
 
 The pocketfft engine is also checked against the long-double direct sum.
 """
@@ -199,3 +202,42 @@ def test_fft_engine_matches_direct_long_double_sum(op):
         f = lambda e: F.fft_scalar_vect_dcl(V, pos, 2, mp, e)
     a, b = f("fft"), f("direct")
     assert rel(a, b) <= 1e-13
+
+
+def test_no_overshoot_product_with_inactive_bounds_is_the_bilinear_product():
+    """Bounds containing every grid value of the scalar: the clamp is the
+    identity, so pb = 1 gives the band-limited (grid-independent) product."""
+    rng = np.random.default_rng(11)
+    M, mp, npt = 8, 12, 25
+    V = band_limited(rng, npt, M, 6)
+    s = band_limited(rng, npt, M) * 0.05
+    s[:, 0, 0] = 1.0
+    out = F.fft_scalar_vect_no_overshoot([0.1, 5.0, 2.0], V, s, 1, mp)
+    assert rel(out, F.fft_scalar_vect_dcl(V, s, 1, mp)) <= 1e-13
+    al, be = s[:, 0, 0].copy(), np.zeros(npt)
+    W = band_limited(rng, npt, M, 6)
+    P = np.concatenate([times_alpha_beta_cos(W[:, 2 * c:2 * c + 2], al, be) for c in range(3)], axis=1)
+    s0 = np.zeros((npt, 2, M)); s0[:, 0, 0] = al
+    assert rel(F.fft_scalar_vect_no_overshoot([0.5, 2.0], P, s0, 2, mp), W) <= 1e-13
+
+
+def test_no_overshoot_active_bounds_clamp_to_a_constant():
+    rng = np.random.default_rng(12)
+    M, mp, npt = 6, 9, 10
+    V = band_limited(rng, npt, M, 6)
+    s = np.zeros((npt, 2, M)); s[:, 0, 0] = 10.0; s[:, 0, 1] = 1.0      # s in [9, 11] > max bound
+    assert rel(F.fft_scalar_vect_no_overshoot([0.5, 3.0], V, s, 1, mp), 3.0 * V) <= 1e-13
+    assert rel(F.fft_scalar_vect_no_overshoot([0.5, 3.0], V, s, 2, mp), V / 3.0) <= 1e-13
+
+
+def test_no_overshoot_level_set():
+    M, mp, npt = 6, 9, 4
+    c = np.zeros((npt, 2, M)); c[:, 0, 0], c[:, 0, 2] = 0.5, 0.3       # within [0.2, 0.8]: identity
+    assert np.max(np.abs(F.fft_no_overshoot_level_set(c, mp) - c)) <= 1e-15
+    hi = np.zeros_like(c); hi[:, 0, 0], hi[:, 1, 1] = 1.5, 0.4
+    one = np.zeros_like(c); one[:, 0, 0] = 1.0
+    assert np.max(np.abs(F.fft_no_overshoot_level_set(hi, mp) - one)) <= 1e-15
+    assert np.max(np.abs(F.fft_no_overshoot_level_set(-hi, mp))) <= 1e-15
+    rng = np.random.default_rng(13)
+    r = rng.standard_normal((npt, 2, M)) * 0.5 + 0.5
+    assert rel(F.fft_no_overshoot_level_set(r, mp), F.fft_no_overshoot_level_set(r, mp, "direct")) <= 1e-13
