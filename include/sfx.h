@@ -156,6 +156,19 @@ int sfx_fft_no_overshoot_level_set(double* c1_inout, int64_t np, int32_t nb_fiel
 int sfx_fft_no_overshoot_level_set_dev(double* c1_inout, int64_t np, int32_t nb_field, int32_t m_max_c,
                                        int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_PAR_ENTRO_VISC_DCL (fft_parallel.F90:4055): V1_in(np,6,m) vector,
+ * c1_in(np,2,m) scalar, c_in_real(2*m_max_pad-1, np) a field given on the grid
+ * angles; V_out(np,6,m) = coefficients of (coeff1_in_level - c_in_real) V1 and
+ * c_out(np,2,m) of MAX(0, c (1 - c)) (:4209-4219).  hloc_gauss is accepted for
+ * signature parity and not read (nullable). */
+int sfx_fft_par_entro_visc_dcl(const double* V1_in, const double* c1_in, const double* c_in_real,
+                               const double* hloc_gauss, double coeff1_in_level, double* V_out, double* c_out,
+                               int64_t np, int32_t nb_field1, int32_t nb_field2, int32_t m_max_c, int32_t nb_procs,
+                               int64_t bloc_size, int32_t m_max_pad, double* temps);
+int sfx_fft_par_entro_visc_dcl_dev(const double* V1_in, const double* c1_in, const double* c_in_real,
+                                   double coeff1_in_level, double* V_out, double* c_out, int64_t np,
+                                   int32_t m_max_c, int32_t m_max_pad, void* stream);
+
 /* Replaces FFT_MAX_VEL_DCL (fft_parallel.F90:1426): *V_out = max over points and
  * grid angles of |v| (nb_field 6) or |c| (nb_field 2).  The _dev form writes
  * the maximum to device memory. */

@@ -20,8 +20,9 @@ the reference does: ``fft_par_div_dcl`` (:667-830), ``fft_scalar_vect_dcl``
 with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 :2234-2246), ``fft_squareroot_scalar`` (:4601-4771), ``fft_tensor_dcl`` with
 ``opt_tension`` (:1688-1706), ``fft_max_vel_dcl`` (:1426-1541),
-``fft_scalar_vect_no_overshoot`` (:1222-1424) and
-``fft_no_overshoot_level_set`` (:3649-3819).
+``fft_scalar_vect_no_overshoot`` (:1222-1424),
+``fft_no_overshoot_level_set`` (:3649-3819) and ``fft_par_entro_visc_dcl``
+(:4055-4292).
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
 used for larger parity cases and the CPU baseline) and ``"direct"`` (a direct
@@ -38,7 +39,7 @@ __all__ = [
     "fft_par_real", "fft_par_cross_prod_dcl", "fft_par_dot_prod_dcl",
     "fft_par_prod_dcl", "fft_scalar_vect_dcl", "fft_tensor_dcl",
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
-    "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set",
+    "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
     "regul", "OPS",
 ]
 
@@ -360,3 +361,30 @@ def fft_no_overshoot_level_set(c1_in, m_max_pad, engine="fft"):
     c = np.asarray(c1_in, dtype=np.float64)
     _check(c.shape[1], c.shape[2], "FFT_NO_OVERSHOOT_LEVEL_SET")
     return fft_product([c[:, 0:2]], lambda ru: np.maximum(0.0, np.minimum(1.0, ru[:, 0:1])), m_max_pad, engine)
+
+
+def fft_par_entro_visc_dcl(V1_in, c1_in, c_in_real, coeff1_in_level, m_max_pad, engine="fft"):
+    """FFT_PAR_ENTRO_VISC_DCL (fft_parallel.F90:4055-4292): with c_in_real
+    (N, np) given on the grid, V_out = coefficients of -visc_entro * V1 with
+    visc_entro = -coeff1_in_level + c_in_real (:4212-4219), and c_out of
+    MAX(0, c (1 - c)) (:4209).  Returns (V_out (np,6,M), c_out (np,2,M))."""
+    V = np.asarray(V1_in, dtype=np.float64)
+    c = np.asarray(c1_in, dtype=np.float64)
+    cr = np.asarray(c_in_real, dtype=np.float64)
+    npt, nb1, M = V.shape
+    if nb1 != 6 or c.shape[1] != 2 or M == 0:
+        raise ValueError("error in problem type while calling FFT_PAR_ENTRO_VISC_DCL")
+    N = n_grid(m_max_pad)
+    if cr.shape != (N, npt):
+        raise ValueError("c_in_real must be (2*m_max_pad-1, np)")
+    outs_v, outs_c = [], []
+    for s0 in range(0, npt, _CHUNK):
+        sl = slice(s0, s0 + _CHUNK)
+        cu = pack(np.concatenate([V[sl], c[sl]], axis=1), m_max_pad)
+        ru = np.asarray(c2r(cu, N, engine), dtype=np.float64)        # (n, 4, N)
+        vis = -coeff1_in_level + cr[:, sl].T                             # (n, N)
+        pv = -vis[:, None, :] * ru[:, 0:3]
+        pc = np.maximum(0.0, ru[:, 3:4] * (1.0 - ru[:, 3:4]))
+        outs_v.append(unpack(np.asarray(r2c(pv, m_max_pad, engine), dtype=np.complex128), N, M))
+        outs_c.append(unpack(np.asarray(r2c(pc, m_max_pad, engine), dtype=np.complex128), N, M))
+    return np.concatenate(outs_v, axis=0), np.concatenate(outs_c, axis=0)

@@ -57,6 +57,9 @@ SIGNATURES = {
                                          _i32, _vp],
     "sfx_fft_scalar_vect_no_overshoot_dev": [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_no_overshoot_level_set": [_vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_par_entro_visc_dcl": [_vp, _vp, _vp, _vp, _d, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i64, _i32,
+                                   _vp],
+    "sfx_fft_par_entro_visc_dcl_dev": [_vp, _vp, _vp, _d, _vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_no_overshoot_level_set_dev": [_vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_max_vel_dcl_dev": [_vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_mesh_create": [_vp, _vp, _i32, _i32, _i64, _i64, _vp],

@@ -29,7 +29,7 @@ constexpr int EXACT_NJ_ALIGN = 32;   // c2r table rows padded to a multiple of e
 constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
-               EX_SVCLAMP = 6, EX_CLAMP01 = 7 };
+               EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -38,7 +38,8 @@ constexpr int exact_jc(int op, int nin) {
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
-  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0 : 1;
+  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0
+         : op == EX_ENTRO ? 4 : 1;
 }
 
 // Fourier field: cos coefficient of mode m at point p at base[p*sp + m*sm],
@@ -72,6 +73,9 @@ struct ExactArgs {
   double eps;           // EX_HEAVI: regul half-width (coeff_tanh)
   int pb;               // EX_SVCLAMP: 1 product, 2 quotient
   double smin, smax;    // EX_SVCLAMP: bounds of the scalar
+  const double* creal;  // EX_ENTRO: real-space field c_in_real(N, np) at [p*N + j]
+  int64_t n_creal;
+  double coeff;         // EX_ENTRO: coeff1_in_level
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -83,10 +87,16 @@ SFX_DEV double exact_regul(double phi, double eps) {
   return 1.0;
 }
 
-// The pointwise operation at one angle: u (NI fields) -> w (NO fields).
+// The pointwise operation at one angle: u (NI fields) -> w (NO fields); x is
+// the real-space input at that angle (EX_ENTRO only).
 template <int OP, int NI, int NO>
-SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO > 0 ? NO : 1]) {
-  if constexpr (OP == EX_DIV) {
+SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO > 0 ? NO : 1], double x) {
+  if constexpr (OP == EX_ENTRO) {
+    const double v = a.coeff - x;                                 // -visc_entro, :4212-4219
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = v * u[k];
+    w[3] = fmax(0.0, u[3] * (1.0 - u[3]));                        // :4209
+  } else if constexpr (OP == EX_DIV) {
     w[0] = u[0] / u[1];                                           // :782
   } else if constexpr (OP == EX_SV2) {
 #pragma unroll
@@ -167,9 +177,16 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
       up[f] = E[f][q] + O[f][q];
       um[f] = E[f][q] - O[f][q];
     }
+    double xp = 0.0, xm = 0.0;
+    if constexpr (OP == EX_ENTRO) {
+      const int64_t N = 2 * a.K + 1;
+      SFX_CHECK("exact creal", p * N + (j > 0 ? N - j : 0), a.n_creal);
+      xp = a.creal[p * N + j];
+      xm = j > 0 ? a.creal[p * N + (N - j)] : 0.0;
+    }
     double wp[NO > 0 ? NO : 1], wm[NO > 0 ? NO : 1];
-    exact_op<OP, NI, NO>(a, up, wp);
-    if (j > 0) exact_op<OP, NI, NO>(a, um, wm);
+    exact_op<OP, NI, NO>(a, up, wp, xp);
+    if (j > 0) exact_op<OP, NI, NO>(a, um, wm, xm);
     if constexpr (OP == EX_MAXVEL) {
       vmax = fmax(vmax, wp[0]);
       if (j > 0) vmax = fmax(vmax, wm[0]);

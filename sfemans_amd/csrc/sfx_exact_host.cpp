@@ -42,7 +42,8 @@ static int exact_tables(int M, int K, const double** ctab, const double** rtab) 
 }
 
 // Staging slots of the exact-grid products (the FFT products use 0-2).
-enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX_MAX = 7 };
+enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX_MAX = 7, SLOT_EX_IN3 = 8,
+       SLOT_EX_OUT2 = 9 };
 
 // Common set-up: sizes, tables, scratch.  `a.in`, `a.out`, nin, nout filled by the caller.
 static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad) {
@@ -120,6 +121,15 @@ static int clamp01_dev(const double* c, double* out, int64_t np, int nb, int M, 
   exact_setup_sqrt(a, c, out, np, nb, M);
   int rc = exact_prepare("FFT_NO_OVERSHOOT_LEVEL_SET", a, EX_CLAMP01, np, M, mp);
   return rc ? rc : exact_run("FFT_NO_OVERSHOOT_LEVEL_SET", EX_CLAMP01, a, nullptr, st);
+}
+
+static int entro_dev(const double* V, const double* c, const double* creal, double coeff, double* Vout,
+                     double* cout, int64_t np, int M, int mp, hipStream_t st) {
+  if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_PAR_ENTRO_VISC_DCL: null c_in_real");
+  ExactArgs a;
+  exact_setup_entro(a, V, c, creal, coeff, Vout, cout, np, M, mp);
+  int rc = exact_prepare("FFT_PAR_ENTRO_VISC_DCL", a, EX_ENTRO, np, M, mp);
+  return rc ? rc : exact_run("FFT_PAR_ENTRO_VISC_DCL", EX_ENTRO, a, nullptr, st);
 }
 
 // V1 (n_int, np, 2, M), values (n_int + 1) -> (np, 2, M)
@@ -349,6 +359,48 @@ int sfx_fft_scalar_vect_no_overshoot(const double* scalar_bounds, int32_t n_boun
   if ((rc = from_dev(nullptr, d3, 0))) return rc;
   t.t2 = now_s();
   if ((rc = from_dev(V_out, d3, n1))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_par_entro_visc_dcl_dev(const double* V1_in, const double* c1_in, const double* c_in_real,
+                                   double coeff1_in_level, double* V_out, double* c_out, int64_t np,
+                                   int32_t m_max_c, int32_t m_max_pad, void* stream) {
+  clear_err();
+  return entro_dev(V1_in, c1_in, c_in_real, coeff1_in_level, V_out, c_out, np, m_max_c, m_max_pad,
+                   (hipStream_t)stream);
+}
+
+int sfx_fft_par_entro_visc_dcl(const double* V1_in, const double* c1_in, const double* c_in_real,
+                               const double* hloc_gauss, double coeff1_in_level, double* V_out, double* c_out,
+                               int64_t np, int32_t nb_field1, int32_t nb_field2, int32_t m_max_c, int32_t nb_procs,
+                               int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  clear_err();
+  (void)hloc_gauss;   // read by the reference but not used by this product
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field1 % 2 != 0 || nb_field2 % 2 != 0 || m_max_c == 0)
+    return fail(SFX_ERR_ARG, " BUG in FFT_PAR_ENTRO_VISC_DCL ");
+  if (nb_field1 != 6 || nb_field2 != 2)
+    return fail(SFX_ERR_ARG, "error in problem type while calling FFT_PAR_ENTRO_VISC_DCL");
+  int rc = check_procs("FFT_PAR_ENTRO_VISC_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (m_max_pad < 1 || m_max_pad > 4096) return fail(SFX_ERR_ARG, "FFT_PAR_ENTRO_VISC_DCL: m_max_pad=%d", m_max_pad);
+  const size_t n1 = (size_t)np * 6 * m_max_c, n2 = (size_t)np * 2 * m_max_c, nr = (size_t)np * (2 * m_max_pad - 1);
+  void *d1, *d2, *dr, *d3, *d4;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, c1_in, n2, &d2)) ||
+      (rc = to_dev(SLOT_EX_IN3, c_in_real, nr, &dr)) || (rc = staging(SLOT_EX_OUT, n1 * 8, &d3)) ||
+      (rc = staging(SLOT_EX_OUT2, n2 * 8, &d4)))
+    return rc;
+  t.t1 = now_s();
+  if ((rc = entro_dev((const double*)d1, (const double*)d2, (const double*)dr, coeff1_in_level, (double*)d3,
+                      (double*)d4, np, m_max_c, m_max_pad, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(V_out, d3, n1)) || (rc = from_dev(c_out, d4, n2))) return rc;
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;

@@ -87,6 +87,25 @@ inline void exact_setup_svclamp(ExactArgs& a, const double* V, const double* s, 
   a.pb = pb; a.smin = smin; a.smax = smax;
 }
 
+// FFT_PAR_ENTRO_VISC_DCL: V (np, 6, M), c (np, 2, M), c_in_real (N, np) ->
+// V_out (np, 6, M) = (coeff - c_in_real) V, c_out (np, 2, M) = MAX(0, c (1 - c)).
+inline void exact_setup_entro(ExactArgs& a, const double* V, const double* c, const double* creal, double coeff,
+                              double* Vout, double* cout, int64_t np, int M, int m_max_pad) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 4; a.nout = 4;
+  for (int k = 0; k < 3; ++k) {
+    a.in[k] = exact_col(V, np, 6, k);
+    a.out[k] = exact_colo(Vout, np, 6, k);
+    a.n_in[k] = a.n_out[k] = np * 6 * (int64_t)M - 2 * k * np;
+  }
+  a.in[3] = exact_col(c, np, 2, 0);
+  a.out[3] = exact_colo(cout, np, 2, 0);
+  a.n_in[3] = a.n_out[3] = np * 2 * (int64_t)M;
+  a.creal = creal;
+  a.n_creal = (2 * (int64_t)m_max_pad - 1) * np;
+  a.coeff = coeff;
+}
+
 // FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
 // -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {

@@ -19,7 +19,7 @@ __all__ = [
     "FFT_PAR_CROSS_PROD_DCL", "FFT_PAR_DOT_PROD_DCL", "FFT_PAR_PROD_DCL",
     "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "FFT_PAR_DIV_DCL", "FFT_HEAVISIDE_DCL",
     "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "FFT_SCALAR_VECT_NO_OVERSHOOT",
-    "FFT_NO_OVERSHOOT_LEVEL_SET", "fortran_tensor", "fortran_empty",
+    "FFT_NO_OVERSHOOT_LEVEL_SET", "FFT_PAR_ENTRO_VISC_DCL", "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
 
@@ -259,3 +259,27 @@ def FFT_MAX_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=N
     out = np.zeros(1)
     check(L.sfx_fft_max_vel_dcl(ptr(V1_in), out.ctypes.data, np_, nb, M, nb_procs, bloc_size, m_max_pad))
     return float(out[0])
+
+
+def FFT_PAR_ENTRO_VISC_DCL(communicator, V1_in, c1_in, c_in_real, hloc_gauss, coeff1_in_level, V_out, c_out,
+                           nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """V_out(np,6,m) = (coeff1_in_level - c_in_real) V1 and c_out(np,2,m) =
+    MAX(0, c (1 - c)) on the grid of 2*m_max_pad-1 angles, c_in_real given on
+    that grid, shape (2*m_max_pad-1, np) (fft_parallel.F90:4055)."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    nb2 = fdims(c1_in)[1]
+    fdims(V_out), fdims(c_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    nr, npr = fdims(c_in_real)
+    if nr != 2 * m_max_pad - 1 or npr != np_:
+        raise ValueError("c_in_real must be (2*m_max_pad-1, np)")
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_par_entro_visc_dcl_dev(ptr(V1_in), ptr(c1_in), ptr(c_in_real), float(coeff1_in_level),
+                                               ptr(V_out), ptr(c_out), np_, M, m_max_pad, stream_ptr()))
+    else:
+        check(L.sfx_fft_par_entro_visc_dcl(ptr(V1_in), ptr(c1_in), ptr(c_in_real), ptr(hloc_gauss),
+                                           float(coeff1_in_level), ptr(V_out), ptr(c_out), np_, nb, nb2, M,
+                                           nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
+    return V_out, c_out

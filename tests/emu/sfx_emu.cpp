@@ -402,7 +402,8 @@ static void emu_exact_grid(const ExactArgs& a, double& vmax) {
 // op: 0 DIV (in1 c1, in2 c2), 1 SV2 (in1 V, in2 s), 2 SQRT (in1 c (np,nb,M), out a copy of c),
 // 3 HEAVISIDE (in1 V1 (n_int,np,2,M), values), 4 TENSION (in1, in2 (np,6,M), out (3,np,6,M)),
 // 5 MAX_VEL (in1 (np,nb,M), out[0] = maximum), 6 SCALAR_VECT_NO_OVERSHOOT (in1 V, in2 s,
-// nb = pb, values = {min, max}), 7 NO_OVERSHOOT_LEVEL_SET (as 2).
+// nb = pb, values = {min, max}), 7 NO_OVERSHOOT_LEVEL_SET (as 2), 8 ENTRO_VISC (in1 V,
+// in2 = [c (np,2,M) | c_in_real (N,np)], values = {coeff}, out = [V_out | c_out]).
 extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, const double* in2,
                          const double* values, int n_values, double eps, int nb, double* out) {
   ExactArgs a;
@@ -415,6 +416,9 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_MAXVEL: exact_setup_maxvel(a, in1, np, nb, M); break;
     case EX_SVCLAMP: exact_setup_svclamp(a, in1, in2, out, np, M, nb, values[0], values[1]); break;
     case EX_CLAMP01: exact_setup_sqrt(a, in1, out, np, nb, M); break;
+    case EX_ENTRO:   // in2 = c (np,2,M) then c_in_real (N,np) packed after it; out = V_out then c_out
+      exact_setup_entro(a, in1, in2, in2 + 2 * np * M, values[0], out, out + 6 * np * M, np, M, mp);
+      break;
     default: return 1;
   }
   exact_setup_sizes(a, np, M, mp);
@@ -433,6 +437,7 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_TENSION: emu_exact_grid<EX_TENSION, 6>(a, vmax); break;
     case EX_SVCLAMP: emu_exact_grid<EX_SVCLAMP, 4>(a, vmax); break;
     case EX_CLAMP01: emu_exact_grid<EX_CLAMP01, 1>(a, vmax); break;
+    case EX_ENTRO: emu_exact_grid<EX_ENTRO, 4>(a, vmax); break;
     case EX_HEAVI:
       if (a.nin == 1) emu_exact_grid<EX_HEAVI, 1>(a, vmax);
       else if (a.nin == 2) emu_exact_grid<EX_HEAVI, 2>(a, vmax);

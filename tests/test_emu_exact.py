@@ -116,3 +116,21 @@ def test_no_overshoot_level_set(npt, M, mp, nb):
     exp = c.copy()
     exp[:, 0:2] = F.fft_no_overshoot_level_set(c, mp)
     assert rel(out, exp) <= 1e-12
+
+
+EX_ENTRO = 8
+
+
+@pytest.mark.parametrize("npt,M,mp", [(9, 8, 12), (5, 16, 24), (3, 64, 96)])
+def test_entro_visc(npt, M, mp):
+    rng = np.random.default_rng(90 + M)
+    V = fields(rng, npt, 6, M)
+    c = fields(rng, npt, 2, M, 0.3)
+    c[:, 0, 0] = 0.5
+    N = 2 * mp - 1
+    creal = rng.standard_normal((N, npt))
+    packed = np.concatenate([c.ravel(order="F"), creal.ravel(order="F")])
+    out = emu(EX_ENTRO, npt, M, mp, V, packed, values=[0.3], out=np.zeros(npt * 8 * M))
+    Vo, co = F.fft_par_entro_visc_dcl(V, c, creal, 0.3, mp)
+    assert rel(out[:npt * 6 * M].reshape((npt, 6, M), order="F"), Vo) <= 1e-12
+    assert rel(out[npt * 6 * M:].reshape((npt, 2, M), order="F"), co) <= 1e-12

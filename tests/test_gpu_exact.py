@@ -170,6 +170,26 @@ def test_no_overshoot_level_set_in_place(npt, M, nb):
     assert rel(d.cpu().numpy(), exp) <= TOL
 
 
+@pytest.mark.parametrize("npt,M", [(800, 8), (600, 32), (300, 64)])
+def test_entro_visc(npt, M):
+    print(f"case entro_visc np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(90 + M)
+    V = fields(rng, npt, 6, M)
+    c = fields(rng, npt, 2, M, 0.3)
+    c[:, 0, 0] = 0.5
+    mp = 3 * M // 2
+    creal = np.asfortranarray(rng.standard_normal((2 * mp - 1, npt)))
+    Vo, co = F.fft_par_entro_visc_dcl(V, c, creal, 0.3, mp)
+    hv, hc = np.zeros((npt, 6, M), order="F"), np.zeros((npt, 2, M), order="F")
+    FP.FFT_PAR_ENTRO_VISC_DCL(None, V, c, creal, None, 0.3, hv, hc)
+    assert rel(hv, Vo) <= TOL and rel(hc, co) <= TOL
+    dv, dc = FP.fortran_empty((npt, 6, M)), FP.fortran_empty((npt, 2, M))
+    FP.FFT_PAR_ENTRO_VISC_DCL(None, FP.fortran_tensor(V), FP.fortran_tensor(c), FP.fortran_tensor(creal), None,
+                              0.3, dv, dc)
+    torch.cuda.synchronize()
+    assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
+
+
 def test_m_max_pad_other_than_three_halves():
     """The grid follows m_max_pad: odd and larger pads give other coefficients."""
     rng = np.random.default_rng(60)

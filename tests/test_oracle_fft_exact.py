@@ -13,7 +13,10 @@ scaling, truncation) through a closed form that does not use the oracle:
 * velocity maximum: constant fields and single modes;
 * no-overshoot products: inactive bounds give the band-limited product or
   quotient, active bounds a constant factor; the [0, 1] clamp is the
-  identity inside and a constant outside.
+  identity inside and a constant outside;
+* entropy viscosity: with c_in_real = coeff - s on the grid (s band-limited,
+  its grid values by explicit sums) the vector part is the product s V; with
+  0 <= c <= 1 the scalar part is c - c*c; it clamps at zero above 1.
 
 The pocketfft engine is also checked against the long-double direct sum.
 """
@@ -241,3 +244,40 @@ def test_no_overshoot_level_set():
     rng = np.random.default_rng(13)
     r = rng.standard_normal((npt, 2, M)) * 0.5 + 0.5
     assert rel(F.fft_no_overshoot_level_set(r, mp), F.fft_no_overshoot_level_set(r, mp, "direct")) <= 1e-13
+
+
+def grid_values(C, mp):
+    """Values on the N = 2*mp-1 grid angles of fields C (np, 2, M) by explicit
+    sums a_0 + sum a_m cos(m t) + b_m sin(m t): (N, np)."""
+    N = 2 * mp - 1
+    t = 2 * np.pi * np.arange(N) / N
+    m = np.arange(C.shape[2])
+    cs, sn = np.cos(np.outer(t, m)), np.sin(np.outer(t, m))
+    sn[:, 0] = 0.0
+    return cs @ C[:, 0, :].T + sn @ C[:, 1, :].T
+
+
+def test_entro_visc_vector_part_is_a_band_limited_product():
+    """c_in_real = coeff - s on the grid, s band-limited: V_out is the
+    coefficients of s*V, which the scalar-vector product gives."""
+    rng = np.random.default_rng(14)
+    M, mp, npt = 8, 12, 20
+    V = band_limited(rng, npt, M, 6)
+    S = band_limited(rng, npt, M)
+    coeff = 0.7
+    creal = coeff - grid_values(S, mp)
+    c = band_limited(rng, npt, M) * 0.05
+    c[:, 0, 0] = 0.5
+    Vo, co = F.fft_par_entro_visc_dcl(V, c, creal, coeff, mp)
+    assert rel(Vo, F.fft_scalar_vect_dcl(V, S, 1, mp)) <= 1e-12
+    # 0 <= c <= 1 on the grid: MAX(0, c(1-c)) = c - c*c, a band-limited product
+    assert np.min(grid_values(c, mp)) > 0 and np.max(grid_values(c, mp)) < 1
+    assert rel(co, c - F.fft_par_prod_dcl(c, c, mp)) <= 1e-12
+
+
+def test_entro_visc_scalar_part_clamps_at_zero():
+    M, mp, npt = 6, 9, 5
+    V = np.zeros((npt, 6, M))
+    c = np.zeros((npt, 2, M)); c[:, 0, 0], c[:, 0, 1] = 2.0, 0.5      # c in [1.5, 2.5]: c(1-c) < 0
+    _, co = F.fft_par_entro_visc_dcl(V, c, np.zeros((2 * mp - 1, npt)), 1.0, mp)
+    assert np.max(np.abs(co)) == 0.0
