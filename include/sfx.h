@@ -169,6 +169,29 @@ int sfx_fft_par_entro_visc_dcl_dev(const double* V1_in, const double* c1_in, con
                                    double coeff1_in_level, double* V_out, double* c_out, int64_t np,
                                    int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM (fft_parallel.F90:3419): V1_in,
+ * V2_in, V3_in (np,6,m) the three momentum gradients, c_in_real(2*m_max_pad-1, np)
+ * the entropy viscosity on the grid angles, les_coeff1_mom the module input the
+ * reference reads (inputs%LES_coeff1_mom); V_out(3,np,6,m), V_out(a,:,2f-1:2f,:)
+ * = coefficients of (c_in_real - les_coeff1_mom) V_a,f (:3552-3568). */
+int sfx_fft_compute_entropy_visc_grad_mom(const double* V1_in, const double* V2_in, const double* V3_in,
+                                          const double* c_in_real, double les_coeff1_mom, double* V_out, int64_t np,
+                                          int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                                          int32_t m_max_pad, double* temps);
+int sfx_fft_compute_entropy_visc_grad_mom_dev(const double* V1_in, const double* V2_in, const double* V3_in,
+                                              const double* c_in_real, double les_coeff1_mom, double* V_out,
+                                              int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
+
+/* Replaces FFT_COMPUTE_DIFFU_MOM (fft_parallel.F90:2939): V1_out = s V1 and
+ * V2_out = s V2 with V1, V2 (np,6,m) vectors and s = V3 (np,2,m) a scalar;
+ * band-limited products, computed by the product kernel. */
+int sfx_fft_compute_diffu_mom(const double* V1_in, const double* V2_in, const double* V3_in, double* V1_out,
+                              double* V2_out, int64_t np, int32_t nb_field1, int32_t nb_field2, int32_t nb_field3,
+                              int32_t m_max_c, int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
+                              double* temps);
+int sfx_fft_compute_diffu_mom_dev(const double* V1_in, const double* V2_in, const double* V3_in, double* V1_out,
+                                  double* V2_out, int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
+
 /* Replaces FFT_MAX_VEL_DCL (fft_parallel.F90:1426): *V_out = max over points and
  * grid angles of |v| (nb_field 6) or |c| (nb_field 2).  The _dev form writes
  * the maximum to device memory. */

@@ -21,8 +21,9 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 :2234-2246), ``fft_squareroot_scalar`` (:4601-4771), ``fft_tensor_dcl`` with
 ``opt_tension`` (:1688-1706), ``fft_max_vel_dcl`` (:1426-1541),
 ``fft_scalar_vect_no_overshoot`` (:1222-1424),
-``fft_no_overshoot_level_set`` (:3649-3819) and ``fft_par_entro_visc_dcl``
-(:4055-4292).
+``fft_no_overshoot_level_set`` (:3649-3819), ``fft_par_entro_visc_dcl``
+(:4055-4292) and ``fft_compute_entropy_visc_grad_mom`` (:3419-3647);
+``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
 used for larger parity cases and the CPU baseline) and ``"direct"`` (a direct
@@ -40,6 +41,7 @@ __all__ = [
     "fft_par_prod_dcl", "fft_scalar_vect_dcl", "fft_tensor_dcl",
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
+    "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom",
     "regul", "OPS",
 ]
 
@@ -388,3 +390,34 @@ def fft_par_entro_visc_dcl(V1_in, c1_in, c_in_real, coeff1_in_level, m_max_pad, 
         outs_v.append(unpack(np.asarray(r2c(pv, m_max_pad, engine), dtype=np.complex128), N, M))
         outs_c.append(unpack(np.asarray(r2c(pc, m_max_pad, engine), dtype=np.complex128), N, M))
     return np.concatenate(outs_v, axis=0), np.concatenate(outs_c, axis=0)
+
+
+def fft_compute_entropy_visc_grad_mom(V1_in, V2_in, V3_in, c_in_real, les_coeff1_mom, m_max_pad, engine="fft"):
+    """FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM (fft_parallel.F90:3419-3647): with
+    visco_entro = -LES_coeff1_mom + c_in_real on the grid (:3553), V_out(a, n,
+    2f-1:2f, m) = coefficients of visco_entro * V_a,f (:3560-3568, :3635-3640)."""
+    Vs = [np.asarray(x, dtype=np.float64) for x in (V1_in, V2_in, V3_in)]
+    cr = np.asarray(c_in_real, dtype=np.float64)
+    npt, nb, M = Vs[0].shape
+    _check(nb, M, "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM")
+    N = n_grid(m_max_pad)
+    if cr.shape != (N, npt):
+        raise ValueError("c_in_real must be (2*m_max_pad-1, np)")
+    res = np.empty((3, npt, 6, M))
+    for s0 in range(0, npt, _CHUNK):
+        sl = slice(s0, s0 + _CHUNK)
+        vis = -les_coeff1_mom + cr[:, sl].T                              # (n, N)
+        for a in range(3):
+            ru = np.asarray(c2r(pack(Vs[a][sl], m_max_pad), N, engine), dtype=np.float64)
+            pc = r2c(vis[:, None, :] * ru, m_max_pad, engine)
+            res[a, sl] = unpack(np.asarray(pc, dtype=np.complex128), N, M)
+    return res
+
+
+def fft_compute_diffu_mom(V1_in, V2_in, V3_in, m_max_pad, engine="fft"):
+    """FFT_COMPUTE_DIFFU_MOM (fft_parallel.F90:2939-3158): (s V1, s V2) with
+    s = V3_in (:3082-3087)."""
+    if np.shape(V1_in)[1] != 6 or np.shape(V2_in)[1] != 6 or np.shape(V3_in)[1] != 2:
+        raise ValueError("error on inputs while calling FFT_COMPUTE_DIFFU_MOM")
+    return (fft_product([V1_in, V3_in], "scalar_vect_1", m_max_pad, engine),
+            fft_product([V2_in, V3_in], "scalar_vect_1", m_max_pad, engine))

@@ -243,6 +243,26 @@ int sfx_fft_tensor_dcl(const double* V1_in, const double* V2_in, double* V_out, 
   return run_fftprod_host(OP_TENSOR, nm, V1_in, V2_in, V_out, np, 6, 6, 18, m_max_c, temps);
 }
 
+// FFT_COMPUTE_DIFFU_MOM: two band-limited products s * V1 and s * V2.
+int sfx_fft_compute_diffu_mom(const double* V1_in, const double* V2_in, const double* V3_in, double* V1_out,
+                              double* V2_out, int64_t np, int32_t nb_field1, int32_t nb_field2, int32_t nb_field3,
+                              int32_t m_max_c, int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
+                              double* temps) {
+  clear_err();
+  const char* nm = "FFT_COMPUTE_DIFFU_MOM";
+  if (nb_field1 != 6 || nb_field2 != 6 || nb_field3 != 2 || m_max_c == 0)
+    return fail(SFX_ERR_ARG, "error on inputs while calling FFT_COMPUTE_DIFFU_MOM");
+  int rc;
+  if ((rc = check_modes(nm, np, m_max_c, m_max_pad)) || (rc = check_procs(nm, np, nb_procs, bloc_size))) return rc;
+  double t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+  if ((rc = run_fftprod_host(OP_SVECT1, nm, V1_in, V3_in, V1_out, np, 6, 2, 6, m_max_c, temps ? t1 : nullptr)))
+    return rc;
+  if ((rc = run_fftprod_host(OP_SVECT1, nm, V2_in, V3_in, V2_out, np, 6, 2, 6, m_max_c, temps ? t2 : nullptr)))
+    return rc;
+  if (temps) for (int i = 0; i < 3; ++i) temps[i] = t1[i] + t2[i];
+  return SFX_OK;
+}
+
 #define DEV_CHECK(nm)                                            \
   clear_err();                                                   \
   {                                                              \
@@ -283,6 +303,16 @@ int sfx_fft_tensor_dcl_dev(const double* V1_in, const double* V2_in, double* V_o
   if (opt_tension) return exact_tension_dev(V1_in, V2_in, V_out, np, m_max_c, m_max_pad, (hipStream_t)stream);
   DEV_CHECK("FFT_TENSOR_DCL");
   return run_fftprod_dev(OP_TENSOR, "FFT_TENSOR_DCL", V1_in, V2_in, V_out, np, 6, 6, 18, m_max_c,
+                         (hipStream_t)stream);
+}
+
+int sfx_fft_compute_diffu_mom_dev(const double* V1_in, const double* V2_in, const double* V3_in, double* V1_out,
+                                  double* V2_out, int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream) {
+  DEV_CHECK("FFT_COMPUTE_DIFFU_MOM");
+  int rc = run_fftprod_dev(OP_SVECT1, "FFT_COMPUTE_DIFFU_MOM", V1_in, V3_in, V1_out, np, 6, 2, 6, m_max_c,
+                           (hipStream_t)stream);
+  if (rc) return rc;
+  return run_fftprod_dev(OP_SVECT1, "FFT_COMPUTE_DIFFU_MOM", V2_in, V3_in, V2_out, np, 6, 2, 6, m_max_c,
                          (hipStream_t)stream);
 }
 

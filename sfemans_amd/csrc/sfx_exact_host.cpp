@@ -43,7 +43,7 @@ static int exact_tables(int M, int K, const double** ctab, const double** rtab) 
 
 // Staging slots of the exact-grid products (the FFT products use 0-2).
 enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX_MAX = 7, SLOT_EX_IN3 = 8,
-       SLOT_EX_OUT2 = 9 };
+       SLOT_EX_OUT2 = 9, SLOT_EX_IN4 = 10 };
 
 // Common set-up: sizes, tables, scratch.  `a.in`, `a.out`, nin, nout filled by the caller.
 static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad) {
@@ -130,6 +130,15 @@ static int entro_dev(const double* V, const double* c, const double* creal, doub
   exact_setup_entro(a, V, c, creal, coeff, Vout, cout, np, M, mp);
   int rc = exact_prepare("FFT_PAR_ENTRO_VISC_DCL", a, EX_ENTRO, np, M, mp);
   return rc ? rc : exact_run("FFT_PAR_ENTRO_VISC_DCL", EX_ENTRO, a, nullptr, st);
+}
+
+static int gradmom_dev(const double* V1, const double* V2, const double* V3, const double* creal, double coeff,
+                       double* out, int64_t np, int M, int mp, hipStream_t st) {
+  if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: null c_in_real");
+  ExactArgs a;
+  exact_setup_gradmom(a, V1, V2, V3, creal, coeff, out, np, M, mp);
+  int rc = exact_prepare("FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM", a, EX_GRADMOM, np, M, mp);
+  return rc ? rc : exact_run("FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM", EX_GRADMOM, a, nullptr, st);
 }
 
 // V1 (n_int, np, 2, M), values (n_int + 1) -> (np, 2, M)
@@ -401,6 +410,45 @@ int sfx_fft_par_entro_visc_dcl(const double* V1_in, const double* c1_in, const d
   if ((rc = from_dev(nullptr, d3, 0))) return rc;
   t.t2 = now_s();
   if ((rc = from_dev(V_out, d3, n1)) || (rc = from_dev(c_out, d4, n2))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_compute_entropy_visc_grad_mom_dev(const double* V1_in, const double* V2_in, const double* V3_in,
+                                              const double* c_in_real, double les_coeff1_mom, double* V_out,
+                                              int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream) {
+  clear_err();
+  return gradmom_dev(V1_in, V2_in, V3_in, c_in_real, les_coeff1_mom, V_out, np, m_max_c, m_max_pad,
+                     (hipStream_t)stream);
+}
+
+int sfx_fft_compute_entropy_visc_grad_mom(const double* V1_in, const double* V2_in, const double* V3_in,
+                                          const double* c_in_real, double les_coeff1_mom, double* V_out, int64_t np,
+                                          int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                                          int32_t m_max_pad, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field != 6 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM ");
+  int rc = check_procs("FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (m_max_pad < 1 || m_max_pad > 4096)
+    return fail(SFX_ERR_ARG, "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: m_max_pad=%d", m_max_pad);
+  const size_t n1 = (size_t)np * 6 * m_max_c, nr = (size_t)np * (2 * m_max_pad - 1), no = 3 * n1;
+  void *d1, *d2, *d3, *dr, *dout;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, V2_in, n1, &d2)) ||
+      (rc = to_dev(SLOT_EX_IN4, V3_in, n1, &d3)) || (rc = to_dev(SLOT_EX_IN3, c_in_real, nr, &dr)) ||
+      (rc = staging(SLOT_EX_OUT, no * 8, &dout)))
+    return rc;
+  t.t1 = now_s();
+  if ((rc = gradmom_dev((const double*)d1, (const double*)d2, (const double*)d3, (const double*)dr,
+                        les_coeff1_mom, (double*)dout, np, m_max_c, m_max_pad, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, dout, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(V_out, dout, no))) return rc;
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;

@@ -106,6 +106,25 @@ inline void exact_setup_entro(ExactArgs& a, const double* V, const double* c, co
   a.coeff = coeff;
 }
 
+// FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: V1, V2, V3 (np, 6, M), c_in_real (N, np) ->
+// out (3, np, 6, M), out(a, :, 2f:2f+1, :) = coefficients of (c_in_real - coeff) V_a,f.
+inline void exact_setup_gradmom(ExactArgs& a, const double* V1, const double* V2, const double* V3,
+                                const double* creal, double coeff, double* out, int64_t np, int M, int m_max_pad) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 9; a.nout = 9;
+  const double* V[3] = {V1, V2, V3};
+  for (int r = 0; r < 3; ++r)
+    for (int f = 0; f < 3; ++f) {
+      a.in[3 * r + f] = exact_col(V[r], np, 6, f);
+      a.n_in[3 * r + f] = np * 6 * (int64_t)M - 2 * f * np;
+      a.out[3 * r + f] = exact_outf(out + r + 3 * np * 2 * f, 3, 3 * np * 6, 3 * np);
+      a.n_out[3 * r + f] = 3 * np * 6 * (int64_t)M - r - 3 * np * 2 * f;
+    }
+  a.creal = creal;
+  a.n_creal = (2 * (int64_t)m_max_pad - 1) * np;
+  a.coeff = coeff;
+}
+
 // FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
 // -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {

@@ -19,7 +19,8 @@ __all__ = [
     "FFT_PAR_CROSS_PROD_DCL", "FFT_PAR_DOT_PROD_DCL", "FFT_PAR_PROD_DCL",
     "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "FFT_PAR_DIV_DCL", "FFT_HEAVISIDE_DCL",
     "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "FFT_SCALAR_VECT_NO_OVERSHOOT",
-    "FFT_NO_OVERSHOOT_LEVEL_SET", "FFT_PAR_ENTRO_VISC_DCL", "fortran_tensor", "fortran_empty",
+    "FFT_NO_OVERSHOOT_LEVEL_SET", "FFT_PAR_ENTRO_VISC_DCL", "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM",
+    "FFT_COMPUTE_DIFFU_MOM", "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
 
@@ -283,3 +284,44 @@ def FFT_PAR_ENTRO_VISC_DCL(communicator, V1_in, c1_in, c_in_real, hloc_gauss, co
                                            float(coeff1_in_level), ptr(V_out), ptr(c_out), np_, nb, nb2, M,
                                            nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
     return V_out, c_out
+
+
+def FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM(communicator, V1_in, V2_in, V3_in, c_in_real, V_out, les_coeff1_mom,
+                                      nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """V_out(3,np,6,m): coefficients of (c_in_real - les_coeff1_mom) V_a on the grid
+    of 2*m_max_pad-1 angles (fft_parallel.F90:3419); ``les_coeff1_mom`` is the
+    reference's inputs%LES_coeff1_mom, passed explicitly."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    fdims(V2_in), fdims(V3_in), fdims(V_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    if tuple(fdims(c_in_real)) != (2 * m_max_pad - 1, np_):
+        raise ValueError("c_in_real must be (2*m_max_pad-1, np)")
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_compute_entropy_visc_grad_mom_dev(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(c_in_real),
+                                                          float(les_coeff1_mom), ptr(V_out), np_, M, m_max_pad,
+                                                          stream_ptr()))
+    else:
+        check(L.sfx_fft_compute_entropy_visc_grad_mom(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(c_in_real),
+                                                      float(les_coeff1_mom), ptr(V_out), np_, nb, M, nb_procs,
+                                                      bloc_size, m_max_pad, _temps_ptr(temps)))
+    return V_out
+
+
+def FFT_COMPUTE_DIFFU_MOM(communicator, V1_in, V2_in, V3_in, V1_out, V2_out, nb_procs=1, bloc_size=None,
+                          m_max_pad=None, temps=None):
+    """V1_out = s V1, V2_out = s V2 with s = V3_in (fft_parallel.F90:2939)."""
+    _single_rank(communicator)
+    np_, nb1, M = fdims(V1_in)
+    nb2, nb3 = fdims(V2_in)[1], fdims(V3_in)[1]
+    fdims(V1_out), fdims(V2_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_compute_diffu_mom_dev(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(V1_out), ptr(V2_out), np_, M,
+                                              m_max_pad, stream_ptr()))
+    else:
+        check(L.sfx_fft_compute_diffu_mom(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(V1_out), ptr(V2_out), np_, nb1,
+                                          nb2, nb3, M, nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
+    return V1_out, V2_out

@@ -134,3 +134,17 @@ def test_entro_visc(npt, M, mp):
     Vo, co = F.fft_par_entro_visc_dcl(V, c, creal, 0.3, mp)
     assert rel(out[:npt * 6 * M].reshape((npt, 6, M), order="F"), Vo) <= 1e-12
     assert rel(out[npt * 6 * M:].reshape((npt, 2, M), order="F"), co) <= 1e-12
+
+
+EX_GRADMOM = 9
+
+
+@pytest.mark.parametrize("npt,M,mp", [(7, 8, 12), (3, 32, 48)])
+def test_entropy_visc_grad_mom(npt, M, mp):
+    rng = np.random.default_rng(95 + M)
+    Vs = [fields(rng, npt, 6, M) for _ in range(3)]
+    N = 2 * mp - 1
+    creal = rng.standard_normal((N, npt))
+    packed = np.concatenate([v.ravel(order="F") for v in Vs] + [creal.ravel(order="F")])
+    out = emu(EX_GRADMOM, npt, M, mp, packed, values=[0.25], out=np.zeros((3, npt, 6, M), order="F"))
+    assert rel(out, F.fft_compute_entropy_visc_grad_mom(*Vs, creal, 0.25, mp)) <= 1e-12

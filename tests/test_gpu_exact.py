@@ -190,6 +190,32 @@ def test_entro_visc(npt, M):
     assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
 
 
+@pytest.mark.parametrize("npt,M", [(500, 8), (300, 64)])
+def test_entropy_visc_grad_mom_and_diffu_mom(npt, M):
+    print(f"case entropy_visc_grad_mom / diffu_mom np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(95 + M)
+    Vs = [fields(rng, npt, 6, M) for _ in range(3)]
+    mp = 3 * M // 2
+    creal = np.asfortranarray(rng.standard_normal((2 * mp - 1, npt)))
+    exp = F.fft_compute_entropy_visc_grad_mom(*Vs, creal, 0.25, mp)
+    h = np.zeros((3, npt, 6, M), order="F")
+    FP.FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM(None, *Vs, creal, h, 0.25)
+    assert rel(h, exp) <= TOL
+    d = FP.fortran_empty((3, npt, 6, M))
+    FP.FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM(None, *[FP.fortran_tensor(v) for v in Vs], FP.fortran_tensor(creal), d, 0.25)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+    s = fields(rng, npt, 2, M)
+    e1, e2 = F.fft_compute_diffu_mom(Vs[0], Vs[1], s, mp)
+    o1, o2 = np.zeros((npt, 6, M), order="F"), np.zeros((npt, 6, M), order="F")
+    FP.FFT_COMPUTE_DIFFU_MOM(None, Vs[0], Vs[1], s, o1, o2)
+    assert rel(o1, e1) <= TOL and rel(o2, e2) <= TOL
+    d1, d2 = FP.fortran_empty((npt, 6, M)), FP.fortran_empty((npt, 6, M))
+    FP.FFT_COMPUTE_DIFFU_MOM(None, FP.fortran_tensor(Vs[0]), FP.fortran_tensor(Vs[1]), FP.fortran_tensor(s), d1, d2)
+    torch.cuda.synchronize()
+    assert rel(d1.cpu().numpy(), e1) <= TOL and rel(d2.cpu().numpy(), e2) <= TOL
+
+
 def test_m_max_pad_other_than_three_halves():
     """The grid follows m_max_pad: odd and larger pads give other coefficients."""
     rng = np.random.default_rng(60)

@@ -16,7 +16,9 @@ scaling, truncation) through a closed form that does not use the oracle:
   identity inside and a constant outside;
 * entropy viscosity: with c_in_real = coeff - s on the grid (s band-limited,
   its grid values by explicit sums) the vector part is the product s V; with
-  0 <= c <= 1 the scalar part is c - c*c; it clamps at zero above 1.
+  0 <= c <= 1 the scalar part is c - c*c; it clamps at zero above 1; the
+  momentum-gradient form likewise gives s V_a; the diffusion products are
+  scalar-vector products.
 
 The pocketfft engine is also checked against the long-double direct sum.
 """
@@ -281,3 +283,26 @@ def test_entro_visc_scalar_part_clamps_at_zero():
     c = np.zeros((npt, 2, M)); c[:, 0, 0], c[:, 0, 1] = 2.0, 0.5      # c in [1.5, 2.5]: c(1-c) < 0
     _, co = F.fft_par_entro_visc_dcl(V, c, np.zeros((2 * mp - 1, npt)), 1.0, mp)
     assert np.max(np.abs(co)) == 0.0
+
+
+def test_grad_mom_entropy_viscosity_is_a_band_limited_product():
+    """c_in_real = coeff + s on the grid, s band-limited: each output is the
+    scalar-vector product s V_a."""
+    rng = np.random.default_rng(15)
+    M, mp, npt = 8, 12, 12
+    Vs = [band_limited(rng, npt, M, 6) for _ in range(3)]
+    S = band_limited(rng, npt, M)
+    coeff = 0.4
+    out = F.fft_compute_entropy_visc_grad_mom(*Vs, coeff + grid_values(S, mp), coeff, mp)
+    for a in range(3):
+        assert rel(out[a], F.fft_scalar_vect_dcl(Vs[a], S, 1, mp)) <= 1e-12
+
+
+def test_diffu_mom_is_two_scalar_vector_products():
+    rng = np.random.default_rng(16)
+    M, mp, npt = 6, 9, 10
+    V1, V2, s = band_limited(rng, npt, M, 6), band_limited(rng, npt, M, 6), band_limited(rng, npt, M)
+    o1, o2 = F.fft_compute_diffu_mom(V1, V2, s, mp)
+    assert rel(o1, F.fft_scalar_vect_dcl(V1, s, 1, mp)) <= 1e-14
+    assert rel(o2, F.fft_scalar_vect_dcl(V2, s, 1, mp)) <= 1e-14
+    assert rel(o1, F.fft_compute_diffu_mom(V1, V2, s, mp, "direct")[0]) <= 1e-13
