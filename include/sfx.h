@@ -169,6 +169,23 @@ int sfx_fft_par_entro_visc_dcl_dev(const double* V1_in, const double* c1_in, con
                                    double coeff1_in_level, double* V_out, double* c_out, int64_t np,
                                    int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_PAR_COMPR_ENTRO_VISC_DCL (fft_parallel.F90:3160): V1_in = grad(phi),
+ * V2_in = grad(phi_reg) (np,6,m), c1_in = phi (np,2,m), c_in_real(2*m_max_pad-1, np)
+ * the entropy viscosity on the grid angles, level_set_comp_factor the module
+ * input the reference reads (inputs%level_set_comp_factor); on the grid
+ * c2 = MAX(0, phi (1 - phi)), V_out = coefficients of
+ * (coeff1_in_level - c_in_real) V1 + c_in_real comp c2/(|V2| + 1e-14) V2 and
+ * c_out of c2 (:3316-3330).  hloc_gauss is accepted for signature parity. */
+int sfx_fft_par_compr_entro_visc_dcl(const double* V1_in, const double* V2_in, const double* c1_in,
+                                     const double* c_in_real, const double* hloc_gauss, double coeff1_in_level,
+                                     double level_set_comp_factor, double* V_out, double* c_out, int64_t np,
+                                     int32_t nb_field1, int32_t nb_field2, int32_t nb_field3, int32_t m_max_c,
+                                     int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps);
+int sfx_fft_par_compr_entro_visc_dcl_dev(const double* V1_in, const double* V2_in, const double* c1_in,
+                                         const double* c_in_real, double coeff1_in_level,
+                                         double level_set_comp_factor, double* V_out, double* c_out, int64_t np,
+                                         int32_t m_max_c, int32_t m_max_pad, void* stream);
+
 /* Replaces FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM (fft_parallel.F90:3419): V1_in,
  * V2_in, V3_in (np,6,m) the three momentum gradients, c_in_real(2*m_max_pad-1, np)
  * the entropy viscosity on the grid angles, les_coeff1_mom the module input the

@@ -22,7 +22,8 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 ``opt_tension`` (:1688-1706), ``fft_max_vel_dcl`` (:1426-1541),
 ``fft_scalar_vect_no_overshoot`` (:1222-1424),
 ``fft_no_overshoot_level_set`` (:3649-3819), ``fft_par_entro_visc_dcl``
-(:4055-4292) and ``fft_compute_entropy_visc_grad_mom`` (:3419-3647);
+(:4055-4292), ``fft_par_compr_entro_visc_dcl`` (:3160-3417) and
+``fft_compute_entropy_visc_grad_mom`` (:3419-3647);
 ``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
@@ -41,7 +42,7 @@ __all__ = [
     "fft_par_prod_dcl", "fft_scalar_vect_dcl", "fft_tensor_dcl",
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
-    "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom",
+    "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom", "fft_par_compr_entro_visc_dcl",
     "regul", "OPS",
 ]
 
@@ -421,3 +422,36 @@ def fft_compute_diffu_mom(V1_in, V2_in, V3_in, m_max_pad, engine="fft"):
         raise ValueError("error on inputs while calling FFT_COMPUTE_DIFFU_MOM")
     return (fft_product([V1_in, V3_in], "scalar_vect_1", m_max_pad, engine),
             fft_product([V2_in, V3_in], "scalar_vect_1", m_max_pad, engine))
+
+
+def fft_par_compr_entro_visc_dcl(V1_in, V2_in, c1_in, c_in_real, coeff1_in_level, level_set_comp_factor,
+                                 m_max_pad, engine="fft"):
+    """FFT_PAR_COMPR_ENTRO_VISC_DCL (fft_parallel.F90:3160-3417), on the grid
+    (:3316-3330): c2 = MAX(0, phi (1 - phi)), |grad phi_reg| + 1e-14,
+    visc_entro = -coeff1_in_level + c_in_real, visc_comp = c_in_real *
+    level_set_comp_factor * c2 / |grad phi_reg|; V_out = coefficients of
+    -visc_entro grad phi + visc_comp grad phi_reg, c_out of c2."""
+    V1 = np.asarray(V1_in, dtype=np.float64)
+    V2 = np.asarray(V2_in, dtype=np.float64)
+    c = np.asarray(c1_in, dtype=np.float64)
+    cr = np.asarray(c_in_real, dtype=np.float64)
+    npt, _, M = V1.shape
+    if V1.shape[1] != 6 or V2.shape[1] != 6 or c.shape[1] != 2 or M == 0:
+        raise ValueError("error in problem type while calling FFT_PAR_COMPR_ENTRO_VISC_DCL")
+    N = n_grid(m_max_pad)
+    if cr.shape != (N, npt):
+        raise ValueError("c_in_real must be (2*m_max_pad-1, np)")
+    outs_v, outs_c = [], []
+    for s0 in range(0, npt, _CHUNK):
+        sl = slice(s0, s0 + _CHUNK)
+        ru = np.asarray(c2r(pack(np.concatenate([V1[sl], V2[sl], c[sl]], axis=1), m_max_pad), N, engine),
+                        dtype=np.float64)                                  # (n, 7, N)
+        c2 = np.maximum(0.0, ru[:, 6] * (1.0 - ru[:, 6]))
+        ng = np.sqrt(ru[:, 3] ** 2 + ru[:, 4] ** 2 + ru[:, 5] ** 2) + 1e-14
+        x = cr[:, sl].T
+        ve = -coeff1_in_level + x
+        vc = x * level_set_comp_factor * c2 / ng
+        pv = -ve[:, None, :] * ru[:, 0:3] + vc[:, None, :] * ru[:, 3:6]
+        outs_v.append(unpack(np.asarray(r2c(pv, m_max_pad, engine), dtype=np.complex128), N, M))
+        outs_c.append(unpack(np.asarray(r2c(c2[:, None, :], m_max_pad, engine), dtype=np.complex128), N, M))
+    return np.concatenate(outs_v, axis=0), np.concatenate(outs_c, axis=0)

@@ -80,6 +80,7 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
     case EX_CLAMP01: e = grid_launch<EX_CLAMP01, 1>(a, st, &nblk); break;
     case EX_ENTRO: e = grid_launch<EX_ENTRO, 4>(a, st, &nblk); break;
     case EX_GRADMOM: e = grid_launch<EX_GRADMOM, 9>(a, st, &nblk); break;
+    case EX_COMPR: e = grid_launch<EX_COMPR, 7>(a, st, &nblk); break;
     case EX_HEAVI:
       switch (a.nin) {
         case 1: e = grid_launch<EX_HEAVI, 1>(a, st, &nblk); break;

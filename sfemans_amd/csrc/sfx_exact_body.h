@@ -29,7 +29,7 @@ constexpr int EXACT_NJ_ALIGN = 32;   // c2r table rows padded to a multiple of e
 constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
-               EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9 };
+               EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -39,7 +39,7 @@ constexpr int exact_jc(int op, int nin) {
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
   return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0
-         : op == EX_ENTRO ? 4 : op == EX_GRADMOM ? 9 : 1;
+         : (op == EX_ENTRO || op == EX_COMPR) ? 4 : op == EX_GRADMOM ? 9 : 1;
 }
 
 // Fourier field: cos coefficient of mode m at point p at base[p*sp + m*sm],
@@ -75,7 +75,8 @@ struct ExactArgs {
   double smin, smax;    // EX_SVCLAMP: bounds of the scalar
   const double* creal;  // EX_ENTRO: real-space field c_in_real(N, np) at [p*N + j]
   int64_t n_creal;
-  double coeff;         // EX_ENTRO: coeff1_in_level; EX_GRADMOM: LES_coeff1_mom
+  double coeff;         // EX_ENTRO, EX_COMPR: coeff1_in_level; EX_GRADMOM: LES_coeff1_mom
+  double coeff2;        // EX_COMPR: level_set_comp_factor
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -96,6 +97,15 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
 #pragma unroll
     for (int k = 0; k < 3; ++k) w[k] = v * u[k];
     w[3] = fmax(0.0, u[3] * (1.0 - u[3]));                        // :4209
+  } else if constexpr (OP == EX_COMPR) {
+    // inputs grad(phi) (0-2), grad(phi_reg) (3-5), phi (6)            :3316-3330
+    const double c2 = fmax(0.0, u[6] * (1.0 - u[6]));
+    const double ng = sqrt(u[3] * u[3] + u[4] * u[4] + u[5] * u[5]) + 1.e-14;
+    const double ve = -a.coeff + x;
+    const double vc = x * a.coeff2 * c2 / ng;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = -ve * u[k] + vc * u[3 + k];
+    w[3] = c2;
   } else if constexpr (OP == EX_GRADMOM) {
     const double v = x - a.coeff;                                 // visco_entro, :3552-3568
 #pragma unroll
@@ -182,7 +192,7 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
       um[f] = E[f][q] - O[f][q];
     }
     double xp = 0.0, xm = 0.0;
-    if constexpr (OP == EX_ENTRO || OP == EX_GRADMOM) {
+    if constexpr (OP == EX_ENTRO || OP == EX_GRADMOM || OP == EX_COMPR) {
       const int64_t N = 2 * a.K + 1;
       SFX_CHECK("exact creal", p * N + (j > 0 ? N - j : 0), a.n_creal);
       xp = a.creal[p * N + j];

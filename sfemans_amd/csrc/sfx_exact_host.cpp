@@ -132,6 +132,15 @@ static int entro_dev(const double* V, const double* c, const double* creal, doub
   return rc ? rc : exact_run("FFT_PAR_ENTRO_VISC_DCL", EX_ENTRO, a, nullptr, st);
 }
 
+static int compr_dev(const double* V1, const double* V2, const double* c, const double* creal, double coeff,
+                     double comp, double* Vout, double* cout, int64_t np, int M, int mp, hipStream_t st) {
+  if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_PAR_COMPR_ENTRO_VISC_DCL: null c_in_real");
+  ExactArgs a;
+  exact_setup_compr(a, V1, V2, c, creal, coeff, comp, Vout, cout, np, M, mp);
+  int rc = exact_prepare("FFT_PAR_COMPR_ENTRO_VISC_DCL", a, EX_COMPR, np, M, mp);
+  return rc ? rc : exact_run("FFT_PAR_COMPR_ENTRO_VISC_DCL", EX_COMPR, a, nullptr, st);
+}
+
 static int gradmom_dev(const double* V1, const double* V2, const double* V3, const double* creal, double coeff,
                        double* out, int64_t np, int M, int mp, hipStream_t st) {
   if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: null c_in_real");
@@ -406,6 +415,50 @@ int sfx_fft_par_entro_visc_dcl(const double* V1_in, const double* c1_in, const d
   t.t1 = now_s();
   if ((rc = entro_dev((const double*)d1, (const double*)d2, (const double*)dr, coeff1_in_level, (double*)d3,
                       (double*)d4, np, m_max_c, m_max_pad, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, d3, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(V_out, d3, n1)) || (rc = from_dev(c_out, d4, n2))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_par_compr_entro_visc_dcl_dev(const double* V1_in, const double* V2_in, const double* c1_in,
+                                         const double* c_in_real, double coeff1_in_level,
+                                         double level_set_comp_factor, double* V_out, double* c_out, int64_t np,
+                                         int32_t m_max_c, int32_t m_max_pad, void* stream) {
+  clear_err();
+  return compr_dev(V1_in, V2_in, c1_in, c_in_real, coeff1_in_level, level_set_comp_factor, V_out, c_out, np,
+                   m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_par_compr_entro_visc_dcl(const double* V1_in, const double* V2_in, const double* c1_in,
+                                     const double* c_in_real, const double* hloc_gauss, double coeff1_in_level,
+                                     double level_set_comp_factor, double* V_out, double* c_out, int64_t np,
+                                     int32_t nb_field1, int32_t nb_field2, int32_t nb_field3, int32_t m_max_c,
+                                     int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, double* temps) {
+  clear_err();
+  (void)hloc_gauss;   // read by the reference but not used by this product
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_PAR_COMPR_ENTRO_VISC_DCL ");
+  if (nb_field1 != 6 || nb_field2 != 6 || nb_field3 != 2)
+    return fail(SFX_ERR_ARG, "error in problem type while calling FFT_PAR_COMPR_ENTRO_VISC_DCL");
+  int rc = check_procs("FFT_PAR_COMPR_ENTRO_VISC_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (m_max_pad < 1 || m_max_pad > 4096)
+    return fail(SFX_ERR_ARG, "FFT_PAR_COMPR_ENTRO_VISC_DCL: m_max_pad=%d", m_max_pad);
+  const size_t n1 = (size_t)np * 6 * m_max_c, n2 = (size_t)np * 2 * m_max_c, nr = (size_t)np * (2 * m_max_pad - 1);
+  void *d1, *d2, *dc, *dr, *d3, *d4;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, V2_in, n1, &d2)) ||
+      (rc = to_dev(SLOT_EX_IN4, c1_in, n2, &dc)) || (rc = to_dev(SLOT_EX_IN3, c_in_real, nr, &dr)) ||
+      (rc = staging(SLOT_EX_OUT, n1 * 8, &d3)) || (rc = staging(SLOT_EX_OUT2, n2 * 8, &d4)))
+    return rc;
+  t.t1 = now_s();
+  if ((rc = compr_dev((const double*)d1, (const double*)d2, (const double*)dc, (const double*)dr, coeff1_in_level,
+                      level_set_comp_factor, (double*)d3, (double*)d4, np, m_max_c, m_max_pad, nullptr)))
     return rc;
   if ((rc = from_dev(nullptr, d3, 0))) return rc;
   t.t2 = now_s();

@@ -106,6 +106,22 @@ inline void exact_setup_entro(ExactArgs& a, const double* V, const double* c, co
   a.coeff = coeff;
 }
 
+// FFT_PAR_COMPR_ENTRO_VISC_DCL: grad(phi) V1, grad(phi_reg) V2 (np, 6, M), phi c (np, 2, M),
+// c_in_real (N, np) -> V_out (np, 6, M), c_out (np, 2, M).
+inline void exact_setup_compr(ExactArgs& a, const double* V1, const double* V2, const double* c, const double* creal,
+                              double coeff, double comp, double* Vout, double* cout, int64_t np, int M,
+                              int m_max_pad) {
+  exact_setup_entro(a, V1, c, creal, coeff, Vout, cout, np, M, m_max_pad);
+  a.nin = 7;
+  a.in[6] = a.in[3];
+  a.n_in[6] = a.n_in[3];
+  for (int k = 0; k < 3; ++k) {
+    a.in[3 + k] = exact_col(V2, np, 6, k);
+    a.n_in[3 + k] = np * 6 * (int64_t)M - 2 * k * np;
+  }
+  a.coeff2 = comp;
+}
+
 // FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: V1, V2, V3 (np, 6, M), c_in_real (N, np) ->
 // out (3, np, 6, M), out(a, :, 2f:2f+1, :) = coefficients of (c_in_real - coeff) V_a,f.
 inline void exact_setup_gradmom(ExactArgs& a, const double* V1, const double* V2, const double* V3,

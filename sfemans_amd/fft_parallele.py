@@ -20,7 +20,7 @@ __all__ = [
     "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "FFT_PAR_DIV_DCL", "FFT_HEAVISIDE_DCL",
     "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "FFT_SCALAR_VECT_NO_OVERSHOOT",
     "FFT_NO_OVERSHOOT_LEVEL_SET", "FFT_PAR_ENTRO_VISC_DCL", "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM",
-    "FFT_COMPUTE_DIFFU_MOM", "fortran_tensor", "fortran_empty",
+    "FFT_COMPUTE_DIFFU_MOM", "FFT_PAR_COMPR_ENTRO_VISC_DCL", "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
 
@@ -325,3 +325,29 @@ def FFT_COMPUTE_DIFFU_MOM(communicator, V1_in, V2_in, V3_in, V1_out, V2_out, nb_
         check(L.sfx_fft_compute_diffu_mom(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(V1_out), ptr(V2_out), np_, nb1,
                                           nb2, nb3, M, nb_procs, bloc_size, m_max_pad, _temps_ptr(temps)))
     return V1_out, V2_out
+
+
+def FFT_PAR_COMPR_ENTRO_VISC_DCL(communicator, V1_in, V2_in, c1_in, c_in_real, hloc_gauss, coeff1_in_level,
+                                 V_out, c_out, level_set_comp_factor, nb_procs=1, bloc_size=None, m_max_pad=None,
+                                 temps=None):
+    """Compressive entropy viscosity of a level set (fft_parallel.F90:3160):
+    V_out(np,6,m), c_out(np,2,m); ``level_set_comp_factor`` is the reference's
+    inputs%level_set_comp_factor, passed explicitly."""
+    _single_rank(communicator)
+    np_, nb1, M = fdims(V1_in)
+    nb2, nb3 = fdims(V2_in)[1], fdims(c1_in)[1]
+    fdims(V_out), fdims(c_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    if tuple(fdims(c_in_real)) != (2 * m_max_pad - 1, np_):
+        raise ValueError("c_in_real must be (2*m_max_pad-1, np)")
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_par_compr_entro_visc_dcl_dev(ptr(V1_in), ptr(V2_in), ptr(c1_in), ptr(c_in_real),
+                                                     float(coeff1_in_level), float(level_set_comp_factor),
+                                                     ptr(V_out), ptr(c_out), np_, M, m_max_pad, stream_ptr()))
+    else:
+        check(L.sfx_fft_par_compr_entro_visc_dcl(ptr(V1_in), ptr(V2_in), ptr(c1_in), ptr(c_in_real), ptr(hloc_gauss),
+                                                 float(coeff1_in_level), float(level_set_comp_factor), ptr(V_out),
+                                                 ptr(c_out), np_, nb1, nb2, nb3, M, nb_procs, bloc_size, m_max_pad,
+                                                 _temps_ptr(temps)))
+    return V_out, c_out

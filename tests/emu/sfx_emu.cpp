@@ -422,6 +422,11 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_GRADMOM:   // in1 = [V1 | V2 | V3] (3 x np*6*M) then c_in_real (N, np); values = {coeff}
       exact_setup_gradmom(a, in1, in1 + 6 * np * M, in1 + 12 * np * M, in1 + 18 * np * M, values[0], out, np, M, mp);
       break;
+    case EX_COMPR:   // in1 = [V1 | V2] (2 x np*6*M), in2 = [c (np,2,M) | c_in_real (N,np)],
+                     // values = {coeff, comp}, out = [V_out | c_out]
+      exact_setup_compr(a, in1, in1 + 6 * np * M, in2, in2 + 2 * np * M, values[0], values[1], out,
+                        out + 6 * np * M, np, M, mp);
+      break;
     default: return 1;
   }
   exact_setup_sizes(a, np, M, mp);
@@ -442,6 +447,7 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_CLAMP01: emu_exact_grid<EX_CLAMP01, 1>(a, vmax); break;
     case EX_ENTRO: emu_exact_grid<EX_ENTRO, 4>(a, vmax); break;
     case EX_GRADMOM: emu_exact_grid<EX_GRADMOM, 9>(a, vmax); break;
+    case EX_COMPR: emu_exact_grid<EX_COMPR, 7>(a, vmax); break;
     case EX_HEAVI:
       if (a.nin == 1) emu_exact_grid<EX_HEAVI, 1>(a, vmax);
       else if (a.nin == 2) emu_exact_grid<EX_HEAVI, 2>(a, vmax);

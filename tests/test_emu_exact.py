@@ -148,3 +148,22 @@ def test_entropy_visc_grad_mom(npt, M, mp):
     packed = np.concatenate([v.ravel(order="F") for v in Vs] + [creal.ravel(order="F")])
     out = emu(EX_GRADMOM, npt, M, mp, packed, values=[0.25], out=np.zeros((3, npt, 6, M), order="F"))
     assert rel(out, F.fft_compute_entropy_visc_grad_mom(*Vs, creal, 0.25, mp)) <= 1e-12
+
+
+EX_COMPR = 10
+
+
+@pytest.mark.parametrize("npt,M,mp", [(7, 8, 12), (3, 32, 48)])
+def test_compression_entropy_viscosity(npt, M, mp):
+    rng = np.random.default_rng(97 + M)
+    V1, V2 = fields(rng, npt, 6, M), fields(rng, npt, 6, M)
+    c = fields(rng, npt, 2, M, 0.3)
+    c[:, 0, 0] = 0.5
+    N = 2 * mp - 1
+    creal = rng.standard_normal((N, npt))
+    in1 = np.concatenate([V1.ravel(order="F"), V2.ravel(order="F")])
+    in2 = np.concatenate([c.ravel(order="F"), creal.ravel(order="F")])
+    out = emu(EX_COMPR, npt, M, mp, in1, in2, values=[0.3, 1.7], out=np.zeros(npt * 8 * M))
+    Vo, co = F.fft_par_compr_entro_visc_dcl(V1, V2, c, creal, 0.3, 1.7, mp)
+    assert rel(out[:npt * 6 * M].reshape((npt, 6, M), order="F"), Vo) <= 1e-12
+    assert rel(out[npt * 6 * M:].reshape((npt, 2, M), order="F"), co) <= 1e-12

@@ -216,6 +216,26 @@ def test_entropy_visc_grad_mom_and_diffu_mom(npt, M):
     assert rel(d1.cpu().numpy(), e1) <= TOL and rel(d2.cpu().numpy(), e2) <= TOL
 
 
+@pytest.mark.parametrize("npt,M", [(500, 8), (300, 64)])
+def test_compression_entropy_viscosity(npt, M):
+    print(f"case compr_entro_visc np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(97 + M)
+    V1, V2 = fields(rng, npt, 6, M), fields(rng, npt, 6, M)
+    c = fields(rng, npt, 2, M, 0.3)
+    c[:, 0, 0] = 0.5
+    mp = 3 * M // 2
+    creal = np.asfortranarray(rng.standard_normal((2 * mp - 1, npt)))
+    Vo, co = F.fft_par_compr_entro_visc_dcl(V1, V2, c, creal, 0.3, 1.7, mp)
+    hv, hc = np.zeros((npt, 6, M), order="F"), np.zeros((npt, 2, M), order="F")
+    FP.FFT_PAR_COMPR_ENTRO_VISC_DCL(None, V1, V2, c, creal, None, 0.3, hv, hc, 1.7)
+    assert rel(hv, Vo) <= TOL and rel(hc, co) <= TOL
+    dv, dc = FP.fortran_empty((npt, 6, M)), FP.fortran_empty((npt, 2, M))
+    FP.FFT_PAR_COMPR_ENTRO_VISC_DCL(None, FP.fortran_tensor(V1), FP.fortran_tensor(V2), FP.fortran_tensor(c),
+                                    FP.fortran_tensor(creal), None, 0.3, dv, dc, 1.7)
+    torch.cuda.synchronize()
+    assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
+
+
 def test_m_max_pad_other_than_three_halves():
     """The grid follows m_max_pad: odd and larger pads give other coefficients."""
     rng = np.random.default_rng(60)

@@ -18,7 +18,8 @@ scaling, truncation) through a closed form that does not use the oracle:
   its grid values by explicit sums) the vector part is the product s V; with
   0 <= c <= 1 the scalar part is c - c*c; it clamps at zero above 1; the
   momentum-gradient form likewise gives s V_a; the diffusion products are
-  scalar-vector products.
+  scalar-vector products; the compression form reduces to the entropy form
+  without compression and has a closed form for constant fields.
 
 The pocketfft engine is also checked against the long-double direct sum.
 """
@@ -306,3 +307,27 @@ def test_diffu_mom_is_two_scalar_vector_products():
     assert rel(o1, F.fft_scalar_vect_dcl(V1, s, 1, mp)) <= 1e-14
     assert rel(o2, F.fft_scalar_vect_dcl(V2, s, 1, mp)) <= 1e-14
     assert rel(o1, F.fft_compute_diffu_mom(V1, V2, s, mp, "direct")[0]) <= 1e-13
+
+
+def test_compression_viscosity_reduces_to_entropy_viscosity_and_constant_case():
+    rng = np.random.default_rng(17)
+    M, mp, npt = 8, 12, 10
+    N = 2 * mp - 1
+    V1, V2 = band_limited(rng, npt, M, 6), band_limited(rng, npt, M, 6)
+    c = band_limited(rng, npt, M) * 0.05
+    c[:, 0, 0] = 0.5
+    cr = rng.standard_normal((N, npt))
+    a = F.fft_par_compr_entro_visc_dcl(V1, V2, c, cr, 0.3, 0.0, mp)
+    b = F.fft_par_entro_visc_dcl(V1, c, cr, 0.3, mp)
+    assert rel(a[0], b[0]) <= 1e-13 and rel(a[1], b[1]) <= 1e-13
+    # constant grad(phi_reg) = g, phi = phi0, viscosity x0: a constant compression term
+    g = rng.standard_normal((npt, 3))
+    G = np.zeros((npt, 6, M)); G[:, 0::2, 0] = g
+    phi0, x0, comp, coeff = 0.3, 0.8, 1.7, 0.2
+    C0 = np.zeros((npt, 2, M)); C0[:, 0, 0] = phi0
+    Vo, co = F.fft_par_compr_entro_visc_dcl(V1, G, C0, np.full((N, npt), x0), coeff, comp, mp)
+    vc = x0 * comp * phi0 * (1 - phi0) / (np.sqrt(np.sum(g ** 2, axis=1)) + 1e-14)
+    ex = (coeff - x0) * V1
+    ex[:, 0::2, 0] += vc[:, None] * g
+    assert rel(Vo, ex) <= 1e-13
+    assert np.allclose(co[:, 0, 0], phi0 * (1 - phi0), atol=1e-15) and np.max(np.abs(co[:, :, 1:])) < 1e-15
