@@ -186,6 +186,26 @@ int sfx_fft_par_compr_entro_visc_dcl_dev(const double* V1_in, const double* V2_i
                                          double level_set_comp_factor, double* V_out, double* c_out, int64_t np,
                                          int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_COMPUTE_ENTROPY_VISC (fft_parallel.F90:2418): V1_in velocity,
+ * V2_in residual, V3_in..V5_in the three rows of a tensor (np,6,m), hloc_gauss(np)
+ * the mesh size per Gauss point, points grouped l_G per element (np a multiple
+ * of l_G, l_G <= 16).  On the grid: |v| maximised over the element's points and
+ * the neighbouring angles (periodic) (:2571-2591), visco = -les_coeff1 +
+ * MIN(les_coeff4 |v|max, les_coeff2 h |v . res|/(residual_normalization + 1e-14))
+ * (:2601-2607); Vk_out = coefficients of visco V(k+2) (:2612-2620).  The
+ * reference also reduces max |v| over the ranks without using it; that is not
+ * computed.  les_coeff1/2/4 are the module inputs it reads (inputs%LES_coeff*). */
+int sfx_fft_compute_entropy_visc(const double* V1_in, const double* V2_in, const double* V3_in, const double* V4_in,
+                                 const double* V5_in, const double* hloc_gauss, double* V1_out, double* V2_out,
+                                 double* V3_out, int64_t np, int32_t nb_field, int32_t m_max_c, int32_t nb_procs,
+                                 int64_t bloc_size, int32_t m_max_pad, double residual_normalization, int32_t l_G,
+                                 double les_coeff1, double les_coeff2, double les_coeff4, double* temps);
+int sfx_fft_compute_entropy_visc_dev(const double* V1_in, const double* V2_in, const double* V3_in,
+                                    const double* V4_in, const double* V5_in, const double* hloc_gauss,
+                                    double* V1_out, double* V2_out, double* V3_out, int64_t np, int32_t m_max_c,
+                                    int32_t m_max_pad, double residual_normalization, int32_t l_G,
+                                    double les_coeff1, double les_coeff2, double les_coeff4, void* stream);
+
 /* Replaces FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM (fft_parallel.F90:3419): V1_in,
  * V2_in, V3_in (np,6,m) the three momentum gradients, c_in_real(2*m_max_pad-1, np)
  * the entropy viscosity on the grid angles, les_coeff1_mom the module input the

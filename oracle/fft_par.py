@@ -23,7 +23,8 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 ``fft_scalar_vect_no_overshoot`` (:1222-1424),
 ``fft_no_overshoot_level_set`` (:3649-3819), ``fft_par_entro_visc_dcl``
 (:4055-4292), ``fft_par_compr_entro_visc_dcl`` (:3160-3417) and
-``fft_compute_entropy_visc_grad_mom`` (:3419-3647);
+``fft_compute_entropy_visc_grad_mom`` (:3419-3647), ``fft_compute_entropy_visc``
+(:2418-2699);
 ``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
@@ -43,6 +44,7 @@ __all__ = [
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
     "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom", "fft_par_compr_entro_visc_dcl",
+    "fft_compute_entropy_visc",
     "regul", "OPS",
 ]
 
@@ -455,3 +457,38 @@ def fft_par_compr_entro_visc_dcl(V1_in, V2_in, c1_in, c_in_real, coeff1_in_level
         outs_v.append(unpack(np.asarray(r2c(pv, m_max_pad, engine), dtype=np.complex128), N, M))
         outs_c.append(unpack(np.asarray(r2c(c2[:, None, :], m_max_pad, engine), dtype=np.complex128), N, M))
     return np.concatenate(outs_v, axis=0), np.concatenate(outs_c, axis=0)
+
+
+def fft_compute_entropy_visc(V1_in, V2_in, V3_in, V4_in, V5_in, hloc_gauss, residual_normalization, l_G,
+                             les_coeff1, les_coeff2, les_coeff4, m_max_pad, engine="fft"):
+    """FFT_COMPUTE_ENTROPY_VISC (fft_parallel.F90:2418-2699), one rank: |v| on
+    the grid (:2571), its maximum over the l_G points of each element
+    (:2572-2577) and over the angles j-1, j, j+1, periodic (:2578-2591);
+    visco = -LES_coeff1 + MIN(LES_coeff4 |v|max, LES_coeff2 h |v . res| /
+    (residual_normalization + 1e-14)) (:2601-2607); outputs the coefficients
+    of visco V3, visco V4, visco V5 (:2612-2620).  Returns three (np, 6, M)."""
+    Vs = [np.asarray(x, dtype=np.float64) for x in (V1_in, V2_in, V3_in, V4_in, V5_in)]
+    h = np.asarray(hloc_gauss, dtype=np.float64)
+    npt, nb, M = Vs[0].shape
+    _check(nb, M, "FFT_COMPUTE_ENTROPY_VISC")
+    if npt % l_G:
+        raise ValueError("np must be a multiple of l_G")
+    N = n_grid(m_max_pad)
+    chunk = l_G * max(1, _CHUNK // l_G)
+    outs = [[], [], []]
+    for s0 in range(0, npt, chunk):
+        sl = slice(s0, s0 + chunk)
+        ru = [np.asarray(c2r(pack(V[sl], m_max_pad), N, engine), dtype=np.float64) for V in Vs]   # (n, 3, N)
+        n = ru[0].shape[0]
+        nv = np.sqrt(np.sum(ru[0] ** 2, axis=1))                                                  # (n, N)
+        ne = nv.reshape(n // l_G, l_G, N).max(axis=1)                                             # (ne, N)
+        if N >= 3:
+            ne = np.maximum(np.maximum(np.roll(ne, 1, axis=1), ne), np.roll(ne, -1, axis=1))
+        sm = np.repeat(ne, l_G, axis=0)                                                           # (n, N)
+        dot = np.abs(np.sum(ru[0] * ru[1], axis=1))
+        vis = -les_coeff1 + np.minimum(les_coeff4 * sm,
+                                       les_coeff2 * h[sl][:, None] * dot / (residual_normalization + 1e-14))
+        for k in range(3):
+            pc = r2c(vis[:, None, :] * ru[2 + k], m_max_pad, engine)
+            outs[k].append(unpack(np.asarray(pc, dtype=np.complex128), N, M))
+    return tuple(np.concatenate(o, axis=0) for o in outs)

@@ -66,6 +66,10 @@ SIGNATURES = {
     "sfx_fft_par_compr_entro_visc_dcl": [_vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _i64, _i32, _i32, _i32, _i32,
                                          _i32, _i64, _i32, _vp],
     "sfx_fft_par_compr_entro_visc_dcl_dev": [_vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_compute_entropy_visc": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64,
+                                     _i32, _d, _i32, _d, _d, _d, _vp],
+    "sfx_fft_compute_entropy_visc_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _d, _i32,
+                                         _d, _d, _d, _vp],
     "sfx_fft_compute_diffu_mom": [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i64, _i32, _vp],
     "sfx_fft_compute_diffu_mom_dev": [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_no_overshoot_level_set_dev": [_vp, _i64, _i32, _i32, _i32, _vp],

@@ -36,6 +36,42 @@ __global__ void __launch_bounds__(EXACT_THREADS) exact_r2c_kernel(ExactArgs a, i
   exact_r2c_body(a, (L / per) * EXACT_THREADS + threadIdx.x, r / nmc, r % nmc);
 }
 
+// Per element and angle, the maximum of |v| over the element's lg points
+// (:2572-2577): a work-group holds EXACT_ENORM_E consecutive elements and one
+// block of 16 angle pairs; every thread computes |v| of its point at its
+// angles into LDS, then the element maxima are taken from there.
+constexpr int EXACT_ENORM_E = 32;
+__global__ void __launch_bounds__(512) exact_enorm_kernel(ExactArgs a, int nch, int64_t me) {
+  extern __shared__ double red[];   // [thread][2 * EXACT_NORM_JC + 1]
+  constexpr int RW = 2 * EXACT_NORM_JC + 1;
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t eb = L / nch;
+  const int jc = (int)(L % nch), j0 = jc * EXACT_NORM_JC;
+  const int lg = a.lg, t = threadIdx.x;
+  const int64_t p = eb * EXACT_ENORM_E * lg + t;
+  double nrm[2 * EXACT_NORM_JC];
+  if (p < a.np) {
+    exact_norm_body(a, p, j0, nrm);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 2 * EXACT_NORM_JC; ++v) nrm[v] = 0.0;
+  }
+#pragma unroll
+  for (int v = 0; v < 2 * EXACT_NORM_JC; ++v) red[t * RW + v] = nrm[v];
+  __syncthreads();
+  const int N = 2 * a.K + 1;
+  for (int item = t; item < EXACT_ENORM_E * 2 * EXACT_NORM_JC; item += blockDim.x) {
+    const int le = item / (2 * EXACT_NORM_JC), v = item % (2 * EXACT_NORM_JC);
+    const int64_t e = eb * EXACT_ENORM_E + le;
+    const int q = v % EXACT_NORM_JC, j = j0 + q;
+    const bool mirror = v >= EXACT_NORM_JC;
+    if (e >= me || j > a.K || (mirror && j == 0)) continue;
+    double x = red[(le * lg) * RW + v];
+    for (int l = 1; l < lg; ++l) x = fmax(x, red[(le * lg + l) * RW + v]);
+    a.nint[e * N + (mirror ? N - j : j)] = x;
+  }
+}
+
 // Maximum of n values into out[0] (one work-group).
 __global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* v, int64_t n, double* out) {
   __shared__ double red[EXACT_THREADS];
@@ -81,6 +117,22 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
     case EX_ENTRO: e = grid_launch<EX_ENTRO, 4>(a, st, &nblk); break;
     case EX_GRADMOM: e = grid_launch<EX_GRADMOM, 9>(a, st, &nblk); break;
     case EX_COMPR: e = grid_launch<EX_COMPR, 7>(a, st, &nblk); break;
+    case EX_EVISC: {
+      // element maxima of |v| first, then the products
+      const int64_t me = a.np / a.lg;
+      const int nch = (a.K + 1 + EXACT_NORM_JC - 1) / EXACT_NORM_JC;
+      const int64_t nb = ((me + EXACT_ENORM_E - 1) / EXACT_ENORM_E) * nch;
+      const int threads = EXACT_ENORM_E * a.lg;
+      if (a.lg < 1 || a.lg > 16 || me * a.lg != a.np || nb > 0x7fffffffLL) return hipErrorInvalidValue;
+      if (nb > 0) {
+        const size_t shmem = (size_t)threads * (2 * EXACT_NORM_JC + 1) * sizeof(double);
+        hipLaunchKernelGGL(exact_enorm_kernel, dim3((unsigned)nb), dim3(threads), shmem, st, a, nch, me);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+      }
+      e = grid_launch<EX_EVISC, 15>(a, st, &nblk);
+      break;
+    }
     case EX_HEAVI:
       switch (a.nin) {
         case 1: e = grid_launch<EX_HEAVI, 1>(a, st, &nblk); break;

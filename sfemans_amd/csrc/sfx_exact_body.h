@@ -24,22 +24,23 @@
 
 namespace sfx {
 
-constexpr int EXACT_MAX_IN = 9, EXACT_MAX_OUT = 9, EXACT_MAX_INT = 4;
+constexpr int EXACT_MAX_IN = 15, EXACT_MAX_OUT = 9, EXACT_MAX_INT = 4;
 constexpr int EXACT_NJ_ALIGN = 32;   // c2r table rows padded to a multiple of every JC
 constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
-               EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10 };
+               EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10,
+               EX_EVISC = 11 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
 constexpr int exact_jc(int op, int nin) {
-  return op == EX_SQRT ? 32 : nin <= 2 ? 16 : nin <= 6 ? 8 : 4;
+  return op == EX_SQRT ? 32 : nin <= 2 ? 16 : nin <= 6 ? 8 : nin <= 9 ? 4 : 2;
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
   return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0
-         : (op == EX_ENTRO || op == EX_COMPR) ? 4 : op == EX_GRADMOM ? 9 : 1;
+         : (op == EX_ENTRO || op == EX_COMPR) ? 4 : (op == EX_GRADMOM || op == EX_EVISC) ? 9 : 1;
 }
 
 // Fourier field: cos coefficient of mode m at point p at base[p*sp + m*sm],
@@ -76,7 +77,12 @@ struct ExactArgs {
   const double* creal;  // EX_ENTRO: real-space field c_in_real(N, np) at [p*N + j]
   int64_t n_creal;
   double coeff;         // EX_ENTRO, EX_COMPR: coeff1_in_level; EX_GRADMOM: LES_coeff1_mom
-  double coeff2;        // EX_COMPR: level_set_comp_factor
+  double coeff2;        // EX_COMPR: level_set_comp_factor; EX_EVISC: LES_coeff2
+  double coeff3, coeff4;  // EX_EVISC: LES_coeff4, residual_normalization
+  const double* hloc;   // EX_EVISC: mesh size per point (np)
+  double* nint;         // EX_EVISC: per element and angle, the maximum of |v| over its points, [e*N + j]
+  int lg;               // EX_EVISC: points per element
+  int64_t n_nint;
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -91,8 +97,16 @@ SFX_DEV double exact_regul(double phi, double eps) {
 // The pointwise operation at one angle: u (NI fields) -> w (NO fields); x is
 // the real-space input at that angle (EX_ENTRO only).
 template <int OP, int NI, int NO>
-SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO > 0 ? NO : 1], double x) {
-  if constexpr (OP == EX_ENTRO) {
+SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO > 0 ? NO : 1], double x,
+                      double y) {
+  if constexpr (OP == EX_EVISC) {
+    // x: |v| maximised over the element and the neighbouring angles, y: the
+    // point's mesh size; v (0-2), residual (3-5), three tensor rows (6-14)   :2601-2620
+    const double dot = u[0] * u[3] + u[1] * u[4] + u[2] * u[5];
+    const double ve = -a.coeff + fmin(a.coeff3 * x, a.coeff2 * y * fabs(dot) / (a.coeff4 + 1.e-14));
+#pragma unroll
+    for (int k = 0; k < 9; ++k) w[k] = ve * u[6 + k];
+  } else if constexpr (OP == EX_ENTRO) {
     const double v = a.coeff - x;                                 // -visc_entro, :4212-4219
 #pragma unroll
     for (int k = 0; k < 3; ++k) w[k] = v * u[k];
@@ -148,11 +162,9 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
 
 // c2r of the NI input fields on angles j0..j0+JC-1 (and their mirrors), the
 // operation, S and D of the NO outputs (or the running maximum for EX_MAXVEL).
-template <int OP, int NI, int NO = exact_nout(OP), int JC = exact_jc(OP, NI)>
-SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax) {
-  if (p >= a.np) return;
-  const int j0 = jc * JC;
-  double E[NI][JC], O[NI][JC];
+// E and O of the NI input fields at point p on angles j0..j0+JC-1.
+template <int NI, int JC>
+SFX_DEV void exact_c2r(const ExactArgs& a, int64_t p, int j0, double (&E)[NI][JC], double (&O)[NI][JC]) {
 #pragma unroll
   for (int f = 0; f < NI; ++f)
 #pragma unroll
@@ -180,6 +192,25 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
       }
     }
   }
+}
+
+// Maximum of the element's values at angles j-1, j, j+1 (periodic) of the
+// per-element maxima (:2572-2591; one angle: the value itself).
+SFX_DEV double exact_evisc_norm(const ExactArgs& a, int64_t e, int j) {
+  const int N = 2 * a.K + 1;
+  const double* r = a.nint + e * N;
+  SFX_CHECK("exact nint", e * N + N - 1, a.n_nint);
+  if (N < 3) return r[j];
+  const int jm = j == 0 ? N - 1 : j - 1, jp = j == N - 1 ? 0 : j + 1;
+  return fmax(fmax(r[jm], r[j]), r[jp]);
+}
+
+template <int OP, int NI, int NO = exact_nout(OP), int JC = exact_jc(OP, NI)>
+SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax) {
+  if (p >= a.np) return;
+  const int j0 = jc * JC;
+  double E[NI][JC], O[NI][JC];
+  exact_c2r<NI, JC>(a, p, j0, E, O);
   const int64_t plane = (int64_t)(a.K + 1) * a.np;
 #pragma unroll
   for (int q = 0; q < JC; ++q) {
@@ -191,16 +222,21 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
       up[f] = E[f][q] + O[f][q];
       um[f] = E[f][q] - O[f][q];
     }
-    double xp = 0.0, xm = 0.0;
+    double xp = 0.0, xm = 0.0, y = 0.0;
     if constexpr (OP == EX_ENTRO || OP == EX_GRADMOM || OP == EX_COMPR) {
       const int64_t N = 2 * a.K + 1;
       SFX_CHECK("exact creal", p * N + (j > 0 ? N - j : 0), a.n_creal);
       xp = a.creal[p * N + j];
       xm = j > 0 ? a.creal[p * N + (N - j)] : 0.0;
+    } else if constexpr (OP == EX_EVISC) {
+      const int64_t e = p / a.lg;
+      xp = exact_evisc_norm(a, e, j);
+      xm = j > 0 ? exact_evisc_norm(a, e, 2 * a.K + 1 - j) : 0.0;
+      y = a.hloc[p];
     }
     double wp[NO > 0 ? NO : 1], wm[NO > 0 ? NO : 1];
-    exact_op<OP, NI, NO>(a, up, wp, xp);
-    if (j > 0) exact_op<OP, NI, NO>(a, um, wm, xm);
+    exact_op<OP, NI, NO>(a, up, wp, xp, y);
+    if (j > 0) exact_op<OP, NI, NO>(a, um, wm, xm, y);
     if constexpr (OP == EX_MAXVEL) {
       vmax = fmax(vmax, wp[0]);
       if (j > 0) vmax = fmax(vmax, wm[0]);
@@ -213,6 +249,27 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
         a.W[o + plane] = j > 0 ? wp[g] - wm[g] : 0.0;
       }
     }
+  }
+}
+
+// |v| of the first three input fields at point p, angles j0..j0+15 and their
+// mirrors: nrm[q] at t_j, nrm[16 + q] at t_{N-j} (0 for j = 0 or j > K).
+constexpr int EXACT_NORM_JC = 16;
+SFX_DEV void exact_norm_body(const ExactArgs& a, int64_t p, int j0, double (&nrm)[2 * EXACT_NORM_JC]) {
+  double E[3][EXACT_NORM_JC], O[3][EXACT_NORM_JC];
+  exact_c2r<3, EXACT_NORM_JC>(a, p, j0, E, O);
+#pragma unroll
+  for (int q = 0; q < EXACT_NORM_JC; ++q) {
+    const int j = j0 + q;
+    double sp = 0.0, sm = 0.0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double up = E[f][q] + O[f][q], um = E[f][q] - O[f][q];
+      sp = fma(up, up, sp);
+      sm = fma(um, um, sm);
+    }
+    nrm[q] = j <= a.K ? sqrt(sp) : 0.0;                              // :2571
+    nrm[EXACT_NORM_JC + q] = (j > 0 && j <= a.K) ? sqrt(sm) : 0.0;
   }
 }
 

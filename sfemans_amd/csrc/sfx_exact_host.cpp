@@ -43,7 +43,7 @@ static int exact_tables(int M, int K, const double** ctab, const double** rtab) 
 
 // Staging slots of the exact-grid products (the FFT products use 0-2).
 enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX_MAX = 7, SLOT_EX_IN3 = 8,
-       SLOT_EX_OUT2 = 9, SLOT_EX_IN4 = 10 };
+       SLOT_EX_OUT2 = 9, SLOT_EX_IN4 = 10, SLOT_EX_NINT = 11 };
 
 // Common set-up: sizes, tables, scratch.  `a.in`, `a.out`, nin, nout filled by the caller.
 static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad) {
@@ -139,6 +139,24 @@ static int compr_dev(const double* V1, const double* V2, const double* c, const 
   exact_setup_compr(a, V1, V2, c, creal, coeff, comp, Vout, cout, np, M, mp);
   int rc = exact_prepare("FFT_PAR_COMPR_ENTRO_VISC_DCL", a, EX_COMPR, np, M, mp);
   return rc ? rc : exact_run("FFT_PAR_COMPR_ENTRO_VISC_DCL", EX_COMPR, a, nullptr, st);
+}
+
+static int evisc_dev(const double* const (&V)[5], const double* hloc, double* const (&O)[3], int64_t np, int M,
+                     int mp, int lg, double c1, double c2, double c4, double res, hipStream_t st) {
+  const char* nm = "FFT_COMPUTE_ENTROPY_VISC";
+  if (lg < 1 || lg > 16) return fail(SFX_ERR_UNSUPPORTED, "%s: l_G=%d (1..16)", nm, lg);
+  if (np % lg != 0) return fail(SFX_ERR_ARG, "%s: np=%lld is not a multiple of l_G=%d", nm, (long long)np, lg);
+  if (!hloc && np > 0) return fail(SFX_ERR_ARG, "%s: null hloc_gauss", nm);
+  if (mp < 1 || mp > 4096) return fail(SFX_ERR_ARG, "%s: m_max_pad=%d", nm, mp);
+  ExactArgs a;
+  exact_setup_evisc(a, V, hloc, O, np, M, lg, c1, c2, c4, res);
+  a.n_nint = (np / lg) * (2 * (int64_t)mp - 1);
+  void* w = nullptr;
+  int rc = staging(SLOT_EX_NINT, (size_t)(a.n_nint > 0 ? a.n_nint : 1) * 8, &w);
+  if (rc) return rc;
+  a.nint = (double*)w;
+  if ((rc = exact_prepare(nm, a, EX_EVISC, np, M, mp))) return rc;
+  return exact_run(nm, EX_EVISC, a, nullptr, st);
 }
 
 static int gradmom_dev(const double* V1, const double* V2, const double* V3, const double* creal, double coeff,
@@ -463,6 +481,55 @@ int sfx_fft_par_compr_entro_visc_dcl(const double* V1_in, const double* V2_in, c
   if ((rc = from_dev(nullptr, d3, 0))) return rc;
   t.t2 = now_s();
   if ((rc = from_dev(V_out, d3, n1)) || (rc = from_dev(c_out, d4, n2))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_compute_entropy_visc_dev(const double* V1_in, const double* V2_in, const double* V3_in,
+                                    const double* V4_in, const double* V5_in, const double* hloc_gauss,
+                                    double* V1_out, double* V2_out, double* V3_out, int64_t np, int32_t m_max_c,
+                                    int32_t m_max_pad, double residual_normalization, int32_t l_G,
+                                    double les_coeff1, double les_coeff2, double les_coeff4, void* stream) {
+  clear_err();
+  const double* V[5] = {V1_in, V2_in, V3_in, V4_in, V5_in};
+  double* O[3] = {V1_out, V2_out, V3_out};
+  return evisc_dev(V, hloc_gauss, O, np, m_max_c, m_max_pad, l_G, les_coeff1, les_coeff2, les_coeff4,
+                   residual_normalization, (hipStream_t)stream);
+}
+
+int sfx_fft_compute_entropy_visc(const double* V1_in, const double* V2_in, const double* V3_in, const double* V4_in,
+                                 const double* V5_in, const double* hloc_gauss, double* V1_out, double* V2_out,
+                                 double* V3_out, int64_t np, int32_t nb_field, int32_t m_max_c, int32_t nb_procs,
+                                 int64_t bloc_size, int32_t m_max_pad, double residual_normalization, int32_t l_G,
+                                 double les_coeff1, double les_coeff2, double les_coeff4, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field != 6 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_COMPUTE_ENTROPY_VISC ");
+  int rc = check_procs("FFT_COMPUTE_ENTROPY_VISC", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  const size_t n1 = (size_t)np * 6 * m_max_c;
+  void* d[5];
+  void *dh, *o1, *o2, *o3;
+  HostRun t;
+  t.t0 = now_s();
+  const double* src[5] = {V1_in, V2_in, V3_in, V4_in, V5_in};
+  const int slots[5] = {SLOT_EX_IN1, SLOT_EX_IN2, SLOT_EX_IN3, SLOT_EX_IN4, 12};
+  for (int i = 0; i < 5; ++i)
+    if ((rc = to_dev(slots[i], src[i], n1, &d[i]))) return rc;
+  if ((rc = to_dev(13, hloc_gauss, (size_t)np, &dh)) || (rc = staging(SLOT_EX_OUT, n1 * 8, &o1)) ||
+      (rc = staging(SLOT_EX_OUT2, n1 * 8, &o2)) || (rc = staging(14, n1 * 8, &o3)))
+    return rc;
+  t.t1 = now_s();
+  const double* V[5] = {(const double*)d[0], (const double*)d[1], (const double*)d[2], (const double*)d[3],
+                        (const double*)d[4]};
+  double* O[3] = {(double*)o1, (double*)o2, (double*)o3};
+  if ((rc = evisc_dev(V, (const double*)dh, O, np, m_max_c, m_max_pad, l_G, les_coeff1, les_coeff2, les_coeff4,
+                      residual_normalization, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, o1, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(V1_out, o1, n1)) || (rc = from_dev(V2_out, o2, n1)) || (rc = from_dev(V3_out, o3, n1))) return rc;
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;

@@ -141,6 +141,28 @@ inline void exact_setup_gradmom(ExactArgs& a, const double* V1, const double* V2
   a.coeff = coeff;
 }
 
+// FFT_COMPUTE_ENTROPY_VISC: velocity V1, residual V2, tensor rows V3, V4, V5
+// (np, 6, M), hloc (np), lg points per element -> O1, O2, O3 (np, 6, M).  The
+// per-element maxima array (me * N doubles) is set by the caller in a.nint.
+inline void exact_setup_evisc(ExactArgs& a, const double* const (&V)[5], const double* hloc, double* const (&O)[3],
+                              int64_t np, int M, int lg, double c1, double c2, double c4, double res) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 15; a.nout = 9;
+  for (int v = 0; v < 5; ++v)
+    for (int k = 0; k < 3; ++k) {
+      a.in[3 * v + k] = exact_col(V[v], np, 6, k);
+      a.n_in[3 * v + k] = np * 6 * (int64_t)M - 2 * k * np;
+    }
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) {
+      a.out[3 * r + k] = exact_colo(O[r], np, 6, k);
+      a.n_out[3 * r + k] = np * 6 * (int64_t)M - 2 * k * np;
+    }
+  a.hloc = hloc;
+  a.lg = lg;
+  a.coeff = c1; a.coeff2 = c2; a.coeff3 = c4; a.coeff4 = res;
+}
+
 // FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
 // -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {

@@ -20,7 +20,8 @@ __all__ = [
     "FFT_SCALAR_VECT_DCL", "FFT_TENSOR_DCL", "FFT_PAR_DIV_DCL", "FFT_HEAVISIDE_DCL",
     "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "FFT_SCALAR_VECT_NO_OVERSHOOT",
     "FFT_NO_OVERSHOOT_LEVEL_SET", "FFT_PAR_ENTRO_VISC_DCL", "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM",
-    "FFT_COMPUTE_DIFFU_MOM", "FFT_PAR_COMPR_ENTRO_VISC_DCL", "fortran_tensor", "fortran_empty",
+    "FFT_COMPUTE_DIFFU_MOM", "FFT_PAR_COMPR_ENTRO_VISC_DCL", "FFT_COMPUTE_ENTROPY_VISC",
+    "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
 
@@ -351,3 +352,29 @@ def FFT_PAR_COMPR_ENTRO_VISC_DCL(communicator, V1_in, V2_in, c1_in, c_in_real, h
                                                  ptr(c_out), np_, nb1, nb2, nb3, M, nb_procs, bloc_size, m_max_pad,
                                                  _temps_ptr(temps)))
     return V_out, c_out
+
+
+def FFT_COMPUTE_ENTROPY_VISC(communicator, communicator_S, V1_in, V2_in, V3_in, V4_in, V5_in, hloc_gauss,
+                             V1_out, V2_out, V3_out, residual_normalization, l_G, les_coeff1, les_coeff2,
+                             les_coeff4, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """Entropy viscosity times three tensor rows (fft_parallel.F90:2418); the
+    LES coefficients are the reference's inputs%LES_coeff1/2/4, passed
+    explicitly.  Points are grouped l_G per element."""
+    _single_rank(communicator)
+    _single_rank(communicator_S)
+    np_, nb, M = fdims(V1_in)
+    for x in (V2_in, V3_in, V4_in, V5_in, V1_out, V2_out, V3_out):
+        fdims(x)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    args = (float(residual_normalization), int(l_G), float(les_coeff1), float(les_coeff2), float(les_coeff4))
+    if is_device(V1_in):
+        check(L.sfx_fft_compute_entropy_visc_dev(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(V4_in), ptr(V5_in),
+                                                 ptr(hloc_gauss), ptr(V1_out), ptr(V2_out), ptr(V3_out), np_, M,
+                                                 m_max_pad, *args, stream_ptr()))
+    else:
+        check(L.sfx_fft_compute_entropy_visc(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(V4_in), ptr(V5_in),
+                                             ptr(np.ascontiguousarray(hloc_gauss, dtype=np.float64)), ptr(V1_out),
+                                             ptr(V2_out), ptr(V3_out), np_, nb, M, nb_procs, bloc_size, m_max_pad,
+                                             *args, _temps_ptr(temps)))
+    return V1_out, V2_out, V3_out

@@ -391,6 +391,65 @@ extern "C" int emu_batch(int n, const int32_t* jj, const int64_t* jj_off, const 
 // ---- exact-grid products (non-polynomial FFT_PAR_*)
 #include "../../sfemans_amd/csrc/sfx_exact_setup.h"
 
+// FFT_COMPUTE_ENTROPY_VISC: V = [V1 .. V5] (5 x np*6*M), out = [O1 | O2 | O3];
+// the element maxima as exact_enorm_kernel computes them (32 elements and 16
+// angle pairs per work-group, one thread per point).
+extern "C" int emu_exact_evisc(int64_t np, int M, int mp, int lg, const double* V, const double* hloc, double c1,
+                               double c2, double c4, double res, double* out) {
+  const int64_t n6 = np * 6 * M;
+  const double* Vp[5] = {V, V + n6, V + 2 * n6, V + 3 * n6, V + 4 * n6};
+  double* O[3] = {out, out + n6, out + 2 * n6};
+  ExactArgs a;
+  exact_setup_evisc(a, Vp, hloc, O, np, M, lg, c1, c2, c4, res);
+  exact_setup_sizes(a, np, M, mp);
+  std::vector<double> tab;
+  exact_host_tables(M, a.K, tab);
+  a.ctab = tab.data();
+  a.rtab = tab.data() + exact_ctab_size(M, a.K);
+  const int N = 2 * a.K + 1;
+  const int64_t me = np / lg;
+  std::vector<double> nint((size_t)me * N + 1, -1.0);
+  a.nint = nint.data();
+  a.n_nint = me * N;
+  constexpr int RW = 2 * EXACT_NORM_JC + 1;
+  const int nch = (a.K + 1 + EXACT_NORM_JC - 1) / EXACT_NORM_JC;
+  std::vector<double> red((size_t)32 * lg * RW);
+  for (int64_t eb = 0; eb < (me + 31) / 32; ++eb)
+    for (int jc = 0; jc < nch; ++jc) {
+      const int j0 = jc * EXACT_NORM_JC;
+      for (int t = 0; t < 32 * lg; ++t) {
+        const int64_t p = eb * 32 * lg + t;
+        double nrm[2 * EXACT_NORM_JC] = {0};
+        if (p < np) exact_norm_body(a, p, j0, nrm);
+        for (int v = 0; v < 2 * EXACT_NORM_JC; ++v) red[(size_t)t * RW + v] = nrm[v];
+      }
+      for (int le = 0; le < 32; ++le)
+        for (int v = 0; v < 2 * EXACT_NORM_JC; ++v) {
+          const int64_t e = eb * 32 + le;
+          const int j = j0 + v % EXACT_NORM_JC;
+          const bool mirror = v >= EXACT_NORM_JC;
+          if (e >= me || j > a.K || (mirror && j == 0)) continue;
+          double x = red[(size_t)(le * lg) * RW + v];
+          for (int l = 1; l < lg; ++l) x = std::max(x, red[(size_t)(le * lg + l) * RW + v]);
+          nint[(size_t)e * N + (mirror ? N - j : j)] = x;
+        }
+    }
+  for (double x : std::vector<double>(nint.begin(), nint.begin() + me * N))
+    if (x < 0) return 3;   // every (element, angle) written
+  a.n_W = 2 * (int64_t)a.nout * (a.K + 1) * np;
+  std::vector<double> W((size_t)a.n_W + 1);
+  a.W = W.data();
+  double vmax = 0.0;
+  constexpr int JC = exact_jc(EX_EVISC, 15);
+  for (int64_t p = 0; p < np; ++p)
+    for (int jc = 0; jc < (a.K + 1 + JC - 1) / JC; ++jc) exact_grid_body<EX_EVISC, 15>(a, p, jc, vmax);
+  const int nmc = (M + EXACT_MC - 1) / EXACT_MC;
+  for (int64_t p = 0; p < np; ++p)
+    for (int g = 0; g < a.nout; ++g)
+      for (int mc = 0; mc < nmc; ++mc) exact_r2c_body(a, p, g, mc);
+  return 0;
+}
+
 template <int OP, int NI>
 static void emu_exact_grid(const ExactArgs& a, double& vmax) {
   constexpr int JC = exact_jc(OP, NI);

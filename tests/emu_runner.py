@@ -51,6 +51,8 @@ def _load():
     L.emu_batch.restype = C.c_int
     L.emu_exact.argtypes = [i32, i64, i32, i32, vp, vp, vp, i32, C.c_double, i32, vp]
     L.emu_exact.restype = C.c_int
+    L.emu_exact_evisc.argtypes = [i64, i32, i32, i32, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, vp]
+    L.emu_exact_evisc.restype = C.c_int
     return L
 
 

@@ -167,3 +167,20 @@ def test_compression_entropy_viscosity(npt, M, mp):
     Vo, co = F.fft_par_compr_entro_visc_dcl(V1, V2, c, creal, 0.3, 1.7, mp)
     assert rel(out[:npt * 6 * M].reshape((npt, 6, M), order="F"), Vo) <= 1e-12
     assert rel(out[npt * 6 * M:].reshape((npt, 2, M), order="F"), co) <= 1e-12
+
+
+@pytest.mark.parametrize("ne,M,mp,lg", [(5, 8, 12, 7), (40, 4, 6, 7), (3, 32, 48, 3)])
+def test_entropy_visc(ne, M, mp, lg):
+    rng = np.random.default_rng(99 + M)
+    npt = ne * lg
+    Vs = [fields(rng, npt, 6, M) for _ in range(5)]
+    h = rng.uniform(0.1, 1.0, npt)
+    packed = np.concatenate([v.ravel(order="F") for v in Vs])
+    rc, arrs = emu_runner.call("emu_exact_evisc", npt, M, mp, lg, packed, h, 0.3, 0.9, 0.8, 1.7,
+                               np.zeros(3 * npt * 6 * M))
+    assert rc == 0
+    out = arrs[-1]
+    exp = F.fft_compute_entropy_visc(*Vs, h, 1.7, lg, 0.3, 0.9, 0.8, mp)
+    for k in range(3):
+        got = out[k * npt * 6 * M:(k + 1) * npt * 6 * M].reshape((npt, 6, M), order="F")
+        assert rel(got, exp[k]) <= 1e-12

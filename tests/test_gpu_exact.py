@@ -236,6 +236,26 @@ def test_compression_entropy_viscosity(npt, M):
     assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
 
 
+@pytest.mark.parametrize("ne,M", [(300, 8), (40, 64), (7000, 4)])
+def test_entropy_visc(ne, M):
+    print(f"case entropy_visc elements={ne} M={M}", flush=True)
+    rng = np.random.default_rng(99 + M)
+    npt = 7 * ne
+    Vs = [fields(rng, npt, 6, M) for _ in range(5)]
+    h = rng.uniform(0.1, 1.0, npt)
+    exp = F.fft_compute_entropy_visc(*Vs, h, 1.7, 7, 0.3, 0.9, 0.8, 3 * M // 2)
+    outs = [np.zeros((npt, 6, M), order="F") for _ in range(3)]
+    FP.FFT_COMPUTE_ENTROPY_VISC(None, None, *Vs, h, *outs, 1.7, 7, 0.3, 0.9, 0.8)
+    for k in range(3):
+        assert rel(outs[k], exp[k]) <= TOL
+    douts = [FP.fortran_empty((npt, 6, M)) for _ in range(3)]
+    FP.FFT_COMPUTE_ENTROPY_VISC(None, None, *[FP.fortran_tensor(v) for v in Vs],
+                                torch.as_tensor(h, device="cuda"), *douts, 1.7, 7, 0.3, 0.9, 0.8)
+    torch.cuda.synchronize()
+    for k in range(3):
+        assert rel(douts[k].cpu().numpy(), exp[k]) <= TOL
+
+
 def test_m_max_pad_other_than_three_halves():
     """The grid follows m_max_pad: odd and larger pads give other coefficients."""
     rng = np.random.default_rng(60)

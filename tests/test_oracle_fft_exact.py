@@ -19,7 +19,9 @@ scaling, truncation) through a closed form that does not use the oracle:
   0 <= c <= 1 the scalar part is c - c*c; it clamps at zero above 1; the
   momentum-gradient form likewise gives s V_a; the diffusion products are
   scalar-vector products; the compression form reduces to the entropy form
-  without compression and has a closed form for constant fields.
+  without compression and has a closed form for constant fields; the full
+  entropy viscosity has closed forms with either bound of its MIN active
+  (the element and angle maxima through direct cosine sums).
 
 The pocketfft engine is also checked against the long-double direct sum.
 """
@@ -331,3 +333,52 @@ def test_compression_viscosity_reduces_to_entropy_viscosity_and_constant_case():
     ex[:, 0::2, 0] += vc[:, None] * g
     assert rel(Vo, ex) <= 1e-13
     assert np.allclose(co[:, 0, 0], phi0 * (1 - phi0), atol=1e-15) and np.max(np.abs(co[:, :, 1:])) < 1e-15
+
+
+def test_entropy_visc_with_the_velocity_bound_active():
+    """LES_coeff2 huge: visco = -c1 + c4 max|v|, the maximum over the element's
+    points and the angles j-1, j, j+1.  With v = (a cos t, 0, 0) the same at
+    every point of an element, max|v|(t_j) = |a| max(|cos t_{j-1}|, |cos t_j|,
+    |cos t_{j+1}|); with a constant V3 the output is the direct cosine sum of
+    visco(t_j) V3."""
+    M, mp, lg, ne = 6, 9, 7, 3
+    N = 2 * mp - 1
+    npt = lg * ne
+    rng = np.random.default_rng(18)
+    amp = rng.uniform(0.5, 2.0, ne)
+    V1 = np.zeros((npt, 6, M)); V1[:, 0, 1] = np.repeat(amp, lg)
+    V2 = rng.standard_normal((npt, 6, M))
+    c3 = rng.standard_normal(npt)
+    V3 = np.zeros((npt, 6, M)); V3[:, 0, 0] = c3
+    Z = np.zeros((npt, 6, M))
+    h = rng.uniform(0.1, 1.0, npt)
+    c1, c4 = 0.3, 0.8
+    o1, o2, o3 = F.fft_compute_entropy_visc(V1, V2, V3, Z, Z, h, 1.0, lg, c1, 1e30, c4, mp)
+    t = 2 * np.pi * np.arange(N) / N
+    ac = np.abs(np.cos(t))
+    mx = np.maximum(np.maximum(np.roll(ac, 1), ac), np.roll(ac, -1))
+    vis = -c1 + c4 * np.repeat(amp, lg)[:, None] * mx[None, :]                # (np, N)
+    m = np.arange(M)
+    a = (vis * c3[:, None]) @ np.cos(np.outer(t, m)) * (2.0 / N)
+    a[:, 0] /= 2.0
+    b = (vis * c3[:, None]) @ np.sin(np.outer(t, m)) * (2.0 / N)
+    b[:, 0] = 0.0
+    assert np.max(np.abs(o1[:, 0] - a)) <= 1e-13 and np.max(np.abs(o1[:, 1] - b)) <= 1e-13
+    assert np.max(np.abs(o2)) == 0.0 and np.max(np.abs(o3)) == 0.0
+
+
+def test_entropy_visc_with_the_residual_bound_active():
+    """LES_coeff4 huge: visco = -c1 + c2 h |v . res|/(res_norm + 1e-14); with v
+    and res constant in angle the viscosity is constant per point."""
+    M, mp, lg, npt = 5, 8, 7, 14
+    rng = np.random.default_rng(19)
+    v, r = rng.standard_normal((npt, 3)), rng.standard_normal((npt, 3))
+    V1 = np.zeros((npt, 6, M)); V1[:, 0::2, 0] = v
+    V2 = np.zeros((npt, 6, M)); V2[:, 0::2, 0] = r
+    Vs = [band_limited(rng, npt, M, 6) for _ in range(3)]
+    h = rng.uniform(0.1, 1.0, npt)
+    c1, c2, res = 0.1, 0.7, 2.5
+    out = F.fft_compute_entropy_visc(V1, V2, *Vs, h, res, lg, c1, c2, 1e30, mp)
+    vis = -c1 + c2 * h * np.abs(np.sum(v * r, axis=1)) / (res + 1e-14)
+    for k in range(3):
+        assert rel(out[k], vis[:, None, None] * Vs[k]) <= 1e-13
