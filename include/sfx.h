@@ -206,6 +206,25 @@ int sfx_fft_compute_entropy_visc_dev(const double* V1_in, const double* V2_in, c
                                     int32_t m_max_pad, double residual_normalization, int32_t l_G,
                                     double les_coeff1, double les_coeff2, double les_coeff4, void* stream);
 
+/* Replaces FFT_COMPUTE_ENTROPY_VISC_MOM (fft_parallel.F90:2701): velocity V1_in,
+ * momentum V2_in, residual V3_in (np,6,m), mass residual c1_in (np,2,m),
+ * hloc_gauss(np), points grouped l_G per element (np a multiple of l_G <= 16).
+ * On the grid: |u| and |m| maximised over the element's points and the
+ * neighbouring angles (:2839-2870), the raw viscosity les_coeff2 h
+ * MAX(|u . res|, |c |u|^2|) maximised likewise (:2890-2916), vmax the maximum
+ * of |u| (one rank: over all points and angles); outputs on the grid angles,
+ * c1_real_out(2*m_max_pad-1, np) and, if not null, opt_c2_real_out (:2918-2931).
+ * les_coeff2/4 are the module inputs the reference reads. */
+int sfx_fft_compute_entropy_visc_mom(const double* V1_in, const double* V2_in, const double* V3_in,
+                                     const double* c1_in, const double* hloc_gauss, double* c1_real_out,
+                                     double* opt_c2_real_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                     int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, int32_t l_G,
+                                     double les_coeff2, double les_coeff4, double* temps);
+int sfx_fft_compute_entropy_visc_mom_dev(const double* V1_in, const double* V2_in, const double* V3_in,
+                                         const double* c1_in, const double* hloc_gauss, double* c1_real_out,
+                                         double* opt_c2_real_out, int64_t np, int32_t m_max_c, int32_t m_max_pad,
+                                         int32_t l_G, double les_coeff2, double les_coeff4, void* stream);
+
 /* Replaces FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM (fft_parallel.F90:3419): V1_in,
  * V2_in, V3_in (np,6,m) the three momentum gradients, c_in_real(2*m_max_pad-1, np)
  * the entropy viscosity on the grid angles, les_coeff1_mom the module input the

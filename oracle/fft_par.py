@@ -24,7 +24,7 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 ``fft_no_overshoot_level_set`` (:3649-3819), ``fft_par_entro_visc_dcl``
 (:4055-4292), ``fft_par_compr_entro_visc_dcl`` (:3160-3417) and
 ``fft_compute_entropy_visc_grad_mom`` (:3419-3647), ``fft_compute_entropy_visc``
-(:2418-2699);
+(:2418-2699), ``fft_compute_entropy_visc_mom`` (:2701-2937);
 ``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
@@ -44,7 +44,7 @@ __all__ = [
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
     "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom", "fft_par_compr_entro_visc_dcl",
-    "fft_compute_entropy_visc",
+    "fft_compute_entropy_visc", "fft_compute_entropy_visc_mom",
     "regul", "OPS",
 ]
 
@@ -492,3 +492,50 @@ def fft_compute_entropy_visc(V1_in, V2_in, V3_in, V4_in, V5_in, hloc_gauss, resi
             pc = r2c(vis[:, None, :] * ru[2 + k], m_max_pad, engine)
             outs[k].append(unpack(np.asarray(pc, dtype=np.complex128), N, M))
     return tuple(np.concatenate(o, axis=0) for o in outs)
+
+
+def _element_angle_max(x, l_G):
+    """(n, N) per point and angle -> per point: the maximum over the point's
+    element (l_G consecutive points) and the angles j-1, j, j+1 (periodic)."""
+    n, N = x.shape
+    e = x.reshape(n // l_G, l_G, N).max(axis=1)
+    if N >= 3:
+        e = np.maximum(np.maximum(np.roll(e, 1, axis=1), e), np.roll(e, -1, axis=1))
+    return np.repeat(e, l_G, axis=0)
+
+
+def fft_compute_entropy_visc_mom(V1_in, V2_in, V3_in, c1_in, hloc_gauss, l_G, les_coeff2, les_coeff4,
+                                 m_max_pad, want_c2=False, engine="fft"):
+    """FFT_COMPUTE_ENTROPY_VISC_MOM (fft_parallel.F90:2701-2937), one rank: |u|,
+    |m| on the grid (:2839-2840) and the raw viscosity LES_coeff2 h
+    MAX(|u . res|, |c |u|^2|) (:2890-2893), each maximised over the element
+    and the angles j-1, j, j+1 (:2841-2870, :2896-2916); vmax = max |u|
+    (:2872-2876); c1_real_out = MIN(4 c4 vmax, 16 vis/(nv nm + 1e-30)) when
+    l_G = 3, else MIN(c4 nv, vis/(nv nm + 1e-30)) and opt_c2_real_out =
+    MIN(c4 vmax, vis/(nv nm + 1e-30)) (:2918-2931).  Returns (N, np) arrays."""
+    V1, V2, V3 = (np.asarray(x, dtype=np.float64) for x in (V1_in, V2_in, V3_in))
+    c = np.asarray(c1_in, dtype=np.float64)
+    h = np.asarray(hloc_gauss, dtype=np.float64)
+    npt, nb, M = V1.shape
+    _check(nb, M, "FFT_COMPUTE_ENTROPY_VISC_MOM")
+    if npt % l_G:
+        raise ValueError("np must be a multiple of l_G")
+    N = n_grid(m_max_pad)
+    nv, nm, vis = (np.empty((npt, N)) for _ in range(3))
+    chunk = l_G * max(1, _CHUNK // l_G)
+    for s0 in range(0, npt, chunk):
+        sl = slice(s0, s0 + chunk)
+        ru = np.asarray(c2r(pack(np.concatenate([V1[sl], V2[sl], V3[sl], c[sl, 0:2]], axis=1), m_max_pad), N,
+                            engine), dtype=np.float64)                       # (n, 10, N)
+        uu = np.sum(ru[:, 0:3] ** 2, axis=1)
+        raw = les_coeff2 * h[sl][:, None] * np.maximum(np.abs(np.sum(ru[:, 0:3] * ru[:, 6:9], axis=1)),
+                                                       np.abs(ru[:, 9] * uu))
+        nv[sl] = _element_angle_max(np.sqrt(uu), l_G)
+        nm[sl] = _element_angle_max(np.sqrt(np.sum(ru[:, 3:6] ** 2, axis=1)), l_G)
+        vis[sl] = _element_angle_max(raw, l_G)
+    vmax = float(np.max(nv)) if npt else 0.0
+    q = vis / (nv * nm + 1e-30)
+    if l_G == 3:
+        return np.minimum(4 * les_coeff4 * vmax, 16 * q).T.copy(), None
+    out2 = np.minimum(les_coeff4 * vmax, q).T.copy() if want_c2 else None
+    return np.minimum(les_coeff4 * nv, q).T.copy(), out2

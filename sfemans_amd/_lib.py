@@ -70,6 +70,10 @@ SIGNATURES = {
                                      _i32, _d, _i32, _d, _d, _d, _vp],
     "sfx_fft_compute_entropy_visc_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _d, _i32,
                                          _d, _d, _d, _vp],
+    "sfx_fft_compute_entropy_visc_mom": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32,
+                                         _i32, _d, _d, _vp],
+    "sfx_fft_compute_entropy_visc_mom_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _d, _d,
+                                             _vp],
     "sfx_fft_compute_diffu_mom": [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i64, _i32, _vp],
     "sfx_fft_compute_diffu_mom_dev": [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_no_overshoot_level_set_dev": [_vp, _i64, _i32, _i32, _i32, _vp],

@@ -72,6 +72,65 @@ __global__ void __launch_bounds__(512) exact_enorm_kernel(ExactArgs a, int nch, 
   }
 }
 
+// FFT_COMPUTE_ENTROPY_VISC_MOM, first pass: per element and angle the maxima
+// of |u|, |m| and the raw viscosity over the element's points (the angle
+// maxima are taken when they are read), and each work-group's maximum of |u|
+// into pmax.  Work-group: EXACT_ENORM_E elements x EXACT_EMOM_JC angle pairs.
+__global__ void __launch_bounds__(512) exact_emom_kernel(ExactArgs a, int nch, int64_t me) {
+  extern __shared__ double red[];   // [thread][3 * 2 * EXACT_EMOM_JC + 1]
+  constexpr int NV = 2 * EXACT_EMOM_JC, RW = 3 * NV + 1;
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t eb = L / nch;
+  const int j0 = (int)(L % nch) * EXACT_EMOM_JC;
+  const int lg = a.lg, t = threadIdx.x;
+  const int64_t p = eb * EXACT_ENORM_E * lg + t;
+  double nv[NV], nm[NV], vr[NV];
+  if (p < a.np) {
+    exact_emom_body(a, p, j0, nv, nm, vr);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nv[v] = nm[v] = vr[v] = 0.0;
+  }
+  double bmax = 0.0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    red[t * RW + v] = nv[v];
+    red[t * RW + NV + v] = nm[v];
+    red[t * RW + 2 * NV + v] = vr[v];
+    bmax = fmax(bmax, nv[v]);
+  }
+  __syncthreads();
+  const int N = 2 * a.K + 1;
+  for (int item = t; item < EXACT_ENORM_E * 3 * NV; item += blockDim.x) {
+    const int le = item / (3 * NV), w = item % (3 * NV), k = w / NV, v = w % NV;
+    const int64_t e = eb * EXACT_ENORM_E + le;
+    const int j = j0 + v % EXACT_EMOM_JC;
+    const bool mirror = v >= EXACT_EMOM_JC;
+    if (e >= me || j > a.K || (mirror && j == 0)) continue;
+    double x = red[(le * lg) * RW + w];
+    for (int l = 1; l < lg; ++l) x = fmax(x, red[(le * lg + l) * RW + w]);
+    double* dst = k == 0 ? a.nint : k == 1 ? a.nint2 : a.nint3;
+    dst[e * N + (mirror ? N - j : j)] = x;
+  }
+  __syncthreads();
+  // the work-group's maximum of |u| (reuses the first column of red)
+  red[t * RW] = bmax;
+  __syncthreads();
+  if (t == 0) {
+    double m = 0.0;
+    for (int i = 0; i < (int)blockDim.x; ++i) m = fmax(m, red[i * RW]);
+    a.pmax[blockIdx.x] = m;
+  }
+}
+
+// Second pass: one thread per (point, angle), angles fastest.
+__global__ void __launch_bounds__(256) exact_emom_out_kernel(ExactArgs a) {
+  const int N = 2 * a.K + 1;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.np * N) return;
+  exact_emom_out_body(a, i / N, (int)(i % N));
+}
+
 // Maximum of n values into out[0] (one work-group).
 __global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* v, int64_t n, double* out) {
   __shared__ double red[EXACT_THREADS];
@@ -90,6 +149,30 @@ __global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* 
 int64_t exact_grid_blocks(int op, int nin, int64_t np, int K) {
   const int JC = exact_jc(op, nin);
   return ((np + EXACT_THREADS - 1) / EXACT_THREADS) * ((K + 1 + JC - 1) / JC);
+}
+
+hipError_t launch_exact_emom(const ExactArgs& a, double* dmax, hipStream_t st) {
+  const int64_t me = a.np / a.lg;
+  const int nch = (a.K + 1 + EXACT_EMOM_JC - 1) / EXACT_EMOM_JC;
+  const int64_t nb = ((me + EXACT_ENORM_E - 1) / EXACT_ENORM_E) * nch;
+  const int threads = EXACT_ENORM_E * a.lg;
+  if (a.lg < 1 || a.lg > 16 || me * a.lg != a.np || nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (nb == 0) return hipSuccess;
+  const size_t shmem = (size_t)threads * (3 * 2 * EXACT_EMOM_JC + 1) * sizeof(double);
+  hipLaunchKernelGGL(exact_emom_kernel, dim3((unsigned)nb), dim3(threads), shmem, st, a, nch, me);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nb, dmax);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int64_t n = a.np * (2 * (int64_t)a.K + 1);
+  const int64_t nb2 = (n + 255) / 256;
+  if (nb2 > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(exact_emom_out_kernel, dim3((unsigned)nb2), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+int64_t exact_emom_blocks(int64_t np, int lg, int K) {
+  return ((np / lg + EXACT_ENORM_E - 1) / EXACT_ENORM_E) * ((K + 1 + EXACT_EMOM_JC - 1) / EXACT_EMOM_JC);
 }
 
 template <int OP, int NI>

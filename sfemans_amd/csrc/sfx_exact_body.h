@@ -83,6 +83,13 @@ struct ExactArgs {
   double* nint;         // EX_EVISC: per element and angle, the maximum of |v| over its points, [e*N + j]
   int lg;               // EX_EVISC: points per element
   int64_t n_nint;
+  // FFT_COMPUTE_ENTROPY_VISC_MOM (exact_emom_*): per element and angle maxima of
+  // |v| (nint), |m| (nint2) and of the raw viscosity (nint3); real-space outputs
+  double* nint2;
+  double* nint3;
+  double* rout1;        // c1_real_out (N, np) at [p*N + j]
+  double* rout2;        // opt_c2_real_out or null
+  const double* dmax;   // device scalar: the maximum of |v| over points and angles
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -270,6 +277,65 @@ SFX_DEV void exact_norm_body(const ExactArgs& a, int64_t p, int j0, double (&nrm
     }
     nrm[q] = j <= a.K ? sqrt(sp) : 0.0;                              // :2571
     nrm[EXACT_NORM_JC + q] = (j > 0 && j <= a.K) ? sqrt(sm) : 0.0;
+  }
+}
+
+// FFT_COMPUTE_ENTROPY_VISC_MOM, inputs u (0-2), m (3-5), residual (6-8) and the
+// mass residual c (9): at point p, angles j0..j0+EXACT_EMOM_JC-1 and their
+// mirrors (slot q and EXACT_EMOM_JC + q), |u|, |m| (:2839-2840) and the raw
+// viscosity LES_coeff2 h MAX(|u . res|, |c |u|^2|) (:2890-2893).
+constexpr int EXACT_EMOM_JC = 4;
+SFX_DEV void exact_emom_body(const ExactArgs& a, int64_t p, int j0, double (&nv)[2 * EXACT_EMOM_JC],
+                             double (&nm)[2 * EXACT_EMOM_JC], double (&vr)[2 * EXACT_EMOM_JC]) {
+  double E[10][EXACT_EMOM_JC], O[10][EXACT_EMOM_JC];
+  exact_c2r<10, EXACT_EMOM_JC>(a, p, j0, E, O);
+  const double h = a.hloc[p];
+#pragma unroll
+  for (int q = 0; q < EXACT_EMOM_JC; ++q) {
+    const int j = j0 + q;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      double u[10];
+#pragma unroll
+      for (int f = 0; f < 10; ++f) u[f] = side == 0 ? E[f][q] + O[f][q] : E[f][q] - O[f][q];
+      const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      const double mm = u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
+      const double ur = u[0] * u[6] + u[1] * u[7] + u[2] * u[8];
+      const bool ok = side == 0 ? j <= a.K : (j > 0 && j <= a.K);
+      const int s = side * EXACT_EMOM_JC + q;
+      nv[s] = ok ? sqrt(uu) : 0.0;
+      nm[s] = ok ? sqrt(mm) : 0.0;
+      vr[s] = ok ? a.coeff2 * h * fmax(fabs(ur), fabs(u[9] * uu)) : 0.0;
+    }
+  }
+}
+
+// Periodic maximum over angles j-1, j, j+1 of row e of a per-element array.
+SFX_DEV double exact_smooth3(const double* arr, int64_t e, int j, int N) {
+  const double* r = arr + e * N;
+  if (N < 3) return r[j];
+  const int jm = j == 0 ? N - 1 : j - 1, jp = j == N - 1 ? 0 : j + 1;
+  return fmax(fmax(r[jm], r[j]), r[jp]);
+}
+
+// Outputs at point p and angle j (:2914-2931): with nv, nm, vis the element and
+// angle maxima of |u|, |m| and the raw viscosity and vmax the maximum of |u|,
+// l_G = 3: MIN(4 c4 vmax, 16 vis/(nv nm + 1e-30)); otherwise MIN(c4 nv, vis/(nv nm
+// + 1e-30)) and, when asked for, MIN(c4 vmax, vis/(nv nm + 1e-30)).
+SFX_DEV void exact_emom_out_body(const ExactArgs& a, int64_t p, int j) {
+  const int N = 2 * a.K + 1;
+  const int64_t e = p / a.lg;
+  SFX_CHECK("exact emom", e * N + N - 1, a.n_nint);
+  const double nv = exact_smooth3(a.nint, e, j, N), nm = exact_smooth3(a.nint2, e, j, N);
+  const double vis = exact_smooth3(a.nint3, e, j, N);
+  const double vmax = a.dmax[0], c4 = a.coeff3;
+  const double q = vis / (nv * nm + 1.e-30);
+  const int64_t o = p * N + j;
+  if (a.lg == 3) {
+    a.rout1[o] = fmin(4.0 * c4 * vmax, 16.0 * q);
+  } else {
+    a.rout1[o] = fmin(c4 * nv, q);
+    if (a.rout2) a.rout2[o] = fmin(c4 * vmax, q);
   }
 }
 

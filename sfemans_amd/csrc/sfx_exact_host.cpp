@@ -159,6 +159,37 @@ static int evisc_dev(const double* const (&V)[5], const double* hloc, double* co
   return exact_run(nm, EX_EVISC, a, nullptr, st);
 }
 
+static int emom_dev(const double* V1, const double* V2, const double* V3, const double* c, const double* hloc,
+                    double* out1, double* out2, int64_t np, int M, int mp, int lg, double c2, double c4,
+                    hipStream_t st) {
+  const char* nm = "FFT_COMPUTE_ENTROPY_VISC_MOM";
+  if (M <= 0) return fail(SFX_ERR_ARG, "BUG in %s: m_max_c==0", nm);
+  if (M > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, M);
+  if (mp < M || mp > 4096) return fail(SFX_ERR_ARG, "%s: m_max_pad=%d", nm, mp);
+  if (lg < 1 || lg > 16) return fail(SFX_ERR_UNSUPPORTED, "%s: l_G=%d (1..16)", nm, lg);
+  if (np < 0 || np % lg != 0) return fail(SFX_ERR_ARG, "%s: np=%lld is not a multiple of l_G=%d", nm, (long long)np, lg);
+  if (np == 0) return SFX_OK;
+  if (!hloc || !out1) return fail(SFX_ERR_ARG, "%s: null hloc_gauss or c1_real_out", nm);
+  ExactArgs a;
+  exact_setup_emom(a, V1, V2, V3, c, hloc, out1, out2, np, M, lg, c2, c4);
+  exact_setup_sizes(a, np, M, mp);
+  int rc = exact_tables(M, a.K, &a.ctab, &a.rtab);
+  if (rc) return rc;
+  const int64_t ne = (np / lg) * (2 * (int64_t)mp - 1);
+  void *w = nullptr, *pm = nullptr, *dm = nullptr;
+  if ((rc = staging(SLOT_EX_NINT, (size_t)3 * ne * 8, &w)) ||
+      (rc = staging(15, (size_t)exact_emom_blocks(np, lg, a.K) * 8, &pm)) || (rc = staging(SLOT_EX_MAX, 8, &dm)))
+    return rc;
+  a.nint = (double*)w;
+  a.nint2 = a.nint + ne;
+  a.nint3 = a.nint2 + ne;
+  a.n_nint = ne;
+  a.pmax = (double*)pm;
+  a.dmax = (const double*)dm;
+  hipError_t e = launch_exact_emom(a, (double*)dm, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
 static int gradmom_dev(const double* V1, const double* V2, const double* V3, const double* creal, double coeff,
                        double* out, int64_t np, int M, int mp, hipStream_t st) {
   if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: null c_in_real");
@@ -530,6 +561,48 @@ int sfx_fft_compute_entropy_visc(const double* V1_in, const double* V2_in, const
   if ((rc = from_dev(nullptr, o1, 0))) return rc;
   t.t2 = now_s();
   if ((rc = from_dev(V1_out, o1, n1)) || (rc = from_dev(V2_out, o2, n1)) || (rc = from_dev(V3_out, o3, n1))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_compute_entropy_visc_mom_dev(const double* V1_in, const double* V2_in, const double* V3_in,
+                                        const double* c1_in, const double* hloc_gauss, double* c1_real_out,
+                                        double* opt_c2_real_out, int64_t np, int32_t m_max_c, int32_t m_max_pad,
+                                        int32_t l_G, double les_coeff2, double les_coeff4, void* stream) {
+  clear_err();
+  return emom_dev(V1_in, V2_in, V3_in, c1_in, hloc_gauss, c1_real_out, opt_c2_real_out, np, m_max_c, m_max_pad,
+                  l_G, les_coeff2, les_coeff4, (hipStream_t)stream);
+}
+
+int sfx_fft_compute_entropy_visc_mom(const double* V1_in, const double* V2_in, const double* V3_in,
+                                     const double* c1_in, const double* hloc_gauss, double* c1_real_out,
+                                     double* opt_c2_real_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                     int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad, int32_t l_G,
+                                     double les_coeff2, double les_coeff4, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field != 6 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_COMPUTE_ENTROPY_VISC_MOM ");
+  int rc = check_procs("FFT_COMPUTE_ENTROPY_VISC_MOM", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (m_max_pad < 1 || m_max_pad > 4096) return fail(SFX_ERR_ARG, "FFT_COMPUTE_ENTROPY_VISC_MOM: m_max_pad");
+  const size_t n1 = (size_t)np * 6 * m_max_c, n2 = (size_t)np * 2 * m_max_c, nr = (size_t)np * (2 * m_max_pad - 1);
+  void *d1, *d2, *d3, *dc, *dh, *o1, *o2 = nullptr;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, V2_in, n1, &d2)) ||
+      (rc = to_dev(SLOT_EX_IN3, V3_in, n1, &d3)) || (rc = to_dev(SLOT_EX_IN4, c1_in, n2, &dc)) ||
+      (rc = to_dev(13, hloc_gauss, (size_t)np, &dh)) || (rc = staging(SLOT_EX_OUT, nr * 8, &o1)) ||
+      (opt_c2_real_out && (rc = staging(SLOT_EX_OUT2, nr * 8, &o2))))
+    return rc;
+  t.t1 = now_s();
+  if ((rc = emom_dev((const double*)d1, (const double*)d2, (const double*)d3, (const double*)dc, (const double*)dh,
+                     (double*)o1, (double*)o2, np, m_max_c, m_max_pad, l_G, les_coeff2, les_coeff4, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, o1, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(c1_real_out, o1, nr))) return rc;
+  if (opt_c2_real_out && (rc = from_dev(opt_c2_real_out, o2, nr))) return rc;
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;

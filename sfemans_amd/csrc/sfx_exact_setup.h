@@ -163,6 +163,28 @@ inline void exact_setup_evisc(ExactArgs& a, const double* const (&V)[5], const d
   a.coeff = c1; a.coeff2 = c2; a.coeff3 = c4; a.coeff4 = res;
 }
 
+// FFT_COMPUTE_ENTROPY_VISC_MOM: u V1, momentum V2, residual V3 (np, 6, M), mass
+// residual c (np, 2, M), hloc (np); real-space outputs (N, np).  The caller sets
+// the per-element arrays, pmax and the sizes.
+inline void exact_setup_emom(ExactArgs& a, const double* V1, const double* V2, const double* V3, const double* c,
+                             const double* hloc, double* out1, double* out2, int64_t np, int M, int lg, double c2,
+                             double c4) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 10; a.nout = 0;
+  const double* V[3] = {V1, V2, V3};
+  for (int v = 0; v < 3; ++v)
+    for (int k = 0; k < 3; ++k) {
+      a.in[3 * v + k] = exact_col(V[v], np, 6, k);
+      a.n_in[3 * v + k] = np * 6 * (int64_t)M - 2 * k * np;
+    }
+  a.in[9] = exact_col(c, np, 2, 0);
+  a.n_in[9] = np * 2 * (int64_t)M;
+  a.hloc = hloc;
+  a.rout1 = out1; a.rout2 = out2;
+  a.lg = lg;
+  a.coeff2 = c2; a.coeff3 = c4;
+}
+
 // FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
 // -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {

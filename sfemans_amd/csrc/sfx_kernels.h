@@ -65,6 +65,10 @@ hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
 struct ExactArgs;
 hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t st);
 int64_t exact_grid_blocks(int op, int nin, int64_t np, int K);
+// FFT_COMPUTE_ENTROPY_VISC_MOM: element maxima (pmax: exact_emom_blocks doubles),
+// the maximum of |u| into dmax, then the real-space outputs.
+hipError_t launch_exact_emom(const ExactArgs& a, double* dmax, hipStream_t st);
+int64_t exact_emom_blocks(int64_t np, int lg, int K);
 #endif
 
 // Pass lists (GV_PAIRS) run as one single-pass GV_CURL_CROSS launch per

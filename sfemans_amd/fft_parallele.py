@@ -21,6 +21,7 @@ __all__ = [
     "FFT_SQUAREROOT_SCALAR", "FFT_MAX_VEL_DCL", "FFT_SCALAR_VECT_NO_OVERSHOOT",
     "FFT_NO_OVERSHOOT_LEVEL_SET", "FFT_PAR_ENTRO_VISC_DCL", "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM",
     "FFT_COMPUTE_DIFFU_MOM", "FFT_PAR_COMPR_ENTRO_VISC_DCL", "FFT_COMPUTE_ENTROPY_VISC",
+    "FFT_COMPUTE_ENTROPY_VISC_MOM",
     "fortran_tensor", "fortran_empty",
     "is_device", "ptr", "fdims", "stream_ptr",
 ]
@@ -378,3 +379,30 @@ def FFT_COMPUTE_ENTROPY_VISC(communicator, communicator_S, V1_in, V2_in, V3_in, 
                                              ptr(V2_out), ptr(V3_out), np_, nb, M, nb_procs, bloc_size, m_max_pad,
                                              *args, _temps_ptr(temps)))
     return V1_out, V2_out, V3_out
+
+
+def FFT_COMPUTE_ENTROPY_VISC_MOM(communicator, communicator_S, V1_in, V2_in, V3_in, c1_in, hloc_gauss,
+                                 c1_real_out, l_G, les_coeff2, les_coeff4, nb_procs=1, bloc_size=None,
+                                 m_max_pad=None, opt_c2_real_out=None, temps=None):
+    """Entropy viscosities of the momentum (and level-set) equations on the grid
+    (fft_parallel.F90:2701); outputs (2*m_max_pad-1, np).  The LES coefficients
+    are the reference's inputs%LES_coeff2/4, passed explicitly."""
+    _single_rank(communicator)
+    _single_rank(communicator_S)
+    np_, nb, M = fdims(V1_in)
+    fdims(V2_in), fdims(V3_in), fdims(c1_in), fdims(c1_real_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    if tuple(fdims(c1_real_out)) != (2 * m_max_pad - 1, np_):
+        raise ValueError("c1_real_out must be (2*m_max_pad-1, np)")
+    L = lib()
+    if is_device(V1_in):
+        check(L.sfx_fft_compute_entropy_visc_mom_dev(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(c1_in), ptr(hloc_gauss),
+                                                     ptr(c1_real_out), ptr(opt_c2_real_out), np_, M, m_max_pad,
+                                                     int(l_G), float(les_coeff2), float(les_coeff4), stream_ptr()))
+    else:
+        check(L.sfx_fft_compute_entropy_visc_mom(ptr(V1_in), ptr(V2_in), ptr(V3_in), ptr(c1_in),
+                                                 ptr(np.ascontiguousarray(hloc_gauss, dtype=np.float64)),
+                                                 ptr(c1_real_out), ptr(opt_c2_real_out), np_, nb, M, nb_procs,
+                                                 bloc_size, m_max_pad, int(l_G), float(les_coeff2),
+                                                 float(les_coeff4), _temps_ptr(temps)))
+    return c1_real_out, opt_c2_real_out

@@ -526,3 +526,57 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
       for (int mc = 0; mc < nmc; ++mc) exact_r2c_body(a, p, g, mc);
   return 0;
 }
+
+// FFT_COMPUTE_ENTROPY_VISC_MOM: V = [V1 | V2 | V3] (3 x np*6*M), c (np,2,M),
+// outputs (N, np); out2 may be null.  Both passes as the kernels run them.
+extern "C" int emu_exact_emom(int64_t np, int M, int mp, int lg, const double* V, const double* c, const double* hloc,
+                              double c2, double c4, double* out1, double* out2) {
+  const int64_t n6 = np * 6 * M;
+  ExactArgs a;
+  exact_setup_emom(a, V, V + n6, V + 2 * n6, c, hloc, out1, out2, np, M, lg, c2, c4);
+  exact_setup_sizes(a, np, M, mp);
+  std::vector<double> tab;
+  exact_host_tables(M, a.K, tab);
+  a.ctab = tab.data();
+  a.rtab = tab.data() + exact_ctab_size(M, a.K);
+  const int N = 2 * a.K + 1;
+  const int64_t me = np / lg, ne = me * N;
+  std::vector<double> arr((size_t)3 * ne + 1, -1.0);
+  a.nint = arr.data(); a.nint2 = a.nint + ne; a.nint3 = a.nint2 + ne; a.n_nint = ne;
+  constexpr int NV = 2 * EXACT_EMOM_JC;
+  const int nch = (a.K + 1 + EXACT_EMOM_JC - 1) / EXACT_EMOM_JC;
+  double vmax = 0.0;
+  std::vector<double> red((size_t)32 * lg * 3 * NV);
+  for (int64_t eb = 0; eb < (me + 31) / 32; ++eb)
+    for (int jc = 0; jc < nch; ++jc) {
+      const int j0 = jc * EXACT_EMOM_JC;
+      for (int t = 0; t < 32 * lg; ++t) {
+        const int64_t p = eb * 32 * lg + t;
+        double nv[NV] = {0}, nm[NV] = {0}, vr[NV] = {0};
+        if (p < np) exact_emom_body(a, p, j0, nv, nm, vr);
+        for (int v = 0; v < NV; ++v) {
+          red[(size_t)t * 3 * NV + v] = nv[v];
+          red[(size_t)t * 3 * NV + NV + v] = nm[v];
+          red[(size_t)t * 3 * NV + 2 * NV + v] = vr[v];
+          vmax = std::max(vmax, nv[v]);
+        }
+      }
+      for (int le = 0; le < 32; ++le)
+        for (int w = 0; w < 3 * NV; ++w) {
+          const int64_t e = eb * 32 + le;
+          const int k = w / NV, v = w % NV, j = j0 + v % EXACT_EMOM_JC;
+          const bool mirror = v >= EXACT_EMOM_JC;
+          if (e >= me || j > a.K || (mirror && j == 0)) continue;
+          double x = red[(size_t)(le * lg) * 3 * NV + w];
+          for (int l = 1; l < lg; ++l) x = std::max(x, red[(size_t)(le * lg + l) * 3 * NV + w]);
+          double* dst = k == 0 ? a.nint : k == 1 ? a.nint2 : a.nint3;
+          dst[e * N + (mirror ? N - j : j)] = x;
+        }
+    }
+  for (int64_t i = 0; i < 3 * ne; ++i)
+    if (arr[i] < 0) return 3;   // every (element, angle) written
+  a.dmax = &vmax;
+  for (int64_t p = 0; p < np; ++p)
+    for (int j = 0; j < N; ++j) exact_emom_out_body(a, p, j);
+  return 0;
+}

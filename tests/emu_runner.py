@@ -53,6 +53,8 @@ def _load():
     L.emu_exact.restype = C.c_int
     L.emu_exact_evisc.argtypes = [i64, i32, i32, i32, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, vp]
     L.emu_exact_evisc.restype = C.c_int
+    L.emu_exact_emom.argtypes = [i64, i32, i32, i32, vp, vp, vp, C.c_double, C.c_double, vp, vp]
+    L.emu_exact_emom.restype = C.c_int
     return L
 
 

@@ -184,3 +184,22 @@ def test_entropy_visc(ne, M, mp, lg):
     for k in range(3):
         got = out[k * npt * 6 * M:(k + 1) * npt * 6 * M].reshape((npt, 6, M), order="F")
         assert rel(got, exp[k]) <= 1e-12
+
+
+@pytest.mark.parametrize("ne,M,mp,lg,c2out", [(5, 8, 12, 7, True), (40, 4, 6, 7, False), (6, 16, 24, 3, False)])
+def test_entropy_visc_mom(ne, M, mp, lg, c2out):
+    rng = np.random.default_rng(101 + M)
+    npt = ne * lg
+    Vs = [fields(rng, npt, 6, M) for _ in range(3)]
+    c = fields(rng, npt, 2, M)
+    h = rng.uniform(0.1, 1.0, npt)
+    N = 2 * mp - 1
+    packed = np.concatenate([v.ravel(order="F") for v in Vs])
+    o2 = np.zeros((N, npt), order="F") if c2out else None
+    rc, arrs = emu_runner.call("emu_exact_emom", npt, M, mp, lg, packed, c, h, 0.6, 0.9,
+                               np.zeros((N, npt), order="F"), o2)
+    assert rc == 0
+    e1, e2 = F.fft_compute_entropy_visc_mom(*Vs, c, h, lg, 0.6, 0.9, mp, c2out)
+    assert rel(arrs[3 if not c2out else -2], e1) <= 1e-12
+    if c2out:
+        assert rel(arrs[-1], e2) <= 1e-12

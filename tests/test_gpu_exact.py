@@ -256,6 +256,33 @@ def test_entropy_visc(ne, M):
         assert rel(douts[k].cpu().numpy(), exp[k]) <= TOL
 
 
+@pytest.mark.parametrize("ne,M,lg,c2out", [(300, 8, 7, True), (40, 64, 7, False), (2000, 4, 3, False)])
+def test_entropy_visc_mom(ne, M, lg, c2out):
+    print(f"case entropy_visc_mom elements={ne} M={M} l_G={lg}", flush=True)
+    rng = np.random.default_rng(101 + M)
+    npt = lg * ne
+    Vs = [fields(rng, npt, 6, M) for _ in range(3)]
+    c = fields(rng, npt, 2, M)
+    h = rng.uniform(0.1, 1.0, npt)
+    mp = 3 * M // 2
+    N = 2 * mp - 1
+    e1, e2 = F.fft_compute_entropy_visc_mom(*Vs, c, h, lg, 0.6, 0.9, mp, c2out)
+    o1 = np.zeros((N, npt), order="F")
+    o2 = np.zeros((N, npt), order="F") if c2out else None
+    FP.FFT_COMPUTE_ENTROPY_VISC_MOM(None, None, *Vs, c, h, o1, lg, 0.6, 0.9, opt_c2_real_out=o2)
+    assert rel(o1, e1) <= TOL
+    if c2out:
+        assert rel(o2, e2) <= TOL
+    d1 = FP.fortran_empty((N, npt))
+    d2 = FP.fortran_empty((N, npt)) if c2out else None
+    FP.FFT_COMPUTE_ENTROPY_VISC_MOM(None, None, *[FP.fortran_tensor(v) for v in Vs], FP.fortran_tensor(c),
+                                    torch.as_tensor(h, device="cuda"), d1, lg, 0.6, 0.9, opt_c2_real_out=d2)
+    torch.cuda.synchronize()
+    assert rel(d1.cpu().numpy(), e1) <= TOL
+    if c2out:
+        assert rel(d2.cpu().numpy(), e2) <= TOL
+
+
 def test_m_max_pad_other_than_three_halves():
     """The grid follows m_max_pad: odd and larger pads give other coefficients."""
     rng = np.random.default_rng(60)

@@ -21,7 +21,8 @@ scaling, truncation) through a closed form that does not use the oracle:
   scalar-vector products; the compression form reduces to the entropy form
   without compression and has a closed form for constant fields; the full
   entropy viscosity has closed forms with either bound of its MIN active
-  (the element and angle maxima through direct cosine sums).
+  (the element and angle maxima through direct cosine sums); the momentum
+  form for fields constant in angle, per element, for l_G = 7 and 3.
 
 The pocketfft engine is also checked against the long-double direct sum.
 """
@@ -382,3 +383,47 @@ def test_entropy_visc_with_the_residual_bound_active():
     vis = -c1 + c2 * h * np.abs(np.sum(v * r, axis=1)) / (res + 1e-14)
     for k in range(3):
         assert rel(out[k], vis[:, None, None] * Vs[k]) <= 1e-13
+
+
+@pytest.mark.parametrize("lg", [7, 3])
+def test_entropy_visc_mom_for_fields_constant_in_angle(lg):
+    """Mode-0 fields: every grid quantity is constant in angle, so the element
+    and angle maxima are maxima over the element's points and the outputs
+    have a closed form per element."""
+    M, mp, ne = 5, 8, 4
+    N = 2 * mp - 1
+    npt = lg * ne
+    rng = np.random.default_rng(20 + lg)
+    u, m, r = (rng.standard_normal((npt, 3)) for _ in range(3))
+    cc = rng.standard_normal(npt)
+    V1, V2, V3 = (np.zeros((npt, 6, M)) for _ in range(3))
+    V1[:, 0::2, 0], V2[:, 0::2, 0], V3[:, 0::2, 0] = u, m, r
+    C = np.zeros((npt, 2, M)); C[:, 0, 0] = cc
+    h = rng.uniform(0.1, 1.0, npt)
+    c2, c4 = 0.6, 0.9
+    o1, o2 = F.fft_compute_entropy_visc_mom(V1, V2, V3, C, h, lg, c2, c4, mp, want_c2=True)
+    uu = np.sum(u ** 2, axis=1)
+    raw = c2 * h * np.maximum(np.abs(np.sum(u * r, axis=1)), np.abs(cc * uu))
+    emax = lambda x: np.repeat(x.reshape(ne, lg).max(axis=1), lg)
+    nv, nm, vis = emax(np.sqrt(uu)), emax(np.sqrt(np.sum(m ** 2, axis=1))), emax(raw)
+    vmax = np.max(np.sqrt(uu))
+    q = vis / (nv * nm + 1e-30)
+    if lg == 3:
+        ex1 = np.minimum(4 * c4 * vmax, 16 * q)
+        assert o2 is None
+    else:
+        ex1 = np.minimum(c4 * nv, q)
+        assert np.allclose(o2, np.minimum(c4 * vmax, q)[None, :], rtol=1e-13, atol=0)
+    assert o1.shape == (N, npt)
+    assert np.allclose(o1, ex1[None, :], rtol=1e-13, atol=0)
+
+
+def test_entropy_visc_mom_engines_agree():
+    rng = np.random.default_rng(22)
+    M, mp, lg, npt = 6, 9, 7, 14
+    Vs = [rng.standard_normal((npt, 6, M)) for _ in range(3)]
+    C = rng.standard_normal((npt, 2, M))
+    h = rng.uniform(0.1, 1.0, npt)
+    a = F.fft_compute_entropy_visc_mom(*Vs, C, h, lg, 0.6, 0.9, mp, True)
+    b = F.fft_compute_entropy_visc_mom(*Vs, C, h, lg, 0.6, 0.9, mp, True, "direct")
+    assert rel(a[0], b[0]) <= 1e-12 and rel(a[1], b[1]) <= 1e-12
