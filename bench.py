@@ -35,6 +35,8 @@ coefficients, mode-0 sine set to zero):
 * ``exact``: the six non-polynomial FFT_PAR products (Heaviside of a level
   set, pb = 2 scalar-vector, division, square root, tension tensor, velocity
   maximum) on the reference grid N = 191 at 1,404,928 points, m_max = 64.
+* ``entropy``: the six entropy-viscosity products at 1,404,928 Gauss points
+  (200,704 elements of 7), m_max = 64, grid N = 191.
 * ``full``: the explicit nonlinear step at m_max = 128 on the 224x448 mesh:
   (curl u) x u -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and
   (rho u) (x) u with a level-set density, and the convective adjoint.
@@ -249,6 +251,56 @@ def wl_exact(torch, args):
     return phases, npt * M, cfg, ("exact", None, M)
 
 
+def wl_entropy(torch, args):
+    from sfemans_amd import fft_parallele as FP
+    npt, M, lg = 1404928, 64, 7
+    mp = 3 * M // 2
+    N = 2 * mp - 1
+    dec = torch.exp(-torch.arange(M, device="cuda", dtype=torch.float64) / 4.0)
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 12)
+
+    def fld(nb, mean=None):
+        t = torch.randn((M, nb, npt), generator=gen, device="cuda", dtype=torch.float64) * dec[:, None, None]
+        t[0, 1::2, :] = 0.0
+        if mean is not None:
+            t[0, 0, :] = mean
+        return t.permute(2, 1, 0)
+    u, mom, res, g1, g2, g3 = (fld(6) for _ in range(6))
+    c, rmass = fld(2, 0.5), fld(2)
+    creal = torch.rand((npt, N), generator=gen, device="cuda", dtype=torch.float64).t()   # (N, np), Fortran
+    h = torch.rand(npt, generator=gen, device="cuda", dtype=torch.float64) * 0.01
+    v6 = [FP.fortran_empty((npt, 6, M)) for _ in range(3)]
+    c2o = FP.fortran_empty((npt, 2, M))
+    t3 = FP.fortran_empty((3, npt, 6, M))
+    r1, r2 = FP.fortran_empty((N, npt)), FP.fortran_empty((N, npt))
+    f_t = 4.0 * M * mp
+    b = 8 * M * npt
+    ops = [
+        ("par_entro_visc", lambda: FP.FFT_PAR_ENTRO_VISC_DCL(None, u, c, creal, None, 0.3, v6[0], c2o),
+         8 * b + 8 * N * npt, 6 * f_t * npt, "exact_grid_kernel<ENTRO,4> + exact_r2c_kernel"),
+        ("compr_entro_visc", lambda: FP.FFT_PAR_COMPR_ENTRO_VISC_DCL(None, u, g1, c, creal, None, 0.3, v6[0], c2o,
+                                                                      1.5),
+         14 * b + 8 * N * npt, 11 * f_t * npt, "exact_grid_kernel<COMPR,7> + exact_r2c_kernel"),
+        ("entropy_visc_grad_mom", lambda: FP.FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM(None, g1, g2, g3, creal, t3, 0.2),
+         36 * b + 8 * N * npt, 18 * f_t * npt, "exact_grid_kernel<GRADMOM,9> + exact_r2c_kernel"),
+        ("entropy_visc", lambda: FP.FFT_COMPUTE_ENTROPY_VISC(None, None, u, res, g1, g2, g3, h, *v6, 1.0, lg, 0.3,
+                                                             0.9, 0.8),
+         48 * b + 8 * npt, 24 * f_t * npt, "exact_enorm_kernel + exact_grid_kernel<EVISC,15> + exact_r2c_kernel"),
+        ("entropy_visc_mom", lambda: FP.FFT_COMPUTE_ENTROPY_VISC_MOM(None, None, u, mom, res, rmass, h, r1, lg, 0.6,
+                                                                     0.9, opt_c2_real_out=r2),
+         20 * b + 8 * npt + 16 * N * npt, 10 * f_t * npt, "exact_emom_kernel + exact_emom_out_kernel"),
+        ("diffu_mom", lambda: FP.FFT_COMPUTE_DIFFU_MOM(None, g1, g2, c, v6[1], v6[2]), 26 * b,
+         npt * fft_flops_per_point(M, 9, 3, 9) * 2, "fftprod_kernel<64,SVECT1> x 2"),
+    ]
+    phases = [Phase(n, f, bb, fl, k) for n, f, bb, fl, k in ops]
+    cfg = {"workload": "entropy-viscosity FFT_PAR products on the reference grid N=191 at 1,404,928 Gauss "
+                       "points (200,704 elements of 7), m_max=64: FFT_PAR_ENTRO_VISC_DCL, "
+                       "FFT_PAR_COMPR_ENTRO_VISC_DCL, FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM, FFT_COMPUTE_ENTROPY_VISC, "
+                       "FFT_COMPUTE_ENTROPY_VISC_MOM, FFT_COMPUTE_DIFFU_MOM",
+           "points": npt, "m_max": M, "grid": N}
+    return phases, npt * M, cfg, ("entropy", None, M)
+
+
 def wl_fftprod(torch, args):
     from sfemans_amd import fft_parallele as FP
     npt, M = 1404928, 64
@@ -403,7 +455,7 @@ def wl_full(torch, args):
 
 
 WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch,
-             "full": wl_full, "nsrhs": wl_nsrhs, "exact": wl_exact}
+             "full": wl_full, "nsrhs": wl_nsrhs, "exact": wl_exact, "entropy": wl_entropy}
 
 
 # ------------------------------------------------------------- CPU baseline
@@ -453,6 +505,36 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
         return {"value": n * npt * M / dt, "unit": "Gauss-pt*modes/s", "cores": 1, "kind": "port",
                 "sample": f"the six non-polynomial FFT_PAR products restated in numpy (pocketfft on the grid "
                           f"N={2 * mp - 1}), {n} passes over {npt} points x m_max={M}, 1 thread, {dt:.1f} s"}
+    if kind == "entropy":
+        # the oracle's six entropy-viscosity products on 2,058 points (294 elements of 7), same m_max
+        rng = np.random.default_rng(4)
+        npt, lg = 2058, 7
+        mp = 3 * M // 2
+        N = 2 * mp - 1
+        dec = np.exp(-np.arange(M) / 4.0)
+        fld = lambda nb: np.asfortranarray(rng.standard_normal((npt, nb, M)) * dec)
+        u, mom, res, g1, g2, g3 = (fld(6) for _ in range(6))
+        c, rmass = fld(2), fld(2)
+        c[:, 0, 0] = 0.5
+        creal = rng.random((N, npt))
+        h = rng.random(npt) * 0.01
+
+        def run():
+            F.fft_par_entro_visc_dcl(u, c, creal, 0.3, mp)
+            F.fft_par_compr_entro_visc_dcl(u, g1, c, creal, 0.3, 1.5, mp)
+            F.fft_compute_entropy_visc_grad_mom(g1, g2, g3, creal, 0.2, mp)
+            F.fft_compute_entropy_visc(u, res, g1, g2, g3, h, 1.0, lg, 0.3, 0.9, 0.8, mp)
+            F.fft_compute_entropy_visc_mom(u, mom, res, rmass, h, lg, 0.6, 0.9, mp, True)
+            F.fft_compute_diffu_mom(g1, g2, c, mp)
+        run()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() < t0 + sample_s or n == 0:
+            run()
+            n += 1
+        dt = time.perf_counter() - t0
+        return {"value": n * npt * M / dt, "unit": "Gauss-pt*modes/s", "cores": 1, "kind": "port",
+                "sample": f"the six entropy-viscosity FFT_PAR products restated in numpy (pocketfft on the grid "
+                          f"N={N}), {n} passes over {npt} points x m_max={M}, 1 thread, {dt:.1f} s"}
     if kind == "nsrhs":
         from oracle import assembly
         from sfemans_amd.mesh import rectangle_p2
