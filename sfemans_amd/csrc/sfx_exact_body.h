@@ -30,7 +30,7 @@ constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
                EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10,
-               EX_EVISC = 11 };
+               EX_EVISC = 11, EX_EVISCV = 12 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -39,7 +39,7 @@ constexpr int exact_jc(int op, int nin) {
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
-  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_MAXVEL ? 0
+  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : (op == EX_MAXVEL || op == EX_EVISCV) ? 0
          : (op == EX_ENTRO || op == EX_COMPR) ? 4 : (op == EX_GRADMOM || op == EX_EVISC) ? 9 : 1;
 }
 
@@ -90,6 +90,7 @@ struct ExactArgs {
   double* rout1;        // c1_real_out (N, np) at [p*N + j]
   double* rout2;        // opt_c2_real_out or null
   const double* dmax;   // device scalar: the maximum of |v| over points and angles
+  int64_t n_nint2;      // EX_EVISCV: size of rout1
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -106,13 +107,17 @@ SFX_DEV double exact_regul(double phi, double eps) {
 template <int OP, int NI, int NO>
 SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO > 0 ? NO : 1], double x,
                       double y) {
-  if constexpr (OP == EX_EVISC) {
+  if constexpr (OP == EX_EVISC || OP == EX_EVISCV) {
     // x: |v| maximised over the element and the neighbouring angles, y: the
     // point's mesh size; v (0-2), residual (3-5), three tensor rows (6-14)   :2601-2620
     const double dot = u[0] * u[3] + u[1] * u[4] + u[2] * u[5];
     const double ve = -a.coeff + fmin(a.coeff3 * x, a.coeff2 * y * fabs(dot) / (a.coeff4 + 1.e-14));
+    if constexpr (OP == EX_EVISCV) {
+      w[0] = ve;                      // the viscosity alone, on the grid
+    } else {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) w[k] = ve * u[6 + k];
+      for (int k = 0; k < 9; ++k) w[k] = ve * u[6 + k];
+    }
   } else if constexpr (OP == EX_ENTRO) {
     const double v = a.coeff - x;                                 // -visc_entro, :4212-4219
 #pragma unroll
@@ -235,7 +240,7 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
       SFX_CHECK("exact creal", p * N + (j > 0 ? N - j : 0), a.n_creal);
       xp = a.creal[p * N + j];
       xm = j > 0 ? a.creal[p * N + (N - j)] : 0.0;
-    } else if constexpr (OP == EX_EVISC) {
+    } else if constexpr (OP == EX_EVISC || OP == EX_EVISCV) {
       const int64_t e = p / a.lg;
       xp = exact_evisc_norm(a, e, j);
       xm = j > 0 ? exact_evisc_norm(a, e, 2 * a.K + 1 - j) : 0.0;
@@ -247,6 +252,12 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
     if constexpr (OP == EX_MAXVEL) {
       vmax = fmax(vmax, wp[0]);
       if (j > 0) vmax = fmax(vmax, wm[0]);
+    } else if constexpr (OP == EX_EVISCV) {
+      // real-space output (N, np) at [p*N + angle]
+      const int64_t N = 2 * a.K + 1;
+      SFX_CHECK("exact rout", p * N + (j > 0 ? N - j : 0), a.n_nint2);
+      a.rout1[p * N + j] = wp[0];
+      if (j > 0) a.rout1[p * N + (N - j)] = wm[0];
     } else {
 #pragma unroll
       for (int g = 0; g < NO; ++g) {

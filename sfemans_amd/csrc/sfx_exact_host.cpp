@@ -155,6 +155,10 @@ static int evisc_dev(const double* const (&V)[5], const double* hloc, double* co
   int rc = staging(SLOT_EX_NINT, (size_t)(a.n_nint > 0 ? a.n_nint : 1) * 8, &w);
   if (rc) return rc;
   a.nint = (double*)w;
+  // the viscosity on the grid between the two product passes (N, np)
+  void* vr = nullptr;
+  if ((rc = staging(15, (size_t)(2 * (int64_t)mp - 1) * (np > 0 ? np : 1) * 8, &vr))) return rc;
+  a.rout2 = (double*)vr;
   if ((rc = exact_prepare(nm, a, EX_EVISC, np, M, mp))) return rc;
   return exact_run(nm, EX_EVISC, a, nullptr, st);
 }

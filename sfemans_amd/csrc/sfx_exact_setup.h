@@ -163,6 +163,26 @@ inline void exact_setup_evisc(ExactArgs& a, const double* const (&V)[5], const d
   a.coeff = c1; a.coeff2 = c2; a.coeff3 = c4; a.coeff4 = res;
 }
 
+// FFT_COMPUTE_ENTROPY_VISC in two passes from its full argument set `a`: `v`
+// writes the viscosity on the grid into vreal (N, np) from the velocity and
+// residual; `w` multiplies the three tensor rows by it (EX_GRADMOM with no
+// offset) and carries the r2c outputs.
+inline void exact_evisc_split(const ExactArgs& a, double* vreal, ExactArgs& v, ExactArgs& w) {
+  v = a;
+  v.nin = 6; v.nout = 0;
+  v.rout1 = vreal;
+  v.n_nint2 = (2 * (int64_t)a.K + 1) * a.np;
+  w = a;
+  w.nin = 9; w.nout = 9;
+  for (int f = 0; f < 9; ++f) {
+    w.in[f] = a.in[6 + f];
+    w.n_in[f] = a.n_in[6 + f];
+  }
+  w.creal = vreal;
+  w.n_creal = v.n_nint2;
+  w.coeff = 0.0;
+}
+
 // FFT_COMPUTE_ENTROPY_VISC_MOM: u V1, momentum V2, residual V3 (np, 6, M), mass
 // residual c (np, 2, M), hloc (np); real-space outputs (N, np).  The caller sets
 // the per-element arrays, pmax and the sizes.

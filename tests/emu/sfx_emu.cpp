@@ -391,6 +391,14 @@ extern "C" int emu_batch(int n, const int32_t* jj, const int64_t* jj_off, const 
 // ---- exact-grid products (non-polynomial FFT_PAR_*)
 #include "../../sfemans_amd/csrc/sfx_exact_setup.h"
 
+template <int OP, int NI>
+static void emu_exact_grid(const ExactArgs& a, double& vmax) {
+  constexpr int JC = exact_jc(OP, NI);
+  const int nch = (a.K + 1 + JC - 1) / JC;
+  for (int64_t p = 0; p < a.np; ++p)
+    for (int jc = 0; jc < nch; ++jc) exact_grid_body<OP, NI>(a, p, jc, vmax);
+}
+
 // FFT_COMPUTE_ENTROPY_VISC: V = [V1 .. V5] (5 x np*6*M), out = [O1 | O2 | O3];
 // the element maxima as exact_enorm_kernel computes them (32 elements and 16
 // angle pairs per work-group, one thread per point).
@@ -439,23 +447,18 @@ extern "C" int emu_exact_evisc(int64_t np, int M, int mp, int lg, const double* 
   a.n_W = 2 * (int64_t)a.nout * (a.K + 1) * np;
   std::vector<double> W((size_t)a.n_W + 1);
   a.W = W.data();
+  // as the library launches it: the viscosity on the grid, then the products
+  std::vector<double> vreal((size_t)N * np + 1);
+  ExactArgs v, w;
+  exact_evisc_split(a, vreal.data(), v, w);
   double vmax = 0.0;
-  constexpr int JC = exact_jc(EX_EVISC, 15);
-  for (int64_t p = 0; p < np; ++p)
-    for (int jc = 0; jc < (a.K + 1 + JC - 1) / JC; ++jc) exact_grid_body<EX_EVISC, 15>(a, p, jc, vmax);
+  emu_exact_grid<EX_EVISCV, 6>(v, vmax);
+  emu_exact_grid<EX_GRADMOM, 9>(w, vmax);
   const int nmc = (M + EXACT_MC - 1) / EXACT_MC;
   for (int64_t p = 0; p < np; ++p)
-    for (int g = 0; g < a.nout; ++g)
-      for (int mc = 0; mc < nmc; ++mc) exact_r2c_body(a, p, g, mc);
+    for (int g = 0; g < w.nout; ++g)
+      for (int mc = 0; mc < nmc; ++mc) exact_r2c_body(w, p, g, mc);
   return 0;
-}
-
-template <int OP, int NI>
-static void emu_exact_grid(const ExactArgs& a, double& vmax) {
-  constexpr int JC = exact_jc(OP, NI);
-  const int nch = (a.K + 1 + JC - 1) / JC;
-  for (int64_t p = 0; p < a.np; ++p)
-    for (int jc = 0; jc < nch; ++jc) exact_grid_body<OP, NI>(a, p, jc, vmax);
 }
 
 // op: 0 DIV (in1 c1, in2 c2), 1 SV2 (in1 V, in2 s), 2 SQRT (in1 c (np,nb,M), out a copy of c),
