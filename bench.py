@@ -285,7 +285,8 @@ def wl_entropy(torch, args):
          36 * b + 8 * N * npt, 18 * f_t * npt, "exact_grid_kernel<GRADMOM,9> + exact_r2c_kernel"),
         ("entropy_visc", lambda: FP.FFT_COMPUTE_ENTROPY_VISC(None, None, u, res, g1, g2, g3, h, *v6, 1.0, lg, 0.3,
                                                              0.9, 0.8),
-         48 * b + 8 * npt, 24 * f_t * npt, "exact_enorm_kernel + exact_grid_kernel<EVISC,15> + exact_r2c_kernel"),
+         48 * b + 8 * npt, 24 * f_t * npt,
+         "exact_enorm_kernel + exact_grid_kernel<EVISCV,6> + exact_grid_kernel<GRADMOM,9> + exact_r2c_kernel"),
         ("entropy_visc_mom", lambda: FP.FFT_COMPUTE_ENTROPY_VISC_MOM(None, None, u, mom, res, rmass, h, r1, lg, 0.6,
                                                                      0.9, opt_c2_real_out=r2),
          20 * b + 8 * npt + 16 * N * npt, 10 * f_t * npt, "exact_emom_kernel + exact_emom_out_kernel"),
