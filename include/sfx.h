@@ -238,6 +238,26 @@ int sfx_fft_compute_entropy_visc_grad_mom_dev(const double* V1_in, const double*
                                               const double* c_in_real, double les_coeff1_mom, double* V_out,
                                               int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_COMPRESSION_LEVEL_SET_DCL (fft_parallel.F90:3821): V1_in
+ * (np,6,m) the level-set gradient (components 1-3 used), V2_in (np,6,m) the
+ * velocity, c_in (np,2,m) the level set phi -> c_out (np,2,m), the
+ * coefficients of level_set_comp_factor*4*les_coeff4*min(1, max|v|)
+ * *(2 regul(phi, 0.01) - 1)*(max(1-(2phi-1)^2, 0)/(2 h_min_distance) - |grad phi|)
+ * (:3997-4006).  max|v| is taken over all points and angles (one rank): the
+ * element and angle maxima the reference takes first (:3960-3984) do not
+ * change it, so hloc_gauss and l_G are accepted and not read.
+ * level_set_comp_factor, les_coeff4 and h_min_distance are the module inputs
+ * the reference reads (inputs%...). */
+int sfx_fft_compression_level_set_dcl(const double* V1_in, const double* V2_in, const double* c_in, double* c_out,
+                                      const double* hloc_gauss, int32_t l_G, int64_t np, int32_t nb_field,
+                                      int32_t m_max_c, int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
+                                      double level_set_comp_factor, double les_coeff4, double h_min_distance,
+                                      double* temps);
+int sfx_fft_compression_level_set_dcl_dev(const double* V1_in, const double* V2_in, const double* c_in,
+                                          double* c_out, int64_t np, int32_t m_max_c, int32_t m_max_pad,
+                                          double level_set_comp_factor, double les_coeff4, double h_min_distance,
+                                          void* stream);
+
 /* Replaces FFT_COMPUTE_DIFFU_MOM (fft_parallel.F90:2939): V1_out = s V1 and
  * V2_out = s V2 with V1, V2 (np,6,m) vectors and s = V3 (np,2,m) a scalar;
  * band-limited products, computed by the product kernel. */

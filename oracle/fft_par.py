@@ -24,7 +24,8 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 ``fft_no_overshoot_level_set`` (:3649-3819), ``fft_par_entro_visc_dcl``
 (:4055-4292), ``fft_par_compr_entro_visc_dcl`` (:3160-3417) and
 ``fft_compute_entropy_visc_grad_mom`` (:3419-3647), ``fft_compute_entropy_visc``
-(:2418-2699), ``fft_compute_entropy_visc_mom`` (:2701-2937);
+(:2418-2699), ``fft_compute_entropy_visc_mom`` (:2701-2937),
+``fft_compression_level_set_dcl`` (:3821-4053);
 ``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
@@ -44,7 +45,7 @@ __all__ = [
     "fft_par_div_dcl", "fft_heaviside_dcl", "fft_squareroot_scalar", "fft_max_vel_dcl",
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
     "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom", "fft_par_compr_entro_visc_dcl",
-    "fft_compute_entropy_visc", "fft_compute_entropy_visc_mom",
+    "fft_compute_entropy_visc", "fft_compute_entropy_visc_mom", "fft_compression_level_set_dcl",
     "regul", "OPS",
 ]
 
@@ -457,6 +458,40 @@ def fft_par_compr_entro_visc_dcl(V1_in, V2_in, c1_in, c_in_real, coeff1_in_level
         outs_v.append(unpack(np.asarray(r2c(pv, m_max_pad, engine), dtype=np.complex128), N, M))
         outs_c.append(unpack(np.asarray(r2c(c2[:, None, :], m_max_pad, engine), dtype=np.complex128), N, M))
     return np.concatenate(outs_v, axis=0), np.concatenate(outs_c, axis=0)
+
+
+def fft_compression_level_set_dcl(V1_in, V2_in, c_in, l_G, level_set_comp_factor, les_coeff4, h_min_distance,
+                                  m_max_pad, engine="fft"):
+    """FFT_COMPRESSION_LEVEL_SET_DCL (fft_parallel.F90:3821-4053), one rank:
+    grad phi = V1 components 1-3, velocity = V2 components 1-3, phi = c.
+    |v| is maximised over each element (l_G consecutive points) and the angles
+    j-1, j, j+1 (:3960-3984; points past the last whole element keep |v|),
+    vmax = MIN(1, max over all) (:3987-3998); the product is
+    level_set_comp_factor*4*les_coeff4*vmax*(2 regul(phi, 0.01) - 1)
+    *(MAX(1-(2phi-1)^2, 0)/(2 h_min_distance) - |grad phi|) (:4001-4005)."""
+    V1 = np.asarray(V1_in, dtype=np.float64)
+    V2 = np.asarray(V2_in, dtype=np.float64)
+    c = np.asarray(c_in, dtype=np.float64)
+    npt, nb, M = V1.shape
+    _check(nb, M, "FFT_COMPRESSION_LEVEL_SET_DCL")
+    N = n_grid(m_max_pad)
+    nv = np.empty((npt, N))
+    for s0 in range(0, npt, _CHUNK):
+        sl = slice(s0, s0 + _CHUNK)
+        ru = np.asarray(c2r(pack(V2[sl, 0:6], m_max_pad), N, engine), dtype=np.float64)
+        nv[sl] = np.sqrt(np.sum(ru[:, 0:3] ** 2, axis=1))
+    ne = npt // l_G
+    if ne:
+        nv[:ne * l_G] = _element_angle_max(nv[:ne * l_G], l_G)
+    vmax = min(1.0, float(np.max(nv))) if npt else 1.0
+    coeff = level_set_comp_factor * 4 * les_coeff4 * vmax
+
+    def op(ru):
+        ng = np.sqrt(ru[:, 0] ** 2 + ru[:, 1] ** 2 + ru[:, 2] ** 2)
+        phi = ru[:, 3]
+        return (coeff * (2.0 * regul(phi, 0.01) - 1.0)
+                * (np.maximum(1.0 - (2 * phi - 1.0) ** 2, 0.0) / (2 * h_min_distance) - ng))[:, None]
+    return fft_product([V1[:, 0:6], c[:, 0:2]], op, m_max_pad, engine)
 
 
 def fft_compute_entropy_visc(V1_in, V2_in, V3_in, V4_in, V5_in, hloc_gauss, residual_normalization, l_G,

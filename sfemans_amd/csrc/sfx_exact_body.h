@@ -30,7 +30,7 @@ constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
                EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10,
-               EX_EVISC = 11, EX_EVISCV = 12 };
+               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -118,6 +118,13 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
 #pragma unroll
       for (int k = 0; k < 9; ++k) w[k] = ve * u[6 + k];
     }
+  } else if constexpr (OP == EX_CLS) {
+    // grad(phi) (0-2), phi (3); coeff = level_set_comp_factor * 4 * LES_coeff4,
+    // coeff2 = h_min_distance, dmax = max |v| over points and angles   :3997-4006
+    const double vm = fmin(1.0, a.dmax[0]);
+    const double ng = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double t = 2.0 * u[3] - 1.0;
+    w[0] = a.coeff * vm * (2.0 * exact_regul(u[3], 0.01) - 1.0) * (fmax(1.0 - t * t, 0.0) / (2.0 * a.coeff2) - ng);
   } else if constexpr (OP == EX_ENTRO) {
     const double v = a.coeff - x;                                 // -visc_entro, :4212-4219
 #pragma unroll

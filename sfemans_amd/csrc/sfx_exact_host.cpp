@@ -232,6 +232,21 @@ static int maxvel_dev(const double* V1, double* max_out, int64_t np, int nb, int
   return rc ? rc : exact_run("FFT_MAX_VEL_DCL", EX_MAXVEL, a, max_out, st);
 }
 
+static int cls_dev(const double* V1, const double* V2, const double* c, double* cout, int64_t np, int M, int mp,
+                   double comp, double c4, double hmin, hipStream_t st) {
+  const char* nm = "FFT_COMPRESSION_LEVEL_SET_DCL";
+  // max |velocity| over points and angles (the element and angle maxima the
+  // reference takes first do not change it)
+  void* dm = nullptr;
+  int rc = staging(SLOT_EX_MAX, 8, &dm);
+  if (rc) return rc;
+  if ((rc = maxvel_dev(V2, (double*)dm, np, 6, M, mp, st))) return rc;
+  ExactArgs a;
+  exact_setup_cls(a, V1, c, cout, (const double*)dm, np, M, comp, c4, hmin);
+  if ((rc = exact_prepare(nm, a, EX_CLS, np, M, mp))) return rc;
+  return exact_run(nm, EX_CLS, a, nullptr, st);
+}
+
 // ---- host-pointer paths: stage, run, copy back, synchronise
 struct HostRun {
   double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
@@ -607,6 +622,46 @@ int sfx_fft_compute_entropy_visc_mom(const double* V1_in, const double* V2_in, c
   t.t2 = now_s();
   if ((rc = from_dev(c1_real_out, o1, nr))) return rc;
   if (opt_c2_real_out && (rc = from_dev(opt_c2_real_out, o2, nr))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_compression_level_set_dcl_dev(const double* V1_in, const double* V2_in, const double* c_in,
+                                          double* c_out, int64_t np, int32_t m_max_c, int32_t m_max_pad,
+                                          double level_set_comp_factor, double les_coeff4, double h_min_distance,
+                                          void* stream) {
+  clear_err();
+  return cls_dev(V1_in, V2_in, c_in, c_out, np, m_max_c, m_max_pad, level_set_comp_factor, les_coeff4,
+                 h_min_distance, (hipStream_t)stream);
+}
+
+int sfx_fft_compression_level_set_dcl(const double* V1_in, const double* V2_in, const double* c_in, double* c_out,
+                                      const double* hloc_gauss, int32_t l_G, int64_t np, int32_t nb_field,
+                                      int32_t m_max_c, int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad,
+                                      double level_set_comp_factor, double les_coeff4, double h_min_distance,
+                                      double* temps) {
+  clear_err();
+  (void)hloc_gauss;   // read by the reference but not used by this product
+  (void)l_G;          // the element maxima it takes do not change the maximum used
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (nb_field != 6 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_COMPRESSION_LEVEL_SET_DCL ");
+  int rc = check_procs("FFT_COMPRESSION_LEVEL_SET_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  const size_t n1 = (size_t)np * 6 * m_max_c, n2 = (size_t)np * 2 * m_max_c;
+  void *d1, *d2, *dc, *o;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = to_dev(SLOT_EX_IN2, V2_in, n1, &d2)) ||
+      (rc = to_dev(SLOT_EX_IN4, c_in, n2, &dc)) || (rc = staging(SLOT_EX_OUT, n2 * 8, &o)))
+    return rc;
+  t.t1 = now_s();
+  if ((rc = cls_dev((const double*)d1, (const double*)d2, (const double*)dc, (double*)o, np, m_max_c, m_max_pad,
+                    level_set_comp_factor, les_coeff4, h_min_distance, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, o, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(c_out, o, n2))) return rc;
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;

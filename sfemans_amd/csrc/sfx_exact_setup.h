@@ -205,6 +205,25 @@ inline void exact_setup_emom(ExactArgs& a, const double* V1, const double* V2, c
   a.coeff2 = c2; a.coeff3 = c4;
 }
 
+// FFT_COMPRESSION_LEVEL_SET_DCL, product pass: grad(phi) V1 (np, 6, M), phi c
+// (np, 2, M) -> c_out (np, 2, M); dmax (device) the maximum of |velocity|.
+inline void exact_setup_cls(ExactArgs& a, const double* V1, const double* c, double* cout, const double* dmax,
+                            int64_t np, int M, double comp, double les_coeff4, double h_min_distance) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 4; a.nout = 1;
+  for (int k = 0; k < 3; ++k) {
+    a.in[k] = exact_col(V1, np, 6, k);
+    a.n_in[k] = np * 6 * (int64_t)M - 2 * k * np;
+  }
+  a.in[3] = exact_col(c, np, 2, 0);
+  a.n_in[3] = np * 2 * (int64_t)M;
+  a.out[0] = exact_colo(cout, np, 2, 0);
+  a.n_out[0] = np * 2 * (int64_t)M;
+  a.dmax = dmax;
+  a.coeff = comp * 4.0 * les_coeff4;
+  a.coeff2 = h_min_distance;
+}
+
 // FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
 // -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {

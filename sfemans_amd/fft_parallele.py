@@ -355,6 +355,31 @@ def FFT_PAR_COMPR_ENTRO_VISC_DCL(communicator, V1_in, V2_in, c1_in, c_in_real, h
     return V_out, c_out
 
 
+def FFT_COMPRESSION_LEVEL_SET_DCL(communicator_F, communicator_S, V1_in, V2_in, c_in, c_out, hloc_gauss, l_G,
+                                  level_set_comp_factor, les_coeff4, h_min_distance, nb_procs=1, bloc_size=None,
+                                  m_max_pad=None, temps=None):
+    """Level-set compression term (fft_parallel.F90:3821): V1_in the level-set
+    gradient, V2_in the velocity (np,6,m), c_in the level set (np,2,m) ->
+    c_out (np,2,m).  level_set_comp_factor, les_coeff4 and h_min_distance are
+    the reference's inputs%level_set_comp_factor, LES_coeff4, h_min_distance,
+    passed explicitly; hloc_gauss and l_G do not change the result."""
+    _single_rank(communicator_F)
+    _single_rank(communicator_S)
+    np_, nb, M = fdims(V1_in)
+    fdims(V2_in), fdims(c_in), fdims(c_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    args = (float(level_set_comp_factor), float(les_coeff4), float(h_min_distance))
+    if is_device(V1_in):
+        check(L.sfx_fft_compression_level_set_dcl_dev(ptr(V1_in), ptr(V2_in), ptr(c_in), ptr(c_out), np_, M,
+                                                      m_max_pad, *args, stream_ptr()))
+    else:
+        check(L.sfx_fft_compression_level_set_dcl(ptr(V1_in), ptr(V2_in), ptr(c_in), ptr(c_out), None, int(l_G),
+                                                  np_, nb, M, nb_procs, bloc_size, m_max_pad, *args,
+                                                  _temps_ptr(temps)))
+    return c_out
+
+
 def FFT_COMPUTE_ENTROPY_VISC(communicator, communicator_S, V1_in, V2_in, V3_in, V4_in, V5_in, hloc_gauss,
                              V1_out, V2_out, V3_out, residual_normalization, l_G, les_coeff1, les_coeff2,
                              les_coeff4, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):

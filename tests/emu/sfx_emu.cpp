@@ -465,10 +465,12 @@ extern "C" int emu_exact_evisc(int64_t np, int M, int mp, int lg, const double* 
 // 3 HEAVISIDE (in1 V1 (n_int,np,2,M), values), 4 TENSION (in1, in2 (np,6,M), out (3,np,6,M)),
 // 5 MAX_VEL (in1 (np,nb,M), out[0] = maximum), 6 SCALAR_VECT_NO_OVERSHOOT (in1 V, in2 s,
 // nb = pb, values = {min, max}), 7 NO_OVERSHOOT_LEVEL_SET (as 2), 8 ENTRO_VISC (in1 V,
-// in2 = [c (np,2,M) | c_in_real (N,np)], values = {coeff}, out = [V_out | c_out]).
+// in2 = [c (np,2,M) | c_in_real (N,np)], values = {coeff}, out = [V_out | c_out]); 9-10, 13
+// below.
 extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, const double* in2,
                          const double* values, int n_values, double eps, int nb, double* out) {
   ExactArgs a;
+  double cls_vmax = 0.0;
   switch (op) {
     case EX_DIV: exact_setup_div(a, in1, in2, out, np, M); break;
     case EX_SV2: exact_setup_sv2(a, in1, in2, out, np, M); break;
@@ -489,6 +491,13 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
       exact_setup_compr(a, in1, in1 + 6 * np * M, in2, in2 + 2 * np * M, values[0], values[1], out,
                         out + 6 * np * M, np, M, mp);
       break;
+    case EX_CLS: {   // in1 = [grad phi | velocity] (2 x np*6*M), in2 = phi (np,2,M),
+                     // values = {level_set_comp_factor, LES_coeff4, h_min_distance}
+      const int rc = emu_exact(EX_MAXVEL, np, M, mp, in1 + 6 * np * M, nullptr, nullptr, 0, 0.0, 6, &cls_vmax);
+      if (rc) return rc;
+      exact_setup_cls(a, in1, in2, out, &cls_vmax, np, M, values[0], values[1], values[2]);
+      break;
+    }
     default: return 1;
   }
   exact_setup_sizes(a, np, M, mp);
@@ -510,6 +519,7 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_ENTRO: emu_exact_grid<EX_ENTRO, 4>(a, vmax); break;
     case EX_GRADMOM: emu_exact_grid<EX_GRADMOM, 9>(a, vmax); break;
     case EX_COMPR: emu_exact_grid<EX_COMPR, 7>(a, vmax); break;
+    case EX_CLS: emu_exact_grid<EX_CLS, 4>(a, vmax); break;
     case EX_HEAVI:
       if (a.nin == 1) emu_exact_grid<EX_HEAVI, 1>(a, vmax);
       else if (a.nin == 2) emu_exact_grid<EX_HEAVI, 2>(a, vmax);

@@ -203,3 +203,18 @@ def test_entropy_visc_mom(ne, M, mp, lg, c2out):
     assert rel(arrs[3 if not c2out else -2], e1) <= 1e-12
     if c2out:
         assert rel(arrs[-1], e2) <= 1e-12
+
+
+EX_CLS = 13
+
+
+@pytest.mark.parametrize("npt,M,mp,vs", [(9, 8, 12, 0.3), (5, 16, 24, 3.0), (3, 64, 96, 0.3)])
+def test_compression_level_set(npt, M, mp, vs):
+    rng = np.random.default_rng(103 + M)
+    G, V = fields(rng, npt, 6, M), fields(rng, npt, 6, M, vs)
+    c = fields(rng, npt, 2, M, 0.2)
+    c[:, 0, 0] = 0.5
+    in1 = np.concatenate([G.ravel(order="F"), V.ravel(order="F")])
+    vals = [1.3, 0.7, 0.05]
+    out = emu(EX_CLS, npt, M, mp, in1, c, values=vals, out=np.zeros((npt, 2, M), order="F"))
+    assert rel(out, F.fft_compression_level_set_dcl(G, V, c, 1, *vals, mp)) <= 1e-12

@@ -236,6 +236,26 @@ def test_compression_entropy_viscosity(npt, M):
     assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
 
 
+@pytest.mark.parametrize("npt,M,vs", [(500, 8, 0.3), (300, 64, 3.0), (30000, 4, 0.3)])
+def test_compression_level_set(npt, M, vs):
+    print(f"case compression_level_set np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(103 + M)
+    G, V = fields(rng, npt, 6, M), fields(rng, npt, 6, M, vs)
+    c = fields(rng, npt, 2, M, 0.2)
+    c[:, 0, 0] = 0.5
+    vals = (1.3, 0.7, 0.05)
+    exp = F.fft_compression_level_set_dcl(G, V, c, 7, *vals, 3 * M // 2)
+    h = np.ones(npt)
+    out = np.zeros((npt, 2, M), order="F")
+    FP.FFT_COMPRESSION_LEVEL_SET_DCL(None, None, G, V, c, out, h, 7, *vals)
+    assert rel(out, exp) <= TOL
+    dout = FP.fortran_empty((npt, 2, M))
+    FP.FFT_COMPRESSION_LEVEL_SET_DCL(None, None, FP.fortran_tensor(G), FP.fortran_tensor(V), FP.fortran_tensor(c),
+                                     dout, None, 7, *vals)
+    torch.cuda.synchronize()
+    assert rel(dout.cpu().numpy(), exp) <= TOL
+
+
 @pytest.mark.parametrize("ne,M", [(300, 8), (40, 64), (7000, 4)])
 def test_entropy_visc(ne, M):
     print(f"case entropy_visc elements={ne} M={M}", flush=True)

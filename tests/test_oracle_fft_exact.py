@@ -427,3 +427,69 @@ def test_entropy_visc_mom_engines_agree():
     a = F.fft_compute_entropy_visc_mom(*Vs, C, h, lg, 0.6, 0.9, mp, True)
     b = F.fft_compute_entropy_visc_mom(*Vs, C, h, lg, 0.6, 0.9, mp, True, "direct")
     assert rel(a[0], b[0]) <= 1e-12 and rel(a[1], b[1]) <= 1e-12
+
+
+@pytest.mark.parametrize("vscale", [1.0, 5.0])
+def test_compression_level_set_for_fields_constant_in_angle(vscale):
+    """phi, grad phi constant in angle; velocity v_r = b + c cos t (b, c >= 0),
+    so max |v| = max(b + c), reached at t = 0, a grid angle; the last point sits
+    past the last whole element.  Per point the product is the constant
+    K min(1, vmax) s (MAX(1-(2phi-1)^2, 0)/(2h) - |g|), s = 2 regul(phi) - 1:
+    s = 1 above 0.51, -1 below 0.49, 0 at 0.5 and 11/16 at 0.505."""
+    M, mp, lg = 6, 9, 3
+    phi = np.array([0.8, 0.2, 0.505, 1.3, 0.5, 0.3, 0.9])
+    s = np.array([1.0, -1.0, 11 / 16, 1.0, 0.0, -1.0, 1.0])
+    npt = phi.size
+    rng = np.random.default_rng(23)
+    g = rng.standard_normal((npt, 3))
+    G = np.zeros((npt, 6, M)); G[:, 0::2, 0] = g
+    C = np.zeros((npt, 2, M)); C[:, 0, 0] = phi
+    b = np.array([0.1, 0.2, 0.05, 0.0, 0.3, 0.1, 0.4]) * vscale
+    c = np.array([0.2, 0.1, 0.0, 0.1, 0.2, 0.0, 0.3]) * vscale
+    V = np.zeros((npt, 6, M)); V[:, 0, 0] = b; V[:, 0, 1] = c
+    vmax = min(1.0, float(np.max(b + c)))
+    comp, c4, h = 1.3, 0.7, 0.05
+    out = F.fft_compression_level_set_dcl(G, V, C, lg, comp, c4, h, mp)
+    q = np.maximum(1 - (2 * phi - 1) ** 2, 0.0)
+    ex = np.zeros((npt, 2, M))
+    ex[:, 0, 0] = comp * 4 * c4 * vmax * s * (q / (2 * h) - np.sqrt(np.sum(g ** 2, axis=1)))
+    assert rel(out, ex) <= 1e-13
+
+
+@pytest.mark.parametrize("mp", [12, 16])
+def test_compression_level_set_step_by_explicit_sums(mp):
+    """phi = 0.5 + 0.3 cos t: no grid angle has |0.3 cos t_j| <= 0.01, so
+    s_j = sign(cos t_j) exactly; grad phi = (a cos t, a sin t, 0) has |g| = |a|;
+    v = (0.4 + 0.3 cos t, 0, 0) gives vmax = 0.7.  The grid values are
+    f_j = K 0.7 s_j ((1 - 0.36 cos^2 t_j)/(2h) - |a|); the coefficients are
+    the sums a_0 = sum f_j / N, a_m = 2 sum f_j cos(m t_j) / N,
+    b_m = 2 sum f_j sin(m t_j) / N."""
+    N = 2 * mp - 1
+    M = 8
+    t = 2 * np.pi * np.arange(N) / N
+    assert np.min(np.abs(0.3 * np.cos(t))) > 0.01
+    a_ = -0.6
+    G = np.zeros((2, 6, M)); G[:, 0, 1] = a_; G[:, 3, 1] = a_
+    C = np.zeros((2, 2, M)); C[:, 0, 0] = 0.5; C[:, 0, 1] = 0.3
+    V = np.zeros((2, 6, M)); V[:, 0, 0] = 0.4; V[:, 0, 1] = 0.3
+    comp, c4, h = 2.0, 0.25, 0.1
+    out = F.fft_compression_level_set_dcl(G, V, C, 2, comp, c4, h, mp)
+    f = comp * 4 * c4 * 0.7 * np.sign(np.cos(t)) * ((1 - 0.36 * np.cos(t) ** 2) / (2 * h) - abs(a_))
+    m = np.arange(M)[:, None]
+    ex = np.zeros((2, 2, M))
+    ex[:, 0] = (2.0 / N) * (np.cos(m * t) @ f)
+    ex[:, 0, 0] /= 2
+    ex[:, 1] = (2.0 / N) * (np.sin(m * t) @ f)
+    ex[:, 1, 0] = 0.0
+    assert rel(out, ex) <= 1e-13
+
+
+def test_compression_level_set_engines_agree():
+    rng = np.random.default_rng(24)
+    M, mp, npt = 6, 9, 10
+    G, V = rng.standard_normal((npt, 6, M)), rng.standard_normal((npt, 6, M))
+    C = rng.standard_normal((npt, 2, M)) * 0.2
+    C[:, 0, 0] = 0.5
+    a = F.fft_compression_level_set_dcl(G, V, C, 2, 1.1, 0.6, 0.07, mp)
+    b = F.fft_compression_level_set_dcl(G, V, C, 2, 1.1, 0.6, 0.07, mp, "direct")
+    assert rel(a, b) <= 1e-12
