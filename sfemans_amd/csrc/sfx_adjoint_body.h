@@ -51,13 +51,20 @@ struct AdjLds {
   static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
   static constexpr int NODAL = 2 * LY::NOD6;                              // doubles from BIG
   static constexpr int XCH = NODAL;                                       // first wave region
+  // Element sums by local node (work item: element, node, mode offset)
+  // for the convective form on 7 waves, else by component (element,
+  // component, mode offset); measured faster each where it is used.
+  static constexpr bool BY_NODE = !SPLIT && NW > 4;
   // red[i * SI + k * SK] for entry i = local point * L + mode offset and
-  // weight k: entry-major for the convective form, weight-major for the
-  // split form (its 24 weights per entry would conflict on LDS banks)
+  // weight k.  By node: entry-major (an item reads the 18 weights of its
+  // entries).  By component: weight-major, a wave writes 64 consecutive
+  // entries of one weight and reads 16 consecutive entries of each of 4
+  // weights SK apart; the 16-double pad puts those groups in different
+  // bank halves.
   static constexpr int RPL = 7 * LY::E * (MP / 4);
-  static constexpr int RED = RPL * AdjRed<SPLIT>::K;
-  static constexpr int SI = SPLIT ? 1 : AdjRed<SPLIT>::K;
-  static constexpr int SK = SPLIT ? RPL : 1;
+  static constexpr int SI = BY_NODE ? AdjRed<SPLIT>::K : 1;
+  static constexpr int SK = BY_NODE ? 1 : RPL + 16;
+  static constexpr int RED = BY_NODE ? RPL * AdjRed<SPLIT>::K : SK * AdjRed<SPLIT>::K;
   // the reduction area aliases the staging and wave regions (used after them)
   static constexpr int BIGD = (NODAL + NW * RW > RED) ? NODAL + NW * RW : RED;
   static constexpr int TOTAL = LY::BIG + (BIGD + 1) / 2;   // double2 units
@@ -356,53 +363,31 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       }
     }
     SFX_SYNC();
-    // Split form: work item (element, component k, mode offset mi) for the
-    // six local nodes of the element, so the item reads its 28 weights from
-    // LDS once for all six nodes.  Convective form: work item (element,
-    // local node n, mode offset mi) for the six components (its 18 weights
-    // per point are fewer LDS reads than the shape tables of six nodes).
-    // Either way each sum runs over the 7 Gauss points in order, and mode is
-    // fastest: consecutive lanes store consecutive modes of one entry (elo
-    // layout: entry, component, mode).
-    for (int it = tid; it < E * 6 * L; it += NT) {
-      const int mi = it % L;
-      const int nk = (it / L) % 6;
-      const int e = it / (6 * L);
-      const int mm = mi + L * r;
-      const int64_t eg = e0 + e;
-      if (eg >= a.me || mm >= a.M) continue;
-      double acc[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
-      constexpr int RP = AL::SK;
-      if constexpr (SPLIT) {
-        const int k = nk;
-#pragma unroll
+    // By component: work item (element, component k, mode offset mi) for
+    // the six local nodes of the element: the item reads its weights of
+    // component k from LDS once for all six nodes, and the shape tables
+    // (uniform across the wave) come through the scalar cache into scalar
+    // registers, not from LDS.  By node: work item (element, local node n,
+    // mode offset mi) for the six components, tables from LDS.  Either way
+    // each sum runs over the 7 Gauss points in order, and mode is fastest:
+    // consecutive lanes store consecutive modes of one entry (elo layout:
+    // entry, component, mode).
+    if constexpr (AL::BY_NODE) {
+      for (int it = tid; it < E * 6 * L; it += NT) {
+        const int mi = it % L;
+        const int n = (it / L) % 6;
+        const int e = it / (6 * L);
+        const int mm = mi + L * r;
+        const int64_t eg = e0 + e;
+        if (eg >= a.me || mm >= a.M) continue;
+        double acc[6] = {0, 0, 0, 0, 0, 0};
         for (int lg2 = 0; lg2 < 7; ++lg2) {
           const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
-          double v[4];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) v[j] = src[(4 * k + j) * RP];
-#pragma unroll
-          for (int n = 0; n < 6; ++n) {
-            const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
-            acc[n] = fma(w, v[0], acc[n]);
-            acc2[n] = fma(w, v[1], fma(dx, v[2], fma(dy, v[3], acc2[n])));
-          }
-        }
-#pragma unroll
-        for (int n = 0; n < 6; ++n) {
-          const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
-          SFX_CHECK("elo store", o, A.nelo);
-          A.elo[o] = acc[n];
-          A.elo2[o] = acc2[n];
-        }
-      } else {
-        const int n = nk;
-        for (int lg2 = 0; lg2 < 7; ++lg2) {
-          const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
+          SFX_CHECK("red read", ((7 * e + lg2) * L + mi) * AL::SI + KR - 1, AL::RED);
           const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
 #pragma unroll
           for (int k = 0; k < 6; ++k)
-            acc[k] = fma(w, src[(3 * k) * RP], fma(dx, src[(3 * k + 1) * RP], fma(dy, src[(3 * k + 2) * RP], acc[k])));
+            acc[k] = fma(w, src[3 * k], fma(dx, src[3 * k + 1], fma(dy, src[3 * k + 2], acc[k])));
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -410,6 +395,52 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
           SFX_CHECK("elo store", o, A.nelo);
           A.elo[o] = acc[k];
         }
+      }
+      continue;
+    }
+    for (int it = tid; it < E * 6 * L; it += NT) {
+      const int mi = it % L;
+      const int k = (it / L) % 6;
+      const int e = it / (6 * L);
+      const int mm = mi + L * r;
+      const int64_t eg = e0 + e;
+      if (eg >= a.me || mm >= a.M) continue;
+      double acc[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
+      constexpr int RP = AL::SK;
+      constexpr int KW = SPLIT ? 4 : 3;
+      // the shape tables (global memory, read-only here) through a constant
+      // address-space pointer: uniform indices make them scalar loads; the
+      // pointer is redefined here so the 126 values are loaded in this loop
+      // rather than hoisted out of it (they would not fit in scalar registers)
+      const SFX_KARG double* tb = (const SFX_KARG double*)a.tab;
+      SFX_OPAQUE_S(tb);
+#pragma unroll
+      for (int lg2 = 0; lg2 < 7; ++lg2) {
+        const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
+        double v[KW];
+#pragma unroll
+        for (int j = 0; j < KW; ++j) {
+          SFX_CHECK("red read", (KW * k + j) * RP + ((7 * e + lg2) * L + mi) * AL::SI, AL::RED);
+          v[j] = src[(KW * k + j) * RP];
+        }
+#pragma unroll
+        for (int n = 0; n < 6; ++n) {
+          const double w = tb[n + 6 * lg2], dx = tb[42 + 2 * (n + 6 * lg2)];
+          const double dy = tb[42 + 2 * (n + 6 * lg2) + 1];
+          if constexpr (SPLIT) {
+            acc[n] = fma(w, v[0], acc[n]);
+            acc2[n] = fma(w, v[1], fma(dx, v[2], fma(dy, v[3], acc2[n])));
+          } else {
+            acc[n] = fma(w, v[0], fma(dx, v[1], fma(dy, v[2], acc[n])));
+          }
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < 6; ++n) {
+        const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
+        SFX_CHECK("elo store", o, A.nelo);
+        A.elo[o] = acc[n];
+        if (SPLIT) A.elo2[o] = acc2[n];
       }
     }
   }

@@ -21,6 +21,8 @@ void sfx_emu_wave_sync();
 #define SFX_SYNC() sfx_emu_sync()
 #define SFX_WAVE_SYNC() sfx_emu_wave_sync()
 #define SFX_OPAQUE(x) do { } while (0)
+#define SFX_OPAQUE_S(x) do { } while (0)
+#define SFX_KARG
 #define SFX_SCHED_BARRIER() do { } while (0)
 #define SFX_CHECK(what, idx, n) \
   do { if ((long long)(idx) < 0 || (long long)(idx) >= (long long)(n)) sfx_emu_oob(what, (long long)(idx), (long long)(n), __FILE__, __LINE__); } while (0)
@@ -43,6 +45,12 @@ void sfx_emu_wave_sync();
 // is recomputed where it is used instead of being kept live (or spilled)
 // across a long stretch of code.  Emits no instruction.
 #define SFX_OPAQUE(x) __asm__ volatile("" : "+v"(x))
+// The same for a wave-uniform value kept in scalar registers (a pointer
+// read through redefined at this point: loads through it are not hoisted
+// above it).
+#define SFX_OPAQUE_S(x) __asm__ volatile("" : "+s"(x))
+// Address space of the kernel arguments: uniform loads from it are scalar loads.
+#define SFX_KARG __attribute__((address_space(4)))
 // Keep the instruction scheduler from moving code across this point (for
 // example loads hoisted above it, whose destinations would be live early).
 #define SFX_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
