@@ -238,6 +238,20 @@ int sfx_fft_compute_entropy_visc_grad_mom_dev(const double* V1_in, const double*
                                               const double* c_in_real, double les_coeff1_mom, double* V_out,
                                               int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_PAR_REAL (fft_parallel.F90:28): V1_in (np,nb_field,m_max_c)
+ * -> V_out(N, nb_field/2, np), column-major, the values of each (cos, sin)
+ * column pair on the N grid angles 2*pi*j/N (the synthesis of
+ * :98-140).  N = sfx_fft_par_real_n(m_max_c, opt_nb_plane) = 2*m_max_pad-1
+ * with m_max_pad = (opt_nb_plane+1)/2 when opt_nb_plane > 2*m_max_c-1, else
+ * m_max_c (:76-84; opt_nb_plane <= 0 when absent).  The reference allocates
+ * V_out itself; here the caller passes N*(nb_field/2)*np doubles.  nb_procs
+ * must be 1 (the MPI_ALLTOALL is not implemented). */
+int32_t sfx_fft_par_real_n(int32_t m_max_c, int32_t opt_nb_plane);
+int sfx_fft_par_real(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                     int32_t nb_procs, int32_t opt_nb_plane);
+int sfx_fft_par_real_dev(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                         int32_t opt_nb_plane, void* stream);
+
 /* Replaces FFT_COMPRESSION_LEVEL_SET_DCL (fft_parallel.F90:3821): V1_in
  * (np,6,m) the level-set gradient (components 1-3 used), V2_in (np,6,m) the
  * velocity, c_in (np,2,m) the level set phi -> c_out (np,2,m), the

@@ -63,6 +63,9 @@ SIGNATURES = {
     "sfx_fft_compute_entropy_visc_grad_mom": [_vp, _vp, _vp, _vp, _d, _vp, _i64, _i32, _i32, _i32, _i64, _i32,
                                               _vp],
     "sfx_fft_compute_entropy_visc_grad_mom_dev": [_vp, _vp, _vp, _vp, _d, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_par_real_n": [_i32, _i32],
+    "sfx_fft_par_real": [_vp, _vp, _i64, _i32, _i32, _i32, _i32],
+    "sfx_fft_par_real_dev": [_vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_compression_level_set_dcl": [_vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i32, _i32, _i64, _i32, _d,
                                           _d, _d, _vp],
     "sfx_fft_compression_level_set_dcl_dev": [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _d, _d, _d, _vp],

@@ -202,6 +202,7 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
     case EX_GRADMOM: e = grid_launch<EX_GRADMOM, 9>(a, st, &nblk); break;
     case EX_COMPR: e = grid_launch<EX_COMPR, 7>(a, st, &nblk); break;
     case EX_CLS: e = grid_launch<EX_CLS, 4>(a, st, &nblk); break;
+    case EX_REAL: e = grid_launch<EX_REAL, 1>(a, st, &nblk); break;
     case EX_EVISC: {
       // element maxima of |v| first, then the products
       const int64_t me = a.np / a.lg;

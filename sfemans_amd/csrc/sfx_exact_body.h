@@ -30,7 +30,7 @@ constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
                EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10,
-               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13 };
+               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13, EX_REAL = 14 };
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -39,7 +39,7 @@ constexpr int exact_jc(int op, int nin) {
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
-  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : (op == EX_MAXVEL || op == EX_EVISCV) ? 0
+  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : (op == EX_MAXVEL || op == EX_EVISCV || op == EX_REAL) ? 0
          : (op == EX_ENTRO || op == EX_COMPR) ? 4 : (op == EX_GRADMOM || op == EX_EVISC) ? 9 : 1;
 }
 
@@ -90,7 +90,8 @@ struct ExactArgs {
   double* rout1;        // c1_real_out (N, np) at [p*N + j]
   double* rout2;        // opt_c2_real_out or null
   const double* dmax;   // device scalar: the maximum of |v| over points and angles
-  int64_t n_nint2;      // EX_EVISCV: size of rout1
+  int64_t n_nint2;      // EX_EVISCV, EX_REAL: size of rout1
+  int64_t rstride;      // EX_REAL: doubles between the points of rout1
   int64_t n_in[EXACT_MAX_IN], n_out[EXACT_MAX_OUT];   // bounds, emulator checks
 };
 
@@ -125,6 +126,8 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
     const double ng = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     const double t = 2.0 * u[3] - 1.0;
     w[0] = a.coeff * vm * (2.0 * exact_regul(u[3], 0.01) - 1.0) * (fmax(1.0 - t * t, 0.0) / (2.0 * a.coeff2) - ng);
+  } else if constexpr (OP == EX_REAL) {
+    w[0] = u[0];
   } else if constexpr (OP == EX_ENTRO) {
     const double v = a.coeff - x;                                 // -visc_entro, :4212-4219
 #pragma unroll
@@ -259,6 +262,12 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
     if constexpr (OP == EX_MAXVEL) {
       vmax = fmax(vmax, wp[0]);
       if (j > 0) vmax = fmax(vmax, wm[0]);
+    } else if constexpr (OP == EX_REAL) {
+      // real planes: the value at angle j of point p at rout1[p*rstride + j]
+      const int64_t N = 2 * a.K + 1;
+      SFX_CHECK("exact real", p * a.rstride + (j > 0 ? N - j : 0), a.n_nint2);
+      a.rout1[p * a.rstride + j] = wp[0];
+      if (j > 0) a.rout1[p * a.rstride + (N - j)] = wm[0];
     } else if constexpr (OP == EX_EVISCV) {
       // real-space output (N, np) at [p*N + angle]
       const int64_t N = 2 * a.K + 1;

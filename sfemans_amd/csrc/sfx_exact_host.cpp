@@ -247,6 +247,20 @@ static int cls_dev(const double* V1, const double* V2, const double* c, double* 
   return exact_run(nm, EX_CLS, a, nullptr, st);
 }
 
+// V (np, nb, M) -> real planes out(N, nb/2, np), N = 2*mp - 1: one pass per field
+static int real_dev(const double* V, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
+  const char* nm = "FFT_PAR_REAL";
+  if (nb % 2 != 0 || M == 0) return fail(SFX_ERR_ARG, "Bug in FFT_PAR_REAL: MOD(nb_field,2)/=0 .OR. m_max_c==0");
+  const int N = 2 * mp - 1;
+  for (int f = 0; f < nb / 2; ++f) {
+    ExactArgs a;
+    exact_setup_real(a, V, f, nb, out, np, M, N);
+    int rc = exact_prepare(nm, a, EX_REAL, np, M, mp);
+    if (rc || (rc = exact_run(nm, EX_REAL, a, nullptr, st))) return rc;
+  }
+  return SFX_OK;
+}
+
 // ---- host-pointer paths: stage, run, copy back, synchronise
 struct HostRun {
   double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
@@ -730,6 +744,36 @@ int sfx_fft_max_vel_dcl(const double* V1_in, double* V_out, int64_t np, int32_t 
     return rc;
   if ((rc = maxvel_dev((const double*)d1, (double*)dm, np, nb_field, m_max_c, m_max_pad, nullptr))) return rc;
   return from_dev(V_out, dm, 1);
+}
+
+int32_t sfx_fft_par_real_n(int32_t m_max_c, int32_t opt_nb_plane) {
+  // fft_parallel.F90:76-84 with nb_procs = 1 (m_max = m_max_c)
+  const int32_t mp = (opt_nb_plane > 2 * m_max_c - 1) ? (opt_nb_plane + 1) / 2 : m_max_c;
+  return 2 * mp - 1;
+}
+
+int sfx_fft_par_real_dev(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                         int32_t opt_nb_plane, void* stream) {
+  clear_err();
+  const int N = sfx_fft_par_real_n(m_max_c, opt_nb_plane);
+  return real_dev(V1_in, V_out, np, nb_field, m_max_c, (N + 1) / 2, (hipStream_t)stream);
+}
+
+int sfx_fft_par_real(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                     int32_t nb_procs, int32_t opt_nb_plane) {
+  clear_err();
+  if (nb_field % 2 != 0 || m_max_c == 0)
+    return fail(SFX_ERR_ARG, "Bug in FFT_PAR_REAL: MOD(nb_field,2)/=0 .OR. m_max_c==0");
+  if (nb_procs != 1)
+    return fail(SFX_ERR_UNSUPPORTED, "FFT_PAR_REAL: nb_procs=%d (one rank: no MPI_ALLTOALL)", nb_procs);
+  if (!V_out) return fail(SFX_ERR_ARG, "FFT_PAR_REAL: null V_out");
+  const int N = sfx_fft_par_real_n(m_max_c, opt_nb_plane);
+  const size_t nin = (size_t)np * nb_field * m_max_c, nout = (size_t)N * (nb_field / 2) * np;
+  void *d1, *o;
+  int rc;
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, nin, &d1)) || (rc = staging(SLOT_EX_OUT, nout * 8 + 8, &o))) return rc;
+  if ((rc = real_dev((const double*)d1, (double*)o, np, nb_field, m_max_c, (N + 1) / 2, nullptr))) return rc;
+  return from_dev(V_out, o, nout);
 }
 
 }  // extern "C"

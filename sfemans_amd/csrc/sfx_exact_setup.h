@@ -224,6 +224,18 @@ inline void exact_setup_cls(ExactArgs& a, const double* V1, const double* c, dou
   a.coeff2 = h_min_distance;
 }
 
+// FFT_PAR_REAL, field f of V (np, nb, M) -> its values on the N = 2K+1 grid
+// angles, out(N, nb/2, np) column-major: out[j + N*(f + nb/2*p)].
+inline void exact_setup_real(ExactArgs& a, const double* V, int f, int nb, double* out, int64_t np, int M, int N) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 1; a.nout = 0;
+  a.in[0] = exact_col(V, np, nb, f);
+  a.n_in[0] = np * nb * (int64_t)M - 2 * (int64_t)f * np;
+  a.rout1 = out + (int64_t)N * f;
+  a.rstride = (int64_t)N * (nb / 2);
+  a.n_nint2 = a.rstride * np - (int64_t)N * f;
+}
+
 // FFT_SQUAREROOT_SCALAR and FFT_NO_OVERSHOOT_LEVEL_SET: first scalar of c (np, nb, M)
 // -> first scalar of out (np, nb, M).
 inline void exact_setup_sqrt(ExactArgs& a, const double* c, double* out, int64_t np, int nb, int M) {

@@ -355,6 +355,33 @@ def FFT_PAR_COMPR_ENTRO_VISC_DCL(communicator, V1_in, V2_in, c1_in, c_in_real, h
     return V_out, c_out
 
 
+def FFT_PAR_REAL(communicator, V1_in, V_out=None, opt_nb_plane=None):
+    """Real planes of a Fourier field (fft_parallel.F90:28): V1_in (np, nb_field,
+    m_max_c) -> V_out (N, nb_field/2, np), the values on the N angles 2 pi j/N,
+    N = 2*m_max_pad - 1 with m_max_pad = (opt_nb_plane+1)//2 when opt_nb_plane
+    > 2*m_max_c - 1, else m_max_c.  The reference allocates V_out; here it is
+    allocated when not given (on the device of V1_in) and returned."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    L = lib()
+    onp = 0 if opt_nb_plane is None else int(opt_nb_plane)
+    N = L.sfx_fft_par_real_n(M, onp)
+    shape = (N, nb // 2, np_)
+    if is_device(V1_in):
+        if V_out is None:
+            V_out = fortran_empty(shape)
+        if tuple(fdims(V_out)) != shape:
+            raise ValueError(f"V_out must be {shape}")
+        check(L.sfx_fft_par_real_dev(ptr(V1_in), ptr(V_out), np_, nb, M, onp, stream_ptr()))
+    else:
+        if V_out is None:
+            V_out = np.zeros(shape, order="F")
+        if tuple(fdims(V_out)) != shape:
+            raise ValueError(f"V_out must be {shape}")
+        check(L.sfx_fft_par_real(ptr(V1_in), ptr(V_out), np_, nb, M, 1, onp))
+    return V_out
+
+
 def FFT_COMPRESSION_LEVEL_SET_DCL(communicator_F, communicator_S, V1_in, V2_in, c_in, c_out, hloc_gauss, l_G,
                                   level_set_comp_factor, les_coeff4, h_min_distance, nb_procs=1, bloc_size=None,
                                   m_max_pad=None, temps=None):

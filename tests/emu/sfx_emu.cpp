@@ -471,6 +471,19 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
                          const double* values, int n_values, double eps, int nb, double* out) {
   ExactArgs a;
   double cls_vmax = 0.0;
+  if (op == EX_REAL) {   // 14 FFT_PAR_REAL: in1 = V (np, nb, M), out = (N, nb/2, np), N = 2mp-1
+    std::vector<double> tab;
+    for (int f = 0; f < nb / 2; ++f) {
+      exact_setup_real(a, in1, f, nb, out, np, M, 2 * mp - 1);
+      exact_setup_sizes(a, np, M, mp);
+      if (f == 0) exact_host_tables(M, a.K, tab);
+      a.ctab = tab.data();
+      a.rtab = tab.data() + exact_ctab_size(M, a.K);
+      double vmax = 0.0;
+      emu_exact_grid<EX_REAL, 1>(a, vmax);
+    }
+    return 0;
+  }
   switch (op) {
     case EX_DIV: exact_setup_div(a, in1, in2, out, np, M); break;
     case EX_SV2: exact_setup_sv2(a, in1, in2, out, np, M); break;

@@ -218,3 +218,15 @@ def test_compression_level_set(npt, M, mp, vs):
     vals = [1.3, 0.7, 0.05]
     out = emu(EX_CLS, npt, M, mp, in1, c, values=vals, out=np.zeros((npt, 2, M), order="F"))
     assert rel(out, F.fft_compression_level_set_dcl(G, V, c, 1, *vals, mp)) <= 1e-12
+
+
+EX_REAL = 14
+
+
+@pytest.mark.parametrize("npt,M,mp,nb", [(9, 8, 8, 6), (5, 16, 40, 2), (3, 1, 1, 4), (4, 64, 64, 2)])
+def test_fft_par_real(npt, M, mp, nb):
+    rng = np.random.default_rng(107 + M)
+    V = fields(rng, npt, nb, M)
+    N = 2 * mp - 1
+    out = emu(EX_REAL, npt, M, mp, V, nb=nb, out=np.zeros((N, nb // 2, npt), order="F"))
+    assert rel(out, F.fft_par_real(V, mp)) <= 1e-12

@@ -236,6 +236,21 @@ def test_compression_entropy_viscosity(npt, M):
     assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
 
 
+@pytest.mark.parametrize("npt,M,nb,onp", [(700, 8, 6, None), (500, 64, 2, None), (300, 16, 4, 95),
+                                          (20000, 4, 6, 7), (2, 3, 2, 1)])
+def test_fft_par_real(npt, M, nb, onp):
+    print(f"case fft_par_real np={npt} M={M} nb={nb} opt_nb_plane={onp}", flush=True)
+    rng = np.random.default_rng(107 + M)
+    V = fields(rng, npt, nb, M)
+    mp = (onp + 1) // 2 if onp is not None and onp > 2 * M - 1 else M
+    exp = F.fft_par_real(V, mp)
+    out = FP.FFT_PAR_REAL(None, V, opt_nb_plane=onp)
+    assert out.shape == exp.shape and rel(out, exp) <= TOL
+    dout = FP.FFT_PAR_REAL(None, FP.fortran_tensor(V), opt_nb_plane=onp)
+    torch.cuda.synchronize()
+    assert rel(dout.cpu().numpy(), exp) <= TOL
+
+
 @pytest.mark.parametrize("npt,M,vs", [(500, 8, 0.3), (300, 64, 3.0), (30000, 4, 0.3)])
 def test_compression_level_set(npt, M, vs):
     print(f"case compression_level_set np={npt} M={M}", flush=True)

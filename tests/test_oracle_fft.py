@@ -162,3 +162,22 @@ def test_fft_par_real_planes():
     N = 2 * M - 1
     th = 2 * np.pi * np.arange(N) / N
     assert np.allclose(R[:, 0, 0], 1 + 2 * np.sin(3 * th), atol=1e-13)
+
+
+@pytest.mark.parametrize("mp", [4, 9, 40])
+def test_fft_par_real_on_a_finer_grid(mp):
+    """FFT_PAR_REAL with opt_nb_plane = 2*mp-1 > 2M-1: the values of
+    a_0 + sum a_m cos(m t) + b_m sin(m t) at t_j = 2 pi j/N, N = 2*mp-1, summed
+    here directly (the mode-0 sine is ignored)."""
+    rng = np.random.default_rng(mp)
+    M, npt = 4, 3
+    V = rng.standard_normal((npt, 4, M))
+    R = F.fft_par_real(V, mp)
+    N = 2 * mp - 1
+    t = 2 * np.pi * np.arange(N) / N
+    m = np.arange(1, M)[:, None]
+    for f in range(2):
+        a, b = V[:, 2 * f], V[:, 2 * f + 1]
+        ex = a[:, :1] + a[:, 1:] @ np.cos(m * t) + b[:, 1:] @ np.sin(m * t)
+        assert R.shape == (N, 2, npt)
+        assert np.max(np.abs(R[:, f, :].T - ex)) <= 1e-13 * np.max(np.abs(ex))
