@@ -238,6 +238,28 @@ int sfx_fft_compute_entropy_visc_grad_mom_dev(const double* V1_in, const double*
                                               const double* c_in_real, double les_coeff1_mom, double* V_out,
                                               int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_MAX_MIN_VEL_DCL (fft_parallel.F90:4467): V_out[0] = max of |v|
+ * over the grid (|v| from the first three fields if nb_field = 6, else
+ * |first field|), V_out[1] = the minimum as the reference computes it
+ * (:4563-4576): its start value is the first positive |v| on the last grid
+ * angle, so it is the minimum of the positive |v| if |v| > 0 somewhere on
+ * the last angle, else 0.  _dev: V_out is device memory (2 doubles). */
+int sfx_fft_max_min_vel_dcl(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                            int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad);
+int sfx_fft_max_min_vel_dcl_dev(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                int32_t m_max_pad, void* stream);
+
+/* Replaces FFT_CHECK_INTERFACE (fft_parallel.F90:2267): V1_in(nb_fluids-1,
+ * np, nb_field = 2, m) level sets; interface_ok = 0 if at some point and grid
+ * angle a level set in [0.1, 0.9] is below the next one (:2388-2404), else 1
+ * (always 1 for two fluids).  At most 5 fluids.  _dev: interface_ok is device
+ * memory (one int32). */
+int sfx_fft_check_interface(const double* V1_in, int32_t nb_fluids, int32_t* interface_ok, int64_t np,
+                            int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                            int32_t m_max_pad, double* temps);
+int sfx_fft_check_interface_dev(const double* V1_in, int32_t nb_fluids, int32_t* interface_ok, int64_t np,
+                                int32_t nb_field, int32_t m_max_c, int32_t m_max_pad, void* stream);
+
 /* Replaces FFT_PAR_REAL (fft_parallel.F90:28): V1_in (np,nb_field,m_max_c)
  * -> V_out(N, nb_field/2, np), column-major, the values of each (cos, sin)
  * column pair on the N grid angles 2*pi*j/N (the synthesis of

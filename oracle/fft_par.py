@@ -25,7 +25,8 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 (:4055-4292), ``fft_par_compr_entro_visc_dcl`` (:3160-3417) and
 ``fft_compute_entropy_visc_grad_mom`` (:3419-3647), ``fft_compute_entropy_visc``
 (:2418-2699), ``fft_compute_entropy_visc_mom`` (:2701-2937),
-``fft_compression_level_set_dcl`` (:3821-4053);
+``fft_compression_level_set_dcl`` (:3821-4053), ``fft_max_min_vel_dcl``
+(:4467-4599) and ``fft_check_interface`` (:2267-2416);
 ``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
@@ -46,6 +47,7 @@ __all__ = [
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
     "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom", "fft_par_compr_entro_visc_dcl",
     "fft_compute_entropy_visc", "fft_compute_entropy_visc_mom", "fft_compression_level_set_dcl",
+    "fft_max_min_vel_dcl", "fft_check_interface",
     "regul", "OPS",
 ]
 
@@ -341,6 +343,62 @@ def fft_max_vel_dcl(V1_in, m_max_pad, engine="fft"):
             nrm = np.sqrt(ru[:, 0] ** 2)
         best = max(best, float(np.max(nrm)))
     return best
+
+
+def _grid_norms(V, m_max_pad, engine):
+    """|v| (nb_field 6: first three fields; else |first field|) on the grid,
+    (np, N), as FFT_MAX_VEL_DCL and FFT_MAX_MIN_VEL_DCL form it."""
+    npt, nb, M = V.shape
+    N = n_grid(m_max_pad)
+    out = np.empty((npt, N))
+    for s in range(0, npt, _CHUNK):
+        ru = np.asarray(c2r(pack(V[s:s + _CHUNK], m_max_pad), N, engine), dtype=np.float64)
+        if nb == 6:
+            out[s:s + _CHUNK] = np.sqrt(ru[:, 0] ** 2 + ru[:, 1] ** 2 + ru[:, 2] ** 2)
+        else:
+            out[s:s + _CHUNK] = np.sqrt(ru[:, 0] ** 2)
+    return out
+
+
+def fft_max_min_vel_dcl(V1_in, m_max_pad, engine="fft"):
+    """FFT_MAX_MIN_VEL_DCL (fft_parallel.F90:4467-4599), one rank: (max, min) of
+    |v| on the grid (:4559-4562).  The minimum follows the reference's loops
+    (:4563-4576): the first loop's EXIT leaves only the inner loop over
+    points, so its start value is that of the last angle, the first positive
+    |v| there (or the last point's value, 0, when there is none); the second
+    loop lowers it to any smaller positive value."""
+    V = np.asarray(V1_in, dtype=np.float64)
+    npt, nb, M = V.shape
+    if nb % 2 != 0 or M == 0:
+        raise ValueError(" BUG in FFT_MAX_MIN_VEL_DCL ")
+    nrm = _grid_norms(V, m_max_pad, engine)
+    vmax = float(np.max(nrm))
+    last = nrm[:, -1]
+    pos = np.nonzero(last > 0.0)[0]
+    start = float(last[pos[0]]) if pos.size else float(last[-1])
+    lower = nrm[(nrm < start) & (nrm > 0.0)]
+    return vmax, (float(np.min(lower)) if lower.size else start)
+
+
+def fft_check_interface(V1_in, nb_fluids, m_max_pad, engine="fft"):
+    """FFT_CHECK_INTERFACE (fft_parallel.F90:2267-2416), one rank: V1_in
+    (nb_fluids-1, np, 2, m) level sets; 0 if at some point and grid angle a
+    level set in [0.1, 0.9] is below the next one (:2388-2404), else 1."""
+    V = np.asarray(V1_in, dtype=np.float64)
+    n_int, npt, nb, M = V.shape
+    if nb % 2 != 0 or M == 0:
+        raise ValueError(" BUG in FFT_CHECK_INTERFACE ")
+    if nb != 2:
+        raise ValueError("BUG in FFT_CHECK_INTERFACE: pb with V1_in dimensions")
+    if nb_fluids == 2:
+        return 1
+    N = n_grid(m_max_pad)
+    ru = [np.asarray(c2r(pack(V[i], m_max_pad), N, engine), dtype=np.float64)[:, 0] for i in range(n_int)]
+    for i in range(nb_fluids - 2):
+        a, b = ru[i], ru[i + 1]
+        if np.any((0.1 <= a) & (a <= 0.9) & (a < b)):
+            return 0
+    return 1
 
 
 def fft_scalar_vect_no_overshoot(scalar_bounds, V1_in, V2_in, pb, m_max_pad, engine="fft"):

@@ -75,8 +75,8 @@ int get_twiddles(int MP, Twiddles* tw) {
 
 // ---- grow-only device staging buffers for the host-pointer entry points
 struct Staging {
-  void* p[16] = {nullptr};
-  size_t n[16] = {0};
+  void* p[20] = {nullptr};
+  size_t n[20] = {0};
 };
 static std::map<int, Staging> g_staging;
 

@@ -15,9 +15,9 @@ template <int OP, int NI>
 __global__ void __launch_bounds__(EXACT_THREADS) exact_grid_kernel(ExactArgs a, int nch) {
   const int64_t L = xcd_block(blockIdx.x, gridDim.x);
   const int64_t p = (L / nch) * EXACT_THREADS + threadIdx.x;
-  double vmax = 0.0;
+  double vmax = OP == EX_VELSTAT ? -INFINITY : 0.0;
   exact_grid_body<OP, NI>(a, p, (int)(L % nch), vmax);
-  if constexpr (OP == EX_MAXVEL) {
+  if constexpr (exact_is_reduction(OP)) {
     __shared__ double red[EXACT_THREADS];
     red[threadIdx.x] = vmax;
     __syncthreads();
@@ -133,9 +133,10 @@ __global__ void __launch_bounds__(256) exact_emom_out_kernel(ExactArgs a) {
 }
 
 // Maximum of n values into out[0] (one work-group).
-__global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* v, int64_t n, double* out) {
+__global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* v, int64_t n, double* out,
+                                                                  double init) {
   __shared__ double red[EXACT_THREADS];
-  double m = 0.0;
+  double m = init;
   for (int64_t i = threadIdx.x; i < n; i += EXACT_THREADS) m = fmax(m, v[i]);
   red[threadIdx.x] = m;
   __syncthreads();
@@ -146,7 +147,29 @@ __global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* 
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// Blocks of the angle pass (also the size of the EX_MAXVEL partial array).
+// FFT_MAX_MIN_VEL_DCL: out[1] = min of the positive |v| if |v| > 0 somewhere
+// on the last angle, else 0 (t[0] = max of -|v| over |v| > 0, t[1] the flag).
+__global__ void exact_minvel_kernel(const double* t, double* out) {
+  if (threadIdx.x == 0) out[1] = t[1] >= 1.0 ? -t[0] : 0.0;
+}
+
+// FFT_CHECK_INTERFACE: interface_ok = 0 if a violation was found (flag null:
+// nothing was checked).
+__global__ void exact_checkif_kernel(const double* flag, int32_t* ok) {
+  if (threadIdx.x == 0) ok[0] = (flag != nullptr && flag[0] >= 1.0) ? 0 : 1;
+}
+
+hipError_t launch_exact_minvel(const double* t, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(exact_minvel_kernel, dim3(1), dim3(64), 0, st, t, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_exact_checkif(const double* flag, int32_t* ok, hipStream_t st) {
+  hipLaunchKernelGGL(exact_checkif_kernel, dim3(1), dim3(64), 0, st, flag, ok);
+  return hipGetLastError();
+}
+
+// Blocks of the angle pass (also the size of a reduction's partial array).
 int64_t exact_grid_blocks(int op, int nin, int64_t np, int K) {
   const int JC = exact_jc(op, nin);
   return ((np + EXACT_THREADS - 1) / EXACT_THREADS) * ((K + 1 + JC - 1) / JC);
@@ -163,7 +186,7 @@ hipError_t launch_exact_emom(const ExactArgs& a, double* dmax, hipStream_t st) {
   hipLaunchKernelGGL(exact_emom_kernel, dim3((unsigned)nb), dim3(threads), shmem, st, a, nch, me);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nb, dmax);
+  hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nb, dmax, 0.0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int64_t n = a.np * (2 * (int64_t)a.K + 1);
   const int64_t nb2 = (n + 255) / 256;
@@ -240,10 +263,23 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
       if (a.nin == 3) e = grid_launch<EX_MAXVEL, 3>(a, st, &nblk);
       else if (a.nin == 1) e = grid_launch<EX_MAXVEL, 1>(a, st, &nblk);
       break;
+    case EX_VELSTAT:
+      if (a.nin == 3) e = grid_launch<EX_VELSTAT, 3>(a, st, &nblk);
+      else if (a.nin == 1) e = grid_launch<EX_VELSTAT, 1>(a, st, &nblk);
+      break;
+    case EX_CHECKIF:
+      switch (a.nin) {
+        case 2: e = grid_launch<EX_CHECKIF, 2>(a, st, &nblk); break;
+        case 3: e = grid_launch<EX_CHECKIF, 3>(a, st, &nblk); break;
+        case 4: e = grid_launch<EX_CHECKIF, 4>(a, st, &nblk); break;
+      }
+      break;
   }
   if (e != hipSuccess) return e;
-  if (op == EX_MAXVEL) {
-    hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nblk, max_out);
+  if (exact_is_reduction(op)) {
+    const double init = op == EX_VELSTAT ? -INFINITY : 0.0;
+    hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nblk, max_out,
+                       init);
     return hipGetLastError();
   }
   const int nmc = (a.M + EXACT_MC - 1) / EXACT_MC;

@@ -30,7 +30,11 @@ constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
                EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10,
-               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13, EX_REAL = 14 };
+               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13, EX_REAL = 14, EX_VELSTAT = 15, EX_CHECKIF = 16 };
+
+// Operations reduced to one value over all points and angles (a maximum per
+// work-group, then one reduction block) instead of transformed back.
+constexpr bool exact_is_reduction(int op) { return op == EX_MAXVEL || op == EX_VELSTAT || op == EX_CHECKIF; }
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -39,7 +43,7 @@ constexpr int exact_jc(int op, int nin) {
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
-  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : (op == EX_MAXVEL || op == EX_EVISCV || op == EX_REAL) ? 0
+  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : (exact_is_reduction(op) || op == EX_EVISCV || op == EX_REAL) ? 0
          : (op == EX_ENTRO || op == EX_COMPR) ? 4 : (op == EX_GRADMOM || op == EX_EVISC) ? 9 : 1;
 }
 
@@ -68,7 +72,7 @@ struct ExactArgs {
   ExactOut out[EXACT_MAX_OUT];
   double* W;            // S of output g, angle j, point p at W[(2g*(K+1) + j)*np + p]; D at +(K+1)*np
   int64_t n_W;
-  double* pmax;         // EX_MAXVEL: one maximum per work-group
+  double* pmax;         // reductions: one maximum per work-group
   int n_int;            // EX_HEAVI: interfaces
   double vals[EXACT_MAX_INT + 1];
   double eps;           // EX_HEAVI: regul half-width (coeff_tanh)
@@ -167,6 +171,13 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
       rho = rho * (1.0 - h) + a.vals[i + 1] * h;
     }
     w[0] = rho;
+  } else if constexpr (OP == EX_CHECKIF) {
+    // 1 where a level set in [0.1, 0.9] lies below the next one   :2394-2402
+    double bad = 0.0;
+#pragma unroll
+    for (int i = 0; i + 1 < NI; ++i)
+      if (0.1 <= u[i] && u[i] <= 0.9 && u[i] < u[i + 1]) bad = 1.0;
+    w[0] = bad;
   } else if constexpr (OP == EX_TENSION) {
     const double nrm = sqrt(u[3] * u[3] + u[4] * u[4] + u[5] * u[5]);   // :1690-1706
     const double inv = nrm + 1.e-14;
@@ -259,9 +270,19 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
     double wp[NO > 0 ? NO : 1], wm[NO > 0 ? NO : 1];
     exact_op<OP, NI, NO>(a, up, wp, xp, y);
     if (j > 0) exact_op<OP, NI, NO>(a, um, wm, xm, y);
-    if constexpr (OP == EX_MAXVEL) {
+    if constexpr (OP == EX_MAXVEL || OP == EX_CHECKIF) {
       vmax = fmax(vmax, wp[0]);
       if (j > 0) vmax = fmax(vmax, wm[0]);
+    } else if constexpr (OP == EX_VELSTAT) {
+      // n_int 0: the maximum of -|v| over the angles where |v| > 0 (running
+      // value starts at -inf); n_int 1: 1 if |v| > 0 at the last angle N-1,
+      // the mirror of j = 1 (j = 0 itself when N = 1)
+      if (a.n_int == 0) {
+        if (wp[0] > 0.0) vmax = fmax(vmax, -wp[0]);
+        if (j > 0 && wm[0] > 0.0) vmax = fmax(vmax, -wm[0]);
+      } else if (a.K == 0 ? wp[0] > 0.0 : (j == 1 && wm[0] > 0.0)) {
+        vmax = fmax(vmax, 1.0);
+      }
     } else if constexpr (OP == EX_REAL) {
       // real planes: the value at angle j of point p at rout1[p*rstride + j]
       const int64_t N = 2 * a.K + 1;

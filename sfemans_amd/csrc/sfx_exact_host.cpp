@@ -56,7 +56,7 @@ static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M
   int rc = exact_tables(M, a.K, &a.ctab, &a.rtab);
   if (rc) return rc;
   void* w = nullptr;
-  if (op == EX_MAXVEL) {
+  if (exact_is_reduction(op)) {
     const int64_t nb = exact_grid_blocks(op, a.nin, np, a.K);
     if ((rc = staging(SLOT_EX_W, (size_t)(nb > 0 ? nb : 1) * 8, &w))) return rc;
     a.pmax = (double*)w;
@@ -69,7 +69,7 @@ static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M
 }
 
 static int exact_run(const char* nm, int op, const ExactArgs& a, double* max_out, hipStream_t st) {
-  if (a.np == 0 && op != EX_MAXVEL) return SFX_OK;
+  if (a.np == 0 && !exact_is_reduction(op)) return SFX_OK;
   hipError_t e = launch_exact(op, a, max_out, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
@@ -259,6 +259,44 @@ static int real_dev(const double* V, double* out, int64_t np, int nb, int M, int
     if (rc || (rc = exact_run(nm, EX_REAL, a, nullptr, st))) return rc;
   }
   return SFX_OK;
+}
+
+// V1 (np, nb, M) -> out[0] = max |v|, out[1] = the reference's minimum (device)
+static int maxminvel_dev(const double* V1, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
+  const char* nm = "FFT_MAX_MIN_VEL_DCL";
+  void* t = nullptr;
+  int rc = staging(16, 2 * 8, &t);
+  if (rc || (rc = maxvel_dev(V1, out, np, nb, M, mp, st))) return rc;
+  for (int mode = 0; mode < 2; ++mode) {
+    ExactArgs a;
+    exact_setup_velstat(a, V1, np, nb, M, mode);
+    if ((rc = exact_prepare(nm, a, EX_VELSTAT, np, M, mp)) ||
+        (rc = exact_run(nm, EX_VELSTAT, a, (double*)t + mode, st)))
+      return rc;
+  }
+  hipError_t e = launch_exact_minvel((const double*)t, out, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
+// V1 (nb_fluids-1, np, 2, M) -> ok[0] (device int)
+static int checkif_dev(const double* V1, int nb_fluids, int32_t* ok, int64_t np, int M, int mp, hipStream_t st) {
+  const char* nm = "FFT_CHECK_INTERFACE";
+  const int n_int = nb_fluids - 1;
+  if (n_int < 1) return fail(SFX_ERR_ARG, "%s: nb_fluids=%d < 2", nm, nb_fluids);
+  if (n_int > EXACT_MAX_INT) return fail(SFX_ERR_UNSUPPORTED, "%s: %d level sets > %d", nm, n_int, EXACT_MAX_INT);
+  void* f = nullptr;
+  int rc = staging(17, 8, &f);
+  if (rc) return rc;
+  if (n_int == 1) {
+    f = nullptr;   // two fluids: nothing to compare (:2390-2392), interface_ok = 1
+  } else {
+    ExactArgs a;
+    exact_setup_checkif(a, V1, n_int, np, M);
+    if ((rc = exact_prepare(nm, a, EX_CHECKIF, np, M, mp)) || (rc = exact_run(nm, EX_CHECKIF, a, (double*)f, st)))
+      return rc;
+  }
+  hipError_t e = launch_exact_checkif((const double*)f, ok, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
 
 // ---- host-pointer paths: stage, run, copy back, synchronise
@@ -744,6 +782,64 @@ int sfx_fft_max_vel_dcl(const double* V1_in, double* V_out, int64_t np, int32_t 
     return rc;
   if ((rc = maxvel_dev((const double*)d1, (double*)dm, np, nb_field, m_max_c, m_max_pad, nullptr))) return rc;
   return from_dev(V_out, dm, 1);
+}
+
+int sfx_fft_max_min_vel_dcl_dev(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                                int32_t m_max_pad, void* stream) {
+  clear_err();
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_MAX_MIN_VEL_DCL ");
+  return maxminvel_dev(V1_in, V_out, np, nb_field, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_max_min_vel_dcl(const double* V1_in, double* V_out, int64_t np, int32_t nb_field, int32_t m_max_c,
+                            int32_t nb_procs, int64_t bloc_size, int32_t m_max_pad) {
+  clear_err();
+  if (!V_out) return fail(SFX_ERR_ARG, "FFT_MAX_MIN_VEL_DCL: null V_out");
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_MAX_MIN_VEL_DCL ");
+  int rc = check_procs("FFT_MAX_MIN_VEL_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  void *d1, *dm;
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, (size_t)np * nb_field * m_max_c, &d1)) ||
+      (rc = staging(SLOT_EX_MAX, 16, &dm)))
+    return rc;
+  if ((rc = maxminvel_dev((const double*)d1, (double*)dm, np, nb_field, m_max_c, m_max_pad, nullptr))) return rc;
+  return from_dev(V_out, dm, 2);
+}
+
+int sfx_fft_check_interface_dev(const double* V1_in, int32_t nb_fluids, int32_t* interface_ok, int64_t np,
+                                int32_t nb_field, int32_t m_max_c, int32_t m_max_pad, void* stream) {
+  clear_err();
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_CHECK_INTERFACE ");
+  if (nb_field != 2) return fail(SFX_ERR_ARG, "BUG in FFT_CHECK_INTERFACE: pb with V1_in dimensions");
+  return checkif_dev(V1_in, nb_fluids, interface_ok, np, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_check_interface(const double* V1_in, int32_t nb_fluids, int32_t* interface_ok, int64_t np,
+                            int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                            int32_t m_max_pad, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  if (!interface_ok) return fail(SFX_ERR_ARG, "FFT_CHECK_INTERFACE: null interface_ok");
+  if (nb_field % 2 != 0 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_CHECK_INTERFACE ");
+  if (nb_field != 2) return fail(SFX_ERR_ARG, "BUG in FFT_CHECK_INTERFACE: pb with V1_in dimensions");
+  int rc = check_procs("FFT_CHECK_INTERFACE", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (nb_fluids < 2) return fail(SFX_ERR_ARG, "FFT_CHECK_INTERFACE: nb_fluids=%d < 2", nb_fluids);
+  const size_t n1 = (size_t)(nb_fluids - 1) * np * 2 * m_max_c;
+  void *d1, *dk;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, V1_in, n1, &d1)) || (rc = staging(SLOT_EX_MAX, 8, &dk))) return rc;
+  t.t1 = now_s();
+  if ((rc = checkif_dev((const double*)d1, nb_fluids, (int32_t*)dk, np, m_max_c, m_max_pad, nullptr))) return rc;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "FFT_CHECK_INTERFACE");
+  t.t2 = now_s();
+  e = hipMemcpy(interface_ok, dk, sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "D2H");
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
 }
 
 int32_t sfx_fft_par_real_n(int32_t m_max_c, int32_t opt_nb_plane) {

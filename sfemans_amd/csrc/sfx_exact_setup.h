@@ -288,4 +288,21 @@ inline void exact_setup_maxvel(ExactArgs& a, const double* V1, int64_t np, int n
   }
 }
 
+// FFT_MAX_MIN_VEL_DCL statistics (mode 0: max of -|v| where |v| > 0; mode
+// 1: |v| > 0 on the last angle), V1 (np, nb, M) as for the maximum.
+inline void exact_setup_velstat(ExactArgs& a, const double* V1, int64_t np, int nb, int M, int mode) {
+  exact_setup_maxvel(a, V1, np, nb, M);
+  a.n_int = mode;
+}
+
+// FFT_CHECK_INTERFACE: V1 (n_int, np, 2, M) level sets, as for the Heaviside.
+inline void exact_setup_checkif(ExactArgs& a, const double* V1, int n_int, int64_t np, int M) {
+  memset(&a, 0, sizeof(a));
+  a.nin = n_int; a.nout = 0; a.n_int = n_int;
+  for (int i = 0; i < n_int; ++i) {
+    a.in[i] = exact_field(V1 + i, n_int, (int64_t)n_int * np * 2, (int64_t)n_int * np);
+    a.n_in[i] = (int64_t)n_int * np * 2 * M - i;
+  }
+}
+
 }  // namespace sfx

@@ -264,6 +264,53 @@ def FFT_MAX_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=N
     return float(out[0])
 
 
+def FFT_MAX_MIN_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=None, V_out=None):
+    """(max, min) of |v| over the points and the 2*m_max_pad-1 angles
+    (fft_parallel.F90:4467); the minimum as the reference computes it: of the
+    positive values, or 0 when |v| = 0 on the whole last angle.  Host arrays:
+    returns a (2,) array.  CUDA tensors: writes into ``V_out`` (2 float64,
+    allocated when omitted) on the current stream and returns it."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(V1_in)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        import torch
+        if V_out is None:
+            V_out = torch.empty(2, dtype=torch.float64, device=V1_in.device)
+        check(L.sfx_fft_max_min_vel_dcl_dev(ptr(V1_in), ptr(V_out), np_, nb, M, m_max_pad, stream_ptr()))
+        return V_out
+    out = np.zeros(2)
+    check(L.sfx_fft_max_min_vel_dcl(ptr(V1_in), out.ctypes.data, np_, nb, M, nb_procs, bloc_size, m_max_pad))
+    return out
+
+
+def FFT_CHECK_INTERFACE(communicator, V1_in, nb_fluids, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None,
+                        interface_ok=None):
+    """interface_ok (fft_parallel.F90:2267): 0 if at some point and grid angle
+    a level set in [0.1, 0.9] lies below the next one, else 1.  V1_in is
+    (nb_fluids-1, np, 2, m).  Host arrays: returns an int.  CUDA tensors:
+    writes into ``interface_ok`` (1 int32, allocated when omitted) on the
+    current stream and returns it."""
+    _single_rank(communicator)
+    nl, np_, nb, M = fdims(V1_in)
+    if nl != nb_fluids - 1:
+        raise ValueError("V1_in must hold nb_fluids-1 level sets")
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    L = lib()
+    if is_device(V1_in):
+        import torch
+        if interface_ok is None:
+            interface_ok = torch.empty(1, dtype=torch.int32, device=V1_in.device)
+        check(L.sfx_fft_check_interface_dev(ptr(V1_in), int(nb_fluids), ptr(interface_ok), np_, nb, M, m_max_pad,
+                                            stream_ptr()))
+        return interface_ok
+    ok = np.zeros(1, dtype=np.int32)
+    check(L.sfx_fft_check_interface(ptr(V1_in), int(nb_fluids), ok.ctypes.data, np_, nb, M, nb_procs, bloc_size,
+                                    m_max_pad, _temps_ptr(temps)))
+    return int(ok[0])
+
+
 def FFT_PAR_ENTRO_VISC_DCL(communicator, V1_in, c1_in, c_in_real, hloc_gauss, coeff1_in_level, V_out, c_out,
                            nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
     """V_out(np,6,m) = (coeff1_in_level - c_in_real) V1 and c_out(np,2,m) =

@@ -471,6 +471,40 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
                          const double* values, int n_values, double eps, int nb, double* out) {
   ExactArgs a;
   double cls_vmax = 0.0;
+  if (op == EX_VELSTAT) {   // 15 FFT_MAX_MIN_VEL_DCL: in1 (np, nb, M) -> out[0] max, out[1] min
+    int rc = emu_exact(EX_MAXVEL, np, M, mp, in1, nullptr, nullptr, 0, 0.0, nb, out);
+    if (rc) return rc;
+    double t[2];
+    std::vector<double> tab;
+    for (int mode = 0; mode < 2; ++mode) {
+      exact_setup_velstat(a, in1, np, nb, M, mode);
+      exact_setup_sizes(a, np, M, mp);
+      if (mode == 0) exact_host_tables(M, a.K, tab);
+      a.ctab = tab.data();
+      a.rtab = tab.data() + exact_ctab_size(M, a.K);
+      t[mode] = -INFINITY;
+      if (a.nin == 3) emu_exact_grid<EX_VELSTAT, 3>(a, t[mode]);
+      else emu_exact_grid<EX_VELSTAT, 1>(a, t[mode]);
+    }
+    out[1] = t[1] >= 1.0 ? -t[0] : 0.0;
+    return 0;
+  }
+  if (op == EX_CHECKIF) {   // 16 FFT_CHECK_INTERFACE: in1 (nb - 1, np, 2, M), nb = nb_fluids -> out[0] = ok
+    if (nb == 2) { out[0] = 1.0; return 0; }
+    exact_setup_checkif(a, in1, nb - 1, np, M);
+    exact_setup_sizes(a, np, M, mp);
+    std::vector<double> tab;
+    exact_host_tables(M, a.K, tab);
+    a.ctab = tab.data();
+    a.rtab = tab.data() + exact_ctab_size(M, a.K);
+    double f = 0.0;
+    if (a.nin == 2) emu_exact_grid<EX_CHECKIF, 2>(a, f);
+    else if (a.nin == 3) emu_exact_grid<EX_CHECKIF, 3>(a, f);
+    else if (a.nin == 4) emu_exact_grid<EX_CHECKIF, 4>(a, f);
+    else return 2;
+    out[0] = f >= 1.0 ? 0.0 : 1.0;
+    return 0;
+  }
   if (op == EX_REAL) {   // 14 FFT_PAR_REAL: in1 = V (np, nb, M), out = (N, nb/2, np), N = 2mp-1
     std::vector<double> tab;
     for (int f = 0; f < nb / 2; ++f) {

@@ -230,3 +230,27 @@ def test_fft_par_real(npt, M, mp, nb):
     N = 2 * mp - 1
     out = emu(EX_REAL, npt, M, mp, V, nb=nb, out=np.zeros((N, nb // 2, npt), order="F"))
     assert rel(out, F.fft_par_real(V, mp)) <= 1e-12
+
+
+EX_VELSTAT, EX_CHECKIF = 15, 16
+
+
+@pytest.mark.parametrize("npt,M,mp,nb", [(13, 8, 12, 6), (7, 16, 24, 2), (300, 5, 8, 6), (5, 4, 6, 4)])
+def test_max_min_vel(npt, M, mp, nb):
+    rng = np.random.default_rng(110 + M)
+    V = fields(rng, npt, nb, M)
+    out = emu(EX_VELSTAT, npt, M, mp, V, nb=nb, out=np.zeros(2))
+    vmax, vmin = F.fft_max_min_vel_dcl(V, mp)
+    assert abs(out[0] - vmax) <= 1e-13 * vmax and abs(out[1] - vmin) <= 1e-13 * vmax
+
+
+@pytest.mark.parametrize("npt,M,mp,nf,shift", [(9, 8, 12, 3, 0.0), (9, 8, 12, 3, 0.4), (6, 16, 24, 5, 0.3),
+                                              (4, 4, 6, 2, 0.0)])
+def test_check_interface(npt, M, mp, nf, shift):
+    rng = np.random.default_rng(120 + M)
+    V = rng.standard_normal((nf - 1, npt, 2, M)) * 0.01 * np.exp(-np.arange(M) / 3.0)
+    for i in range(nf - 1):
+        V[i, :, 0, 0] = 0.8 - 0.2 * i + (shift if i == nf - 2 else 0.0)   # ordered unless shifted up
+    V = np.asfortranarray(V)
+    out = emu(EX_CHECKIF, npt, M, mp, V, nb=nf, out=np.zeros(1))
+    assert out[0] == F.fft_check_interface(V, nf, mp)

@@ -236,6 +236,38 @@ def test_compression_entropy_viscosity(npt, M):
     assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
 
 
+@pytest.mark.parametrize("npt,M,nb", [(5000, 8, 6), (3000, 16, 2), (100000, 4, 6), (1, 3, 6), (700, 64, 4)])
+def test_max_min_vel(npt, M, nb):
+    print(f"case max_min_vel np={npt} M={M} nb={nb}", flush=True)
+    rng = np.random.default_rng(110 + M)
+    V = fields(rng, npt, nb, M)
+    vmax, vmin = F.fft_max_min_vel_dcl(V, 3 * M // 2)
+    out = FP.FFT_MAX_MIN_VEL_DCL(None, V)
+    assert abs(out[0] - vmax) <= 1e-13 * vmax and abs(out[1] - vmin) <= 1e-13 * vmax
+    d = FP.FFT_MAX_MIN_VEL_DCL(None, FP.fortran_tensor(V))
+    torch.cuda.synchronize()
+    d = d.cpu().numpy()
+    assert abs(d[0] - vmax) <= 1e-13 * vmax and abs(d[1] - vmin) <= 1e-13 * vmax
+    z = FP.FFT_MAX_MIN_VEL_DCL(None, np.zeros((npt, nb, M), order="F"))
+    assert z[0] == 0.0 and z[1] == 0.0
+
+
+@pytest.mark.parametrize("npt,M,nf,shift", [(900, 8, 3, 0.0), (900, 8, 3, 0.4), (600, 16, 5, 0.3),
+                                            (400, 4, 2, 0.0), (20000, 32, 4, 0.0)])
+def test_check_interface(npt, M, nf, shift):
+    print(f"case check_interface np={npt} M={M} nb_fluids={nf}", flush=True)
+    rng = np.random.default_rng(120 + M)
+    V = rng.standard_normal((nf - 1, npt, 2, M)) * 0.01 * np.exp(-np.arange(M) / 3.0)
+    for i in range(nf - 1):
+        V[i, :, 0, 0] = 0.8 - 0.2 * i + (shift if i == nf - 2 else 0.0)
+    V = np.asfortranarray(V)
+    exp = F.fft_check_interface(V, nf, 3 * M // 2)
+    assert FP.FFT_CHECK_INTERFACE(None, V, nf) == exp
+    d = FP.FFT_CHECK_INTERFACE(None, FP.fortran_tensor(V), nf)
+    torch.cuda.synchronize()
+    assert int(d.cpu()[0]) == exp
+
+
 @pytest.mark.parametrize("npt,M,nb,onp", [(700, 8, 6, None), (500, 64, 2, None), (300, 16, 4, 95),
                                           (20000, 4, 6, 7), (2, 3, 2, 1)])
 def test_fft_par_real(npt, M, nb, onp):

@@ -493,3 +493,50 @@ def test_compression_level_set_engines_agree():
     a = F.fft_compression_level_set_dcl(G, V, C, 2, 1.1, 0.6, 0.07, mp)
     b = F.fft_compression_level_set_dcl(G, V, C, 2, 1.1, 0.6, 0.07, mp, "direct")
     assert rel(a, b) <= 1e-12
+
+
+def test_max_min_vel_constant_fields_and_zero():
+    """Constant |v| per point: max and the smallest positive value; zero
+    points are skipped; an all-zero field gives (0, 0)."""
+    M, mp = 4, 6
+    V = np.zeros((4, 6, M))
+    V[:, 0::2, 0] = [[1, 2, 2], [0, 0, 0], [0, 3, 4], [2, 0, 0]]      # |v| = 3, 0, 5, 2
+    vmax, vmin = F.fft_max_min_vel_dcl(V, mp)
+    assert abs(vmax - 5.0) <= 1e-14 and abs(vmin - 2.0) <= 1e-14
+    assert F.fft_max_min_vel_dcl(np.zeros((3, 2, M)), mp) == (0.0, 0.0)
+    C = np.zeros((2, 4, M)); C[:, 0, 0] = [-1.5, 0.5]; C[:, 2, 0] = 9.0   # nb 4: first field only
+    assert F.fft_max_min_vel_dcl(C, mp) == (1.5, 0.5)
+
+
+def test_max_min_vel_by_direct_sums():
+    """Random fields: max and min of |v| on the grid by explicit cosine/sine
+    sums (every value is positive, so the minimum is the plain minimum)."""
+    rng = np.random.default_rng(25)
+    M, mp, npt = 5, 8, 6
+    V = rng.standard_normal((npt, 6, M))
+    V[:, 0, 0] += 10.0
+    N = 2 * mp - 1
+    t = 2 * np.pi * np.arange(N) / N
+    m = np.arange(1, M)[:, None]
+    vals = [V[:, 2 * f, :1] + V[:, 2 * f, 1:] @ np.cos(m * t) + V[:, 2 * f + 1, 1:] @ np.sin(m * t) for f in range(3)]
+    nrm = np.sqrt(sum(v ** 2 for v in vals))
+    vmax, vmin = F.fft_max_min_vel_dcl(V, mp)
+    assert abs(vmax - nrm.max()) <= 1e-13 * nrm.max() and abs(vmin - nrm.min()) <= 1e-13 * nrm.max()
+
+
+def test_check_interface():
+    M, mp, npt = 4, 6, 3
+    def ls(*c0, c1=0.0):
+        V = np.zeros((len(c0), npt, 2, M))
+        for i, c in enumerate(c0):
+            V[i, :, 0, 0] = c
+        V[0, :, 0, 1] = c1
+        return V
+    assert F.fft_check_interface(ls(0.5, 0.7), 3, mp) == 0          # 0.5 in [0.1, 0.9] below 0.7
+    assert F.fft_check_interface(ls(0.5, 0.3), 3, mp) == 1
+    assert F.fft_check_interface(ls(0.95, 0.99), 3, mp) == 1        # outside [0.1, 0.9]
+    assert F.fft_check_interface(ls(0.05, 0.99), 3, mp) == 1
+    assert F.fft_check_interface(ls(0.6, 0.5, 0.55), 4, mp) == 0    # second pair
+    assert F.fft_check_interface(ls(0.6, 0.5, 0.4), 4, mp) == 1
+    assert F.fft_check_interface(ls(0.5, 0.5, c1=0.3), 3, mp) == 0  # 0.5 + 0.3 cos t < 0.5 where cos t < 0
+    assert F.fft_check_interface(ls(0.2), 2, mp) == 1
