@@ -238,6 +238,30 @@ int sfx_fft_compute_entropy_visc_grad_mom_dev(const double* V1_in, const double*
                                               const double* c_in_real, double les_coeff1_mom, double* V_out,
                                               int64_t np, int32_t m_max_c, int32_t m_max_pad, void* stream);
 
+/* Replaces FFT_PAR_VAR_ETA_PROD_T_DCL (fft_parallel.F90:1798) and
+ * FFT_PAR_VAR_ETA_PROD_GAUSS_DCL (:2015): c_out(np,nb_field,m) = coefficients
+ * of eta * c_in on the grid N = 2*m_max_pad-1, nb_field 2 or 6 (:1954-1962).
+ * The reference calls a user function ETA(mesh, angles, N, nb, ne, time) for
+ * eta on the angles 2*pi*j/N; here the caller passes its values,
+ * eta_grid(N, np) column-major (the Fortran shim calls the same function to
+ * fill it: nodes for the _T_ form, Gauss points for the _GAUSS_ form). */
+int sfx_fft_par_var_eta_prod_dcl(const double* eta_grid, const double* c_in, double* c_out, int64_t np,
+                                 int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                                 int32_t m_max_pad, double* temps);
+int sfx_fft_par_var_eta_prod_dcl_dev(const double* eta_grid, const double* c_in, double* c_out, int64_t np,
+                                     int32_t nb_field, int32_t m_max_c, int32_t m_max_pad, void* stream);
+
+/* Analysis of values given on the grid, the last step of every FFT_PAR_*
+ * product (fft_parallel.F90:277-324 for the cross product): real_in(N, np)
+ * on the angles 2*pi*j/N, N = 2*m_max_pad-1 -> c_out(np,2,m_max_c), mode 0
+ * the mean, mode m the (cos, sin) coefficients.  With sfx_fft_par_real it
+ * composes FFT_PAR_SCAL_FUNCT (fft_parallel.F90:4294), whose user function
+ * funct(x) runs where the caller's code runs: real planes, funct applied to
+ * each value, then this analysis. */
+int sfx_fft_par_from_real(const double* real_in, double* c_out, int64_t np, int32_t m_max_c, int32_t m_max_pad);
+int sfx_fft_par_from_real_dev(const double* real_in, double* c_out, int64_t np, int32_t m_max_c, int32_t m_max_pad,
+                              void* stream);
+
 /* Replaces FFT_MAX_MIN_VEL_DCL (fft_parallel.F90:4467): V_out[0] = max of |v|
  * over the grid (|v| from the first three fields if nb_field = 6, else
  * |first field|), V_out[1] = the minimum as the reference computes it

@@ -26,7 +26,8 @@ with pb = 2 (:1021-1220), ``fft_heaviside_dcl`` with ``regul`` (:832-1019,
 ``fft_compute_entropy_visc_grad_mom`` (:3419-3647), ``fft_compute_entropy_visc``
 (:2418-2699), ``fft_compute_entropy_visc_mom`` (:2701-2937),
 ``fft_compression_level_set_dcl`` (:3821-4053), ``fft_max_min_vel_dcl``
-(:4467-4599) and ``fft_check_interface`` (:2267-2416);
+(:4467-4599), ``fft_check_interface`` (:2267-2416), ``fft_par_var_eta_prod``
+(:1798-2232) and ``fft_par_scal_funct`` (:4294-4465);
 ``fft_compute_diffu_mom`` (:2939-3158) is bilinear.
 
 Two DFT engines are available: ``"fft"`` (numpy's pocketfft, the fast path
@@ -47,7 +48,7 @@ __all__ = [
     "fft_scalar_vect_no_overshoot", "fft_no_overshoot_level_set", "fft_par_entro_visc_dcl",
     "fft_compute_entropy_visc_grad_mom", "fft_compute_diffu_mom", "fft_par_compr_entro_visc_dcl",
     "fft_compute_entropy_visc", "fft_compute_entropy_visc_mom", "fft_compression_level_set_dcl",
-    "fft_max_min_vel_dcl", "fft_check_interface",
+    "fft_max_min_vel_dcl", "fft_check_interface", "fft_par_var_eta_prod", "fft_par_scal_funct",
     "regul", "OPS",
 ]
 
@@ -378,6 +379,48 @@ def fft_max_min_vel_dcl(V1_in, m_max_pad, engine="fft"):
     start = float(last[pos[0]]) if pos.size else float(last[-1])
     lower = nrm[(nrm < start) & (nrm > 0.0)]
     return vmax, (float(np.min(lower)) if lower.size else start)
+
+
+def fft_par_var_eta_prod(eta_grid, c_in, m_max_pad, engine="fft"):
+    """FFT_PAR_VAR_ETA_PROD_T_DCL (fft_parallel.F90:1798-2013) and
+    FFT_PAR_VAR_ETA_PROD_GAUSS_DCL (:2015-2232), one rank: coefficients of
+    eta * c on the grid (:1954-1962), eta_grid (N, np) the user function's
+    values on the angles 2 pi j/N, c (np, 2 or 6, m)."""
+    c = np.asarray(c_in, dtype=np.float64)
+    eta = np.asarray(eta_grid, dtype=np.float64)
+    npt, nb, M = c.shape
+    if M == 0:
+        raise ValueError(" BUG in FFT_PAR_VAR_ETA_PROD_T_DCL ")
+    if nb not in (2, 6):
+        raise ValueError("error in nb_field of c_in while calling FFT_PAR_VAR_ETA_PROD_T_DCL")
+    N = n_grid(m_max_pad)
+    if eta.shape != (N, npt):
+        raise ValueError("eta must be (2*m_max_pad-1, np)")
+    outs = []
+    for s0 in range(0, npt, _CHUNK):
+        sl = slice(s0, s0 + _CHUNK)
+        ru = np.asarray(c2r(pack(c[sl], m_max_pad), N, engine), dtype=np.float64)
+        pr = ru * eta[:, sl].T[:, None, :]
+        outs.append(unpack(np.asarray(r2c(pr, m_max_pad, engine), dtype=np.complex128), N, M))
+    return np.concatenate(outs, axis=0)
+
+
+def fft_par_scal_funct(c1_inout, funct, m_max_pad, engine="fft"):
+    """FFT_PAR_SCAL_FUNCT (fft_parallel.F90:4294-4465), one rank: columns 1-2
+    replaced by the coefficients of funct(first field) on the grid
+    (:4412-4416), the other columns unchanged.  funct is applied value by
+    value."""
+    c = np.array(c1_inout, dtype=np.float64)
+    npt, nb, M = c.shape
+    _check(nb, M, "FFT_PAR_SCAL_FUNCT")
+    N = n_grid(m_max_pad)
+    f = np.vectorize(funct, otypes=[np.float64])
+    for s0 in range(0, npt, _CHUNK):
+        sl = slice(s0, s0 + _CHUNK)
+        ru = np.asarray(c2r(pack(c[sl, 0:2], m_max_pad), N, engine), dtype=np.float64)
+        pr = f(ru[:, 0])[:, None, :]
+        c[sl, 0:2] = unpack(np.asarray(r2c(pr, m_max_pad, engine), dtype=np.complex128), N, M)
+    return c
 
 
 def fft_check_interface(V1_in, nb_fluids, m_max_pad, engine="fft"):

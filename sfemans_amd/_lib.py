@@ -63,6 +63,10 @@ SIGNATURES = {
     "sfx_fft_compute_entropy_visc_grad_mom": [_vp, _vp, _vp, _vp, _d, _vp, _i64, _i32, _i32, _i32, _i64, _i32,
                                               _vp],
     "sfx_fft_compute_entropy_visc_grad_mom_dev": [_vp, _vp, _vp, _vp, _d, _vp, _i64, _i32, _i32, _vp],
+    "sfx_fft_par_var_eta_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
+    "sfx_fft_par_var_eta_prod_dcl_dev": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp],
+    "sfx_fft_par_from_real": [_vp, _vp, _i64, _i32, _i32],
+    "sfx_fft_par_from_real_dev": [_vp, _vp, _i64, _i32, _i32, _vp],
     "sfx_fft_max_min_vel_dcl": [_vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32],
     "sfx_fft_max_min_vel_dcl_dev": [_vp, _vp, _i64, _i32, _i32, _i32, _vp],
     "sfx_fft_check_interface": [_vp, _i32, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],

@@ -226,6 +226,9 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
     case EX_COMPR: e = grid_launch<EX_COMPR, 7>(a, st, &nblk); break;
     case EX_CLS: e = grid_launch<EX_CLS, 4>(a, st, &nblk); break;
     case EX_REAL: e = grid_launch<EX_REAL, 1>(a, st, &nblk); break;
+    case EX_ETA: e = grid_launch<EX_ETA, 1>(a, st, &nblk); break;
+    case EX_ETA3: e = grid_launch<EX_ETA3, 3>(a, st, &nblk); break;
+    case EX_FROMREAL: e = grid_launch<EX_FROMREAL, 1>(a, st, &nblk); break;
     case EX_EVISC: {
       // element maxima of |v| first, then the products
       const int64_t me = a.np / a.lg;

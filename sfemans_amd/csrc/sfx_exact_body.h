@@ -30,7 +30,8 @@ constexpr int EXACT_MC = 16;         // modes per r2c thread
 
 enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4, EX_MAXVEL = 5,
                EX_SVCLAMP = 6, EX_CLAMP01 = 7, EX_ENTRO = 8, EX_GRADMOM = 9, EX_COMPR = 10,
-               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13, EX_REAL = 14, EX_VELSTAT = 15, EX_CHECKIF = 16 };
+               EX_EVISC = 11, EX_EVISCV = 12, EX_CLS = 13, EX_REAL = 14, EX_VELSTAT = 15, EX_CHECKIF = 16,
+               EX_ETA = 17, EX_ETA3 = 18, EX_FROMREAL = 19 };
 
 // Operations reduced to one value over all points and angles (a maximum per
 // work-group, then one reduction block) instead of transformed back.
@@ -43,7 +44,7 @@ constexpr int exact_jc(int op, int nin) {
 }
 // Inputs and outputs of each operation (EX_HEAVI: nin = interfaces; EX_MAXVEL: 3 or 1).
 constexpr int exact_nout(int op) {
-  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : (exact_is_reduction(op) || op == EX_EVISCV || op == EX_REAL) ? 0
+  return (op == EX_SV2 || op == EX_SVCLAMP) ? 3 : op == EX_TENSION ? 9 : op == EX_ETA3 ? 3 : (exact_is_reduction(op) || op == EX_EVISCV || op == EX_REAL) ? 0
          : (op == EX_ENTRO || op == EX_COMPR) ? 4 : (op == EX_GRADMOM || op == EX_EVISC) ? 9 : 1;
 }
 
@@ -132,6 +133,12 @@ SFX_DEV void exact_op(const ExactArgs& a, const double (&u)[NI], double (&w)[NO 
     w[0] = a.coeff * vm * (2.0 * exact_regul(u[3], 0.01) - 1.0) * (fmax(1.0 - t * t, 0.0) / (2.0 * a.coeff2) - ng);
   } else if constexpr (OP == EX_REAL) {
     w[0] = u[0];
+  } else if constexpr (OP == EX_ETA || OP == EX_ETA3) {
+    // x: eta at this point and angle                              :1954-1962
+#pragma unroll
+    for (int k = 0; k < NI; ++k) w[k] = x * u[k];
+  } else if constexpr (OP == EX_FROMREAL) {
+    w[0] = x;
   } else if constexpr (OP == EX_ENTRO) {
     const double v = a.coeff - x;                                 // -visc_entro, :4212-4219
 #pragma unroll
@@ -243,7 +250,14 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
   if (p >= a.np) return;
   const int j0 = jc * JC;
   double E[NI][JC], O[NI][JC];
-  exact_c2r<NI, JC>(a, p, j0, E, O);
+  if constexpr (OP == EX_FROMREAL) {
+#pragma unroll
+    for (int f = 0; f < NI; ++f)
+#pragma unroll
+      for (int q = 0; q < JC; ++q) E[f][q] = O[f][q] = 0.0;   // no Fourier input
+  } else {
+    exact_c2r<NI, JC>(a, p, j0, E, O);
+  }
   const int64_t plane = (int64_t)(a.K + 1) * a.np;
 #pragma unroll
   for (int q = 0; q < JC; ++q) {
@@ -256,7 +270,8 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
       um[f] = E[f][q] - O[f][q];
     }
     double xp = 0.0, xm = 0.0, y = 0.0;
-    if constexpr (OP == EX_ENTRO || OP == EX_GRADMOM || OP == EX_COMPR) {
+    if constexpr (OP == EX_ENTRO || OP == EX_GRADMOM || OP == EX_COMPR || OP == EX_ETA || OP == EX_ETA3 ||
+                  OP == EX_FROMREAL) {
       const int64_t N = 2 * a.K + 1;
       SFX_CHECK("exact creal", p * N + (j > 0 ? N - j : 0), a.n_creal);
       xp = a.creal[p * N + j];

@@ -299,6 +299,31 @@ static int checkif_dev(const double* V1, int nb_fluids, int32_t* ok, int64_t np,
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
 
+// eta (N, np), c (np, nb, M) -> out (np, nb, M), nb = 2 or 6
+static int eta_dev(const double* eta, const double* c, double* out, int64_t np, int nb, int M, int mp,
+                   hipStream_t st) {
+  const char* nm = "FFT_PAR_VAR_ETA_PROD_DCL";
+  if (M == 0) return fail(SFX_ERR_ARG, " BUG in FFT_PAR_VAR_ETA_PROD_T_DCL ");
+  if (nb != 2 && nb != 6)
+    return fail(SFX_ERR_ARG, "error in nb_field of c_in while calling FFT_PAR_VAR_ETA_PROD_T_DCL");
+  if (!eta && np > 0) return fail(SFX_ERR_ARG, "%s: null eta", nm);
+  const int op = nb == 2 ? EX_ETA : EX_ETA3;
+  ExactArgs a;
+  exact_setup_eta(a, eta, c, out, np, nb, M, mp);
+  int rc = exact_prepare(nm, a, op, np, M, mp);
+  return rc ? rc : exact_run(nm, op, a, nullptr, st);
+}
+
+// real (N, np) values on the grid -> c (np, 2, M)
+static int fromreal_dev(const double* real, double* c, int64_t np, int M, int mp, hipStream_t st) {
+  const char* nm = "FFT_PAR_FROM_REAL";
+  if (!real && np > 0) return fail(SFX_ERR_ARG, "%s: null real_in", nm);
+  ExactArgs a;
+  exact_setup_fromreal(a, real, c, np, M, mp);
+  int rc = exact_prepare(nm, a, EX_FROMREAL, np, M, mp);
+  return rc ? rc : exact_run(nm, EX_FROMREAL, a, nullptr, st);
+}
+
 // ---- host-pointer paths: stage, run, copy back, synchronise
 struct HostRun {
   double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
@@ -840,6 +865,58 @@ int sfx_fft_check_interface(const double* V1_in, int32_t nb_fluids, int32_t* int
   t.t3 = now_s();
   set_temps(temps, t);
   return SFX_OK;
+}
+
+int sfx_fft_par_var_eta_prod_dcl_dev(const double* eta_grid, const double* c_in, double* c_out, int64_t np,
+                                     int32_t nb_field, int32_t m_max_c, int32_t m_max_pad, void* stream) {
+  clear_err();
+  return eta_dev(eta_grid, c_in, c_out, np, nb_field, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_par_var_eta_prod_dcl(const double* eta_grid, const double* c_in, double* c_out, int64_t np,
+                                 int32_t nb_field, int32_t m_max_c, int32_t nb_procs, int64_t bloc_size,
+                                 int32_t m_max_pad, double* temps) {
+  clear_err();
+  if (temps) temps[0] = temps[1] = temps[2] = 0.0;
+  int rc = check_procs("FFT_PAR_VAR_ETA_PROD_DCL", np, nb_procs, bloc_size);
+  if (rc) return rc;
+  if (m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in FFT_PAR_VAR_ETA_PROD_T_DCL ");
+  if (nb_field != 2 && nb_field != 6)
+    return fail(SFX_ERR_ARG, "error in nb_field of c_in while calling FFT_PAR_VAR_ETA_PROD_T_DCL");
+  const size_t nc = (size_t)np * nb_field * m_max_c, ne = (size_t)(2 * (int64_t)m_max_pad - 1) * np;
+  void *de, *dc, *o;
+  HostRun t;
+  t.t0 = now_s();
+  if ((rc = to_dev(SLOT_EX_IN1, eta_grid, ne, &de)) || (rc = to_dev(SLOT_EX_IN2, c_in, nc, &dc)) ||
+      (rc = staging(SLOT_EX_OUT, nc * 8, &o)))
+    return rc;
+  t.t1 = now_s();
+  if ((rc = eta_dev((const double*)de, (const double*)dc, (double*)o, np, nb_field, m_max_c, m_max_pad, nullptr)))
+    return rc;
+  if ((rc = from_dev(nullptr, o, 0))) return rc;
+  t.t2 = now_s();
+  if ((rc = from_dev(c_out, o, nc))) return rc;
+  t.t3 = now_s();
+  set_temps(temps, t);
+  return SFX_OK;
+}
+
+int sfx_fft_par_from_real_dev(const double* real_in, double* c_out, int64_t np, int32_t m_max_c, int32_t m_max_pad,
+                              void* stream) {
+  clear_err();
+  return fromreal_dev(real_in, c_out, np, m_max_c, m_max_pad, (hipStream_t)stream);
+}
+
+int sfx_fft_par_from_real(const double* real_in, double* c_out, int64_t np, int32_t m_max_c, int32_t m_max_pad) {
+  clear_err();
+  if (m_max_c <= 0) return fail(SFX_ERR_ARG, "FFT_PAR_FROM_REAL: m_max_c == 0");
+  if (m_max_pad < m_max_c) return fail(SFX_ERR_ARG, "FFT_PAR_FROM_REAL: m_max_pad < m_max_c");
+  const size_t nr = (size_t)(2 * (int64_t)m_max_pad - 1) * np, nc = (size_t)np * 2 * m_max_c;
+  void *dr, *o;
+  int rc;
+  if ((rc = to_dev(SLOT_EX_IN1, real_in, nr, &dr)) || (rc = staging(SLOT_EX_OUT, nc * 8, &o))) return rc;
+  if ((rc = fromreal_dev((const double*)dr, (double*)o, np, m_max_c, m_max_pad, nullptr))) return rc;
+  return from_dev(c_out, o, nc);
 }
 
 int32_t sfx_fft_par_real_n(int32_t m_max_c, int32_t opt_nb_plane) {

@@ -288,6 +288,32 @@ inline void exact_setup_maxvel(ExactArgs& a, const double* V1, int64_t np, int n
   }
 }
 
+// FFT_PAR_VAR_ETA_PROD_T_DCL / _GAUSS_DCL: eta (N, np) on the grid angles,
+// c (np, nb, M), nb = 2 or 6 -> out (np, nb, M), the coefficients of eta * c.
+inline void exact_setup_eta(ExactArgs& a, const double* eta, const double* c, double* out, int64_t np, int nb,
+                            int M, int m_max_pad) {
+  memset(&a, 0, sizeof(a));
+  a.nin = a.nout = nb / 2;
+  for (int k = 0; k < a.nin; ++k) {
+    a.in[k] = exact_col(c, np, nb, k);
+    a.out[k] = exact_colo(out, np, nb, k);
+    a.n_in[k] = a.n_out[k] = np * nb * (int64_t)M - 2 * k * np;
+  }
+  a.creal = eta;
+  a.n_creal = (2 * (int64_t)m_max_pad - 1) * np;
+}
+
+// Analysis of values on the grid: real (N, np) -> c (np, 2, M), the
+// truncated, scaled real DFT the products end with.
+inline void exact_setup_fromreal(ExactArgs& a, const double* real, double* c, int64_t np, int M, int m_max_pad) {
+  memset(&a, 0, sizeof(a));
+  a.nin = 1; a.nout = 1;
+  a.out[0] = exact_colo(c, np, 2, 0);
+  a.n_out[0] = np * 2 * (int64_t)M;
+  a.creal = real;
+  a.n_creal = (2 * (int64_t)m_max_pad - 1) * np;
+}
+
 // FFT_MAX_MIN_VEL_DCL statistics (mode 0: max of -|v| where |v| > 0; mode
 // 1: |v| > 0 on the last angle), V1 (np, nb, M) as for the maximum.
 inline void exact_setup_velstat(ExactArgs& a, const double* V1, int64_t np, int nb, int M, int mode) {

@@ -264,6 +264,106 @@ def FFT_MAX_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=N
     return float(out[0])
 
 
+def _grid_angles(m_max_pad):
+    n = 2 * m_max_pad - 1
+    return 2 * np.pi * np.arange(n) / n
+
+
+def _eta_prod(name, communicator, eta, H_mesh, c_in, c_out, nb_procs, bloc_size, m_max_pad, time, temps):
+    _single_rank(communicator)
+    np_, nb, M = fdims(c_in)
+    fdims(c_out)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    N = 2 * m_max_pad - 1
+    if callable(eta):
+        # the reference's ETA(mesh, angles, nb_angles, nb, ne, time), points nb..ne (1-based)
+        eg = eta(H_mesh, _grid_angles(m_max_pad), N, 1, np_, time)
+    else:
+        eg = eta
+    if tuple(fdims(eg) if is_device(eg) else np.shape(eg)) != (N, np_):
+        raise ValueError(f"{name}: eta must be (2*m_max_pad-1, np)")
+    L = lib()
+    if is_device(c_in):
+        if not is_device(eg):
+            eg = fortran_tensor(np.asarray(eg, dtype=np.float64), device=c_in.device)
+        check(L.sfx_fft_par_var_eta_prod_dcl_dev(ptr(eg), ptr(c_in), ptr(c_out), np_, nb, M, m_max_pad,
+                                                 stream_ptr()))
+    else:
+        eg = np.asfortranarray(eg, dtype=np.float64)
+        check(L.sfx_fft_par_var_eta_prod_dcl(ptr(eg), ptr(c_in), ptr(c_out), np_, nb, M, nb_procs, bloc_size,
+                                             m_max_pad, _temps_ptr(temps)))
+    return c_out
+
+
+def FFT_PAR_VAR_ETA_PROD_T_DCL(communicator, eta, H_mesh, c_in, c_out, nb_procs=1, bloc_size=None, m_max_pad=None,
+                               time=0.0, temps=None):
+    """c_out = coefficients of eta * c_in (nb_field 2 or 6) on the grid of
+    2*m_max_pad-1 angles (fft_parallel.F90:1798).  eta is the reference's
+    user function eta(H_mesh, angles, nb_angles, nb, ne, time) -> (nb_angles,
+    ne-nb+1) values at the nodes, called once for all points, or its values
+    as an (2*m_max_pad-1, np) array."""
+    return _eta_prod("FFT_PAR_VAR_ETA_PROD_T_DCL", communicator, eta, H_mesh, c_in, c_out, nb_procs, bloc_size,
+                     m_max_pad, time, temps)
+
+
+def FFT_PAR_VAR_ETA_PROD_GAUSS_DCL(communicator, eta, H_mesh, c_in, c_out, nb_procs=1, bloc_size=None,
+                                   m_max_pad=None, time=0.0, temps=None):
+    """As FFT_PAR_VAR_ETA_PROD_T_DCL with eta at the Gauss points
+    (fft_parallel.F90:2015)."""
+    return _eta_prod("FFT_PAR_VAR_ETA_PROD_GAUSS_DCL", communicator, eta, H_mesh, c_in, c_out, nb_procs,
+                     bloc_size, m_max_pad, time, temps)
+
+
+def FFT_PAR_FROM_REAL(real_in, c_out, m_max_pad=None):
+    """Coefficients (np, 2, m) of values given on the 2*m_max_pad-1 grid
+    angles, real_in (2*m_max_pad-1, np): the analysis every product ends with."""
+    np_, _, M = fdims(c_out)
+    if m_max_pad is None:
+        m_max_pad = (fdims(real_in)[0] + 1) // 2 if is_device(real_in) else (np.shape(real_in)[0] + 1) // 2
+    L = lib()
+    if is_device(c_out):
+        check(L.sfx_fft_par_from_real_dev(ptr(real_in), ptr(c_out), np_, M, m_max_pad, stream_ptr()))
+    else:
+        check(L.sfx_fft_par_from_real(ptr(np.asfortranarray(real_in, dtype=np.float64)), ptr(c_out), np_, M,
+                                      m_max_pad))
+    return c_out
+
+
+def FFT_PAR_SCAL_FUNCT(communicator, c1_inout, funct, nb_procs=1, bloc_size=None, m_max_pad=None, temps=None):
+    """Columns 1-2 of c1_inout replaced by the coefficients of funct(first
+    field) on the grid of 2*m_max_pad-1 angles (fft_parallel.F90:4294).  The
+    synthesis and the analysis run on the GPU; funct runs here, on the real
+    planes: it is called once on the whole array (numpy for host data, a
+    torch tensor for CUDA data) and must act value by value; a function that
+    does not accept an array is applied to each value."""
+    _single_rank(communicator)
+    np_, nb, M = fdims(c1_inout)
+    bloc_size, m_max_pad = _defaults(np_, M, nb_procs, bloc_size, m_max_pad)
+    N = 2 * m_max_pad - 1
+    first = c1_inout[:, 0:2, :]
+    dev = is_device(c1_inout)
+    if dev:
+        import torch
+        R = FFT_PAR_REAL(communicator, first.contiguous().permute(2, 1, 0).contiguous().permute(2, 1, 0),
+                         opt_nb_plane=N)
+        vals = funct(R[:, 0, :])
+        out = fortran_empty((np_, 2, M), device=c1_inout.device)
+        FFT_PAR_FROM_REAL(vals.contiguous().t().contiguous().t(), out, m_max_pad)
+        c1_inout[:, 0:2, :] = out
+        return c1_inout
+    R = FFT_PAR_REAL(communicator, np.asfortranarray(first), opt_nb_plane=N)[:, 0, :]
+    try:
+        vals = np.asarray(funct(R), dtype=np.float64)
+        if vals.shape != R.shape:
+            raise ValueError
+    except Exception:
+        vals = np.vectorize(funct, otypes=[np.float64])(R)
+    out = np.zeros((np_, 2, M), order="F")
+    FFT_PAR_FROM_REAL(vals, out, m_max_pad)
+    c1_inout[:, 0:2, :] = out
+    return c1_inout
+
+
 def FFT_MAX_MIN_VEL_DCL(communicator, V1_in, nb_procs=1, bloc_size=None, m_max_pad=None, V_out=None):
     """(max, min) of |v| over the points and the 2*m_max_pad-1 angles
     (fft_parallel.F90:4467); the minimum as the reference computes it: of the

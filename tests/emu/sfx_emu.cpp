@@ -545,6 +545,9 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
       exact_setup_cls(a, in1, in2, out, &cls_vmax, np, M, values[0], values[1], values[2]);
       break;
     }
+    case EX_ETA:     // 17/18 eta products: in1 = c (np, nb, M), in2 = eta (N, np)
+    case EX_ETA3: exact_setup_eta(a, in2, in1, out, np, nb, M, mp); break;
+    case EX_FROMREAL: exact_setup_fromreal(a, in2, out, np, M, mp); break;   // 19: in2 = values (N, np)
     default: return 1;
   }
   exact_setup_sizes(a, np, M, mp);
@@ -567,6 +570,9 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
     case EX_GRADMOM: emu_exact_grid<EX_GRADMOM, 9>(a, vmax); break;
     case EX_COMPR: emu_exact_grid<EX_COMPR, 7>(a, vmax); break;
     case EX_CLS: emu_exact_grid<EX_CLS, 4>(a, vmax); break;
+    case EX_ETA: emu_exact_grid<EX_ETA, 1>(a, vmax); break;
+    case EX_ETA3: emu_exact_grid<EX_ETA3, 3>(a, vmax); break;
+    case EX_FROMREAL: emu_exact_grid<EX_FROMREAL, 1>(a, vmax); break;
     case EX_HEAVI:
       if (a.nin == 1) emu_exact_grid<EX_HEAVI, 1>(a, vmax);
       else if (a.nin == 2) emu_exact_grid<EX_HEAVI, 2>(a, vmax);

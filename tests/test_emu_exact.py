@@ -254,3 +254,33 @@ def test_check_interface(npt, M, mp, nf, shift):
     V = np.asfortranarray(V)
     out = emu(EX_CHECKIF, npt, M, mp, V, nb=nf, out=np.zeros(1))
     assert out[0] == F.fft_check_interface(V, nf, mp)
+
+
+EX_ETA, EX_ETA3, EX_FROMREAL = 17, 18, 19
+
+
+@pytest.mark.parametrize("npt,M,mp,nb", [(9, 8, 12, 2), (7, 16, 24, 6), (3, 64, 96, 6), (4, 1, 1, 2)])
+def test_var_eta_prod(npt, M, mp, nb):
+    rng = np.random.default_rng(130 + M)
+    c = fields(rng, npt, nb, M)
+    eta = np.asfortranarray(rng.uniform(0.5, 2.0, (2 * mp - 1, npt)))
+    out = emu(EX_ETA if nb == 2 else EX_ETA3, npt, M, mp, c, eta, nb=nb, out=np.zeros((npt, nb, M), order="F"))
+    assert rel(out, F.fft_par_var_eta_prod(eta, c, mp)) <= 1e-12
+
+
+@pytest.mark.parametrize("npt,M,mp", [(9, 8, 12), (5, 32, 48), (2, 3, 3)])
+def test_from_real_inverts_real_planes(npt, M, mp):
+    """The analysis of FFT_PAR_REAL's planes gives the coefficients back
+    (mode-0 sine zero), and the analysis of any values X matches the oracle's
+    product of X with the constant 1."""
+    rng = np.random.default_rng(140 + M)
+    c = fields(rng, npt, 2, M)
+    c[:, 1, 0] = 0.0
+    R = np.asfortranarray(F.fft_par_real(c, mp)[:, 0, :])
+    out = emu(EX_FROMREAL, npt, M, mp, c, R, out=np.zeros((npt, 2, M), order="F"))
+    assert rel(out, c) <= 1e-12
+    X = np.asfortranarray(rng.standard_normal((2 * mp - 1, npt)))
+    out = emu(EX_FROMREAL, npt, M, mp, c, X, out=np.zeros((npt, 2, M), order="F"))
+    one = np.zeros((npt, 2, M))
+    one[:, 0, 0] = 1.0                       # the constant 1: eta * 1 = eta
+    assert rel(out, F.fft_par_var_eta_prod(X, one, mp)) <= 1e-12

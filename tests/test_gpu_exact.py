@@ -236,6 +236,46 @@ def test_compression_entropy_viscosity(npt, M):
     assert rel(dv.cpu().numpy(), Vo) <= TOL and rel(dc.cpu().numpy(), co) <= TOL
 
 
+@pytest.mark.parametrize("npt,M,nb", [(900, 8, 2), (700, 16, 6), (300, 64, 6), (20000, 4, 2)])
+def test_var_eta_prod(npt, M, nb):
+    print(f"case var_eta_prod np={npt} M={M} nb={nb}", flush=True)
+    rng = np.random.default_rng(130 + M)
+    c = fields(rng, npt, nb, M)
+    mp = 3 * M // 2
+    N = 2 * mp - 1
+    r = rng.uniform(0.5, 2.0, npt)
+
+    def eta(mesh, angles, nb_angles, i0, i1, time):   # the reference's user-function form
+        assert nb_angles == N and (i0, i1) == (1, npt) and mesh is None
+        return (1.0 + 0.5 * np.cos(angles))[:, None] * r[None, :] + time
+
+    exp = F.fft_par_var_eta_prod(eta(None, 2 * np.pi * np.arange(N) / N, N, 1, npt, 0.25), c, mp)
+    out = np.zeros((npt, nb, M), order="F")
+    FP.FFT_PAR_VAR_ETA_PROD_T_DCL(None, eta, None, c, out, time=0.25)
+    assert rel(out, exp) <= TOL
+    dout = FP.fortran_empty((npt, nb, M))
+    FP.FFT_PAR_VAR_ETA_PROD_GAUSS_DCL(None, eta, None, FP.fortran_tensor(c), dout, time=0.25)
+    torch.cuda.synchronize()
+    assert rel(dout.cpu().numpy(), exp) <= TOL
+
+
+@pytest.mark.parametrize("npt,M", [(800, 8), (500, 32), (3000, 4)])
+def test_scal_funct(npt, M):
+    print(f"case scal_funct np={npt} M={M}", flush=True)
+    rng = np.random.default_rng(150 + M)
+    c = fields(rng, npt, 4, M, 0.1)
+    c[:, 0, 0] = 1.0                       # first field in about [0.6, 1.4] on the grid
+    mp = 3 * M // 2
+    exp = F.fft_par_scal_funct(c, lambda x: 1.0 / x, mp)
+    h = np.asfortranarray(c.copy())
+    FP.FFT_PAR_SCAL_FUNCT(None, h, lambda x: 1.0 / x)
+    assert rel(h, exp) <= TOL
+    d = FP.fortran_tensor(c)
+    FP.FFT_PAR_SCAL_FUNCT(None, d, lambda x: 1.0 / x)
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), exp) <= TOL
+
+
 @pytest.mark.parametrize("npt,M,nb", [(5000, 8, 6), (3000, 16, 2), (100000, 4, 6), (1, 3, 6), (700, 64, 4)])
 def test_max_min_vel(npt, M, nb):
     print(f"case max_min_vel np={npt} M={M} nb={nb}", flush=True)

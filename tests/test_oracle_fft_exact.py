@@ -540,3 +540,38 @@ def test_check_interface():
     assert F.fft_check_interface(ls(0.6, 0.5, 0.4), 4, mp) == 1
     assert F.fft_check_interface(ls(0.5, 0.5, c1=0.3), 3, mp) == 0  # 0.5 + 0.3 cos t < 0.5 where cos t < 0
     assert F.fft_check_interface(ls(0.2), 2, mp) == 1
+
+
+@pytest.mark.parametrize("nb", [2, 6])
+def test_var_eta_prod_closed_forms(nb):
+    """eta constant per point: eta * c; eta = alpha + beta cos t: the product
+    rule of times_alpha_beta_cos, per (cos, sin) column pair."""
+    rng = np.random.default_rng(26)
+    M, mp, npt = 6, 9, 5
+    N = 2 * mp - 1
+    C = band_limited(rng, npt, M, nb)
+    e0 = rng.uniform(0.5, 2.0, npt)
+    out = F.fft_par_var_eta_prod(np.tile(e0, (N, 1)), C, mp)
+    assert rel(out, e0[:, None, None] * C) <= 1e-13
+    al, be = rng.uniform(1, 2, npt), rng.uniform(-1, 1, npt)
+    t = 2 * np.pi * np.arange(N) / N
+    out = F.fft_par_var_eta_prod(al[None, :] + be[None, :] * np.cos(t)[:, None], C, mp)
+    ex = np.concatenate([times_alpha_beta_cos(C[:, 2 * k:2 * k + 2], al, be) for k in range(nb // 2)], axis=1)
+    assert rel(out, ex) <= 1e-13
+
+
+def test_scal_funct_closed_forms():
+    """funct(x) = 2x + 1: 2c + 1 in mode 0; funct(x) = x^2 with c = a + b cos t:
+    a^2 + b^2/2 + 2ab cos t + (b^2/2) cos 2t; columns 3-4 untouched."""
+    M, mp, npt = 5, 8, 3
+    rng = np.random.default_rng(27)
+    C = band_limited(rng, npt, M, 4)
+    out = F.fft_par_scal_funct(C, lambda x: 2 * x + 1, mp)
+    ex = C.copy(); ex[:, 0:2] *= 2; ex[:, 0, 0] += 1
+    assert rel(out, ex) <= 1e-13
+    a, b = np.array([1.0, -0.5, 2.0]), np.array([0.3, 1.0, -2.0])
+    D = np.zeros((npt, 2, M)); D[:, 0, 0] = a; D[:, 0, 1] = b
+    out = F.fft_par_scal_funct(D, lambda x: x * x, mp)
+    ex = np.zeros_like(D)
+    ex[:, 0, 0] = a * a + b * b / 2; ex[:, 0, 1] = 2 * a * b; ex[:, 0, 2] = b * b / 2
+    assert np.max(np.abs(out - ex)) <= 1e-14 * 10
