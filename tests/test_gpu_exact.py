@@ -264,7 +264,7 @@ def test_scal_funct(npt, M):
     print(f"case scal_funct np={npt} M={M}", flush=True)
     rng = np.random.default_rng(150 + M)
     c = fields(rng, npt, 4, M, 0.1)
-    c[:, 0, 0] = 1.0                       # first field in about [0.6, 1.4] on the grid
+    c[:, 0, 0] = 1.0                       # first field in about [0.5, 1.5] on the grid
     mp = 3 * M // 2
     exp = F.fft_par_scal_funct(c, lambda x: 1.0 / x, mp)
     h = np.asfortranarray(c.copy())
