@@ -6,7 +6,8 @@
 ! The routines keep the reference's names, argument order and meaning
 ! (FFT_PAR_CROSS_PROD_DCL :147, FFT_PAR_DOT_PROD_DCL :330, FFT_PAR_PROD_DCL :502,
 ! FFT_PAR_DIV_DCL :667, FFT_HEAVISIDE_DCL :832, FFT_SCALAR_VECT_DCL :1021,
-! FFT_MAX_VEL_DCL :1426, FFT_TENSOR_DCL :1543, FFT_SQUAREROOT_SCALAR :4601).  The communicator is the
+! FFT_MAX_VEL_DCL :1426, FFT_TENSOR_DCL :1543, FFT_SQUAREROOT_SCALAR :4601,
+! FFT_PAR_REAL :28, FFT_CHECK_INTERFACE :2267, FFT_MAX_MIN_VEL_DCL :4467).  The communicator is the
 ! legacy integer MPI handle; nb_procs must be 1.  A non-zero status from
 ! libsfx stops the program with the library's message, as the reference does
 ! with WRITE + STOP or error_petsc.  Non-contiguous actual arguments (array
@@ -18,7 +19,8 @@ MODULE fft_parallele
   PRIVATE
   PUBLIC :: FFT_PAR_CROSS_PROD_DCL, FFT_PAR_DOT_PROD_DCL, FFT_PAR_PROD_DCL, &
        FFT_SCALAR_VECT_DCL, FFT_TENSOR_DCL, FFT_PAR_DIV_DCL, FFT_HEAVISIDE_DCL, &
-       FFT_SQUAREROOT_SCALAR, FFT_MAX_VEL_DCL, sfx_mesh_register, sfx_mesh_release, &
+       FFT_SQUAREROOT_SCALAR, FFT_MAX_VEL_DCL, FFT_MAX_MIN_VEL_DCL, FFT_PAR_REAL, FFT_CHECK_INTERFACE, &
+       sfx_mesh_register, sfx_mesh_release, &
        sfx_smb_cross_prod_gauss, sfx_smb_curlh_cross_b_gauss
 
   INTERFACE
@@ -95,6 +97,34 @@ MODULE fft_parallele
        INTEGER(C_INT32_T), VALUE :: nbf, m, nbp, mpad
        INTEGER(C_INT) :: rc
      END FUNCTION c_maxvel
+     FUNCTION c_maxminvel(V1, vo, np, nbf, m, nbp, bloc, mpad) &
+          BIND(C, NAME='sfx_fft_max_min_vel_dcl') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: V1, vo
+       INTEGER(C_INT64_T), VALUE :: np, bloc
+       INTEGER(C_INT32_T), VALUE :: nbf, m, nbp, mpad
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_maxminvel
+     FUNCTION c_checkif(V1, nbfl, ok, np, nbf, m, nbp, bloc, mpad, temps) &
+          BIND(C, NAME='sfx_fft_check_interface') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: V1, ok, temps
+       INTEGER(C_INT32_T), VALUE :: nbfl, nbf, m, nbp, mpad
+       INTEGER(C_INT64_T), VALUE :: np, bloc
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_checkif
+     FUNCTION c_real_n(m, nplane) BIND(C, NAME='sfx_fft_par_real_n') RESULT(n)
+       IMPORT :: C_INT32_T
+       INTEGER(C_INT32_T), VALUE :: m, nplane
+       INTEGER(C_INT32_T) :: n
+     END FUNCTION c_real_n
+     FUNCTION c_real(V1, Vo, np, nbf, m, nbp, nplane) BIND(C, NAME='sfx_fft_par_real') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: V1, Vo
+       INTEGER(C_INT64_T), VALUE :: np
+       INTEGER(C_INT32_T), VALUE :: nbf, m, nbp, nplane
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_real
      FUNCTION c_mesh_create(jj, rr, n_w, l_G, me, np, handle) BIND(C, NAME='sfx_mesh_create') RESULT(rc)
        IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
        TYPE(C_PTR), VALUE :: jj, rr
@@ -334,6 +364,67 @@ CONTAINS
     V_out = vo
     DEALLOCATE(a1)
   END SUBROUTINE FFT_MAX_VEL_DCL
+
+  SUBROUTINE FFT_PAR_REAL(communicator, V1_in, V_out, opt_nb_plane)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(IN) :: V1_in
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE :: V_out
+    INTEGER, OPTIONAL :: opt_nb_plane
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE, TARGET :: a1, ao
+    INTEGER(C_INT32_T) :: nplane, n
+    INTEGER(C_INT) :: rc
+    nplane = 0
+    IF (PRESENT(opt_nb_plane)) nplane = INT(opt_nb_plane, C_INT32_T)
+    n = c_real_n(INT(SIZE(V1_in,3),C_INT32_T), nplane)
+    IF (ALLOCATED(V_out)) DEALLOCATE(V_out)
+    ALLOCATE(V_out(n, SIZE(V1_in,2)/2, SIZE(V1_in,1)))
+    ALLOCATE(a1, SOURCE=V1_in)
+    ALLOCATE(ao(n, MAX(SIZE(V1_in,2)/2,1), MAX(SIZE(V1_in,1),1)))
+    rc = c_real(C_LOC(a1), C_LOC(ao), INT(SIZE(V1_in,1),C_INT64_T), INT(SIZE(V1_in,2),C_INT32_T), &
+         INT(SIZE(V1_in,3),C_INT32_T), 1_C_INT32_T, nplane)
+    CALL sfx_stop(rc, 'FFT_PAR_REAL')
+    V_out = ao(:, 1:SIZE(V1_in,2)/2, 1:SIZE(V1_in,1))
+    DEALLOCATE(a1, ao)
+  END SUBROUTINE FFT_PAR_REAL
+
+  SUBROUTINE FFT_MAX_MIN_VEL_DCL(communicator, V1_in, V_out, nb_procs, bloc_size, m_max_pad)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(IN) :: V1_in
+    REAL(KIND=8), DIMENSION(2), INTENT(OUT) :: V_out
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE, TARGET :: a1
+    REAL(KIND=8), DIMENSION(2), TARGET :: vo
+    INTEGER(C_INT) :: rc
+    ALLOCATE(a1, SOURCE=V1_in)
+    rc = c_maxminvel(C_LOC(a1), C_LOC(vo), INT(SIZE(V1_in,1),C_INT64_T), INT(SIZE(V1_in,2),C_INT32_T), &
+         INT(SIZE(V1_in,3),C_INT32_T), INT(nb_procs,C_INT32_T), INT(bloc_size,C_INT64_T), &
+         INT(m_max_pad,C_INT32_T))
+    CALL sfx_stop(rc, 'FFT_MAX_MIN_VEL_DCL')
+    V_out = vo
+    DEALLOCATE(a1)
+  END SUBROUTINE FFT_MAX_MIN_VEL_DCL
+
+  SUBROUTINE FFT_CHECK_INTERFACE(communicator, V1_in, nb_fluids, interface_ok, nb_procs, bloc_size, m_max_pad, &
+       temps)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:,:), INTENT(IN) :: V1_in
+    INTEGER, INTENT(IN) :: nb_fluids
+    INTEGER, INTENT(OUT) :: interface_ok
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), DIMENSION(:), OPTIONAL, INTENT(INOUT) :: temps
+    REAL(KIND=8), DIMENSION(:,:,:,:), ALLOCATABLE, TARGET :: a1
+    REAL(KIND=8), DIMENSION(3), TARGET :: tloc
+    INTEGER(C_INT32_T), TARGET :: ok
+    INTEGER(C_INT) :: rc
+    ALLOCATE(a1, SOURCE=V1_in)
+    rc = c_checkif(C_LOC(a1), INT(nb_fluids,C_INT32_T), C_LOC(ok), INT(SIZE(V1_in,2),C_INT64_T), &
+         INT(SIZE(V1_in,3),C_INT32_T), INT(SIZE(V1_in,4),C_INT32_T), INT(nb_procs,C_INT32_T), &
+         INT(bloc_size,C_INT64_T), INT(m_max_pad,C_INT32_T), temps_ptr(temps, tloc))
+    CALL sfx_stop(rc, 'FFT_CHECK_INTERFACE')
+    interface_ok = ok
+    IF (PRESENT(temps)) temps(1:3) = tloc
+    DEALLOCATE(a1)
+  END SUBROUTINE FFT_CHECK_INTERFACE
 
   ! Register the P2 mesh once (mesh%jj, mesh%rr of def_type_mesh).
   FUNCTION sfx_mesh_register(jj, rr) RESULT(handle)

@@ -1,7 +1,8 @@
 !===
 ! Exercises the fft_parallele shim the way the reference's callers do: a
-! cross product, a dot product, a division, a square root and a velocity
-! maximum (on the reference's own grid), and the smb_cross_prod_gauss_sft_par
+! cross product, a dot product, a division, a square root, the velocity
+! maximum and (max, min), the real planes (on the reference's own grid), and
+! the smb_cross_prod_gauss_sft_par
 ! body on a small P2 mesh read from a file.  Inputs are read from and outputs written to
 ! raw little-endian float64 files named on the command line, so that the
 ! calling test can check them.
@@ -11,7 +12,9 @@
 !   <dir>/v1.bin, v2.bin (np,6,m_max); jj.bin (6,me) int32; rr.bin (2,npm);
 !   <dir>/vn.bin (npm,6,m_max) nodal velocity; den.bin (np,2,m_max) a positive scalar
 ! writes <dir>/cross.bin (np,6,m), dot.bin (np,2,m), div.bin (np,2,m) = v1(:,1:2)/den,
-!   sqrt.bin (np,2,m) = SQRT(MAX(0,MIN(1,den))), maxvel.bin (1) = max |v1|, smb.bin (7*me,6,m)
+!   sqrt.bin (np,2,m) = SQRT(MAX(0,MIN(1,den))), maxvel.bin (1) = max |v1|, smb.bin (7*me,6,m),
+!   maxmin.bin (2) = FFT_MAX_MIN_VEL_DCL of v1, real.bin (2*m_max+1,3,np) = FFT_PAR_REAL of v1
+!   with opt_nb_plane = 2*m_max+1
 !===
 PROGRAM sfx_fortran_driver
   USE, INTRINSIC :: ISO_C_BINDING
@@ -21,6 +24,8 @@ PROGRAM sfx_fortran_driver
   INTEGER :: np, nbf, m_max, me, npm, u, i
   REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE :: v1, v2, vc, vd, vn, vs, den, vq, vr
   REAL(KIND=8), DIMENSION(1,1,1) :: vmax
+  REAL(KIND=8), DIMENSION(2,1,1) :: vmm
+  REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE :: vreal
   INTEGER, DIMENSION(:,:), ALLOCATABLE :: jj
   INTEGER(C_INT32_T), DIMENSION(:,:), ALLOCATABLE :: jj32
   REAL(KIND=8), DIMENSION(:,:), ALLOCATABLE :: rr
@@ -52,6 +57,10 @@ PROGRAM sfx_fortran_driver
   CALL write_real(TRIM(dir)//'/div.bin', vq)
   CALL write_real(TRIM(dir)//'/sqrt.bin', vr)
   CALL write_real(TRIM(dir)//'/maxvel.bin', vmax)
+  CALL FFT_MAX_MIN_VEL_DCL(comm, v1, vmm(:,1,1), 1, np, 3*m_max/2)
+  CALL write_real(TRIM(dir)//'/maxmin.bin', vmm)
+  CALL FFT_PAR_REAL(comm, v1, vreal, 2*m_max+1)
+  CALL write_real(TRIM(dir)//'/real.bin', vreal)
 
   ALLOCATE(jj32(6,me), jj(6,me), rr(2,npm), vn(npm,6,m_max), vs(7*me,6,m_max), list_mode(m_max))
   OPEN(NEWUNIT=u, FILE=TRIM(dir)//'/jj.bin', ACCESS='stream', FORM='unformatted', STATUS='old')
