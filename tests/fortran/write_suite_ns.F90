@@ -1,0 +1,76 @@
+!===
+! Writes a Navier-Stokes restart file with the record sequence of
+! write_restart_ns (SOURCES/INITIALISATION/restart.F90:59-97), so that the
+! reader in sfemans_amd/restart.py is checked against what a Fortran
+! compiler's sequential unformatted WRITE puts on disk.
+!
+! usage: write_suite_ns <file> <variant>
+!   variant 0: per sub-domain header with dt, two level sets, modes 0 3 5
+!   variant 1: mono header without dt, no level sets, modes 2 7
+! Values: un(p,k,i) = p + 10 k + 100 i + 0.25 and similar (see below).
+!===
+PROGRAM write_suite_ns
+  IMPLICIT NONE
+  CHARACTER(LEN=1024) :: fname, arg
+  INTEGER :: variant, np_v, np_p, m, i, p, k, l
+  INTEGER, DIMENSION(:), ALLOCATABLE :: list_mode
+  REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE :: un, un_m1, pn, pn_m1, incpn, incpn_m1
+  REAL(KIND=8), DIMENSION(:,:,:,:), ALLOCATABLE :: ls, ls_m1
+  REAL(KIND=8) :: time, dt, max_vel
+  CALL GET_COMMAND_ARGUMENT(1, fname)
+  CALL GET_COMMAND_ARGUMENT(2, arg)
+  READ(arg, *) variant
+  np_v = 7; np_p = 4; time = 1.5d0; dt = 0.01d0; max_vel = 2.5d0
+  IF (variant == 0) THEN
+     m = 3
+     ALLOCATE(list_mode(m)); list_mode = (/0, 3, 5/)
+  ELSE
+     m = 2
+     ALLOCATE(list_mode(m)); list_mode = (/2, 7/)
+  END IF
+  ALLOCATE(un(np_v,6,m), un_m1(np_v,6,m), pn(np_p,2,m), pn_m1(np_p,2,m), incpn(np_p,2,m), incpn_m1(np_p,2,m))
+  ALLOCATE(ls(2,np_v,2,m), ls_m1(2,np_v,2,m))
+  DO i = 1, m
+     DO k = 1, 6
+        DO p = 1, np_v
+           un(p,k,i) = p + 10*k + 100*i + 0.25d0
+           un_m1(p,k,i) = -(p + 10*k + 100*i) - 0.5d0
+        END DO
+     END DO
+     DO k = 1, 2
+        DO p = 1, np_p
+           pn(p,k,i) = 1000 + p + 10*k + 100*i
+           pn_m1(p,k,i) = 2000 + p + 10*k + 100*i
+           incpn(p,k,i) = 3000 + p + 10*k + 100*i
+           incpn_m1(p,k,i) = 4000 + p + 10*k + 100*i
+        END DO
+        DO p = 1, np_v
+           DO l = 1, 2
+              ls(l,p,k,i) = 0.125d0*l + p + 10*k + 100*i
+              ls_m1(l,p,k,i) = -0.125d0*l + p + 10*k + 100*i
+           END DO
+        END DO
+     END DO
+  END DO
+  OPEN(UNIT=10, FILE=TRIM(fname), FORM='unformatted', STATUS='replace')
+  IF (variant == 0) THEN
+     WRITE(10) time, 1, 1, SIZE(list_mode), dt
+  ELSE
+     WRITE(10) time, np_v, np_p, 1, SIZE(list_mode)
+  END IF
+  DO i = 1, SIZE(list_mode)
+     WRITE(10) list_mode(i)
+     WRITE(10) un(:,:,i)
+     WRITE(10) un_m1(:,:,i)
+     WRITE(10) pn(:,:,i)
+     WRITE(10) pn_m1(:,:,i)
+     WRITE(10) incpn(:,:,i)
+     WRITE(10) incpn_m1(:,:,i)
+     IF (variant == 0) THEN
+        WRITE(10) ls(:,:,:,i)
+        WRITE(10) ls_m1(:,:,:,i)
+        WRITE(10) max_vel + i
+     END IF
+  END DO
+  CLOSE(10)
+END PROGRAM write_suite_ns
