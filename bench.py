@@ -251,6 +251,60 @@ def wl_exact(torch, args):
     return phases, npt * M, cfg, ("exact", None, M)
 
 
+def wl_grid(torch, args):
+    """The grid-function and reduction FFT_PAR routines at 1.4M points, m_max = 64."""
+    from sfemans_amd import fft_parallele as FP
+    npt, M = 1404928, 64
+    mp = 3 * M // 2
+    N = 2 * mp - 1
+    dec = torch.exp(-torch.arange(M, device="cuda", dtype=torch.float64) / 4.0)
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 13)
+
+    def fld(nb, scale=1.0, mean=None):
+        t = torch.randn((M, nb, npt), generator=gen, device="cuda", dtype=torch.float64) * dec[:, None, None] * scale
+        t[0, 1::2, :] = 0.0
+        if mean is not None:
+            t[0, 0, :] = mean
+        return t.permute(2, 1, 0)
+    u, gphi = fld(6), fld(6)
+    phi = fld(2, 0.2, 0.5)
+    ls = torch.randn((M, 2, npt, 2), generator=gen, device="cuda", dtype=torch.float64) * 0.01 * dec[:, None, None, None]
+    ls[0, 0, :, 0] += 0.8
+    ls[0, 0, :, 1] += 0.6
+    ls[0, 1] = 0.0
+    ls = ls.permute(3, 2, 1, 0)                                     # (2, np, 2, M): three fluids
+    eta = torch.rand((npt, N), generator=gen, device="cuda", dtype=torch.float64).t() + 0.5   # (N, np) column-major
+    planes = FP.fortran_empty((N, 3, npt))
+    back = FP.fortran_empty((npt, 2, M))
+    c_out, v_out = FP.fortran_empty((npt, 2, M)), FP.fortran_empty((npt, 6, M))
+    mm = torch.empty(2, dtype=torch.float64, device="cuda")
+    ok = torch.empty(1, dtype=torch.int32, device="cuda")
+    f_t = 4.0 * M * mp                                              # flops of one real DFT per point
+    b = 8 * M * npt                                                 # bytes of one (cos, sin) column pair
+    r = 8 * N * npt                                                 # bytes of one field on the grid
+    phases = [
+        Phase("par_real", lambda: FP.FFT_PAR_REAL(None, u, planes, opt_nb_plane=N), 6 * b + 3 * r, 3 * f_t * npt,
+              "exact_grid_kernel<REAL,1> x3"),
+        Phase("from_real", lambda: FP.FFT_PAR_FROM_REAL(eta, back, mp), r + 2 * b, f_t * npt,
+              "exact_grid_kernel<FROMREAL,1> + exact_r2c_kernel"),
+        Phase("var_eta_prod", lambda: FP.FFT_PAR_VAR_ETA_PROD_T_DCL(None, eta, None, u, v_out), r + 12 * b,
+              6 * f_t * npt, "exact_grid_kernel<ETA3,3> + exact_r2c_kernel"),
+        Phase("compression_level_set", lambda: FP.FFT_COMPRESSION_LEVEL_SET_DCL(None, None, gphi, u, phi, c_out,
+                                                                               None, 7, 1.0, 0.5, 0.01),
+              16 * b, 8 * f_t * npt, "exact_grid_kernel<MAXVEL,3> + exact_max_kernel + exact_grid_kernel<CLS,4> "
+                                     "+ exact_r2c_kernel"),
+        Phase("max_min_vel", lambda: FP.FFT_MAX_MIN_VEL_DCL(None, u, V_out=mm), 18 * b, 9 * f_t * npt,
+              "exact_grid_kernel<MAXVEL,3> + exact_grid_kernel<VELSTAT,3> x2 + reductions"),
+        Phase("check_interface", lambda: FP.FFT_CHECK_INTERFACE(None, ls, 3, interface_ok=ok), 4 * b, 2 * f_t * npt,
+              "exact_grid_kernel<CHECKIF,2> + exact_max_kernel"),
+    ]
+    cfg = {"workload": "grid-function and reduction FFT_PAR routines on the reference grid N=2*m_max_pad-1=191 at "
+                       "1,404,928 points, m_max=64: PAR_REAL (3 planes), the analysis of values on the grid, "
+                       "VAR_ETA_PROD (vector), COMPRESSION_LEVEL_SET, MAX_MIN_VEL, CHECK_INTERFACE (3 fluids)",
+           "points": npt, "m_max": M, "grid": N}
+    return phases, npt * M, cfg, ("grid", None, M)
+
+
 def wl_entropy(torch, args):
     from sfemans_amd import fft_parallele as FP
     npt, M, lg = 1404928, 64, 7
@@ -456,7 +510,8 @@ def wl_full(torch, args):
 
 
 WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch,
-             "full": wl_full, "nsrhs": wl_nsrhs, "exact": wl_exact, "entropy": wl_entropy}
+             "full": wl_full, "nsrhs": wl_nsrhs, "exact": wl_exact, "entropy": wl_entropy,
+             "grid": wl_grid}
 
 
 # ------------------------------------------------------------- CPU baseline
@@ -506,6 +561,40 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
         return {"value": n * npt * M / dt, "unit": "Gauss-pt*modes/s", "cores": 1, "kind": "port",
                 "sample": f"the six non-polynomial FFT_PAR products restated in numpy (pocketfft on the grid "
                           f"N={2 * mp - 1}), {n} passes over {npt} points x m_max={M}, 1 thread, {dt:.1f} s"}
+    if kind == "grid":
+        # the oracle's grid-function and reduction routines on 2,048 points, same m_max
+        rng = np.random.default_rng(5)
+        npt = 2048
+        mp = 3 * M // 2
+        N = 2 * mp - 1
+        dec = np.exp(-np.arange(M) / 4.0)
+        fld = lambda nb, sc=1.0: np.asfortranarray(rng.standard_normal((npt, nb, M)) * dec * sc)
+        u, gphi, phi = fld(6), fld(6), fld(2, 0.2)
+        phi[:, 0, 0] = 0.5
+        ls = np.asfortranarray(rng.standard_normal((2, npt, 2, M)) * 0.01 * dec)
+        ls[0, :, 0, 0] += 0.8
+        ls[1, :, 0, 0] += 0.6
+        eta = rng.random((N, npt)) + 0.5
+        one = np.zeros((npt, 2, M))
+        one[:, 0, 0] = 1.0
+
+        def run():
+            R = F.fft_par_real(u, mp)
+            F.fft_par_var_eta_prod(R[:, 0, :], one, mp)
+            F.fft_par_var_eta_prod(eta, u, mp)
+            F.fft_compression_level_set_dcl(gphi, u, phi, 7, 1.0, 0.5, 0.01, mp)
+            F.fft_max_min_vel_dcl(u, mp)
+            F.fft_check_interface(ls, 3, mp)
+        run()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() < t0 + sample_s or n == 0:
+            run()
+            n += 1
+        dt = time.perf_counter() - t0
+        return {"value": n * npt * M / dt, "unit": "Gauss-pt*modes/s", "cores": 1, "kind": "port",
+                "sample": f"the six grid-function and reduction FFT_PAR routines restated in numpy (pocketfft on "
+                          f"the grid N={N}; the analysis as the product with 1), {n} passes over {npt} points x "
+                          f"m_max={M}, 1 thread, {dt:.1f} s"}
     if kind == "entropy":
         # the oracle's six entropy-viscosity products on 2,058 points (294 elements of 7), same m_max
         rng = np.random.default_rng(4)

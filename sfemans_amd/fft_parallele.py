@@ -318,8 +318,11 @@ def FFT_PAR_FROM_REAL(real_in, c_out, m_max_pad=None):
     """Coefficients (np, 2, m) of values given on the 2*m_max_pad-1 grid
     angles, real_in (2*m_max_pad-1, np): the analysis every product ends with."""
     np_, _, M = fdims(c_out)
+    rshape = tuple(fdims(real_in)) if is_device(real_in) else np.shape(real_in)
     if m_max_pad is None:
-        m_max_pad = (fdims(real_in)[0] + 1) // 2 if is_device(real_in) else (np.shape(real_in)[0] + 1) // 2
+        m_max_pad = (rshape[0] + 1) // 2
+    if tuple(rshape) != (2 * m_max_pad - 1, np_):
+        raise ValueError("real_in must be (2*m_max_pad-1, np)")
     L = lib()
     if is_device(c_out):
         check(L.sfx_fft_par_from_real_dev(ptr(real_in), ptr(c_out), np_, M, m_max_pad, stream_ptr()))
