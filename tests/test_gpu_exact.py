@@ -429,3 +429,24 @@ def test_div_full_size_sampled_and_closed_form():
     assert rel(out, q1) <= 1e-12
     idx = rng.choice(npt, 2000, replace=False)
     assert rel(out[idx], F.fft_par_div_dcl(c1[idx], q2[idx], 96)) <= TOL
+
+
+def test_empty_and_single_point_grid_routines():
+    """np = 0: reductions give the reference's empty values (max 0, min 0,
+    interface_ok 1), the other routines leave nothing to write; one point
+    with m_max = 1 (a constant): the planes are the constant."""
+    print("case grid routines np=0 and np=1", flush=True)
+    M = 4
+    z6, z2 = np.zeros((0, 6, M), order="F"), np.zeros((0, 2, M), order="F")
+    assert list(FP.FFT_MAX_MIN_VEL_DCL(None, z6)) == [0.0, 0.0]
+    assert FP.FFT_CHECK_INTERFACE(None, np.zeros((2, 0, 2, M), order="F"), 3) == 1
+    assert FP.FFT_PAR_REAL(None, z6).shape == (2 * M - 1, 3, 0)
+    assert FP.FFT_PAR_VAR_ETA_PROD_T_DCL(None, np.zeros((2 * (3 * M // 2) - 1, 0), order="F"), None, z2,
+                                         np.zeros((0, 2, M), order="F")).shape == (0, 2, M)
+    one = np.zeros((1, 2, 1), order="F")
+    one[0, 0, 0] = 2.5
+    R = FP.FFT_PAR_REAL(None, one, opt_nb_plane=9)
+    assert R.shape == (9, 1, 1) and np.all(R == 2.5)
+    back = np.zeros((1, 2, 1), order="F")
+    FP.FFT_PAR_FROM_REAL(np.asfortranarray(R[:, 0, :]), back, 5)
+    assert abs(back[0, 0, 0] - 2.5) <= 1e-15 and back[0, 1, 0] == 0.0
