@@ -29,6 +29,42 @@ __global__ void __launch_bounds__(EXACT_THREADS) exact_grid_kernel(ExactArgs a, 
   }
 }
 
+// FFT_PAR_REAL: the values of one field at 256 points and 16 angle pairs,
+// written through an LDS tile so that 16 consecutive lanes store 16
+// consecutive angles of one point (128-byte runs of its row), first the
+// angles j0..j0+15, then their mirrors N-j0-15..N-j0.
+constexpr int EXACT_REAL_JC = 16;
+__global__ void __launch_bounds__(EXACT_THREADS) exact_real_kernel(ExactArgs a, int nch) {
+  __shared__ double tile[EXACT_THREADS][EXACT_REAL_JC + 1];
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t p0 = (L / nch) * EXACT_THREADS;
+  const int j0 = (int)(L % nch) * EXACT_REAL_JC;
+  const int t = threadIdx.x;
+  const int64_t p = p0 + t;
+  const int64_t N = 2 * a.K + 1;
+  double E[1][EXACT_REAL_JC], O[1][EXACT_REAL_JC];
+  if (p < a.np) {
+    exact_c2r<1, EXACT_REAL_JC>(a, p, j0, E, O);
+  } else {
+#pragma unroll
+    for (int q = 0; q < EXACT_REAL_JC; ++q) E[0][q] = O[0][q] = 0.0;
+  }
+  for (int half = 0; half < 2; ++half) {
+#pragma unroll
+    for (int q = 0; q < EXACT_REAL_JC; ++q) tile[t][q] = half == 0 ? E[0][q] + O[0][q] : E[0][q] - O[0][q];
+    __syncthreads();
+    for (int i = t; i < EXACT_THREADS * EXACT_REAL_JC; i += EXACT_THREADS) {
+      const int pt = i / EXACT_REAL_JC, q = i % EXACT_REAL_JC;
+      const int64_t pp = p0 + pt;
+      const int jj = j0 + q;
+      if (pp >= a.np || jj > a.K || (half == 1 && jj == 0)) continue;
+      const int64_t ang = half == 0 ? jj : N - jj;
+      a.rout1[pp * a.rstride + ang] = tile[pt][q];
+    }
+    __syncthreads();
+  }
+}
+
 // Logical block -> (point block, output field and mode block).
 __global__ void __launch_bounds__(EXACT_THREADS) exact_r2c_kernel(ExactArgs a, int nmc) {
   const int64_t L = xcd_block(blockIdx.x, gridDim.x);
@@ -225,7 +261,13 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
     case EX_GRADMOM: e = grid_launch<EX_GRADMOM, 9>(a, st, &nblk); break;
     case EX_COMPR: e = grid_launch<EX_COMPR, 7>(a, st, &nblk); break;
     case EX_CLS: e = grid_launch<EX_CLS, 4>(a, st, &nblk); break;
-    case EX_REAL: e = grid_launch<EX_REAL, 1>(a, st, &nblk); break;
+    case EX_REAL: {
+      const int nch = (a.K + 1 + EXACT_REAL_JC - 1) / EXACT_REAL_JC;
+      const int64_t nb = ((a.np + EXACT_THREADS - 1) / EXACT_THREADS) * nch;
+      if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+      if (nb > 0) hipLaunchKernelGGL(exact_real_kernel, dim3((unsigned)nb), dim3(EXACT_THREADS), 0, st, a, nch);
+      return hipGetLastError();
+    }
     case EX_ETA: e = grid_launch<EX_ETA, 1>(a, st, &nblk); break;
     case EX_ETA3: e = grid_launch<EX_ETA3, 3>(a, st, &nblk); break;
     case EX_FROMREAL: e = grid_launch<EX_FROMREAL, 1>(a, st, &nblk); break;
