@@ -284,7 +284,7 @@ def wl_grid(torch, args):
     r = 8 * N * npt                                                 # bytes of one field on the grid
     phases = [
         Phase("par_real", lambda: FP.FFT_PAR_REAL(None, u, planes, opt_nb_plane=N), 6 * b + 3 * r, 3 * f_t * npt,
-              "exact_grid_kernel<REAL,1> x3"),
+              "exact_real_kernel x3"),
         Phase("from_real", lambda: FP.FFT_PAR_FROM_REAL(eta, back, mp), r + 2 * b, f_t * npt,
               "exact_grid_kernel<FROMREAL,1> + exact_r2c_kernel"),
         Phase("var_eta_prod", lambda: FP.FFT_PAR_VAR_ETA_PROD_T_DCL(None, eta, None, u, v_out), r + 12 * b,
