@@ -293,8 +293,8 @@ def wl_grid(torch, args):
                                                                                None, 7, 1.0, 0.5, 0.01),
               16 * b, 8 * f_t * npt, "exact_grid_kernel<MAXVEL,3> + exact_max_kernel + exact_grid_kernel<CLS,4> "
                                      "+ exact_r2c_kernel"),
-        Phase("max_min_vel", lambda: FP.FFT_MAX_MIN_VEL_DCL(None, u, V_out=mm), 18 * b, 9 * f_t * npt,
-              "exact_grid_kernel<MAXVEL,3> + exact_grid_kernel<VELSTAT,3> x2 + reductions"),
+        Phase("max_min_vel", lambda: FP.FFT_MAX_MIN_VEL_DCL(None, u, V_out=mm), 6 * b, 3 * f_t * npt,
+              "exact_velstat_kernel<3> + exact_velstat_final_kernel"),
         Phase("check_interface", lambda: FP.FFT_CHECK_INTERFACE(None, ls, 3, interface_ok=ok), 4 * b, 2 * f_t * npt,
               "exact_grid_kernel<CHECKIF,2> + exact_max_kernel"),
     ]

@@ -15,7 +15,7 @@ template <int OP, int NI>
 __global__ void __launch_bounds__(EXACT_THREADS) exact_grid_kernel(ExactArgs a, int nch) {
   const int64_t L = xcd_block(blockIdx.x, gridDim.x);
   const int64_t p = (L / nch) * EXACT_THREADS + threadIdx.x;
-  double vmax = OP == EX_VELSTAT ? -INFINITY : 0.0;
+  double vmax = 0.0;
   exact_grid_body<OP, NI>(a, p, (int)(L % nch), vmax);
   if constexpr (exact_is_reduction(OP)) {
     __shared__ double red[EXACT_THREADS];
@@ -183,21 +183,73 @@ __global__ void __launch_bounds__(EXACT_THREADS) exact_max_kernel(const double* 
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// FFT_MAX_MIN_VEL_DCL: out[1] = min of the positive |v| if |v| > 0 somewhere
-// on the last angle, else 0 (t[0] = max of -|v| over |v| > 0, t[1] the flag).
-__global__ void exact_minvel_kernel(const double* t, double* out) {
-  if (threadIdx.x == 0) out[1] = t[1] >= 1.0 ? -t[0] : 0.0;
+// FFT_MAX_MIN_VEL_DCL in one pass: three maxima per work-group into
+// part[3*block + k], then exact_velstat_final_kernel.
+template <int NI>
+__global__ void __launch_bounds__(EXACT_THREADS) exact_velstat_kernel(ExactArgs a, int nch, double* part) {
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t p = (L / nch) * EXACT_THREADS + threadIdx.x;
+  double r[3] = {0.0, -INFINITY, 0.0};
+  exact_velstat_body<NI>(a, p, (int)(L % nch), r);
+  __shared__ double red[3][EXACT_THREADS];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = r[k];
+  __syncthreads();
+  for (int s = EXACT_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * (int64_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[0] = max |v|; out[1] = min of the positive |v| if |v| > 0 somewhere on
+// the last angle, else 0 (the reference's loops, fft_parallel.F90:4563-4576).
+__global__ void __launch_bounds__(EXACT_THREADS) exact_velstat_final_kernel(const double* part, int64_t n,
+                                                                            double* out) {
+  __shared__ double red[3][EXACT_THREADS];
+  double r[3] = {0.0, -INFINITY, 0.0};
+  for (int64_t i = threadIdx.x; i < n; i += EXACT_THREADS)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = fmax(r[k], part[3 * i + k]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = r[k];
+  __syncthreads();
+  for (int s = EXACT_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = red[0][0];
+    out[1] = red[2][0] >= 1.0 ? -red[1][0] : 0.0;
+  }
+}
+
+int64_t exact_velstat_blocks(int nin, int64_t np, int K) { return exact_grid_blocks(EX_MAXVEL, nin, np, K); }
+
+hipError_t launch_exact_velstat(const ExactArgs& a, double* part, double* out, hipStream_t st) {
+  const int JC = exact_jc(EX_MAXVEL, a.nin);
+  const int nch = (a.K + 1 + JC - 1) / JC;
+  const int64_t nb = exact_grid_blocks(EX_MAXVEL, a.nin, a.np, a.K);
+  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (nb > 0) {
+    if (a.nin == 3) hipLaunchKernelGGL(exact_velstat_kernel<3>, dim3((unsigned)nb), dim3(EXACT_THREADS), 0, st, a,
+                                       nch, part);
+    else hipLaunchKernelGGL(exact_velstat_kernel<1>, dim3((unsigned)nb), dim3(EXACT_THREADS), 0, st, a, nch, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(exact_velstat_final_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)part, nb, out);
+  return hipGetLastError();
 }
 
 // FFT_CHECK_INTERFACE: interface_ok = 0 if a violation was found (flag null:
 // nothing was checked).
 __global__ void exact_checkif_kernel(const double* flag, int32_t* ok) {
   if (threadIdx.x == 0) ok[0] = (flag != nullptr && flag[0] >= 1.0) ? 0 : 1;
-}
-
-hipError_t launch_exact_minvel(const double* t, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(exact_minvel_kernel, dim3(1), dim3(64), 0, st, t, out);
-  return hipGetLastError();
 }
 
 hipError_t launch_exact_checkif(const double* flag, int32_t* ok, hipStream_t st) {
@@ -308,10 +360,6 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
       if (a.nin == 3) e = grid_launch<EX_MAXVEL, 3>(a, st, &nblk);
       else if (a.nin == 1) e = grid_launch<EX_MAXVEL, 1>(a, st, &nblk);
       break;
-    case EX_VELSTAT:
-      if (a.nin == 3) e = grid_launch<EX_VELSTAT, 3>(a, st, &nblk);
-      else if (a.nin == 1) e = grid_launch<EX_VELSTAT, 1>(a, st, &nblk);
-      break;
     case EX_CHECKIF:
       switch (a.nin) {
         case 2: e = grid_launch<EX_CHECKIF, 2>(a, st, &nblk); break;
@@ -322,9 +370,8 @@ hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t
   }
   if (e != hipSuccess) return e;
   if (exact_is_reduction(op)) {
-    const double init = op == EX_VELSTAT ? -INFINITY : 0.0;
     hipLaunchKernelGGL(exact_max_kernel, dim3(1), dim3(EXACT_THREADS), 0, st, (const double*)a.pmax, nblk, max_out,
-                       init);
+                       0.0);
     return hipGetLastError();
   }
   const int nmc = (a.M + EXACT_MC - 1) / EXACT_MC;

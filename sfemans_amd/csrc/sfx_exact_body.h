@@ -35,7 +35,7 @@ enum ExactOp { EX_DIV = 0, EX_SV2 = 1, EX_SQRT = 2, EX_HEAVI = 3, EX_TENSION = 4
 
 // Operations reduced to one value over all points and angles (a maximum per
 // work-group, then one reduction block) instead of transformed back.
-constexpr bool exact_is_reduction(int op) { return op == EX_MAXVEL || op == EX_VELSTAT || op == EX_CHECKIF; }
+constexpr bool exact_is_reduction(int op) { return op == EX_MAXVEL || op == EX_CHECKIF; }
 
 // Angles per thread of the angle pass (2*NI*JC accumulators; the tension
 // tensor, 96 of them, re-reads its six inputs fewest times at 8).
@@ -288,16 +288,6 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
     if constexpr (OP == EX_MAXVEL || OP == EX_CHECKIF) {
       vmax = fmax(vmax, wp[0]);
       if (j > 0) vmax = fmax(vmax, wm[0]);
-    } else if constexpr (OP == EX_VELSTAT) {
-      // n_int 0: the maximum of -|v| over the angles where |v| > 0 (running
-      // value starts at -inf); n_int 1: 1 if |v| > 0 at the last angle N-1,
-      // the mirror of j = 1 (j = 0 itself when N = 1)
-      if (a.n_int == 0) {
-        if (wp[0] > 0.0) vmax = fmax(vmax, -wp[0]);
-        if (j > 0 && wm[0] > 0.0) vmax = fmax(vmax, -wm[0]);
-      } else if (a.K == 0 ? wp[0] > 0.0 : (j == 1 && wm[0] > 0.0)) {
-        vmax = fmax(vmax, 1.0);
-      }
     } else if constexpr (OP == EX_REAL) {
       // real planes: the value at angle j of point p at rout1[p*rstride + j]
       const int64_t N = 2 * a.K + 1;
@@ -319,6 +309,39 @@ SFX_DEV void exact_grid_body(const ExactArgs& a, int64_t p, int jc, double& vmax
         a.W[o + plane] = j > 0 ? wp[g] - wm[g] : 0.0;
       }
     }
+  }
+}
+
+// FFT_MAX_MIN_VEL_DCL in one pass: at point p, angle block jc (JC angles and
+// their mirrors), the running maximum of |v| (r[0]), of -|v| over |v| > 0
+// (r[1], starts at -inf) and whether |v| > 0 at the last angle N-1 (r[2]:
+// 1 or 0: the last angle is the mirror of j = 1, or j = 0 itself when
+// N = 1); |v| from three fields (nin = 3) or the first one.
+template <int NI, int JC = exact_jc(EX_MAXVEL, NI)>
+SFX_DEV void exact_velstat_body(const ExactArgs& a, int64_t p, int jc, double (&r)[3]) {
+  if (p >= a.np) return;
+  const int j0 = jc * JC;
+  double E[NI][JC], O[NI][JC];
+  exact_c2r<NI, JC>(a, p, j0, E, O);
+#pragma unroll
+  for (int q = 0; q < JC; ++q) {
+    const int j = j0 + q;
+    if (j > a.K) break;
+    double up[NI], um[NI], wp[1], wm[1];
+#pragma unroll
+    for (int f = 0; f < NI; ++f) {
+      up[f] = E[f][q] + O[f][q];
+      um[f] = E[f][q] - O[f][q];
+    }
+    exact_op<EX_MAXVEL, NI, 0>(a, up, wp, 0.0, 0.0);
+    exact_op<EX_MAXVEL, NI, 0>(a, um, wm, 0.0, 0.0);
+    r[0] = fmax(r[0], wp[0]);
+    if (wp[0] > 0.0) r[1] = fmax(r[1], -wp[0]);
+    if (j > 0) {
+      r[0] = fmax(r[0], wm[0]);
+      if (wm[0] > 0.0) r[1] = fmax(r[1], -wm[0]);
+    }
+    if (a.K == 0 ? wp[0] > 0.0 : (j == 1 && wm[0] > 0.0)) r[2] = 1.0;
   }
 }
 

@@ -264,17 +264,15 @@ static int real_dev(const double* V, double* out, int64_t np, int nb, int M, int
 // V1 (np, nb, M) -> out[0] = max |v|, out[1] = the reference's minimum (device)
 static int maxminvel_dev(const double* V1, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
   const char* nm = "FFT_MAX_MIN_VEL_DCL";
-  void* t = nullptr;
-  int rc = staging(16, 2 * 8, &t);
-  if (rc || (rc = maxvel_dev(V1, out, np, nb, M, mp, st))) return rc;
-  for (int mode = 0; mode < 2; ++mode) {
-    ExactArgs a;
-    exact_setup_velstat(a, V1, np, nb, M, mode);
-    if ((rc = exact_prepare(nm, a, EX_VELSTAT, np, M, mp)) ||
-        (rc = exact_run(nm, EX_VELSTAT, a, (double*)t + mode, st)))
-      return rc;
-  }
-  hipError_t e = launch_exact_minvel((const double*)t, out, st);
+  ExactArgs a;
+  exact_setup_maxvel(a, V1, np, nb, M);
+  int rc = exact_prepare(nm, a, EX_MAXVEL, np, M, mp);
+  if (rc) return rc;
+  // one pass: three maxima per work-group, then one reduction block
+  void* part = nullptr;
+  const int64_t nblk = exact_velstat_blocks(a.nin, np, a.K);
+  if ((rc = staging(16, (size_t)(nblk > 0 ? nblk : 1) * 3 * 8, &part))) return rc;
+  hipError_t e = launch_exact_velstat(a, (double*)part, out, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
 

@@ -314,13 +314,6 @@ inline void exact_setup_fromreal(ExactArgs& a, const double* real, double* c, in
   a.n_creal = (2 * (int64_t)m_max_pad - 1) * np;
 }
 
-// FFT_MAX_MIN_VEL_DCL statistics (mode 0: max of -|v| where |v| > 0; mode
-// 1: |v| > 0 on the last angle), V1 (np, nb, M) as for the maximum.
-inline void exact_setup_velstat(ExactArgs& a, const double* V1, int64_t np, int nb, int M, int mode) {
-  exact_setup_maxvel(a, V1, np, nb, M);
-  a.n_int = mode;
-}
-
 // FFT_CHECK_INTERFACE: V1 (n_int, np, 2, M) level sets, as for the Heaviside.
 inline void exact_setup_checkif(ExactArgs& a, const double* V1, int n_int, int64_t np, int M) {
   memset(&a, 0, sizeof(a));

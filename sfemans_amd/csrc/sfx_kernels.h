@@ -64,7 +64,8 @@ hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
 // Exact-grid products (sfx_exact_body.h): angle pass, then r2c, or the maximum into max_out.
 struct ExactArgs;
 hipError_t launch_exact(int op, const ExactArgs& a, double* max_out, hipStream_t st);
-hipError_t launch_exact_minvel(const double* t, double* out, hipStream_t st);
+hipError_t launch_exact_velstat(const ExactArgs& a, double* part, double* out, hipStream_t st);
+int64_t exact_velstat_blocks(int nin, int64_t np, int K);
 hipError_t launch_exact_checkif(const double* flag, int32_t* ok, hipStream_t st);
 int64_t exact_grid_blocks(int op, int nin, int64_t np, int K);
 // FFT_COMPUTE_ENTROPY_VISC_MOM: element maxima (pmax: exact_emom_blocks doubles),

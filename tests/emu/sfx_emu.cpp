@@ -474,19 +474,21 @@ extern "C" int emu_exact(int op, int64_t np, int M, int mp, const double* in1, c
   if (op == EX_VELSTAT) {   // 15 FFT_MAX_MIN_VEL_DCL: in1 (np, nb, M) -> out[0] max, out[1] min
     int rc = emu_exact(EX_MAXVEL, np, M, mp, in1, nullptr, nullptr, 0, 0.0, nb, out);
     if (rc) return rc;
-    double t[2];
+    exact_setup_maxvel(a, in1, np, nb, M);
+    exact_setup_sizes(a, np, M, mp);
     std::vector<double> tab;
-    for (int mode = 0; mode < 2; ++mode) {
-      exact_setup_velstat(a, in1, np, nb, M, mode);
-      exact_setup_sizes(a, np, M, mp);
-      if (mode == 0) exact_host_tables(M, a.K, tab);
-      a.ctab = tab.data();
-      a.rtab = tab.data() + exact_ctab_size(M, a.K);
-      t[mode] = -INFINITY;
-      if (a.nin == 3) emu_exact_grid<EX_VELSTAT, 3>(a, t[mode]);
-      else emu_exact_grid<EX_VELSTAT, 1>(a, t[mode]);
-    }
-    out[1] = t[1] >= 1.0 ? -t[0] : 0.0;
+    exact_host_tables(M, a.K, tab);
+    a.ctab = tab.data();
+    a.rtab = tab.data() + exact_ctab_size(M, a.K);
+    double r[3] = {0.0, -INFINITY, 0.0};
+    const int JC = exact_jc(EX_MAXVEL, a.nin), nch = (a.K + 1 + JC - 1) / JC;
+    for (int64_t p = 0; p < np; ++p)
+      for (int jc = 0; jc < nch; ++jc) {
+        if (a.nin == 3) exact_velstat_body<3>(a, p, jc, r);
+        else exact_velstat_body<1>(a, p, jc, r);
+      }
+    if (r[0] != out[0]) return 3;   // the one-pass maximum equals FFT_MAX_VEL_DCL's
+    out[1] = r[2] >= 1.0 ? -r[1] : 0.0;
     return 0;
   }
   if (op == EX_CHECKIF) {   // 16 FFT_CHECK_INTERFACE: in1 (nb - 1, np, 2, M), nb = nb_fluids -> out[0] = ok
