@@ -1,4 +1,4 @@
-"""Navier-Stokes restart files of SFEMaNS (``suite_ns*``): reader and writer.
+"""Navier-Stokes and Maxwell restart files of SFEMaNS (``suite_ns*``, ``suite_maxwell*``).
 
 The files are Fortran sequential unformatted files: each record is framed by
 its length in bytes as a 4-byte little-endian integer, before and after.
@@ -14,7 +14,12 @@ read_restart_ns :106-337):
   ``level_set(:,:,:,i)`` (n_int, np_v, 2), ``level_set_m1`` and ``max_vel``
   (:82-94).
 
-The reader returns the fields for the modes the caller asks for, with the
+Maxwell restarts (``suite_maxwell*``, write_restart_maxwell :1019-1058,
+read_restart_maxwell :1064-1275) have the same header; per mode the mode
+number, Hn, Hn1, Bn, Bn1 (np_H, 6) and phin, phin1 (np_phi, 2), each
+replaced by a record holding the integer 1 when its mesh is empty.
+
+The readers return the fields for the modes the caller asks for, with the
 reference's rules: a mode absent from the file is filled with ``val_init``
 or 0 (:284-306); with ``interpol`` the i-th mode of the rank's share is
 taken whatever its number and ``list_mode`` is updated (:244-255); with
@@ -29,7 +34,8 @@ import struct
 
 import numpy as np
 
-__all__ = ["suite_ns_name", "read_records", "write_records", "read_restart_ns", "write_restart_ns"]
+__all__ = ["suite_ns_name", "suite_maxwell_name", "read_records", "write_records", "read_restart_ns",
+           "write_restart_ns", "read_restart_maxwell", "write_restart_maxwell"]
 
 
 def suite_ns_name(filename: str, rang_S: int = 0, mono: bool = False, it: int | None = None) -> str:
@@ -41,6 +47,11 @@ def suite_ns_name(filename: str, rang_S: int = 0, mono: bool = False, it: int | 
     if it is not None:
         parts += f"_I{it:03d}"
     return f"{parts}.{filename}"
+
+
+def suite_maxwell_name(filename: str, rang_S: int = 0, mono: bool = False, it: int | None = None) -> str:
+    """File name of the Maxwell restart: suite_maxwell[_S<rank>][_I<it>].<filename>."""
+    return "suite_maxwell" + suite_ns_name(filename, rang_S, mono, it)[len("suite_ns"):]
 
 
 def read_records(path: str) -> list[bytes]:
@@ -182,4 +193,83 @@ def read_restart_ns(path, list_mode, np_v, np_p, n_int=None, mono=False, val_ini
     res["list_mode"] = list_mode
     if n_int is not None:
         res["max_vel"] = max_vel
+    return res
+
+
+def write_restart_maxwell(path, time, list_mode, Hn, Hn1, Bn, Bn1, phin, phin1, dt=None, mono=False, nb_procs_S=1,
+                          nb_procs_F=1, append=False, H_empty=False, phi_empty=False):
+    """Write a Maxwell restart as write_restart_maxwell does (restart.F90:1019-1058):
+    per mode the mode number, Hn, Hn1, Bn, Bn1 (np_H, 6) or four records
+    holding the integer 1 when the H mesh is empty, then phin, phin1
+    (np_phi, 2) or two such records when the phi mesh is empty."""
+    Hn = np.asarray(Hn, dtype=np.float64)
+    phin = np.asarray(phin, dtype=np.float64)
+    one = struct.pack("<i", 1)
+    recs = []
+    if not append:
+        if mono:
+            h = struct.pack("<diiii", float(time), 0 if H_empty else Hn.shape[0], 0 if phi_empty else phin.shape[0],
+                            nb_procs_F, len(list_mode))
+        else:
+            h = struct.pack("<diii", float(time), nb_procs_S, nb_procs_F, len(list_mode))
+        if dt is not None:
+            h += struct.pack("<d", float(dt))
+        recs.append(h)
+    for i, mode in enumerate(list_mode):
+        recs.append(struct.pack("<i", int(mode)))
+        for a in (Hn, Hn1, Bn, Bn1):
+            recs.append(one if H_empty else np.asarray(a, dtype="<f8")[..., i].tobytes(order="F"))
+        for a in (phin, phin1):
+            recs.append(one if phi_empty else np.asarray(a, dtype="<f8")[..., i].tobytes(order="F"))
+    write_records(path, recs, append=append)
+
+
+def read_restart_maxwell(path, list_mode, np_H, np_phi, mono=False, val_init=None, interpol=False, dt=None,
+                         rang_F=0, nb_procs_S=None):
+    """Read the modes ``list_mode`` of a Maxwell restart as read_restart_maxwell
+    does (restart.F90:1064-1275).  np_H = 0 (np_phi = 0) stands for an empty
+    H (phi) mesh, whose records are skipped.  Returns Hn, Hn1, Bn, Bn1
+    (np_H, 6, m), phin, phin1 (np_phi, 2, m), list_mode and the header
+    values; missing modes, val_init, interpol and dt as for read_restart_ns."""
+    recs = read_records(path)
+    if not recs:
+        raise ValueError(f"{path}: empty restart file")
+    hd = _header(recs[0], mono, dt is not None)
+    if nb_procs_S is not None and not mono and hd["nb_procs_S"] != nb_procs_S:
+        raise ValueError("BUG in read_restart: nb_procs_Sr /= nb_procs_S")
+    list_mode = [int(m) for m in list_mode]
+    M = len(list_mode)
+    nb_mode_r = hd["nb_procs_F"] * hd["m_max_c"]
+    if len(recs) < 1 + nb_mode_r * 7:
+        raise ValueError(f"{path}: {len(recs)} records, expected {1 + nb_mode_r * 7}")
+    names = [("Hn", (np_H, 6)), ("Hn1", (np_H, 6)), ("Bn", (np_H, 6)), ("Bn1", (np_H, 6)),
+             ("phin", (np_phi, 2)), ("phin1", (np_phi, 2))]
+    out = {n: np.zeros(sh + (M,), order="F") for n, sh in names}
+    for i in range(M):
+        found = False
+        for j in range(nb_mode_r):
+            base = 1 + j * 7
+            (mode,) = struct.unpack("<i", recs[base][:4])
+            if interpol:
+                if j != rang_F * M + i:
+                    continue
+                list_mode[i] = mode
+            if mode != list_mode[i]:
+                continue
+            for k, (n, sh) in enumerate(names):
+                if sh[0] > 0:
+                    out[n][..., i] = _farray(recs[base + 1 + k], sh)
+            found = True
+            break
+        if not found:
+            fill = 0.0 if val_init is None else float(val_init)
+            for n in out:
+                out[n][..., i] = fill
+    if dt is not None and abs((dt - hd["dt"]) / dt) > 1e-4:
+        r = dt / hd["dt"]
+        for a, b in (("Hn1", "Hn"), ("Bn1", "Bn"), ("phin1", "phin")):
+            out[a] = r * out[a] + (1.0 - r) * out[b]
+    res = dict(hd)
+    res.update(out)
+    res["list_mode"] = list_mode
     return res

@@ -7,6 +7,9 @@
 ! usage: write_suite_ns <file> <variant>
 !   variant 0: per sub-domain header with dt, two level sets, modes 0 3 5
 !   variant 1: mono header without dt, no level sets, modes 2 7
+!   variant 2: Maxwell file (write_restart_maxwell, restart.F90:1019-1058), per
+!              sub-domain header with dt, H and B on 7 nodes, phi on 4, modes 0 3 5
+!   variant 3: Maxwell file, mono header, empty phi mesh, modes 2 7
 ! Values: un(p,k,i) = p + 10 k + 100 i + 0.25 and similar (see below).
 !===
 PROGRAM write_suite_ns
@@ -21,7 +24,7 @@ PROGRAM write_suite_ns
   CALL GET_COMMAND_ARGUMENT(2, arg)
   READ(arg, *) variant
   np_v = 7; np_p = 4; time = 1.5d0; dt = 0.01d0; max_vel = 2.5d0
-  IF (variant == 0) THEN
+  IF (variant == 0 .OR. variant == 2) THEN
      m = 3
      ALLOCATE(list_mode(m)); list_mode = (/0, 3, 5/)
   ELSE
@@ -53,6 +56,30 @@ PROGRAM write_suite_ns
      END DO
   END DO
   OPEN(UNIT=10, FILE=TRIM(fname), FORM='unformatted', STATUS='replace')
+  IF (variant >= 2) THEN
+     ! H, H_m1, B, B_m1 = un, un_m1, un + 0.5, un_m1 - 0.5; phi = pn, phi_m1 = pn_m1
+     IF (variant == 2) THEN
+        WRITE(10) time, 1, 1, SIZE(list_mode), dt
+     ELSE
+        WRITE(10) time, np_v, 0, 1, SIZE(list_mode)
+     END IF
+     DO i = 1, SIZE(list_mode)
+        WRITE(10) list_mode(i)
+        WRITE(10) un(:,:,i)
+        WRITE(10) un_m1(:,:,i)
+        WRITE(10) un(:,:,i) + 0.5d0
+        WRITE(10) un_m1(:,:,i) - 0.5d0
+        IF (variant == 2) THEN
+           WRITE(10) pn(:,:,i)
+           WRITE(10) pn_m1(:,:,i)
+        ELSE
+           WRITE(10) 1
+           WRITE(10) 1
+        END IF
+     END DO
+     CLOSE(10)
+     STOP
+  END IF
   IF (variant == 0) THEN
      WRITE(10) time, 1, 1, SIZE(list_mode), dt
   ELSE

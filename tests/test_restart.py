@@ -42,7 +42,7 @@ def fortran_files(tmp_path_factory):
     subprocess.run([FC, "-O0", os.path.join(HERE, "fortran", "write_suite_ns.F90"), "-o", str(exe)],
                    check=True, cwd=d, timeout=300)
     files = {}
-    for v in (0, 1):
+    for v in (0, 1, 2, 3):
         f = d / f"suite_ns_{v}"
         subprocess.run([str(exe), str(f), str(v)], check=True, timeout=60)
         files[v] = str(f)
@@ -108,3 +108,32 @@ def test_corrupt_file_is_rejected(tmp_path):
     p.write_bytes(b"\x08\x00\x00\x00" + b"\x00" * 8 + b"\x07\x00\x00\x00")
     with pytest.raises(ValueError):
         R.read_records(str(p))
+
+
+def test_reads_fortran_written_maxwell_files(fortran_files):
+    ex = expected(7, 4, 3)
+    r = R.read_restart_maxwell(fortran_files[2], [5, 0, 4], 7, 4, dt=0.01, nb_procs_S=1)
+    assert np.array_equal(r["Hn"][..., 0], ex["un"][..., 2]) and np.array_equal(r["Hn1"][..., 1], ex["un_m1"][..., 0])
+    assert np.array_equal(r["Bn"][..., 0], ex["un"][..., 2] + 0.5)
+    assert np.array_equal(r["Bn1"][..., 1], ex["un_m1"][..., 0] - 0.5)
+    assert np.array_equal(r["phin"][..., 0], ex["pn"][..., 2]) and np.array_equal(r["phin1"][..., 1], ex["pn_m1"][..., 0])
+    assert np.all(r["Hn"][..., 2] == 0.0)                                   # mode 4 absent
+    r = R.read_restart_maxwell(fortran_files[2], [0, 3, 5], 7, 4, dt=0.005)
+    assert np.allclose(r["Bn1"], 0.5 * (ex["un_m1"] - 0.5) + 0.5 * (ex["un"] + 0.5), atol=1e-12)
+    ex2 = expected(7, 4, 2)
+    r = R.read_restart_maxwell(fortran_files[3], [7, 2], 7, 0, mono=True)    # empty phi mesh
+    assert (r["np_v"], r["np_p"]) == (7, 0) and r["phin"].shape == (0, 2, 2)
+    assert np.array_equal(r["Hn"][..., 1], ex2["un"][..., 0]) and np.array_equal(r["Bn"][..., 0], ex2["un"][..., 1] + 0.5)
+
+
+def test_maxwell_writer_round_trip(tmp_path):
+    rng = np.random.default_rng(8)
+    f = {"Hn": rng.standard_normal((9, 6, 2)), "Hn1": rng.standard_normal((9, 6, 2)),
+         "Bn": rng.standard_normal((9, 6, 2)), "Bn1": rng.standard_normal((9, 6, 2)),
+         "phin": rng.standard_normal((3, 2, 2)), "phin1": rng.standard_normal((3, 2, 2))}
+    p = str(tmp_path / R.suite_maxwell_name("run", mono=True))
+    R.write_restart_maxwell(p, 2.0, [0, 6], **f, mono=True)
+    r = R.read_restart_maxwell(p, [6, 0], 9, 3, mono=True)
+    for k, v in f.items():
+        assert np.array_equal(r[k], v[..., ::-1])
+    assert os.path.basename(p) == "suite_maxwell.run"
