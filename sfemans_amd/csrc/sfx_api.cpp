@@ -74,13 +74,15 @@ int get_twiddles(int MP, Twiddles* tw) {
 }
 
 // ---- grow-only device staging buffers for the host-pointer entry points
+constexpr int N_STAGING = 20;   // staging slots per device
 struct Staging {
-  void* p[20] = {nullptr};
-  size_t n[20] = {0};
+  void* p[N_STAGING] = {nullptr};
+  size_t n[N_STAGING] = {0};
 };
 static std::map<int, Staging> g_staging;
 
 int staging(int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot >= N_STAGING) return fail(SFX_ERR_ARG, "staging slot %d out of range", slot);
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");

@@ -43,7 +43,8 @@ static int exact_tables(int M, int K, const double** ctab, const double** rtab) 
 
 // Staging slots of the exact-grid products (the FFT products use 0-2).
 enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX_MAX = 7, SLOT_EX_IN3 = 8,
-       SLOT_EX_OUT2 = 9, SLOT_EX_IN4 = 10, SLOT_EX_NINT = 11 };
+       SLOT_EX_OUT2 = 9, SLOT_EX_IN4 = 10, SLOT_EX_NINT = 11, SLOT_EX_IN5 = 12, SLOT_EX_OUT3 = 14,
+       SLOT_EX_AUX = 15, SLOT_EX_STAT = 16, SLOT_EX_FLAG = 17 };
 
 // Common set-up: sizes, tables, scratch.  `a.in`, `a.out`, nin, nout filled by the caller.
 static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad) {
@@ -157,7 +158,7 @@ static int evisc_dev(const double* const (&V)[5], const double* hloc, double* co
   a.nint = (double*)w;
   // the viscosity on the grid between the two product passes (N, np)
   void* vr = nullptr;
-  if ((rc = staging(15, (size_t)(2 * (int64_t)mp - 1) * (np > 0 ? np : 1) * 8, &vr))) return rc;
+  if ((rc = staging(SLOT_EX_AUX, (size_t)(2 * (int64_t)mp - 1) * (np > 0 ? np : 1) * 8, &vr))) return rc;
   a.rout2 = (double*)vr;
   if ((rc = exact_prepare(nm, a, EX_EVISC, np, M, mp))) return rc;
   return exact_run(nm, EX_EVISC, a, nullptr, st);
@@ -182,7 +183,7 @@ static int emom_dev(const double* V1, const double* V2, const double* V3, const 
   const int64_t ne = (np / lg) * (2 * (int64_t)mp - 1);
   void *w = nullptr, *pm = nullptr, *dm = nullptr;
   if ((rc = staging(SLOT_EX_NINT, (size_t)3 * ne * 8, &w)) ||
-      (rc = staging(15, (size_t)exact_emom_blocks(np, lg, a.K) * 8, &pm)) || (rc = staging(SLOT_EX_MAX, 8, &dm)))
+      (rc = staging(SLOT_EX_AUX, (size_t)exact_emom_blocks(np, lg, a.K) * 8, &pm)) || (rc = staging(SLOT_EX_MAX, 8, &dm)))
     return rc;
   a.nint = (double*)w;
   a.nint2 = a.nint + ne;
@@ -271,7 +272,7 @@ static int maxminvel_dev(const double* V1, double* out, int64_t np, int nb, int 
   // one pass: three maxima per work-group, then one reduction block
   void* part = nullptr;
   const int64_t nblk = exact_velstat_blocks(a.nin, np, a.K);
-  if ((rc = staging(16, (size_t)(nblk > 0 ? nblk : 1) * 3 * 8, &part))) return rc;
+  if ((rc = staging(SLOT_EX_STAT, (size_t)(nblk > 0 ? nblk : 1) * 3 * 8, &part))) return rc;
   hipError_t e = launch_exact_velstat(a, (double*)part, out, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
@@ -283,7 +284,7 @@ static int checkif_dev(const double* V1, int nb_fluids, int32_t* ok, int64_t np,
   if (n_int < 1) return fail(SFX_ERR_ARG, "%s: nb_fluids=%d < 2", nm, nb_fluids);
   if (n_int > EXACT_MAX_INT) return fail(SFX_ERR_UNSUPPORTED, "%s: %d level sets > %d", nm, n_int, EXACT_MAX_INT);
   void* f = nullptr;
-  int rc = staging(17, 8, &f);
+  int rc = staging(SLOT_EX_FLAG, 8, &f);
   if (rc) return rc;
   if (n_int == 1) {
     f = nullptr;   // two fluids: nothing to compare (:2390-2392), interface_ok = 1
@@ -639,11 +640,11 @@ int sfx_fft_compute_entropy_visc(const double* V1_in, const double* V2_in, const
   HostRun t;
   t.t0 = now_s();
   const double* src[5] = {V1_in, V2_in, V3_in, V4_in, V5_in};
-  const int slots[5] = {SLOT_EX_IN1, SLOT_EX_IN2, SLOT_EX_IN3, SLOT_EX_IN4, 12};
+  const int slots[5] = {SLOT_EX_IN1, SLOT_EX_IN2, SLOT_EX_IN3, SLOT_EX_IN4, SLOT_EX_IN5};
   for (int i = 0; i < 5; ++i)
     if ((rc = to_dev(slots[i], src[i], n1, &d[i]))) return rc;
   if ((rc = to_dev(13, hloc_gauss, (size_t)np, &dh)) || (rc = staging(SLOT_EX_OUT, n1 * 8, &o1)) ||
-      (rc = staging(SLOT_EX_OUT2, n1 * 8, &o2)) || (rc = staging(14, n1 * 8, &o3)))
+      (rc = staging(SLOT_EX_OUT2, n1 * 8, &o2)) || (rc = staging(SLOT_EX_OUT3, n1 * 8, &o3)))
     return rc;
   t.t1 = now_s();
   const double* V[5] = {(const double*)d[0], (const double*)d[1], (const double*)d[2], (const double*)d[3],
