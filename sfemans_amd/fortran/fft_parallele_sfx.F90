@@ -7,7 +7,9 @@
 ! (FFT_PAR_CROSS_PROD_DCL :147, FFT_PAR_DOT_PROD_DCL :330, FFT_PAR_PROD_DCL :502,
 ! FFT_PAR_DIV_DCL :667, FFT_HEAVISIDE_DCL :832, FFT_SCALAR_VECT_DCL :1021,
 ! FFT_MAX_VEL_DCL :1426, FFT_TENSOR_DCL :1543, FFT_SQUAREROOT_SCALAR :4601,
-! FFT_PAR_REAL :28, FFT_CHECK_INTERFACE :2267, FFT_MAX_MIN_VEL_DCL :4467).  The communicator is the
+! FFT_PAR_REAL :28, FFT_CHECK_INTERFACE :2267, FFT_MAX_MIN_VEL_DCL :4467,
+! FFT_PAR_SCAL_FUNCT :4294: the real planes on the GPU, the caller's funct
+! here, then the analysis on the GPU).  The communicator is the
 ! legacy integer MPI handle; nb_procs must be 1.  A non-zero status from
 ! libsfx stops the program with the library's message, as the reference does
 ! with WRITE + STOP or error_petsc.  Non-contiguous actual arguments (array
@@ -20,6 +22,7 @@ MODULE fft_parallele
   PUBLIC :: FFT_PAR_CROSS_PROD_DCL, FFT_PAR_DOT_PROD_DCL, FFT_PAR_PROD_DCL, &
        FFT_SCALAR_VECT_DCL, FFT_TENSOR_DCL, FFT_PAR_DIV_DCL, FFT_HEAVISIDE_DCL, &
        FFT_SQUAREROOT_SCALAR, FFT_MAX_VEL_DCL, FFT_MAX_MIN_VEL_DCL, FFT_PAR_REAL, FFT_CHECK_INTERFACE, &
+       FFT_PAR_SCAL_FUNCT, &
        sfx_mesh_register, sfx_mesh_release, &
        sfx_smb_cross_prod_gauss, sfx_smb_curlh_cross_b_gauss
 
@@ -113,6 +116,13 @@ MODULE fft_parallele
        INTEGER(C_INT64_T), VALUE :: np, bloc
        INTEGER(C_INT) :: rc
      END FUNCTION c_checkif
+     FUNCTION c_from_real(R, Co, np, m, mpad) BIND(C, NAME='sfx_fft_par_from_real') RESULT(rc)
+       IMPORT :: C_PTR, C_INT32_T, C_INT64_T, C_INT
+       TYPE(C_PTR), VALUE :: R, Co
+       INTEGER(C_INT64_T), VALUE :: np
+       INTEGER(C_INT32_T), VALUE :: m, mpad
+       INTEGER(C_INT) :: rc
+     END FUNCTION c_from_real
      FUNCTION c_real_n(m, nplane) BIND(C, NAME='sfx_fft_par_real_n') RESULT(n)
        IMPORT :: C_INT32_T
        INTEGER(C_INT32_T), VALUE :: m, nplane
@@ -386,6 +396,45 @@ CONTAINS
     V_out = ao(:, 1:SIZE(V1_in,2)/2, 1:SIZE(V1_in,1))
     DEALLOCATE(a1, ao)
   END SUBROUTINE FFT_PAR_REAL
+
+  SUBROUTINE FFT_PAR_SCAL_FUNCT(communicator, c1_inout, funct, nb_procs, bloc_size, m_max_pad, temps)
+    INTEGER :: communicator
+    REAL(KIND=8), DIMENSION(:,:,:), INTENT(INOUT) :: c1_inout
+    INTERFACE
+       FUNCTION funct(x) RESULT(vv)
+         REAL(KIND=8) :: x
+         REAL(KIND=8) :: vv
+       END FUNCTION funct
+    END INTERFACE
+    INTEGER, INTENT(IN) :: nb_procs, bloc_size, m_max_pad
+    REAL(KIND=8), DIMENSION(:), OPTIONAL, INTENT(INOUT) :: temps
+    REAL(KIND=8), DIMENSION(:,:,:), ALLOCATABLE, TARGET :: a1, ao
+    REAL(KIND=8), DIMENSION(:,:), ALLOCATABLE, TARGET :: rv
+    INTEGER(C_INT32_T) :: n
+    INTEGER(C_INT) :: rc
+    INTEGER :: i, j, np, m
+    IF (PRESENT(temps)) temps = 0.d0
+    np = SIZE(c1_inout,1); m = SIZE(c1_inout,3)
+    IF (nb_procs /= 1 .OR. m_max_pad < m) THEN
+       WRITE(*,*) 'FFT_PAR_SCAL_FUNCT: nb_procs must be 1 and m_max_pad >= m_max_c'
+       STOP
+    END IF
+    n = 2*m_max_pad - 1
+    ALLOCATE(a1(np, 2, m), ao(np, 2, m), rv(n, MAX(np,1)))
+    a1 = c1_inout(:, 1:2, :)
+    ! values of the first field on the n grid angles (opt_nb_plane = n)
+    rc = c_real(C_LOC(a1), C_LOC(rv), INT(np,C_INT64_T), 2_C_INT32_T, INT(m,C_INT32_T), 1_C_INT32_T, n)
+    CALL sfx_stop(rc, 'FFT_PAR_SCAL_FUNCT')
+    DO j = 1, np
+       DO i = 1, n
+          rv(i,j) = funct(rv(i,j))
+       END DO
+    END DO
+    rc = c_from_real(C_LOC(rv), C_LOC(ao), INT(np,C_INT64_T), INT(m,C_INT32_T), INT(m_max_pad,C_INT32_T))
+    CALL sfx_stop(rc, 'FFT_PAR_SCAL_FUNCT')
+    c1_inout(:, 1:2, :) = ao
+    DEALLOCATE(a1, ao, rv)
+  END SUBROUTINE FFT_PAR_SCAL_FUNCT
 
   SUBROUTINE FFT_MAX_MIN_VEL_DCL(communicator, V1_in, V_out, nb_procs, bloc_size, m_max_pad)
     INTEGER :: communicator

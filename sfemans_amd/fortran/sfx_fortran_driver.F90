@@ -14,7 +14,7 @@
 ! writes <dir>/cross.bin (np,6,m), dot.bin (np,2,m), div.bin (np,2,m) = v1(:,1:2)/den,
 !   sqrt.bin (np,2,m) = SQRT(MAX(0,MIN(1,den))), maxvel.bin (1) = max |v1|, smb.bin (7*me,6,m),
 !   maxmin.bin (2) = FFT_MAX_MIN_VEL_DCL of v1, real.bin (2*m_max+1,3,np) = FFT_PAR_REAL of v1
-!   with opt_nb_plane = 2*m_max+1
+!   with opt_nb_plane = 2*m_max+1, scal.bin (np,2,m) = FFT_PAR_SCAL_FUNCT of den with funct(x) = x*x
 !===
 PROGRAM sfx_fortran_driver
   USE, INTRINSIC :: ISO_C_BINDING
@@ -61,6 +61,9 @@ PROGRAM sfx_fortran_driver
   CALL write_real(TRIM(dir)//'/maxmin.bin', vmm)
   CALL FFT_PAR_REAL(comm, v1, vreal, 2*m_max+1)
   CALL write_real(TRIM(dir)//'/real.bin', vreal)
+  vr = den
+  CALL FFT_PAR_SCAL_FUNCT(comm, vr, square, 1, np, 3*m_max/2)
+  CALL write_real(TRIM(dir)//'/scal.bin', vr)
 
   ALLOCATE(jj32(6,me), jj(6,me), rr(2,npm), vn(npm,6,m_max), vs(7*me,6,m_max), list_mode(m_max))
   OPEN(NEWUNIT=u, FILE=TRIM(dir)//'/jj.bin', ACCESS='stream', FORM='unformatted', STATUS='old')
@@ -79,6 +82,12 @@ PROGRAM sfx_fortran_driver
   WRITE(*,'(A)') 'sfx_fortran_driver: ok'
 
 CONTAINS
+
+  FUNCTION square(x) RESULT(vv)
+    REAL(KIND=8) :: x
+    REAL(KIND=8) :: vv
+    vv = x*x
+  END FUNCTION square
 
   SUBROUTINE read_real(name, a)
     CHARACTER(LEN=*), INTENT(IN) :: name

@@ -64,5 +64,6 @@ def test_fortran_shim(tmp_path):
     mm = load("maxmin", (2, 1, 1))[:, 0, 0]
     assert abs(mm[0] - vmx) <= 1e-13 * vmx and abs(mm[1] - vmn) <= 1e-13 * vmx
     assert rel(load("real", (2 * M + 1, 3, npt)), F.fft_par_real(v1, M + 1)) <= TOL
+    assert rel(load("scal", (npt, 2, M)), F.fft_par_scal_funct(den, lambda x: x * x, mp)) <= TOL
     exp = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, np.arange(M), vn)
     assert rel(load("smb", (m.n_gauss, 6, M)), exp) <= TOL
