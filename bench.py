@@ -66,11 +66,19 @@ def mp_of(M: int) -> int:
     return mp
 
 
-def fft_flops_per_point(M: int, n_inv: int, n_fwd: int, n_prod: int) -> float:
-    """Flops per Gauss point of the azimuthal part: transforms of length N = 3*MP at
-    2.5 N log2 N each (complex signals pack two real fields), plus the products."""
+def fft_flops_per_point(M: int, n_in: int, n_out: int, n_prod: int) -> float:
+    """Flops per point of the azimuthal part: one real transform of length
+    N = 3*MP per real field in and out at 2.5 N log2 N each, plus n_prod
+    flops per grid angle for the pointwise product."""
     N = 3 * mp_of(M)
-    return (n_inv + n_fwd) * 2.5 * N * np.log2(N) + n_prod * N
+    return (n_in + n_out) * 2.5 * N * np.log2(N) + n_prod * N
+
+
+def b2_flops(G: int, M: int) -> float:
+    """Flops of smb_cross_prod_gauss_sft_par: interpolation and curl (168 per
+    Gauss-pt*mode) plus six real fields in, three out and the cross product
+    (9 flops) per grid angle."""
+    return G * M * 168.0 + G * fft_flops_per_point(M, 6, 3, 9)
 
 
 def _fields(torch, npts, M, decay, seeds, ncol=6):
@@ -86,8 +94,14 @@ def _fields(torch, npts, M, decay, seeds, ncol=6):
 
 
 class Phase:
-    def __init__(self, name, fn, bytes_, flops, kernel):
+    """One library call of a step.  ``bytes_`` are its compulsory HBM bytes
+    (inputs read once, outputs written once, mesh once); ``overhead`` the
+    bytes of work arrays it writes and reads back, reported apart and never
+    counted as achieved traffic."""
+
+    def __init__(self, name, fn, bytes_, flops, kernel, overhead=0):
         self.name, self.fn, self.bytes, self.flops, self.kernel = name, fn, bytes_, flops, kernel
+        self.overhead = overhead
 
 
 # ---------------------------------------------------------------- workloads
@@ -106,17 +120,18 @@ def wl_nl(torch, args):
     dbar = FP.fortran_empty((npn, 6, M))
     work = torch.empty(6 * m.me * 6 * M, dtype=torch.float64, device="cuda")
     mesh_bytes = m.me * 24 + npn * 16
+    # compulsory bytes: every input read once, every output written once, the mesh once
     fwd_bytes = npn * 6 * M * 8 + G * 6 * M * 8 + mesh_bytes
-    # adjoint: ybar and U read, per-element contributions written and read back, dbar written
-    adj_bytes = G * 6 * M * 8 + npn * 6 * M * 8 + 2 * (6 * m.me * 6 * M * 8) + npn * 6 * M * 8 + mesh_bytes
-    interp = 6 * 6 * 3 * 2 + 40          # values + 2 derivatives of 6 components, curl
-    f_fwd = G * M * interp + G * fft_flops_per_point(M, 9, 3, 9)
+    # adjoint: ybar and U read, dbar written (the element work array is re-traffic, not credited)
+    adj_bytes = G * 6 * M * 8 + npn * 6 * M * 8 + npn * 6 * M * 8 + mesh_bytes
+    f_fwd = b2_flops(G, M)
     f_adj = 2 * f_fwd
     phases = [
         Phase("forward", lambda: NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, fwd_out),
               fwd_bytes, f_fwd, "gauss_kernel<64,CROSS_CURL>"),
         Phase("adjoint", lambda: NL.nl_adjoint(mesh, modes, u, ybar, dbar, work),
-              adj_bytes, f_adj, "adjoint_kernel<64> + adjoint_gather_kernel"),
+              adj_bytes, f_adj, "adjoint_kernel<64> + adjoint_gather_kernel",
+              overhead=2 * (6 * m.me * 6 * M * 8)),
     ]
     cfg = {"workload": "nonlinear term (curl u) x u at Gauss points + adjoint of its linearization, "
                        "224x448 P2 mesh (200,704 triangles, 1,404,928 Gauss points), m_max=64",
@@ -147,15 +162,16 @@ def wl_mhd(torch, args):
     rotv_bytes = 3 * npn * 6 * M * 8 + G * 6 * M * 8 + mesh_bytes
     phases = [
         Phase("rotv_mhd", lambda: NL.nl_rotv_mhd(mesh, modes, u, H, B, 1.0, work), rotv_bytes,
-              G * M * 300 + G * fft_flops_per_point(M, 18, 3, 18), "gauss_kernel<64,PAIRS> (2 passes)"),
+              2 * b2_flops(G, M), "gauss_kernel<64,PAIRS> (2 passes)"),
         # full NS RHS (rhs_ns_gauss_3x3 + rhs_3x3): rotv_v read once, V1m, ff (6 columns) and
-        # P, P_inc (2 columns) read, element contributions written and read back, vb written
+        # P, P_inc (2 columns) read, vb written, mesh read; the element contributions
+        # written and read back are re-traffic
         Phase("rhs_ns_3x3", lambda: NL.rhs_ns_3x3(mesh, modes, work, v1m, fsrc, pr, pinc, v145, v236, welem),
-              G * 6 * M * 8 + npn * 16 * M * 8 + 2 * 36 * m.me * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8
-              + 12 * 6 * m.me + npn * 16,
-              G * M * (6 * 2 * 6 + 16 * 6 * 2), "rhs_ns_elem_kernel + rhs_gather_kernel"),
+              G * 6 * M * 8 + npn * 16 * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8 + 12 * 6 * m.me + npn * 16,
+              G * M * (6 * 2 * 6 + 16 * 6 * 2), "rhs_ns_elem_kernel + rhs_gather_kernel",
+              overhead=2 * 36 * m.me * M * 8),
         Phase("u_x_B_nodes", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb), 3 * npn * 6 * M * 8,
-              npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<64,CROSS>"),
+              npn * fft_flops_per_point(M, 6, 3, 9), "fftprod_kernel<64,CROSS>"),
     ]
     cfg = {"workload": "MHD nonlinear terms of one step, 224x448 P2 mesh (200,704 triangles), m_max=64: "
                        "rotv_v=(curl u)xu-(curl H)xB at Gauss points -> full per-mode NS RHS (rhs_ns_gauss_3x3 + rhs_3x3), "
@@ -182,14 +198,13 @@ def wl_nsrhs(torch, args):
     v236 = FP.fortran_empty((npn, 3, M))
     welem = torch.empty(36 * m.me * M, dtype=torch.float64, device="cuda")
     # rotv_v and the tensor read once, V1m, ff (6 columns) and P, P_inc (2 columns) read,
-    # element contributions written and read back, vb written, mesh read
-    rest = (G * 6 * M * 8 + npn * 16 * M * 8 + 2 * 36 * m.me * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8
-            + 12 * 6 * m.me + npn * 16)
+    # vb written, mesh read (the element contributions written and read back are re-traffic)
+    rest = G * 6 * M * 8 + npn * 16 * M * 8 + npn * 6 * M * 8 + m.me * 7 * 8 + 12 * 6 * m.me + npn * 16
     phases = [
         Phase("rhs_ns_3x3_tensor",
               lambda: NL.rhs_ns_3x3(mesh, modes, rotv, v1m, fsrc, pr, pinc, v145, v236, welem, opt_tensor=tens),
               rest + G * 18 * M * 8, G * M * (6 * 2 * 6 + 16 * 6 * 2 + 6 * 6 * 6 + 20),
-              "rhs_ns_elem_kernel<true> + rhs_gather_kernel"),
+              "rhs_ns_elem_kernel<true> + rhs_gather_kernel", overhead=2 * 36 * m.me * M * 8),
     ]
     cfg = {"workload": "full per-mode NS right-hand side with rhs_3x3's tensor (LES / variable viscosity), "
                        "224x448 P2 mesh (200,704 triangles, 1,404,928 Gauss points), m_max=64: "
@@ -345,7 +360,7 @@ def wl_entropy(torch, args):
                                                                      0.9, opt_c2_real_out=r2),
          20 * b + 8 * npt + 16 * N * npt, 10 * f_t * npt, "exact_emom_kernel + exact_emom_out_kernel"),
         ("diffu_mom", lambda: FP.FFT_COMPUTE_DIFFU_MOM(None, g1, g2, c, v6[1], v6[2]), 26 * b,
-         npt * fft_flops_per_point(M, 9, 3, 9) * 2, "fftprod_kernel<64,SVECT1> x 2"),
+         npt * fft_flops_per_point(M, 4, 3, 3) * 2, "fftprod_kernel<64,SVECT1> x 2"),
     ]
     phases = [Phase(n, f, bb, fl, k) for n, f, bb, fl, k in ops]
     cfg = {"workload": "entropy-viscosity FFT_PAR products on the reference grid N=191 at 1,404,928 Gauss "
@@ -362,7 +377,7 @@ def wl_fftprod(torch, args):
     a, b = _fields(torch, npt, M, 4.0, (SEED + 2, SEED + 3))
     out = FP.fortran_empty((npt, 6, M))
     phases = [Phase("cross_prod", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, a, b, out), 3 * npt * 6 * M * 8,
-                    npt * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<64,CROSS>")]
+                    npt * fft_flops_per_point(M, 6, 3, 9), "fftprod_kernel<64,CROSS>")]
     cfg = {"workload": "FFT_PAR_CROSS_PROD_DCL on Gauss-point arrays (1,404,928 points), m_max=64",
            "points": npt, "m_max": M, "grid": 3 * mp_of(M)}
     return phases, npt * M, cfg, ("fftprod", None, M)
@@ -386,12 +401,14 @@ def wl_tangent(torch, args):
     dbar = FP.fortran_empty((npn, 6, M))
     work = torch.empty(6 * m.me * 6 * M, dtype=torch.float64, device="cuda")
     tb = 2 * npn * 6 * M * 8 + G * 6 * M * 8
-    ab = G * 6 * M * 8 + npn * 6 * M * 8 + 2 * (6 * m.me * 6 * M * 8) + npn * 6 * M * 8
+    # adjoint: ybar and U read, dbar written; its element work array is re-traffic
+    ab = G * 6 * M * 8 + npn * 6 * M * 8 + npn * 6 * M * 8
     phases = [
         Phase("tangent", lambda: NL.nl_tangent(mesh, modes, U, d, tout), tb,
-              G * fft_flops_per_point(M, 18, 3, 18), "gauss_kernel<32,PAIRS> (2 passes)"),
+              2 * b2_flops(G, M), "gauss_kernel<32,PAIRS> (2 passes)"),
         Phase("adjoint", lambda: NL.nl_adjoint(mesh, modes, U, ybar, dbar, work), ab,
-              G * fft_flops_per_point(M, 18, 6, 18), "adjoint_kernel<32> + adjoint_gather_kernel"),
+              2 * b2_flops(G, M), "adjoint_kernel<32> + adjoint_gather_kernel",
+              overhead=2 * (6 * m.me * 6 * M * 8)),
     ]
     cfg = {"workload": "tangent + adjoint of the convective term around a base flow (modes 0-4, amplitude 0.1), "
                        "158x316 P2 mesh (99,856 triangles), m_max=32",
@@ -418,7 +435,7 @@ def wl_batch(torch, args):
         items.append((mesh, list(range(M)), u, out))
         units += m.n_gauss * M
         byts += (m.np + m.n_gauss) * 6 * M * 8 + m.me * 24 + m.np * 16
-        flops += m.n_gauss * fft_flops_per_point(M, 9, 3, 9)
+        flops += b2_flops(m.n_gauss, M)
     b = BT.Batch(items)
     b.autotune(reps=2)
     b.capture()
@@ -431,34 +448,12 @@ def wl_batch(torch, args):
     return phases, units, cfg, ("batch", None, None)
 
 
-def _level_set_density(torch, rr, M, x0=0.3, radius=0.25, eps=0.05):
-    """rho = 1 + 0.5 tanh(phi/eps), phi = |x - x0| - radius for a sphere centred
-    at (x0, 0, 0), sampled on 2M azimuthal points per node and transformed once
-    into (cos, sin) coefficients of modes 0..M-1 (np, 2, M)."""
-    r = torch.tensor(rr[0], device="cuda", dtype=torch.float64)[:, None]
-    z = torch.tensor(rr[1], device="cuda", dtype=torch.float64)[:, None]
-    n = 2 * M
-    th = torch.arange(n, device="cuda", dtype=torch.float64)[None, :] * (2 * np.pi / n)
-    out = torch.empty((M, 2, r.shape[0]), device="cuda", dtype=torch.float64)
-    for i0 in range(0, r.shape[0], 65536):
-        rs, zs = r[i0:i0 + 65536], z[i0:i0 + 65536]
-        phi = torch.sqrt((rs * torch.cos(th) - x0) ** 2 + (rs * torch.sin(th)) ** 2 + zs ** 2) - radius
-        rho = 1.0 + 0.5 * torch.tanh(phi / eps)
-        c = torch.fft.rfft(rho, dim=1)[:, :M] / n      # c_m, rho = sum c_m e^{i m theta}
-        a = 2.0 * c.real
-        b = -2.0 * c.imag
-        a[:, 0] = c[:, 0].real
-        b[:, 0] = 0.0
-        out[:, 0, i0:i0 + 65536] = a.T
-        out[:, 1, i0:i0 + 65536] = b.T
-    return out.permute(2, 1, 0)
-
-
 def wl_full(torch, args):
     """Full explicit nonlinear step, forward and adjoint, 200,704 triangles, m_max = 128."""
     from sfemans_amd import fft_parallele as FP
     from sfemans_amd import nonlinear as NL
     from sfemans_amd.mesh import rectangle_p2
+    from sfemans_amd.synth import level_set_density
     m = rectangle_p2(224, 448)
     mesh = NL.Mesh.from_p2(m)
     M = 128
@@ -466,7 +461,7 @@ def wl_full(torch, args):
     G, npn, me = m.n_gauss, m.np, m.me
     u, B = _fields(torch, npn, M, 4.0, (SEED + 5, SEED + 6))
     T, = _fields(torch, npn, M, 4.0, (SEED + 7,), ncol=2)
-    rho = _level_set_density(torch, m.rr, M)
+    rho = level_set_density(torch, m.rr, M)
     ybar, = _fields(torch, G, M, 4.0, (SEED + 8,))
     hc = torch.ones(me, dtype=torch.float64, device="cuda")
     rotv = FP.fortran_empty((G, 6, M))
@@ -483,24 +478,24 @@ def wl_full(torch, args):
     nod6, gau6 = npn * 6 * M * 8, G * 6 * M * 8
     phases = [
         Phase("cross_curl", lambda: NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, rotv),
-              nod6 + gau6, G * fft_flops_per_point(M, 9, 3, 9), "gauss_kernel<128,CROSS_CURL>"),
+              nod6 + gau6, b2_flops(G, M), "gauss_kernel<128,CROSS_CURL>"),
         Phase("rhs_3x3", lambda: NL.rhs_3x3_all_modes(mesh, rotv, v145, v236, -1.0, welem),
-              gau6 + 2 * 36 * me * M * 8 + nod6, G * M * 72, "rhs_elem_kernel + rhs_gather_kernel"),
+              gau6 + nod6, G * M * 72, "rhs_elem_kernel + rhs_gather_kernel", overhead=2 * 36 * me * M * 8),
         Phase("u_x_B_nodes", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb), 3 * nod6,
-              npn * fft_flops_per_point(M, 9, 3, 9), "fftprod_kernel<128,CROSS>"),
+              npn * fft_flops_per_point(M, 6, 3, 9), "fftprod_kernel<128,CROSS>"),
         Phase("u_grad_T", lambda: NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, u, T, cg),
-              nod6 + nod6 // 3 + G * 2 * M * 8, G * fft_flops_per_point(M, 12, 3, 8), "gauss_kernel<128,UGRADC>"),
+              nod6 + nod6 // 3 + G * 2 * M * 8, G * M * 100 + G * fft_flops_per_point(M, 8, 1, 8), "gauss_kernel<128,UGRADC>"),
         # (G, 2, M) column-major viewed as (G, 2M) column-major: column c + 2m
         Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, torch.as_strided(cg, (G, 2 * M), (1, G)), vT, wq),
-              2 * npn * 2 * M * 8 + G * 2 * M * 8 + 2 * 6 * me * 2 * M * 8, G * 2 * M * 30,
-              "qs00_elem_kernel + qs00_gather_kernel"),
+              2 * npn * 2 * M * 8 + G * 2 * M * 8, G * 2 * M * 30,
+              "qs00_elem_kernel + qs00_gather_kernel", overhead=2 * 6 * me * 2 * M * 8),
         Phase("rho_u", lambda: FP.FFT_SCALAR_VECT_DCL(None, u, rho, rhou, 1), 2 * nod6 + nod6 // 3,
-              npn * fft_flops_per_point(M, 6, 3, 3), "fftprod_kernel<128,SVECT1>"),
+              npn * fft_flops_per_point(M, 4, 3, 3), "fftprod_kernel<128,SVECT1>"),
         Phase("m_tensor_u", lambda: FP.FFT_TENSOR_DCL(None, rhou, u, mten), 5 * nod6,
-              npn * fft_flops_per_point(M, 9, 9, 9), "fftprod_kernel<128,TENSOR>"),
+              npn * fft_flops_per_point(M, 6, 9, 9), "fftprod_kernel<128,TENSOR>"),
         Phase("adjoint", lambda: NL.nl_adjoint(mesh, modes, u, ybar, dbar, welem),
-              gau6 + 2 * nod6 + 2 * 36 * me * M * 8, 2 * G * fft_flops_per_point(M, 9, 3, 9),
-              "adjoint_kernel<128> + adjoint_gather_kernel"),
+              gau6 + 2 * nod6, 2 * b2_flops(G, M),
+              "adjoint_kernel<128> + adjoint_gather_kernel", overhead=2 * 36 * me * M * 8),
     ]
     cfg = {"workload": "full explicit nonlinear step, forward + adjoint, 224x448 P2 mesh (200,704 triangles), "
                        "m_max=128: (curl u)xu -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and (rho u) (x) u "
@@ -651,30 +646,40 @@ def cpu_baseline(kind, mesh, M, sample_s: float):
                           f"16x16 P2 mesh ({sm.me} triangles) x m_max={M}, 1 thread, {dt:.1f} s"}
     if kind in ("nl", "mhd", "tangent", "batch", "full") and OC.available():
         from sfemans_amd.mesh import rectangle_p2
-        # forward nonlinear term on a 64x64 P2 mesh (8,192 triangles), same m_max
-        sm = rectangle_p2(64, 64)
-        u = fourier_field(sm.np, 6, M, 3, 4.0)
-        modes = np.arange(M, dtype=np.int32)
-        units = sm.n_gauss * M
-        out = np.empty((sm.n_gauss, 6, M), order="F")
-        nth = _cpu_threads()
-        rates = {}
-        for th in ((1, nth) if nth > 1 else (1,)):
-            OC.smb_cross_prod_gauss(sm.jj, sm.rr, modes, u, th, out)          # warm-up
+
+        def rate(sm, Mm, th, fft, budget):
+            """Median Gauss-pt*modes/s of the C restatement on mesh sm, m_max Mm, th threads."""
+            u = fourier_field(sm.np, 6, Mm, 3, 4.0)
+            modes = np.arange(Mm, dtype=np.int32)
+            out = np.empty((sm.n_gauss, 6, Mm), order="F")
+            OC.smb_cross_prod_gauss(sm.jj, sm.rr, modes, u, th, out, fft=fft)          # warm-up
             times = []
-            t_end = time.perf_counter() + 0.5 * sample_s
+            t_end = time.perf_counter() + budget
             while time.perf_counter() < t_end or len(times) < 3:
                 t0 = time.perf_counter()
-                OC.smb_cross_prod_gauss(sm.jj, sm.rr, modes, u, th, out)
+                OC.smb_cross_prod_gauss(sm.jj, sm.rr, modes, u, th, out, fft=fft)
                 times.append(time.perf_counter() - t0)
-            rates[th] = (units / float(np.median(times)), len(times))
-        nmax = max(rates)
-        return {"value": rates[nmax][0], "unit": "Gauss-pt*modes/s", "cores": nmax, "kind": "port",
-                "value_1core": rates[1][0],
-                "sample": f"smb_cross_prod_gauss_sft_par restated in C (OpenMP over element blocks, direct "
-                          f"transforms on the reference grid N={2 * (3 * M // 2) - 1}), 64x64 P2 mesh "
-                          f"({sm.me} triangles) x m_max={M}, median of {rates[nmax][1]} calls on {nmax} threads "
-                          f"and of {rates[1][1]} calls on 1 thread; forward term only"}
+            return sm.n_gauss * Mm / float(np.median(times)), len(times)
+
+        # forward nonlinear term on a 64x64 P2 mesh (8,192 triangles), same m_max, FFT variant
+        sm = rectangle_p2(64, 64)
+        nth = _cpu_threads()
+        v_all, n_all = rate(sm, M, nth, True, 0.35 * sample_s)
+        v_1, n_1 = rate(sm, M, 1, True, 0.25 * sample_s) if nth > 1 else (v_all, n_all)
+        v_dir, n_dir = rate(sm, M, nth, False, 0.15 * sample_s)
+        # the 40x80 rectangle (6,400 triangles) at m_max = 8 on one core (one MPI rank)
+        c1 = rectangle_p2(40, 80)
+        v_c1, n_c1 = rate(c1, 8, 1, True, 0.25 * sample_s)
+        return {"value": v_all, "unit": "Gauss-pt*modes/s", "cores": nth, "kind": "port",
+                "value_1core": v_1, "value_direct_reference_grid": v_dir,
+                "rectangle_40x80_m8_1core": v_c1,
+                "sample": f"smb_cross_prod_gauss_sft_par restated in C: Gauss interpolation + curl, then a "
+                          f"mixed-radix (4,2,3) FFT on N={OC.fft_grid(M)} (no FFTW on the host), OpenMP over "
+                          f"element blocks; 64x64 P2 mesh ({sm.me} triangles) x m_max={M}, median of {n_all} "
+                          f"calls on {nth} threads and of {n_1} calls on 1 thread; value_direct_reference_grid: "
+                          f"direct sums on N={2 * (3 * M // 2) - 1}, median of {n_dir} calls on {nth} threads; "
+                          f"rectangle_40x80_m8_1core: the 40x80 mesh ({c1.me} triangles) at m_max=8 "
+                          f"(N={OC.fft_grid(8)}), median of {n_c1} calls on 1 thread; forward term only"}
     if kind in ("nl", "mhd", "tangent", "batch", "full") and mesh is not None:
         from oracle import smb
         t_end = time.perf_counter() + sample_s
@@ -756,7 +761,6 @@ def main():
         torch.cuda.synchronize()
         del x, y
     phases, units, cfg, cpu_kind = WORKLOADS[args.workload](torch, args)
-    cfg.pop("_keep", None) if False else None
     for _ in range(args.warmup):
         for ph in phases:
             ph.fn()
@@ -779,10 +783,22 @@ def main():
     dom = int(np.argmax(ph_ms))
     dp = phases[dom]
     achieved = dp.bytes / (ph_ms[dom] / 1000.0) / 1e9
-    phase_info = {ph.name: {"ms": ph_ms[i], "kernel": ph.kernel, "GB_per_launch": ph.bytes / 1e9,
-                            "hbm_frac": ph.bytes / (ph_ms[i] / 1000.0) / 1e9 / HBM_PEAK_GBS,
-                            "fp64_frac": ph.flops / (ph_ms[i] / 1000.0) / 1e12 / FP64_PEAK_TFLOPS}
-                  for i, ph in enumerate(phases)}
+
+    def roofs(ph, ms):
+        hbm = ph.bytes / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS
+        fp64 = ph.flops / (ms / 1000.0) / 1e12 / FP64_PEAK_TFLOPS
+        # the roof that bounds the kernel: the one its work would take longer to meet at peak
+        binding = "HBM" if ph.bytes / (HBM_PEAK_GBS * 1e9) > ph.flops / (FP64_PEAK_TFLOPS * 1e12) else "FP64"
+        return hbm, fp64, binding
+
+    phase_info = {}
+    for i, ph in enumerate(phases):
+        hbm, fp64, binding = roofs(ph, ph_ms[i])
+        phase_info[ph.name] = {"ms": ph_ms[i], "kernel": ph.kernel, "GB_per_launch": ph.bytes / 1e9,
+                               "GFLOP_per_launch": ph.flops / 1e9, "hbm_frac": hbm, "fp64_frac": fp64,
+                               "binding_roof": binding,
+                               "binding_frac": hbm if binding == "HBM" else fp64,
+                               "work_array_GB": ph.overhead / 1e9}
     cfg["phases"] = phase_info
     cfg.pop("_keep", None)
     kind, mesh, M = cpu_kind
@@ -796,9 +812,13 @@ def main():
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dp.name),
-                     "kernel": dp.kernel,
-                     "phase": dp.name, "kernel_ms": ph_ms[dom],
-                     "fp64_tflops": dp.flops / (ph_ms[dom] / 1000.0) / 1e12},
+                     "algorithmic_bytes": dp.bytes, "work_array_bytes": dp.overhead,
+                     "kernel": dp.kernel, "phase": dp.name, "kernel_ms": ph_ms[dom],
+                     "flops": dp.flops, "fp64_tflops": dp.flops / (ph_ms[dom] / 1000.0) / 1e12,
+                     "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                     "fp64_frac": phase_info[dp.name]["fp64_frac"],
+                     "binding_roof": phase_info[dp.name]["binding_roof"],
+                     "binding_frac": phase_info[dp.name]["binding_frac"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

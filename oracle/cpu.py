@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-__all__ = ["LIB_PATH", "available", "smb_cross_prod_gauss"]
+__all__ = ["LIB_PATH", "available", "smb_cross_prod_gauss", "fft_grid"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "libsfx_cpu.so")
 _lib = None
@@ -29,12 +29,26 @@ def _load():
         vp = C.c_void_p
         L.sfx_cpu_smb_cross_prod_gauss.argtypes = [vp, vp, C.c_int64, C.c_int64, vp, C.c_int32, vp, vp, C.c_int32]
         L.sfx_cpu_smb_cross_prod_gauss.restype = C.c_int
+        L.sfx_cpu_smb_cross_prod_gauss_fft.argtypes = L.sfx_cpu_smb_cross_prod_gauss.argtypes
+        L.sfx_cpu_smb_cross_prod_gauss_fft.restype = C.c_int
         _lib = L
     return _lib
 
 
-def smb_cross_prod_gauss(jj, rr, list_mode, V_in, nthreads: int = 1, out=None):
-    """``P_M[(curl V) x V]`` at the Gauss points, (7 me, 6, M), on ``nthreads`` cores."""
+def fft_grid(M: int) -> int:
+    """Grid of the FFT variant: the smallest N = 3*2^k with N >= 3M - 2."""
+    n = 3
+    while n < 3 * M - 2:
+        n *= 2
+    return n
+
+
+def smb_cross_prod_gauss(jj, rr, list_mode, V_in, nthreads: int = 1, out=None, fft: bool = False):
+    """``P_M[(curl V) x V]`` at the Gauss points, (7 me, 6, M), on ``nthreads`` cores.
+
+    ``fft=False``: direct sums on the reference grid N = 2*m_max_pad - 1;
+    ``fft=True``: the mixed-radix FFT on N = :func:`fft_grid` (M).
+    """
     jj = np.asfortranarray(jj, dtype=np.int32)
     rr = np.asfortranarray(rr, dtype=np.float64)
     V = np.asfortranarray(V_in, dtype=np.float64)
@@ -44,7 +58,9 @@ def smb_cross_prod_gauss(jj, rr, list_mode, V_in, nthreads: int = 1, out=None):
         raise ValueError("V_in must be (np, 6, M) with M = len(list_mode)")
     if out is None:
         out = np.empty((7 * me, 6, M), order="F")
-    rc = _load().sfx_cpu_smb_cross_prod_gauss(jj.ctypes.data, rr.ctypes.data, me, npn, lm.ctypes.data, M,
+    L = _load()
+    fn = L.sfx_cpu_smb_cross_prod_gauss_fft if fft else L.sfx_cpu_smb_cross_prod_gauss
+    rc = fn(jj.ctypes.data, rr.ctypes.data, me, npn, lm.ctypes.data, M,
                                                V.ctypes.data, out.ctypes.data, nthreads)
     if rc != 0:
         raise ValueError("sfx_cpu_smb_cross_prod_gauss: bad arguments")

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["splitmix64", "normal", "fourier_field", "SEED_BASE"]
+__all__ = ["splitmix64", "normal", "fourier_field", "level_set_density", "SEED_BASE"]
 
 SEED_BASE = 0x5F3E
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -53,3 +53,30 @@ def fourier_field(npts: int, nb_field: int, m_max: int, seed: int,
         if mode == 0:
             v[:, 1::2, i] = 0.0
     return v
+
+
+def level_set_density(torch, rr, m_max: int, x0: float = 0.3, radius: float = 0.25, eps: float = 0.05):
+    """Variable density of a level set, as (cos, sin) coefficients on the device.
+
+    rho = 1 + 0.5 tanh(phi/eps) with phi = |x - x0| - radius, the signed
+    distance to a sphere centred at (x0, 0, 0), sampled at every node on 2*m_max
+    azimuthal points and transformed once into the coefficients of modes
+    0..m_max-1; returns a Fortran-strided CUDA tensor (np, 2, m_max).
+    """
+    r = torch.tensor(rr[0], device="cuda", dtype=torch.float64)[:, None]
+    z = torch.tensor(rr[1], device="cuda", dtype=torch.float64)[:, None]
+    n = 2 * m_max
+    th = torch.arange(n, device="cuda", dtype=torch.float64)[None, :] * (2 * np.pi / n)
+    out = torch.empty((m_max, 2, r.shape[0]), device="cuda", dtype=torch.float64)
+    for i0 in range(0, r.shape[0], 65536):
+        rs, zs = r[i0:i0 + 65536], z[i0:i0 + 65536]
+        phi = torch.sqrt((rs * torch.cos(th) - x0) ** 2 + (rs * torch.sin(th)) ** 2 + zs ** 2) - radius
+        rho = 1.0 + 0.5 * torch.tanh(phi / eps)
+        c = torch.fft.rfft(rho, dim=1)[:, :m_max] / n      # rho = sum_m c_m e^{i m theta}
+        a = 2.0 * c.real
+        b = -2.0 * c.imag
+        a[:, 0] = c[:, 0].real
+        b[:, 0] = 0.0
+        out[:, 0, i0:i0 + 65536] = a.T
+        out[:, 1, i0:i0 + 65536] = b.T
+    return out.permute(2, 1, 0)

@@ -1,0 +1,211 @@
+"""The benchmark configurations themselves on the GPU, checked against the oracle.
+
+* 40x80 rectangle (6,400 triangles), m_max = 8, coefficients N(0,1)*exp(-m/2)
+  from seed 0x5F3E + 1: the plain (curl u) x u term on every Gauss point, its
+  rhs_3x3 assembly and its adjoint, against the full oracle.
+* 224x448 rectangle (200,704 triangles), m_max = 128: the full explicit
+  nonlinear step (cross-curl -> rhs_3x3, u x B at the nodes, u.grad T ->
+  qs_00_gauss, rho u and (rho u) (x) u with the level-set density
+  rho = 1 + 0.5 tanh(phi/0.05) of a sphere, the convective adjoint) at
+  size: sampled elements and nodes against the oracle, plus antisymmetry,
+  linearity and the dot-product test over the whole arrays.
+
+Tolerance: relative max-norm error <= 1e-12 (FP64); dot-product test
+|<T d, y> - <d, T^T y>| <= 1e-12 |T d| |y|.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import assembly, smb  # noqa: E402
+from oracle import fft_par as F  # noqa: E402
+from sfemans_amd import fft_parallele as FP  # noqa: E402
+from sfemans_amd import nonlinear as NL  # noqa: E402
+from sfemans_amd.fft_parallele import fortran_empty, fortran_tensor  # noqa: E402
+from sfemans_amd.mesh import rectangle_p2  # noqa: E402
+from sfemans_amd.synth import SEED_BASE, fourier_field, level_set_density  # noqa: E402
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+
+
+def test_rectangle_40x80_m8_full_oracle():
+    """40x80 rectangle, 6,400 triangles, m_max = 8: every Gauss point and node vs the oracle."""
+    print("case 40x80 rectangle m_max=8", flush=True)
+    m = rectangle_p2(40, 80)
+    assert m.me == 6400
+    mesh = NL.Mesh.from_p2(m)
+    M = 8
+    modes = np.arange(M)
+    u = fourier_field(m.np, 6, M, SEED_BASE + 1, 2.0)
+    out = np.zeros((m.n_gauss, 6, M), order="F")
+    NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, out)
+    exp = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u)
+    assert rel(out, exp) <= TOL
+    # the assembled right-hand sides of -rotv_v, every mode
+    v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_3x3_all_modes(mesh, fortran_tensor(out), v1, v2, -1.0)
+    torch.cuda.synchronize()
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, exp)
+    assert rel(v1.cpu().numpy(), -e1) <= TOL and rel(v2.cpu().numpy(), -e2) <= TOL
+    # the adjoint of the linearisation around u: dot-product test
+    d = fourier_field(m.np, 6, M, SEED_BASE + 11, 2.0)
+    y = np.asfortranarray(np.random.default_rng(1).standard_normal((m.n_gauss, 6, M)))
+    td = np.zeros((m.n_gauss, 6, M), order="F")
+    ty = np.zeros((m.np, 6, M), order="F")
+    NL.nl_tangent(mesh, modes, u, d, td)
+    NL.nl_adjoint(mesh, modes, u, y, ty)
+    exp_t = (smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, u, d)
+             + smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, d, u))
+    assert rel(td, exp_t) <= TOL
+    assert abs(np.sum(td * y) - np.sum(d * ty)) <= 1e-12 * np.linalg.norm(td) * np.linalg.norm(y)
+
+
+def _elements_of(m, nodes):
+    """Elements having one of ``nodes`` (0-based) among their six nodes, and the
+    sub-mesh (jj renumbered, rr) they form, with the sampled nodes' new numbers."""
+    mask = np.zeros(m.np + 1, dtype=bool)
+    mask[nodes + 1] = True
+    els = np.nonzero(mask[m.jj].any(axis=0))[0]
+    jj = m.jj[:, els]
+    used = np.unique(jj)
+    remap = np.zeros(m.np + 1, dtype=np.int32)
+    remap[used] = np.arange(1, used.size + 1)
+    return els, np.asfortranarray(remap[jj]), np.asfortranarray(m.rr[:, used - 1]), remap[nodes + 1] - 1
+
+
+def _gauss_rows(els):
+    return (els[:, None] * 7 + np.arange(7)[None, :]).ravel()
+
+
+def test_full_step_224x448_m128_at_size():
+    """Full explicit nonlinear step at size: 200,704 triangles, m_max = 128."""
+    m = rectangle_p2(224, 448)
+    mesh = NL.Mesh.from_p2(m)
+    M = 128
+    mp = 3 * M // 2
+    modes = np.arange(M)
+    G, npn, me = m.n_gauss, m.np, m.me
+    rng = np.random.default_rng(0x5F3E + 5)
+    sel_e = np.sort(rng.choice(me, 200, replace=False))
+    sel_n = np.sort(rng.choice(npn, 300, replace=False))
+    gsel = _gauss_rows(sel_e)
+    dev = lambda a: torch.as_tensor(a, device="cuda")
+
+    def fld(seed, ncol=6, n=npn):
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        dec = torch.exp(-torch.arange(M, device="cuda", dtype=torch.float64) / 4.0)
+        t = torch.randn((M, ncol, n), generator=g, device="cuda", dtype=torch.float64) * dec[:, None, None]
+        t[0, 1::2, :] = 0.0
+        return t.permute(2, 1, 0)
+
+    u, B = fld(SEED_BASE + 5), fld(SEED_BASE + 6)
+    T = fld(SEED_BASE + 7, 2)
+    rho = level_set_density(torch, m.rr, M)
+    hc = torch.ones(me, dtype=torch.float64, device="cuda")
+
+    # (curl u) x u at the Gauss points: sampled elements vs the oracle
+    print("case full step 224x448 m_max=128: cross-curl", flush=True)
+    rotv = fortran_empty((G, 6, M))
+    NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, rotv)
+    torch.cuda.synchronize()
+    u_h = u.cpu().numpy()
+    exp = smb.smb_cross_prod_gauss_sft_par(m.jj[:, sel_e], m.rr, modes, u_h)
+    assert rel(rotv[dev(gsel)].cpu().numpy(), exp) <= TOL
+
+    # rhs_3x3 of -rotv_v: sampled nodes (every element around them) vs the oracle
+    print("case full step: rhs_3x3", flush=True)
+    v145, v236 = fortran_empty((npn, 3, M)), fortran_empty((npn, 3, M))
+    NL.rhs_3x3_all_modes(mesh, rotv, v145, v236, -1.0)
+    torch.cuda.synchronize()
+    els, jj_s, rr_s, loc = _elements_of(m, sel_n)
+    g_s = rotv[dev(_gauss_rows(els))].cpu().numpy()
+    e1, e2 = assembly.rhs_3x3_all_modes(jj_s, rr_s, g_s)
+    assert rel(v145[dev(sel_n)].cpu().numpy(), -e1[loc]) <= TOL
+    assert rel(v236[dev(sel_n)].cpu().numpy(), -e2[loc]) <= TOL
+
+    # u x B at the nodes: sampled nodes vs the oracle, antisymmetry over all nodes
+    print("case full step: u x B at the nodes", flush=True)
+    uxb, bxu = fortran_empty((npn, 6, M)), fortran_empty((npn, 6, M))
+    FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb)
+    FP.FFT_PAR_CROSS_PROD_DCL(None, B, u, bxu)
+    torch.cuda.synchronize()
+    B_s = B[dev(sel_n)].cpu().numpy()
+    exp = F.fft_par_cross_prod_dcl(np.asfortranarray(u_h[sel_n]), np.asfortranarray(B_s), mp)
+    assert rel(uxb[dev(sel_n)].cpu().numpy(), exp) <= TOL
+    assert (uxb + bxu).abs().max().item() <= 1e-13 * uxb.abs().max().item()
+    del bxu
+
+    # u.grad T (with heat capacity 1) -> qs_00_gauss
+    print("case full step: u.grad T", flush=True)
+    cg = fortran_empty((G, 2, M))
+    NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, u, T, cg)
+    torch.cuda.synchronize()
+    T_h = T.cpu().numpy()
+    exp = smb.smb_ugradc_gauss_fft_par(m.jj[:, sel_e], m.rr, modes, np.ones(sel_e.size), u_h, T_h)
+    assert rel(cg[dev(gsel)].cpu().numpy(), exp) <= TOL
+    print("case full step: qs_00_gauss", flush=True)
+    ff0 = torch.zeros((2 * M, npn), dtype=torch.float64, device="cuda").T
+    vT = fortran_empty((npn, 2 * M))
+    cg2 = torch.as_strided(cg, (G, 2 * M), (1, G))
+    NL.qs_00_gauss(mesh, hc, ff0, cg2, vT)
+    torch.cuda.synchronize()
+    cg_s = cg2[dev(_gauss_rows(els))].cpu().numpy()
+    got = vT[dev(sel_n)].cpu().numpy()
+    for c in (0, 1, 2, 7, 64, 2 * M - 1):
+        e = assembly.qs_00_gauss(jj_s, rr_s, np.ones(els.size), np.zeros(rr_s.shape[1]), cg_s[:, c])
+        assert rel(got[:, c], e[loc]) <= TOL, c
+    del cg, cg2, vT
+
+    # rho u and (rho u) (x) u with the level-set density
+    print("case full step: rho u, (rho u) (x) u", flush=True)
+    rhou = fortran_empty((npn, 6, M))
+    FP.FFT_SCALAR_VECT_DCL(None, u, rho, rhou, 1)
+    mten = torch.empty((M, 6, npn, 3), dtype=torch.float64, device="cuda").permute(3, 2, 1, 0)
+    FP.FFT_TENSOR_DCL(None, rhou, u, mten)
+    torch.cuda.synchronize()
+    rho_h = rho[dev(sel_n)].cpu().numpy()
+    exp_rhou = F.fft_scalar_vect_dcl(np.asfortranarray(u_h[sel_n]), np.asfortranarray(rho_h), 1, mp)
+    got_rhou = rhou[dev(sel_n)].cpu().numpy()
+    assert rel(got_rhou, exp_rhou) <= TOL
+    exp = F.fft_tensor_dcl(np.asfortranarray(got_rhou), np.asfortranarray(u_h[sel_n]), mp)
+    assert rel(mten[:, dev(sel_n)].cpu().numpy(), exp) <= TOL
+    # the density stays within its range on the grid: its mean is between 0.5 and 1.5
+    assert 0.5 <= rho[:, 0, 0].min().item() and rho[:, 0, 0].max().item() <= 1.5
+    del mten, rhou
+
+    # convective adjoint: dot-product test over the whole arrays, linearity
+    print("case full step: adjoint", flush=True)
+    d = fld(SEED_BASE + 9)
+    y = fld(SEED_BASE + 8, 6, G)
+    td = fortran_empty((G, 6, M))
+    NL.nl_tangent(mesh, modes, u, d, td)
+    welem = torch.empty(6 * me * 6 * M, dtype=torch.float64, device="cuda")
+    ty = fortran_empty((npn, 6, M))
+    NL.nl_adjoint(mesh, modes, u, y, ty, welem)
+    torch.cuda.synchronize()
+    lhs = torch.sum(td * y).item()
+    rhs = torch.sum(d * ty).item()
+    assert abs(lhs - rhs) <= 1e-12 * torch.linalg.vector_norm(td).item() * torch.linalg.vector_norm(y).item()
+    ty2 = fortran_empty((npn, 6, M))
+    y2 = y * (-0.5)
+    NL.nl_adjoint(mesh, modes, u, y2, ty2, welem)
+    torch.cuda.synchronize()
+    assert (ty2 + 0.5 * ty).abs().max().item() <= 1e-13 * ty.abs().max().item()
+    # the tangent on sampled elements vs the oracle
+    d_h = d.cpu().numpy()
+    exp = (smb.smb_CurlH_cross_B_gauss_sft_par(m.jj[:, sel_e[:60]], m.rr, modes, u_h, d_h)
+           + smb.smb_CurlH_cross_B_gauss_sft_par(m.jj[:, sel_e[:60]], m.rr, modes, d_h, u_h))
+    assert rel(td[dev(_gauss_rows(sel_e[:60]))].cpu().numpy(), exp) <= TOL

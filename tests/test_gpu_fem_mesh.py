@@ -53,7 +53,7 @@ def test_cross_curl(name, doms, M):
     assert rel(out, smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u)) <= TOL
 
 
-@pytest.mark.parametrize("name,doms,M", CASES[:3])
+@pytest.mark.parametrize("name,doms,M", CASES)
 def test_curlh_cross_b_and_ugradc(name, doms, M):
     print(f"case curlh_cross_b/ugradc {name} domains={doms} M={M}", flush=True)
     m = load(name, doms)
@@ -70,7 +70,7 @@ def test_curlh_cross_b_and_ugradc(name, doms, M):
     assert rel(out2, smb.smb_ugradc_gauss_fft_par(m.jj, m.rr, modes, hc, H, c)) <= TOL
 
 
-@pytest.mark.parametrize("name,doms,M", CASES[:3])
+@pytest.mark.parametrize("name,doms,M", CASES)
 def test_rhs_ns_3x3(name, doms, M):
     print(f"case rhs_ns_3x3 {name} domains={doms} M={M}", flush=True)
     m = load(name, doms)

@@ -52,7 +52,7 @@ def test_host_path_matches_oracle(name, M, npt):
     assert rel(out, ref(V1, V2, 3 * M // 2)) <= TOL
 
 
-@pytest.mark.parametrize("M", [3, 8, 64])
+@pytest.mark.parametrize("M", [3, 8, 64, 100, 128, 256])
 def test_scalar_vect_and_tensor(M):
     npt = 50
     print(f"case svect/tensor M={M}", flush=True)

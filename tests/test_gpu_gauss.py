@@ -54,7 +54,7 @@ def test_cross_curl_precession_selected_modes_device_path():
     assert rel(dout.cpu().numpy(), exp) <= TOL
 
 
-@pytest.mark.parametrize("M", [3, 8, 64])
+@pytest.mark.parametrize("M", [3, 8, 64, 128])
 def test_curlh_cross_b(M):
     m = rectangle_p2(5, 6)
     mesh = NL.Mesh.from_p2(m)
@@ -65,7 +65,7 @@ def test_curlh_cross_b(M):
     assert rel(out, smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, H, B)) <= TOL
 
 
-@pytest.mark.parametrize("M", [1, 4, 8, 32, 64])
+@pytest.mark.parametrize("M", [1, 4, 8, 32, 64, 100, 128, 256])
 def test_ugradc(M):
     m = rectangle_p2(4, 5)
     mesh = NL.Mesh.from_p2(m)

@@ -4,6 +4,8 @@ Both restate smb_cross_prod_gauss_sft_par (subroutine_ns_with_velocity.F90:514-6
 on the reference grid N = 2*m_max_pad - 1; the numpy restatement is pinned by
 tests/test_oracle_*.py (long-double direct DFT, closed forms, known answers).
 The C one is the CPU baseline of bench.py, so it must compute the same thing.
+Its FFT variant runs on the smooth grid N = 3*2^k >= 3M - 2, on which the
+bilinear product has the same coefficients up to rounding.
 """
 import os
 import subprocess
@@ -25,13 +27,15 @@ def built():
     assert cpu.available()
 
 
-@pytest.mark.parametrize("nx,ny,M", [(3, 2, 1), (2, 2, 2), (2, 3, 3), (4, 4, 8), (5, 3, 13), (6, 4, 64)])
+@pytest.mark.parametrize("nx,ny,M", [(3, 2, 1), (2, 2, 2), (2, 3, 3), (4, 4, 8), (5, 3, 13), (6, 4, 64),
+                                     (2, 2, 100), (3, 3, 128)])
 @pytest.mark.parametrize("threads", [1, 3])
-def test_c_restatement_matches_numpy(nx, ny, M, threads):
+@pytest.mark.parametrize("fft", [False, True])
+def test_c_restatement_matches_numpy(nx, ny, M, threads, fft):
     m = rectangle_p2(nx, ny)
     u = fourier_field(m.np, 6, M, 3, 4.0)
     lm = np.arange(M)
-    a = cpu.smb_cross_prod_gauss(m.jj, m.rr, lm, u, threads)
+    a = cpu.smb_cross_prod_gauss(m.jj, m.rr, lm, u, threads, fft=fft)
     b = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, lm, u)
     scale = max(np.max(np.abs(b)), 1e-300)
     assert np.max(np.abs(a - b)) / scale <= 1e-13
@@ -44,9 +48,10 @@ def test_c_restatement_list_mode():
     m = rectangle_p2(3, 3)
     lm = np.array([0, 2, 5, 7])
     u = fourier_field(m.np, 6, lm.size, 4, 2.0)
-    a = cpu.smb_cross_prod_gauss(m.jj, m.rr, lm, u, 2)
     b = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, lm, u)
-    assert np.max(np.abs(a - b)) / np.max(np.abs(b)) <= 1e-13
+    for fft in (False, True):
+        a = cpu.smb_cross_prod_gauss(m.jj, m.rr, lm, u, 2, fft=fft)
+        assert np.max(np.abs(a - b)) / np.max(np.abs(b)) <= 1e-13
 
 
 def test_c_restatement_rejects_bad_shapes():
@@ -54,3 +59,7 @@ def test_c_restatement_rejects_bad_shapes():
     u = fourier_field(m.np, 6, 4, 1, 2.0)
     with pytest.raises(ValueError):
         cpu.smb_cross_prod_gauss(m.jj, m.rr, np.arange(3), u)
+
+
+def test_fft_grid_sizes():
+    assert [cpu.fft_grid(M) for M in (1, 2, 3, 8, 13, 64, 100, 128)] == [3, 6, 12, 24, 48, 192, 384, 384]
