@@ -341,8 +341,12 @@ int sfx_fft_max_vel_dcl_dev(const double* V1_in, double* V_out, int64_t np, int3
  * initialisation: INIT, SOURCES/INITIALISATION/initialization.F90:1153-1222)
  * and is immutable afterwards.  jj(6,me) is 1-based with the reference local
  * numbering (mesh%jj, SOURCES/MESH/def_type_mesh.F90:74-112), rr(2,np) the
- * node coordinates (r, z).  Elements must be straight-sided (each midpoint
- * node at the middle of its edge), as built by create_iso_grid_distributed.
+ * node coordinates (r, z).  Elements may be curved (midpoint nodes moved onto
+ * a curved or spherical boundary, as create_iso_grid_distributed does,
+ * SOURCES/MESH/refine_mesh.F90:342-353): the kernels take the Jacobian of
+ * the isoparametric map at every Gauss point from all six nodes
+ * (SOURCES/GAUSS_POINTS/gauss_points_2d.F90:108-122).  An element whose
+ * Jacobian vanishes at a Gauss point is rejected (SFX_ERR_ARG).
  * ------------------------------------------------------------------------- */
 typedef struct sfx_mesh_s* sfx_mesh_t;
 

@@ -24,11 +24,13 @@
 // elements is staged in LDS one chunk of L modes at a time.
 // Lane l interpolates mode cL + l of its Gauss point; the partner part of the
 // packed spectrum (mode MP - m) is taken from the partner lane through LDS.
-// Geometry is recomputed per element from the node coordinates (straight-
-// sided P2: constant Jacobian), radius from the six nodes as the reference.
+// Geometry is recomputed per Gauss point from the six node coordinates (the
+// isoparametric Jacobian, exact for curved P2 elements), radius from the six
+// nodes as the reference.
 #pragma once
 #include "sfx_engine.h"
 #include "sfx_fftprod_body.h"
+#include "sfx_geom.h"
 #include "sfx_kernels.h"
 
 namespace sfx {
@@ -147,20 +149,22 @@ SFX_DEV void curl6(const double (&W)[6], const double (&Dr)[6], const double (&D
 }
 
 // Geometry of Gauss point l of the element with node coordinates geo[0..5]:
-// the affine Jacobian from the three vertices and the radius
-// SUM(rr(1,jj(:,m))*ww(:,l)) from all six nodes (subroutine_ns_with_velocity.F90:574).
+// the Jacobian of the isoparametric map from all six nodes at this point,
+// dr(k,k1) = SUM(rr(k,jj(:,m))*dd(k1,:,l)), inverted as dw = J^{-T} dd
+// (gauss_points_2d.F90:108-121), so curved P2 elements (midpoints moved onto
+// a curved boundary, refine_mesh.F90:342-353) are exact; and the radius
+// SUM(rr(1,jj(:,m))*ww(:,l)) (subroutine_ns_with_velocity.F90:574).
 SFX_DEV void gauss_geom(const double2* geo, const double* tab, int l, GaussGeom& g) {
-  const double2 v1 = geo[0], v2 = geo[1], v3 = geo[2];
-  const double j11 = v2.x - v1.x, j12 = v3.x - v1.x;   // dr/dx, dr/dy
-  const double j21 = v2.y - v1.y, j22 = v3.y - v1.y;   // dz/dx, dz/dy
-  const double idet = 1.0 / (j11 * j22 - j12 * j21);
-  g.a11 = j22 * idet;  g.a12 = -j21 * idet;
-  g.a21 = -j12 * idet; g.a22 = j11 * idet;
   g.w = tab + 6 * l;
   g.d = tab + 42 + 12 * l;
-  double ray = 0;
+  double rn[6], zn[6], ray = 0;
 #pragma unroll
-  for (int n = 0; n < 6; ++n) ray = fma(geo[n].x, g.w[n], ray);
+  for (int n = 0; n < 6; ++n) {
+    rn[n] = geo[n].x;
+    zn[n] = geo[n].y;
+    ray = fma(rn[n], g.w[n], ray);
+  }
+  p2_inverse_jacobian(rn, zn, g.d, g.a11, g.a12, g.a21, g.a22);
   g.iray = 1.0 / ray;
 }
 

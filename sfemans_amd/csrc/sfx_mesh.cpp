@@ -67,21 +67,20 @@ int sfx_mesh_create(const int32_t* jj, const double* rr, int32_t n_w, int32_t l_
   if (n_w != 6 || l_G != 7)
     return fail(SFX_ERR_UNSUPPORTED, "sfx_mesh_create: only P2 triangles (n_w=6, l_G=7) are supported");
   if (me < 0 || np < 0 || (me > 0 && (!jj || !rr))) return fail(SFX_ERR_ARG, "sfx_mesh_create: bad sizes");
+  double tab[126];
+  p2_reference_tables(tab);
   for (int64_t e = 0; e < me; ++e) {
     for (int n = 0; n < 6; ++n) {
       const int32_t v = jj[6 * e + n];
       if (v < 1 || v > np) return fail(SFX_ERR_ARG, "sfx_mesh_create: jj(%d,%lld)=%d out of 1..np", n + 1, (long long)e + 1, v);
     }
-    // straight-sided P2: node 3+k is the midpoint of the edge opposite vertex k
-    for (int k = 0; k < 3; ++k) {
-      const int a = (k + 1) % 3, b = (k + 2) % 3;
-      const int64_t na = jj[6 * e + a] - 1, nb = jj[6 * e + b] - 1, nm = jj[6 * e + 3 + k] - 1;
-      for (int c = 0; c < 2; ++c) {
-        const double mid = 0.5 * (rr[2 * na + c] + rr[2 * nb + c]);
-        const double len = std::fabs(rr[2 * na + c] - rr[2 * nb + c]) + std::fabs(rr[2 * na + 1 - c] - rr[2 * nb + 1 - c]);
-        if (std::fabs(rr[2 * nm + c] - mid) > 1e-10 * (len + 1e-300) + 1e-14)
-          return fail(SFX_ERR_UNSUPPORTED, "sfx_mesh_create: element %lld is not straight-sided P2", (long long)e + 1);
-      }
+    // curved elements are allowed (the Jacobian is taken per Gauss point);
+    // the reference divides by det J at every Gauss point (gauss_points_2d.F90:115-120)
+    for (int l = 0; l < 7; ++l) {
+      const double det = p2_jacobian_det(jj + 6 * e, rr, tab, l);
+      if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+        return fail(SFX_ERR_ARG, "sfx_mesh_create: element %lld is degenerate (det J = %g at Gauss point %d)",
+                    (long long)e + 1, det, l + 1);
     }
   }
   return mesh_upload(jj, rr, me, np, out);

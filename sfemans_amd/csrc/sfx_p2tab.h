@@ -40,21 +40,33 @@ inline void p2_stroud_weights(double* p) {
   p[0] = a; p[1] = p[2] = p[3] = b1; p[4] = p[5] = p[6] = b2;
 }
 
-// Per element and Gauss point: rj(l,m) * ray(l,m) of an affine P2 element
-// (gauss_points_2d.F90:122; ray = SUM(rr(1,jj(:,m))*ww(:,l))).
+// Determinant of the isoparametric Jacobian of element J (1-based node ids)
+// at Gauss point l: dr(k,k1) = SUM(rr(k,jj(:,m))*dd(k1,:,l)) (gauss_points_2d.F90:108-114).
+inline double p2_jacobian_det(const int* J, const double* rr, const double* tab, int l) {
+  const double* d = tab + 42 + 12 * l;
+  double j11 = 0, j12 = 0, j21 = 0, j22 = 0;
+  for (int n = 0; n < 6; ++n) {
+    const double r = rr[2 * (J[n] - 1)], z = rr[2 * (J[n] - 1) + 1];
+    j11 += r * d[2 * n];
+    j12 += r * d[2 * n + 1];
+    j21 += z * d[2 * n];
+    j22 += z * d[2 * n + 1];
+  }
+  return j11 * j22 - j12 * j21;
+}
+
+// Per element and Gauss point: rj(l,m) * ray(l,m) = |det J(l)| p(l) ray(l)
+// (gauss_points_2d.F90:122; ray = SUM(rr(1,jj(:,m))*ww(:,l))), with the
+// Jacobian of the isoparametric map at the point (curved elements included).
 inline void p2_assembly_weights(const int* jj, const double* rr, long long me, const double* tab, double* wgt) {
   double p[7];
   p2_stroud_weights(p);
   for (long long e = 0; e < me; ++e) {
     const int* J = jj + 6 * e;
-    const double* v1 = rr + 2 * (J[0] - 1);
-    const double* v2 = rr + 2 * (J[1] - 1);
-    const double* v3 = rr + 2 * (J[2] - 1);
-    const double det = (v2[0] - v1[0]) * (v3[1] - v1[1]) - (v3[0] - v1[0]) * (v2[1] - v1[1]);
     for (int l = 0; l < 7; ++l) {
       double ray = 0;
       for (int n = 0; n < 6; ++n) ray += rr[2 * (J[n] - 1)] * tab[n + 6 * l];
-      wgt[7 * e + l] = std::fabs(det) * p[l] * ray;
+      wgt[7 * e + l] = std::fabs(p2_jacobian_det(J, rr, tab, l)) * p[l] * ray;
     }
   }
 }

@@ -15,6 +15,7 @@
 // with comp = (1,4,5) for vb_145 and (2,3,6) for vb_236 (:75-95).
 #pragma once
 #include "sfx_gather.h"
+#include "sfx_geom.h"
 #include "sfx_portable.h"
 
 namespace sfx {
@@ -137,12 +138,6 @@ SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
     rn[n] = a.rr[2 * j[n]];
     zn[n] = a.rr[2 * j[n] + 1];
   }
-  // affine map of the straight-sided element from its vertices:
-  // d/dr = a11 d/dx + a12 d/dy, d/dz = a21 d/dx + a22 d/dy
-  const double j11 = rn[1] - rn[0], j12 = rn[2] - rn[0];
-  const double j21 = zn[1] - zn[0], j22 = zn[2] - zn[0];
-  const double idet = 1.0 / (j11 * j22 - j12 * j21);
-  const double a11 = j22 * idet, a12 = -j21 * idet, a21 = -j12 * idet, a22 = j11 * idet;
   const double mv = (double)a.lmode[m];
   double V[6][6], F[6][6], P[6][2], Q[6][2];
 #pragma unroll
@@ -173,6 +168,10 @@ SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
   for (int l = 0; l < 7; ++l) {
     const double* w = a.tab + 6 * l;
     const double* d = a.tab + 42 + 12 * l;
+    // isoparametric Jacobian at this Gauss point: d/dr = a11 d/dx + a12 d/dy,
+    // d/dz = a21 d/dx + a22 d/dy (exact for curved elements)
+    double a11, a12, a21, a22;
+    p2_inverse_jacobian(rn, zn, d, a11, a12, a21, a22);
     double ray = 0, ft[6] = {0, 0, 0, 0, 0, 0}, fs[6] = {0, 0, 0, 0, 0, 0};
     double pv[2] = {0, 0}, px[2] = {0, 0}, py[2] = {0, 0}, qv[2] = {0, 0}, qx[2] = {0, 0}, qy[2] = {0, 0};
 #pragma unroll
@@ -219,10 +218,25 @@ SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
   // The tensor terms in a second pass over the Gauss points, once the nodal
   // fields are dead: the two passes never hold both in registers.
   if constexpr (TENSOR) {
+    // the node coordinates again (from L2), so that they are not held over
+    // the first pass: the opaque pointer keeps these loads from being merged
+    // with the first ones
+    const double* rr2 = a.rr;
+    const int32_t* jj2 = a.jj;
+    SFX_OPAQUE_S(rr2);
+    SFX_OPAQUE_S(jj2);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const int64_t jn = (int64_t)jj2[6 * e + n] - 1;
+      rn[n] = rr2[2 * jn];
+      zn[n] = rr2[2 * jn + 1];
+    }
 #pragma unroll 1
     for (int l = 0; l < 7; ++l) {
       const double* w = a.tab + 6 * l;
       const double* d = a.tab + 42 + 12 * l;
+      double a11, a12, a21, a22;
+      p2_inverse_jacobian(rn, zn, d, a11, a12, a21, a22);
       double ray = 0;
 #pragma unroll
       for (int n = 0; n < 6; ++n) ray = fma(rn[n], w[n], ray);

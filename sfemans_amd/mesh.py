@@ -12,6 +12,11 @@ midpoint of the edge opposite local vertex ``k``
 ``rectangle_p2`` produces the same structure for an ``nx x ny`` grid of squares
 on ``[r0,r1] x [z0,z1]``, each square split into two triangles, elements
 numbered row by row and edges numbered in order of first encounter.
+``sector_p2`` meshes an annular sector ``rho0 <= |x| <= rho1``,
+``theta0 <= atan2(z, r) <= theta1`` with curved P2 elements: the midpoint of
+every edge on one of the two arcs is moved radially onto the circle, as the
+reference places the midpoints of sides on a spherical boundary
+(SOURCES/MESH/refine_mesh.F90:342-353, 1043-1047).
 ``load_fem_p2`` reads the P1 section of a formatted SFEMaNS ``.FEM`` mesh file
 (SOURCES/MESH/load_mesh.F90:59-93: ``np nw me nws mes``, then per cell
 ``jj(nw) neigh(3) i_d``, per boundary side ``jjs(nws) neighs sides``, per node
@@ -24,7 +29,8 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["P2Mesh", "rectangle_p2", "p2_from_p1", "read_fem_p1", "load_fem_p2"]
+__all__ = ["P2Mesh", "rectangle_p2", "sector_p2", "p2_from_p1", "read_fem_p1", "load_fem_p2",
+           "edge_ends"]
 
 
 class P2Mesh:
@@ -90,6 +96,43 @@ def rectangle_p2(nx: int, ny: int, r0: float = 0.0, r1: float = 1.0,
     tri[1::2] = t2
     tri.sort(axis=1)
     return p2_from_p1(tri, vr, vz, nx, ny)
+
+
+def edge_ends(jj: np.ndarray) -> np.ndarray:
+    """(3, 2, me) 1-based end nodes of the edge of each local midpoint node
+    3+k: the two vertices other than local vertex k."""
+    jj = np.asarray(jj)
+    return np.stack([np.stack([jj[1], jj[2]]), np.stack([jj[0], jj[2]]), np.stack([jj[0], jj[1]])])
+
+
+def sector_p2(nrho: int, ntheta: int, rho0: float = 0.5, rho1: float = 1.0, theta0: float = -np.pi / 3,
+              theta1: float = np.pi / 3, curved: bool = True) -> P2Mesh:
+    """P2 mesh of ``2*nrho*ntheta`` triangles on the annular sector
+    ``rho0 <= sqrt(r^2+z^2) <= rho1``, ``theta0 <= atan2(z, r) <= theta1``
+    (r > 0 when |theta| < pi/2).  Vertices of an (rho, theta) grid, squares
+    split as in ``rectangle_p2``, then, with ``curved``, the midpoint of every
+    edge whose two ends lie on the inner or the outer arc rescaled onto that
+    circle about the origin: rr = rr_ref * radius / |rr_ref|
+    (refine_mesh.F90:1043-1047)."""
+    if nrho < 1 or ntheta < 1:
+        raise ValueError("nrho and ntheta must be >= 1")
+    base = rectangle_p2(nrho, ntheta, rho0, rho1, theta0, theta1)
+    nv = (nrho + 1) * (ntheta + 1)
+    rho, th = base.rr[0, :nv].copy(), base.rr[1, :nv].copy()
+    vr, vz = rho * np.cos(th), rho * np.sin(th)
+    tri = base.jj[0:3].T.astype(np.int64) - 1
+    m = p2_from_p1(tri, vr, vz, nrho, ntheta)
+    if curved:
+        ends = edge_ends(m.jj) - 1                                   # (3, 2, me) 0-based
+        mids = m.jj[3:6] - 1                                         # (3, me)
+        on_in = np.isclose(rho, rho0, rtol=0, atol=1e-12 * rho1)
+        on_out = np.isclose(rho, rho1, rtol=0, atol=1e-12 * rho1)
+        for ring, radius in ((on_in, rho0), (on_out, rho1)):
+            sel = ring[ends[:, 0]] & ring[ends[:, 1]]                # both ends on this arc
+            nodes = np.unique(mids[sel])
+            rr = m.rr[:, nodes]
+            m.rr[:, nodes] = rr * (radius / np.sqrt(np.sum(rr * rr, axis=0)))
+    return m
 
 
 def p2_from_p1(tri: np.ndarray, vr: np.ndarray, vz: np.ndarray, nx: int = 0, ny: int = 0) -> P2Mesh:

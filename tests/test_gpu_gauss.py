@@ -98,10 +98,14 @@ def test_mesh_validation_errors():
     with pytest.raises(SfxError):
         NL.Mesh(jj, m.rr)
     rr = m.rr.copy(order="F")
-    rr[:, m.jj[3, 0] - 1] += 0.01      # curved edge
+    rr[:, m.jj[3, 0] - 1] += 0.01      # a curved edge is accepted
+    NL.Mesh(m.jj, rr).close()
+    rr = m.rr.copy(order="F")
+    rr[:, m.jj[1, 0] - 1] = rr[:, m.jj[0, 0] - 1]   # two vertices at one place: det J = 0
+    rr[:, m.jj[5, 0] - 1] = rr[:, m.jj[0, 0] - 1]
     with pytest.raises(SfxError) as e:
         NL.Mesh(m.jj, rr)
-    assert e.value.code == 2
+    assert e.value.code == 1
 
 
 def test_full_size_mhd_sampled():
