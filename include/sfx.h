@@ -353,6 +353,10 @@ typedef struct sfx_mesh_s* sfx_mesh_t;
 int sfx_mesh_create(const int32_t* jj, const double* rr, int32_t n_w, int32_t l_G, int64_t me, int64_t np,
                     sfx_mesh_t* out);
 int sfx_mesh_destroy(sfx_mesh_t mesh);
+/* Boundary sides of the mesh, for the boundary term of rhs_3x3: jjs(3, mes)
+ * 1-based (mesh%jjs, SOURCES/MESH/def_type_mesh.F90:76; the two ends of the
+ * side, then its midpoint node).  Replaces any sides registered before. */
+int sfx_mesh_set_boundary(sfx_mesh_t mesh, const int32_t* jjs, int64_t mes);
 int sfx_mesh_info(sfx_mesh_t mesh, int64_t* me, int64_t* np);
 
 /* ---------------------------------------------------------------------------
@@ -448,6 +452,20 @@ int sfx_nl_adjoint_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_
  * (SOURCES/SOLVER/assembling_rhs.F90:5): one mode, rhs_in(G,6) ->
  * vb_145(np,3), vb_236(np,3).  `mode` is accepted for signature parity. */
 int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_145, double* vb_236);
+/* Replaces rhs_3x3 with all its options (SOURCES/SOLVER/assembling_rhs.F90:5-189),
+ * one mode, host arrays: rhs_in(G,6), opt_tensor(3,G,6) (nullable; its 1/r
+ * and mode terms use `mode`, :56-68), opt_tensor_scaln_bdy(3*mes,6) at the
+ * three Gauss points of every boundary side (nullable; needs
+ * sfx_mesh_set_boundary, :129-185) -> vb_145(np,3), vb_236(np,3). */
+int sfx_rhs_3x3_mode(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, const double* opt_tensor,
+                     const double* opt_tensor_scaln_bdy, double* vb_145, double* vb_236);
+/* The boundary term of rhs_3x3 for every mode, ADDED to vb_145(np,3,m) and
+ * vb_236(np,3,m) (device arrays) times scale: tensor_bdy(3*mes,6,m), per side
+ * and Gauss point smb = T*ray*rjs with points on the axis (ray < 1e-10)
+ * skipped (assembling_rhs.F90:129-185); each boundary node adds its sides in
+ * side order after the volume terms, as the reference's VecSetValues calls do. */
+int sfx_rhs_3x3_bdy_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* tensor_bdy, double* vb_145,
+                        double* vb_236, double scale, void* stream);
 /* All modes at once on the device: rhs_gauss(G,6,m) -> vb_145(np,3,m),
  * vb_236(np,3,m), each multiplied by `scale`.  work_elem is a caller-provided
  * device buffer of 36*me*m doubles (the per-element contributions). */
@@ -473,6 +491,26 @@ int sfx_rhs_ns_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
                        const double* V1m, const double* ff, const double* P, const double* P_inc,
                        const double* opt_tensor, double* work_elem, double* vb_145, double* vb_236,
                        void* stream);
+
+/* Replaces rhs_ns_gauss_3x3 with its penalty options (if_ns_penalty,
+ * if_impose_vel_in_solids; SOURCES/SOLVER/computing_rhs_gauss.F90:7-146)
+ * followed by rhs_3x3 for every mode: rhs0 = ft + fp_inc + fs - rotv_v at the
+ * Gauss points, minus imposed_vel on the slot mode0_slot (the slot i with
+ * list_mode(i) = 0, or -1) (:54-121); then the product with the penalty
+ * function penal_grid(2*m_max_pad-1, G), its values on the grid angles at the
+ * Gauss points, m_max_pad = 3*m_max_c/2 (FFT_PAR_VAR_ETA_PROD_GAUSS_DCL,
+ * :124-129); then imposed_vel added back on mode0_slot and fp added
+ * (:132-145), and the result assembled with opt_tensor as
+ * sfx_rhs_ns_3x3_dev does.  imposed_vel(np,6) is the nodal array
+ * 3*imposed_velocity_by_penalty/(2 dt) (:107) or null; the caller evaluates
+ * penal_in_real_space on the grid (the user function of the reference).
+ * work_gauss: 2*G*6*m doubles; work_elem: 36*me*m doubles.  The penalty
+ * product uses the exact-grid scratch described above. */
+int sfx_rhs_ns_3x3_penal_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, int32_t mode0_slot,
+                             const double* rotv_v, const double* V1m, const double* ff, const double* P,
+                             const double* P_inc, const double* penal_grid, const double* imposed_vel,
+                             const double* opt_tensor, double* work_gauss, double* work_elem, double* vb_145,
+                             double* vb_236, void* stream);
 
 /* Replaces qs_00_gauss (SOURCES/SOLVER/fem_rhs_axi.F90:940): vect(np) from
  * heat_capa(me), nodal ff(np) and Gauss ff_gauss(G). */

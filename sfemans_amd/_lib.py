@@ -111,7 +111,12 @@ SIGNATURES = {
     "sfx_nl_adjoint_lorentz_dev": [_vp, _vp, _i32, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp],
     "sfx_nl_adjoint_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_rhs_3x3": [_vp, _i32, _vp, _vp, _vp],
+    "sfx_rhs_3x3_mode": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
+    "sfx_rhs_3x3_bdy_dev": [_vp, _i32, _vp, _vp, _vp, _d, _vp],
+    "sfx_mesh_set_boundary": [_vp, _vp, _i64],
     "sfx_rhs_3x3_dev": [_vp, _i32, _vp, _vp, _vp, _d, _vp, _vp],
+    "sfx_rhs_ns_3x3_penal_dev": [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp, _vp],
     "sfx_rhs_ns_3x3_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],
     "sfx_qs_00_gauss_dev": [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -52,6 +52,7 @@ struct GaussArgs {
 };
 
 struct RhsArgs;
+struct BdyArgs;
 struct AdjArgs;
 struct GatherArgs;
 #ifndef SFX_EMULATE
@@ -59,6 +60,8 @@ hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, const GatherAr
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st);
 hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
+hipError_t launch_rhs_bdy(const BdyArgs& a, hipStream_t st);
+hipError_t launch_rhs_ns_gauss(const RhsArgs& a, hipStream_t st);
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);
 hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
 // Exact-grid products (sfx_exact_body.h): angle pass, then r2c, or the maximum into max_out.

@@ -8,6 +8,7 @@
 #include "../../include/sfx.h"
 #include "sfx_api_internal.h"
 #include "sfx_kernels.h"
+#include "sfx_bdy_setup.h"
 #include "sfx_p2tab.h"
 #include "sfx_rhs_body.h"
 #include "sfx_adjoint_body.h"
@@ -94,7 +95,44 @@ int sfx_mesh_destroy(sfx_mesh_t m) {
   (void)hipFree(m->adj_ptr);
   (void)hipFree(m->adj_ent);
   (void)hipFree(m->wgt);
+  (void)hipFree(m->bptr);
+  (void)hipFree(m->bent);
+  (void)hipFree(m->bnode);
+  (void)hipFree(m->bw);
   delete m;
+  return SFX_OK;
+}
+
+int sfx_mesh_set_boundary(sfx_mesh_t m, const int32_t* jjs, int64_t mes) {
+  clear_err();
+  if (!m) return fail(SFX_ERR_ARG, "sfx_mesh_set_boundary: null mesh");
+  if (mes < 0 || (mes > 0 && !jjs)) return fail(SFX_ERR_ARG, "sfx_mesh_set_boundary: bad sizes");
+  for (int64_t s = 0; s < 3 * mes; ++s)
+    if (jjs[s] < 1 || jjs[s] > m->np)
+      return fail(SFX_ERR_ARG, "sfx_mesh_set_boundary: jjs(%d,%lld)=%d out of 1..np", (int)(s % 3) + 1,
+                  (long long)(s / 3) + 1, jjs[s]);
+  std::vector<double> rr(2 * m->np);
+  hipError_t e;
+  if (m->np && (e = hipMemcpy(rr.data(), m->rr, 2 * m->np * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_fail(e, "sfx_mesh_set_boundary: coordinates");
+  BdyTables t;
+  build_boundary_tables(jjs, mes, rr.data(), m->np, t);
+  (void)hipFree(m->bptr); (void)hipFree(m->bent); (void)hipFree(m->bnode); (void)hipFree(m->bw);
+  m->bptr = nullptr; m->bent = nullptr; m->bnode = nullptr; m->bw = nullptr; m->mes = -1;
+  const int64_t nb = (int64_t)t.node.size();
+  if ((e = hipMalloc(&m->bptr, (nb + 1) * sizeof(int64_t))) != hipSuccess ||
+      (e = hipMalloc(&m->bent, t.ent.size() * sizeof(int32_t) + 4)) != hipSuccess ||
+      (e = hipMalloc(&m->bnode, nb * sizeof(int64_t) + 8)) != hipSuccess ||
+      (e = hipMalloc(&m->bw, t.bw.size() * sizeof(double) + 8)) != hipSuccess)
+    return fail(SFX_ERR_NOMEM, "sfx_mesh_set_boundary: hipMalloc: %s", hipGetErrorString(e));
+  if ((e = hipMemcpy(m->bptr, t.ptr.data(), (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+      (nb && (e = hipMemcpy(m->bent, t.ent.data(), t.ent.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess) ||
+      (nb && (e = hipMemcpy(m->bnode, t.node.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess) ||
+      (mes && (e = hipMemcpy(m->bw, t.bw.data(), t.bw.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess))
+    return hip_fail(e, "sfx_mesh_set_boundary: upload");
+  m->mes = mes;
+  m->nb = nb;
+  m->n_bent = (int64_t)t.ent.size();
   return SFX_OK;
 }
 
@@ -453,12 +491,12 @@ int rhs3x3_dev(sfx_mesh_t mesh, int32_t M, const double* rhs, double* v145, doub
 
 int rhs_ns_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M, const double* rotv_v, const double* V1m,
                const double* ff, const double* P, const double* P_inc, const double* tensor, double* work_elem,
-               double* v145, double* v236, hipStream_t st) {
+               double* v145, double* v236, hipStream_t st, double rsign = -1.0) {
   if (!work_elem && mesh->me > 0 && M > 0) return fail(SFX_ERR_ARG, "rhs_ns_3x3: null work_elem");
   if (!list_mode_dev && M > 0) return fail(SFX_ERR_ARG, "rhs_ns_3x3: null list_mode");
   RhsArgs a = rhs_args(mesh, M, 1.0);
   a.rr = mesh->rr; a.lmode = list_mode_dev;
-  a.rhs = rotv_v; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = P_inc; a.tens = tensor;
+  a.rhs = rotv_v; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = P_inc; a.tens = tensor; a.rsign = rsign;
   a.v145 = v145; a.v236 = v236; a.elo = work_elem;
   a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = a.np * 6 * (int64_t)M; a.n_tens = 3 * a.n_rhs;
   a.n_pres = a.n_pinc = a.np * 2 * (int64_t)M;
@@ -481,12 +519,113 @@ int sfx_rhs_ns_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
                     vb_236, (hipStream_t)stream);
 }
 
+int sfx_rhs_ns_3x3_penal_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, int32_t mode0_slot,
+                             const double* rotv_v, const double* V1m, const double* ff, const double* P,
+                             const double* P_inc, const double* penal_grid, const double* imposed_vel,
+                             const double* opt_tensor, double* work_gauss, double* work_elem, double* vb_145,
+                             double* vb_236, void* stream) {
+  clear_err();
+  const char* nm = "rhs_ns_3x3 (penalty)";
+  if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
+  if (m_max_c <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
+  if (!penal_grid) return fail(SFX_ERR_ARG, "%s: null penal_grid", nm);
+  if (!work_gauss && mesh->me > 0) return fail(SFX_ERR_ARG, "%s: null work_gauss", nm);
+  if (mode0_slot < -1 || mode0_slot >= m_max_c) return fail(SFX_ERR_ARG, "%s: mode0_slot out of range", nm);
+  if (!work_elem && mesh->me > 0) return fail(SFX_ERR_ARG, "%s: null work_elem", nm);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t G = 7 * mesh->me, nG = G * 6 * (int64_t)m_max_c;
+  double* g0 = work_gauss;
+  double* g1 = work_gauss + nG;
+  // (1) ft + fp_inc + fs - rotv_v at the Gauss points, minus the imposed
+  //     velocity on the mode-0 slot (computing_rhs_gauss.F90:54-121)
+  RhsArgs a = rhs_args(mesh, m_max_c, 1.0);
+  a.rr = mesh->rr; a.lmode = list_mode_dev;
+  a.rhs = rotv_v; a.v1m = V1m; a.fsrc = ff; a.pinc = P_inc; a.rsign = -1.0;
+  a.imp = imposed_vel; a.imp_slot = imposed_vel ? mode0_slot : -1; a.isign = -1.0; a.gout = g0;
+  a.n_rhs = nG; a.n_v1m = a.n_fsrc = a.np * 6 * (int64_t)m_max_c; a.n_pinc = a.np * 2 * (int64_t)m_max_c;
+  a.n_imp = a.np * 6; a.n_gout = nG;
+  hipError_t e = launch_rhs_ns_gauss(a, st);
+  if (e != hipSuccess) return hip_fail(e, nm);
+  // (2) the product with the penalty function on the grid, m_max_pad = 3*m/2
+  //     (FFT_PAR_VAR_ETA_PROD_GAUSS_DCL, :124-129)
+  int rc = sfx_fft_par_var_eta_prod_dcl_dev(penal_grid, g0, g1, G, 6, m_max_c, (3 * m_max_c) / 2, stream);
+  if (rc) return rc;
+  // (3) plus the imposed velocity on the mode-0 slot, plus fp, assembled by
+  //     rhs_3x3 with its optional tensor (:132-145)
+  RhsArgs b = rhs_args(mesh, m_max_c, 1.0);
+  b.rr = mesh->rr; b.lmode = list_mode_dev;
+  b.rhs = g1; b.pres = P; b.tens = opt_tensor; b.rsign = 1.0;
+  b.imp = imposed_vel; b.imp_slot = imposed_vel ? mode0_slot : -1; b.isign = 1.0;
+  b.v145 = vb_145; b.v236 = vb_236; b.elo = work_elem;
+  b.n_rhs = nG; b.n_pres = b.np * 2 * (int64_t)m_max_c; b.n_tens = 3 * nG; b.n_imp = b.np * 6;
+  b.n_out = b.np * 3 * (int64_t)m_max_c; b.n_elo = 36 * b.me * (int64_t)m_max_c;
+  e = launch_rhs_ns(b, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
 int sfx_rhs_3x3_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* rhs_gauss, double* vb_145, double* vb_236,
                     double scale, double* work_elem, void* stream) {
   clear_err();
   if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3: null mesh");
   if (m_max_c < 0) return fail(SFX_ERR_ARG, "rhs_3x3: m_max_c < 0");
   return rhs3x3_dev(mesh, m_max_c, rhs_gauss, vb_145, vb_236, scale, work_elem, (hipStream_t)stream);
+}
+
+int sfx_rhs_3x3_bdy_dev(sfx_mesh_t mesh, int32_t m_max_c, const double* tensor_bdy, double* vb_145,
+                        double* vb_236, double scale, void* stream) {
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3 boundary term: null mesh");
+  if (mesh->mes < 0)
+    return fail(SFX_ERR_ARG, "rhs_3x3 boundary term: no boundary sides registered (sfx_mesh_set_boundary)");
+  if (m_max_c < 0) return fail(SFX_ERR_ARG, "rhs_3x3 boundary term: m_max_c < 0");
+  if (mesh->nb == 0 || m_max_c == 0) return SFX_OK;
+  if (!tensor_bdy || !vb_145 || !vb_236) return fail(SFX_ERR_ARG, "rhs_3x3 boundary term: null array");
+  BdyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  double d[9], p[3];
+  p2_edge_tables(a.wws, d, p);
+  a.bptr = mesh->bptr; a.bent = mesh->bent; a.bnode = mesh->bnode; a.bw = mesh->bw;
+  a.nb = mesh->nb; a.mes = mesh->mes; a.np = mesh->np; a.M = m_max_c; a.scale = scale;
+  a.tb = tensor_bdy; a.v145 = vb_145; a.v236 = vb_236;
+  a.n_tb = 3 * mesh->mes * 6 * (int64_t)m_max_c; a.n_out = mesh->np * 3 * (int64_t)m_max_c; a.n_ent = mesh->n_bent;
+  hipError_t e = launch_rhs_bdy(a, (hipStream_t)stream);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, "rhs_3x3 boundary term");
+}
+
+int sfx_rhs_3x3_mode(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, const double* opt_tensor,
+                     const double* opt_tensor_scaln_bdy, double* vb_145, double* vb_236) {
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "rhs_3x3: null mesh");
+  if (opt_tensor_scaln_bdy && mesh->mes < 0)
+    return fail(SFX_ERR_ARG, "rhs_3x3: opt_tensor_scaln_bdy needs the boundary sides (sfx_mesh_set_boundary)");
+  const size_t nG = (size_t)7 * mesh->me * 6, nN = (size_t)mesh->np * 3;
+  const size_t nB = opt_tensor_scaln_bdy ? (size_t)3 * mesh->mes * 6 : 0;
+  void *dr, *d1, *d2, *dw, *dt, *db, *dm;
+  int rc;
+  if ((rc = staging(0, nG * 8 + 8, &dr)) || (rc = staging(1, nN * 8 + 8, &d1)) || (rc = staging(2, nN * 8 + 8, &d2)) ||
+      (rc = staging(3, (size_t)36 * mesh->me * 8 + 8, &dw)) || (rc = staging(4, (opt_tensor ? 3 * nG : 0) * 8 + 8, &dt)) ||
+      (rc = staging(5, nB * 8 + 8, &db)) || (rc = staging(6, 8, &dm)))
+    return rc;
+  hipError_t e;
+  if ((e = hipMemcpy(dr, rhs_in, nG * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (opt_tensor && (e = hipMemcpy(dt, opt_tensor, 3 * nG * 8, hipMemcpyHostToDevice)) != hipSuccess) ||
+      (nB && (e = hipMemcpy(db, opt_tensor_scaln_bdy, nB * 8, hipMemcpyHostToDevice)) != hipSuccess) ||
+      (e = hipMemcpy(dm, &mode, sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "H2D");
+  // with opt_tensor: the NS right-hand-side element kernel with rhs_in as its
+  // Gauss field (sign +1) and every other field absent, plus the tensor terms
+  if (opt_tensor) {
+    if ((rc = rhs_ns_dev(mesh, (const int32_t*)dm, 1, (const double*)dr, nullptr, nullptr, nullptr, nullptr,
+                         (const double*)dt, (double*)dw, (double*)d1, (double*)d2, nullptr, 1.0)))
+      return rc;
+  } else if ((rc = rhs3x3_dev(mesh, 1, (const double*)dr, (double*)d1, (double*)d2, 1.0, (double*)dw, nullptr))) {
+    return rc;
+  }
+  if (nB && (rc = sfx_rhs_3x3_bdy_dev(mesh, 1, (const double*)db, (double*)d1, (double*)d2, 1.0, nullptr)))
+    return rc;
+  if ((e = hipMemcpy(vb_145, d1, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  if ((e = hipMemcpy(vb_236, d2, nN * 8, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "D2H");
+  return SFX_OK;
 }
 
 int sfx_rhs_3x3(sfx_mesh_t mesh, int32_t mode, const double* rhs_in, double* vb_145, double* vb_236) {

@@ -16,6 +16,11 @@ __global__ void __launch_bounds__(256) rhs_ns_elem_kernel(RhsArgs a) {
   rhs_ns_elem_body<TENSOR>(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
+// The Gauss values of the NS right-hand side before the penalty product.
+__global__ void __launch_bounds__(256) rhs_ns_gauss_kernel(RhsArgs a) {
+  rhs_ns_elem_body<false, true>(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
+}
+
 __global__ void __launch_bounds__(GATHER_THREADS) rhs_gather_kernel(RhsArgs a) {
   __shared__ double tile[gather_tile_doubles<6>()];
   rhs_gather_block(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, tile);
@@ -54,6 +59,23 @@ hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st) {
   if (e != hipSuccess || nn == 0) return e;
   const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;
   hipLaunchKernelGGL(rhs_gather_kernel, dim3((unsigned)nb), dim3(GATHER_THREADS), 0, st, a);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) rhs_bdy_kernel(BdyArgs a) {
+  rhs_bdy_body(a, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+// Boundary term of rhs_3x3, added to vb after the volume gather.
+hipError_t launch_rhs_bdy(const BdyArgs& a, hipStream_t st) {
+  const int64_t n = a.nb * (int64_t)a.M;
+  if (n > 0) hipLaunchKernelGGL(rhs_bdy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs_ns_gauss(const RhsArgs& a, hipStream_t st) {
+  const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0;
+  if (ne > 0) hipLaunchKernelGGL(rhs_ns_gauss_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -48,7 +48,14 @@ struct RhsArgs {
   const double* pres;       // (np, 2, M) pressure on the velocity mesh
   const double* pinc;       // (np, 2, M) pressure increment on the velocity mesh
   const double* tens;       // (3, G, 6, M) opt_tensor of rhs_3x3
-  int64_t n_v1m, n_fsrc, n_pres, n_pinc, n_tens;
+  double rsign;             // sign of the Gauss field rhs: -1 for -rotv_v, +1 for rhs_3x3's rhs_in
+  // velocity imposed in solids by penalty (computing_rhs_gauss.F90:103-121):
+  // isign * SUM(imp*ww) added to the mode slot imp_slot (-1: none)
+  const double* imp;        // (np, 6) 3 u_imposed / (2 dt)
+  int imp_slot;
+  double isign;
+  double* gout;             // (G, 6, M): with the Gauss output, rhs without fp goes here
+  int64_t n_v1m, n_fsrc, n_pres, n_pinc, n_tens, n_imp, n_gout;
 };
 
 // Work item gid of the element kernels -> (element e, mode or column m).  A
@@ -113,6 +120,8 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
 // options (SOURCES/SOLVER/computing_rhs_gauss.F90:7-146) fused with the
 // element share of rhs_3x3, one thread per (element, mode) (elem_item):
 //   rhs(l, :) = (ft + fp_inc + fs - rotv_v(7e+l, :)) + fp         (:100, :145)
+// (rsign = -1; with rsign = +1 and the other fields absent it is rhs_3x3's
+// own rhs_in, so the kernel also serves rhs_3x3 with opt_tensor)
 //   ft = SUM(V1m*ww), fs = SUM(ff*ww),
 //   fp = (-d_r P_c, -d_r P_s, -P_s/ray*m, P_c/ray*m, -d_z P_c, -d_z P_s) (:69-97)
 // with P and P_inc the pressure injected onto the velocity mesh; then
@@ -123,7 +132,11 @@ SFX_DEV void rhs_elem_body(const RhsArgs& a, int64_t gid) {
 // with f_tensor the 1/r and mode terms (row k/2 of T, its cos columns for
 // even k, sin for odd).
 // The Gauss array rhs_gauss is never stored.
-template <bool TENSOR>
+// With GOUT the kernel writes the Gauss values (ft + fp_inc + fs - rotv_v,
+// minus the imposed velocity on its slot) to gout instead of assembling them:
+// the first step of the penalty branch, whose product with the penalty
+// function mixes the modes at each Gauss point (computing_rhs_gauss.F90:124-129).
+template <bool TENSOR, bool GOUT = false>
 SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
   int64_t e;
   int m;
@@ -159,6 +172,7 @@ SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
       Q[n][k] = a.pinc ? a.pinc[i2] : 0.0;
     }
   }
+  const bool imposed = a.imp && m == a.imp_slot;
   double loc[6][6];
 #pragma unroll
   for (int i = 0; i < 6; ++i)
@@ -210,11 +224,28 @@ SFX_DEV void rhs_ns_elem_body(const RhsArgs& a, int64_t gid) {
     for (int k = 0; k < 6; ++k) {
       SFX_CHECK("rotv load", gi + a.G * k, a.rhs ? a.n_rhs : 1 << 30);
       const double rv = a.rhs ? a.rhs[gi + a.G * k] : 0.0;
-      const double r = ((ft[k] + fq[k] + fs[k] - rv) + fp[k]) * wl;
+      double base = ft[k] + fq[k] + fs[k] + a.rsign * rv;
+      if (imposed) {
+        double iv = 0;
 #pragma unroll
-      for (int i = 0; i < 6; ++i) loc[i][k] = fma(w[i], r, loc[i][k]);
+        for (int n = 0; n < 6; ++n) {
+          const int64_t jn = (int64_t)a.jj[6 * e + n] - 1;
+          SFX_CHECK("imposed velocity load", jn + a.np * k, a.n_imp);
+          iv = fma(a.imp[jn + a.np * k], w[n], iv);
+        }
+        base = base + a.isign * iv;
+      }
+      if constexpr (GOUT) {
+        SFX_CHECK("gauss store", gi + a.G * k, a.n_gout);
+        a.gout[gi + a.G * k] = base;
+      } else {
+        const double r = (base + fp[k]) * wl;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) loc[i][k] = fma(w[i], r, loc[i][k]);
+      }
     }
   }
+  if constexpr (GOUT) return;
   // The tensor terms in a second pass over the Gauss points, once the nodal
   // fields are dead: the two passes never hold both in registers.
   if constexpr (TENSOR) {
@@ -338,6 +369,77 @@ SFX_DEV void qs00_gather_block(const RhsArgs& a, int64_t bid, int tid, double* t
                    SFX_CHECK("vect store", n + a.np * c, a.n_out);
                    a.vect[n + a.np * c] = a.scale * v;
                  });
+}
+
+// ---- boundary term of rhs_3x3 (opt_tensor_scaln_bdy, assembling_rhs.F90:129-185)
+// For every boundary side ms (mesh%jjs: two ends, then the midpoint) and its
+// three Gauss points, smb = T_bdy(3 ms + ls, :) * ray * rjs (points on the
+// axis, ray < 1e-10, skipped), added to the side's nodes with the edge shape
+// values wws and the (1,4,5)/(2,3,6) packing of the volume term.  Each
+// boundary node is gathered by one thread from its (side, local node) list
+// in side order, after the volume sums (the reference's VecSetValues order:
+// elements first, then sides).
+struct BdyArgs {
+  const int64_t* bptr;      // (nb+1) offsets of each boundary node's entries
+  const int32_t* bent;      // 3*ms + local node, side order
+  const int64_t* bnode;     // (nb) node index (0-based)
+  const double* bw;         // (3, mes): ray * rjs per side Gauss point, 0 where skipped
+  double wws[9];            // wws(ns, ls) at 3*ls + ns
+  int64_t nb, mes, np;
+  int M;
+  double scale;
+  const double* tb;         // (3*mes, 6, M)
+  double* v145;             // (np, 3, M)
+  double* v236;             // (np, 3, M)
+  int64_t n_tb, n_out, n_ent;
+};
+
+// One thread per (boundary node, mode), nodes fastest.
+SFX_DEV void rhs_bdy_body(const BdyArgs& a, int64_t gid) {
+  if (a.nb <= 0) return;
+  const int64_t b = gid % a.nb, m = gid / a.nb;
+  if (m >= a.M) return;
+  const int64_t n = a.bnode[b];
+  // component k -> (array, column): 1,4,5 -> vb_145 columns 1,2,3; 2,3,6 -> vb_236 columns 1,2,3
+  int64_t o[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int col = (k == 0 || k == 1) ? 0 : (k == 2 || k == 3) ? 1 : 2;
+    o[k] = n + a.np * (col + 3 * m);
+    SFX_CHECK("vb bdy", o[k], a.n_out);
+  }
+  double acc[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const bool v145 = (k == 0 || k == 3 || k == 4);
+    acc[k] = (v145 ? a.v145 : a.v236)[o[k]];
+  }
+  for (int64_t t = a.bptr[b]; t < a.bptr[b + 1]; ++t) {
+    SFX_CHECK("bdy entry", t, a.n_ent);
+    const int32_t en = a.bent[t];
+    const int64_t ms = en / 3;
+    const int ns = en % 3;
+    double side[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int ls = 0; ls < 3; ++ls) {
+      const double wl = a.bw[3 * ms + ls];
+      if (wl == 0.0) continue;                      // a point on the axis (ray < 1e-10)
+      const double c = a.wws[3 * ls + ns] * wl;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const int64_t it = 3 * ms + ls + 3 * a.mes * (k + 6 * m);
+        SFX_CHECK("T_bdy load", it, a.n_tb);
+        side[k] = fma(c, a.tb[it], side[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] = fma(a.scale, side[k], acc[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const bool v145 = (k == 0 || k == 3 || k == 4);
+    (v145 ? a.v145 : a.v236)[o[k]] = acc[k];
+  }
 }
 
 }  // namespace sfx

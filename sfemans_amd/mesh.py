@@ -30,7 +30,7 @@ from __future__ import annotations
 import numpy as np
 
 __all__ = ["P2Mesh", "rectangle_p2", "sector_p2", "p2_from_p1", "read_fem_p1", "load_fem_p2",
-           "edge_ends"]
+           "edge_ends", "boundary_sides"]
 
 
 class P2Mesh:
@@ -103,6 +103,25 @@ def edge_ends(jj: np.ndarray) -> np.ndarray:
     3+k: the two vertices other than local vertex k."""
     jj = np.asarray(jj)
     return np.stack([np.stack([jj[1], jj[2]]), np.stack([jj[0], jj[2]]), np.stack([jj[0], jj[1]])])
+
+
+def boundary_sides(jj: np.ndarray) -> np.ndarray:
+    """Boundary sides of a P2 mesh, ``jjs(3, mes)`` 1-based as mesh%jjs holds
+    them: the two end vertices, then the midpoint node.  A side is an edge of
+    a single element; sides are listed in order of their element, then of the
+    edge opposite local vertex 0, 1, 2."""
+    jj = np.asarray(jj)
+    ends = edge_ends(jj)                                        # (3, 2, me)
+    mids = jj[3:6]                                              # (3, me)
+    flat_mid = mids.T.ravel()                                   # element-major
+    uniq, cnt = np.unique(flat_mid, return_counts=True)
+    single = set(uniq[cnt == 1].tolist())
+    cols = []
+    for e in range(jj.shape[1]):
+        for k in range(3):
+            if int(mids[k, e]) in single:
+                cols.append((ends[k, 0, e], ends[k, 1, e], mids[k, e]))
+    return np.asfortranarray(np.array(cols, dtype=np.int32).reshape(-1, 3).T)
 
 
 def sector_p2(nrho: int, ntheta: int, rho0: float = 0.5, rho1: float = 1.0, theta0: float = -np.pi / 3,

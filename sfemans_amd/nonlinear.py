@@ -19,7 +19,7 @@ from ._lib import lib, check
 from .fft_parallele import fdims, is_device, ptr, stream_ptr
 
 __all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "smb_CurlH_cross_B_gauss_sft_par",
-           "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes",
+           "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes", "rhs_3x3_bdy",
            "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd", "nl_adjoint", "nl_adjoint_lorentz",
            "nl_adjoint_mhd"]
 
@@ -39,8 +39,21 @@ class Mesh:
         self._lm_cache = {}
 
     @classmethod
-    def from_p2(cls, m):
-        return cls(m.jj, m.rr)
+    def from_p2(cls, m, boundary=False):
+        """Mesh of a P2Mesh; with ``boundary`` its boundary sides are registered too."""
+        mesh = cls(m.jj, m.rr)
+        if boundary:
+            from .mesh import boundary_sides
+            mesh.set_boundary(boundary_sides(m.jj))
+        return mesh
+
+    def set_boundary(self, jjs):
+        """Register the boundary sides jjs(3, mes) (mesh%jjs), for rhs_3x3's boundary term."""
+        jjs = np.asfortranarray(jjs, dtype=np.int32)
+        if jjs.ndim != 2 or jjs.shape[0] != 3:
+            raise ValueError("expected jjs(3, mes)")
+        check(lib().sfx_mesh_set_boundary(self._h, jjs.ctypes.data, int(jjs.shape[1])))
+        self.mes = int(jjs.shape[1])
 
     @property
     def handle(self):
@@ -149,13 +162,31 @@ def nl_rotv_mhd(mesh: Mesh, list_mode, u, H, B, coeff_lorentz, rotv_v):
     return rotv_v
 
 
-def rhs_3x3(mesh: Mesh, mode, rhs_in, vb_145, vb_236):
-    """One mode (SOURCES/SOLVER/assembling_rhs.F90:5): rhs_in(G,6) -> vb_145(np,3), vb_236(np,3).
+def rhs_3x3(mesh: Mesh, mode, rhs_in, vb_145, vb_236, opt_tensor=None, opt_tensor_scaln_bdy=None):
+    """One mode (SOURCES/SOLVER/assembling_rhs.F90:5-189): rhs_in(G,6) -> vb_145(np,3), vb_236(np,3).
 
-    Host arrays; ``mode`` is only used by the reference's opt_tensor terms.
+    Host arrays.  ``opt_tensor`` (3,G,6) adds the weak divergence of the
+    tensor (its 1/r terms use ``mode``); ``opt_tensor_scaln_bdy`` (3*mes,6)
+    the boundary term (the mesh's boundary sides must be registered).
     """
     fdims(rhs_in), fdims(vb_145), fdims(vb_236)
-    check(lib().sfx_rhs_3x3(mesh.handle, int(mode), ptr(rhs_in), ptr(vb_145), ptr(vb_236)))
+    if opt_tensor is None and opt_tensor_scaln_bdy is None:
+        check(lib().sfx_rhs_3x3(mesh.handle, int(mode), ptr(rhs_in), ptr(vb_145), ptr(vb_236)))
+    else:
+        opt = lambda x: None if x is None else ptr(np.asfortranarray(x, dtype=np.float64))
+        t = None if opt_tensor is None else np.asfortranarray(opt_tensor, dtype=np.float64)
+        b = None if opt_tensor_scaln_bdy is None else np.asfortranarray(opt_tensor_scaln_bdy, dtype=np.float64)
+        check(lib().sfx_rhs_3x3_mode(mesh.handle, int(mode), ptr(rhs_in), None if t is None else ptr(t),
+                                     None if b is None else ptr(b), ptr(vb_145), ptr(vb_236)))
+    return vb_145, vb_236
+
+
+def rhs_3x3_bdy(mesh: Mesh, tensor_bdy, vb_145, vb_236, scale=1.0):
+    """Add rhs_3x3's boundary term (assembling_rhs.F90:129-185) for every mode
+    to the device arrays vb_145(np,3,m), vb_236(np,3,m): tensor_bdy(3*mes,6,m)."""
+    M = fdims(tensor_bdy)[2]
+    check(lib().sfx_rhs_3x3_bdy_dev(mesh.handle, M, ptr(tensor_bdy), ptr(vb_145), ptr(vb_236), float(scale),
+                                    stream_ptr()))
     return vb_145, vb_236
 
 
@@ -179,7 +210,7 @@ def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0, work_ele
 
 
 def rhs_ns_3x3(mesh: Mesh, list_mode, rotv_v, V1m, ff, P, P_inc, vb_145, vb_236, work_elem=None,
-               opt_tensor=None):
+               opt_tensor=None, penal_grid=None, imposed_vel=None, work_gauss=None):
     """Full explicit NS right-hand side assembled for every mode (device arrays).
 
     rhs_ns_gauss_3x3 without the penalty options
@@ -188,12 +219,26 @@ def rhs_ns_3x3(mesh: Mesh, list_mode, rotv_v, V1m, ff, P, P_inc, vb_145, vb_236,
     pressure and its increment on the velocity mesh, (np, 2, m);
     ``opt_tensor``: rhs_3x3's tensor at the Gauss points, (3, G, 6, m).  Any
     of ``rotv_v``, ``V1m``, ``ff``, ``P``, ``P_inc``, ``opt_tensor`` may be
-    None (zero).
+    None (zero).  ``penal_grid`` (2*m_max_pad-1, G), m_max_pad = 3m/2: the
+    penalty function on the grid angles at the Gauss points
+    (``if_ns_penalty``, computing_rhs_gauss.F90:103-141); ``imposed_vel``
+    (np, 6): 3*imposed_velocity/(2 dt) (``if_impose_vel_in_solids``);
+    ``work_gauss``: (2*G*6*m) buffer (allocated when omitted).
     """
     M = len(list_mode)
     lm = mesh.list_mode_device(list_mode)
     work_elem = _work_elem(mesh, M, work_elem)
     opt = lambda x: 0 if x is None else ptr(x)
+    if penal_grid is not None:
+        import torch
+        if work_gauss is None:
+            work_gauss = torch.empty(2 * mesh.n_gauss * 6 * M, dtype=torch.float64, device="cuda")
+        zero = [i for i, md in enumerate(list_mode) if int(md) == 0]
+        check(lib().sfx_rhs_ns_3x3_penal_dev(mesh.handle, lm.data_ptr(), M, zero[0] if zero else -1, opt(rotv_v),
+                                             opt(V1m), opt(ff), opt(P), opt(P_inc), ptr(penal_grid),
+                                             opt(imposed_vel), opt(opt_tensor), work_gauss.data_ptr(),
+                                             work_elem.data_ptr(), ptr(vb_145), ptr(vb_236), stream_ptr()))
+        return vb_145, vb_236
     check(lib().sfx_rhs_ns_3x3_dev(mesh.handle, lm.data_ptr(), M, opt(rotv_v), opt(V1m), opt(ff), opt(P),
                                    opt(P_inc), opt(opt_tensor), work_elem.data_ptr(), ptr(vb_145), ptr(vb_236),
                                    stream_ptr()))

@@ -219,6 +219,50 @@ extern "C" int emu_rhs(int which, const int32_t* jj, const double* rr, int64_t m
   return 0;
 }
 
+#include "../../sfemans_amd/csrc/sfx_bdy_setup.h"
+
+// Boundary term of rhs_3x3 added to vb145/vb236 (np, 3, M).
+extern "C" int emu_rhs_bdy(const int32_t* jjs, int64_t mes, const double* rr, int64_t np, int M, const double* tb,
+                           double* v145, double* v236, double scale) {
+  BdyTables t;
+  build_boundary_tables(jjs, mes, rr, np, t);
+  BdyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  double d[9], p[3];
+  p2_edge_tables(a.wws, d, p);
+  a.bptr = t.ptr.data(); a.bent = t.ent.data(); a.bnode = t.node.data(); a.bw = t.bw.data();
+  a.nb = (int64_t)t.node.size(); a.mes = mes; a.np = np; a.M = M; a.scale = scale;
+  a.tb = tb; a.v145 = v145; a.v236 = v236;
+  a.n_tb = 3 * mes * 6 * (int64_t)M; a.n_out = np * 3 * (int64_t)M; a.n_ent = (int64_t)t.ent.size();
+  for (int64_t g = 0; g < a.nb * (int64_t)M; ++g) rhs_bdy_body(a, g);
+  return 0;
+}
+
+// The Gauss values of the NS right-hand side (the first step of the penalty
+// branch): gout(G, 6, M) = ft + fp_inc + fs - rotv_v - (imposed velocity on imp_slot).
+extern "C" int emu_rhs_ns_gauss(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm,
+                                int M, const double* rotv, const double* V1m, const double* ff, const double* Pinc,
+                                const double* imp, int imp_slot, double* gout) {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> ent;
+  build_adjacency(jj, me, np, ptr, ent);
+  double tab[126];
+  p2_reference_tables(tab);
+  std::vector<double> wgt(7 * me);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.adj_ptr = ptr.data(); a.adj_ent = ent.data(); a.wgt = wgt.data(); a.tab = tab;
+  a.jj = jj; a.me = me; a.np = np; a.G = 7 * me; a.M = M; a.scale = 1.0; a.n_adj = (int64_t)ent.size();
+  a.rr = rr; a.lmode = lm;
+  a.rhs = rotv; a.v1m = V1m; a.fsrc = ff; a.pinc = Pinc; a.rsign = -1.0;
+  a.imp = imp; a.imp_slot = imp ? imp_slot : -1; a.isign = -1.0; a.gout = gout;
+  a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = np * 6 * (int64_t)M; a.n_pinc = np * 2 * (int64_t)M;
+  a.n_imp = np * 6; a.n_gout = a.G * 6 * (int64_t)M;
+  for (int64_t g = 0; g < elem_items(me, M); ++g) rhs_ns_elem_body<false, true>(a, g);
+  return 0;
+}
+
 extern "C" int emu_rhs_ns(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm, int M,
                           const double* rotv, const double* V1m, const double* ff, const double* P,
                           const double* Pinc, const double* tens, double* v145, double* v236) {
@@ -235,7 +279,7 @@ extern "C" int emu_rhs_ns(const int32_t* jj, const double* rr, int64_t me, int64
   a.jj = jj; a.me = me; a.np = np; a.G = 7 * me; a.M = M; a.scale = 1.0; a.n_adj = (int64_t)ent.size();
   std::vector<double> elo((size_t)36 * me * M + 1);
   a.rr = rr; a.lmode = lm;
-  a.rhs = rotv; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = Pinc; a.tens = tens;
+  a.rhs = rotv; a.v1m = V1m; a.fsrc = ff; a.pres = P; a.pinc = Pinc; a.tens = tens; a.rsign = -1.0;
   a.v145 = v145; a.v236 = v236; a.elo = elo.data();
   a.n_rhs = a.G * 6 * (int64_t)M; a.n_v1m = a.n_fsrc = np * 6 * (int64_t)M; a.n_pres = a.n_pinc = np * 2 * (int64_t)M;
   a.n_tens = 3 * a.n_rhs;

@@ -45,6 +45,10 @@ def _load():
     L.emu_rhs.restype = C.c_int
     L.emu_rhs_ns.argtypes = [vp, vp, i64, i64, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
     L.emu_rhs_ns.restype = C.c_int
+    L.emu_rhs_ns_gauss.argtypes = [vp, vp, i64, i64, vp, i32, vp, vp, vp, vp, vp, i32, vp]
+    L.emu_rhs_ns_gauss.restype = C.c_int
+    L.emu_rhs_bdy.argtypes = [vp, i64, vp, i64, i32, vp, vp, vp, d]
+    L.emu_rhs_bdy.restype = C.c_int
     L.emu_adjoint.argtypes = [vp, vp, i64, i64, vp, vp, vp, i32, vp, vp, d, vp]
     L.emu_adjoint.restype = C.c_int
     L.emu_batch.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]

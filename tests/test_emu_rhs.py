@@ -83,3 +83,48 @@ def test_rhs_ns_3x3_with_tensor(nx, ny, modes, with_all):
                                   z2 if Pinc is None else Pinc, rotv)
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g, lm, T)
     assert rel(arrs[-2], e1) <= 1e-12 and rel(arrs[-1], e2) <= 1e-12
+
+
+@pytest.mark.parametrize("mesh_kind,M", [("rect", 1), ("rect", 5), ("sector", 3)])
+def test_rhs_3x3_boundary_term(mesh_kind, M):
+    """opt_tensor_scaln_bdy (assembling_rhs.F90:129-185) added after the volume
+    sums, vs the oracle, on a rectangle (sides on the axis skipped) and on a
+    sector with curved sides."""
+    from sfemans_amd.mesh import boundary_sides, sector_p2
+    m = rectangle_p2(3, 4) if mesh_kind == "rect" else sector_p2(3, 5)
+    jjs = boundary_sides(m.jj)
+    rng = np.random.default_rng(11)
+    tb = np.asfortranarray(rng.standard_normal((3 * jjs.shape[1], 6, M)))
+    base1 = np.asfortranarray(rng.standard_normal((m.np, 3, M)))
+    base2 = np.asfortranarray(rng.standard_normal((m.np, 3, M)))
+    rc, arrs = emu_runner.call("emu_rhs_bdy", jjs, jjs.shape[1], m.rr, m.np, M, tb, base1.copy(order="F"),
+                               base2.copy(order="F"), 2.0)
+    assert rc == 0
+    e1, e2 = assembly.rhs_3x3_bdy(jjs, m.rr, tb)
+    assert rel(arrs[-2], base1 + 2.0 * e1) <= 1e-13 and rel(arrs[-1], base2 + 2.0 * e2) <= 1e-13
+    touched = np.zeros(m.np, bool)
+    touched[jjs.ravel() - 1] = True
+    assert np.all(arrs[-2][~touched] == base1[~touched])
+
+
+@pytest.mark.parametrize("modes,slot", [([0, 1, 2], 0), ([3, 0, 5, 1], 1), ([1, 2], -1)])
+def test_rhs_ns_gauss_values_with_imposed_velocity(modes, slot):
+    """The first step of rhs_ns_gauss_3x3's penalty branch: the Gauss values
+    ft + fp_inc + fs - rotv_v, minus the imposed velocity on the mode-0 slot
+    (computing_rhs_gauss.F90:54-121), vs the oracle."""
+    m = rectangle_p2(3, 2)
+    M = len(modes)
+    lm = np.asarray(modes, dtype=np.int32)
+    rotv, V1m, ff, P, Pinc = _ns_inputs(m, M, 13)
+    imp = np.asfortranarray(np.random.default_rng(14).standard_normal((m.np, 6)))
+    out = np.zeros((m.n_gauss, 6, M), order="F")
+    rc, arrs = emu_runner.call("emu_rhs_ns_gauss", m.jj, m.rr, m.me, m.np, lm, M, rotv, V1m, ff, Pinc, imp, slot, out)
+    assert rc == 0
+    z2 = np.zeros_like(P)
+    exp = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, V1m, ff, z2, Pinc, rotv)
+    if slot >= 0:
+        from oracle.gauss_p2 import gauss_tables
+        ww = gauss_tables(m.jj, m.rr)[0]
+        ig = np.einsum("enk,nl->elk", imp[(m.jj - 1).T], ww).reshape(m.n_gauss, 6)
+        exp[:, :, slot] -= ig
+    assert rel(arrs[-1], exp) <= 1e-13

@@ -273,3 +273,81 @@ def test_tangent_matches_bilinear_form_and_central_difference(M):
         NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, np.asfortranarray(U + eps * D), fp)
         NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, np.asfortranarray(U - eps * D), fm)
         assert rel((fp - fm) / (2 * eps), out) <= tol
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_rhs_3x3_all_options_one_mode(curved):
+    """rhs_3x3 with opt_tensor and opt_tensor_scaln_bdy (assembling_rhs.F90:5-189),
+    one mode through the host entry, vs the oracle; and the device boundary
+    term for every mode added after the fused NS right-hand side."""
+    from oracle import assembly
+    from sfemans_amd.mesh import boundary_sides, sector_p2
+    m = sector_p2(4, 6) if curved else rectangle_p2(5, 4)
+    mesh = NL.Mesh.from_p2(m, boundary=True)
+    jjs = boundary_sides(m.jj)
+    mes = jjs.shape[1]
+    rng = np.random.default_rng(12)
+    F = lambda *s: np.asfortranarray(rng.standard_normal(s))
+    mode = 3
+    f, T, tb = F(m.n_gauss, 6), F(3, m.n_gauss, 6), F(3 * mes, 6)
+    v1 = np.zeros((m.np, 3), order="F"); v2 = np.zeros((m.np, 3), order="F")
+    NL.rhs_3x3(mesh, mode, f, v1, v2, opt_tensor=T, opt_tensor_scaln_bdy=tb)
+    a1, a2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, f[:, :, None], np.array([mode]), T[..., None])
+    b1, b2 = assembly.rhs_3x3_bdy(jjs, m.rr, tb[:, :, None])
+    assert rel(v1, (a1 + b1)[:, :, 0]) <= TOL and rel(v2, (a2 + b2)[:, :, 0]) <= TOL
+    # the boundary term alone and the tensor alone
+    NL.rhs_3x3(mesh, mode, np.zeros_like(f), v1, v2, opt_tensor_scaln_bdy=tb)
+    assert rel(v1, b1[:, :, 0]) <= TOL and rel(v2, b2[:, :, 0]) <= TOL
+    NL.rhs_3x3(mesh, mode, f, v1, v2, opt_tensor=T)
+    assert rel(v1, a1[:, :, 0]) <= TOL and rel(v2, a2[:, :, 0]) <= TOL
+    # all modes on the device: the NS right-hand side with the tensor, then the boundary term
+    modes = [0, 1, 2, 5, 8]
+    M = len(modes)
+    rotv, T5, tb5 = F(m.n_gauss, 6, M), F(3, m.n_gauss, 6, M), F(3 * mes, 6, M)
+    d1, d2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_ns_3x3(mesh, modes, fortran_tensor(rotv), None, None, None, None, d1, d2, opt_tensor=fortran_tensor(T5))
+    NL.rhs_3x3_bdy(mesh, fortran_tensor(tb5), d1, d2)
+    torch.cuda.synchronize()
+    a1, a2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, -rotv, np.asarray(modes), T5)
+    b1, b2 = assembly.rhs_3x3_bdy(jjs, m.rr, tb5)
+    assert rel(d1.cpu().numpy(), a1 + b1) <= TOL and rel(d2.cpu().numpy(), a2 + b2) <= TOL
+
+
+def test_rhs_3x3_boundary_needs_sides():
+    from sfemans_amd._lib import SfxError
+    m = rectangle_p2(2, 2)
+    mesh = NL.Mesh.from_p2(m)
+    z = np.zeros((m.n_gauss, 6), order="F")
+    v = np.zeros((m.np, 3), order="F")
+    with pytest.raises(SfxError) as e:
+        NL.rhs_3x3(mesh, 0, z, v, v.copy(order="F"), opt_tensor_scaln_bdy=np.zeros((24, 6), order="F"))
+    assert e.value.code == 1
+
+
+@pytest.mark.parametrize("modes,tensor,imposed", [([0, 1, 2, 3], False, True), ([2, 0, 5], True, True),
+                                                  ([1, 3], True, False), (list(range(16)), False, True)])
+def test_rhs_ns_3x3_penalty(modes, tensor, imposed):
+    """rhs_ns_gauss_3x3 with if_ns_penalty and if_impose_vel_in_solids
+    (computing_rhs_gauss.F90:103-141), assembled by rhs_3x3, vs the oracle:
+    relative error <= 1e-12."""
+    from oracle import assembly
+    m = rectangle_p2(6, 5)
+    mesh = NL.Mesh.from_p2(m)
+    M = len(modes)
+    N = 2 * (3 * M // 2) - 1
+    print(f"case rhs_ns_3x3 penalty modes={modes} tensor={tensor} imposed={imposed}", flush=True)
+    rng = np.random.default_rng(15)
+    F = lambda *s: np.asfortranarray(rng.standard_normal(s))
+    rotv, V1m, ff, P, Pinc = F(m.n_gauss, 6, M), F(m.np, 6, M), F(m.np, 6, M), F(m.np, 2, M), F(m.np, 2, M)
+    T = F(3, m.n_gauss, 6, M) if tensor else None
+    penal = np.asfortranarray(rng.uniform(0.0, 1.0, (N, m.n_gauss)))
+    imp = F(m.np, 6) if imposed else None
+    d = lambda x: None if x is None else fortran_tensor(x)
+    v1, v2 = fortran_empty((m.np, 3, M)), fortran_empty((m.np, 3, M))
+    NL.rhs_ns_3x3(mesh, modes, d(rotv), d(V1m), d(ff), d(P), d(Pinc), v1, v2, opt_tensor=d(T),
+                  penal_grid=d(penal), imposed_vel=d(imp))
+    torch.cuda.synchronize()
+    lm = np.asarray(modes)
+    g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, V1m, ff, P, Pinc, rotv, penal_grid=penal, imposed_vel=imp)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g, lm, T)
+    assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
