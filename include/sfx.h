@@ -441,6 +441,56 @@ int sfx_nl_adjoint_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_
                            double* work_elem, double* dUbar, double* dHbar, double* dBbar, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * The inner product of SFEMaNS fields and the adjoint in it.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces dot_product_SF (SOURCES/SOLVER/tn_axi.F90:9-37; dot_product_champ,
+ * fem_tn_ns_mhd.F90:92-135; the element loop of fem_tn_axi.F90:8-48), one rank:
+ * 2 pi SUM_k w_k SUM_c SUM_(e,l) SUM(v ww) SUM(w ww) rj ray over the nodal
+ * arrays v, w (np, nb_field, m_max_c), w_k = 1 if list_mode(k) = 0, else 1/2.
+ * LightKrylov's dot on SFEMaNS fields is this product
+ * (SOURCES/TOOLS/def_type_field.F90:176-198).  The sum is formed in a fixed
+ * order (deterministic).  _dev: result_dev is one device double, work holds
+ * sfx_dot_product_sf_work(...) doubles; the host form copies v and w. */
+int64_t sfx_dot_product_sf_work(sfx_mesh_t mesh, int32_t m_max_c, int32_t nb_field);
+int sfx_dot_product_sf_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, int32_t nb_field,
+                           const double* v, const double* w, double* result_dev, double* work, void* stream);
+int sfx_dot_product_sf(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, int32_t nb_field, const double* v,
+                       const double* w, double* result);
+
+/* The r-weighted P2 mass matrix M_ij = SUM_(e,l) ww(i,l) ww(j,l) rj ray, the
+ * Gram matrix of dot_product_SF per column (no counterpart as a routine in
+ * the reference, which solves with PETSc): y = M x for ncols nodal columns
+ * (np, ncols); work_elem: 6*me*ncols doubles. */
+int sfx_mass_apply_dev(sfx_mesh_t mesh, int32_t ncols, const double* x, double* y, double* work_elem, void* stream);
+/* x = M^{-1} b for ncols columns: Jacobi-preconditioned conjugate gradients on
+ * every column at once, until (r, D^{-1} r) <= rtol^2 (r0, D^{-1} r0) in every
+ * column (rtol <= 0: 1e-14; max_iter <= 0: 1000), checked every 8 iterations
+ * (the call synchronises the stream).  work: sfx_mass_solve_work(mesh, ncols)
+ * doubles; iters (nullable) receives the iteration count.  SFX_ERR_ARG if
+ * not converged. */
+int64_t sfx_mass_solve_work(sfx_mesh_t mesh, int32_t ncols);
+int sfx_mass_solve_dev(sfx_mesh_t mesh, int32_t ncols, const double* b, double* x, double rtol, int32_t max_iter,
+                       double* work, int32_t* iters, void* stream);
+
+/* The linearized convective term as an operator on nodal fields, and its
+ * adjoint in dot_product_SF, the inner product LightKrylov uses (no
+ * counterpart in the reference, whose LightKrylov operator binds rmatvec to
+ * the forward map, SOURCES/LIGHTKRYLOV_OBJECTS/exponential_propagator.F90:61-62):
+ *   A dU = M^{-1} R T_U dU   (the L2(r dr dz) projection onto P2, mode by mode
+ *                             and column by column, of sfx_nl_tangent; R f =
+ *                             SUM_l ww(i,l) f(l) rj ray, qs_00_gauss's right-hand side)
+ *   A* y = M^{-1} w^{-1} T_U^T R^T (w y)   so that <A x, y>_SF = <x, A* y>_SF,
+ * with T_U^T the Euclidean transpose (sfx_nl_adjoint_dev) and w the mode
+ * weights.  All arrays (np, 6, m) on the device; rtol as sfx_mass_solve_dev;
+ * work: sfx_nl_proj_work(mesh, m_max_c) doubles.  Both synchronise the stream. */
+int64_t sfx_nl_proj_work(sfx_mesh_t mesh, int32_t m_max_c);
+int sfx_nl_tangent_proj_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                            const double* dU, double* out, double rtol, double* work, void* stream);
+int sfx_nl_adjoint_sf_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* U,
+                          const double* ybar, double* out, double rtol, double* work, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Right-hand-side assembly into dense local vectors.  On one rank the PETSc
  * index of component ki of node i is (ki-1)*np + i
  * (SOURCES/INITIALISATION/st_matrix.F90:101), so vb_145/vb_236 are (np,3)
@@ -517,7 +567,8 @@ int sfx_rhs_ns_3x3_penal_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int3
 int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, const double* ff_gauss,
                     double* vect);
 /* ncols independent right-hand sides: ff(np,ncols), ff_gauss(G,ncols) -> vect(np,ncols).
- * work_elem is a caller-provided device buffer of 6*me*ncols doubles. */
+ * work_elem is a caller-provided device buffer of 6*me*ncols doubles.  A null
+ * heat_capa counts as 1, a null ff or ff_gauss as zero. */
 int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa, const double* ff,
                         const double* ff_gauss, double* work_elem, double* vect, void* stream);
 

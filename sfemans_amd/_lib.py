@@ -119,6 +119,15 @@ SIGNATURES = {
                                  _vp, _vp],
     "sfx_rhs_ns_3x3_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],
+    "sfx_dot_product_sf_work": [_vp, _i32, _i32],
+    "sfx_dot_product_sf_dev": [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
+    "sfx_dot_product_sf": [_vp, _vp, _i32, _i32, _vp, _vp, _vp],
+    "sfx_mass_apply_dev": [_vp, _i32, _vp, _vp, _vp, _vp],
+    "sfx_mass_solve_work": [_vp, _i32],
+    "sfx_mass_solve_dev": [_vp, _i32, _vp, _vp, _d, _i32, _vp, _vp, _vp],
+    "sfx_nl_proj_work": [_vp, _i32],
+    "sfx_nl_tangent_proj_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp],
+    "sfx_nl_adjoint_sf_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp],
     "sfx_qs_00_gauss_dev": [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_nl_rhs_mhd_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp],
     "sfx_batch_create": [_vp, _i32, _vp],
@@ -129,7 +138,8 @@ SIGNATURES = {
     "sfx_batch_info": [_vp, _vp, _vp, _vp, _vp, _vp],
     "sfx_batch_destroy": [_vp],
 }
-_RESTYPE = {"sfx_last_error": C.c_char_p, "sfx_version": C.c_char_p}
+_RESTYPE = {"sfx_last_error": C.c_char_p, "sfx_version": C.c_char_p, "sfx_dot_product_sf_work": C.c_int64,
+            "sfx_mass_solve_work": C.c_int64, "sfx_nl_proj_work": C.c_int64}
 
 _lib = None
 

@@ -18,6 +18,7 @@ struct sfx_mesh_s {
   int32_t* adj_ent = nullptr;   // entries e*6 + local node, element order
   int64_t n_adj = 0;
   double* wgt = nullptr;        // (7, me) rj * ray
+  double* mdinv = nullptr;      // (np) 1 / diagonal of the r-weighted mass matrix
   // boundary sides (mesh%jjs) for the boundary term of rhs_3x3; mes = -1: none registered
   int64_t mes = -1, nb = 0, n_bent = 0;
   int64_t* bptr = nullptr;      // (nb+1) CSR over boundary nodes

@@ -52,6 +52,8 @@ struct GaussArgs {
 };
 
 struct RhsArgs;
+struct InnerArgs;
+struct PcgArgs;
 struct BdyArgs;
 struct AdjArgs;
 struct GatherArgs;
@@ -62,6 +64,12 @@ hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_rhs_bdy(const BdyArgs& a, hipStream_t st);
 hipError_t launch_rhs_ns_gauss(const RhsArgs& a, hipStream_t st);
+hipError_t launch_gauss_values(const InnerArgs& a, hipStream_t st);
+int64_t dotsf_blocks(int64_t me, int ncols);
+hipError_t launch_dotsf(const InnerArgs& a, double* result, hipStream_t st);
+hipError_t launch_coldot(const double* A, const double* B, int64_t np, int ncols, double* out, hipStream_t st);
+hipError_t launch_pcg(int which, const PcgArgs& a, hipStream_t st);
+hipError_t launch_scale_cols(const InnerArgs& a, hipStream_t st);
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st);
 hipError_t launch_fftprod(int op, const FftProdArgs& a, hipStream_t st);
 // Exact-grid products (sfx_exact_body.h): angle pass, then r2c, or the maximum into max_out.

@@ -34,6 +34,13 @@ static int mesh_upload(const int32_t* jj, const double* rr, int64_t me, int64_t 
     for (int n = 0; n < 6; ++n) ent[fill[jj[6 * e + n] - 1]++] = (int32_t)(6 * e + n);
   std::vector<double> wgt(7 * me);
   p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  // diagonal of the r-weighted mass matrix, SUM ww(i,l)^2 rj ray (the Jacobi
+  // preconditioner of the mass solves)
+  std::vector<double> mdinv(np, 0.0);
+  for (int64_t e = 0; e < me; ++e)
+    for (int n = 0; n < 6; ++n)
+      for (int l = 0; l < 7; ++l) mdinv[jj[6 * e + n] - 1] += tab[n + 6 * l] * tab[n + 6 * l] * wgt[7 * e + l];
+  for (int64_t i = 0; i < np; ++i) mdinv[i] = mdinv[i] > 0.0 ? 1.0 / mdinv[i] : 0.0;
   m->n_adj = (int64_t)ent.size();
   hipError_t e;
   if ((e = hipMalloc(&m->wgt, 7 * me * sizeof(double) + 8)) != hipSuccess ||
@@ -41,7 +48,8 @@ static int mesh_upload(const int32_t* jj, const double* rr, int64_t me, int64_t 
       (e = hipMalloc(&m->rr, 2 * np * sizeof(double) + 8)) != hipSuccess ||
       (e = hipMalloc(&m->tab, 126 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&m->adj_ptr, (np + 1) * sizeof(int64_t))) != hipSuccess ||
-      (e = hipMalloc(&m->adj_ent, ent.size() * sizeof(int32_t) + 4)) != hipSuccess) {
+      (e = hipMalloc(&m->adj_ent, ent.size() * sizeof(int32_t) + 4)) != hipSuccess ||
+      (e = hipMalloc(&m->mdinv, np * sizeof(double) + 8)) != hipSuccess) {
     sfx_mesh_destroy(m);
     return fail(SFX_ERR_NOMEM, "sfx_mesh_create: hipMalloc: %s", hipGetErrorString(e));
   }
@@ -50,7 +58,8 @@ static int mesh_upload(const int32_t* jj, const double* rr, int64_t me, int64_t 
       (e = hipMemcpy(m->tab, tab, sizeof(tab), hipMemcpyHostToDevice)) != hipSuccess ||
       (me && (e = hipMemcpy(m->wgt, wgt.data(), 7 * me * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) ||
       (e = hipMemcpy(m->adj_ptr, ptr.data(), (np + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
-      (ent.size() && (e = hipMemcpy(m->adj_ent, ent.data(), ent.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)) {
+      (ent.size() && (e = hipMemcpy(m->adj_ent, ent.data(), ent.size() * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess) ||
+      (np && (e = hipMemcpy(m->mdinv, mdinv.data(), np * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)) {
     sfx_mesh_destroy(m);
     return hip_fail(e, "sfx_mesh_create: upload");
   }
@@ -95,6 +104,7 @@ int sfx_mesh_destroy(sfx_mesh_t m) {
   (void)hipFree(m->adj_ptr);
   (void)hipFree(m->adj_ent);
   (void)hipFree(m->wgt);
+  (void)hipFree(m->mdinv);
   (void)hipFree(m->bptr);
   (void)hipFree(m->bent);
   (void)hipFree(m->bnode);

@@ -336,10 +336,12 @@ SFX_DEV void qs00_elem_body(const RhsArgs& a, int64_t gid) {
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     const int64_t fi = (int64_t)a.jj[6 * e + j] - 1 + a.np * (int64_t)c;
-    SFX_CHECK("ff load", fi, a.n_ff);
-    ffl[j] = a.ff[fi];
+    SFX_CHECK("ff load", fi, a.ff ? a.n_ff : 1 << 30);
+    ffl[j] = a.ff ? a.ff[fi] : 0.0;
   }
-  const double hc = a.heat_capa[e];
+  // a null heat capacity is 1; a null nodal or Gauss source is zero (the mass
+  // matrix applied to ff, or the Galerkin right-hand side of ff_gauss alone)
+  const double hc = a.heat_capa ? a.heat_capa[e] : 1.0;
   double fl[7];
 #pragma unroll
   for (int l = 0; l < 7; ++l) {
@@ -347,8 +349,8 @@ SFX_DEV void qs00_elem_body(const RhsArgs& a, int64_t gid) {
 #pragma unroll
     for (int j = 0; j < 6; ++j) s = fma(ffl[j], a.tab[j + 6 * l], s);
     const int64_t gi = 7 * e + l + a.G * (int64_t)c;
-    SFX_CHECK("ffg load", gi, a.n_ffg);
-    fl[l] = (hc * s + a.ffg[gi]) * a.wgt[7 * e + l];
+    SFX_CHECK("ffg load", gi, a.ffg ? a.n_ffg : 1 << 30);
+    fl[l] = (hc * s + (a.ffg ? a.ffg[gi] : 0.0)) * a.wgt[7 * e + l];
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) {

@@ -26,7 +26,7 @@ def build_emulator() -> str:
     if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(d) for d in deps):
         return LIB
     os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
+    tmp = LIB + f".tmp{os.getpid()}"      # one per process: parallel test workers may build at once
     subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fPIC", "-shared", "-pthread",
                     "-o", tmp, SRC], check=True)
     os.replace(tmp, LIB)
