@@ -9,9 +9,9 @@ __global__ void __launch_bounds__(256) gauss_values_kernel(InnerArgs a) {
   gauss_values_body(a, xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
 }
 
-__global__ void __launch_bounds__(INNER_THREADS) dotsf_kernel(InnerArgs a) {
+__global__ void __launch_bounds__(INNER_THREADS) dotsf_kernel(InnerArgs a, int64_t nitems) {
   __shared__ double sh[INNER_THREADS];
-  dotsf_block(a, blockIdx.x, threadIdx.x, sh);
+  dotsf_block(a, blockIdx.x, gridDim.x, nitems, threadIdx.x, sh);
 }
 
 __global__ void __launch_bounds__(INNER_THREADS) sum_partials_kernel(const double* partial, int64_t n, double scale,
@@ -48,12 +48,14 @@ hipError_t launch_gauss_values(const InnerArgs& a, hipStream_t st) {
 }
 
 int64_t dotsf_blocks(int64_t me, int ncols) {
-  return me > 0 && ncols > 0 ? (elem_items(me, ncols) + INNER_THREADS - 1) / INNER_THREADS : 0;
+  const int64_t nb = me > 0 && ncols > 0 ? (elem_items(me, ncols) + INNER_THREADS - 1) / INNER_THREADS : 0;
+  return nb < DOTSF_MAX_BLOCKS ? nb : DOTSF_MAX_BLOCKS;
 }
 
 hipError_t launch_dotsf(const InnerArgs& a, double* result, hipStream_t st) {
   const int64_t nb = dotsf_blocks(a.me, a.ncols);
-  if (nb > 0) hipLaunchKernelGGL(dotsf_kernel, dim3((unsigned)nb), dim3(INNER_THREADS), 0, st, a);
+  const int64_t nitems = a.me > 0 && a.ncols > 0 ? elem_items(a.me, a.ncols) : 0;
+  if (nb > 0) hipLaunchKernelGGL(dotsf_kernel, dim3((unsigned)nb), dim3(INNER_THREADS), 0, st, a, nitems);
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(INNER_THREADS), 0, st, (const double*)a.partial, nb,
                      2.0 * 3.14159265358979323846, result);
   return hipGetLastError();

@@ -72,13 +72,30 @@ SFX_DEV double block_sum(double* sh, int tid) {
   return sh[0];
 }
 
-// Work-group bid of the dot product: its threads take the (element, column)
-// items bid*INNER_THREADS + tid, each SUM_l v_g w_g rj ray times the mode
-// weight; the group's sum goes to partial[bid].
-SFX_DEV void dotsf_block(const InnerArgs& a, int64_t bid, int tid, double* sh) {
+// At most this many work-groups (and partial sums) per dot product.
+constexpr int DOTSF_MAX_BLOCKS = 2048;
+
+// Work-group bid of nblk: its threads take the (element, column) items
+// (bid + j*nblk)*INNER_THREADS + tid for j = 0, 1, ..., each adding
+// SUM_l v_g w_g rj ray times the mode weight in that order; the group's sum
+// goes to partial[bid].  The split depends only on the sizes, so the sum is
+// formed in the same order on every call.
+SFX_DEV double dotsf_item(const InnerArgs& a, int64_t gid);
+SFX_DEV void dotsf_block(const InnerArgs& a, int64_t bid, int64_t nblk, int64_t nitems, int tid, double* sh) {
+  double t = 0;
+  for (int64_t g = bid * INNER_THREADS + tid; g < nitems; g += nblk * INNER_THREADS) t += dotsf_item(a, g);
+  sh[tid] = t;
+  const double s = block_sum(sh, tid);
+  if (tid == 0) {
+    SFX_CHECK("partial store", bid, a.n_part);
+    a.partial[bid] = s;
+  }
+}
+
+SFX_DEV double dotsf_item(const InnerArgs& a, int64_t gid) {
   int64_t e;
   int c;
-  elem_item(bid * INNER_THREADS + tid, a.ncols, e, c);
+  elem_item(gid, a.ncols, e, c);
   double t = 0;
   if (e < a.me && c < a.ncols) {
     double vn[6], wn[6];
@@ -101,12 +118,7 @@ SFX_DEV void dotsf_block(const InnerArgs& a, int64_t bid, int tid, double* sh) {
     }
     t *= col_scale(a, c);
   }
-  sh[tid] = t;
-  const double s = block_sum(sh, tid);
-  if (tid == 0) {
-    SFX_CHECK("partial store", bid, a.n_part);
-    a.partial[bid] = s;
-  }
+  return t;
 }
 
 // One work-group: out[0] = scale * SUM of the n partials, summed in a fixed order.
