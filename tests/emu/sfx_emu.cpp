@@ -692,3 +692,74 @@ extern "C" int emu_exact_emom(int64_t np, int M, int mp, int lg, const double* V
     for (int j = 0; j < N; ++j) exact_emom_out_body(a, p, j);
   return 0;
 }
+
+// ---- inner product (dot_product_SF), Gauss values, column dot products
+#include "../../sfemans_amd/csrc/sfx_inner_body.h"
+
+static InnerArgs emu_inner_args(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm,
+                                int M, int nc, std::vector<double>& wgt, double* tab) {
+  p2_reference_tables(tab);
+  wgt.assign(7 * me + 1, 0.0);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  InnerArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.jj = jj; a.tab = tab; a.wgt = wgt.data(); a.lmode = lm;
+  a.nc = nc; a.ncols = nc * M; a.me = me; a.np = np; a.G = 7 * me;
+  a.n_z = np * (int64_t)a.ncols;
+  return a;
+}
+
+// dot_product_SF with the partial sums split over nblk work-groups (nblk >= 1).
+extern "C" int emu_dotsf(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm, int M,
+                         int nc, const double* v, const double* w, int64_t nblk, double* out) {
+  double tab[126];
+  std::vector<double> wgt;
+  InnerArgs a = emu_inner_args(jj, rr, me, np, lm, M, nc, wgt, tab);
+  a.w0 = 1.0; a.wm = 0.5; a.v = v; a.w = w;
+  const int64_t nitems = me > 0 && a.ncols > 0 ? elem_items(me, a.ncols) : 0;
+  std::vector<double> part(nblk, 1e300);     // poisoned: a partial left unwritten shows up
+  a.partial = part.data(); a.n_part = nblk;
+  run_grid(nblk, INNER_THREADS, (INNER_THREADS + 1) / 2, [&](int t, int64_t b, double2* sm) {
+    dotsf_block(a, b, nblk, nitems, t, reinterpret_cast<double*>(sm));
+  });
+  run_grid(1, INNER_THREADS, (INNER_THREADS + 1) / 2, [&](int t, int64_t, double2* sm) {
+    sum_partials_block(part.data(), nblk, 2.0 * 3.14159265358979323846, out, t, reinterpret_cast<double*>(sm));
+  });
+  return 0;
+}
+
+// Gauss values of nodal columns z (np, nc*M) -> out (7*me, nc*M), scaled
+// per column by w0 (mode 0) or wm, times rj*ray when weighted.
+extern "C" int emu_gauss_values(const int32_t* jj, const double* rr, int64_t me, int64_t np, const int32_t* lm, int M,
+                                int nc, int weighted, double w0, double wm, const double* z, double* out) {
+  double tab[126];
+  std::vector<double> wgt;
+  InnerArgs a = emu_inner_args(jj, rr, me, np, lm, M, nc, wgt, tab);
+  a.w0 = w0; a.wm = wm; a.weighted = weighted; a.z = z; a.out = out;
+  a.n_out = a.G * (int64_t)a.ncols;
+  const int64_t n = me > 0 && a.ncols > 0 ? elem_items(me, a.ncols) : 0;
+  for (int64_t g = 0; g < n; ++g) gauss_values_body(a, g);
+  return 0;
+}
+
+// out[c] = SUM_i A(i,c) B(i,c), one work-group per column.
+extern "C" int emu_coldot(const double* A, const double* B, int64_t np, int ncols, double* out) {
+  run_grid(ncols, INNER_THREADS, (INNER_THREADS + 1) / 2, [&](int t, int64_t c, double2* sm) {
+    coldot_block(A, B, np, out, c, t, reinterpret_cast<double*>(sm));
+  });
+  return 0;
+}
+
+// Sanitizer self-check of the test harness (CPU only): which = 1 reads one
+// element past a heap array, which = 2 overflows a signed int, 0 does
+// neither.  Under the sanitizer build the first two must stop the process.
+extern "C" int emu_sanitizer_probe(int which, int n) {
+  std::vector<int> v(n > 0 ? n : 1, 1);
+  volatile int s = v[0];
+  if (which == 1) s = s + v.data()[v.size()];
+  if (which == 2) {
+    volatile int big = 2147483647 - 1;
+    s = big + n;
+  }
+  return s;
+}

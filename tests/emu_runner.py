@@ -4,13 +4,24 @@ The emulator (tests/emu/sfx_emu.cpp) compiles the same kernel-body headers as
 the GPU library with SFX_EMULATE, runs each work-group as std::threads and
 checks every computed index against its buffer.  A violation aborts, so each
 call runs in a child process.
+
+Sanitizer build: with SFX_EMU_SANITIZE=1 in the environment (or
+``call(..., sanitize=True)``) the emulator is built with AddressSanitizer and
+UndefinedBehaviorSanitizer (host code only; g++, no GPU code) and each call
+runs in a fresh Python child with the ASan runtime preloaded; any report
+ends the child with a non-zero status, which ``call`` turns into an error.
+``SFX_EMU_SANITIZE=1 python -m pytest tests -m "not gpu" -k emu`` runs the
+emulator tests under both sanitizers.
 """
 from __future__ import annotations
 
 import ctypes as C
 import multiprocessing as mp
 import os
+import pickle
 import subprocess
+import sys
+import tempfile
 
 import numpy as np
 
@@ -18,23 +29,30 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "emu", "sfx_emu.cpp")
 OUT_DIR = os.path.join(HERE, "emu", "_build")
 LIB = os.path.join(OUT_DIR, "libsfx_emu.so")
+LIB_SAN = os.path.join(OUT_DIR, "libsfx_emu_asan_ubsan.so")
 CSRC = os.path.join(os.path.dirname(HERE), "sfemans_amd", "csrc")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
 
 
-def build_emulator() -> str:
+def sanitize_default() -> bool:
+    return os.environ.get("SFX_EMU_SANITIZE", "") == "1"
+
+
+def build_emulator(sanitize: bool = False) -> str:
+    lib = LIB_SAN if sanitize else LIB
     deps = [SRC] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
-    if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(d) for d in deps):
-        return LIB
+    if os.path.exists(lib) and os.path.getmtime(lib) >= max(os.path.getmtime(d) for d in deps):
+        return lib
     os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = LIB + f".tmp{os.getpid()}"      # one per process: parallel test workers may build at once
-    subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fPIC", "-shared", "-pthread",
-                    "-o", tmp, SRC], check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    tmp = lib + f".tmp{os.getpid()}"      # one per process: parallel test workers may build at once
+    subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fPIC", "-shared", "-pthread"]
+                   + (SAN_FLAGS if sanitize else []) + ["-o", tmp, SRC], check=True)
+    os.replace(tmp, lib)
+    return lib
 
 
-def _load():
-    L = C.CDLL(LIB)
+def _load(path: str = LIB):
+    L = C.CDLL(path)
     vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
     L.emu_fftprod.argtypes = [i32, vp, vp, vp, i64, i32, i32, i32, i32, i32]
     L.emu_fftprod.restype = C.c_int
@@ -59,24 +77,61 @@ def _load():
     L.emu_exact_evisc.restype = C.c_int
     L.emu_exact_emom.argtypes = [i64, i32, i32, i32, vp, vp, vp, C.c_double, C.c_double, vp, vp]
     L.emu_exact_emom.restype = C.c_int
+    L.emu_dotsf.argtypes = [vp, vp, i64, i64, vp, i32, i32, vp, vp, i64, vp]
+    L.emu_dotsf.restype = C.c_int
+    L.emu_gauss_values.argtypes = [vp, vp, i64, i64, vp, i32, i32, i32, d, d, vp, vp]
+    L.emu_gauss_values.restype = C.c_int
+    L.emu_coldot.argtypes = [vp, vp, i64, i32, vp]
+    L.emu_coldot.restype = C.c_int
+    L.emu_sanitizer_probe.argtypes = [i32, i32]
+    L.emu_sanitizer_probe.restype = C.c_int
     return L
+
+
+def _run(L, fname, args):
+    arrays = [np.asfortranarray(a) if isinstance(a, np.ndarray) else a for a in args]
+    ptrs = [a.ctypes.data if isinstance(a, np.ndarray) else a for a in arrays]
+    ptrs = [None if (a is None) else p for a, p in zip(arrays, ptrs)]
+    rc = getattr(L, fname)(*ptrs)
+    return rc, [a for a in arrays if isinstance(a, np.ndarray)]
 
 
 def _child(conn, fname, args):
     try:
-        L = _load()
-        arrays = [np.asfortranarray(a) if isinstance(a, np.ndarray) else a for a in args]
-        ptrs = [a.ctypes.data if isinstance(a, np.ndarray) else a for a in arrays]
-        ptrs = [None if (a is None) else p for a, p in zip(arrays, ptrs)]
-        rc = getattr(L, fname)(*ptrs)
-        conn.send((rc, [a for a in arrays if isinstance(a, np.ndarray)]))
+        conn.send(_run(_load(), fname, args))
     except Exception as e:  # pragma: no cover
         conn.send((repr(e), None))
     conn.close()
 
 
-def call(fname: str, *args, timeout: float = 300.0):
+def _call_sanitized(fname: str, args, timeout: float):
+    """One call of the sanitizer build in a fresh interpreter with the ASan
+    runtime preloaded (it must come first in the process)."""
+    lib = build_emulator(sanitize=True)
+    asan = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True,
+                          check=True).stdout.strip()
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = asan
+    env["ASAN_OPTIONS"] = "detect_leaks=0:halt_on_error=1:detect_stack_use_after_return=0"
+    env["UBSAN_OPTIONS"] = "halt_on_error=1:print_stacktrace=1"
+    with tempfile.TemporaryDirectory(prefix="sfx_emu_san") as d:
+        fin, fout = os.path.join(d, "in.pkl"), os.path.join(d, "out.pkl")
+        with open(fin, "wb") as f:
+            pickle.dump((lib, fname, list(args)), f)
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--child", fin, fout], env=env,
+                           capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"sanitized emulator call {fname} exited with {p.returncode}:\n{p.stderr[-4000:]}")
+        with open(fout, "rb") as f:
+            return pickle.load(f)
+
+
+def call(fname: str, *args, timeout: float = 300.0, sanitize: bool | None = None):
     """Run emulator entry ``fname`` in a child process; returns (rc, arrays)."""
+    if sanitize is None:
+        sanitize = sanitize_default()
+    if sanitize:
+        return _call_sanitized(fname, args, timeout)
     build_emulator()
     ctx = mp.get_context("fork")
     a, b = ctx.Pipe()
@@ -93,3 +148,12 @@ def call(fname: str, *args, timeout: float = 300.0):
     if p.exitcode not in (0, None):
         raise RuntimeError(f"emulator child exited with {p.exitcode} (out-of-bounds abort?)")
     return res
+
+
+if __name__ == "__main__" and len(sys.argv) == 4 and sys.argv[1] == "--child":
+    # sanitized child: arguments and results pass through files this module wrote
+    with open(sys.argv[2], "rb") as f:
+        lib_path, fn, call_args = pickle.load(f)
+    res = _run(_load(lib_path), fn, call_args)
+    with open(sys.argv[3], "wb") as f:
+        pickle.dump(res, f)
