@@ -45,6 +45,15 @@ const char* sfx_version(void);
 /* Select and initialise the device used by subsequent calls on this thread. */
 int sfx_init(int32_t device);
 
+/* Scratch of the "_dev" entry points.  The exact-grid products (the
+ * non-polynomial FFT_PAR_* routines and the entropy viscosities) keep a
+ * grow-only scratch per (device, stream): calls on different streams run
+ * concurrently without sharing it.  sfx_stream_release waits for the stream
+ * and frees its scratch; call it before destroying a stream that ran such
+ * products.  No counterpart in the reference (FFTW plans and temporary
+ * arrays there are per call). */
+int sfx_stream_release(void* stream);
+
 /* ---------------------------------------------------------------------------
  * Pseudo-spectral products at arbitrary points (Gauss points or nodes).
  * ------------------------------------------------------------------------- */
@@ -106,9 +115,13 @@ int sfx_fft_tensor_dcl_dev(const double* V1_in, const double* V2_in, double* V_o
 /* ---------------------------------------------------------------------------
  * Non-polynomial products on the reference's own azimuthal grid
  * N = 2*m_max_pad-1 (odd), as real DFTs: their coefficients depend on N.
- * m_max_pad >= m_max_c is required.  The library keeps a grow-only scratch
- * buffer (2 * outputs * m_max_pad * np doubles) per device; calls that
- * overlap on one device race on it.
+ * m_max_pad >= m_max_c is required.  The "_dev" forms keep a grow-only
+ * scratch (up to 2 * outputs * m_max_pad * np doubles) per (device, stream):
+ * calls on different streams may overlap.  A call that must grow it waits
+ * for its stream first, so under HIP graph capture make one call of the
+ * largest size beforehand.  sfx_stream_release frees a stream's scratch.
+ * The host-pointer forms stage their arrays in per-device buffers: call
+ * them from one host thread at a time per device.
  * ------------------------------------------------------------------------- */
 
 /* Replaces FFT_PAR_DIV_DCL (fft_parallel.F90:667): c_out(np,2,m) = c1 / c2. */

@@ -36,6 +36,7 @@ SIGNATURES = {
     "sfx_last_error": [],
     "sfx_version": [],
     "sfx_init": [_i32],
+    "sfx_stream_release": [_vp],
     "sfx_fft_par_cross_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
     "sfx_fft_par_dot_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i64, _i32, _vp],
     "sfx_fft_par_prod_dcl": [_vp, _vp, _vp, _i64, _i32, _i32, _i64, _i32, _vp],

@@ -100,6 +100,49 @@ int staging(int slot, size_t bytes, void** out) {
   return SFX_OK;
 }
 
+// ---- grow-only device scratch of the device-pointer entry points, one set
+// per (device, stream).  Work queued on one stream runs in order, so its
+// calls may reuse one buffer; calls on two streams get two buffers.  A buffer
+// is replaced only after its stream has drained (the kernels queued before
+// may still read the old one).
+static std::map<std::pair<int, uintptr_t>, Staging> g_stream_scratch;
+
+int stream_scratch(int slot, size_t bytes, hipStream_t st, void** out) {
+  if (slot < 0 || slot >= N_STAGING) return fail(SFX_ERR_ARG, "scratch slot %d out of range", slot);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_mu);
+  Staging& s = g_stream_scratch[std::make_pair(dev, (uintptr_t)st)];
+  if (s.n[slot] < bytes) {
+    if (s.p[slot]) {
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "scratch: hipStreamSynchronize");
+      (void)hipFree(s.p[slot]);
+    }
+    s.p[slot] = nullptr;
+    s.n[slot] = 0;
+    e = hipMalloc(&s.p[slot], bytes ? bytes : 8);
+    if (e != hipSuccess) return fail(SFX_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    s.n[slot] = bytes;
+  }
+  *out = s.p[slot];
+  return SFX_OK;
+}
+
+int release_stream_scratch(hipStream_t st) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_stream_scratch.find(std::make_pair(dev, (uintptr_t)st));
+  if (it == g_stream_scratch.end()) return SFX_OK;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "scratch release: hipStreamSynchronize");
+  for (int i = 0; i < N_STAGING; ++i)
+    if (it->second.p[i]) (void)hipFree(it->second.p[i]);
+  g_stream_scratch.erase(it);
+  return SFX_OK;
+}
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -169,6 +212,11 @@ extern "C" {
 const char* sfx_last_error(void) { return g_err; }
 
 const char* sfx_version(void) { return "sfx 0.1 (gfx950, FP64)"; }
+
+int sfx_stream_release(void* stream) {
+  clear_err();
+  return release_stream_scratch((hipStream_t)stream);
+}
 
 int sfx_init(int32_t device) {
   clear_err();

@@ -34,6 +34,8 @@ int hip_fail(hipError_t e, const char* where);
 void clear_err();
 int get_twiddles(int MP, Twiddles* tw);
 int staging(int slot, size_t bytes, void** out);
+int stream_scratch(int slot, size_t bytes, hipStream_t st, void** out);
+int release_stream_scratch(hipStream_t st);
 double now_s();
 int check_modes(const char* name, int64_t np, int32_t m_max_c, int32_t m_max_pad);
 int check_procs(const char* name, int64_t np, int32_t nb_procs, int64_t bloc_size);

@@ -1,4 +1,8 @@
 // C-ABI of the exact-grid products: DFT tables, scratch, argument checks.
+//
+// The device-pointer paths keep their scratch per (device, stream)
+// (stream_scratch): calls on different streams never share it.  The
+// host-pointer paths stage their arrays in the per-device staging slots.
 #include <math.h>
 #include <string.h>
 
@@ -47,7 +51,8 @@ enum { SLOT_EX_IN1 = 3, SLOT_EX_IN2 = 4, SLOT_EX_OUT = 5, SLOT_EX_W = 6, SLOT_EX
        SLOT_EX_AUX = 15, SLOT_EX_STAT = 16, SLOT_EX_FLAG = 17 };
 
 // Common set-up: sizes, tables, scratch.  `a.in`, `a.out`, nin, nout filled by the caller.
-static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad) {
+static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M, int m_max_pad,
+                         hipStream_t st) {
   if (M <= 0) return fail(SFX_ERR_ARG, "BUG in %s: m_max_c==0", nm);
   if (np < 0) return fail(SFX_ERR_ARG, "%s: np < 0", nm);
   if (M > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, M);
@@ -59,11 +64,11 @@ static int exact_prepare(const char* nm, ExactArgs& a, int op, int64_t np, int M
   void* w = nullptr;
   if (exact_is_reduction(op)) {
     const int64_t nb = exact_grid_blocks(op, a.nin, np, a.K);
-    if ((rc = staging(SLOT_EX_W, (size_t)(nb > 0 ? nb : 1) * 8, &w))) return rc;
+    if ((rc = stream_scratch(SLOT_EX_W, (size_t)(nb > 0 ? nb : 1) * 8, st, &w))) return rc;
     a.pmax = (double*)w;
   } else {
     a.n_W = 2 * (int64_t)a.nout * (a.K + 1) * np;
-    if ((rc = staging(SLOT_EX_W, (size_t)(a.n_W > 0 ? a.n_W : 1) * 8, &w))) return rc;
+    if ((rc = stream_scratch(SLOT_EX_W, (size_t)(a.n_W > 0 ? a.n_W : 1) * 8, st, &w))) return rc;
     a.W = (double*)w;
   }
   return SFX_OK;
@@ -81,7 +86,7 @@ static int exact_run(const char* nm, int op, const ExactArgs& a, double* max_out
 static int div_dev(const double* c1, const double* c2, double* c, int64_t np, int M, int mp, hipStream_t st) {
   ExactArgs a;
   exact_setup_div(a, c1, c2, c, np, M);
-  int rc = exact_prepare("FFT_PAR_DIV_DCL", a, EX_DIV, np, M, mp);
+  int rc = exact_prepare("FFT_PAR_DIV_DCL", a, EX_DIV, np, M, mp, st);
   return rc ? rc : exact_run("FFT_PAR_DIV_DCL", EX_DIV, a, nullptr, st);
 }
 
@@ -89,7 +94,7 @@ static int div_dev(const double* c1, const double* c2, double* c, int64_t np, in
 static int sv2_dev(const double* V, const double* s, double* out, int64_t np, int M, int mp, hipStream_t st) {
   ExactArgs a;
   exact_setup_sv2(a, V, s, out, np, M);
-  int rc = exact_prepare("FFT_SCALAR_VECT_DCL", a, EX_SV2, np, M, mp);
+  int rc = exact_prepare("FFT_SCALAR_VECT_DCL", a, EX_SV2, np, M, mp, st);
   return rc ? rc : exact_run("FFT_SCALAR_VECT_DCL", EX_SV2, a, nullptr, st);
 }
 
@@ -97,7 +102,7 @@ static int sv2_dev(const double* V, const double* s, double* out, int64_t np, in
 static int sqrt_dev(const double* c, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
   ExactArgs a;
   exact_setup_sqrt(a, c, out, np, nb, M);
-  int rc = exact_prepare("FFT_SQUAREROOT_SCALAR", a, EX_SQRT, np, M, mp);
+  int rc = exact_prepare("FFT_SQUAREROOT_SCALAR", a, EX_SQRT, np, M, mp, st);
   return rc ? rc : exact_run("FFT_SQUAREROOT_SCALAR", EX_SQRT, a, nullptr, st);
 }
 
@@ -112,7 +117,7 @@ static int svclamp_dev(const double* bounds, int nbd, const double* V, const dou
   }
   ExactArgs a;
   exact_setup_svclamp(a, V, s, out, np, M, pb, lo, hi);
-  int rc = exact_prepare("FFT_SCALAR_VECT_NO_OVERSHOOT", a, EX_SVCLAMP, np, M, mp);
+  int rc = exact_prepare("FFT_SCALAR_VECT_NO_OVERSHOOT", a, EX_SVCLAMP, np, M, mp, st);
   return rc ? rc : exact_run("FFT_SCALAR_VECT_NO_OVERSHOOT", EX_SVCLAMP, a, nullptr, st);
 }
 
@@ -120,7 +125,7 @@ static int svclamp_dev(const double* bounds, int nbd, const double* V, const dou
 static int clamp01_dev(const double* c, double* out, int64_t np, int nb, int M, int mp, hipStream_t st) {
   ExactArgs a;
   exact_setup_sqrt(a, c, out, np, nb, M);
-  int rc = exact_prepare("FFT_NO_OVERSHOOT_LEVEL_SET", a, EX_CLAMP01, np, M, mp);
+  int rc = exact_prepare("FFT_NO_OVERSHOOT_LEVEL_SET", a, EX_CLAMP01, np, M, mp, st);
   return rc ? rc : exact_run("FFT_NO_OVERSHOOT_LEVEL_SET", EX_CLAMP01, a, nullptr, st);
 }
 
@@ -129,7 +134,7 @@ static int entro_dev(const double* V, const double* c, const double* creal, doub
   if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_PAR_ENTRO_VISC_DCL: null c_in_real");
   ExactArgs a;
   exact_setup_entro(a, V, c, creal, coeff, Vout, cout, np, M, mp);
-  int rc = exact_prepare("FFT_PAR_ENTRO_VISC_DCL", a, EX_ENTRO, np, M, mp);
+  int rc = exact_prepare("FFT_PAR_ENTRO_VISC_DCL", a, EX_ENTRO, np, M, mp, st);
   return rc ? rc : exact_run("FFT_PAR_ENTRO_VISC_DCL", EX_ENTRO, a, nullptr, st);
 }
 
@@ -138,7 +143,7 @@ static int compr_dev(const double* V1, const double* V2, const double* c, const 
   if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_PAR_COMPR_ENTRO_VISC_DCL: null c_in_real");
   ExactArgs a;
   exact_setup_compr(a, V1, V2, c, creal, coeff, comp, Vout, cout, np, M, mp);
-  int rc = exact_prepare("FFT_PAR_COMPR_ENTRO_VISC_DCL", a, EX_COMPR, np, M, mp);
+  int rc = exact_prepare("FFT_PAR_COMPR_ENTRO_VISC_DCL", a, EX_COMPR, np, M, mp, st);
   return rc ? rc : exact_run("FFT_PAR_COMPR_ENTRO_VISC_DCL", EX_COMPR, a, nullptr, st);
 }
 
@@ -153,14 +158,14 @@ static int evisc_dev(const double* const (&V)[5], const double* hloc, double* co
   exact_setup_evisc(a, V, hloc, O, np, M, lg, c1, c2, c4, res);
   a.n_nint = (np / lg) * (2 * (int64_t)mp - 1);
   void* w = nullptr;
-  int rc = staging(SLOT_EX_NINT, (size_t)(a.n_nint > 0 ? a.n_nint : 1) * 8, &w);
+  int rc = stream_scratch(SLOT_EX_NINT, (size_t)(a.n_nint > 0 ? a.n_nint : 1) * 8, st, &w);
   if (rc) return rc;
   a.nint = (double*)w;
   // the viscosity on the grid between the two product passes (N, np)
   void* vr = nullptr;
-  if ((rc = staging(SLOT_EX_AUX, (size_t)(2 * (int64_t)mp - 1) * (np > 0 ? np : 1) * 8, &vr))) return rc;
+  if ((rc = stream_scratch(SLOT_EX_AUX, (size_t)(2 * (int64_t)mp - 1) * (np > 0 ? np : 1) * 8, st, &vr))) return rc;
   a.rout2 = (double*)vr;
-  if ((rc = exact_prepare(nm, a, EX_EVISC, np, M, mp))) return rc;
+  if ((rc = exact_prepare(nm, a, EX_EVISC, np, M, mp, st))) return rc;
   return exact_run(nm, EX_EVISC, a, nullptr, st);
 }
 
@@ -182,8 +187,9 @@ static int emom_dev(const double* V1, const double* V2, const double* V3, const 
   if (rc) return rc;
   const int64_t ne = (np / lg) * (2 * (int64_t)mp - 1);
   void *w = nullptr, *pm = nullptr, *dm = nullptr;
-  if ((rc = staging(SLOT_EX_NINT, (size_t)3 * ne * 8, &w)) ||
-      (rc = staging(SLOT_EX_AUX, (size_t)exact_emom_blocks(np, lg, a.K) * 8, &pm)) || (rc = staging(SLOT_EX_MAX, 8, &dm)))
+  if ((rc = stream_scratch(SLOT_EX_NINT, (size_t)3 * ne * 8, st, &w)) ||
+      (rc = stream_scratch(SLOT_EX_AUX, (size_t)exact_emom_blocks(np, lg, a.K) * 8, st, &pm)) ||
+      (rc = stream_scratch(SLOT_EX_MAX, 8, st, &dm)))
     return rc;
   a.nint = (double*)w;
   a.nint2 = a.nint + ne;
@@ -200,7 +206,7 @@ static int gradmom_dev(const double* V1, const double* V2, const double* V3, con
   if (!creal && np > 0) return fail(SFX_ERR_ARG, "FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM: null c_in_real");
   ExactArgs a;
   exact_setup_gradmom(a, V1, V2, V3, creal, coeff, out, np, M, mp);
-  int rc = exact_prepare("FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM", a, EX_GRADMOM, np, M, mp);
+  int rc = exact_prepare("FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM", a, EX_GRADMOM, np, M, mp, st);
   return rc ? rc : exact_run("FFT_COMPUTE_ENTROPY_VISC_GRAD_MOM", EX_GRADMOM, a, nullptr, st);
 }
 
@@ -213,7 +219,7 @@ static int heavi_dev(const double* V1, const double* values, int n_values, doubl
     return fail(SFX_ERR_UNSUPPORTED, "FFT_HEAVISIDE_DCL: %d interfaces > %d", n_int, EXACT_MAX_INT);
   ExactArgs a;
   exact_setup_heavi(a, V1, values, n_values, out, np, M, eps);
-  int rc = exact_prepare("FFT_HEAVISIDE_DCL", a, EX_HEAVI, np, M, mp);
+  int rc = exact_prepare("FFT_HEAVISIDE_DCL", a, EX_HEAVI, np, M, mp, st);
   return rc ? rc : exact_run("FFT_HEAVISIDE_DCL", EX_HEAVI, a, nullptr, st);
 }
 
@@ -221,7 +227,7 @@ static int heavi_dev(const double* V1, const double* values, int n_values, doubl
 static int tension_dev(const double* V1, const double* V2, double* out, int64_t np, int M, int mp, hipStream_t st) {
   ExactArgs a;
   exact_setup_tension(a, V1, V2, out, np, M);
-  int rc = exact_prepare("FFT_TENSOR_DCL", a, EX_TENSION, np, M, mp);
+  int rc = exact_prepare("FFT_TENSOR_DCL", a, EX_TENSION, np, M, mp, st);
   return rc ? rc : exact_run("FFT_TENSOR_DCL", EX_TENSION, a, nullptr, st);
 }
 
@@ -229,7 +235,7 @@ static int tension_dev(const double* V1, const double* V2, double* out, int64_t 
 static int maxvel_dev(const double* V1, double* max_out, int64_t np, int nb, int M, int mp, hipStream_t st) {
   ExactArgs a;
   exact_setup_maxvel(a, V1, np, nb, M);
-  int rc = exact_prepare("FFT_MAX_VEL_DCL", a, EX_MAXVEL, np, M, mp);
+  int rc = exact_prepare("FFT_MAX_VEL_DCL", a, EX_MAXVEL, np, M, mp, st);
   return rc ? rc : exact_run("FFT_MAX_VEL_DCL", EX_MAXVEL, a, max_out, st);
 }
 
@@ -239,12 +245,12 @@ static int cls_dev(const double* V1, const double* V2, const double* c, double* 
   // max |velocity| over points and angles (the element and angle maxima the
   // reference takes first do not change it)
   void* dm = nullptr;
-  int rc = staging(SLOT_EX_MAX, 8, &dm);
+  int rc = stream_scratch(SLOT_EX_MAX, 8, st, &dm);
   if (rc) return rc;
   if ((rc = maxvel_dev(V2, (double*)dm, np, 6, M, mp, st))) return rc;
   ExactArgs a;
   exact_setup_cls(a, V1, c, cout, (const double*)dm, np, M, comp, c4, hmin);
-  if ((rc = exact_prepare(nm, a, EX_CLS, np, M, mp))) return rc;
+  if ((rc = exact_prepare(nm, a, EX_CLS, np, M, mp, st))) return rc;
   return exact_run(nm, EX_CLS, a, nullptr, st);
 }
 
@@ -256,7 +262,7 @@ static int real_dev(const double* V, double* out, int64_t np, int nb, int M, int
   for (int f = 0; f < nb / 2; ++f) {
     ExactArgs a;
     exact_setup_real(a, V, f, nb, out, np, M, N);
-    int rc = exact_prepare(nm, a, EX_REAL, np, M, mp);
+    int rc = exact_prepare(nm, a, EX_REAL, np, M, mp, st);
     if (rc || (rc = exact_run(nm, EX_REAL, a, nullptr, st))) return rc;
   }
   return SFX_OK;
@@ -267,12 +273,12 @@ static int maxminvel_dev(const double* V1, double* out, int64_t np, int nb, int 
   const char* nm = "FFT_MAX_MIN_VEL_DCL";
   ExactArgs a;
   exact_setup_maxvel(a, V1, np, nb, M);
-  int rc = exact_prepare(nm, a, EX_MAXVEL, np, M, mp);
+  int rc = exact_prepare(nm, a, EX_MAXVEL, np, M, mp, st);
   if (rc) return rc;
   // one pass: three maxima per work-group, then one reduction block
   void* part = nullptr;
   const int64_t nblk = exact_velstat_blocks(a.nin, np, a.K);
-  if ((rc = staging(SLOT_EX_STAT, (size_t)(nblk > 0 ? nblk : 1) * 3 * 8, &part))) return rc;
+  if ((rc = stream_scratch(SLOT_EX_STAT, (size_t)(nblk > 0 ? nblk : 1) * 3 * 8, st, &part))) return rc;
   hipError_t e = launch_exact_velstat(a, (double*)part, out, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
@@ -284,14 +290,14 @@ static int checkif_dev(const double* V1, int nb_fluids, int32_t* ok, int64_t np,
   if (n_int < 1) return fail(SFX_ERR_ARG, "%s: nb_fluids=%d < 2", nm, nb_fluids);
   if (n_int > EXACT_MAX_INT) return fail(SFX_ERR_UNSUPPORTED, "%s: %d level sets > %d", nm, n_int, EXACT_MAX_INT);
   void* f = nullptr;
-  int rc = staging(SLOT_EX_FLAG, 8, &f);
+  int rc = stream_scratch(SLOT_EX_FLAG, 8, st, &f);
   if (rc) return rc;
   if (n_int == 1) {
     f = nullptr;   // two fluids: nothing to compare (:2390-2392), interface_ok = 1
   } else {
     ExactArgs a;
     exact_setup_checkif(a, V1, n_int, np, M);
-    if ((rc = exact_prepare(nm, a, EX_CHECKIF, np, M, mp)) || (rc = exact_run(nm, EX_CHECKIF, a, (double*)f, st)))
+    if ((rc = exact_prepare(nm, a, EX_CHECKIF, np, M, mp, st)) || (rc = exact_run(nm, EX_CHECKIF, a, (double*)f, st)))
       return rc;
   }
   hipError_t e = launch_exact_checkif((const double*)f, ok, st);
@@ -309,7 +315,7 @@ static int eta_dev(const double* eta, const double* c, double* out, int64_t np, 
   const int op = nb == 2 ? EX_ETA : EX_ETA3;
   ExactArgs a;
   exact_setup_eta(a, eta, c, out, np, nb, M, mp);
-  int rc = exact_prepare(nm, a, op, np, M, mp);
+  int rc = exact_prepare(nm, a, op, np, M, mp, st);
   return rc ? rc : exact_run(nm, op, a, nullptr, st);
 }
 
@@ -319,7 +325,7 @@ static int fromreal_dev(const double* real, double* c, int64_t np, int M, int mp
   if (!real && np > 0) return fail(SFX_ERR_ARG, "%s: null real_in", nm);
   ExactArgs a;
   exact_setup_fromreal(a, real, c, np, M, mp);
-  int rc = exact_prepare(nm, a, EX_FROMREAL, np, M, mp);
+  int rc = exact_prepare(nm, a, EX_FROMREAL, np, M, mp, st);
   return rc ? rc : exact_run(nm, EX_FROMREAL, a, nullptr, st);
 }
 
@@ -357,7 +363,7 @@ static int inplace_dev(int op, const char* nm, double* c1_inout, int64_t np, int
   if (nb_field % 2 != 0 || nb_field < 2 || m_max_c == 0) return fail(SFX_ERR_ARG, " BUG in %s ", nm);
   const size_t n = (size_t)np * nb_field * (m_max_c > 0 ? m_max_c : 0);
   void* d = nullptr;
-  int rc = staging(SLOT_EX_IN1, n * 8, &d);
+  int rc = stream_scratch(SLOT_EX_IN1, n * 8, st, &d);
   if (rc) return rc;
   hipError_t e = n ? hipMemcpyAsync(d, c1_inout, n * 8, hipMemcpyDeviceToDevice, st) : hipSuccess;
   if (e != hipSuccess) return hip_fail(e, nm);

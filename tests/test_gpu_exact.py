@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 from oracle import fft_par as F  # noqa: E402
 from sfemans_amd import fft_parallele as FP  # noqa: E402
-from sfemans_amd._lib import SfxError  # noqa: E402
+from sfemans_amd._lib import SfxError, check, lib  # noqa: E402
 from tests.test_oracle_fft_exact import band_limited, times_alpha_beta_cos  # noqa: E402
 
 TOL = 1e-12
@@ -450,3 +450,53 @@ def test_empty_and_single_point_grid_routines():
     back = np.zeros((1, 2, 1), order="F")
     FP.FFT_PAR_FROM_REAL(np.asfortranarray(R[:, 0, :]), back, 5)
     assert abs(back[0, 0, 0] - 2.5) <= 1e-15 and back[0, 1, 0] == 0.0
+
+
+def test_two_streams_keep_their_own_scratch():
+    """Exact-grid products on two streams at once: each stream has its own
+    scratch, so the results equal those of the same calls made one after the
+    other (bit for bit).  Stream A grows its scratch between two calls (small
+    division, then the tension tensor) while stream B runs divisions."""
+    print("case two streams: tension + division", flush=True)
+    rng = np.random.default_rng(91)
+    nA, nB, M = 40000, 30000, 16
+    V1, V2 = FP.fortran_tensor(fields(rng, nA, 6, M)), FP.fortran_tensor(fields(rng, nA, 6, M))
+    c1, c2 = fields(rng, nB, 2, M), fields(rng, nB, 2, M, 0.1)
+    c2[:, 0, 0] = 2.0
+    a1, a2 = FP.fortran_tensor(c1[:500]), FP.fortran_tensor(c2[:500])
+    c1, c2 = FP.fortran_tensor(c1), FP.fortran_tensor(c2)
+    refT = FP.fortran_empty((3, nA, 6, M))
+    refD = FP.fortran_empty((nB, 2, M))
+    refS = FP.fortran_empty((500, 2, M))
+    FP.FFT_TENSOR_DCL(None, V1, V2, refT, opt_tension=True)
+    FP.FFT_PAR_DIV_DCL(None, c1, c2, refD)
+    FP.FFT_PAR_DIV_DCL(None, a1, a2, refS)
+    torch.cuda.synchronize()
+    sA, sB = torch.cuda.Stream(), torch.cuda.Stream()
+    outT = [FP.fortran_empty((3, nA, 6, M)) for _ in range(3)]
+    outS = [FP.fortran_empty((500, 2, M)) for _ in range(3)]
+    outD = [FP.fortran_empty((nB, 2, M)) for _ in range(6)]
+    torch.cuda.synchronize()
+    try:
+        for i in range(3):
+            with torch.cuda.stream(sA):
+                FP.FFT_PAR_DIV_DCL(None, a1, a2, outS[i])
+                FP.FFT_TENSOR_DCL(None, V1, V2, outT[i], opt_tension=True)
+            with torch.cuda.stream(sB):
+                FP.FFT_PAR_DIV_DCL(None, c1, c2, outD[2 * i])
+                FP.FFT_PAR_DIV_DCL(None, c1, c2, outD[2 * i + 1])
+        torch.cuda.synchronize()
+    finally:
+        check(lib().sfx_stream_release(sA.cuda_stream))
+        check(lib().sfx_stream_release(sB.cuda_stream))
+    for o in outT:
+        assert torch.equal(o, refT)
+    for o in outS:
+        assert torch.equal(o, refS)
+    for o in outD:
+        assert torch.equal(o, refD)
+    # the released streams' scratch is gone; a new call on the default stream still works
+    d = FP.fortran_empty((nB, 2, M))
+    FP.FFT_PAR_DIV_DCL(None, c1, c2, d)
+    torch.cuda.synchronize()
+    assert torch.equal(d, refD)
