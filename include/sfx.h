@@ -470,6 +470,17 @@ int sfx_dot_product_sf_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_
                            const double* v, const double* w, double* result_dev, double* work, void* stream);
 int sfx_dot_product_sf(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c, int32_t nb_field, const double* v,
                        const double* w, double* result);
+/* No counterpart in the reference: sfx_dot_product_sf_dev for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  It takes no work array and allocates nothing: the
+ * partial sums (at most 2048 doubles, which covers every size, since the sum
+ * is split over at most 2048 work-groups) live in a fixed 16 KiB device array
+ * that each GPU holds until the process ends.  Every partial is written
+ * before it is read, so nothing needs clearing between calls.  Calls on one
+ * GPU share that array: calls that overlap on one GPU (two streams, two
+ * threads) race. */
+int sfx_dot_product_sf_noalloc(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, int32_t nb_field,
+                               const double* v, const double* w, double* result_dev, void* stream);
 
 /* The r-weighted P2 mass matrix M_ij = SUM_(e,l) ww(i,l) ww(j,l) rj ray, the
  * Gram matrix of dot_product_SF per column (no counterpart as a routine in

@@ -122,6 +122,7 @@ SIGNATURES = {
     "sfx_qs_00_gauss": [_vp, _vp, _vp, _vp, _vp],
     "sfx_dot_product_sf_work": [_vp, _i32, _i32],
     "sfx_dot_product_sf_dev": [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
+    "sfx_dot_product_sf_noalloc": [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp],
     "sfx_dot_product_sf": [_vp, _vp, _i32, _i32, _vp, _vp, _vp],
     "sfx_mass_apply_dev": [_vp, _i32, _vp, _vp, _vp, _vp],
     "sfx_mass_solve_work": [_vp, _i32],

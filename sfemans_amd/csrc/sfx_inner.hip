@@ -1,4 +1,9 @@
-// Launchers of the inner-product, mass-matrix and column kernels (body in sfx_inner_body.h).
+// Launchers of the inner-product, mass-matrix and column kernels (body in
+// sfx_inner_body.h), and the allocation-free form of dot_product_SF.
+#include <mutex>
+
+#include "../../include/sfx.h"
+#include "sfx_api_internal.h"
 #include "sfx_engine.h"
 #include "sfx_inner_body.h"
 #include "sfx_kernels.h"
@@ -82,3 +87,47 @@ hipError_t launch_scale_cols(const InnerArgs& a, hipStream_t st) {
 }
 
 }  // namespace sfx
+
+// ---- dot_product_SF without allocation: the partial sums of the work-groups
+// in a fixed device array (DOTSF_MAX_BLOCKS doubles, 16 KiB, held by every
+// device the library runs on until the process ends).  The host side never
+// uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[sfx::DOTSF_MAX_BLOCKS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+extern "C" int sfx_dot_product_sf_noalloc(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c,
+                                          int32_t nb_field, const double* v, const double* w, double* result_dev,
+                                          void* stream) {
+  using namespace sfx;
+  clear_err();
+  if (!mesh) return fail(SFX_ERR_ARG, "dot_product_SF (no allocation): null mesh");
+  if (m_max_c < 0 || nb_field <= 0) return fail(SFX_ERR_ARG, "dot_product_SF (no allocation): bad sizes");
+  const int64_t nb = dotsf_blocks(mesh->me, nb_field * m_max_c);
+  if (nb > DOTSF_MAX_BLOCKS)
+    return fail(SFX_ERR_ARG, "dot_product_SF (no allocation): %lld partial sums exceed the %d of the fixed scratch",
+                (long long)nb, DOTSF_MAX_BLOCKS);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "dot_product_SF (no allocation): hipGetDevice");
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(SFX_ERR_UNSUPPORTED, "dot_product_SF (no allocation): device %d", dev);
+  double* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+      // this device's copy of the array, looked up with the device current
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) return hip_fail(e, "dot_product_SF (no allocation): hipGetSymbolAddress");
+      if (!p) return fail(SFX_ERR_HIP, "dot_product_SF (no allocation): hipGetSymbolAddress gave a null address");
+      scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    ptr = scratch_ptr[dev];
+  }
+  return sfx_dot_product_sf_dev(mesh, list_mode_dev, m_max_c, nb_field, v, w, result_dev, ptr, stream);
+}
