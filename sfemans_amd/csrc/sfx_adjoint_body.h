@@ -47,8 +47,19 @@ constexpr int adjoint_waves() { return (!SPLIT && MP == 64) ? 7 : 4; }
 template <int MP, bool SPLIT, int NW>
 struct AdjLds {
   using LY = GaussLds<MP, NW>;
-  // per-wave region: 3 exchanged signals (3 x 256 complex) = 1536 doubles
-  static constexpr int RW = 6 * Shape<MP>::WAVE_CPLX;
+  static constexpr int NT = 64 * NW;
+  // Parking (convective form on 7 waves, 256 registers per lane): the base
+  // spectra wait in LDS while pbar is formed, and pbar waits in LDS during
+  // the base-field transforms, so that neither is spilled to scratch.
+  static constexpr bool PARK = !SPLIT && NW > 4;
+  // signals exchanged per LDS round of the transforms; the per-wave region
+  // also holds the partner swap of the base fields (64 x 12 doubles) and,
+  // with 4 waves, of ybar (64 x 24)
+  static constexpr int GX = PARK ? 2 : 3;
+  static constexpr int RW0 = 2 * GX * Shape<MP>::WAVE_CPLX;
+  static constexpr int RWS = 64 * (NW <= 4 ? 24 : 12);
+  static constexpr int RW = RW0 > RWS ? RW0 : RWS;
+  static_assert(RW >= 2 * GX * Shape<MP>::WAVE_CPLX, "exchange region too small");
   static constexpr int NODAL = 2 * LY::NOD6;                              // doubles from BIG
   static constexpr int XCH = NODAL;                                       // first wave region
   // Element sums by local node (work item: element, node, mode offset)
@@ -67,8 +78,23 @@ struct AdjLds {
   static constexpr int RED = BY_NODE ? RPL * AdjRed<SPLIT>::K : SK * AdjRed<SPLIT>::K;
   // the reduction area aliases the staging and wave regions (used after them)
   static constexpr int BIGD = (NODAL + NW * RW > RED) ? NODAL + NW * RW : RED;
-  static constexpr int TOTAL = LY::BIG + (BIGD + 1) / 2;   // double2 units
+  // parked doubles per lane: PK1 in the staging area (idle once the base
+  // fields are interpolated; slot f of lane t at f * NT + t), PK2 past BIGD
+  // in what is left of the 160 KB
+  static constexpr int LDS_DOUBLES = 160 * 1024 / 8;
+  static constexpr int PK1 = PARK ? NODAL / NT : 0;
+  static constexpr int PK2 = PARK ? (LDS_DOUBLES - 2 * LY::BIG - BIGD) / NT : 0;
+  static constexpr int PK = PK1 + PK2;
+  static constexpr int TOTAL = LY::BIG + (BIGD + PK2 * NT + 1) / 2;   // double2 units
+  static_assert(!PARK || (PK2 >= 0 && 2 * TOTAL <= LDS_DOUBLES), "parking exceeds the LDS");
 };
+
+// Parked slot f (0 <= f < PK) of lane t.
+template <class AL>
+SFX_DEV double& park_slot(double* big, int f, int t) {
+  SFX_CHECK("park slot", f, AL::PK);
+  return f < AL::PK1 ? big[f * AL::NT + t] : big[AL::BIGD + (f - AL::PK1) * AL::NT + t];
+}
 
 // Load ybar at the own and partner modes (MP - m) of Gauss point p, scale
 // for Phi^T and pack the three real output fields as signals (pr, pth) and
@@ -171,7 +197,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
   (void)KR;   // bounds checks only
   const GaussArgs& a = A.g;
   constexpr int L = S::L, PW = S::PW, E = LY::E, N = S::N, NG = groups_per_wave(NW);
-  constexpr int GX = 3;
+  constexpr int GX = AL::GX;
   constexpr int NT = 64 * NW;
   const int wave = tid >> 6, lane = tid & 63;
   const int pt = lane / L, l = lane % L;
@@ -259,6 +285,18 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     if (!has[gi]) continue;
     // pbar = Phi^T ybar on the grid: 2 signals x 3 branches
     double pb[3][4][3];
+    if constexpr (AL::PARK) {
+      static_assert(NG == 1, "parking holds one group per wave");
+      // the last PK of the 48 base-spectrum doubles (Z then B,
+      // [signal][slot][re, im]) wait in LDS while pbar is formed; the
+      // staging area is free once every wave has interpolated from it
+      SFX_SYNC();
+#pragma unroll
+      for (int k = 48 - AL::PK; k < 48; ++k) {
+        const double2 z = k < 24 ? Z[gi][k / 8][(k % 8) / 2] : B[gi][(k - 24) / 8][(k % 8) / 2];
+        park_slot<AL>(big, k - (48 - AL::PK), tid) = (k & 1) ? z.y : z.x;
+      }
+    }
     {
       double2 Zy[2][4], By[2][4];
       load_ybar<MP, (NW <= 4)>(A, p[gi], valid[gi], pt, l, xf, Zy, By);
@@ -275,12 +313,38 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
           pb[j1][r][2] = y[3 + j1][r].x;
         }
     }
+    if constexpr (AL::PARK) {
+      // base spectra back from LDS, the last PK of pbar's 36 doubles
+      // ([branch][slot][component]) out.  The reloads go through an index
+      // the compiler cannot tie to the stores; it would forward the stored
+      // values and keep them in registers otherwise.
+      int tz = tid;
+      SFX_OPAQUE(tz);
+#pragma unroll
+      for (int k = 48 - AL::PK; k < 48; ++k) {
+        const double v = park_slot<AL>(big, k - (48 - AL::PK), tz);
+        double2& z = k < 24 ? Z[gi][k / 8][(k % 8) / 2] : B[gi][(k - 24) / 8][(k % 8) / 2];
+        if (k & 1) z.y = v;
+        else z.x = v;
+      }
+#pragma unroll
+      for (int j1 = 0; j1 < 3; ++j1)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int f = (j1 * 4 + r) * 3 + c;
+            if (f >= 36 - AL::PK) park_slot<AL>(big, f - (36 - AL::PK), tid) = pb[j1][r][c];
+          }
+    }
     double2 q[9][4];
     {
       double2 y[9][4];
 #pragma unroll
       for (int s = 0; s < 3; ++s) prestep<MP>(Z[gi][s], B[gi][s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
       inverse_fft<MP, 9, GX>(y, pt, l, lds, wMP);
+      int tp = tid;
+      SFX_OPAQUE(tp);
       // fields (rA_r, rA_th), (rA_z, wB_r), (wB_th, wB_z);
       // outputs wbar = pbar x rA, rbar = wB x pbar packed as (wb_r, wb_th), (wb_z, rb_r), (rb_th, rb_z)
 #pragma unroll
@@ -289,7 +353,16 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
         for (int r = 0; r < 4; ++r) {
           const double r0 = y[j1][r].x, r1 = y[j1][r].y, r2 = y[3 + j1][r].x;
           const double w0 = y[3 + j1][r].y, w1 = y[6 + j1][r].x, w2 = y[6 + j1][r].y;
-          const double p0 = A.coef * pb[j1][r][0], p1 = A.coef * pb[j1][r][1], p2 = A.coef * pb[j1][r][2];
+          double pv[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int f = (j1 * 4 + r) * 3 + c;
+            // (an if, not a conditional expression: a conditional over two
+            // lvalues selects between their addresses and keeps pb in scratch)
+            if (AL::PARK && f >= 36 - AL::PK) pv[c] = park_slot<AL>(big, f - (36 - AL::PK), tp);
+            else pv[c] = pb[j1][r][c];
+          }
+          const double p0 = A.coef * pv[0], p1 = A.coef * pv[1], p2 = A.coef * pv[2];
           const double wb0 = p1 * r2 - p2 * r1, wb1 = p2 * r0 - p0 * r2, wb2 = p0 * r1 - p1 * r0;
           const double rb0 = w1 * p2 - w2 * p1, rb1 = w2 * p0 - w0 * p2, rb2 = w0 * p1 - w1 * p0;
           q[j1][r] = make_double2(wb0, wb1);
@@ -301,7 +374,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     double2 lo[3][4], hi[3][4];
 #pragma unroll
     for (int t = 0; t < 3; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
-    pair_modes<MP, 3>(lo, hi, pt, l, lds, oa[gi], ob[gi]);
+    pair_modes<MP, 3, GX>(lo, hi, pt, l, lds, oa[gi], ob[gi]);
   }
 
   // ---- Psi^T scaling, curl transpose, reference-coordinate adjoints,
@@ -325,7 +398,14 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       Rb[0] = sa * oa[gi][1][1][r]; Rb[1] = sb * ob[gi][1][1][r];
       Rb[2] = sa * oa[gi][2][0][r]; Rb[3] = sb * ob[gi][2][0][r];
       Rb[4] = sa * oa[gi][2][1][r]; Rb[5] = sb * ob[gi][2][1][r];
-      const GaussGeom& gg = g[gi];
+      // the geometry again (24 FMAs) rather than kept live through the transforms
+      GaussGeom gg;
+      {
+        const int lpg = (wave + NW * gi) * PW + pt;
+        int elg = lpg / 7;
+        SFX_OPAQUE(elg);
+        gauss_geom(geo + 6 * elg, tab, valid[gi] ? lpg % 7 : 0, gg);
+      }
       const double mr = lmode[m < MP ? m : 0] * gg.iray, ir = gg.iray;
       double Dr[6] = {0, 0, 0, 0, 0, 0}, Dz[6] = {0, 0, 0, 0, 0, 0};
       double Ww[6];   // value weights of the W^T part alone (split form)

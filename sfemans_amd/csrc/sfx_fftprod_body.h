@@ -119,29 +119,33 @@ SFX_DEV void load_signal(const FftProdArgs& a, int64_t p, bool valid, int pt, in
 // Pair Q[m] (lo) with Q[N-m] (hi of the partner lane) and store the (cos,sin)
 // coefficients of the two real outputs packed in each complex output signal.
 // fft_parallel.F90:291-324: scale 1/N, mode 0 as is, 2*conj for m >= 1.
-template <int MP, int NOC>
+template <int MP, int NOC, int G = NOC>
 SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4], int pt, int l,
                         double2* lds, double (&oa)[NOC][2][4], double (&ob)[NOC][2][4]) {
   constexpr int L = Shape<MP>::L, N = Shape<MP>::N, WC = Shape<MP>::WAVE_CPLX;
-#pragma unroll
-  for (int t = 0; t < NOC; ++t)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = t * WC + lds_slot<MP>(4, pt, l + L * r);
-      SFX_CHECK("lds pair write", i, NOC * WC);
-      lds[i] = hi[t][r];
-    }
-  SFX_WAVE_SYNC();
+  // exchanged G signals at a time: the wave region holds G * WC complex slots
   double2 qn[NOC][4];
 #pragma unroll
-  for (int t = 0; t < NOC; ++t)
+  for (int t0 = 0; t0 < NOC; t0 += G) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = t * WC + lds_slot<MP>(4, pt, (MP - (l + L * r)) & (MP - 1));
-      SFX_CHECK("lds pair read", i, NOC * WC);
-      qn[t][r] = lds[i];
-    }
-  SFX_WAVE_SYNC();
+    for (int t = t0; t < NOC && t < t0 + G; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = (t - t0) * WC + lds_slot<MP>(4, pt, l + L * r);
+        SFX_CHECK("lds pair write", i, G * WC);
+        lds[i] = hi[t][r];
+      }
+    SFX_WAVE_SYNC();
+#pragma unroll
+    for (int t = t0; t < NOC && t < t0 + G; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = (t - t0) * WC + lds_slot<MP>(4, pt, (MP - (l + L * r)) & (MP - 1));
+        SFX_CHECK("lds pair read", i, G * WC);
+        qn[t][r] = lds[i];
+      }
+    SFX_WAVE_SYNC();
+  }
   const double invN = 1.0 / (double)N;
 #pragma unroll
   for (int t = 0; t < NOC; ++t)
