@@ -214,6 +214,34 @@ def wl_nsrhs(torch, args):
     return phases, G * M, cfg, ("nsrhs", m, M)
 
 
+def wl_b3(torch, args):
+    from sfemans_amd import fft_parallele as FP
+    from sfemans_amd import nonlinear as NL
+    from sfemans_amd.mesh import rectangle_p2
+    m = rectangle_p2(224, 448)
+    mesh = NL.Mesh.from_p2(m)
+    M = 64
+    modes = list(range(M))
+    G, npn = m.n_gauss, m.np
+    u, = _fields(torch, npn, M, 4.0, (SEED + 2,))
+    v145 = FP.fortran_empty((npn, 3, M))
+    v236 = FP.fortran_empty((npn, 3, M))
+    welem = torch.empty(36 * m.me * M, dtype=torch.float64, device="cuda")
+    # compulsory: u read, vb_145 and vb_236 written, mesh (jj, rr, rj*ray) once; the
+    # element contributions written and read back by the gather are re-traffic
+    b3_bytes = npn * 6 * M * 8 + npn * 6 * M * 8 + m.me * 24 + npn * 16 + m.me * 7 * 8
+    phases = [
+        Phase("nl_rhs_3x3", lambda: NL.nl_rhs_3x3(mesh, modes, u, v145, v236, -1.0, welem),
+              b3_bytes, b2_flops(G, M) + 72.0 * G * M, "gauss_rhs_kernel<64> + rhs_gather_kernel",
+              overhead=2 * 36 * m.me * M * 8),
+    ]
+    cfg = {"workload": "nonlinear term fused to the per-mode NS right-hand side (nodal u in, rhs_3x3's "
+                       "vb_145/vb_236 of -(curl u) x u out, no Gauss-point array), 224x448 P2 mesh "
+                       "(200,704 triangles, 1,404,928 Gauss points), m_max=64",
+           "me": m.me, "np": npn, "gauss_points": G, "m_max": M, "grid": 3 * mp_of(M)}
+    return phases, G * M, cfg, ("nl", m, M)
+
+
 def _exact_ops(FP, torch, npt, M, seed):
     """Inputs and calls of the six non-polynomial products on npt points."""
     dec = torch.exp(-torch.arange(M, device="cuda", dtype=torch.float64) / 4.0)
@@ -505,7 +533,7 @@ def wl_full(torch, args):
 
 
 WORKLOADS = {"nl": wl_nl, "mhd": wl_mhd, "fftprod": wl_fftprod, "tangent": wl_tangent, "batch": wl_batch,
-             "full": wl_full, "nsrhs": wl_nsrhs, "exact": wl_exact, "entropy": wl_entropy,
+             "full": wl_full, "nsrhs": wl_nsrhs, "b3": wl_b3, "exact": wl_exact, "entropy": wl_entropy,
              "grid": wl_grid}
 
 

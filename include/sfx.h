@@ -390,6 +390,19 @@ int sfx_smb_cross_prod_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mo
                                          const double* V_in, double* V_out, int32_t precession_in,
                                          double taux_precession, double angle_s_pi, void* stream);
 
+/* smb_cross_prod_gauss_sft_par followed by rhs_3x3 of its output, fused
+ * (subroutine_ns_with_velocity.F90:514-630 then assembling_rhs.F90:5-189
+ * through rhs_ns_gauss_3x3's -rotv_v, computing_rhs_gauss.F90:100):
+ * vb_145(np,3,m), vb_236(np,3,m) = scale * rhs_3x3((curl V_in + 2 Omega) x V_in)
+ * for every mode, with no Gauss-point array: the element sums are formed in
+ * the work-group and gathered node by node in element order, giving the
+ * same values, bit for bit, as the two calls with scale passed to
+ * sfx_rhs_3x3_dev (scale = -1 for the -rotv_v share of the NS right-hand
+ * side).  work_elem: 36*me*m doubles. */
+int sfx_nl_rhs_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* V_in,
+                       double* vb_145, double* vb_236, double scale, int32_t precession_in, double taux_precession,
+                       double angle_s_pi, double* work_elem, void* stream);
+
 /* Replaces smb_CurlH_cross_B_gauss_sft_par (subroutine_ns_with_velocity.F90:632):
  * V_out(G,6,m) = (curl V_in) x W_in, V_in and W_in nodal (np,6,m). */
 int sfx_smb_curlh_cross_b_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,

@@ -37,6 +37,38 @@ static hipError_t launch_v(const GaussArgs& a, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
+// The fused nodal -> rhs_3x3 element sums (RHS form of the body).
+template <int MP, int NW = gauss_waves<MP, GV_CROSS_CURL>()>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(waves_per_simd(NW), waves_per_simd(NW))))
+gauss_rhs_kernel(GaussArgs a) {
+  extern __shared__ double2 smem[];
+  gauss_body<MP, GV_CROSS_CURL, NW, false, true>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+}
+
+template <int MP>
+static hipError_t launch_rhs_mp(const GaussArgs& a, hipStream_t st) {
+  constexpr int NW = gauss_waves<MP, GV_CROSS_CURL>();
+  constexpr int E = GaussLds<MP, NW>::E;
+  const int64_t nblk = (a.me + E - 1) / E;
+  if (nblk == 0) return hipSuccess;
+  const size_t shmem = (size_t)GaussCfg<MP, GV_CROSS_CURL, NW>::LY::TOTAL * sizeof(double2);
+  hipLaunchKernelGGL((gauss_rhs_kernel<MP>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauss_rhs(const GaussArgs& a, hipStream_t st) {
+  switch (a.MP) {
+    case 4: return launch_rhs_mp<4>(a, st);
+    case 8: return launch_rhs_mp<8>(a, st);
+    case 16: return launch_rhs_mp<16>(a, st);
+    case 32: return launch_rhs_mp<32>(a, st);
+    case 64: return launch_rhs_mp<64>(a, st);
+    case 128: return launch_rhs_mp<128>(a, st);
+    case 256: return launch_rhs_mp<256>(a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_gauss(int variant, const GaussArgs& a, hipStream_t st) {
   switch (variant) {
     case GV_CROSS_CURL: return launch_v<GV_CROSS_CURL>(a, st);

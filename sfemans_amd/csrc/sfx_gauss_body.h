@@ -376,7 +376,12 @@ SFX_DEV void gauss_prologue(const GaussArgs& a, int tid, int64_t e0, double2* sm
   }
 }
 
-template <int MP, int V, int NW, bool STAGE_ALL = false>
+// RHS: instead of storing V_out at the Gauss points, form the element sums
+// of rhs_3x3 (assembling_rhs.F90:73-95) in the work-group,
+//   elo(6e+i, k, m) = SUM_l (ww(i,l) wgt(e,l)) V_out(7e+l, k, m),
+// in the order of rhs_elem_body, through LDS one mode slot at a time; a
+// gather (rhs_gather_block) then sums the nodes' entries into vb_145/vb_236.
+template <int MP, int V, int NW, bool STAGE_ALL = false, bool RHS = false>
 SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
   using LY = typename GaussCfg<MP, V, NW, STAGE_ALL>::LY;
@@ -433,6 +438,9 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   constexpr bool GROUP_OUTER = (V == GV_PAIRS);
   constexpr int NGS = GROUP_OUTER ? 1 : NG;
   const int npass = (V == GV_PAIRS) ? a.npass : 1;
+  static_assert(!RHS || ((V == GV_CROSS_CURL || V == GV_PAIRS) && !STAGE_ALL), "RHS: vector products only");
+  // RHS: the coefficients of every group, kept until the element sums
+  double ro[RHS ? NG : 1][RHS ? 6 : 1][4];
   for (int go = 0; go < (GROUP_OUTER ? NG : 1); ++go) {
     double acc[NGS][3][4][3];   // physical product accumulated over the passes
 #pragma unroll
@@ -581,6 +589,17 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
       for (int t = 0; t < NOC; ++t) poststep<MP>(q[gs][3 * t], q[gs][3 * t + 1], q[gs][3 * t + 2], l, lo[t], hi[t], wN);
       double oa[NOC][2][4], ob[NOC][2][4];
       pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
+      if constexpr (RHS) {
+        // column 2o + h' of component o: cos (h' = 0) and sin (h' = 1)
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            ro[gi][2 * o][r] = oa[o / 2][o % 2][r];
+            ro[gi][2 * o + 1][r] = ob[o / 2][o % 2][r];
+          }
+        continue;
+      }
       if (!pick(valid, gi)) continue;
       const int64_t pg = pick(p, gi);
 #pragma unroll
@@ -604,6 +623,61 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
             }
           }
         }
+    }
+  }
+  if constexpr (RHS) {
+    // red[(lp*L + mi)*6 + k]: V_out column k of local point lp at mode offset
+    // mi of the current slot; wl[lp] = wgt(e, l) of the point.  Both alias
+    // the staging and wave regions, which every wave has finished with at the
+    // first barrier.
+    constexpr int NPT = 7 * E;
+    double* red = big;
+    double* wl = big + NPT * L * 6;
+    static_assert(NPT * L * 6 + NPT <= LY::NODAL + NW * LY::RW, "RHS reduction area");
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      SFX_SYNC();
+#pragma unroll
+      for (int gi = 0; gi < NG; ++gi) {
+        if (!has[gi]) continue;
+        const int lp = (wave + NW * gi) * PW + pt;
+        const bool ok = valid[gi] && l + L * r < a.M;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          SFX_CHECK("rhs red write", (lp * L + l) * 6 + k, NPT * L * 6);
+          red[(lp * L + l) * 6 + k] = ok ? ro[gi][k][r] : 0.0;
+        }
+        if (r == 0 && l == 0) {
+          const int64_t wi = valid[gi] ? 7 * e0 + lp : 0;   // points past the last element read entry 0
+          SFX_CHECK("rhs wgt", wi, 7 * a.me);
+          const double wv = a.wgt[wi];
+          wl[lp] = valid[gi] ? wv : 0.0;
+        }
+      }
+      SFX_SYNC();
+      // item (element e, local node n, mode offset mi): 6 components
+      for (int it = tid; it < E * 6 * L; it += NT) {
+        const int mi = it % L;
+        const int n = (it / L) % 6;
+        const int e = it / (6 * L);
+        const int mm = mi + L * r;
+        const int64_t eg = e0 + e;
+        if (eg >= a.me || mm >= a.M) continue;
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int lg = 0; lg < 7; ++lg) {
+          const double c = tab[n + 6 * lg] * wl[7 * e + lg];
+          const double* src = red + ((7 * e + lg) * L + mi) * 6;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) acc[k] = fma(c, src[k], acc[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
+          SFX_CHECK("rhs elo store", o, a.nelo);
+          a.elo[o] = acc[k];
+        }
+      }
     }
   }
 }

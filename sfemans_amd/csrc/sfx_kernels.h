@@ -49,6 +49,11 @@ struct GaussArgs {
   double om[3];             // precession: 2*Omega added to RotV (mode 1: r_c, theta_s; mode 0: z_c)
   const double2* wN;
   const double2* wMP;
+  // element sums of rhs_3x3 formed in the kernel (the fused nodal -> RHS form):
+  // elo(6e+i, k, m) = SUM_l ww(i,l) wgt(e,l) V_out(7e+l, k, m), no Gauss array
+  const double* wgt;        // (7, me): rj * ray
+  double* elo;              // index ((6e+i)*6 + k)*M + m
+  int64_t nelo;
 };
 
 struct RhsArgs;
@@ -60,6 +65,8 @@ struct GatherArgs;
 #ifndef SFX_EMULATE
 hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, const GatherArgs* ga2, hipStream_t st);
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st);
+hipError_t launch_rhs_gather(const RhsArgs& a, hipStream_t st);
+hipError_t launch_gauss_rhs(const GaussArgs& a, hipStream_t st);
 hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_rhs_bdy(const BdyArgs& a, hipStream_t st);

@@ -168,15 +168,38 @@ struct Passes {
   void add(const double* c, const double* v, double k) { pc[n] = c; pv[n] = v; coef[n] = k; ++n; }
 };
 
-int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M,
-                  const double* U, int ncU, const double* U2, int ncU2, const Passes* ps,
-                  const double* heat_capa_dev, double* out, int ncout, const double om[3], hipStream_t st) {
+// Node-centred assembly arguments of a mesh (element sums -> gather).
+static RhsArgs rhs_args(sfx_mesh_t mesh, int32_t M, double scale) {
+  RhsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.adj_ptr = mesh->adj_ptr; a.adj_ent = mesh->adj_ent; a.wgt = mesh->wgt; a.tab = mesh->tab;
+  a.jj = mesh->jj; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me; a.M = M; a.scale = scale;
+  a.n_adj = mesh->n_adj;
+  return a;
+}
+
+// Mesh, modes and twiddles of a fused Gauss-point launch.
+static int gauss_setup(const char* nm, sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M, GaussArgs& a) {
   if (!mesh) return fail(SFX_ERR_ARG, "%s: null mesh", nm);
   if (M <= 0) return fail(SFX_ERR_ARG, "%s: SIZE(list_mode) == 0", nm);
   if (M > 256) return fail(SFX_ERR_UNSUPPORTED, "%s: m_max_c=%d > 256", nm, M);
-  GaussArgs a;
   std::memset(&a, 0, sizeof(a));
   a.jj = mesh->jj; a.rr = mesh->rr; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me;
+  a.list_mode = list_mode_dev; a.tab = mesh->tab;
+  a.M = M; a.MP = mp_for(M);
+  Twiddles tw;
+  int rc = get_twiddles(a.MP, &tw);
+  if (rc) return rc;
+  a.wN = tw.wN; a.wMP = tw.wMP;
+  return SFX_OK;
+}
+
+int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t M,
+                  const double* U, int ncU, const double* U2, int ncU2, const Passes* ps,
+                  const double* heat_capa_dev, double* out, int ncout, const double om[3], hipStream_t st) {
+  GaussArgs a;
+  int rc = gauss_setup(nm, mesh, list_mode_dev, M, a);
+  if (rc) return rc;
   a.U = U; a.U2 = U2;
   a.nU = mesh->np * ncU * (int64_t)M; a.nU2 = mesh->np * ncU2 * (int64_t)M;
   if (ps) {
@@ -186,14 +209,9 @@ int run_gauss_dev(int variant, const char* nm, sfx_mesh_t mesh, const int32_t* l
       a.npc[i] = a.npv[i] = mesh->np * 6 * (int64_t)M;
     }
   }
-  a.heat_capa = heat_capa_dev; a.list_mode = list_mode_dev; a.tab = mesh->tab;
+  a.heat_capa = heat_capa_dev;
   a.Vout = out; a.nout = a.G * ncout * (int64_t)M;
-  a.M = M; a.MP = mp_for(M);
   for (int i = 0; i < 3; ++i) a.om[i] = om ? om[i] : 0.0;
-  Twiddles tw;
-  int rc = get_twiddles(a.MP, &tw);
-  if (rc) return rc;
-  a.wN = tw.wN; a.wMP = tw.wMP;
   if (mesh->me == 0) return SFX_OK;
   if (variant == GV_PAIRS && pairs_as_passes(a.MP)) {
     // one launch per pass, the first storing, the others adding to the output
@@ -283,6 +301,29 @@ int sfx_smb_cross_prod_gauss_sft_par_dev(sfx_mesh_t mesh, const int32_t* list_mo
   precession_om(precession_in, taux_precession, angle_s_pi, om);
   return run_gauss_dev(GV_CROSS_CURL, "smb_cross_prod_gauss_sft_par", mesh, list_mode_dev, m_max_c, V_in, 6,
                        nullptr, 0, nullptr, nullptr, V_out, 6, om, (hipStream_t)stream);
+}
+
+int sfx_nl_rhs_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* V_in,
+                       double* vb_145, double* vb_236, double scale, int32_t precession_in, double taux_precession,
+                       double angle_s_pi, double* work_elem, void* stream) {
+  clear_err();
+  const char* nm = "nl_rhs_3x3";
+  GaussArgs a;
+  int rc = gauss_setup(nm, mesh, list_mode_dev, m_max_c, a);
+  if (rc) return rc;
+  if (!V_in || !vb_145 || !vb_236) return fail(SFX_ERR_ARG, "%s: null array", nm);
+  if (!work_elem && mesh->me > 0) return fail(SFX_ERR_ARG, "%s: null work_elem", nm);
+  precession_om(precession_in, taux_precession, angle_s_pi, a.om);
+  a.U = V_in; a.nU = mesh->np * 6 * (int64_t)m_max_c;
+  a.wgt = mesh->wgt; a.elo = work_elem; a.nelo = 36 * mesh->me * (int64_t)m_max_c;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = launch_gauss_rhs(a, st);
+  if (e != hipSuccess) return hip_fail(e, nm);
+  RhsArgs r = rhs_args(mesh, m_max_c, scale);
+  r.v145 = vb_145; r.v236 = vb_236; r.elo = work_elem;
+  r.n_out = mesh->np * 3 * (int64_t)m_max_c; r.n_elo = a.nelo;
+  e = launch_rhs_gather(r, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
 
 int sfx_smb_curlh_cross_b_gauss_sft_par(sfx_mesh_t mesh, const int32_t* list_mode, int32_t m_max_c,
@@ -479,15 +520,6 @@ int sfx_nl_tangent_mhd_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_
 
 // ---- right-hand-side assembly entry points
 namespace sfx {
-
-static RhsArgs rhs_args(sfx_mesh_t mesh, int32_t M, double scale) {
-  RhsArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.adj_ptr = mesh->adj_ptr; a.adj_ent = mesh->adj_ent; a.wgt = mesh->wgt; a.tab = mesh->tab;
-  a.jj = mesh->jj; a.me = mesh->me; a.np = mesh->np; a.G = 7 * mesh->me; a.M = M; a.scale = scale;
-  a.n_adj = mesh->n_adj;
-  return a;
-}
 
 int rhs3x3_dev(sfx_mesh_t mesh, int32_t M, const double* rhs, double* v145, double* v236, double scale,
                double* work_elem, hipStream_t st) {

@@ -18,7 +18,7 @@ import numpy as np
 from ._lib import lib, check
 from .fft_parallele import fdims, is_device, ptr, stream_ptr
 
-__all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "smb_CurlH_cross_B_gauss_sft_par",
+__all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "nl_rhs_3x3", "smb_CurlH_cross_B_gauss_sft_par",
            "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes", "rhs_3x3_bdy",
            "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd", "nl_adjoint", "nl_adjoint_lorentz",
            "nl_adjoint_mhd"]
@@ -206,6 +206,23 @@ def rhs_3x3_all_modes(mesh: Mesh, rhs_gauss, vb_145, vb_236, scale=1.0, work_ele
     work_elem = _work_elem(mesh, M, work_elem)
     check(lib().sfx_rhs_3x3_dev(mesh.handle, M, ptr(rhs_gauss), ptr(vb_145), ptr(vb_236), float(scale),
                                 work_elem.data_ptr(), stream_ptr()))
+    return vb_145, vb_236
+
+
+def nl_rhs_3x3(mesh: Mesh, list_mode, V_in, vb_145, vb_236, scale=-1.0, work_elem=None, precession_in=False,
+               taux_precession=0.0, angle_s_pi=0.0):
+    """smb_cross_prod_gauss_sft_par then rhs_3x3 of its output, fused (no Gauss
+    array; device arrays): vb_145(np,3,m), vb_236(np,3,m) = scale *
+    rhs_3x3((curl V_in + 2 Omega) x V_in), scale = -1 for the -rotv_v share of
+    the NS right-hand side (computing_rhs_gauss.F90:100).  Equal, bit for bit,
+    to the two calls."""
+    M = len(list_mode)
+    fdims(V_in), fdims(vb_145), fdims(vb_236)
+    lm = mesh.list_mode_device(list_mode)
+    work_elem = _work_elem(mesh, M, work_elem)
+    check(lib().sfx_nl_rhs_3x3_dev(mesh.handle, lm.data_ptr(), M, ptr(V_in), ptr(vb_145), ptr(vb_236),
+                                   float(scale), int(bool(precession_in)), float(taux_precession),
+                                   float(angle_s_pi), work_elem.data_ptr(), stream_ptr()))
     return vb_145, vb_236
 
 

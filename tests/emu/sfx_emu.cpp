@@ -763,3 +763,59 @@ extern "C" int emu_sanitizer_probe(int which, int n) {
   }
   return s;
 }
+
+// ---- fused nodal -> rhs_3x3 (gauss_body in its RHS form, then the gather):
+// vb_145/vb_236 (np, 3, M) = scale * rhs_3x3((curl u + 2 Omega) x u)
+template <int MP>
+static void emu_nl_rhs_mp(const GaussArgs& a) {
+  constexpr int NW = gauss_waves<MP, GV_CROSS_CURL>();
+  constexpr int E = GaussLds<MP, NW>::E;
+  const int64_t nblk = (a.me + E - 1) / E;
+  run_grid(nblk, 64 * NW, GaussCfg<MP, GV_CROSS_CURL, NW>::LY::TOTAL,
+           [&](int t, int64_t b, double2* s) { gauss_body<MP, GV_CROSS_CURL, NW, false, true>(a, t, b, s); });
+}
+
+extern "C" int emu_nl_rhs(const int32_t* jj, const double* rr, int64_t me, int64_t np, const double* U,
+                          const int32_t* list_mode, int M, double* v145, double* v236, double scale, double om0,
+                          double om1, double om2) {
+  GaussArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.jj = jj; a.rr = rr; a.me = me; a.np = np; a.G = 7 * me;
+  a.U = U; a.nU = np * 6 * (int64_t)M;
+  a.list_mode = list_mode;
+  double tab[126];
+  p2_reference_tables(tab);
+  a.tab = tab;
+  std::vector<double> wgt(7 * me + 1);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  std::vector<double> elo((size_t)36 * me * M + 1, 1e300);   // poisoned: every entry must be written
+  a.wgt = wgt.data(); a.elo = elo.data(); a.nelo = 36 * me * (int64_t)M;
+  a.M = M; a.MP = mp_for(M);
+  a.om[0] = om0; a.om[1] = om1; a.om[2] = om2;
+  std::vector<double2> wN, wMP;
+  host_twiddles(a.MP, wN, wMP);
+  a.wN = wN.data(); a.wMP = wMP.data();
+  switch (a.MP) {
+    case 4: emu_nl_rhs_mp<4>(a); break;
+    case 8: emu_nl_rhs_mp<8>(a); break;
+    case 16: emu_nl_rhs_mp<16>(a); break;
+    case 32: emu_nl_rhs_mp<32>(a); break;
+    case 64: emu_nl_rhs_mp<64>(a); break;
+    case 128: emu_nl_rhs_mp<128>(a); break;
+    case 256: emu_nl_rhs_mp<256>(a); break;
+    default: return 1;
+  }
+  for (int64_t i = 0; i < a.nelo; ++i)
+    if (elo[i] == 1e300) return 3;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> ent;
+  build_adjacency(jj, me, np, ptr, ent);
+  RhsArgs r;
+  std::memset(&r, 0, sizeof(r));
+  r.adj_ptr = ptr.data(); r.adj_ent = ent.data(); r.wgt = wgt.data(); r.tab = tab;
+  r.jj = jj; r.me = me; r.np = np; r.G = 7 * me; r.M = M; r.scale = scale; r.n_adj = (int64_t)ent.size();
+  r.v145 = v145; r.v236 = v236; r.elo = elo.data(); r.n_out = np * 3 * (int64_t)M; r.n_elo = a.nelo;
+  run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<6>() + 1) / 2,
+           [&](int t, int64_t b, double2* sm) { rhs_gather_block(r, b, t, reinterpret_cast<double*>(sm)); });
+  return 0;
+}

@@ -83,6 +83,8 @@ def _load(path: str = LIB):
     L.emu_gauss_values.restype = C.c_int
     L.emu_coldot.argtypes = [vp, vp, i64, i32, vp]
     L.emu_coldot.restype = C.c_int
+    L.emu_nl_rhs.argtypes = [vp, vp, i64, i64, vp, vp, i32, vp, vp, d, d, d, d]
+    L.emu_nl_rhs.restype = C.c_int
     L.emu_sanitizer_probe.argtypes = [i32, i32]
     L.emu_sanitizer_probe.restype = C.c_int
     return L
@@ -137,6 +139,7 @@ def call(fname: str, *args, timeout: float = 300.0, sanitize: bool | None = None
     a, b = ctx.Pipe()
     p = ctx.Process(target=_child, args=(b, fname, args))
     p.start()
+    b.close()   # the child's end: a child that dies then shows as end of file at once
     if not a.poll(timeout):
         p.kill()
         raise TimeoutError(fname)

@@ -97,3 +97,27 @@ def test_tangent_is_symmetrised_bilinear_form(nx, ny, M):
     exp = (smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, U, D)
            + smb.smb_CurlH_cross_B_gauss_sft_par(m.jj, m.rr, modes, D, U))
     assert rel(got, exp) <= TOL
+
+
+@pytest.mark.parametrize("nx,ny,M", [(2, 3, 1), (3, 2, 3), (2, 5, 8), (2, 2, 64), (1, 2, 100)])
+def test_fused_nodal_to_rhs_equals_rhs_3x3_of_the_gauss_output(nx, ny, M):
+    """The fused form (no Gauss array) gives rhs_3x3 of smb_cross_prod_gauss_sft_par's
+    output bit for bit, with the same summation order; and the oracle to 1e-12."""
+    from oracle import assembly
+    m = rectangle_p2(nx, ny)
+    modes = np.arange(M)
+    u = fourier_field(m.np, 6, M, 13, 2.0)
+    lm = np.asarray(modes, dtype=np.int32)
+    v1 = np.zeros((m.np, 3, M), order="F")
+    v2 = np.zeros_like(v1)
+    rc, arrs = emu_runner.call("emu_nl_rhs", m.jj, m.rr, m.me, m.np, u, lm, M, v1, v2, -1.0, 0.0, 0.0, 0.0)
+    assert rc == 0
+    f145, f236 = arrs[-2], arrs[-1]
+    g = run(0, m, u, None, None, None, modes, 6)
+    w1 = np.zeros((m.np, 3, M), order="F")
+    w2 = np.zeros_like(w1)
+    rc, arrs = emu_runner.call("emu_rhs", 0, m.jj, m.rr, m.me, m.np, M, g, w1, w2, None, None, None, None, -1.0)
+    assert rc == 0
+    assert np.array_equal(f145, arrs[3]) and np.array_equal(f236, arrs[4])
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u))
+    assert rel(f145, -e1) <= TOL and rel(f236, -e2) <= TOL

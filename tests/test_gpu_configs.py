@@ -136,6 +136,14 @@ def test_full_step_224x448_m128_at_size():
     assert rel(v145[dev(sel_n)].cpu().numpy(), -e1[loc]) <= TOL
     assert rel(v236[dev(sel_n)].cpu().numpy(), -e2[loc]) <= TOL
 
+    # the same right-hand side fused (no Gauss array): bit for bit
+    print("case full step: fused nodal -> rhs_3x3", flush=True)
+    f145, f236 = fortran_empty((npn, 3, M)), fortran_empty((npn, 3, M))
+    NL.nl_rhs_3x3(mesh, modes, u, f145, f236, -1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(f145, v145) and torch.equal(f236, v236)
+    del f145, f236
+
     # u x B at the nodes: sampled nodes vs the oracle, antisymmetry over all nodes
     print("case full step: u x B at the nodes", flush=True)
     uxb, bxu = fortran_empty((npn, 6, M)), fortran_empty((npn, 6, M))

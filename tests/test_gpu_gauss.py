@@ -350,3 +350,31 @@ def test_rhs_ns_3x3_penalty(modes, tensor, imposed):
     g = assembly.rhs_ns_gauss_3x3(m.jj, m.rr, lm, V1m, ff, P, Pinc, rotv, penal_grid=penal, imposed_vel=imp)
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, g, lm, T)
     assert rel(v1.cpu().numpy(), e1) <= TOL and rel(v2.cpu().numpy(), e2) <= TOL
+
+
+@pytest.mark.parametrize("kind,M,prec", [("rect", 1, False), ("rect", 3, False), ("rect", 8, False),
+                                         ("rect", 16, False), ("rect", 64, False), ("rect", 100, False),
+                                         ("rect", 128, False), ("sector", 5, True)])
+def test_fused_nodal_to_rhs(kind, M, prec):
+    """nl_rhs_3x3 (no Gauss array) equals smb_cross_prod_gauss_sft_par then
+    rhs_3x3 bit for bit, and the oracle to 1e-12."""
+    from oracle import assembly
+    from sfemans_amd.mesh import sector_p2
+    print(f"case fused nodal->rhs {kind} M={M} precession={prec}", flush=True)
+    m = rectangle_p2(4, 5) if kind == "rect" else sector_p2(3, 6)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M) if not prec else np.array([0, 1, 3, 5, 7])
+    taux, ang = (0.3, 0.25) if prec else (0.0, 0.0)
+    u = fourier_field(m.np, 6, len(modes), 14, 2.0)
+    du = fortran_tensor(u)
+    f145, f236 = fortran_empty((m.np, 3, len(modes))), fortran_empty((m.np, 3, len(modes)))
+    NL.nl_rhs_3x3(mesh, modes, du, f145, f236, -1.0, None, prec, taux, ang)
+    g = fortran_empty((m.n_gauss, 6, len(modes)))
+    NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, du, g, prec, taux, ang)
+    w145, w236 = fortran_empty((m.np, 3, len(modes))), fortran_empty((m.np, 3, len(modes)))
+    NL.rhs_3x3_all_modes(mesh, g, w145, w236, -1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(f145, w145) and torch.equal(f236, w236)
+    exp_g = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u, prec, taux, ang)
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, exp_g)
+    assert rel(f145.cpu().numpy(), -e1) <= TOL and rel(f236.cpu().numpy(), -e2) <= TOL
