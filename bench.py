@@ -492,11 +492,9 @@ def wl_full(torch, args):
     rho = level_set_density(torch, m.rr, M)
     ybar, = _fields(torch, G, M, 4.0, (SEED + 8,))
     hc = torch.ones(me, dtype=torch.float64, device="cuda")
-    rotv = FP.fortran_empty((G, 6, M))
     v145, v236 = FP.fortran_empty((npn, 3, M)), FP.fortran_empty((npn, 3, M))
     welem = torch.empty(36 * me * M, dtype=torch.float64, device="cuda")
     uxb = FP.fortran_empty((npn, 6, M))
-    cg = FP.fortran_empty((G, 2, M))
     ff0 = torch.zeros((2 * M, npn), dtype=torch.float64, device="cuda").T
     vT = FP.fortran_empty((npn, 2 * M))
     wq = torch.empty(6 * me * 2 * M, dtype=torch.float64, device="cuda")
@@ -504,19 +502,21 @@ def wl_full(torch, args):
     mten = torch.empty((M, 6, npn, 3), dtype=torch.float64, device="cuda").permute(3, 2, 1, 0)
     dbar = FP.fortran_empty((npn, 6, M))
     nod6, gau6 = npn * 6 * M * 8, G * 6 * M * 8
+    # the two right-hand sides in their fused forms (no Gauss array), the same
+    # values bit for bit as smb_cross_prod_gauss_sft_par then rhs_3x3 and
+    # smb_ugradc_gauss_fft_par then qs_00_gauss
+    vTf = torch.as_strided(vT, (npn, 2, M), (1, npn, 2 * npn))
+    ff0f = torch.as_strided(ff0, (npn, 2, M), (1, npn, 2 * npn))
     phases = [
-        Phase("cross_curl", lambda: NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, rotv),
-              nod6 + gau6, b2_flops(G, M), "gauss_kernel<128,CROSS_CURL>"),
-        Phase("rhs_3x3", lambda: NL.rhs_3x3_all_modes(mesh, rotv, v145, v236, -1.0, welem),
-              gau6 + nod6, G * M * 72, "rhs_elem_kernel + rhs_gather_kernel", overhead=2 * 36 * me * M * 8),
+        Phase("ns_nl_rhs", lambda: NL.nl_rhs_3x3(mesh, modes, u, v145, v236, -1.0, welem),
+              2 * nod6 + me * 80 + npn * 16, b2_flops(G, M) + G * M * 72, "gauss_rhs_kernel<128,CROSS_CURL> + rhs_gather_kernel",
+              overhead=2 * 36 * me * M * 8),
         Phase("u_x_B_nodes", lambda: FP.FFT_PAR_CROSS_PROD_DCL(None, u, B, uxb), 3 * nod6,
               npn * fft_flops_per_point(M, 6, 3, 9), "fftprod_kernel<128,CROSS>"),
-        Phase("u_grad_T", lambda: NL.smb_ugradc_gauss_fft_par(None, mesh, modes, hc, u, T, cg),
-              nod6 + nod6 // 3 + G * 2 * M * 8, G * M * 100 + G * fft_flops_per_point(M, 8, 1, 8), "gauss_kernel<128,UGRADC>"),
-        # (G, 2, M) column-major viewed as (G, 2M) column-major: column c + 2m
-        Phase("qs_00_gauss", lambda: NL.qs_00_gauss(mesh, hc, ff0, torch.as_strided(cg, (G, 2 * M), (1, G)), vT, wq),
-              2 * npn * 2 * M * 8 + G * 2 * M * 8, G * 2 * M * 30,
-              "qs00_elem_kernel + qs00_gather_kernel", overhead=2 * 6 * me * 2 * M * 8),
+        Phase("temperature_nl_rhs", lambda: NL.nl_qs_00(mesh, modes, hc, u, T, vTf, ff0f, -1.0, wq),
+              nod6 + 3 * (nod6 // 3) + me * 80 + npn * 16,
+              G * M * 100 + G * fft_flops_per_point(M, 8, 1, 8) + G * 2 * M * 30,
+              "gauss_rhs_kernel<128,UGRADC> + qs00_gather_kernel", overhead=2 * 6 * me * 2 * M * 8),
         Phase("rho_u", lambda: FP.FFT_SCALAR_VECT_DCL(None, u, rho, rhou, 1), 2 * nod6 + nod6 // 3,
               npn * fft_flops_per_point(M, 4, 3, 3), "fftprod_kernel<128,SVECT1>"),
         Phase("m_tensor_u", lambda: FP.FFT_TENSOR_DCL(None, rhou, u, mten), 5 * nod6,
@@ -526,7 +526,8 @@ def wl_full(torch, args):
               "adjoint_kernel<128> + adjoint_gather_kernel", overhead=2 * 36 * me * M * 8),
     ]
     cfg = {"workload": "full explicit nonlinear step, forward + adjoint, 224x448 P2 mesh (200,704 triangles), "
-                       "m_max=128: (curl u)xu -> rhs_3x3, u x B, u.grad T -> qs_00_gauss, rho u and (rho u) (x) u "
+                       "m_max=128: (curl u)xu -> rhs_3x3 and u.grad T -> qs_00_gauss (fused, no Gauss arrays), u x B, "
+                       "rho u and (rho u) (x) u "
                        "with rho = 1 + 0.5 tanh(phi/0.05) of a sphere, adjoint of the convective term",
            "me": me, "np": npn, "gauss_points": G, "m_max": M, "grid": 3 * mp_of(M)}
     return phases, G * M, cfg, ("full", m, M)

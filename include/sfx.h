@@ -608,6 +608,16 @@ int sfx_qs_00_gauss(sfx_mesh_t mesh, const double* heat_capa, const double* ff, 
  * heat_capa counts as 1, a null ff or ff_gauss as zero. */
 int sfx_qs_00_gauss_dev(sfx_mesh_t mesh, int32_t ncols, const double* heat_capa, const double* ff,
                         const double* ff_gauss, double* work_elem, double* vect, void* stream);
+/* smb_ugradc_gauss_fft_par followed by qs_00_gauss of its output, fused
+ * (subroutine_temperature_with_T.F90:247-350 then fem_rhs_axi.F90:940-985):
+ * vect(np,2,m) = scale * qs_00_gauss(heat_capa, ff, c_gauss) for every mode,
+ * with c_gauss = (hc grad c).u + (div u)(hc c) formed in the work-group and
+ * never stored.  ff (np,2,m), the nodal source, may be null (zero).  Equal,
+ * bit for bit, to the two calls (the second with ncols = 2m).  work_elem:
+ * 12*me*m doubles. */
+int sfx_nl_qs_00_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* heat_capa_dev,
+                     const double* V_in, const double* c_in, const double* ff, double* vect, double scale,
+                     double* work_elem, void* stream);
 
 /* Nonlinear share of the Navier-Stokes right-hand sides of an MHD step,
  * all modes: vb = rhs_3x3(-[(curl u) x u - coeff_lorentz (curl H) x B]).

@@ -99,6 +99,7 @@ SIGNATURES = {
     "sfx_smb_cross_prod_gauss_sft_par": [_vp, _vp, _i32, _vp, _vp, _i32, _d, _d],
     "sfx_smb_cross_prod_gauss_sft_par_dev": [_vp, _vp, _i32, _vp, _vp, _i32, _d, _d, _vp],
     "sfx_nl_rhs_3x3_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _d, _i32, _d, _d, _vp, _vp],
+    "sfx_nl_qs_00_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp],
     "sfx_smb_curlh_cross_b_gauss_sft_par": [_vp, _vp, _i32, _vp, _vp, _vp],
     "sfx_smb_curlh_cross_b_gauss_sft_par_dev": [_vp, _vp, _i32, _vp, _vp, _vp, _vp],
     "sfx_smb_ugradc_gauss_fft_par": [_vp, _vp, _i32, _vp, _vp, _vp, _vp],

@@ -60,7 +60,8 @@ struct AdjLds {
   static constexpr int RWS = 64 * (NW <= 4 ? 24 : 12);
   static constexpr int RW = RW0 > RWS ? RW0 : RWS;
   static_assert(RW >= 2 * GX * Shape<MP>::WAVE_CPLX, "exchange region too small");
-  static constexpr int NODAL = 2 * LY::NOD6;                              // doubles from BIG
+  // nodal staging: two alternating chunk buffers of the base field(s)
+  static constexpr int NODAL = (SPLIT ? 4 : 2) * LY::NOD6;                 // doubles from BIG
   static constexpr int XCH = NODAL;                                       // first wave region
   // Element sums by local node (work item: element, node, mode offset)
   // for the convective form on 7 waves, else by component (element,
@@ -246,22 +247,24 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     if (SPLIT) nodal_fetch<6, MP, NW>(a, A.UB, A.nUB, e0, 0, tid, nid, fb);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      SFX_SYNC();
-      nodal_store<6, MP, NW>(fa, tid, big, LY::ROW6);
-      if (SPLIT) nodal_store<6, MP, NW>(fb, tid, big + LY::NOD6, LY::ROW6);
-      SFX_SYNC();
+      // chunk c into buffer c % 2 (chunk c - 2 there was interpolated by
+      // every wave before the last barrier), fetch chunk c + 1, one barrier
+      double* nodA = big + (c & 1) * (SPLIT ? 2 : 1) * LY::NOD6;
+      nodal_store<6, MP, NW>(fa, tid, nodA, LY::ROW6);
+      if (SPLIT) nodal_store<6, MP, NW>(fb, tid, nodA + LY::NOD6, LY::ROW6);
       if (c < 3) {
         nodal_fetch<6, MP, NW>(a, a.U, a.nU, e0, c + 1, tid, nid, fa);
         if (SPLIT) nodal_fetch<6, MP, NW>(a, A.UB, A.nUB, e0, c + 1, tid, nid, fb);
       }
+      SFX_SYNC();
       const int m = c * L + l;
 #pragma unroll
       for (int gi = 0; gi < NG; ++gi) {
         if (!has[gi]) continue;
         double own[12];
         if (m < a.M && valid[gi]) {
-          if (SPLIT) pass_fields<GV_PAIRS, MP>(a, big, big + LY::NOD6, 6 * el[gi], l, lmode[m], g[gi], 1.0, own);
-          else pass_fields<GV_CROSS_CURL, MP>(a, big, nullptr, 6 * el[gi], l, lmode[m], g[gi], 1.0, own);
+          if (SPLIT) pass_fields<GV_PAIRS, MP>(a, nodA, nodA + LY::NOD6, 6 * el[gi], l, lmode[m], g[gi], 1.0, own);
+          else pass_fields<GV_CROSS_CURL, MP>(a, nodA, nullptr, 6 * el[gi], l, lmode[m], g[gi], 1.0, own);
         } else {
 #pragma unroll
           for (int f = 0; f < 12; ++f) own[f] = 0.0;

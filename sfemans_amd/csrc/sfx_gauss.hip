@@ -37,35 +37,43 @@ static hipError_t launch_v(const GaussArgs& a, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
-// The fused nodal -> rhs_3x3 element sums (RHS form of the body).
-template <int MP, int NW = gauss_waves<MP, GV_CROSS_CURL>()>
+// The fused nodal -> assembly element sums (RHS form of the body):
+// cross-curl -> rhs_3x3, u.grad c -> qs_00_gauss.
+template <int MP, int V, int NW = gauss_waves<MP, V>()>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(waves_per_simd(NW), waves_per_simd(NW))))
 gauss_rhs_kernel(GaussArgs a) {
   extern __shared__ double2 smem[];
-  gauss_body<MP, GV_CROSS_CURL, NW, false, true>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+  gauss_body<MP, V, NW, false, true>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }
 
-template <int MP>
+template <int MP, int V>
 static hipError_t launch_rhs_mp(const GaussArgs& a, hipStream_t st) {
-  constexpr int NW = gauss_waves<MP, GV_CROSS_CURL>();
+  constexpr int NW = gauss_waves<MP, V>();
   constexpr int E = GaussLds<MP, NW>::E;
   const int64_t nblk = (a.me + E - 1) / E;
   if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)GaussCfg<MP, GV_CROSS_CURL, NW>::LY::TOTAL * sizeof(double2);
-  hipLaunchKernelGGL((gauss_rhs_kernel<MP>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
+  const size_t shmem = (size_t)GaussCfg<MP, V, NW>::LY::TOTAL * sizeof(double2);
+  hipLaunchKernelGGL((gauss_rhs_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_gauss_rhs(const GaussArgs& a, hipStream_t st) {
+template <int V>
+static hipError_t launch_rhs_v(const GaussArgs& a, hipStream_t st) {
   switch (a.MP) {
-    case 4: return launch_rhs_mp<4>(a, st);
-    case 8: return launch_rhs_mp<8>(a, st);
-    case 16: return launch_rhs_mp<16>(a, st);
-    case 32: return launch_rhs_mp<32>(a, st);
-    case 64: return launch_rhs_mp<64>(a, st);
-    case 128: return launch_rhs_mp<128>(a, st);
-    case 256: return launch_rhs_mp<256>(a, st);
+    case 4: return launch_rhs_mp<4, V>(a, st);
+    case 8: return launch_rhs_mp<8, V>(a, st);
+    case 16: return launch_rhs_mp<16, V>(a, st);
+    case 32: return launch_rhs_mp<32, V>(a, st);
+    case 64: return launch_rhs_mp<64, V>(a, st);
+    case 128: return launch_rhs_mp<128, V>(a, st);
+    case 256: return launch_rhs_mp<256, V>(a, st);
   }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_gauss_rhs(int variant, const GaussArgs& a, hipStream_t st) {
+  if (variant == GV_CROSS_CURL) return launch_rhs_v<GV_CROSS_CURL>(a, st);
+  if (variant == GV_UGRADC) return launch_rhs_v<GV_UGRADC>(a, st);
   return hipErrorInvalidValue;
 }
 

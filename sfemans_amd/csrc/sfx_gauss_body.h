@@ -88,8 +88,9 @@ struct GaussLds {
 constexpr int GAUSS_LDS_BYTES = 160 * 1024;
 
 // Layout of the fused kernel for variant V.  By default the nodal data is
-// staged one chunk at a time with the next chunk prefetched into registers,
-// so its loads overlap the interpolation of the current one.  With
+// staged one chunk at a time into two alternating buffers, with the next
+// chunk prefetched into registers, so its loads overlap the interpolation
+// of the current one and one barrier per chunk suffices.  With
 // STAGE_ALL (used by the batched launch, measured faster there) all four
 // chunks are staged at once where that fits in LDS and the variant makes a
 // single pass over its fields: one barrier, every load in flight together.
@@ -98,7 +99,9 @@ struct GaussCfg {
   static constexpr int NCB = (V == GV_UGRADC) ? 2 : ((V == GV_PAIRS || V == GV_CURL_CROSS) ? 6 : 0);
   static constexpr bool ALL =
       STAGE_ALL && V != GV_PAIRS && GaussLds<MP, NW, 4, NCB>::TOTAL * 16 <= GAUSS_LDS_BYTES;
-  using LY = GaussLds<MP, NW, ALL ? 4 : 1, ALL ? NCB : 6>;
+  // chunk buffers when not staging all at once: two where they fit
+  static constexpr int NBUF = GaussLds<MP, NW, 2, NCB>::TOTAL * 16 <= GAUSS_LDS_BYTES ? 2 : 1;
+  using LY = GaussLds<MP, NW, ALL ? 4 : NBUF, NCB>;
 };
 
 struct GaussGeom {
@@ -120,11 +123,12 @@ SFX_DEV void interp_mode(const double* nod, int s0, int row, int mi, const Gauss
   for (int k = 0; k < NC; ++k) { W[k] = 0; Dx[k] = 0; Dy[k] = 0; }
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
-    const double* v = nod + (s0 + n) * row + mi * NC;
+    // column-major slot: the lanes of a point read consecutive modes
+    const double* v = nod + (s0 + n) * row + mi;
     const double wn = g.w[n], dx = g.d[2 * n], dy = g.d[2 * n + 1];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      const double x = v[k];
+      const double x = v[k * (MP / 4)];
       W[k] = fma(wn, x, W[k]);
       Dx[k] = fma(dx, x, Dx[k]);
       Dy[k] = fma(dy, x, Dy[k]);
@@ -231,7 +235,8 @@ SFX_DEV void nodal_fetch(const GaussArgs& a, const double* X, int64_t nX, int64_
   }
 }
 
-// Store a fetched chunk to LDS: nod[slot*row + mi*NC + k].
+// Store a fetched chunk to LDS: nod[slot*row + k*L + mi] (columns of L
+// modes, so that the lanes of a point read consecutive doubles).
 template <int NC, int MP, int NW>
 SFX_DEV void nodal_store(const NodalFetch<NC, MP, NW>& f, int tid, double* nod, int row) {
   using NF_ = NodalFetch<NC, MP, NW>;
@@ -242,7 +247,7 @@ SFX_DEV void nodal_store(const NodalFetch<NC, MP, NW>& f, int tid, double* nod, 
     if (idx < NF_::TOT) {
       const int s = idx % S;
       const int rest = idx / S;
-      const int li = s * row + (rest / NC) * NC + rest % NC;
+      const int li = s * row + (rest % NC) * (MP / 4) + rest / NC;
       SFX_CHECK("nodal stage", li, S * row);
       nod[li] = f.v[it];
     }
@@ -250,16 +255,17 @@ SFX_DEV void nodal_store(const NodalFetch<NC, MP, NW>& f, int tid, double* nod, 
 }
 
 // Exchange NF doubles per lane with the partner lane (mode MP - m <-> m).
+// Layout xf[f * 64 + lane]: consecutive lanes, consecutive doubles.
 template <int NF, int MP>
 SFX_DEV void partner_swap(const double (&own)[NF], double (&par)[NF], int pt, int l, double* xf) {
   constexpr int L = MP / 4;
-  const int me_ = (pt * L + l) * NF;
-  const int pa = (pt * L + ((L - l) % L)) * NF;
+  const int me_ = pt * L + l;
+  const int pa = pt * L + ((L - l) % L);
 #pragma unroll
-  for (int f = 0; f < NF; ++f) { SFX_CHECK("field swap", me_ + f, 64 * NF); xf[me_ + f] = own[f]; }
+  for (int f = 0; f < NF; ++f) { SFX_CHECK("field swap", f * 64 + me_, 64 * NF); xf[f * 64 + me_] = own[f]; }
   SFX_WAVE_SYNC();
 #pragma unroll
-  for (int f = 0; f < NF; ++f) { SFX_CHECK("field swap", pa + f, 64 * NF); par[f] = xf[pa + f]; }
+  for (int f = 0; f < NF; ++f) { SFX_CHECK("field swap", f * 64 + pa, 64 * NF); par[f] = xf[f * 64 + pa]; }
   SFX_WAVE_SYNC();
 }
 
@@ -377,10 +383,14 @@ SFX_DEV void gauss_prologue(const GaussArgs& a, int tid, int64_t e0, double2* sm
 }
 
 // RHS: instead of storing V_out at the Gauss points, form the element sums
-// of rhs_3x3 (assembling_rhs.F90:73-95) in the work-group,
-//   elo(6e+i, k, m) = SUM_l (ww(i,l) wgt(e,l)) V_out(7e+l, k, m),
-// in the order of rhs_elem_body, through LDS one mode slot at a time; a
-// gather (rhs_gather_block) then sums the nodes' entries into vb_145/vb_236.
+// of the assembly in the work-group, through LDS one mode slot at a time,
+// in the order of the element kernels; a gather then sums the nodes'
+// entries.  Vector products: rhs_3x3 (assembling_rhs.F90:73-95, as
+// rhs_elem_body), elo(6e+i, k, m) = SUM_l (ww(i,l) wgt(e,l)) V_out(7e+l, k, m),
+// gathered into vb_145/vb_236 (rhs_gather_block).  u.grad c: qs_00_gauss
+// of the Gauss source alone (fem_rhs_axi.F90:958-975, as qs00_elem_body),
+// elo(6e+i, k + 2m) = SUM_l ww(i,l) (c_out(7e+l, k, m) wgt(e,l)), gathered
+// into vect(np, 2, m) (qs00_gather_block).
 template <int MP, int V, int NW, bool STAGE_ALL = false, bool RHS = false>
 SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem) {
   using S = Shape<MP>;
@@ -438,9 +448,10 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   constexpr bool GROUP_OUTER = (V == GV_PAIRS);
   constexpr int NGS = GROUP_OUTER ? 1 : NG;
   const int npass = (V == GV_PAIRS) ? a.npass : 1;
-  static_assert(!RHS || ((V == GV_CROSS_CURL || V == GV_PAIRS) && !STAGE_ALL), "RHS: vector products only");
-  // RHS: the coefficients of every group, kept until the element sums
-  double ro[RHS ? NG : 1][RHS ? 6 : 1][4];
+  static_assert(!RHS || ((V == GV_CROSS_CURL || V == GV_UGRADC) && !STAGE_ALL), "RHS: cross-curl or u.grad c");
+  // RHS: the coefficients of every group (NKR columns), kept until the element sums
+  constexpr int NKR = 2 * NO;
+  double ro[RHS ? NG : 1][RHS ? NKR : 1][4];
   for (int go = 0; go < (GROUP_OUTER ? NG : 1); ++go) {
     double acc[NGS][3][4][3];   // physical product accumulated over the passes
 #pragma unroll
@@ -500,21 +511,25 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         nodal_fetch<6, MP, NW>(a, XA, nXA, e0, 0, tid, nid, fa[0]);
         if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, 0, tid, nid, fb[0]);
       }
+      constexpr int NBUF = GaussCfg<MP, V, NW, STAGE_ALL>::NBUF;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        double* nodA = big + (ALL ? c * LY::NODC : 0);
+        double* nodA = big + (ALL ? c : (NBUF == 2 ? (c & 1) : 0)) * LY::NODC;
         double* nodB = nodA + LY::NOD6;
         if constexpr (!ALL) {
-          // stage nodal chunk c once every wave is done with chunk c-1, then
-          // start fetching chunk c+1 while chunk c is interpolated
-          SFX_SYNC();
+          // two buffers: chunk c into buffer c % 2 (its previous chunk, c - 2,
+          // was interpolated by every wave before the last barrier), start
+          // fetching chunk c + 1, and one barrier makes chunk c visible.  One
+          // buffer: wait until every wave is done with chunk c - 1 first.
+          if (NBUF == 1) SFX_SYNC();
           nodal_store<6, MP, NW>(fa[0], tid, nodA, LY::ROW6);
           if (useB) nodal_store<NCB, MP, NW>(fb[0], tid, nodB, NCB == 6 ? LY::ROW6 : LY::ROW2);
-          SFX_SYNC();
+          if (NBUF == 1) SFX_SYNC();
           if (c < 3) {
             nodal_fetch<6, MP, NW>(a, XA, nXA, e0, c + 1, tid, nid, fa[0]);
             if (useB) nodal_fetch<NCB, MP, NW>(a, XB, nXB, e0, c + 1, tid, nid, fb[0]);
           }
+          if (NBUF == 2) SFX_SYNC();
         }
         const int m = c * L + l;
 #pragma unroll
@@ -592,7 +607,7 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
       if constexpr (RHS) {
         // column 2o + h' of component o: cos (h' = 0) and sin (h' = 1)
 #pragma unroll
-        for (int o = 0; o < 3; ++o)
+        for (int o = 0; o < NO; ++o)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             ro[gi][2 * o][r] = oa[o / 2][o % 2][r];
@@ -626,14 +641,14 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
     }
   }
   if constexpr (RHS) {
-    // red[(lp*L + mi)*6 + k]: V_out column k of local point lp at mode offset
-    // mi of the current slot; wl[lp] = wgt(e, l) of the point.  Both alias
-    // the staging and wave regions, which every wave has finished with at the
+    // red[(lp*L + mi)*NKR + k]: column k of local point lp at mode offset mi
+    // of the current slot; wl[lp] = wgt(e, l) of the point.  Both alias the
+    // staging and wave regions, which every wave has finished with at the
     // first barrier.
     constexpr int NPT = 7 * E;
     double* red = big;
-    double* wl = big + NPT * L * 6;
-    static_assert(NPT * L * 6 + NPT <= LY::NODAL + NW * LY::RW, "RHS reduction area");
+    double* wl = big + NPT * L * NKR;
+    static_assert(NPT * L * NKR + NPT <= LY::NODAL + NW * LY::RW, "RHS reduction area");
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       SFX_SYNC();
@@ -643,9 +658,9 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         const int lp = (wave + NW * gi) * PW + pt;
         const bool ok = valid[gi] && l + L * r < a.M;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          SFX_CHECK("rhs red write", (lp * L + l) * 6 + k, NPT * L * 6);
-          red[(lp * L + l) * 6 + k] = ok ? ro[gi][k][r] : 0.0;
+        for (int k = 0; k < NKR; ++k) {
+          SFX_CHECK("rhs red write", (lp * L + l) * NKR + k, NPT * L * NKR);
+          red[(lp * L + l) * NKR + k] = ok ? ro[gi][k][r] : 0.0;
         }
         if (r == 0 && l == 0) {
           const int64_t wi = valid[gi] ? 7 * e0 + lp : 0;   // points past the last element read entry 0
@@ -663,17 +678,44 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         const int mm = mi + L * r;
         const int64_t eg = e0 + e;
         if (eg >= a.me || mm >= a.M) continue;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
+        double acc[NKR];
 #pragma unroll
-        for (int lg = 0; lg < 7; ++lg) {
-          const double c = tab[n + 6 * lg] * wl[7 * e + lg];
-          const double* src = red + ((7 * e + lg) * L + mi) * 6;
+        for (int k = 0; k < NKR; ++k) acc[k] = 0.0;
+        // qs_00_gauss's nodal source at the element's nodes, and hc(e)
+        double ffl[V == GV_UGRADC ? NKR : 1][6];
+        double hce = 1.0;
+        if constexpr (V == GV_UGRADC) {
+          hce = a.heat_capa[eg];
 #pragma unroll
-          for (int k = 0; k < 6; ++k) acc[k] = fma(c, src[k], acc[k]);
+          for (int k = 0; k < NKR; ++k)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+              const int64_t fi = (int64_t)nid[6 * e + j] + a.np * (int64_t)(k + 2 * mm);
+              SFX_CHECK("rhs ff load", fi, a.ffn ? a.nffn : 1 << 30);
+              ffl[k][j] = a.ffn ? a.ffn[fi] : 0.0;
+            }
         }
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
+        for (int lg = 0; lg < 7; ++lg) {
+          const double* src = red + ((7 * e + lg) * L + mi) * NKR;
+          if constexpr (V == GV_UGRADC) {
+#pragma unroll
+            for (int k = 0; k < NKR; ++k) {
+              double sf = 0;
+#pragma unroll
+              for (int j = 0; j < 6; ++j) sf = fma(ffl[k][j], tab[j + 6 * lg], sf);
+              acc[k] = fma(tab[n + 6 * lg], (hce * sf + src[k]) * wl[7 * e + lg], acc[k]);
+            }
+          } else {
+            const double c = tab[n + 6 * lg] * wl[7 * e + lg];
+#pragma unroll
+            for (int k = 0; k < NKR; ++k) acc[k] = fma(c, src[k], acc[k]);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NKR; ++k) {
+          const int64_t o = (V == GV_UGRADC) ? (6 * eg + n) * (2 * (int64_t)a.M) + k + 2 * mm
+                                             : ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
           SFX_CHECK("rhs elo store", o, a.nelo);
           a.elo[o] = acc[k];
         }

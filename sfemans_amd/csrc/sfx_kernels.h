@@ -52,8 +52,10 @@ struct GaussArgs {
   // element sums of rhs_3x3 formed in the kernel (the fused nodal -> RHS form):
   // elo(6e+i, k, m) = SUM_l ww(i,l) wgt(e,l) V_out(7e+l, k, m), no Gauss array
   const double* wgt;        // (7, me): rj * ray
-  double* elo;              // index ((6e+i)*6 + k)*M + m
+  double* elo;              // index ((6e+i)*6 + k)*M + m (u.grad c: (6e+i)*2M + k + 2m)
   int64_t nelo;
+  const double* ffn;        // u.grad c fused to qs_00_gauss: its nodal source (np, 2, M), or null
+  int64_t nffn;
 };
 
 struct RhsArgs;
@@ -66,7 +68,8 @@ struct GatherArgs;
 hipError_t launch_adjoint(const AdjArgs& a, const GatherArgs& ga, const GatherArgs* ga2, hipStream_t st);
 hipError_t launch_rhs3x3(const RhsArgs& a, hipStream_t st);
 hipError_t launch_rhs_gather(const RhsArgs& a, hipStream_t st);
-hipError_t launch_gauss_rhs(const GaussArgs& a, hipStream_t st);
+hipError_t launch_gauss_rhs(int variant, const GaussArgs& a, hipStream_t st);
+hipError_t launch_qs00_gather(const RhsArgs& a, hipStream_t st);
 hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st);
 hipError_t launch_qs00(const RhsArgs& a, hipStream_t st);
 hipError_t launch_rhs_bdy(const BdyArgs& a, hipStream_t st);

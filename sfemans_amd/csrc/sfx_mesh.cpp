@@ -317,12 +317,38 @@ int sfx_nl_rhs_3x3_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_
   a.U = V_in; a.nU = mesh->np * 6 * (int64_t)m_max_c;
   a.wgt = mesh->wgt; a.elo = work_elem; a.nelo = 36 * mesh->me * (int64_t)m_max_c;
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e = launch_gauss_rhs(a, st);
+  hipError_t e = launch_gauss_rhs(GV_CROSS_CURL, a, st);
   if (e != hipSuccess) return hip_fail(e, nm);
   RhsArgs r = rhs_args(mesh, m_max_c, scale);
   r.v145 = vb_145; r.v236 = vb_236; r.elo = work_elem;
   r.n_out = mesh->np * 3 * (int64_t)m_max_c; r.n_elo = a.nelo;
   e = launch_rhs_gather(r, st);
+  return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
+}
+
+int sfx_nl_qs_00_dev(sfx_mesh_t mesh, const int32_t* list_mode_dev, int32_t m_max_c, const double* heat_capa_dev,
+                     const double* V_in, const double* c_in, const double* ff, double* vect, double scale,
+                     double* work_elem, void* stream) {
+  clear_err();
+  const char* nm = "nl_qs_00";
+  GaussArgs a;
+  int rc = gauss_setup(nm, mesh, list_mode_dev, m_max_c, a);
+  if (rc) return rc;
+  if (!heat_capa_dev) return fail(SFX_ERR_ARG, "%s: heat_capa is required", nm);
+  if (!V_in || !c_in || !vect) return fail(SFX_ERR_ARG, "%s: null array", nm);
+  if (!work_elem && mesh->me > 0) return fail(SFX_ERR_ARG, "%s: null work_elem", nm);
+  a.U = V_in; a.nU = mesh->np * 6 * (int64_t)m_max_c;
+  a.U2 = c_in; a.nU2 = mesh->np * 2 * (int64_t)m_max_c;
+  a.heat_capa = heat_capa_dev;
+  a.ffn = ff; a.nffn = mesh->np * 2 * (int64_t)m_max_c;
+  a.wgt = mesh->wgt; a.elo = work_elem; a.nelo = 6 * mesh->me * 2 * (int64_t)m_max_c;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = launch_gauss_rhs(GV_UGRADC, a, st);
+  if (e != hipSuccess) return hip_fail(e, nm);
+  RhsArgs r = rhs_args(mesh, 2 * m_max_c, scale);
+  r.vect = vect; r.elo = work_elem;
+  r.n_out = mesh->np * 2 * (int64_t)m_max_c; r.n_elo = a.nelo;
+  e = launch_qs00_gather(r, st);
   return e == hipSuccess ? SFX_OK : hip_fail(e, nm);
 }
 

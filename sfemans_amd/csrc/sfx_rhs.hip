@@ -54,6 +54,14 @@ hipError_t launch_rhs_gather(const RhsArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// The qs_00_gauss gather alone: element contributions already in a.elo.
+hipError_t launch_qs00_gather(const RhsArgs& a, hipStream_t st) {
+  if (a.np * a.M == 0) return hipSuccess;
+  const int64_t nb = (a.np + GATHER_NODES - 1) / GATHER_NODES;
+  hipLaunchKernelGGL(qs00_gather_kernel, dim3((unsigned)nb), dim3(GATHER_THREADS), 0, st, a);
+  return hipGetLastError();
+}
+
 // Full NS right-hand side fused with its element share, then the gather.
 hipError_t launch_rhs_ns(const RhsArgs& a, hipStream_t st) {
   const int64_t ne = a.me > 0 && a.M > 0 ? elem_items(a.me, a.M) : 0, nn = a.np * a.M;

@@ -18,7 +18,7 @@ import numpy as np
 from ._lib import lib, check
 from .fft_parallele import fdims, is_device, ptr, stream_ptr
 
-__all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "nl_rhs_3x3", "smb_CurlH_cross_B_gauss_sft_par",
+__all__ = ["Mesh", "smb_cross_prod_gauss_sft_par", "nl_rhs_3x3", "nl_qs_00", "smb_CurlH_cross_B_gauss_sft_par",
            "smb_ugradc_gauss_fft_par", "nl_rotv_mhd", "rhs_3x3", "rhs_3x3_all_modes", "rhs_3x3_bdy",
            "qs_00_gauss", "nl_rhs_mhd", "nl_tangent", "nl_tangent_mhd", "nl_adjoint", "nl_adjoint_lorentz",
            "nl_adjoint_mhd"]
@@ -226,6 +226,23 @@ def nl_rhs_3x3(mesh: Mesh, list_mode, V_in, vb_145, vb_236, scale=-1.0, work_ele
     return vb_145, vb_236
 
 
+def nl_qs_00(mesh: Mesh, list_mode, heat_capa, V_in, c_in, vect, ff=None, scale=-1.0, work_elem=None):
+    """smb_ugradc_gauss_fft_par then qs_00_gauss of its output, fused (no Gauss
+    array; device arrays): vect(np,2,m) = scale * qs_00_gauss(heat_capa, ff,
+    (hc grad c).u + (div u)(hc c)), ff (np,2,m) the nodal source or None.
+    Equal, bit for bit, to the two calls."""
+    import torch
+    M = len(list_mode)
+    fdims(V_in), fdims(c_in), fdims(vect)
+    lm = mesh.list_mode_device(list_mode)
+    if work_elem is None:
+        work_elem = torch.empty(12 * mesh.me * M, dtype=torch.float64, device="cuda")
+    check(lib().sfx_nl_qs_00_dev(mesh.handle, lm.data_ptr(), M, ptr(heat_capa), ptr(V_in), ptr(c_in),
+                                 None if ff is None else ptr(ff), ptr(vect), float(scale), work_elem.data_ptr(),
+                                 stream_ptr()))
+    return vect
+
+
 def rhs_ns_3x3(mesh: Mesh, list_mode, rotv_v, V1m, ff, P, P_inc, vb_145, vb_236, work_elem=None,
                opt_tensor=None, penal_grid=None, imposed_vel=None, work_gauss=None):
     """Full explicit NS right-hand side assembled for every mode (device arrays).
@@ -267,9 +284,9 @@ def qs_00_gauss(mesh: Mesh, heat_capa, ff, ff_gauss, vect, work_elem=None):
 
     Device path: ``work_elem`` is a (6*me*ncols) float64 CUDA buffer (allocated when omitted).
     """
-    if is_device(ff):
+    if is_device(vect):
         import torch
-        nc = ff.shape[1] if ff.dim() > 1 else 1
+        nc = vect.shape[1] if vect.dim() > 1 else 1
         if work_elem is None:
             work_elem = torch.empty(6 * mesh.me * nc, dtype=torch.float64, device="cuda")
         check(lib().sfx_qs_00_gauss_dev(mesh.handle, nc, ptr(heat_capa), ptr(ff), ptr(ff_gauss),

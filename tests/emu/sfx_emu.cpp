@@ -819,3 +819,60 @@ extern "C" int emu_nl_rhs(const int32_t* jj, const double* rr, int64_t me, int64
            [&](int t, int64_t b, double2* sm) { rhs_gather_block(r, b, t, reinterpret_cast<double*>(sm)); });
   return 0;
 }
+
+// ---- fused u.grad c -> qs_00_gauss (gauss_body in its RHS form, then the gather):
+// vect (np, 2, M) = scale * qs_00_gauss(hc, ff, (hc grad c).u + (div u)(hc c))
+template <int MP>
+static void emu_nl_qs00_mp(const GaussArgs& a) {
+  constexpr int NW = gauss_waves<MP, GV_UGRADC>();
+  constexpr int E = GaussLds<MP, NW>::E;
+  const int64_t nblk = (a.me + E - 1) / E;
+  run_grid(nblk, 64 * NW, GaussCfg<MP, GV_UGRADC, NW>::LY::TOTAL,
+           [&](int t, int64_t b, double2* s) { gauss_body<MP, GV_UGRADC, NW, false, true>(a, t, b, s); });
+}
+
+extern "C" int emu_nl_qs00(const int32_t* jj, const double* rr, int64_t me, int64_t np, const double* U,
+                           const double* c, const double* hc, const double* ff, const int32_t* list_mode, int M,
+                           double* vect, double scale) {
+  GaussArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.jj = jj; a.rr = rr; a.me = me; a.np = np; a.G = 7 * me;
+  a.U = U; a.nU = np * 6 * (int64_t)M;
+  a.U2 = c; a.nU2 = np * 2 * (int64_t)M;
+  a.heat_capa = hc; a.ffn = ff; a.nffn = np * 2 * (int64_t)M;
+  a.list_mode = list_mode;
+  double tab[126];
+  p2_reference_tables(tab);
+  a.tab = tab;
+  std::vector<double> wgt(7 * me + 1);
+  p2_assembly_weights(jj, rr, me, tab, wgt.data());
+  std::vector<double> elo((size_t)12 * me * M + 1, 1e300);   // poisoned: every entry must be written
+  a.wgt = wgt.data(); a.elo = elo.data(); a.nelo = 12 * me * (int64_t)M;
+  a.M = M; a.MP = mp_for(M);
+  std::vector<double2> wN, wMP;
+  host_twiddles(a.MP, wN, wMP);
+  a.wN = wN.data(); a.wMP = wMP.data();
+  switch (a.MP) {
+    case 4: emu_nl_qs00_mp<4>(a); break;
+    case 8: emu_nl_qs00_mp<8>(a); break;
+    case 16: emu_nl_qs00_mp<16>(a); break;
+    case 32: emu_nl_qs00_mp<32>(a); break;
+    case 64: emu_nl_qs00_mp<64>(a); break;
+    case 128: emu_nl_qs00_mp<128>(a); break;
+    case 256: emu_nl_qs00_mp<256>(a); break;
+    default: return 1;
+  }
+  for (int64_t i = 0; i < a.nelo; ++i)
+    if (elo[i] == 1e300) return 3;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> ent;
+  build_adjacency(jj, me, np, ptr, ent);
+  RhsArgs r;
+  std::memset(&r, 0, sizeof(r));
+  r.adj_ptr = ptr.data(); r.adj_ent = ent.data(); r.wgt = wgt.data(); r.tab = tab;
+  r.jj = jj; r.me = me; r.np = np; r.G = 7 * me; r.M = 2 * M; r.scale = scale; r.n_adj = (int64_t)ent.size();
+  r.vect = vect; r.elo = elo.data(); r.n_out = np * 2 * (int64_t)M; r.n_elo = a.nelo;
+  run_grid((np + GATHER_NODES - 1) / GATHER_NODES, GATHER_THREADS, (gather_tile_doubles<1>() + 1) / 2,
+           [&](int t, int64_t b, double2* sm) { qs00_gather_block(r, b, t, reinterpret_cast<double*>(sm)); });
+  return 0;
+}

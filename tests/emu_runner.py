@@ -85,6 +85,8 @@ def _load(path: str = LIB):
     L.emu_coldot.restype = C.c_int
     L.emu_nl_rhs.argtypes = [vp, vp, i64, i64, vp, vp, i32, vp, vp, d, d, d, d]
     L.emu_nl_rhs.restype = C.c_int
+    L.emu_nl_qs00.argtypes = [vp, vp, i64, i64, vp, vp, vp, vp, vp, i32, vp, d]
+    L.emu_nl_qs00.restype = C.c_int
     L.emu_sanitizer_probe.argtypes = [i32, i32]
     L.emu_sanitizer_probe.restype = C.c_int
     return L

@@ -121,3 +121,35 @@ def test_fused_nodal_to_rhs_equals_rhs_3x3_of_the_gauss_output(nx, ny, M):
     assert np.array_equal(f145, arrs[3]) and np.array_equal(f236, arrs[4])
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u))
     assert rel(f145, -e1) <= TOL and rel(f236, -e2) <= TOL
+
+
+@pytest.mark.parametrize("nx,ny,M,with_ff", [(2, 3, 1, True), (3, 2, 3, False), (2, 5, 8, True), (2, 2, 64, True),
+                                             (1, 2, 100, False)])
+def test_fused_ugradc_to_qs_00_equals_qs_00_of_the_gauss_output(nx, ny, M, with_ff):
+    """u.grad c fused to qs_00_gauss (no Gauss array) gives qs_00_gauss of
+    smb_ugradc_gauss_fft_par's output bit for bit; and the oracle to 1e-12."""
+    from oracle import assembly
+    m = rectangle_p2(nx, ny)
+    modes = np.arange(M)
+    rng = np.random.default_rng(17)
+    u = fourier_field(m.np, 6, M, 15, 2.0)
+    c = fourier_field(m.np, 2, M, 16, 2.0)
+    hc = rng.uniform(0.5, 2.0, m.me)
+    ff = np.asfortranarray(rng.standard_normal((m.np, 2, M))) if with_ff else None
+    lm = np.asarray(modes, dtype=np.int32)
+    v = np.zeros((m.np, 2, M), order="F")
+    rc, arrs = emu_runner.call("emu_nl_qs00", m.jj, m.rr, m.me, m.np, u, c, hc, ff, lm, M, v, -1.0)
+    assert rc == 0
+    fused = arrs[-1]
+    g = run(2, m, u, c, None, hc, modes, 2)
+    w = np.zeros((m.np, 2 * M), order="F")
+    ffc = None if ff is None else np.asfortranarray(ff.reshape(m.np, 2 * M, order="F"))
+    rc, arrs = emu_runner.call("emu_rhs", 1, m.jj, m.rr, m.me, m.np, 2 * M, None, None, None, hc, ffc,
+                               np.asfortranarray(g.reshape(m.n_gauss, 2 * M, order="F")), w, -1.0)
+    assert rc == 0
+    assert np.array_equal(fused.reshape(m.np, 2 * M, order="F"), arrs[-1])
+    eg = smb.smb_ugradc_gauss_fft_par(m.jj, m.rr, modes, hc, u, c)
+    ffz = np.zeros((m.np, 2, M)) if ff is None else ff
+    exp = np.stack([assembly.qs_00_gauss(m.jj, m.rr, hc, ffz[:, k, mm], eg[:, k, mm])
+                    for mm in range(M) for k in range(2)], axis=1).reshape(m.np, 2, M, order="F")
+    assert rel(fused, -exp) <= TOL

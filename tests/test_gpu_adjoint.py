@@ -76,6 +76,18 @@ def test_dot_product_50k_triangles():
     lhs = float((TD * Y).sum())
     rhs = float((D * TtY).sum())
     assert abs(lhs - rhs) <= 1e-12 * float(TD.norm()) * float(Y.norm())
+    # the tangent against central differences of the forward kernel at this size:
+    # exact for the bilinear map at eps = 1 (1e-12); at eps = 1e-6 the quotient
+    # loses about six digits of F's own rounding (about 1e-13 of max|F|, the
+    # 1/r terms of the curl near the axis included): 1e-6 relative, against
+    # an O(1) error for a wrong tangent
+    fp, fm = fortran_empty((m.n_gauss, 6, M)), fortran_empty((m.n_gauss, 6, M))
+    for eps, tol in ((1.0, 1e-12), (1e-6, 1e-6)):
+        NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, U + eps * D, fp)
+        NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, U - eps * D, fm)
+        torch.cuda.synchronize()
+        err = float(((fp - fm) / (2 * eps) - TD).abs().max()) / float(TD.abs().max())
+        assert err <= tol, (eps, err, tol)
 
 
 @pytest.mark.parametrize("nx,ny,M", [(5, 6, 8), (8, 12, 32)])

@@ -175,7 +175,14 @@ def test_full_step_224x448_m128_at_size():
     for c in (0, 1, 2, 7, 64, 2 * M - 1):
         e = assembly.qs_00_gauss(jj_s, rr_s, np.ones(els.size), np.zeros(rr_s.shape[1]), cg_s[:, c])
         assert rel(got[:, c], e[loc]) <= TOL, c
-    del cg, cg2, vT
+    # the same right-hand side fused (no Gauss array): bit for bit
+    print("case full step: fused u.grad T -> qs_00_gauss", flush=True)
+    vF = fortran_empty((npn, 2, M))
+    as3 = lambda t: torch.as_strided(t, (npn, 2, M), (1, npn, 2 * npn))   # column k + 2m -> (k, m)
+    NL.nl_qs_00(mesh, modes, hc, u, T, vF, as3(ff0), 1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(vF, as3(vT))
+    del cg, cg2, vT, vF
 
     # rho u and (rho u) (x) u with the level-set density
     print("case full step: rho u, (rho u) (x) u", flush=True)

@@ -378,3 +378,35 @@ def test_fused_nodal_to_rhs(kind, M, prec):
     exp_g = smb.smb_cross_prod_gauss_sft_par(m.jj, m.rr, modes, u, prec, taux, ang)
     e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, exp_g)
     assert rel(f145.cpu().numpy(), -e1) <= TOL and rel(f236.cpu().numpy(), -e2) <= TOL
+
+
+@pytest.mark.parametrize("M,with_ff", [(1, True), (3, False), (16, True), (64, True), (100, False), (128, True)])
+def test_fused_ugradc_to_qs_00(M, with_ff):
+    """nl_qs_00 (no Gauss array) equals smb_ugradc_gauss_fft_par then
+    qs_00_gauss bit for bit, and the oracle to 1e-12."""
+    from oracle import assembly
+    print(f"case fused u.grad c -> qs_00 M={M} ff={with_ff}", flush=True)
+    m = rectangle_p2(4, 5)
+    mesh = NL.Mesh.from_p2(m)
+    modes = np.arange(M)
+    rng = np.random.default_rng(18)
+    u = fourier_field(m.np, 6, M, 19, 2.0)
+    c = fourier_field(m.np, 2, M, 20, 2.0)
+    hc = rng.uniform(0.5, 2.0, m.me)
+    ff = np.asfortranarray(rng.standard_normal((m.np, 2, M))) if with_ff else None
+    du, dc, dhc = fortran_tensor(u), fortran_tensor(c), torch.as_tensor(hc, device="cuda")
+    dff = None if ff is None else fortran_tensor(ff)
+    v = fortran_empty((m.np, 2, M))
+    NL.nl_qs_00(mesh, modes, dhc, du, dc, v, dff, -1.0)
+    g = fortran_empty((m.n_gauss, 2, M))
+    NL.smb_ugradc_gauss_fft_par(None, mesh, modes, dhc, du, dc, g)
+    w = fortran_empty((m.np, 2, M))
+    flat = lambda t, n: torch.as_strided(t, (n, 2 * M), (1, n))
+    NL.qs_00_gauss(mesh, dhc, None if dff is None else flat(dff, m.np), flat(g, m.n_gauss), flat(w, m.np))
+    torch.cuda.synchronize()
+    assert torch.equal(v, -w)
+    eg = smb.smb_ugradc_gauss_fft_par(m.jj, m.rr, modes, hc, u, c)
+    ffz = np.zeros((m.np, 2, M)) if ff is None else ff
+    exp = np.stack([assembly.qs_00_gauss(m.jj, m.rr, hc, ffz[:, k, mm], eg[:, k, mm])
+                    for mm in range(M) for k in range(2)], axis=1).reshape(m.np, 2, M, order="F")
+    assert rel(v.cpu().numpy(), -exp) <= TOL
