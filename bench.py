@@ -128,7 +128,7 @@ def wl_nl(torch, args):
     f_adj = 2 * f_fwd
     phases = [
         Phase("forward", lambda: NL.smb_cross_prod_gauss_sft_par(None, mesh, modes, u, fwd_out),
-              fwd_bytes, f_fwd, "gauss_kernel<64,CROSS_CURL>"),
+              fwd_bytes, f_fwd, "gauss_pt_kernel<64,CROSS_CURL>"),
         Phase("adjoint", lambda: NL.nl_adjoint(mesh, modes, u, ybar, dbar, work),
               adj_bytes, f_adj, "adjoint_kernel<64> + adjoint_gather_kernel",
               overhead=2 * (6 * m.me * 6 * M * 8)),

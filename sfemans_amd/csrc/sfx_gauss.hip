@@ -12,14 +12,29 @@ gauss_kernel(GaussArgs a) {
   gauss_body<MP, V, NW>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
 }
 
+// Point tiles (see gauss_point_tiles): GAUSS_PT_WAVES waves, two per SIMD.
+template <int MP, int V>
+__global__ void __launch_bounds__(64 * GAUSS_PT_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gauss_pt_kernel(GaussArgs a) {
+  extern __shared__ double2 smem[];
+  gauss_body<MP, V, GAUSS_PT_WAVES, false, false, true>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), smem);
+}
+
 template <int MP, int V>
 static hipError_t launch_mp(const GaussArgs& a, hipStream_t st) {
-  constexpr int NW = gauss_waves<MP, V>();
-  constexpr int E = GaussLds<MP, NW>::E;
-  const int64_t nblk = (a.me + E - 1) / E;
-  if (nblk == 0) return hipSuccess;
-  const size_t shmem = (size_t)GaussCfg<MP, V, NW>::LY::TOTAL * sizeof(double2);
-  hipLaunchKernelGGL((gauss_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
+  if constexpr (gauss_point_tiles<MP, V>()) {
+    constexpr int NW = GAUSS_PT_WAVES;
+    const int64_t nblk = gauss_blocks<MP, V, NW>(a.me);
+    if (nblk == 0) return hipSuccess;
+    const size_t shmem = (size_t)GaussCfg<MP, V, NW>::LY::TOTAL * sizeof(double2);
+    hipLaunchKernelGGL((gauss_pt_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
+  } else {
+    constexpr int NW = gauss_waves<MP, V>();
+    const int64_t nblk = gauss_blocks<MP, V, NW>(a.me);
+    if (nblk == 0) return hipSuccess;
+    const size_t shmem = (size_t)GaussCfg<MP, V, NW>::LY::TOTAL * sizeof(double2);
+    hipLaunchKernelGGL((gauss_kernel<MP, V>), dim3((unsigned)nblk), dim3(64 * NW), shmem, st, a);
+  }
   return hipGetLastError();
 }
 

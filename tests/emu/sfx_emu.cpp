@@ -101,11 +101,15 @@ extern "C" int emu_fftprod(int op, const double* V1, const double* V2, double* V
 
 template <int MP, int V>
 static void emu_gauss_mp(const GaussArgs& a) {
-  constexpr int NW = gauss_waves<MP, V>();
-  constexpr int E = GaussLds<MP, NW>::E;
-  const int64_t nblk = (a.me + E - 1) / E;
-  run_grid(nblk, 64 * NW, GaussCfg<MP, V, NW>::LY::TOTAL,
-           [&](int t, int64_t b, double2* s) { gauss_body<MP, V, NW>(a, t, b, s); });
+  if constexpr (gauss_point_tiles<MP, V>()) {   // as the GPU launcher
+    constexpr int NW = GAUSS_PT_WAVES;
+    run_grid(gauss_blocks<MP, V, NW>(a.me), 64 * NW, GaussCfg<MP, V, NW>::LY::TOTAL,
+             [&](int t, int64_t b, double2* s) { gauss_body<MP, V, NW, false, false, true>(a, t, b, s); });
+  } else {
+    constexpr int NW = gauss_waves<MP, V>();
+    run_grid(gauss_blocks<MP, V, NW>(a.me), 64 * NW, GaussCfg<MP, V, NW>::LY::TOTAL,
+             [&](int t, int64_t b, double2* s) { gauss_body<MP, V, NW>(a, t, b, s); });
+  }
 }
 
 template <int V>
