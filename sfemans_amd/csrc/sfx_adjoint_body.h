@@ -63,20 +63,17 @@ struct AdjLds {
   // nodal staging: two alternating chunk buffers of the base field(s)
   static constexpr int NODAL = (SPLIT ? 4 : 2) * LY::NOD6;                 // doubles from BIG
   static constexpr int XCH = NODAL;                                       // first wave region
-  // Element sums by local node (work item: element, node, mode offset)
-  // for the convective form on 7 waves, else by component (element,
-  // component, mode offset); measured faster each where it is used.
-  static constexpr bool BY_NODE = !SPLIT && NW > 4;
-  // red[i * SI + k * SK] for entry i = local point * L + mode offset and
-  // weight k.  By node: entry-major (an item reads the 18 weights of its
-  // entries).  By component: weight-major, a wave writes 64 consecutive
+  // Element sums by component: work item (element, component, mode offset)
+  // for the six local nodes.  red[i + k * SK] for entry i = local point * L
+  // + mode offset and weight k, weight-major: a wave writes 64 consecutive
   // entries of one weight and reads 16 consecutive entries of each of 4
-  // weights SK apart; the 16-double pad puts those groups in different
-  // bank halves.
+  // weights SK apart; the 16-double pad puts those groups in different bank
+  // halves.  (By local node, entry-major, each item reads all 18 weights of
+  // its 7 entries, six times the LDS reads: 11.18 against 10.93 ms for the
+  // 7-wave form at M = 64, and slower on 4 waves too.)
   static constexpr int RPL = 7 * LY::E * (MP / 4);
-  static constexpr int SI = BY_NODE ? AdjRed<SPLIT>::K : 1;
-  static constexpr int SK = BY_NODE ? 1 : RPL + 16;
-  static constexpr int RED = BY_NODE ? RPL * AdjRed<SPLIT>::K : SK * AdjRed<SPLIT>::K;
+  static constexpr int SK = RPL + 16;
+  static constexpr int RED = SK * AdjRed<SPLIT>::K;
   // the reduction area aliases the staging and wave regions (used after them)
   static constexpr int BIGD = (NODAL + NW * RW > RED) ? NODAL + NW * RW : RED;
   // parked doubles per lane: PK1 in the staging area (idle once the base
@@ -426,13 +423,13 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       Wb[2] += ir * Rb[4]; Dr[2] += Rb[4]; Wb[1] -= mr * Rb[4];
       Wb[3] += ir * Rb[5]; Dr[3] += Rb[5]; Wb[0] += mr * Rb[5];
       const int lp = (wave + NW * gi) * PW + pt;
-      double* dst = red + (lp * L + l) * AL::SI;
+      double* dst = red + (lp * L + l);
       constexpr int RP = AL::SK;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const double xb = fma(gg.a11, Dr[k], gg.a21 * Dz[k]);
         const double yb = fma(gg.a12, Dr[k], gg.a22 * Dz[k]);
-        SFX_CHECK("red write", (KR - 1) * RP + (lp * L + l) * AL::SI, AL::RED);
+        SFX_CHECK("red write", (KR - 1) * RP + (lp * L + l), AL::RED);
         if (SPLIT) {
           dst[(4 * k) * RP] = ok ? Ww[k] : 0.0;
           dst[(4 * k + 1) * RP] = ok ? Wb[k] : 0.0;
@@ -446,41 +443,13 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       }
     }
     SFX_SYNC();
-    // By component: work item (element, component k, mode offset mi) for
-    // the six local nodes of the element: the item reads its weights of
-    // component k from LDS once for all six nodes, and the shape tables
-    // (uniform across the wave) come through the scalar cache into scalar
-    // registers, not from LDS.  By node: work item (element, local node n,
-    // mode offset mi) for the six components, tables from LDS.  Either way
-    // each sum runs over the 7 Gauss points in order, and mode is fastest:
-    // consecutive lanes store consecutive modes of one entry (elo layout:
-    // entry, component, mode).
-    if constexpr (AL::BY_NODE) {
-      for (int it = tid; it < E * 6 * L; it += NT) {
-        const int mi = it % L;
-        const int n = (it / L) % 6;
-        const int e = it / (6 * L);
-        const int mm = mi + L * r;
-        const int64_t eg = e0 + e;
-        if (eg >= a.me || mm >= a.M) continue;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        for (int lg2 = 0; lg2 < 7; ++lg2) {
-          const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
-          SFX_CHECK("red read", ((7 * e + lg2) * L + mi) * AL::SI + KR - 1, AL::RED);
-          const double w = tab[n + 6 * lg2], dx = tab[42 + 2 * (n + 6 * lg2)], dy = tab[42 + 2 * (n + 6 * lg2) + 1];
-#pragma unroll
-          for (int k = 0; k < 6; ++k)
-            acc[k] = fma(w, src[3 * k], fma(dx, src[3 * k + 1], fma(dy, src[3 * k + 2], acc[k])));
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          const int64_t o = ((6 * eg + n) * 6 + k) * (int64_t)a.M + mm;
-          SFX_CHECK("elo store", o, A.nelo);
-          A.elo[o] = acc[k];
-        }
-      }
-      continue;
-    }
+    // Work item (element, component k, mode offset mi) for the six local
+    // nodes of the element: the item reads its weights of component k from
+    // LDS once for all six nodes, and the shape tables (uniform across the
+    // wave) come through the scalar cache into scalar registers, not from
+    // LDS.  Each sum runs over the 7 Gauss points in order, and mode is
+    // fastest: consecutive lanes store consecutive modes of one entry (elo
+    // layout: entry, component, mode).
     for (int it = tid; it < E * 6 * L; it += NT) {
       const int mi = it % L;
       const int k = (it / L) % 6;
@@ -499,11 +468,11 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       SFX_OPAQUE_S(tb);
 #pragma unroll
       for (int lg2 = 0; lg2 < 7; ++lg2) {
-        const double* src = red + ((7 * e + lg2) * L + mi) * AL::SI;
+        const double* src = red + ((7 * e + lg2) * L + mi);
         double v[KW];
 #pragma unroll
         for (int j = 0; j < KW; ++j) {
-          SFX_CHECK("red read", (KW * k + j) * RP + ((7 * e + lg2) * L + mi) * AL::SI, AL::RED);
+          SFX_CHECK("red read", (KW * k + j) * RP + ((7 * e + lg2) * L + mi), AL::RED);
           v[j] = src[(KW * k + j) * RP];
         }
 #pragma unroll

@@ -57,17 +57,18 @@ constexpr int gauss_waves() {
 }
 constexpr int groups_per_wave(int nw) { return (GAUSS_GROUPS + nw - 1) / nw; }
 
-// Point tiles (the plain Gauss-point output of the cross-curl at MP = 64):
-// a work-group of GAUSS_PT_WAVES waves takes GAUSS_PT_WAVES * PW consecutive
-// Gauss points instead of E whole elements.  With 16 points at MP = 64 each
-// plane of the output gets one whole 128-byte line per work-group (28
-// points of whole elements straddle lines, and the partial lines cost 1.7x
-// the output bytes of HBM writes), and the 77 KB of LDS let two work-groups
-// share a CU (256 registers per lane, two waves per SIMD).  Measured at
-// 200,704 triangles: 5.72 -> 5.56 ms, output bit for bit the same.
+// Point tiles (the plain Gauss-point output of the cross-curl at 16 <= MP
+// <= 64): a work-group of GAUSS_PT_WAVES waves takes GAUSS_PT_WAVES * PW
+// consecutive Gauss points instead of E whole elements.  With 16 points at
+// MP = 64 each plane of the output gets one whole 128-byte line per
+// work-group (28 points of whole elements straddle lines, and the partial
+// lines cost 1.7x the output bytes of HBM writes), and the 76-77 KB of LDS
+// let two work-groups share a CU (256 registers per lane, two waves per
+// SIMD).  Measured at 200,704 triangles: 5.72 -> 5.56 ms at M = 64, 2.87 ->
+// 2.64 ms at M = 32, 1.06 -> 1.05 ms at M = 16; output bit for bit the same.
 constexpr int GAUSS_PT_WAVES = 4;
 template <int MP, int V>
-constexpr bool gauss_point_tiles() { return V == GV_CROSS_CURL && MP == 64; }
+constexpr bool gauss_point_tiles() { return V == GV_CROSS_CURL && MP >= 16 && MP <= 64; }
 // Work-groups of the launch over me elements.
 template <int MP, int V, int NW>
 constexpr int64_t gauss_blocks(int64_t me) {

@@ -25,7 +25,7 @@ def run(variant, mesh, U, U2, U3, hc, modes, ncout, cl=0.0, om=(0.0, 0.0, 0.0)):
     return arrs[-1]
 
 
-CASES = [(2, 3, 1), (3, 2, 3), (2, 5, 8), (3, 3, 13), (2, 2, 64), (1, 2, 100)]
+CASES = [(2, 3, 1), (3, 2, 3), (2, 5, 8), (3, 3, 13), (2, 3, 20), (2, 2, 64), (1, 2, 100)]
 
 
 @pytest.mark.parametrize("nx,ny,M", CASES)
