@@ -29,7 +29,10 @@ def gpu():
 
 
 @pytest.mark.parametrize("nx,ny,M", [(2, 3, 1), (3, 2, 3), (5, 4, 8), (4, 7, 13), (6, 5, 16),
-                                      (3, 4, 32), (4, 4, 64), (2, 3, 100), (2, 2, 128)])
+                                      (3, 4, 32), (4, 4, 64), (2, 3, 100), (2, 2, 128),
+                                      # point tiles (16 <= MP <= 64) with a partial last tile,
+                                      # and a single square (2 triangles, 14 Gauss points)
+                                      (3, 3, 64), (1, 1, 64), (1, 1, 32), (1, 1, 16), (5, 3, 40)])
 def test_cross_curl(nx, ny, M):
     print(f"case cross_curl {nx}x{ny} M={M}", flush=True)
     m = rectangle_p2(nx, ny)
