@@ -98,9 +98,14 @@ SFX_DEV double& park_slot(double* big, int f, int t) {
 // for Phi^T and pack the three real output fields as signals (pr, pth) and
 // (pz, 0).  SWAP: each lane loads its own modes only and takes the partner
 // modes from the lanes that loaded them, through the wave's LDS region xf
-// (half the global loads; measured faster with 512 registers per wave,
-// slower with 256, where the extra live values spill).
-template <int MP, bool SWAP>
+// (half the global loads; used with 512 registers per wave).  SWAP2: the
+// same exchange in two rounds of two mode slots, packing as the partners
+// arrive, so that it fits the 256-register form (its wave region holds 12
+// doubles per lane); each load instruction touches 16 cache lines, so
+// halving them shortens the phase in which the work-group's loads queue in
+// the vector memory path (adjoint at M = 64: 10.93 -> 10.56 ms).  Neither:
+// own and partner modes loaded two slots at a time.
+template <int MP, bool SWAP, bool SWAP2 = false>
 SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, double* xf,
                        double2 (&Z)[2][4], double2 (&B)[2][4]) {
   constexpr int L = MP / 4, N = 3 * MP;
@@ -139,13 +144,13 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
       const int64_t i1 = pc + a.G * (c + 6 * (int64_t)mc);
       SFX_CHECK("ybar load", i1, A.nybar);
       v[r][c] = A.ybar[i1];
-      if (!SWAP) {
+      if (!SWAP && !SWAP2) {
         const int64_t i2 = pc + a.G * (c + 6 * (int64_t)kc);
         SFX_CHECK("ybar load", i2, A.nybar);
         w[r][c] = A.ybar[i2];
       }
     }
-    if (!SWAP && (r & 1)) {
+    if (!SWAP && !SWAP2 && (r & 1)) {
       // without SWAP: two slots loaded and packed at a time, so that at most
       // 24 loaded values are live (all 48 at once would push the caller's
       // live spectra out to scratch)
@@ -154,7 +159,57 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
       SFX_SCHED_BARRIER();
     }
   }
-  if (SWAP) {
+  if (SWAP2) {
+    // own modes only, in one batch; the partner modes through xf in two
+    // rounds of two slots (xf[(6 j + c) * 64 + lane], 64 x 12 doubles):
+    // round A slots 0 and 3, round B slots 1 and 2.  The partner of slot r
+    // is slot 3 - r at lane (L - l) % L for l >= 1, slot (4 - r) % 4 of
+    // lane 0 for l = 0 (whose slot-0 partner, mode MP, is dropped by pack).
+    const int lane = pt * L + l, plane = pt * L + (L - l) % L;
+    const bool l0 = (l == 0);
+    double a0[6], a1[6], b0[6], b1[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      SFX_CHECK("ybar swap", (6 + c) * 64 + lane, 64 * 12);
+      xf[c * 64 + lane] = v[0][c];
+      xf[(6 + c) * 64 + lane] = v[3][c];
+    }
+    SFX_WAVE_SYNC();
+    // l >= 1: a0 = slot 3 (partner of r = 0), a1 = slot 0 (partner of r = 3);
+    // l = 0:  a0 = slot 0 (partner of r = 0), a1 = slot 3 (partner of r = 1)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      SFX_CHECK("ybar swap", (6 + c) * 64 + plane, 64 * 12);
+      a0[c] = xf[((l0 ? 0 : 6) + c) * 64 + plane];
+      a1[c] = xf[((l0 ? 6 : 0) + c) * 64 + plane];
+    }
+    SFX_WAVE_SYNC();
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      w[0][c] = a0[c];
+      xf[c * 64 + lane] = v[1][c];
+      xf[(6 + c) * 64 + lane] = v[2][c];
+    }
+    pack(0);
+    SFX_WAVE_SYNC();
+    // l >= 1: b0 = slot 2 (partner of r = 1), b1 = slot 1 (partner of r = 2);
+    // l = 0:  b0 = slot 2 (partner of r = 2), b1 = slot 1 (partner of r = 3)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      b0[c] = xf[(6 + c) * 64 + plane];
+      b1[c] = xf[c * 64 + plane];
+    }
+    SFX_WAVE_SYNC();
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      w[1][c] = l0 ? a1[c] : b0[c];
+      w[2][c] = l0 ? b0[c] : b1[c];
+      w[3][c] = l0 ? b1[c] : a1[c];
+    }
+    pack(1);
+    pack(2);
+    pack(3);
+  } else if (SWAP) {
     // mode MP - m of slot r is held by lane (L - l) % L in slot 3 - r (l >= 1)
     // or 4 - r (l = 0, r >= 1); layout xf[(6 slot + c) * 64 + lane]
     const int lane = pt * L + l, plane = pt * L + (L - l) % L;
@@ -299,7 +354,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     }
     {
       double2 Zy[2][4], By[2][4];
-      load_ybar<MP, (NW <= 4)>(A, p[gi], valid[gi], pt, l, xf, Zy, By);
+      load_ybar<MP, (NW <= 4), AL::PARK>(A, p[gi], valid[gi], pt, l, xf, Zy, By);
       double2 y[6][4];
 #pragma unroll
       for (int s = 0; s < 2; ++s) prestep<MP>(Zy[s], By[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
