@@ -64,8 +64,11 @@ constexpr int groups_per_wave(int nw) { return (GAUSS_GROUPS + nw - 1) / nw; }
 // work-group (28 points of whole elements straddle lines, and the partial
 // lines cost 1.7x the output bytes of HBM writes), and the 76-77 KB of LDS
 // let two work-groups share a CU (256 registers per lane, two waves per
-// SIMD).  Measured at 200,704 triangles: 5.72 -> 5.56 ms at M = 64, 2.87 ->
-// 2.64 ms at M = 32, 1.06 -> 1.05 ms at M = 16; output bit for bit the same.
+// SIMD).  The outputs leave through LDS, each store instruction writing 4
+// planes x 16 points (a lane's own stores touch 16 planes x 4 points, 16
+// cache lines).  Measured at 200,704 triangles: 5.72 -> 5.56 ms at M = 64,
+// 2.87 -> 2.64 ms at M = 32, 1.06 -> 1.05 ms at M = 16; staged stores 5.56
+// -> 5.28 and 2.66 -> 2.52 ms; output bit for bit the same.
 constexpr int GAUSS_PT_WAVES = 4;
 template <int MP, int V>
 constexpr bool gauss_point_tiles() { return V == GV_CROSS_CURL && MP >= 16 && MP <= 64; }
@@ -642,6 +645,31 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           }
         continue;
       }
+      if constexpr (PT) {
+        // point tiles: the work-group's outputs go through LDS (st[P * SR +
+        // lp] for output plane P = 2o + h + 2 NO m), so that the stores below
+        // write whole runs of the tile's points per plane
+        constexpr int TP = NW * PW, SR = TP + 1;
+        static_assert(2 * NO * MP * SR <= LY::NODAL + NW * LY::RW, "output staging");
+        double* st = big;
+        const int lp = wave * PW + pt;
+        SFX_SYNC();   // every wave is done with the staging and wave regions
+#pragma unroll
+        for (int t = 0; t < NOC; ++t)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int o = 2 * t + h;
+            if (o >= NO) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int P = 2 * o + 2 * NO * (l + L * r);
+              SFX_CHECK("out stage", (P + 1) * SR + lp, 2 * NO * MP * SR);
+              st[P * SR + lp] = oa[t][h][r];
+              st[(P + 1) * SR + lp] = ob[t][h][r];
+            }
+          }
+        continue;
+      }
       if (!pick(valid, gi)) continue;
       const int64_t pg = pick(p, gi);
 #pragma unroll
@@ -665,6 +693,20 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
             }
           }
         }
+    }
+  }
+  if constexpr (PT) {
+    constexpr int TP = NW * PW, SR = TP + 1;
+    const double* st = big;
+    SFX_SYNC();
+    const int items = 2 * NO * a.M * TP;
+    for (int it = tid; it < items; it += NT) {
+      const int lp = it % TP, P = it / TP;
+      const int64_t pg = p0 + lp;
+      if (pg >= a.G) continue;
+      SFX_CHECK("out stage read", P * SR + lp, 2 * NO * MP * SR);
+      SFX_CHECK("gauss out", pg + a.G * (int64_t)P, a.nout);
+      a.Vout[pg + a.G * (int64_t)P] = st[P * SR + lp];
     }
   }
   if constexpr (RHS) {
