@@ -426,6 +426,11 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
   constexpr int L = S::L, PW = S::PW, E = LY::E, NG = PT ? 1 : groups_per_wave(NW);
   static_assert(!PT || (!RHS && !STAGE_ALL && (NW * PW + 5) / 7 + 1 <= E), "point tiles: Gauss output only");
   constexpr int NSIG = GVT<V>::NSIG, NO = GVT<V>::NOUT;
+  // Outputs staged in LDS and stored in whole runs of the work-group's points
+  // per plane (one group per wave): the point tiles, and the single-pass
+  // (curl a) x b of the pass lists (tangent at 50,000 triangles, M = 32:
+  // 4.22 -> 4.06 ms; neutral at M = 64; u.grad c gains nothing at M = 64).
+  constexpr bool SOUT = !RHS && !STAGE_ALL && NG == 1 && (PT || V == GV_CURL_CROSS);
   constexpr int NF = 4 * NSIG;
   constexpr int NOC = (NO + 1) / 2;
   constexpr int GX = 2;
@@ -645,11 +650,11 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
           }
         continue;
       }
-      if constexpr (PT) {
-        // point tiles: the work-group's outputs go through LDS (st[P * SR +
-        // lp] for output plane P = 2o + h + 2 NO m), so that the stores below
-        // write whole runs of the tile's points per plane
-        constexpr int TP = NW * PW, SR = TP + 1;
+      if constexpr (SOUT) {
+        // the work-group's outputs go through LDS (st[P * SR + lp] for
+        // output plane P = 2o + h + 2 NO m), so that the stores below write
+        // whole runs of the tile's points per plane
+        constexpr int TP = PT ? NW * PW : 7 * E, SR = TP + 1;
         static_assert(2 * NO * MP * SR <= LY::NODAL + NW * LY::RW, "output staging");
         double* st = big;
         const int lp = wave * PW + pt;
@@ -695,18 +700,19 @@ SFX_DEV void gauss_body(const GaussArgs& a, int tid, int64_t bid, double2* smem)
         }
     }
   }
-  if constexpr (PT) {
-    constexpr int TP = NW * PW, SR = TP + 1;
+  if constexpr (SOUT) {
+    constexpr int TP = PT ? NW * PW : 7 * E, SR = TP + 1;
     const double* st = big;
     SFX_SYNC();
     const int items = 2 * NO * a.M * TP;
     for (int it = tid; it < items; it += NT) {
       const int lp = it % TP, P = it / TP;
       const int64_t pg = p0 + lp;
-      if (pg >= a.G) continue;
+      if (pg >= 7 * a.me) continue;
       SFX_CHECK("out stage read", P * SR + lp, 2 * NO * MP * SR);
       SFX_CHECK("gauss out", pg + a.G * (int64_t)P, a.nout);
-      a.Vout[pg + a.G * (int64_t)P] = st[P * SR + lp];
+      if (V == GV_CURL_CROSS && a.accum) a.Vout[pg + a.G * (int64_t)P] += st[P * SR + lp];
+      else a.Vout[pg + a.G * (int64_t)P] = st[P * SR + lp];
     }
   }
   if constexpr (RHS) {
