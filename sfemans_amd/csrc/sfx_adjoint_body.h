@@ -435,6 +435,16 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
   // ---- Psi^T scaling, curl transpose, reference-coordinate adjoints,
   //      then the sum over the element's Gauss points through LDS
   double* red = big;
+  // the geometry again (24 FMAs), once for the four slots, rather than kept
+  // live through the transforms
+  GaussGeom gr[NG];
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    const int lpg = (wave + NW * gi) * PW + pt;
+    int elg = lpg / 7;
+    SFX_OPAQUE(elg);
+    gauss_geom(geo + 6 * elg, tab, valid[gi] ? lpg % 7 : 0, gr[gi]);
+  }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     SFX_SYNC();   // the reduction area aliases the staging and wave regions
@@ -453,14 +463,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
       Rb[0] = sa * oa[gi][1][1][r]; Rb[1] = sb * ob[gi][1][1][r];
       Rb[2] = sa * oa[gi][2][0][r]; Rb[3] = sb * ob[gi][2][0][r];
       Rb[4] = sa * oa[gi][2][1][r]; Rb[5] = sb * ob[gi][2][1][r];
-      // the geometry again (24 FMAs) rather than kept live through the transforms
-      GaussGeom gg;
-      {
-        const int lpg = (wave + NW * gi) * PW + pt;
-        int elg = lpg / 7;
-        SFX_OPAQUE(elg);
-        gauss_geom(geo + 6 * elg, tab, valid[gi] ? lpg % 7 : 0, gg);
-      }
+      const GaussGeom& gg = gr[gi];
       const double mr = lmode[m < MP ? m : 0] * gg.iray, ir = gg.iray;
       double Dr[6] = {0, 0, 0, 0, 0, 0}, Dz[6] = {0, 0, 0, 0, 0, 0};
       double Ww[6];   // value weights of the W^T part alone (split form)
