@@ -38,11 +38,13 @@ struct AdjArgs {
 // weights of the two outputs apart.
 template <bool SPLIT> struct AdjRed { static constexpr int K = SPLIT ? 24 : 18; };
 
-// Waves per work-group of the adjoint kernel (see gauss_waves): 7 for the
-// convective form at MP = 64 (measured faster there, slower at MP = 32 and
-// 128), 4 elsewhere.
+// Waves per work-group of the adjoint kernel (see gauss_waves): 7 (256
+// registers, the parking form) for the convective form at 16 <= MP <= 128,
+// 4 (512 registers) elsewhere.  Measured at 200,704 triangles, 7 against 4
+// waves: M = 16 2.23 / 2.56 ms, M = 32 4.92 / 5.97, M = 128 25.9 / 29.6
+// (M = 64 was the first size moved to 7).
 template <int MP, bool SPLIT>
-constexpr int adjoint_waves() { return (!SPLIT && MP == 64) ? 7 : 4; }
+constexpr int adjoint_waves() { return (!SPLIT && MP >= 16 && MP <= 128) ? 7 : 4; }
 
 template <int MP, bool SPLIT, int NW>
 struct AdjLds {

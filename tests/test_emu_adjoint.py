@@ -45,7 +45,7 @@ def test_adjoint_equals_bruteforce_transpose(M):
     assert np.max(np.abs(got - exp)) <= 1e-12 * np.max(np.abs(exp))
 
 
-@pytest.mark.parametrize("nx,ny,M", [(2, 3, 5), (3, 3, 16), (2, 2, 64)])
+@pytest.mark.parametrize("nx,ny,M", [(2, 3, 5), (3, 3, 16), (2, 3, 20), (2, 2, 64), (1, 2, 100)])
 def test_dot_product(nx, ny, M):
     m = rectangle_p2(nx, ny)
     modes = np.arange(M)
