@@ -49,11 +49,14 @@ constexpr int GAUSS_WAVES = 4;    // default wave count (batched launches)
 // Waves per work-group of the fused kernel for variant V at size MP.  With 7
 // waves a wave holds one group and a CU runs two waves per SIMD (256
 // registers each), which hides more latency; with 4 waves a wave holds two
-// groups in 512 registers at one wave per SIMD.  7 where one group's state
-// fits in 256 registers (measured faster there), 4 elsewhere.
+// groups in 512 registers at one wave per SIMD.  7 where measured faster:
+// u.grad c; the cross-curl up to MP = 128 (at M = 128, 200,704 triangles,
+// the fused right-hand side 19.1 -> 17.8 ms and the Gauss output 16.4 ->
+// 16.2 ms against 4 waves, despite 152 bytes per lane of spill); the
+// single-pass (curl a) x b up to MP = 64.  4 elsewhere.
 template <int MP, int V>
 constexpr int gauss_waves() {
-  return (V == GV_UGRADC || ((V == GV_CROSS_CURL || V == GV_CURL_CROSS) && MP <= 64)) ? 7 : 4;
+  return (V == GV_UGRADC || (V == GV_CROSS_CURL && MP <= 128) || (V == GV_CURL_CROSS && MP <= 64)) ? 7 : 4;
 }
 constexpr int groups_per_wave(int nw) { return (GAUSS_GROUPS + nw - 1) / nw; }
 
