@@ -39,7 +39,7 @@ def base_flow(npts, M, seed):
     return U
 
 
-@pytest.mark.parametrize("nx,ny,M", [(7, 9, 8), (10, 20, 32), (4, 6, 64)])
+@pytest.mark.parametrize("nx,ny,M", [(7, 9, 8), (6, 8, 16), (10, 20, 32), (4, 6, 64), (3, 5, 100)])
 def test_dot_product(nx, ny, M):
     print(f"case dot-product {nx}x{ny} M={M}", flush=True)
     m = rectangle_p2(nx, ny)
