@@ -98,16 +98,16 @@ SFX_DEV double& park_slot(double* big, int f, int t) {
 
 // Load ybar at the own and partner modes (MP - m) of Gauss point p, scale
 // for Phi^T and pack the three real output fields as signals (pr, pth) and
-// (pz, 0).  SWAP: each lane loads its own modes only and takes the partner
-// modes from the lanes that loaded them, through the wave's LDS region xf
-// (half the global loads; used with 512 registers per wave).  SWAP2: the
-// same exchange in two rounds of two mode slots, packing as the partners
-// arrive, so that it fits the 256-register form (its wave region holds 12
-// doubles per lane); each load instruction touches 16 cache lines, so
-// halving them shortens the phase in which the work-group's loads queue in
-// the vector memory path (adjoint at M = 64: 10.93 -> 10.56 ms).  Neither:
-// own and partner modes loaded two slots at a time.
-template <int MP, bool SWAP, bool SWAP2 = false>
+// (pz, 0).  Each lane loads its own modes only and takes the partner modes
+// from the lanes that loaded them, through the wave's LDS region xf: all
+// four slots at once (64 x 24 doubles, the 512-register form), or with
+// SWAP2 in two rounds of two mode slots, packing as the partners arrive, so
+// that it fits the 256-register form (its wave region holds 12 doubles per
+// lane).  Loading the partner modes as well would double the load
+// instructions, each of which touches 16 cache lines: the work-group's
+// loads queue in the vector memory path (adjoint at M = 64: 10.93 ms with
+// own and partner loads two slots at a time, 10.56 ms with SWAP2).
+template <int MP, bool SWAP2>
 SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, double* xf,
                        double2 (&Z)[2][4], double2 (&B)[2][4]) {
   constexpr int L = MP / 4, N = 3 * MP;
@@ -138,30 +138,16 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
   };
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int m = lo + L * r, k = MP - m;
+    const int m = lo + L * r;
     const int mc = (m < a.M) ? m : 0;
-    const int kc = (m > 0 && k < a.M) ? k : 0;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const int64_t i1 = pc + a.G * (c + 6 * (int64_t)mc);
       SFX_CHECK("ybar load", i1, A.nybar);
       v[r][c] = A.ybar[i1];
-      if (!SWAP && !SWAP2) {
-        const int64_t i2 = pc + a.G * (c + 6 * (int64_t)kc);
-        SFX_CHECK("ybar load", i2, A.nybar);
-        w[r][c] = A.ybar[i2];
-      }
-    }
-    if (!SWAP && !SWAP2 && (r & 1)) {
-      // without SWAP: two slots loaded and packed at a time, so that at most
-      // 24 loaded values are live (all 48 at once would push the caller's
-      // live spectra out to scratch)
-      pack(r - 1);
-      pack(r);
-      SFX_SCHED_BARRIER();
     }
   }
-  if (SWAP2) {
+  if constexpr (SWAP2) {
     // own modes only, in one batch; the partner modes through xf in two
     // rounds of two slots (xf[(6 j + c) * 64 + lane], 64 x 12 doubles):
     // round A slots 0 and 3, round B slots 1 and 2.  The partner of slot r
@@ -211,7 +197,7 @@ SFX_DEV void load_ybar(const AdjArgs& A, int64_t p, bool valid, int pt, int l, d
     pack(1);
     pack(2);
     pack(3);
-  } else if (SWAP) {
+  } else {
     // mode MP - m of slot r is held by lane (L - l) % L in slot 3 - r (l >= 1)
     // or 4 - r (l = 0, r >= 1); layout xf[(6 slot + c) * 64 + lane]
     const int lane = pt * L + l, plane = pt * L + (L - l) % L;
@@ -356,7 +342,7 @@ SFX_DEV void adjoint_body(const AdjArgs& A, int tid, int64_t bid, double2* smem)
     }
     {
       double2 Zy[2][4], By[2][4];
-      load_ybar<MP, (NW <= 4), AL::PARK>(A, p[gi], valid[gi], pt, l, xf, Zy, By);
+      load_ybar<MP, AL::PARK>(A, p[gi], valid[gi], pt, l, xf, Zy, By);
       double2 y[6][4];
 #pragma unroll
       for (int s = 0; s < 2; ++s) prestep<MP>(Zy[s], By[s], l, y[3 * s], y[3 * s + 1], y[3 * s + 2], wN);
