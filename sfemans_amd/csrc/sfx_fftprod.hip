@@ -17,14 +17,6 @@ fftprod_kernel(FftProdArgs a) {
   fftprod_body<MP, OP>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), FFTPROD_WPB, smem);
 }
 
-// The tensor product keeps three output groups live and would spill under
-// the 256-register limit: it runs at 1 wave per SIMD.
-template <int MP, int OP>
-__global__ void __launch_bounds__(64 * FFTPROD_WPB) fftprod_kernel_wide(FftProdArgs a) {
-  extern __shared__ double2 smem[];
-  fftprod_body<MP, OP>(a, threadIdx.x, xcd_block(blockIdx.x, gridDim.x), FFTPROD_WPB, smem);
-}
-
 template <int MP, int OP>
 static hipError_t launch_mp(const FftProdArgs& a, hipStream_t st) {
   constexpr int PW = Shape<MP>::PW;
@@ -32,10 +24,7 @@ static hipError_t launch_mp(const FftProdArgs& a, hipStream_t st) {
   const int64_t nblk = (a.np + pts_per_block - 1) / pts_per_block;
   if (nblk == 0) return hipSuccess;
   const size_t shmem = fftprod_smem_cplx<MP>(FFTPROD_WPB) * sizeof(double2);
-  if (OP == OP_TENSOR)
-    hipLaunchKernelGGL((fftprod_kernel_wide<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
-  else
-    hipLaunchKernelGGL((fftprod_kernel<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
+  hipLaunchKernelGGL((fftprod_kernel<MP, OP>), dim3((unsigned)nblk), dim3(64 * FFTPROD_WPB), shmem, st, a);
   return hipGetLastError();
 }
 

@@ -166,10 +166,16 @@ SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4]
 // instruction then writes runs of consecutive points of one plane instead of
 // the 4-point pieces the transform layout holds.  `stage` is the exchange
 // region of the work-group (free once the modes are paired); the rows of
-// PTS points are padded by one double.
-template <int MP, int NOC, int NO>
+// PTS points are padded by one double.  The tensor product stores row g of
+// its (3, np, 6, M) output: the points of a plane are 3 doubles apart, so a
+// store instruction writes a third of each line it touches (the other rows
+// fill the rest), 16 lines per instruction against 64 from the transform
+// layout.
+// The arrays hold output signals t0 .. t0 + NOC - 1.
+template <int MP, int OP, int NOC, int NO>
 SFX_DEV void stage_store(const FftProdArgs& a, const double (&oa)[NOC][2][4], const double (&ob)[NOC][2][4],
-                         int tid, int wave, int pt, int l, int wpb, int64_t bid, double2* stage2) {
+                         int tid, int wave, int pt, int l, int wpb, int64_t bid, double2* stage2, int g,
+                         int t0 = 0) {
   constexpr int L = Shape<MP>::L, PW = Shape<MP>::PW;
   double* stage = reinterpret_cast<double*>(stage2);
   const int PTS = PW * wpb, ROW = PTS + 1;
@@ -180,7 +186,7 @@ SFX_DEV void stage_store(const FftProdArgs& a, const double (&oa)[NOC][2][4], co
   for (int t = 0; t < NOC; ++t)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int o = 2 * t + h;
+      const int o = 2 * (t0 + t) + h;
       if (o >= NO) continue;
       SFX_SYNC();
 #pragma unroll
@@ -197,7 +203,8 @@ SFX_DEV void stage_store(const FftProdArgs& a, const double (&oa)[NOC][2][4], co
         const int64_t pp = p0 + qq;
         if (pp >= a.np) continue;
         const int c = plane & 1, m = plane >> 1;
-        const int64_t idx = pp + a.np * (int64_t)(2 * o + c + 2 * NO * m);
+        const int64_t idx = (OP == OP_TENSOR) ? g + 3 * (pp + a.np * (int64_t)(2 * o + c + 6 * m))
+                                              : pp + a.np * (int64_t)(2 * o + c + 2 * NO * m);
         SFX_CHECK("Vout store", idx, a.nout);
         a.Vout[idx] = stage[plane * ROW + qq];
       }
@@ -237,53 +244,43 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
   }
   inverse_fft<MP, 3 * NS, GX>(y, pt, l, lds, wMP);
 
+  // The tensor product transforms its output signals one at a time (three
+  // branches each): its 9 input branches stay live through all 3 groups,
+  // and with both output signals at once the kernel needs more than the 256
+  // registers of 2 waves per SIMD.
+  constexpr int NT1 = (OP == OP_TENSOR) ? 1 : NOC;
 #pragma unroll
-  for (int g = 0; g < NG; ++g) {
-    // pointwise op on the 3 branches x 4 positions held by this lane
-    double2 q[3 * NOC][4];
+  for (int g = 0; g < NG; ++g)
 #pragma unroll
-    for (int j1 = 0; j1 < 3; ++j1)
+    for (int t0 = 0; t0 < NOC; t0 += NT1) {
+      // pointwise op on the 3 branches x 4 positions held by this lane
+      double2 q[3 * NT1][4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        double u[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int s = 0; s < NS; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
-        double o[3] = {0, 0, 0};
-        apply_op<OP>(u, g, o);
-#pragma unroll
-        for (int t = 0; t < NOC; ++t)
-          q[3 * t + j1][r] = make_double2(o[2 * t], (2 * t + 1 < NO) ? o[2 * t + 1] : 0.0);
-      }
-    forward_fft<MP, 3 * NOC, GX>(q, pt, l, lds, wMP);
-
-    double2 lo[NOC][4], hi[NOC][4];
-#pragma unroll
-    for (int t = 0; t < NOC; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
-    double oa[NOC][2][4], ob[NOC][2][4];
-    pair_modes<MP, NOC>(lo, hi, pt, l, lds, oa, ob);
-    if (OP != OP_TENSOR) {
-      stage_store<MP, NOC, NO>(a, oa, ob, tid, wave, pt, l, wpb, bid, smem + 4 * MP);
-      continue;
-    }
-    if (!valid) continue;
-#pragma unroll
-    for (int t = 0; t < NOC; ++t)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int o = 2 * t + h;
-        if (o >= NO) continue;
+      for (int j1 = 0; j1 < 3; ++j1)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = l + L * r;
-          if (m >= a.M) continue;
-          // V_out(g, p, 2o:2o+1, m) in a (3, np, 6, M) array
-          const int64_t base = g + 3 * (p + a.np * (int64_t)(2 * o + 6 * m));
-          SFX_CHECK("Vout store", base + 3 * a.np, a.nout);
-          a.Vout[base] = oa[t][h][r];
-          a.Vout[base + 3 * a.np] = ob[t][h][r];
+          double u[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (int s = 0; s < NS; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
+          double o[3] = {0, 0, 0};
+          apply_op<OP>(u, g, o);
+#pragma unroll
+          for (int t = 0; t < NT1; ++t) {
+            const int ot = 2 * (t0 + t);
+            q[3 * t + j1][r] = make_double2(o[ot], (ot + 1 < NO) ? o[ot + 1] : 0.0);
+          }
         }
-      }
-  }
+      forward_fft<MP, 3 * NT1, GX>(q, pt, l, lds, wMP);
+
+      double2 lo[NT1][4], hi[NT1][4];
+#pragma unroll
+      for (int t = 0; t < NT1; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
+      double oa[NT1][2][4], ob[NT1][2][4];
+      pair_modes<MP, NT1>(lo, hi, pt, l, lds, oa, ob);
+      stage_store<MP, OP, NT1, NO>(a, oa, ob, tid, wave, pt, l, wpb, bid, smem + 4 * MP, g, t0);
+      // the next exchanges reuse the stage area other waves may still read
+      if (g + 1 < NG || t0 + NT1 < NOC) SFX_SYNC();
+    }
 }
 
 }  // namespace sfx
