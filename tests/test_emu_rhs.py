@@ -128,3 +128,25 @@ def test_rhs_ns_gauss_values_with_imposed_velocity(modes, slot):
         ig = np.einsum("enk,nl->elk", imp[(m.jj - 1).T], ww).reshape(m.n_gauss, 6)
         exp[:, :, slot] -= ig
     assert rel(arrs[-1], exp) <= 1e-13
+
+
+def _fan_mesh(nt=12):
+    """A vertex shared by nt triangles, in the half plane r > 0."""
+    ang = np.linspace(0.0, 2 * np.pi, nt, endpoint=False)
+    vr = np.concatenate([[2.0], 2.0 + np.cos(ang)])
+    vz = np.concatenate([[0.0], np.sin(ang)])
+    tri = np.array([[0, 1 + i, 1 + (i + 1) % nt] for i in range(nt)])
+    from sfemans_amd.mesh import p2_from_p1
+    return p2_from_p1(tri, vr, vz)
+
+
+def test_rhs_3x3_many_entries():
+    """A node with 12 (element, local node) entries, summed in element order."""
+    m = _fan_mesh(12)
+    M = 3
+    f = np.asfortranarray(np.random.default_rng(4).standard_normal((m.n_gauss, 6, M)))
+    v1 = np.zeros((m.np, 3, M), order="F"); v2 = np.zeros_like(v1)
+    rc, arrs = emu_runner.call("emu_rhs", 0, m.jj, m.rr, m.me, m.np, M, f, v1, v2, None, None, None, None, -1.0)
+    assert rc == 0
+    e1, e2 = assembly.rhs_3x3_all_modes(m.jj, m.rr, f)
+    assert rel(arrs[3], -e1) <= 1e-13 and rel(arrs[4], -e2) <= 1e-13
