@@ -14,13 +14,14 @@
 namespace sfx {
 
 template <int OP> struct OpT;
-// NF: real physical fields after the inverse transform; NO: real outputs per
-// group; NG: output groups processed one after the other (bounds registers).
-template <> struct OpT<OP_CROSS>  { static constexpr int NF = 6, NO = 3, NG = 1; };
-template <> struct OpT<OP_DOT>    { static constexpr int NF = 6, NO = 1, NG = 1; };
-template <> struct OpT<OP_PROD>   { static constexpr int NF = 2, NO = 1, NG = 1; };
-template <> struct OpT<OP_SVECT1> { static constexpr int NF = 4, NO = 3, NG = 1; };
-template <> struct OpT<OP_TENSOR> { static constexpr int NF = 6, NO = 3, NG = 3; };
+// NF: real physical fields after the inverse transform; NO: real outputs,
+// packed two per complex output signal (the tensor's 9 outputs, row-major
+// (row g, column c) -> 3g + c, in 5 signals).
+template <> struct OpT<OP_CROSS>  { static constexpr int NF = 6, NO = 3; };
+template <> struct OpT<OP_DOT>    { static constexpr int NF = 6, NO = 1; };
+template <> struct OpT<OP_PROD>   { static constexpr int NF = 2, NO = 1; };
+template <> struct OpT<OP_SVECT1> { static constexpr int NF = 4, NO = 3; };
+template <> struct OpT<OP_TENSOR> { static constexpr int NF = 6, NO = 9; };
 
 // Field f -> (array, first column, columns of that array, element count).
 template <int OP>
@@ -34,21 +35,19 @@ SFX_DEV const double* field_base(const FftProdArgs& a, int f, int& col, int& nb,
   return first ? a.V1 : a.V2;
 }
 
-// Output group g at one physical point from the real fields u.
+// Real output j at one physical point from the real fields u.
 template <int OP>
-SFX_DEV void apply_op(const double (&u)[6], int g, double (&o)[3]) {
+SFX_DEV double apply_op(const double (&u)[6], int j) {
   if (OP == OP_CROSS) {
-    o[0] = u[1] * u[5] - u[2] * u[4];
-    o[1] = u[2] * u[3] - u[0] * u[5];
-    o[2] = u[0] * u[4] - u[1] * u[3];
+    return j == 0 ? u[1] * u[5] - u[2] * u[4] : j == 1 ? u[2] * u[3] - u[0] * u[5] : u[0] * u[4] - u[1] * u[3];
   } else if (OP == OP_DOT) {
-    o[0] = u[0] * u[3] + u[1] * u[4] + u[2] * u[5];
+    return u[0] * u[3] + u[1] * u[4] + u[2] * u[5];
   } else if (OP == OP_PROD) {
-    o[0] = u[0] * u[1];
+    return u[0] * u[1];
   } else if (OP == OP_SVECT1) {
-    o[0] = u[3] * u[0]; o[1] = u[3] * u[1]; o[2] = u[3] * u[2];
-  } else {  // tensor row g: V1_g * V2_b
-    o[0] = u[g] * u[3]; o[1] = u[g] * u[4]; o[2] = u[g] * u[5];
+    return u[3] * u[j];
+  } else {  // tensor (g, c) = V1_g * V2_c
+    return u[j / 3] * u[3 + j % 3];
   }
 }
 
@@ -166,16 +165,14 @@ SFX_DEV void pair_modes(const double2 (&lo)[NOC][4], const double2 (&hi)[NOC][4]
 // instruction then writes runs of consecutive points of one plane instead of
 // the 4-point pieces the transform layout holds.  `stage` is the exchange
 // region of the work-group (free once the modes are paired); the rows of
-// PTS points are padded by one double.  The tensor product stores row g of
-// its (3, np, 6, M) output: the points of a plane are 3 doubles apart, so a
-// store instruction writes a third of each line it touches (the other rows
-// fill the rest), 16 lines per instruction against 64 from the transform
-// layout.
-// The arrays hold output signals t0 .. t0 + NOC - 1.
+// PTS points are padded by one double.  Tensor output 3g + c goes to row g,
+// column c of the (3, np, 6, M) output: the points of a plane are 3 doubles
+// apart, so a store instruction writes a third of each line it touches (the
+// other rows fill the rest), 16 lines per instruction against 64 from the
+// transform layout.  The arrays hold output signals t0 .. t0 + NOC - 1.
 template <int MP, int OP, int NOC, int NO>
 SFX_DEV void stage_store(const FftProdArgs& a, const double (&oa)[NOC][2][4], const double (&ob)[NOC][2][4],
-                         int tid, int wave, int pt, int l, int wpb, int64_t bid, double2* stage2, int g,
-                         int t0 = 0) {
+                         int tid, int wave, int pt, int l, int wpb, int64_t bid, double2* stage2, int t0 = 0) {
   constexpr int L = Shape<MP>::L, PW = Shape<MP>::PW;
   double* stage = reinterpret_cast<double*>(stage2);
   const int PTS = PW * wpb, ROW = PTS + 1;
@@ -203,7 +200,7 @@ SFX_DEV void stage_store(const FftProdArgs& a, const double (&oa)[NOC][2][4], co
         const int64_t pp = p0 + qq;
         if (pp >= a.np) continue;
         const int c = plane & 1, m = plane >> 1;
-        const int64_t idx = (OP == OP_TENSOR) ? g + 3 * (pp + a.np * (int64_t)(2 * o + c + 6 * m))
+        const int64_t idx = (OP == OP_TENSOR) ? o / 3 + 3 * (pp + a.np * (int64_t)(2 * (o % 3) + c + 6 * m))
                                               : pp + a.np * (int64_t)(2 * o + c + 2 * NO * m);
         SFX_CHECK("Vout store", idx, a.nout);
         a.Vout[idx] = stage[plane * ROW + qq];
@@ -219,7 +216,7 @@ template <int MP, int OP>
 SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, double2* smem) {
   using S = Shape<MP>;
   constexpr int L = S::L, PW = S::PW;
-  constexpr int NF = OpT<OP>::NF, NS = NF / 2, NO = OpT<OP>::NO, NG = OpT<OP>::NG;
+  constexpr int NF = OpT<OP>::NF, NS = NF / 2, NO = OpT<OP>::NO;
   constexpr int NOC = (NO + 1) / 2;
   constexpr int GX = 3;                           // signals per LDS exchange
   const int wave = tid >> 6, lane = tid & 63;
@@ -244,43 +241,39 @@ SFX_DEV void fftprod_body(const FftProdArgs& a, int tid, int64_t bid, int wpb, d
   }
   inverse_fft<MP, 3 * NS, GX>(y, pt, l, lds, wMP);
 
-  // The tensor product transforms its output signals one at a time (three
-  // branches each): its 9 input branches stay live through all 3 groups,
-  // and with both output signals at once the kernel needs more than the 256
+  // The tensor product transforms its 5 output signals one at a time (three
+  // branches each): its 9 input branches stay live through all of them, and
+  // with more output signals at once the kernel needs more than the 256
   // registers of 2 waves per SIMD.
   constexpr int NT1 = (OP == OP_TENSOR) ? 1 : NOC;
 #pragma unroll
-  for (int g = 0; g < NG; ++g)
+  for (int t0 = 0; t0 < NOC; t0 += NT1) {
+    // pointwise op on the 3 branches x 4 positions held by this lane
+    double2 q[3 * NT1][4];
 #pragma unroll
-    for (int t0 = 0; t0 < NOC; t0 += NT1) {
-      // pointwise op on the 3 branches x 4 positions held by this lane
-      double2 q[3 * NT1][4];
+    for (int j1 = 0; j1 < 3; ++j1)
 #pragma unroll
-      for (int j1 = 0; j1 < 3; ++j1)
+      for (int r = 0; r < 4; ++r) {
+        double u[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          double u[6] = {0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < NS; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
 #pragma unroll
-          for (int s = 0; s < NS; ++s) { u[2 * s] = y[3 * s + j1][r].x; u[2 * s + 1] = y[3 * s + j1][r].y; }
-          double o[3] = {0, 0, 0};
-          apply_op<OP>(u, g, o);
-#pragma unroll
-          for (int t = 0; t < NT1; ++t) {
-            const int ot = 2 * (t0 + t);
-            q[3 * t + j1][r] = make_double2(o[ot], (ot + 1 < NO) ? o[ot + 1] : 0.0);
-          }
+        for (int t = 0; t < NT1; ++t) {
+          const int ot = 2 * (t0 + t);
+          q[3 * t + j1][r] = make_double2(apply_op<OP>(u, ot), (ot + 1 < NO) ? apply_op<OP>(u, ot + 1) : 0.0);
         }
-      forward_fft<MP, 3 * NT1, GX>(q, pt, l, lds, wMP);
+      }
+    forward_fft<MP, 3 * NT1, GX>(q, pt, l, lds, wMP);
 
-      double2 lo[NT1][4], hi[NT1][4];
+    double2 lo[NT1][4], hi[NT1][4];
 #pragma unroll
-      for (int t = 0; t < NT1; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
-      double oa[NT1][2][4], ob[NT1][2][4];
-      pair_modes<MP, NT1>(lo, hi, pt, l, lds, oa, ob);
-      stage_store<MP, OP, NT1, NO>(a, oa, ob, tid, wave, pt, l, wpb, bid, smem + 4 * MP, g, t0);
-      // the next exchanges reuse the stage area other waves may still read
-      if (g + 1 < NG || t0 + NT1 < NOC) SFX_SYNC();
-    }
+    for (int t = 0; t < NT1; ++t) poststep<MP>(q[3 * t], q[3 * t + 1], q[3 * t + 2], l, lo[t], hi[t], wN);
+    double oa[NT1][2][4], ob[NT1][2][4];
+    pair_modes<MP, NT1>(lo, hi, pt, l, lds, oa, ob);
+    stage_store<MP, OP, NT1, NO>(a, oa, ob, tid, wave, pt, l, wpb, bid, smem + 4 * MP, t0);
+    // the next exchanges reuse the stage area other waves may still read
+    if (t0 + NT1 < NOC) SFX_SYNC();
+  }
 }
 
 }  // namespace sfx
